@@ -1,0 +1,199 @@
+/*
+ * platoanalyze_amd.h — C ABI of the MI355X-native Tet4 assembly library.
+ *
+ * Drop-in boundary for Plato Analyze's workset residual / Jacobian evaluation
+ * and assembly on linear tetrahedra (SIMP elastostatics and steady thermal).
+ * Every entry point returns an int status (PA_OK = 0) and never throws; the
+ * message of the last failure on the calling thread is pa_last_error().
+ * Array arguments named *_dev are device pointers (hipMalloc'd, e.g. through
+ * pa_malloc); *_host are host pointers.  Work is enqueued on the given
+ * hipStream_t (passed as void*, 0 = default stream); the call returns before
+ * the work finishes unless it says otherwise.
+ *
+ * Layouts are the reference's (src/PlatoStaticsTypes.hpp, src/alg/CrsMatrix.hpp):
+ *   state  u[node*dpn + d]      control z[node]      coords x[node*3 + d]
+ *   connectivity conn[cell*4 + a] (int32)
+ *   block CRS: rowmap[N+1], colidx[nnzb] (sorted, diagonal included), values
+ *   [nnzb * bi * bj] with the entry of block e at e*bi*bj + i*bj + j.
+ */
+#ifndef PLATOANALYZE_AMD_H
+#define PLATOANALYZE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { PA_OK = 0, PA_ERR_ARG = 1, PA_ERR_HIP = 2, PA_ERR_LIMIT = 3, PA_ERR_STATE = 4 };
+
+/* Arithmetic of the assembled values. */
+enum { PA_FP64 = 0, PA_FP32 = 1 };
+
+/* Physics of a vector function (the reference's PhysicsT template argument). */
+enum { PA_MECHANICS = 0, PA_THERMAL = 1 };
+
+/* Scalar function types ("Scalar Function Type" in the reference input). */
+enum { PA_INTERNAL_ELASTIC_ENERGY = 0, PA_INTERNAL_THERMAL_ENERGY = 1 };
+
+typedef struct pa_mesh pa_mesh;
+typedef struct pa_vector_function pa_vector_function;
+typedef struct pa_scalar_function pa_scalar_function;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* pa_last_error(void);
+/* Library version string. */
+const char* pa_version(void);
+
+/* ---------------------------------------------------------------- device */
+/* Thin wrappers so callers without their own HIP runtime can move data.    */
+int pa_device_count(int* count);
+int pa_malloc(void** ptr_dev, size_t bytes);
+int pa_free(void* ptr_dev);
+int pa_memcpy_h2d(void* dst_dev, const void* src_host, size_t bytes);
+int pa_memcpy_d2h(void* dst_host, const void* src_dev, size_t bytes);
+int pa_memcpy_d2d(void* dst_dev, const void* src_dev, size_t bytes, void* stream);
+int pa_memset(void* ptr_dev, int value, size_t bytes, void* stream);
+int pa_stream_create(void** stream);
+int pa_stream_destroy(void* stream);
+int pa_stream_synchronize(void* stream);
+int pa_event_create(void** event);
+int pa_event_destroy(void* event);
+int pa_event_record(void* event, void* stream);
+int pa_event_synchronize(void* event);
+int pa_event_elapsed_ms(float* ms, void* start, void* stop);
+
+/* ---------------------------------------------------------------- mesh */
+/*
+ * Synthetic structured box: nx*ny*nz hexes on [0,lx]x[0,ly]x[0,lz], each split
+ * into 6 tets around the v000->v111 diagonal; node = ix*(ny+1)*(nz+1) +
+ * iy*(nz+1) + iz, hex = (ix*ny + iy)*nz + iz, 6 consecutive tets per hex.
+ * Replaces the BamG test mesh of unit_tests/PlatoTestHelpers.hpp:31-84.
+ * coords_host: (nx+1)(ny+1)(nz+1)*3 doubles; conn_host: 24*nx*ny*nz ints.
+ */
+int pa_box_mesh(int nx, int ny, int nz, double lx, double ly, double lz, double* coords_host, int* conn_host);
+
+/*
+ * Mesh + assembly graph.  Replaces Omega_h's mesh / NodeNodeGraph
+ * (src/OmegaHMesh.cpp:131-149) and the graph half of CreateBlockMatrix
+ * (src/ImplicitFunctors.hpp:932-1005), which the reference rebuilds on every
+ * gradient call; here it is built once.  Copies coords/conn to the device.
+ */
+int pa_mesh_create(const double* coords_host, int num_nodes, const int* conn_host, int num_cells, pa_mesh** mesh);
+int pa_mesh_destroy(pa_mesh* mesh);
+/* Sizes: nodes, cells, CRS blocks. */
+int pa_mesh_sizes(const pa_mesh* mesh, int* num_nodes, int* num_cells, int64_t* num_blocks);
+/* Copy the block-CRS graph to the host (rowmap: N+1, colidx: nnzb). */
+int pa_mesh_graph(const pa_mesh* mesh, int* rowmap_host, int* colidx_host);
+/* Device pointers of the graph (owned by the mesh). */
+int pa_mesh_graph_dev(const pa_mesh* mesh, const int** rowmap_dev, const int** colidx_dev);
+/*
+ * Register a side set (Omega_h side set: for each face, the owning cell and
+ * the three local node ordinals of the face in that cell;
+ * src/SurfaceLoadIntegral.hpp:106-107).
+ */
+int pa_mesh_add_side_set(pa_mesh* mesh, const char* name, int num_faces, const int* face_cells_host,
+                         const int* face_local_nodes_host);
+/* Register a node set (src/AbstractPlatoMesh.hpp node sets). */
+int pa_mesh_add_node_set(pa_mesh* mesh, const char* name, int num_nodes, const int* nodes_host);
+/* Number of nodes and a host copy of a node set. */
+int pa_mesh_node_set(const pa_mesh* mesh, const char* name, int* num_nodes, int* nodes_host);
+
+/* ------------------------------------------------------- vector function */
+/*
+ * Elliptic::VectorFunction<Mechanics<3>|Thermal<3>>
+ * (src/elliptic/VectorFunction.hpp:115-234) with the evaluator of one domain:
+ * ElastostaticResidual (src/elliptic/ElastostaticResidual.hpp:165-262) or
+ * ThermostaticResidual (src/elliptic/ThermostaticResidual.hpp:128-223),
+ * isotropic material, MSIMP penalty (src/Simp.hpp:41-52,130-134).
+ * material[0..1] = {Youngs Modulus, Poissons Ratio} for PA_MECHANICS,
+ * material[0] = {Thermal Conductivity} for PA_THERMAL;
+ * penalty = {Exponent, Minimum Value}.
+ */
+int pa_vector_function_create(pa_mesh* mesh, int physics, const double* material, const double* penalty,
+                              pa_vector_function** vf);
+int pa_vector_function_destroy(pa_vector_function* vf);
+/* Degrees of freedom per node (3 or 1). */
+int pa_vector_function_dofs_per_node(const pa_vector_function* vf, int* dpn);
+/*
+ * Uniform natural BC ("Natural Boundary Conditions/<n>/Type=Uniform") on a side
+ * set: src/NaturalBCs.hpp:150-224, src/SurfaceLoadIntegral.hpp:97-149.
+ * values: dpn doubles (traction or flux).
+ */
+int pa_vector_function_add_uniform_natural_bc(pa_vector_function* vf, const char* side_set, const double* values);
+/*
+ * Constant body load ("Body Loads/<n>/{Index, Function}"), scaled by the
+ * unpenalised mean nodal density: src/BodyLoads.hpp:54-122.
+ */
+int pa_vector_function_add_body_load(pa_vector_function* vf, int dof, double value);
+
+/* VectorFunction::value (src/elliptic/VectorFunction.hpp:250-334): R[N*dpn]. */
+int pa_vector_function_value(const pa_vector_function* vf, const double* u_dev, const double* z_dev, double* r_dev,
+                             void* stream);
+/*
+ * VectorFunction::gradient_u (src/elliptic/VectorFunction.hpp:524-615):
+ * Jacobian values [nnzb*dpn*dpn] on the mesh graph, in FP64 (double*) or FP32
+ * (float*) per `precision`.
+ */
+int pa_vector_function_gradient_u(const pa_vector_function* vf, const double* u_dev, const double* z_dev,
+                                  void* values_dev, int precision, void* stream);
+/* VectorFunction::gradient_u_T (src/elliptic/VectorFunction.hpp:431-523). */
+int pa_vector_function_gradient_u_T(const pa_vector_function* vf, const double* u_dev, const double* z_dev,
+                                    void* values_dev, int precision, void* stream);
+/*
+ * value and gradient_u in one pass over the mesh (same results as the two
+ * separate calls); r_dev: N*dpn doubles.
+ */
+int pa_vector_function_value_gradient_u(const pa_vector_function* vf, const double* u_dev, const double* z_dev,
+                                        double* r_dev, void* values_dev, int precision, void* stream);
+/*
+ * VectorFunction::gradient_z (src/elliptic/VectorFunction.hpp:617-710): the
+ * transposed dR/dz as a block matrix on the node graph with 1 x dpn blocks
+ * (row = control node, col = state node): values [nnzb*dpn] doubles.
+ */
+int pa_vector_function_gradient_z(const pa_vector_function* vf, const double* u_dev, const double* z_dev,
+                                  double* values_dev, void* stream);
+
+/* ------------------------------------------------------- scalar function */
+/*
+ * Elliptic::PhysicsScalarFunction with InternalElasticEnergy
+ * (src/elliptic/InternalElasticEnergy.hpp:132-199) or InternalThermalEnergy
+ * (src/elliptic/InternalThermalEnergy.hpp:96-150).
+ */
+int pa_scalar_function_create(pa_mesh* mesh, int type, const double* material, const double* penalty,
+                              pa_scalar_function** sf);
+int pa_scalar_function_destroy(pa_scalar_function* sf);
+/* PhysicsScalarFunction::value (src/elliptic/PhysicsScalarFunction.hpp:225-284): one double at value_dev. */
+int pa_scalar_function_value(const pa_scalar_function* sf, const double* u_dev, const double* z_dev,
+                             double* value_dev, void* stream);
+/* PhysicsScalarFunction::gradient_u: N*dpn doubles. */
+int pa_scalar_function_gradient_u(const pa_scalar_function* sf, const double* u_dev, const double* z_dev,
+                                  double* grad_dev, void* stream);
+/* PhysicsScalarFunction::gradient_z (src/elliptic/PhysicsScalarFunction.hpp:431-494): N doubles. */
+int pa_scalar_function_gradient_z(const pa_scalar_function* sf, const double* u_dev, const double* z_dev,
+                                  double* grad_dev, void* stream);
+
+/* ---------------------------------------------------------- constraints */
+/*
+ * applyBlockConstraints<dpn> (src/ApplyConstraints.hpp:40-112): symmetric
+ * Dirichlet elimination on a block-CRS matrix of the mesh graph (FP64 or FP32
+ * values per `precision`) and its right-hand side (FP64).
+ */
+int pa_apply_block_constraints(const pa_mesh* mesh, int dpn, void* values_dev, int precision, double* rhs_dev,
+                               int num_bcs, const int* bc_dofs_dev, const double* bc_values_dev, double scale,
+                               void* stream);
+
+/*
+ * Block matrix-vector product y = A x + beta*y on the mesh graph
+ * (src/PlatoMathHelpers.hpp:79-159, MatrixTimesVectorPlusVector).
+ * bi x bj blocks; values FP64 or FP32 per `precision`; x, y FP64.
+ */
+int pa_block_spmv(const pa_mesh* mesh, int bi, int bj, const void* values_dev, int precision, const double* x_dev,
+                  double beta, double* y_dev, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,446 @@
+"""Python host mirror of the MI355X Tet4 assembly library.
+
+The compute path is libplatoanalyze_amd.so (hand-written HIP kernels for
+gfx950 behind the C ABI in include/platoanalyze_amd.h).  This module loads it
+with ctypes and mirrors the reference's interface for this path:
+
+  * ``VectorFunction``  ~ Plato::Elliptic::VectorFunction<PhysicsT>
+    (src/elliptic/VectorFunction.hpp:115-710): value, gradient_u,
+    gradient_u_T, gradient_z;
+  * ``PhysicsScalarFunction`` ~ Plato::Elliptic::PhysicsScalarFunction
+    (src/elliptic/PhysicsScalarFunction.hpp:225-494): value, gradient_u,
+    gradient_z;
+  * ``apply_block_constraints`` ~ Plato::applyBlockConstraints
+    (src/ApplyConstraints.hpp:40-112).
+
+Arrays live in device memory (``DeviceArray``).  There is no CPU fallback:
+importing without the built library raises.  Errors from the library are
+raised as ``PlatoError`` (the reference throws std::runtime_error through
+ANALYZE_THROWERR, src/AnalyzeMacros.hpp:29-33).
+"""
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libplatoanalyze_amd.so"
+
+FP64, FP32 = 0, 1
+MECHANICS, THERMAL = 0, 1
+INTERNAL_ELASTIC_ENERGY, INTERNAL_THERMAL_ENERGY = 0, 1
+
+
+class PlatoError(RuntimeError):
+    pass
+
+
+def _load():
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`")
+    L = ctypes.CDLL(str(LIB_PATH))
+    vp, i, d, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_size_t
+    ip = ctypes.POINTER(ctypes.c_int)
+    sig = {
+        "pa_last_error": ([], ctypes.c_char_p),
+        "pa_version": ([], ctypes.c_char_p),
+        "pa_device_count": ([ip], i),
+        "pa_malloc": ([ctypes.POINTER(vp), sz], i),
+        "pa_free": ([vp], i),
+        "pa_memcpy_h2d": ([vp, vp, sz], i),
+        "pa_memcpy_d2h": ([vp, vp, sz], i),
+        "pa_memcpy_d2d": ([vp, vp, sz, vp], i),
+        "pa_memset": ([vp, i, sz, vp], i),
+        "pa_stream_create": ([ctypes.POINTER(vp)], i),
+        "pa_stream_destroy": ([vp], i),
+        "pa_stream_synchronize": ([vp], i),
+        "pa_event_create": ([ctypes.POINTER(vp)], i),
+        "pa_event_destroy": ([vp], i),
+        "pa_event_record": ([vp, vp], i),
+        "pa_event_synchronize": ([vp], i),
+        "pa_event_elapsed_ms": ([ctypes.POINTER(ctypes.c_float), vp, vp], i),
+        "pa_box_mesh": ([i, i, i, d, d, d, vp, vp], i),
+        "pa_mesh_create": ([vp, i, vp, i, ctypes.POINTER(vp)], i),
+        "pa_mesh_destroy": ([vp], i),
+        "pa_mesh_sizes": ([vp, ip, ip, ctypes.POINTER(ctypes.c_int64)], i),
+        "pa_mesh_graph": ([vp, vp, vp], i),
+        "pa_mesh_graph_dev": ([vp, ctypes.POINTER(vp), ctypes.POINTER(vp)], i),
+        "pa_mesh_add_side_set": ([vp, ctypes.c_char_p, i, vp, vp], i),
+        "pa_mesh_add_node_set": ([vp, ctypes.c_char_p, i, vp], i),
+        "pa_mesh_node_set": ([vp, ctypes.c_char_p, ip, vp], i),
+        "pa_vector_function_create": ([vp, i, vp, vp, ctypes.POINTER(vp)], i),
+        "pa_vector_function_destroy": ([vp], i),
+        "pa_vector_function_dofs_per_node": ([vp, ip], i),
+        "pa_vector_function_add_uniform_natural_bc": ([vp, ctypes.c_char_p, vp], i),
+        "pa_vector_function_add_body_load": ([vp, i, d], i),
+        "pa_vector_function_value": ([vp, vp, vp, vp, vp], i),
+        "pa_vector_function_gradient_u": ([vp, vp, vp, vp, i, vp], i),
+        "pa_vector_function_gradient_u_T": ([vp, vp, vp, vp, i, vp], i),
+        "pa_vector_function_value_gradient_u": ([vp, vp, vp, vp, vp, i, vp], i),
+        "pa_vector_function_gradient_z": ([vp, vp, vp, vp, vp], i),
+        "pa_scalar_function_create": ([vp, i, vp, vp, ctypes.POINTER(vp)], i),
+        "pa_scalar_function_destroy": ([vp], i),
+        "pa_scalar_function_value": ([vp, vp, vp, vp, vp], i),
+        "pa_scalar_function_gradient_u": ([vp, vp, vp, vp, vp], i),
+        "pa_scalar_function_gradient_z": ([vp, vp, vp, vp, vp], i),
+        "pa_apply_block_constraints": ([vp, i, vp, i, vp, i, vp, vp, d, vp], i),
+        "pa_block_spmv": ([vp, i, i, vp, i, vp, d, vp, vp], i),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    return L
+
+
+lib = _load()
+
+
+def _check(status):
+    if status != 0:
+        raise PlatoError(lib.pa_last_error().decode())
+
+
+def _np_ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def version():
+    return lib.pa_version().decode()
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    _check(lib.pa_device_count(ctypes.byref(n)))
+    return n.value
+
+
+# ----------------------------------------------------------------- device
+class DeviceArray:
+    """A flat device allocation with a numpy dtype."""
+
+    def __init__(self, n, dtype=np.float64):
+        self.dtype = np.dtype(dtype)
+        self.size = int(n)
+        self.nbytes = self.size * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        _check(lib.pa_malloc(ctypes.byref(p), max(self.nbytes, 1)))
+        self._ptr = p
+
+    @classmethod
+    def from_numpy(cls, a, dtype=None):
+        a = np.ascontiguousarray(a, dtype=dtype if dtype is not None else a.dtype).ravel()
+        d = cls(a.size, a.dtype)
+        if a.size:
+            _check(lib.pa_memcpy_h2d(d._ptr, _np_ptr(a), a.nbytes))
+        return d
+
+    @property
+    def ptr(self):
+        return self._ptr
+
+    def numpy(self):
+        out = np.empty(self.size, self.dtype)
+        if self.size:
+            _check(lib.pa_memcpy_d2h(_np_ptr(out), self._ptr, self.nbytes))
+        return out
+
+    def zero(self, stream=None):
+        _check(lib.pa_memset(self._ptr, 0, self.nbytes, stream))
+
+    def copy_from(self, other, stream=None):
+        assert other.nbytes == self.nbytes
+        _check(lib.pa_memcpy_d2d(self._ptr, other._ptr, self.nbytes, stream))
+
+    def free(self):
+        if self._ptr is not None and self._ptr.value:
+            lib.pa_free(self._ptr)
+        self._ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        s = ctypes.c_void_p()
+        _check(lib.pa_stream_create(ctypes.byref(s)))
+        self.handle = s
+
+    def synchronize(self):
+        _check(lib.pa_stream_synchronize(self.handle))
+
+    def __del__(self):
+        try:
+            lib.pa_stream_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        e = ctypes.c_void_p()
+        _check(lib.pa_event_create(ctypes.byref(e)))
+        self.handle = e
+
+    def record(self, stream=None):
+        _check(lib.pa_event_record(self.handle, stream.handle if isinstance(stream, Stream) else stream))
+
+    def synchronize(self):
+        _check(lib.pa_event_synchronize(self.handle))
+
+    def elapsed_ms(self, end):
+        ms = ctypes.c_float()
+        _check(lib.pa_event_elapsed_ms(ctypes.byref(ms), self.handle, end.handle))
+        return ms.value
+
+    def __del__(self):
+        try:
+            lib.pa_event_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def _sh(stream):
+    return stream.handle if isinstance(stream, Stream) else stream
+
+
+def synchronize():
+    _check(lib.pa_stream_synchronize(None))
+
+
+# ------------------------------------------------------------------- mesh
+def box_mesh(nx, ny, nz, lx=1.0, ly=1.0, lz=1.0):
+    """Structured box split into 6 tets per hex (host arrays coords, conn)."""
+    N = (nx + 1) * (ny + 1) * (nz + 1)
+    C = 6 * nx * ny * nz
+    coords = np.empty(N * 3, np.float64)
+    conn = np.empty(C * 4, np.int32)
+    _check(lib.pa_box_mesh(nx, ny, nz, lx, ly, lz, _np_ptr(coords), _np_ptr(conn)))
+    return coords, conn
+
+
+def box_side_set(coords, conn, axis, value, tol=1e-12):
+    """Faces of cells with all three face nodes on the plane x[axis] == value,
+    as (cells, local node ordinals F x 3), cell-ascending."""
+    X = coords.reshape(-1, 3)
+    cn = conn.reshape(-1, 4)
+    on = np.abs(X[cn][:, :, axis] - value) < tol
+    cells, lnodes = [], []
+    for f in [(1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2)]:
+        idx = np.nonzero(on[:, f[0]] & on[:, f[1]] & on[:, f[2]])[0]
+        cells.append(idx)
+        lnodes.append(np.tile(np.array(f, np.int32), (idx.size, 1)))
+    cells = np.concatenate(cells).astype(np.int32)
+    lnodes = np.concatenate(lnodes).astype(np.int32)
+    order = np.argsort(cells, kind="stable")
+    return cells[order].copy(), lnodes[order].reshape(-1).copy()
+
+
+class Mesh:
+    """Mesh + block-CRS graph on the device (Omega_h mesh + NodeNodeGraph)."""
+
+    def __init__(self, coords, conn):
+        coords = np.ascontiguousarray(coords, np.float64).ravel()
+        conn = np.ascontiguousarray(conn, np.int32).ravel()
+        self.coords_host, self.conn_host = coords, conn
+        h = ctypes.c_void_p()
+        _check(lib.pa_mesh_create(_np_ptr(coords), coords.size // 3, _np_ptr(conn), conn.size // 4, ctypes.byref(h)))
+        self.handle = h
+        N, C, nnzb = ctypes.c_int(), ctypes.c_int(), ctypes.c_int64()
+        _check(lib.pa_mesh_sizes(h, ctypes.byref(N), ctypes.byref(C), ctypes.byref(nnzb)))
+        self.num_nodes, self.num_cells, self.num_blocks = N.value, C.value, nnzb.value
+
+    @classmethod
+    def box(cls, nx, ny, nz, lx=1.0, ly=1.0, lz=1.0, side_sets=True):
+        coords, conn = box_mesh(nx, ny, nz, lx, ly, lz)
+        m = cls(coords, conn)
+        m.box_dims = (nx, ny, nz, lx, ly, lz)
+        if side_sets:
+            for axis, name in enumerate("xyz"):
+                L = (lx, ly, lz)[axis]
+                for sign, val in (("-", 0.0), ("+", L)):
+                    cells, ln = box_side_set(coords, conn, axis, val)
+                    m.add_side_set(name + sign, cells, ln)
+                    X = coords.reshape(-1, 3)
+                    m.add_node_set(name + sign, np.nonzero(np.abs(X[:, axis] - val) < 1e-12)[0])
+        return m
+
+    def graph(self):
+        rowmap = np.empty(self.num_nodes + 1, np.int32)
+        colidx = np.empty(self.num_blocks, np.int32)
+        _check(lib.pa_mesh_graph(self.handle, _np_ptr(rowmap), _np_ptr(colidx)))
+        return rowmap, colidx
+
+    def add_side_set(self, name, cells, local_nodes):
+        cells = np.ascontiguousarray(cells, np.int32)
+        local_nodes = np.ascontiguousarray(local_nodes, np.int32)
+        _check(lib.pa_mesh_add_side_set(self.handle, name.encode(), cells.size, _np_ptr(cells), _np_ptr(local_nodes)))
+
+    def add_node_set(self, name, nodes):
+        nodes = np.ascontiguousarray(nodes, np.int32)
+        _check(lib.pa_mesh_add_node_set(self.handle, name.encode(), nodes.size, _np_ptr(nodes)))
+
+    def node_set(self, name):
+        n = ctypes.c_int()
+        _check(lib.pa_mesh_node_set(self.handle, name.encode(), ctypes.byref(n), None))
+        out = np.empty(n.value, np.int32)
+        _check(lib.pa_mesh_node_set(self.handle, name.encode(), ctypes.byref(n), _np_ptr(out)))
+        return out
+
+    def __del__(self):
+        try:
+            lib.pa_mesh_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class CrsMatrix:
+    """Block CRS matrix on the mesh graph (src/alg/CrsMatrix.hpp:19-111)."""
+
+    def __init__(self, mesh, bi, bj, values, precision):
+        self.mesh, self.bi, self.bj, self.values, self.precision = mesh, bi, bj, values, precision
+        self.num_rows = mesh.num_nodes * bi
+        self.num_cols = mesh.num_nodes * bj
+        self.is_block_matrix = bi * bj > 1
+
+    def entries(self):
+        return self.values.numpy()
+
+    def spmv(self, x, y, beta=0.0, stream=None):
+        _check(lib.pa_block_spmv(self.mesh.handle, self.bi, self.bj, self.values.ptr, self.precision, x.ptr, beta,
+                                 y.ptr, _sh(stream)))
+        return y
+
+
+def _prec(p):
+    if p in (FP64, "fp64", "f64", np.float64):
+        return FP64, np.float64
+    if p in (FP32, "fp32", "f32", np.float32):
+        return FP32, np.float32
+    raise PlatoError(f"unknown precision {p!r}")
+
+
+def _dev(x, dtype=np.float64):
+    return x if isinstance(x, DeviceArray) else DeviceArray.from_numpy(np.asarray(x, dtype), dtype)
+
+
+class VectorFunction:
+    """Elliptic::VectorFunction<Mechanics<3> | Thermal<3>> on one domain."""
+
+    def __init__(self, mesh, physics, *, youngs=1.0, poisson=0.3, conductivity=1.0, penalty_exponent=3.0,
+                 penalty_min=1e-9):
+        self.mesh = mesh
+        ph = {"Mechanics": MECHANICS, "Thermal": THERMAL, MECHANICS: MECHANICS, THERMAL: THERMAL}[physics]
+        mat = np.array([youngs, poisson] if ph == MECHANICS else [conductivity, 0.0], np.float64)
+        pen = np.array([penalty_exponent, penalty_min], np.float64)
+        h = ctypes.c_void_p()
+        _check(lib.pa_vector_function_create(mesh.handle, ph, _np_ptr(mat), _np_ptr(pen), ctypes.byref(h)))
+        self.handle = h
+        self.physics = ph
+        self.dpn = 3 if ph == MECHANICS else 1
+
+    def size(self):
+        return self.mesh.num_nodes * self.dpn
+
+    def add_uniform_natural_bc(self, side_set, values):
+        v = np.zeros(3, np.float64)
+        v[: len(values)] = values
+        _check(lib.pa_vector_function_add_uniform_natural_bc(self.handle, side_set.encode(), _np_ptr(v)))
+
+    def add_body_load(self, dof, value):
+        _check(lib.pa_vector_function_add_body_load(self.handle, dof, value))
+
+    def value(self, u, z, out=None, stream=None):
+        u, z = _dev(u), _dev(z)
+        out = out if out is not None else DeviceArray(self.size())
+        _check(lib.pa_vector_function_value(self.handle, u.ptr, z.ptr, out.ptr, _sh(stream)))
+        return out
+
+    def gradient_u(self, u, z, precision=FP64, out=None, stream=None):
+        p, dt = _prec(precision)
+        u, z = _dev(u), _dev(z)
+        out = out if out is not None else DeviceArray(self.mesh.num_blocks * self.dpn * self.dpn, dt)
+        _check(lib.pa_vector_function_gradient_u(self.handle, u.ptr, z.ptr, out.ptr, p, _sh(stream)))
+        return CrsMatrix(self.mesh, self.dpn, self.dpn, out, p)
+
+    def gradient_u_T(self, u, z, precision=FP64, out=None, stream=None):
+        p, dt = _prec(precision)
+        u, z = _dev(u), _dev(z)
+        out = out if out is not None else DeviceArray(self.mesh.num_blocks * self.dpn * self.dpn, dt)
+        _check(lib.pa_vector_function_gradient_u_T(self.handle, u.ptr, z.ptr, out.ptr, p, _sh(stream)))
+        return CrsMatrix(self.mesh, self.dpn, self.dpn, out, p)
+
+    def value_gradient_u(self, u, z, precision=FP64, r_out=None, out=None, stream=None):
+        p, dt = _prec(precision)
+        u, z = _dev(u), _dev(z)
+        r_out = r_out if r_out is not None else DeviceArray(self.size())
+        out = out if out is not None else DeviceArray(self.mesh.num_blocks * self.dpn * self.dpn, dt)
+        _check(lib.pa_vector_function_value_gradient_u(self.handle, u.ptr, z.ptr, r_out.ptr, out.ptr, p,
+                                                       _sh(stream)))
+        return r_out, CrsMatrix(self.mesh, self.dpn, self.dpn, out, p)
+
+    def gradient_z(self, u, z, out=None, stream=None):
+        u, z = _dev(u), _dev(z)
+        out = out if out is not None else DeviceArray(self.mesh.num_blocks * self.dpn)
+        _check(lib.pa_vector_function_gradient_z(self.handle, u.ptr, z.ptr, out.ptr, _sh(stream)))
+        return CrsMatrix(self.mesh, 1, self.dpn, out, FP64)
+
+    def __del__(self):
+        try:
+            lib.pa_vector_function_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class PhysicsScalarFunction:
+    """Elliptic::PhysicsScalarFunction with an internal-energy criterion."""
+
+    def __init__(self, mesh, function_type, *, youngs=1.0, poisson=0.3, conductivity=1.0, penalty_exponent=3.0,
+                 penalty_min=1e-9):
+        self.mesh = mesh
+        t = {"Internal Elastic Energy": INTERNAL_ELASTIC_ENERGY,
+             "Internal Thermal Energy": INTERNAL_THERMAL_ENERGY}[function_type]
+        mat = np.array([youngs, poisson] if t == INTERNAL_ELASTIC_ENERGY else [conductivity, 0.0], np.float64)
+        pen = np.array([penalty_exponent, penalty_min], np.float64)
+        h = ctypes.c_void_p()
+        _check(lib.pa_scalar_function_create(mesh.handle, t, _np_ptr(mat), _np_ptr(pen), ctypes.byref(h)))
+        self.handle = h
+        self.dpn = 3 if t == INTERNAL_ELASTIC_ENERGY else 1
+
+    def value(self, u, z, out=None, stream=None):
+        u, z = _dev(u), _dev(z)
+        dev_out = out if out is not None else DeviceArray(1)
+        _check(lib.pa_scalar_function_value(self.handle, u.ptr, z.ptr, dev_out.ptr, _sh(stream)))
+        if out is not None:
+            return out
+        return float(dev_out.numpy()[0])
+
+    def gradient_u(self, u, z, out=None, stream=None):
+        u, z = _dev(u), _dev(z)
+        out = out if out is not None else DeviceArray(self.mesh.num_nodes * self.dpn)
+        _check(lib.pa_scalar_function_gradient_u(self.handle, u.ptr, z.ptr, out.ptr, _sh(stream)))
+        return out
+
+    def gradient_z(self, u, z, out=None, stream=None):
+        u, z = _dev(u), _dev(z)
+        out = out if out is not None else DeviceArray(self.mesh.num_nodes)
+        _check(lib.pa_scalar_function_gradient_z(self.handle, u.ptr, z.ptr, out.ptr, _sh(stream)))
+        return out
+
+    def __del__(self):
+        try:
+            lib.pa_scalar_function_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def apply_block_constraints(matrix, rhs, bc_dofs, bc_values, scale=1.0, stream=None):
+    """Symmetric Dirichlet elimination on `matrix` (CrsMatrix) and `rhs`."""
+    dofs = _dev(np.asarray(bc_dofs, np.int32), np.int32) if not isinstance(bc_dofs, DeviceArray) else bc_dofs
+    vals = _dev(bc_values)
+    _check(lib.pa_apply_block_constraints(matrix.mesh.handle, matrix.bi, matrix.values.ptr, matrix.precision, rhs.ptr,
+                                          dofs.size, dofs.ptr, vals.ptr, scale, _sh(stream)))

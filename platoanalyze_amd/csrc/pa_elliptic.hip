@@ -1,0 +1,991 @@
+// Node-ordered residual / Jacobian / dR/dz assembly on linear tets for SIMP
+// elastostatics and steady thermal diffusion (gfx950).
+//
+// Work decomposition: one group of G lanes per graph row (= mesh node).  The
+// lanes of a group walk the cells incident to the row node (the node -> cell
+// incidence built once per mesh), each lane evaluating one cell: gather the
+// four nodes' coordinates, densities and states, the Tet4 gradients and
+// volume, and the row node's 3x3 (or 1x1) blocks against the cell's four
+// nodes.  Blocks are summed into an LDS copy of the row (slots precomputed per
+// (row, cell) pair), so every matrix value is written to HBM exactly once,
+// with contiguous stores, and without global atomics.  The arithmetic
+// restates, per cell, the reference's chain:
+//   ComputeGradientWorkset<3>   src/ImplicitFunctors.hpp:169-221
+//   Strain / LinearStress       src/Strain.hpp:56-95, src/LinearStress.hpp:102-120
+//   ApplyWeighting + MSIMP      src/ApplyWeighting.hpp:48-65, src/Simp.hpp:130-134
+//   StressDivergence            src/StressDivergence.hpp:70-90
+//   ScalarGrad / ThermalFlux / FluxDivergence  src/ScalarGrad.hpp:58-74,
+//                               src/ThermalFlux.hpp:42-52, src/FluxDivergence.hpp:31-48
+//   BodyLoad (x mean density)   src/BodyLoads.hpp:103-121
+// with the isotropic B^T C B contraction in closed form (lambda, mu).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "pa_internal.h"
+
+namespace pa {
+namespace {
+
+constexpr int kBlock = 256;
+
+struct Material {
+    double lambda, mu;  // isotropic: C(0,1) and C(3,3) of the Voigt stiffness
+    double k;           // conductivity
+    double p, emin;     // MSIMP exponent, minimum value
+};
+
+struct RowMesh {
+    int N;
+    const double4* __restrict__ x4;  // coords padded to 32 bytes
+    const int4* __restrict__ conn;
+    const int* __restrict__ rowmap;
+    const int* __restrict__ n2c_off;
+    const int2* __restrict__ pairs;
+};
+
+// Blocks are dispatched round-robin over the 8 XCDs; renumber them so each
+// XCD walks a contiguous range of rows and its L2 keeps the shared nodes.
+__device__ inline int xcd_block() {
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int per = (nb + 7) / 8;
+    const int x = b % 8, i = b / 8;
+    const int full = nb % 8 == 0 ? 8 : nb % 8;  // XCDs that receive `per` blocks
+    if (x < full) return x * per + i;
+    return full * per + (x - full) * (per - 1) + i;
+}
+
+__device__ inline double penal(double rho, double p, double emin) {
+    // integral exponents by products (matches pow to an ulp)
+    double r;
+    if (p == 3.0) r = rho * rho * rho;
+    else if (p == 1.0) r = rho;
+    else if (p == 2.0) r = rho * rho;
+    else r = pow(rho, p);
+    return emin + (1.0 - emin) * r;
+}
+
+__device__ inline double dpenal(double rho, double p, double emin) {
+    double r;
+    if (p == 3.0) r = 3.0 * rho * rho;
+    else if (p == 1.0) r = 1.0;
+    else if (p == 2.0) r = 2.0 * rho;
+    else r = rho == 0.0 ? 0.0 : p * pow(rho, p - 1.0);
+    return (1.0 - emin) * r;
+}
+
+// Tet4 shape-function gradients and |det J| (ComputeGradientWorkset<3>).
+__device__ inline double tet_gradients(const double4 x[4], double g[4][3]) {
+    const double j11 = x[0].x - x[3].x, j12 = x[1].x - x[3].x, j13 = x[2].x - x[3].x;
+    const double j21 = x[0].y - x[3].y, j22 = x[1].y - x[3].y, j23 = x[2].y - x[3].y;
+    const double j31 = x[0].z - x[3].z, j32 = x[1].z - x[3].z, j33 = x[2].z - x[3].z;
+    const double det = j11 * j22 * j33 + j12 * j23 * j31 + j13 * j21 * j32 - j11 * j23 * j32 - j12 * j21 * j33 -
+                       j13 * j22 * j31;
+    g[0][0] = (j22 * j33 - j23 * j32) / det;
+    g[0][1] = (j13 * j32 - j12 * j33) / det;
+    g[0][2] = (j12 * j23 - j13 * j22) / det;
+    g[1][0] = (j23 * j31 - j21 * j33) / det;
+    g[1][1] = (j11 * j33 - j13 * j31) / det;
+    g[1][2] = (j13 * j21 - j11 * j23) / det;
+    g[2][0] = (j21 * j32 - j22 * j31) / det;
+    g[2][1] = (j12 * j31 - j11 * j32) / det;
+    g[2][2] = (j11 * j22 - j12 * j21) / det;
+    g[3][0] = -(g[0][0] + g[1][0] + g[2][0]);
+    g[3][1] = -(g[0][1] + g[1][1] + g[2][1]);
+    g[3][2] = -(g[0][2] + g[1][2] + g[2][2]);
+    return fabs(det);
+}
+
+// Everything one lane needs about one incident cell.
+struct CellEval {
+    int nodes[4];
+    int a;          // local index of the row node
+    unsigned slots; // packed column slots of the four nodes in the row
+    double g[4][3];
+    double absdet;  // |det J| = 6 V
+    double rho;     // mean nodal density
+};
+
+__device__ inline void load_cell(const RowMesh& m, const double* __restrict__ z, int pair, CellEval& c) {
+    const int2 pr = m.pairs[pair];
+    const int cell = pr.x >> 2;
+    c.a = pr.x & 3;
+    c.slots = (unsigned)pr.y;
+    const int4 nd = m.conn[cell];
+    c.nodes[0] = nd.x;
+    c.nodes[1] = nd.y;
+    c.nodes[2] = nd.z;
+    c.nodes[3] = nd.w;
+    double4 x[4];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) x[n] = m.x4[c.nodes[n]];
+    c.absdet = tet_gradients(x, c.g);
+    c.rho = (((z[c.nodes[0]] + z[c.nodes[1]]) + z[c.nodes[2]]) + z[c.nodes[3]]) / 4.0;
+}
+
+// Voigt strain (xx, yy, zz, yz, xz, xy; engineering shear) of a cell.
+__device__ inline void cell_strain(const CellEval& c, const double* __restrict__ u, double eps[6]) {
+    double un[4][3];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+        un[n][0] = u[3L * c.nodes[n] + 0];
+        un[n][1] = u[3L * c.nodes[n] + 1];
+        un[n][2] = u[3L * c.nodes[n] + 2];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        double s = 0.0;
+#pragma unroll
+        for (int n = 0; n < 4; ++n) s += un[n][d] * c.g[n][d];
+        eps[d] = s;
+    }
+    // (1,2) -> 3, (0,2) -> 4, (0,1) -> 5
+    const int pi[3] = {1, 0, 0}, pj[3] = {2, 2, 1};
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        const int di = pi[v], dj = pj[v];
+        double s = 0.0;
+#pragma unroll
+        for (int n = 0; n < 4; ++n) s += (un[n][dj] * c.g[n][di] + un[n][di] * c.g[n][dj]);
+        eps[3 + v] = s;
+    }
+}
+
+__device__ inline void iso_stress(const Material& mt, const double eps[6], double sig[6]) {
+    const double tr = eps[0] + eps[1] + eps[2];
+    const double twomu = 2.0 * mt.mu;
+    sig[0] = mt.lambda * tr + twomu * eps[0];
+    sig[1] = mt.lambda * tr + twomu * eps[1];
+    sig[2] = mt.lambda * tr + twomu * eps[2];
+    sig[3] = mt.mu * eps[3];
+    sig[4] = mt.mu * eps[4];
+    sig[5] = mt.mu * eps[5];
+}
+
+// B_b^T sigma: row i gets sum_j sigma(voigt(i,j)) * g_b[j].
+__device__ inline void btsig(const double sig[6], const double gb[3], double out[3]) {
+    out[0] = sig[0] * gb[0] + sig[5] * gb[1] + sig[4] * gb[2];
+    out[1] = sig[5] * gb[0] + sig[1] * gb[1] + sig[3] * gb[2];
+    out[2] = sig[4] * gb[0] + sig[3] * gb[1] + sig[2] * gb[2];
+}
+
+// 3x3 stiffness block w * B_a^T C B_b of an isotropic material.
+__device__ inline void iso_block(const Material& mt, double w, const double ga[3], const double gb[3], double K[9]) {
+    const double wl = w * mt.lambda, wm = w * mt.mu;
+    const double dot = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int n = 0; n < 3; ++n) K[i * 3 + n] = wl * ga[i] * gb[n] + wm * ga[n] * gb[i];
+    K[0] += wm * dot;
+    K[4] += wm * dot;
+    K[8] += wm * dot;
+}
+
+// Row-node gradient g[a] with a known only at run time.  Bit masks rather
+// than a select chain: the optimiser folds selects between elements of one
+// array back into a dynamic index, which forces the array into scratch.
+__device__ inline void pick_row(const double g[4][3], int a, double out[3]) {
+    const unsigned long long m0 = 0ULL - (unsigned long long)(a == 0), m1 = 0ULL - (unsigned long long)(a == 1);
+    const unsigned long long m2 = 0ULL - (unsigned long long)(a == 2), m3 = 0ULL - (unsigned long long)(a == 3);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const unsigned long long b = (__double_as_longlong(g[0][d]) & m0) | (__double_as_longlong(g[1][d]) & m1) |
+                                     (__double_as_longlong(g[2][d]) & m2) | (__double_as_longlong(g[3][d]) & m3);
+        out[d] = __longlong_as_double((long long)b);
+    }
+}
+
+template <int G>
+__device__ inline double group_sum(double v) {
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+    return v;
+}
+
+__device__ inline void lds_add(double* p, double v) { atomicAdd(p, v); }
+
+// ---------------------------------------------------------------------------
+// Residual rows: R[row] = sum over incident cells of the row node's cell
+// residual (+ thermal body load).  Also serves criterion gradient_u (scale 2
+// for the thermal energy, no loads).
+// ---------------------------------------------------------------------------
+template <int DPN, int G>
+__global__ __launch_bounds__(kBlock) void k_residual_rows(RowMesh m, Material mt, const double* __restrict__ u,
+                                                          const double* __restrict__ z, double body, double scale,
+                                                          double* __restrict__ R) {
+    constexpr int ROWS = kBlock / G;
+    const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+    const int row = xcd_block() * ROWS + grp;
+    if (row >= m.N) return;  // whole group exits together
+    const int p0 = m.n2c_off[row], p1 = m.n2c_off[row + 1];
+    double r[DPN];
+#pragma unroll
+    for (int i = 0; i < DPN; ++i) r[i] = 0.0;
+    for (int p = p0 + lane; p < p1; p += G) {
+        CellEval c;
+        load_cell(m, z, p, c);
+        const double s = penal(c.rho, mt.p, mt.emin);
+        const double w = s * (c.absdet / 6.0);
+        double ga[3];
+        pick_row(c.g, c.a, ga);
+        if constexpr (DPN == 3) {
+            double eps[6], sig[6], out[3];
+            cell_strain(c, u, eps);
+            iso_stress(mt, eps, sig);
+            btsig(sig, ga, out);
+            r[0] += w * out[0];
+            r[1] += w * out[1];
+            r[2] += w * out[2];
+        } else {
+            double gT[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int n = 0; n < 4; ++n) {
+                const double T = u[c.nodes[n]];
+                gT[0] += T * c.g[n][0];
+                gT[1] += T * c.g[n][1];
+                gT[2] += T * c.g[n][2];
+            }
+            r[0] += w * mt.k * (gT[0] * ga[0] + gT[1] * ga[1] + gT[2] * ga[2]);
+            // body load, scale -1, weight (1/6)|detJ|, basis 1/4, unpenalised density
+            r[0] += -1.0 * (c.absdet / 6.0) * body * 0.25 * c.rho;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < DPN; ++i) r[i] = group_sum<G>(r[i]);
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < DPN; ++i) R[(long)row * DPN + i] = scale * r[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Jacobian rows (optionally with the residual in the same pass).
+// TR: write the transposed operator (block (i,j) = K_ji^T).
+// ---------------------------------------------------------------------------
+template <int DPN, int G, int KMAX, class VT, bool RES, bool TR>
+__global__ __launch_bounds__(kBlock) void k_jacobian_rows(RowMesh m, Material mt, const double* __restrict__ u,
+                                                          const double* __restrict__ z, double body,
+                                                          VT* __restrict__ vals, double* __restrict__ R) {
+    constexpr int BS = DPN * DPN;
+    constexpr int ROWS = kBlock / G;
+    __shared__ double acc[ROWS][KMAX * BS];
+    const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+    const int row = xcd_block() * ROWS + grp;
+    const bool active = row < m.N;
+    int rs = 0, rl = 0, p0 = 0, p1 = 0;
+    if (active) {
+        rs = m.rowmap[row];
+        rl = m.rowmap[row + 1] - rs;
+        p0 = m.n2c_off[row];
+        p1 = m.n2c_off[row + 1];
+    }
+    for (int t = lane; t < rl * BS; t += G) acc[grp][t] = 0.0;
+    __syncthreads();
+    double r[DPN];
+#pragma unroll
+    for (int i = 0; i < DPN; ++i) r[i] = 0.0;
+    for (int p = p0 + lane; p < p1; p += G) {
+        CellEval c;
+        load_cell(m, z, p, c);
+        const double s = penal(c.rho, mt.p, mt.emin);
+        const double w = s * (c.absdet / 6.0);
+        double ga[3];
+        pick_row(c.g, c.a, ga);
+        double* racc = acc[grp];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int slot = (c.slots >> (8 * b)) & 0xff;
+            if constexpr (DPN == 3) {
+                double K[9];
+                if constexpr (!TR) {
+                    iso_block(mt, w, ga, c.g[b], K);
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) lds_add(&racc[slot * 9 + q], K[q]);
+                } else {
+                    iso_block(mt, w, c.g[b], ga, K);
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+#pragma unroll
+                        for (int n = 0; n < 3; ++n) lds_add(&racc[slot * 9 + i * 3 + n], K[n * 3 + i]);
+                }
+            } else {
+                const double* gb = c.g[b];
+                const double kv = TR ? w * mt.k * (gb[0] * ga[0] + gb[1] * ga[1] + gb[2] * ga[2])
+                                     : w * mt.k * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
+                lds_add(&racc[slot], kv);
+            }
+        }
+        if constexpr (RES) {
+            if constexpr (DPN == 3) {
+                double eps[6], sig[6], out[3];
+                cell_strain(c, u, eps);
+                iso_stress(mt, eps, sig);
+                btsig(sig, ga, out);
+                r[0] += w * out[0];
+                r[1] += w * out[1];
+                r[2] += w * out[2];
+            } else {
+                double gT[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+                for (int n = 0; n < 4; ++n) {
+                    const double T = u[c.nodes[n]];
+                    gT[0] += T * c.g[n][0];
+                    gT[1] += T * c.g[n][1];
+                    gT[2] += T * c.g[n][2];
+                }
+                r[0] += w * mt.k * (gT[0] * ga[0] + gT[1] * ga[1] + gT[2] * ga[2]);
+                r[0] += -1.0 * (c.absdet / 6.0) * body * 0.25 * c.rho;
+            }
+        }
+    }
+    __syncthreads();
+    if (active) {
+        VT* dst = vals + (long)rs * BS;
+        for (int t = lane; t < rl * BS; t += G) dst[t] = (VT)acc[grp][t];
+    }
+    if constexpr (RES) {
+#pragma unroll
+        for (int i = 0; i < DPN; ++i) r[i] = group_sum<G>(r[i]);
+        if (active && lane == 0) {
+#pragma unroll
+            for (int i = 0; i < DPN; ++i) R[(long)row * DPN + i] = r[i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// gradient_z rows: block (row = control node a, col = state node b) holds
+// dR_b/dz_a (1 x DPN) = sum over cells with a, b of (1/4) s'(rho) (V) B_b^T C eps
+// (+ thermal body load derivative).
+// ---------------------------------------------------------------------------
+template <int DPN, int G, int KMAX>
+__global__ __launch_bounds__(kBlock) void k_gradz_rows(RowMesh m, Material mt, const double* __restrict__ u,
+                                                       const double* __restrict__ z, double body,
+                                                       double* __restrict__ vals) {
+    constexpr int ROWS = kBlock / G;
+    __shared__ double acc[ROWS][KMAX * DPN];
+    const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+    const int row = xcd_block() * ROWS + grp;
+    const bool active = row < m.N;
+    int rs = 0, rl = 0, p0 = 0, p1 = 0;
+    if (active) {
+        rs = m.rowmap[row];
+        rl = m.rowmap[row + 1] - rs;
+        p0 = m.n2c_off[row];
+        p1 = m.n2c_off[row + 1];
+    }
+    for (int t = lane; t < rl * DPN; t += G) acc[grp][t] = 0.0;
+    __syncthreads();
+    for (int p = p0 + lane; p < p1; p += G) {
+        CellEval c;
+        load_cell(m, z, p, c);
+        const double ds = dpenal(c.rho, mt.p, mt.emin) * 0.25;
+        const double w = ds * (c.absdet / 6.0);
+        double* racc = acc[grp];
+        if constexpr (DPN == 3) {
+            double eps[6], sig[6];
+            cell_strain(c, u, eps);
+            iso_stress(mt, eps, sig);
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int slot = (c.slots >> (8 * b)) & 0xff;
+                double out[3];
+                btsig(sig, c.g[b], out);
+                lds_add(&racc[slot * 3 + 0], w * out[0]);
+                lds_add(&racc[slot * 3 + 1], w * out[1]);
+                lds_add(&racc[slot * 3 + 2], w * out[2]);
+            }
+        } else {
+            double gT[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int n = 0; n < 4; ++n) {
+                const double T = u[c.nodes[n]];
+                gT[0] += T * c.g[n][0];
+                gT[1] += T * c.g[n][1];
+                gT[2] += T * c.g[n][2];
+            }
+            const double bl = -1.0 * (c.absdet / 6.0) * body * 0.25 * 0.25;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int slot = (c.slots >> (8 * b)) & 0xff;
+                const double* gb = c.g[b];
+                lds_add(&racc[slot], w * mt.k * (gT[0] * gb[0] + gT[1] * gb[1] + gT[2] * gb[2]) + bl);
+            }
+        }
+    }
+    __syncthreads();
+    if (active) {
+        double* dst = vals + (long)rs * DPN;
+        for (int t = lane; t < rl * DPN; t += G) dst[t] = acc[grp][t];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Internal energy: per-cell values (block partial sums), nodal gradient_z.
+// ---------------------------------------------------------------------------
+template <int DPN>
+__device__ inline double cell_energy_density(const CellEval& c, const Material& mt, const double* __restrict__ u) {
+    // unpenalised energy per unit volume weight: 1/2 sigma0 : eps, or k |grad T|^2
+    if constexpr (DPN == 3) {
+        double eps[6], sig[6];
+        cell_strain(c, u, eps);
+        iso_stress(mt, eps, sig);
+        double inc = 0.0;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) inc += sig[i] * eps[i];
+        return 0.5 * inc;
+    } else {
+        double gT[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+            const double T = u[c.nodes[n]];
+            gT[0] += T * c.g[n][0];
+            gT[1] += T * c.g[n][1];
+            gT[2] += T * c.g[n][2];
+        }
+        return mt.k * (gT[0] * gT[0] + gT[1] * gT[1] + gT[2] * gT[2]);
+    }
+}
+
+template <int DPN>
+__global__ __launch_bounds__(kBlock) void k_energy_cells(int C, const double4* __restrict__ x4,
+                                                         const int4* __restrict__ conn, Material mt,
+                                                         const double* __restrict__ u, const double* __restrict__ z,
+                                                         double* __restrict__ partials) {
+    __shared__ double red[kBlock / 64];
+    double acc = 0.0;
+    for (long cell = (long)blockIdx.x * kBlock + threadIdx.x; cell < C; cell += (long)gridDim.x * kBlock) {
+        CellEval c;
+        const int4 nd = conn[cell];
+        c.nodes[0] = nd.x;
+        c.nodes[1] = nd.y;
+        c.nodes[2] = nd.z;
+        c.nodes[3] = nd.w;
+        double4 x[4];
+#pragma unroll
+        for (int n = 0; n < 4; ++n) x[n] = x4[c.nodes[n]];
+        c.absdet = tet_gradients(x, c.g);
+        c.rho = (((z[c.nodes[0]] + z[c.nodes[1]]) + z[c.nodes[2]]) + z[c.nodes[3]]) / 4.0;
+        const double s = penal(c.rho, mt.p, mt.emin);
+        acc += s * (c.absdet / 6.0) * cell_energy_density<DPN>(c, mt, u);
+    }
+    acc = group_sum<64>(acc);
+    if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < kBlock / 64; ++i) t += red[i];
+        partials[blockIdx.x] = t;
+    }
+}
+
+__global__ void k_sum_partials(int n, const double* __restrict__ partials, double* __restrict__ out) {
+    // single wave, fixed order
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += 64) acc += partials[i];
+    acc = group_sum<64>(acc);
+    if (threadIdx.x == 0) *out = acc;
+}
+
+template <int DPN, int G>
+__global__ __launch_bounds__(kBlock) void k_energy_gradz_rows(RowMesh m, Material mt, const double* __restrict__ u,
+                                                              const double* __restrict__ z, double* __restrict__ gz) {
+    constexpr int ROWS = kBlock / G;
+    const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+    const int row = xcd_block() * ROWS + grp;
+    if (row >= m.N) return;
+    const int p0 = m.n2c_off[row], p1 = m.n2c_off[row + 1];
+    double acc = 0.0;
+    for (int p = p0 + lane; p < p1; p += G) {
+        CellEval c;
+        load_cell(m, z, p, c);
+        const double ds = dpenal(c.rho, mt.p, mt.emin) * 0.25;
+        acc += ds * (c.absdet / 6.0) * cell_energy_density<DPN>(c, mt, u);
+    }
+    acc = group_sum<G>(acc);
+    if (lane == 0) gz[row] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Uniform natural BC on a side set, in boundary-node order:
+// R[node] += sum over faces (basis 1/3) * value * scale * (1/2) * |cross|.
+// ---------------------------------------------------------------------------
+template <int DPN>
+__global__ void k_natural_bc(int nb, const int* __restrict__ bnodes, const int* __restrict__ boff,
+                             const int* __restrict__ bface, const int* __restrict__ fcells,
+                             const int* __restrict__ flnodes, const int4* __restrict__ conn,
+                             const double4* __restrict__ x4, double v0, double v1, double v2, double scale,
+                             double* __restrict__ R) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    const int node = bnodes[i];
+    double sum = 0.0;
+    for (int k = boff[i]; k < boff[i + 1]; ++k) {
+        const int f = bface[k] >> 2;
+        const int cell = fcells[f];
+        const int4 nd = conn[cell];
+        const int cn[4] = {nd.x, nd.y, nd.z, nd.w};
+        const double4 a = x4[cn[flnodes[3 * f + 0]]];
+        const double4 b = x4[cn[flnodes[3 * f + 1]]];
+        const double4 c = x4[cn[flnodes[3 * f + 2]]];
+        const double J00 = a.x - c.x, J01 = a.y - c.y, J02 = a.z - c.z;
+        const double J10 = b.x - c.x, J11 = b.y - c.y, J12 = b.z - c.z;
+        const double j23 = J01 * J12 - J02 * J11;
+        const double j31 = J02 * J10 - J00 * J12;
+        const double j12 = J00 * J11 - J01 * J10;
+        sum += (scale * 0.5) * sqrt(j23 * j23 + j31 * j31 + j12 * j12);
+    }
+    const double basis = 1.0 / 3.0;
+    R[(long)node * DPN + 0] += basis * v0 * sum;
+    if constexpr (DPN == 3) {
+        R[(long)node * DPN + 1] += basis * v1 * sum;
+        R[(long)node * DPN + 2] += basis * v2 * sum;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// applyBlockConstraints (src/ApplyConstraints.hpp:40-112), one thread per
+// constrained dof as in the reference, then the RHS overwrite.
+// ---------------------------------------------------------------------------
+template <class VT>
+__global__ void k_constraints_1(int dpn, const int* __restrict__ rowmap, const int* __restrict__ colidx,
+                                VT* __restrict__ vals, double* __restrict__ rhs, int nbc, const int* __restrict__ dofs,
+                                const double* __restrict__ bcv, double scale) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbc) return;
+    const int rdof = dofs[b];
+    const double val = scale * bcv[b];
+    const int rnode = rdof / dpn, lr = rdof % dpn;
+    for (int off = rowmap[rnode]; off < rowmap[rnode + 1]; ++off) {
+        const int cnode = colidx[off];
+        for (int lc = 0; lc < dpn; ++lc) {
+            const long ent = (long)dpn * dpn * off + dpn * lr + lc;
+            const int cdof = dpn * cnode + lc;
+            if (cdof == rdof) {
+                vals[ent] = (VT)1.0;
+            } else {
+                const double a = (double)vals[ent];
+                if (val != 0.0) atomicAdd(&rhs[cdof], -a * val);
+                vals[ent] = (VT)0.0;
+                for (int o2 = rowmap[cnode]; o2 < rowmap[cnode + 1]; ++o2)
+                    if (dpn * colidx[o2] + lr == rdof) vals[(long)dpn * dpn * o2 + dpn * lc + lr] = (VT)0.0;
+            }
+        }
+    }
+}
+
+__global__ void k_constraints_2(int nbc, const int* __restrict__ dofs, const double* __restrict__ bcv, double scale,
+                                double* __restrict__ rhs) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nbc) rhs[dofs[b]] = scale * bcv[b];
+}
+
+// y = A x + beta y on the block graph, one group of 4 lanes per block row.
+template <int BI, int BJ, class VT>
+__global__ __launch_bounds__(kBlock) void k_block_spmv(int N, const int* __restrict__ rowmap,
+                                                       const int* __restrict__ colidx, const VT* __restrict__ vals,
+                                                       const double* __restrict__ x, double beta,
+                                                       double* __restrict__ y) {
+    constexpr int G = 4;
+    const int grp = (blockIdx.x * kBlock + threadIdx.x) / G, lane = threadIdx.x % G;
+    const int row = grp;
+    if (row >= N) return;
+    double acc[BI];
+#pragma unroll
+    for (int i = 0; i < BI; ++i) acc[i] = 0.0;
+    for (int e = rowmap[row] + lane; e < rowmap[row + 1]; e += G) {
+        const int c = colidx[e];
+        double xv[BJ];
+#pragma unroll
+        for (int j = 0; j < BJ; ++j) xv[j] = x[(long)c * BJ + j];
+        const VT* v = vals + (long)e * BI * BJ;
+#pragma unroll
+        for (int i = 0; i < BI; ++i)
+#pragma unroll
+            for (int j = 0; j < BJ; ++j) acc[i] += (double)v[i * BJ + j] * xv[j];
+    }
+#pragma unroll
+    for (int i = 0; i < BI; ++i) acc[i] = group_sum<G>(acc[i]);
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < BI; ++i) {
+            const long o = (long)row * BI + i;
+            y[o] = acc[i] + (beta == 0.0 ? 0.0 : beta * y[o]);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ helpers
+Material material_of(int physics, const double* mat, const double* pen) {
+    Material m{};
+    if (physics == PA_MECHANICS) {
+        const double E = mat[0], nu = mat[1];
+        const double c = E / ((1.0 + nu) * (1.0 - 2.0 * nu));
+        m.lambda = c * nu;
+        m.mu = 1.0 / 2.0 * c * (1.0 - 2.0 * nu);
+    } else {
+        m.k = mat[0];
+    }
+    m.p = pen[0];
+    m.emin = pen[1];
+    return m;
+}
+
+RowMesh row_mesh(const pa_mesh* m) {
+    RowMesh r;
+    r.N = m->N;
+    r.x4 = reinterpret_cast<const double4*>(m->d_coords4.p);
+    r.conn = reinterpret_cast<const int4*>(m->d_conn.p);
+    r.rowmap = m->d_rowmap.p;
+    r.n2c_off = m->d_n2c_off.p;
+    r.pairs = m->d_pairs.p;
+    return r;
+}
+
+constexpr int kG = 8;  // lanes per row
+
+int row_blocks(int N, int G) { return (N + kBlock / G - 1) / (kBlock / G); }
+
+void check_launch() { PA_HIP(hipGetLastError()); }
+
+void natural_bcs(const pa_vector_function* vf, double* R, hipStream_t st) {
+    const pa_mesh* m = vf->mesh;
+    const double scale = vf->physics == PA_MECHANICS ? -1.0 : 1.0;  // evaluate_boundary scales
+    for (const auto& bc : vf->natural_bcs) {
+        auto it = m->side_sets.find(bc.side_set);
+        PA_REQUIRE(it != m->side_sets.end(), PA_ERR_ARG, "natural BC: unknown side set " + bc.side_set);
+        const SideSet& s = it->second;
+        if (s.num_bnodes == 0) continue;
+        const int nb = s.num_bnodes;
+        const dim3 grid((nb + 255) / 256), blk(256);
+        if (vf->dpn == 3)
+            k_natural_bc<3><<<grid, blk, 0, st>>>(nb, s.d_bnodes.p, s.d_bface_off.p, s.d_bface.p, s.d_cells.p,
+                                                  s.d_lnodes.p, reinterpret_cast<const int4*>(m->d_conn.p),
+                                                  reinterpret_cast<const double4*>(m->d_coords4.p), bc.values[0],
+                                                  bc.values[1], bc.values[2], scale, R);
+        else
+            k_natural_bc<1><<<grid, blk, 0, st>>>(nb, s.d_bnodes.p, s.d_bface_off.p, s.d_bface.p, s.d_cells.p,
+                                                  s.d_lnodes.p, reinterpret_cast<const int4*>(m->d_conn.p),
+                                                  reinterpret_cast<const double4*>(m->d_coords4.p), bc.values[0],
+                                                  0.0, 0.0, scale, R);
+        check_launch();
+    }
+}
+
+template <int DPN, int KMAX, class VT, bool RES, bool TR>
+void launch_jac(const pa_vector_function* vf, const double* u, const double* z, VT* vals, double* R, hipStream_t st) {
+    const pa_mesh* m = vf->mesh;
+    const Material mt = material_of(vf->physics, vf->material, vf->penalty);
+    const double body = vf->has_body ? vf->body_value : 0.0;
+    // wider groups (fewer rows per block) keep the LDS rows of high-degree meshes small
+    constexpr int G = KMAX <= 16 ? kG : 32;
+    k_jacobian_rows<DPN, G, KMAX, VT, RES, TR>
+        <<<row_blocks(m->N, G), kBlock, 0, st>>>(row_mesh(m), mt, u, z, body, vals, R);
+    check_launch();
+}
+
+template <int DPN, class VT, bool RES, bool TR>
+void dispatch_jac(const pa_vector_function* vf, const double* u, const double* z, VT* vals, double* R,
+                  hipStream_t st) {
+    const int K = vf->mesh->max_row_len;
+    if (K <= 16) launch_jac<DPN, 16, VT, RES, TR>(vf, u, z, vals, R, st);
+    else if (K <= 32) launch_jac<DPN, 32, VT, RES, TR>(vf, u, z, vals, R, st);
+    else if (K <= 64 || DPN == 1) launch_jac<DPN, 64, VT, RES, TR>(vf, u, z, vals, R, st);
+    else throw Error(PA_ERR_LIMIT, "gradient_u: node degree above 63 is not supported");
+}
+
+template <class VT, bool RES, bool TR>
+void jacobian(const pa_vector_function* vf, const double* u, const double* z, VT* vals, double* R, hipStream_t st) {
+    PA_REQUIRE(vf->mesh->max_row_len <= 64, PA_ERR_LIMIT, "gradient_u: node degree above 63 is not supported");
+    if (vf->dpn == 3) dispatch_jac<3, VT, RES, TR>(vf, u, z, vals, R, st);
+    else dispatch_jac<1, VT, RES, TR>(vf, u, z, vals, R, st);
+    if (RES) natural_bcs(vf, R, st);
+}
+
+void residual(const pa_vector_function* vf, const double* u, const double* z, double* R, hipStream_t st) {
+    const pa_mesh* m = vf->mesh;
+    const Material mt = material_of(vf->physics, vf->material, vf->penalty);
+    const double body = vf->has_body ? vf->body_value : 0.0;
+    if (vf->dpn == 3)
+        k_residual_rows<3, kG><<<row_blocks(m->N, kG), kBlock, 0, st>>>(row_mesh(m), mt, u, z, 0.0, 1.0, R);
+    else
+        k_residual_rows<1, kG><<<row_blocks(m->N, kG), kBlock, 0, st>>>(row_mesh(m), mt, u, z, body, 1.0, R);
+    check_launch();
+    natural_bcs(vf, R, st);
+}
+
+void check_vf(const pa_vector_function* vf, const void* a, const void* b, const void* c) {
+    PA_REQUIRE(vf && vf->mesh, PA_ERR_ARG, "vector function: null handle");
+    PA_REQUIRE(a && b && c, PA_ERR_ARG, "vector function: null device pointer");
+}
+
+}  // namespace
+}  // namespace pa
+
+using pa::guarded;
+
+extern "C" {
+
+int pa_vector_function_create(pa_mesh* mesh, int physics, const double* material, const double* penalty,
+                              pa_vector_function** out) {
+    return guarded([&] {
+        PA_REQUIRE(mesh && material && penalty && out, PA_ERR_ARG, "pa_vector_function_create: null argument");
+        PA_REQUIRE(physics == PA_MECHANICS || physics == PA_THERMAL, PA_ERR_ARG,
+                   "pa_vector_function_create: unknown physics");
+        if (physics == PA_MECHANICS)
+            PA_REQUIRE(material[1] > -1.0 && material[1] < 0.5, PA_ERR_ARG, "Poissons Ratio out of (-1, 0.5)");
+        auto* vf = new pa_vector_function();
+        vf->mesh = mesh;
+        vf->physics = physics;
+        vf->dpn = physics == PA_MECHANICS ? 3 : 1;
+        vf->material[0] = material[0];
+        vf->material[1] = physics == PA_MECHANICS ? material[1] : 0.0;
+        vf->penalty[0] = penalty[0];
+        vf->penalty[1] = penalty[1];
+        *out = vf;
+    });
+}
+
+int pa_vector_function_destroy(pa_vector_function* vf) {
+    return guarded([&] { delete vf; });
+}
+
+int pa_vector_function_dofs_per_node(const pa_vector_function* vf, int* dpn) {
+    return guarded([&] {
+        PA_REQUIRE(vf && dpn, PA_ERR_ARG, "null argument");
+        *dpn = vf->dpn;
+    });
+}
+
+int pa_vector_function_add_uniform_natural_bc(pa_vector_function* vf, const char* side_set, const double* values) {
+    return guarded([&] {
+        PA_REQUIRE(vf && side_set && values, PA_ERR_ARG, "pa_vector_function_add_uniform_natural_bc: null argument");
+        PA_REQUIRE(vf->mesh->side_sets.count(side_set), PA_ERR_ARG,
+                   std::string("pa_vector_function_add_uniform_natural_bc: unknown side set ") + side_set);
+        NaturalBC bc;
+        bc.side_set = side_set;
+        for (int i = 0; i < 3; ++i) bc.values[i] = i < vf->dpn ? values[i] : 0.0;
+        vf->natural_bcs.push_back(bc);
+    });
+}
+
+int pa_vector_function_add_body_load(pa_vector_function* vf, int dof, double value) {
+    return guarded([&] {
+        PA_REQUIRE(vf, PA_ERR_ARG, "pa_vector_function_add_body_load: null handle");
+        PA_REQUIRE(vf->physics == PA_THERMAL && dof == 0, PA_ERR_ARG,
+                   "pa_vector_function_add_body_load: only the thermal source (dof 0) is supported");
+        vf->has_body = true;
+        vf->body_dof = dof;
+        vf->body_value += value;
+    });
+}
+
+int pa_vector_function_value(const pa_vector_function* vf, const double* u, const double* z, double* R,
+                             void* stream) {
+    return guarded([&] {
+        pa::check_vf(vf, u, z, R);
+        pa::residual(vf, u, z, R, (hipStream_t)stream);
+    });
+}
+
+int pa_vector_function_gradient_u(const pa_vector_function* vf, const double* u, const double* z, void* vals,
+                                  int precision, void* stream) {
+    return guarded([&] {
+        pa::check_vf(vf, u, z, vals);
+        if (precision == PA_FP64)
+            pa::jacobian<double, false, false>(vf, u, z, (double*)vals, nullptr, (hipStream_t)stream);
+        else if (precision == PA_FP32)
+            pa::jacobian<float, false, false>(vf, u, z, (float*)vals, nullptr, (hipStream_t)stream);
+        else
+            throw pa::Error(PA_ERR_ARG, "gradient_u: unknown precision");
+    });
+}
+
+int pa_vector_function_gradient_u_T(const pa_vector_function* vf, const double* u, const double* z, void* vals,
+                                    int precision, void* stream) {
+    return guarded([&] {
+        pa::check_vf(vf, u, z, vals);
+        if (precision == PA_FP64)
+            pa::jacobian<double, false, true>(vf, u, z, (double*)vals, nullptr, (hipStream_t)stream);
+        else if (precision == PA_FP32)
+            pa::jacobian<float, false, true>(vf, u, z, (float*)vals, nullptr, (hipStream_t)stream);
+        else
+            throw pa::Error(PA_ERR_ARG, "gradient_u_T: unknown precision");
+    });
+}
+
+int pa_vector_function_value_gradient_u(const pa_vector_function* vf, const double* u, const double* z, double* R,
+                                        void* vals, int precision, void* stream) {
+    return guarded([&] {
+        pa::check_vf(vf, u, z, vals);
+        PA_REQUIRE(R, PA_ERR_ARG, "value_gradient_u: null residual");
+        if (precision == PA_FP64)
+            pa::jacobian<double, true, false>(vf, u, z, (double*)vals, R, (hipStream_t)stream);
+        else if (precision == PA_FP32)
+            pa::jacobian<float, true, false>(vf, u, z, (float*)vals, R, (hipStream_t)stream);
+        else
+            throw pa::Error(PA_ERR_ARG, "value_gradient_u: unknown precision");
+    });
+}
+
+int pa_vector_function_gradient_z(const pa_vector_function* vf, const double* u, const double* z, double* vals,
+                                  void* stream) {
+    return guarded([&] {
+        pa::check_vf(vf, u, z, vals);
+        const pa_mesh* m = vf->mesh;
+        PA_REQUIRE(m->max_row_len <= 64, PA_ERR_LIMIT, "gradient_z: node degree above 63 is not supported");
+        const pa::Material mt = pa::material_of(vf->physics, vf->material, vf->penalty);
+        const double body = vf->has_body ? vf->body_value : 0.0;
+        const hipStream_t st = (hipStream_t)stream;
+        const int nb8 = pa::row_blocks(m->N, pa::kG), nb32 = pa::row_blocks(m->N, 32);
+        const double bd = vf->dpn == 3 ? 0.0 : body;
+        if (vf->dpn == 3) {
+            if (m->max_row_len <= 16)
+                pa::k_gradz_rows<3, pa::kG, 16><<<nb8, pa::kBlock, 0, st>>>(pa::row_mesh(m), mt, u, z, bd, vals);
+            else
+                pa::k_gradz_rows<3, 32, 64><<<nb32, pa::kBlock, 0, st>>>(pa::row_mesh(m), mt, u, z, bd, vals);
+        } else {
+            if (m->max_row_len <= 16)
+                pa::k_gradz_rows<1, pa::kG, 16><<<nb8, pa::kBlock, 0, st>>>(pa::row_mesh(m), mt, u, z, bd, vals);
+            else
+                pa::k_gradz_rows<1, 32, 64><<<nb32, pa::kBlock, 0, st>>>(pa::row_mesh(m), mt, u, z, bd, vals);
+        }
+        pa::check_launch();
+    });
+}
+
+int pa_scalar_function_create(pa_mesh* mesh, int type, const double* material, const double* penalty,
+                              pa_scalar_function** out) {
+    return guarded([&] {
+        PA_REQUIRE(mesh && material && penalty && out, PA_ERR_ARG, "pa_scalar_function_create: null argument");
+        PA_REQUIRE(type == PA_INTERNAL_ELASTIC_ENERGY || type == PA_INTERNAL_THERMAL_ENERGY, PA_ERR_ARG,
+                   "pa_scalar_function_create: unknown type");
+        auto* sf = new pa_scalar_function();
+        sf->mesh = mesh;
+        sf->type = type;
+        sf->dpn = type == PA_INTERNAL_ELASTIC_ENERGY ? 3 : 1;
+        sf->material[0] = material[0];
+        sf->material[1] = type == PA_INTERNAL_ELASTIC_ENERGY ? material[1] : 0.0;
+        sf->penalty[0] = penalty[0];
+        sf->penalty[1] = penalty[1];
+        try {
+            sf->d_partials.alloc(1024);
+        } catch (...) {
+            delete sf;
+            throw;
+        }
+        *out = sf;
+    });
+}
+
+int pa_scalar_function_destroy(pa_scalar_function* sf) {
+    return guarded([&] { delete sf; });
+}
+
+int pa_scalar_function_value(const pa_scalar_function* sf, const double* u, const double* z, double* value,
+                             void* stream) {
+    return guarded([&] {
+        PA_REQUIRE(sf && u && z && value, PA_ERR_ARG, "pa_scalar_function_value: null argument");
+        const pa_mesh* m = sf->mesh;
+        const int phys = sf->dpn == 3 ? PA_MECHANICS : PA_THERMAL;
+        const pa::Material mt = pa::material_of(phys, sf->material, sf->penalty);
+        const hipStream_t st = (hipStream_t)stream;
+        const int nb = (int)std::min<long>(1024, (m->C + pa::kBlock - 1) / pa::kBlock);
+        const auto* x4 = reinterpret_cast<const double4*>(m->d_coords4.p);
+        const auto* c4 = reinterpret_cast<const int4*>(m->d_conn.p);
+        if (sf->dpn == 3) pa::k_energy_cells<3><<<nb, pa::kBlock, 0, st>>>(m->C, x4, c4, mt, u, z, sf->d_partials.p);
+        else pa::k_energy_cells<1><<<nb, pa::kBlock, 0, st>>>(m->C, x4, c4, mt, u, z, sf->d_partials.p);
+        pa::check_launch();
+        pa::k_sum_partials<<<1, 64, 0, st>>>(nb, sf->d_partials.p, value);
+        pa::check_launch();
+    });
+}
+
+int pa_scalar_function_gradient_u(const pa_scalar_function* sf, const double* u, const double* z, double* g,
+                                  void* stream) {
+    return guarded([&] {
+        PA_REQUIRE(sf && u && z && g, PA_ERR_ARG, "pa_scalar_function_gradient_u: null argument");
+        const pa_mesh* m = sf->mesh;
+        const int phys = sf->dpn == 3 ? PA_MECHANICS : PA_THERMAL;
+        const pa::Material mt = pa::material_of(phys, sf->material, sf->penalty);
+        const hipStream_t st = (hipStream_t)stream;
+        const int nb = pa::row_blocks(m->N, pa::kG);
+        // d/du of 1/2 u.K u is K u; of the thermal energy u.K u it is 2 K u.
+        if (sf->dpn == 3)
+            pa::k_residual_rows<3, pa::kG><<<nb, pa::kBlock, 0, st>>>(pa::row_mesh(m), mt, u, z, 0.0, 1.0, g);
+        else
+            pa::k_residual_rows<1, pa::kG><<<nb, pa::kBlock, 0, st>>>(pa::row_mesh(m), mt, u, z, 0.0, 2.0, g);
+        pa::check_launch();
+    });
+}
+
+int pa_scalar_function_gradient_z(const pa_scalar_function* sf, const double* u, const double* z, double* g,
+                                  void* stream) {
+    return guarded([&] {
+        PA_REQUIRE(sf && u && z && g, PA_ERR_ARG, "pa_scalar_function_gradient_z: null argument");
+        const pa_mesh* m = sf->mesh;
+        const int phys = sf->dpn == 3 ? PA_MECHANICS : PA_THERMAL;
+        const pa::Material mt = pa::material_of(phys, sf->material, sf->penalty);
+        const hipStream_t st = (hipStream_t)stream;
+        const int nb = pa::row_blocks(m->N, pa::kG);
+        if (sf->dpn == 3) pa::k_energy_gradz_rows<3, pa::kG><<<nb, pa::kBlock, 0, st>>>(pa::row_mesh(m), mt, u, z, g);
+        else pa::k_energy_gradz_rows<1, pa::kG><<<nb, pa::kBlock, 0, st>>>(pa::row_mesh(m), mt, u, z, g);
+        pa::check_launch();
+    });
+}
+
+int pa_apply_block_constraints(const pa_mesh* m, int dpn, void* vals, int precision, double* rhs, int nbc,
+                               const int* dofs, const double* bcv, double scale, void* stream) {
+    return guarded([&] {
+        PA_REQUIRE(m && vals && rhs, PA_ERR_ARG, "pa_apply_block_constraints: null argument");
+        PA_REQUIRE(dpn >= 1 && dpn <= 4, PA_ERR_ARG, "pa_apply_block_constraints: dpn out of range");
+        if (nbc == 0) return;
+        PA_REQUIRE(nbc > 0 && dofs && bcv, PA_ERR_ARG, "pa_apply_block_constraints: bad constraint list");
+        const hipStream_t st = (hipStream_t)stream;
+        const dim3 grid((nbc + 255) / 256), blk(256);
+        if (precision == PA_FP64)
+            pa::k_constraints_1<double><<<grid, blk, 0, st>>>(dpn, m->d_rowmap.p, m->d_colidx.p, (double*)vals, rhs,
+                                                              nbc, dofs, bcv, scale);
+        else if (precision == PA_FP32)
+            pa::k_constraints_1<float><<<grid, blk, 0, st>>>(dpn, m->d_rowmap.p, m->d_colidx.p, (float*)vals, rhs,
+                                                             nbc, dofs, bcv, scale);
+        else
+            throw pa::Error(PA_ERR_ARG, "pa_apply_block_constraints: unknown precision");
+        pa::check_launch();
+        pa::k_constraints_2<<<grid, blk, 0, st>>>(nbc, dofs, bcv, scale, rhs);
+        pa::check_launch();
+    });
+}
+
+int pa_block_spmv(const pa_mesh* m, int bi, int bj, const void* vals, int precision, const double* x, double beta,
+                  double* y, void* stream) {
+    return guarded([&] {
+        PA_REQUIRE(m && vals && x && y, PA_ERR_ARG, "pa_block_spmv: null argument");
+        const hipStream_t st = (hipStream_t)stream;
+        const int nb = (int)(((long)m->N * 4 + pa::kBlock - 1) / pa::kBlock);
+        const int* rm = m->d_rowmap.p;
+        const int* ci = m->d_colidx.p;
+#define PA_SPMV(BI, BJ)                                                                                          \
+    if (bi == BI && bj == BJ) {                                                                                  \
+        if (precision == PA_FP64)                                                                                \
+            pa::k_block_spmv<BI, BJ, double><<<nb, pa::kBlock, 0, st>>>(m->N, rm, ci, (const double*)vals, x,    \
+                                                                         beta, y);                               \
+        else                                                                                                     \
+            pa::k_block_spmv<BI, BJ, float><<<nb, pa::kBlock, 0, st>>>(m->N, rm, ci, (const float*)vals, x,      \
+                                                                        beta, y);                                \
+        pa::check_launch();                                                                                      \
+        return;                                                                                                  \
+    }
+        PA_REQUIRE(precision == PA_FP64 || precision == PA_FP32, PA_ERR_ARG, "pa_block_spmv: unknown precision");
+        PA_SPMV(3, 3)
+        PA_SPMV(1, 1)
+        PA_SPMV(1, 3)
+        PA_SPMV(3, 1)
+        PA_SPMV(4, 4)
+#undef PA_SPMV
+        throw pa::Error(PA_ERR_ARG, "pa_block_spmv: unsupported block shape");
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,134 @@
+// Internal types shared by the host code and the HIP kernels of the library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/platoanalyze_amd.h"
+
+namespace pa {
+
+// Error carried to the C ABI as (status, message).
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+
+#define PA_HIP(call)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            throw ::pa::Error(PA_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+#define PA_REQUIRE(cond, code, msg)                         \
+    do {                                                    \
+        if (!(cond)) throw ::pa::Error((code), (msg));      \
+    } while (0)
+
+// Runs f, converting exceptions to a status code and the thread's last error.
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return PA_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return PA_ERR_STATE;
+    } catch (...) {
+        set_last_error("unknown error");
+        return PA_ERR_STATE;
+    }
+}
+
+// Owning device buffer.
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        release();
+        if (count) PA_HIP(hipMalloc(&p, count * sizeof(T)));
+        n = count;
+    }
+    void upload(const T* h, size_t count) {
+        alloc(count);
+        if (count) PA_HIP(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice));
+    }
+};
+
+// Side set: faces given as (cell, three local node ordinals), plus the
+// derived boundary-node -> face lists used by the node-ordered load kernel.
+struct SideSet {
+    std::vector<int> cells, lnodes;    // host, F and F*3
+    DevBuf<int> d_cells, d_lnodes;     // device copies
+    DevBuf<int> d_bnodes;              // distinct nodes touched, sorted
+    DevBuf<int> d_bface_off;           // nb+1
+    DevBuf<int> d_bface;               // entries: face*4 + position in face (0..2)
+    int num_bnodes = 0;
+};
+
+}  // namespace pa
+
+// Mesh with the assembly graph and the node -> cell incidence used by the
+// node-ordered assembly kernels.
+struct pa_mesh {
+    int N = 0, C = 0;
+    long long nnzb = 0;
+    int max_row_len = 0;
+    int max_cells_per_node = 0;
+    std::vector<int> h_rowmap, h_colidx;
+    pa::DevBuf<double> d_coords;   // N*3 as given
+    pa::DevBuf<double> d_coords4;  // N*4 (x, y, z, 0) for 32-byte gathers
+    pa::DevBuf<int> d_conn;        // C*4
+    pa::DevBuf<int> d_rowmap;      // N+1
+    pa::DevBuf<int> d_colidx;      // nnzb
+    pa::DevBuf<int> d_n2c_off;     // N+1
+    pa::DevBuf<int2> d_pairs;      // 4C: x = cell<<2 | local index of the row node, y = 4 packed uint8 column slots
+    std::map<std::string, pa::SideSet> side_sets;
+    std::map<std::string, std::vector<int>> node_sets;
+};
+
+struct NaturalBC {
+    std::string side_set;
+    double values[3];
+};
+
+struct pa_vector_function {
+    pa_mesh* mesh = nullptr;
+    int physics = PA_MECHANICS;
+    int dpn = 3;
+    double material[2] = {0, 0};
+    double penalty[2] = {3.0, 1e-9};
+    std::vector<NaturalBC> natural_bcs;
+    bool has_body = false;
+    int body_dof = 0;
+    double body_value = 0.0;
+};
+
+struct pa_scalar_function {
+    pa_mesh* mesh = nullptr;
+    int type = PA_INTERNAL_ELASTIC_ENERGY;
+    int dpn = 3;
+    double material[2] = {0, 0};
+    double penalty[2] = {3.0, 1e-9};
+    pa::DevBuf<double> d_partials;  // per-block partial sums of the value reduction
+};
