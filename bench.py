@@ -1,0 +1,197 @@
+"""Benchmark of the Tet4 residual + Jacobian assembly on one MI355X.
+
+A step is one VectorFunction::value plus one VectorFunction::gradient_u of
+SIMP elastostatics on the 128x64x64 hex grid split into 6 tets per hex
+(3,145,728 cells, 545,025 nodes, 1,635,075 dofs; E=1, nu=0.3, p=3,
+Emin=1e-9, Gaussian-random-field densities with tanh projection beta=8,
+eta=0.5), FP64, inputs resident in HBM.  The metric is cells/s of the pair.
+
+The JSON line also carries the roofline of the dominant kernel (the
+gradient_u assembly kernel, timed with HIP events on the stream it runs on)
+and the CPU baseline (the oracle's reference-structured restatement, OpenMP,
+on a 32x64x64 slab of the same grid).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def read_traffic(name):
+    """HBM bytes per launch from the committed counter summary, if present."""
+    p = ROOT / "profiles" / "traffic.txt"
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        if line.startswith(name + ":"):
+            return float(line.split(":", 1)[1].split()[0])
+    return None
+
+
+def cpu_baseline(nx_slab=32):
+    import oracle
+    from platoanalyze_amd.workloads import grf_density
+
+    threads = int(os.environ.get("PA_CPU_THREADS", min(16, os.cpu_count() or 1)))
+    threads = oracle.set_threads(threads)
+    dims = (nx_slab, 64, 64)
+    coords, conn = oracle.box_mesh(*dims, 2.0 * nx_slab / 128, 1.0, 1.0)
+    N = coords.size // 3
+    C = conn.size // 4
+    z = grf_density(dims, 4.0, 8.0, 0.5, 2)
+    u = np.random.default_rng(11).standard_normal(3 * N) * 1e-3
+    fc, fl = oracle.side_set(coords, conn, 0, 2.0 * nx_slab / 128)
+    m = oracle.Model(coords, conn, youngs=1.0, poisson=0.3, penalty_exponent=3.0, penalty_min=1e-9, face_cells=fc,
+                     face_lnodes=fl, flux=(0.0, 0.0, -1.0))
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        m.value("elastic", u, z)
+        m.gradient_u("elastic", u, z)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    return {"value": C / t, "unit": "cells/s", "cores": threads, "kind": "port",
+            "sample": f"oracle value+gradient_u (FAD, per-call graph, row-search atomic scatter) on a "
+                      f"{dims[0]}x{dims[1]}x{dims[2]} slab of the same grid ({C} cells), median of 3"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--mode", default="bench", choices=["bench", "copy", "profile"],
+                    help="copy: run only the calibration copy kernel; profile: kernels only, no CPU leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(the multi-GPU path is described in DESIGN.md, not implemented)", file=sys.stderr)
+        sys.exit(2)
+
+    import platoanalyze_amd as pa
+    from platoanalyze_amd.workloads import WORKLOADS, algorithmic_bytes, density
+
+    stream = pa.Stream()
+    if args.mode == "copy":
+        nbytes = 1 << 30
+        a = pa.DeviceArray(nbytes // 8)
+        b = pa.DeviceArray(nbytes // 8)
+        a.zero()
+        for _ in range(args.warmup + args.steps):
+            b.copy_kernel_from(a, stream)
+        stream.synchronize()
+        log(f"copy kernel: {args.warmup + args.steps} x {nbytes} bytes")
+        return
+
+    name = "elastic_128x64x64"
+    physics, dims, L, dspec = WORKLOADS[name]
+    log(f"bench: building {name}")
+    t0 = time.perf_counter()
+    mesh = pa.Mesh.box(*dims, *L)
+    setup_s = time.perf_counter() - t0
+    N, C, nnzb = mesh.num_nodes, mesh.num_cells, mesh.num_blocks
+    z = pa.DeviceArray.from_numpy(density(dims, dspec))
+    u = pa.DeviceArray.from_numpy(np.random.default_rng(11).standard_normal(3 * N) * 1e-3)
+    vf = pa.VectorFunction(mesh, physics, youngs=1.0, poisson=0.3, penalty_exponent=3.0, penalty_min=1e-9)
+    vf.add_uniform_natural_bc("x+", (0.0, 0.0, -1.0))
+    R = pa.DeviceArray(3 * N)
+    K64 = pa.DeviceArray(9 * nnzb, np.float64)
+    K32 = pa.DeviceArray(9 * nnzb, np.float32)
+    log(f"bench: mesh + graph built in {setup_s:.2f}s (C={C}, N={N}, nnzb={nnzb})")
+
+    def step():
+        vf.value(u, z, out=R, stream=stream)
+        vf.gradient_u(u, z, out=K64, stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    stream.synchronize()
+    e0, e1 = pa.Event(), pa.Event()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    stream.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = e0.elapsed_ms(e1) / args.steps
+    log(f"bench: {args.steps} steps, {ms_step:.4f} ms/step (events), wall {wall * 1e3 / args.steps:.4f} ms/step")
+
+    def timed(fn, reps):
+        fn()
+        stream.synchronize()
+        a, b = pa.Event(), pa.Event()
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        stream.synchronize()
+        return a.elapsed_ms(b) / reps
+
+    reps = max(args.steps, 10)
+    t_jac = timed(lambda: vf.gradient_u(u, z, out=K64, stream=stream), reps)
+    t_val = timed(lambda: vf.value(u, z, out=R, stream=stream), reps)
+    t_jac32 = timed(lambda: vf.gradient_u(u, z, precision=pa.FP32, out=K32, stream=stream), reps)
+    t_fused = timed(lambda: vf.value_gradient_u(u, z, r_out=R, out=K64, stream=stream), reps)
+    t_fused32 = timed(lambda: vf.value_gradient_u(u, z, precision=pa.FP32, r_out=R, out=K32, stream=stream), reps)
+    log(f"bench: gradient_u {t_jac:.4f} ms, value {t_val:.4f} ms, gradient_u fp32 {t_jac32:.4f} ms, "
+        f"fused {t_fused:.4f} ms, fused fp32 {t_fused32:.4f} ms")
+    if args.mode == "profile":
+        return
+
+    jac_bytes = algorithmic_bytes(C, N, nnzb, 3, 8, with_residual=False)
+    step_bytes = algorithmic_bytes(C, N, nnzb, 3, 8)
+    achieved = jac_bytes / (t_jac * 1e-3) / 1e9
+    traffic = read_traffic("k_jacobian_rows_fp64_elastic_128x64x64")
+    cells_s = C / (ms_step * 1e-3)
+    out = {
+        "metric": "cells/s res+Jac assembly",
+        "value": cells_s,
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "SIMP elastostatics Tet4 value+gradient_u, 128x64x64 hex x6 (3,145,728 cells, "
+                               "545,025 nodes, 1,635,075 dof), GRF density + tanh projection beta=8",
+                   "cells": C, "nodes": N, "blocks": nnzb, "precision": "fp64"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_jacobian_rows<3,8,16,double> (gradient_u, FP64)",
+                     "algorithmic_bytes_per_launch": jac_bytes, "avg_launch_ms": t_jac},
+        "kernels_ms": {"gradient_u_fp64": t_jac, "value": t_val, "gradient_u_fp32": t_jac32,
+                       "value_gradient_u_fused_fp64": t_fused, "value_gradient_u_fused_fp32": t_fused32},
+        "step_algorithmic_GBs": step_bytes / (ms_step * 1e-3) / 1e9,
+        "fused_cells_per_s": {"fp64": C / (t_fused * 1e-3), "fp32": C / (t_fused32 * 1e-3)},
+        "setup_s": setup_s,
+    }
+    if not args.no_cpu:
+        log("bench: CPU baseline (oracle, OpenMP)")
+        out["cpu_baseline"] = cpu_baseline()
+        log(f"bench: CPU baseline {out['cpu_baseline']['value']:.4g} cells/s on {out['cpu_baseline']['cores']} threads")
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -55,6 +55,9 @@ int pa_memcpy_h2d(void* dst_dev, const void* src_host, size_t bytes);
 int pa_memcpy_d2h(void* dst_host, const void* src_dev, size_t bytes);
 int pa_memcpy_d2d(void* dst_dev, const void* src_dev, size_t bytes, void* stream);
 int pa_memset(void* ptr_dev, int value, size_t bytes, void* stream);
+/* Device-to-device copy by a plain streaming kernel (16-byte vectors); bytes
+ * must be a multiple of 16.  Used to calibrate HBM byte counters. */
+int pa_copy_kernel(void* dst_dev, const void* src_dev, size_t bytes, void* stream);
 int pa_stream_create(void** stream);
 int pa_stream_destroy(void* stream);
 int pa_stream_synchronize(void* stream);

@@ -51,6 +51,7 @@ def _load():
         "pa_memcpy_d2h": ([vp, vp, sz], i),
         "pa_memcpy_d2d": ([vp, vp, sz, vp], i),
         "pa_memset": ([vp, i, sz, vp], i),
+        "pa_copy_kernel": ([vp, vp, sz, vp], i),
         "pa_stream_create": ([ctypes.POINTER(vp)], i),
         "pa_stream_destroy": ([vp], i),
         "pa_stream_synchronize": ([vp], i),
@@ -151,6 +152,10 @@ class DeviceArray:
     def copy_from(self, other, stream=None):
         assert other.nbytes == self.nbytes
         _check(lib.pa_memcpy_d2d(self._ptr, other._ptr, self.nbytes, stream))
+
+    def copy_kernel_from(self, other, stream=None):
+        assert other.nbytes == self.nbytes
+        _check(lib.pa_copy_kernel(self._ptr, other._ptr, self.nbytes, _sh(stream)))
 
     def free(self):
         if self._ptr is not None and self._ptr.value:

@@ -616,6 +616,10 @@ __global__ __launch_bounds__(kBlock) void k_block_spmv(int N, const int* __restr
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_copy16(long n, const int4* __restrict__ src, int4* __restrict__ dst) {
+    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) dst[i] = src[i];
+}
+
 // ------------------------------------------------------------------ helpers
 Material material_of(int physics, const double* mat, const double* pen) {
     Material m{};
@@ -726,6 +730,17 @@ void check_vf(const pa_vector_function* vf, const void* a, const void* b, const 
 using pa::guarded;
 
 extern "C" {
+
+int pa_copy_kernel(void* dst, const void* src, size_t bytes, void* stream) {
+    return guarded([&] {
+        PA_REQUIRE(dst && src && bytes % 16 == 0, PA_ERR_ARG, "pa_copy_kernel: bytes must be a multiple of 16");
+        const long n = (long)(bytes / 16);
+        if (n == 0) return;
+        const int nb = (int)std::min<long>((n + pa::kBlock - 1) / pa::kBlock, 8192);
+        pa::k_copy16<<<nb, pa::kBlock, 0, (hipStream_t)stream>>>(n, (const int4*)src, (int4*)dst);
+        pa::check_launch();
+    });
+}
 
 int pa_vector_function_create(pa_mesh* mesh, int physics, const double* material, const double* penalty,
                               pa_vector_function** out) {
