@@ -195,6 +195,76 @@ int pa_apply_block_constraints(const pa_mesh* mesh, int dpn, void* values_dev, i
 int pa_block_spmv(const pa_mesh* mesh, int bi, int bj, const void* values_dev, int precision, const double* x_dev,
                   double beta, double* y_dev, void* stream);
 
+/* -------------------------------------------------------------- solver */
+/*
+ * Preconditioned conjugate gradients on a block-CRS matrix of the mesh graph
+ * (square dpn x dpn blocks).  Replaces the reference's linear-solver stacks
+ * for this path (AztecOO CG, src/alg/EpetraLinearSolver.cpp:192-288; AmgX
+ * PCG, src/alg/AmgXConfigs.cpp:110-134).  Vectors and all reductions are
+ * FP64; the matrix is FP64 or FP32 per `precision`.
+ */
+enum { PA_PRECOND_NONE = 0, PA_PRECOND_JACOBI = 1, PA_PRECOND_BLOCK_JACOBI = 2 };
+
+typedef struct pa_solver pa_solver;
+typedef struct {
+    int iterations;      /* CG iterations performed (summed over refinement sweeps) */
+    int sweeps;          /* refinement sweeps (1 for a plain solve) */
+    double rel_residual; /* final ||b - A x|| / ||b|| (FP64 residual) */
+    int converged;       /* 1 if rel_residual <= tolerance */
+} pa_solve_stats;
+
+int pa_solver_create(const pa_mesh* mesh, int dpn, int precond, pa_solver** solver);
+int pa_solver_destroy(pa_solver* solver);
+/*
+ * Solve A x = b starting from the x given.  Stops when the recursive residual
+ * satisfies ||r|| <= tol ||b|| (checked every `check_every` iterations; the
+ * host reads one scalar then) or after max_iter iterations; the returned
+ * rel_residual is recomputed in FP64 from A.  Synchronizes `stream`.
+ */
+int pa_solver_pcg(pa_solver* solver, const void* values_dev, int precision, const double* b_dev, double* x_dev,
+                  double tol, int max_iter, int check_every, pa_solve_stats* stats, void* stream);
+
+/* -------------------------------------------------------------- problem */
+/*
+ * Elliptic::Problem (src/elliptic/Problem.hpp:237-450): solution (one Newton
+ * step: R = value(0, z), K = gradient_u, applyBlockConstraints, solve K du =
+ * -R), criterion value and the self-adjoint criterion gradient
+ * dF/dz = dF/dz|_u + (dR/dz)^T lambda with lambda = -u.
+ *
+ * mode PA_FP64: FP64 assembly and FP64-matrix PCG to `tol`.
+ * mode PA_FP32 (mixed): FP32 assembly and FP32-matrix PCG sweeps to
+ * `inner_tol`, each followed by an FP64 matrix-free residual b - K(z) x
+ * computed by the assembly kernels, until ||r|| <= tol ||b||.
+ */
+typedef struct {
+    int mode;          /* PA_FP64 or PA_FP32 */
+    int precond;       /* PA_PRECOND_* */
+    double tol;        /* relative residual target (FP64 residual) */
+    double inner_tol;  /* relative target of one FP32 sweep (mixed mode) */
+    int max_iter;      /* CG iteration cap per sweep */
+    int max_sweeps;    /* refinement sweep cap (mixed mode) */
+    int check_every;   /* convergence check period (iterations) */
+} pa_problem_options;
+
+typedef struct pa_problem pa_problem;
+int pa_problem_create(pa_vector_function* vf, pa_scalar_function* criterion, int num_bcs, const int* bc_dofs_host,
+                      const double* bc_values_host, const pa_problem_options* options, pa_problem** problem);
+int pa_problem_destroy(pa_problem* problem);
+int pa_problem_set_options(pa_problem* problem, const pa_problem_options* options);
+/* State u (N*dpn) for control z; statistics of the linear solve. Synchronizes. */
+int pa_problem_solution(pa_problem* problem, const double* z_dev, double* u_dev, pa_solve_stats* stats, void* stream);
+/* Criterion value at (u, z), one double at value_dev. */
+int pa_problem_criterion_value(pa_problem* problem, const double* z_dev, const double* u_dev, double* value_dev,
+                               void* stream);
+/*
+ * Total criterion gradient dF/dz (N doubles) with the self-adjoint adjoint
+ * lambda = -u ("Self-Adjoint" = true, src/elliptic/Problem.hpp:423-427), in one
+ * fused node-ordered kernel: per incident cell, dF_e/dz_a + lambda_e.dR_e/dz_a,
+ * without assembling the N x N*dpn dR/dz matrix.
+ */
+int pa_problem_criterion_gradient(pa_problem* problem, const double* z_dev, const double* u_dev, double* grad_dev,
+                                  void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,8 +31,25 @@ MECHANICS, THERMAL = 0, 1
 INTERNAL_ELASTIC_ENERGY, INTERNAL_THERMAL_ENERGY = 0, 1
 
 
+PRECOND_NONE, PRECOND_JACOBI, PRECOND_BLOCK_JACOBI = 0, 1, 2
+
+
 class PlatoError(RuntimeError):
     pass
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("sweeps", ctypes.c_int), ("rel_residual", ctypes.c_double),
+                ("converged", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ProblemOptions(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int), ("precond", ctypes.c_int), ("tol", ctypes.c_double),
+                ("inner_tol", ctypes.c_double), ("max_iter", ctypes.c_int), ("max_sweeps", ctypes.c_int),
+                ("check_every", ctypes.c_int)]
 
 
 def _load():
@@ -86,6 +103,15 @@ def _load():
         "pa_scalar_function_gradient_z": ([vp, vp, vp, vp, vp], i),
         "pa_apply_block_constraints": ([vp, i, vp, i, vp, i, vp, vp, d, vp], i),
         "pa_block_spmv": ([vp, i, i, vp, i, vp, d, vp, vp], i),
+        "pa_solver_create": ([vp, i, i, ctypes.POINTER(vp)], i),
+        "pa_solver_destroy": ([vp], i),
+        "pa_solver_pcg": ([vp, vp, i, vp, vp, d, i, i, ctypes.POINTER(SolveStats), vp], i),
+        "pa_problem_create": ([vp, vp, i, vp, vp, ctypes.POINTER(ProblemOptions), ctypes.POINTER(vp)], i),
+        "pa_problem_destroy": ([vp], i),
+        "pa_problem_set_options": ([vp, ctypes.POINTER(ProblemOptions)], i),
+        "pa_problem_solution": ([vp, vp, vp, ctypes.POINTER(SolveStats), vp], i),
+        "pa_problem_criterion_value": ([vp, vp, vp, vp, vp], i),
+        "pa_problem_criterion_gradient": ([vp, vp, vp, vp, vp], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -449,3 +475,73 @@ def apply_block_constraints(matrix, rhs, bc_dofs, bc_values, scale=1.0, stream=N
     vals = _dev(bc_values)
     _check(lib.pa_apply_block_constraints(matrix.mesh.handle, matrix.bi, matrix.values.ptr, matrix.precision, rhs.ptr,
                                           dofs.size, dofs.ptr, vals.ptr, scale, _sh(stream)))
+
+
+class Solver:
+    """PCG on block-CRS matrices of a mesh graph (FP64 vectors/reductions)."""
+
+    def __init__(self, mesh, dpn, precond=PRECOND_BLOCK_JACOBI):
+        h = ctypes.c_void_p()
+        _check(lib.pa_solver_create(mesh.handle, dpn, precond, ctypes.byref(h)))
+        self.handle, self.mesh, self.dpn = h, mesh, dpn
+
+    def solve(self, matrix, b, x, tol=1e-8, max_iter=10000, check_every=10, stream=None):
+        st = SolveStats()
+        _check(lib.pa_solver_pcg(self.handle, matrix.values.ptr, matrix.precision, b.ptr, x.ptr, tol, max_iter,
+                                 check_every, ctypes.byref(st), _sh(stream)))
+        return st.as_dict()
+
+    def __del__(self):
+        try:
+            lib.pa_solver_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class Problem:
+    """Elliptic::Problem: solution, criterionValue, criterionGradient
+    (src/elliptic/Problem.hpp:237-450), self-adjoint criterion."""
+
+    def __init__(self, vector_function, criterion, bc_dofs, bc_values=None, *, mode=FP64,
+                 precond=PRECOND_BLOCK_JACOBI, tol=1e-8, inner_tol=1e-5, max_iter=20000, max_sweeps=10,
+                 check_every=10):
+        self.vf, self.sf = vector_function, criterion
+        dofs = np.ascontiguousarray(bc_dofs, np.int32)
+        vals = np.zeros(dofs.size) if bc_values is None else np.ascontiguousarray(bc_values, np.float64)
+        self.options = ProblemOptions(_prec(mode)[0], precond, tol, inner_tol, max_iter, max_sweeps, check_every)
+        h = ctypes.c_void_p()
+        _check(lib.pa_problem_create(self.vf.handle, self.sf.handle, dofs.size, _np_ptr(dofs), _np_ptr(vals),
+                                     ctypes.byref(self.options), ctypes.byref(h)))
+        self.handle = h
+        self.last_stats = None
+
+    def set_options(self, **kw):
+        for k, v in kw.items():
+            setattr(self.options, k, _prec(v)[0] if k == "mode" else v)
+        _check(lib.pa_problem_set_options(self.handle, ctypes.byref(self.options)))
+
+    def solution(self, z, out=None, stream=None):
+        z = _dev(z)
+        out = out if out is not None else DeviceArray(self.vf.size())
+        st = SolveStats()
+        _check(lib.pa_problem_solution(self.handle, z.ptr, out.ptr, ctypes.byref(st), _sh(stream)))
+        self.last_stats = st.as_dict()
+        return out
+
+    def criterion_value(self, z, u, out=None, stream=None):
+        z, u = _dev(z), _dev(u)
+        dev = out if out is not None else DeviceArray(1)
+        _check(lib.pa_problem_criterion_value(self.handle, z.ptr, u.ptr, dev.ptr, _sh(stream)))
+        return dev if out is not None else float(dev.numpy()[0])
+
+    def criterion_gradient(self, z, u, out=None, stream=None):
+        z, u = _dev(z), _dev(u)
+        out = out if out is not None else DeviceArray(self.vf.mesh.num_nodes)
+        _check(lib.pa_problem_criterion_gradient(self.handle, z.ptr, u.ptr, out.ptr, _sh(stream)))
+        return out
+
+    def __del__(self):
+        try:
+            lib.pa_problem_destroy(self.handle)
+        except Exception:
+            pass

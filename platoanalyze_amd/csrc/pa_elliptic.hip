@@ -42,6 +42,13 @@ struct RowMesh {
     const int* __restrict__ rowmap;
     const int* __restrict__ n2c_off;
     const int2* __restrict__ pairs;
+    // row patterns
+    const int* __restrict__ pcell;
+    const int* __restrict__ row_pat;
+    const int4* __restrict__ pat_info;
+    const unsigned char* __restrict__ pat_a;
+    const unsigned char* __restrict__ pat_ent;
+    const unsigned* __restrict__ pat_item;
 };
 
 // Blocks are dispatched round-robin over the 8 XCDs; renumber them so each
@@ -75,21 +82,24 @@ __device__ inline double dpenal(double rho, double p, double emin) {
 }
 
 // Tet4 shape-function gradients and |det J| (ComputeGradientWorkset<3>).
+// One correctly rounded reciprocal of det J replaces the reference's nine
+// divisions (each gradient entry differs from cofactor/det by at most 1 ulp).
 __device__ inline double tet_gradients(const double4 x[4], double g[4][3]) {
     const double j11 = x[0].x - x[3].x, j12 = x[1].x - x[3].x, j13 = x[2].x - x[3].x;
     const double j21 = x[0].y - x[3].y, j22 = x[1].y - x[3].y, j23 = x[2].y - x[3].y;
     const double j31 = x[0].z - x[3].z, j32 = x[1].z - x[3].z, j33 = x[2].z - x[3].z;
     const double det = j11 * j22 * j33 + j12 * j23 * j31 + j13 * j21 * j32 - j11 * j23 * j32 - j12 * j21 * j33 -
                        j13 * j22 * j31;
-    g[0][0] = (j22 * j33 - j23 * j32) / det;
-    g[0][1] = (j13 * j32 - j12 * j33) / det;
-    g[0][2] = (j12 * j23 - j13 * j22) / det;
-    g[1][0] = (j23 * j31 - j21 * j33) / det;
-    g[1][1] = (j11 * j33 - j13 * j31) / det;
-    g[1][2] = (j13 * j21 - j11 * j23) / det;
-    g[2][0] = (j21 * j32 - j22 * j31) / det;
-    g[2][1] = (j12 * j31 - j11 * j32) / det;
-    g[2][2] = (j11 * j22 - j12 * j21) / det;
+    const double id = 1.0 / det;
+    g[0][0] = (j22 * j33 - j23 * j32) * id;
+    g[0][1] = (j13 * j32 - j12 * j33) * id;
+    g[0][2] = (j12 * j23 - j13 * j22) * id;
+    g[1][0] = (j23 * j31 - j21 * j33) * id;
+    g[1][1] = (j11 * j33 - j13 * j31) * id;
+    g[1][2] = (j13 * j21 - j11 * j23) * id;
+    g[2][0] = (j21 * j32 - j22 * j31) * id;
+    g[2][1] = (j12 * j31 - j11 * j32) * id;
+    g[2][2] = (j11 * j22 - j12 * j21) * id;
     g[3][0] = -(g[0][0] + g[1][0] + g[2][0]);
     g[3][1] = -(g[0][1] + g[1][1] + g[2][1]);
     g[3][2] = -(g[0][2] + g[1][2] + g[2][2]);
@@ -355,6 +365,180 @@ __global__ __launch_bounds__(kBlock) void k_jacobian_rows(RowMesh m, Material mt
 }
 
 // ---------------------------------------------------------------------------
+// Jacobian rows through row patterns, without atomics (32 lanes per row).
+//  phase 1: lane p < npairs evaluates incident cell p: gradients g, weight
+//           w = s(rho) |det J| / 6; parks (w g_a, g_0..g_3) in LDS; with RES
+//           also the row node's residual (group sum).
+//  phase 2: lane t < nitems sums M = sum (w g_a) (x) g_b over its contribution
+//           entries; an off-diagonal slot is then final and stored as
+//           K = lambda M + mu M^T + mu tr(M) I (thermal: k tr(M)); diagonal
+//           chunks park M in LDS.
+//  phase 3: lanes q < block size add the diagonal chunks and store entry q.
+// ---------------------------------------------------------------------------
+template <int DPN>
+__device__ inline double block_entry(const Material& mt, const double* M, int q) {
+    if constexpr (DPN == 3) {
+        const int i = q / 3, n = q % 3;
+        double v = mt.lambda * M[i * 3 + n] + mt.mu * M[n * 3 + i];
+        if (i == n) v += mt.mu * ((M[0] + M[4]) + M[8]);
+        return v;
+    } else {
+        return mt.k * M[0];
+    }
+}
+
+template <int DPN, class VT, bool RES, int PMAX>
+__global__ __launch_bounds__(kBlock) void k_jacobian_pattern(RowMesh m, Material mt, const double* __restrict__ u,
+                                                             const double* __restrict__ z, double body,
+                                                             VT* __restrict__ vals, double* __restrict__ R) {
+    constexpr int G = 32, ROWS = kBlock / G;
+    constexpr int MS = DPN == 3 ? 9 : 1;  // accumulator entries (block size)
+    constexpr int PD = 12;                // per pair: w g_a (0..2), g_0..g_2 (3..11); g_3 = -(g_0+g_1+g_2)
+    static_assert(PMAX * PD >= 32 * MS, "the pair buffer also stages the output row");
+    __shared__ double pd[ROWS][PMAX * PD];
+    __shared__ double dg[ROWS][4][MS];          // diagonal chunk partial sums
+    __shared__ unsigned ent_s[ROWS][PMAX];      // the row's contribution entries, 4 per word
+    const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+    const int row = xcd_block() * ROWS + grp;
+    const bool active = row < m.N;
+    int4 i0 = make_int4(0, 0, 0, 0), i1 = make_int4(0, 0, 0, 0);
+    int rs = 0, rl = 0, p0 = 0;
+    if (active) {
+        const int pid = m.row_pat[row];
+        rs = m.rowmap[row];
+        p0 = m.n2c_off[row];
+        i0 = m.pat_info[2 * pid];
+        i1 = m.pat_info[2 * pid + 1];
+        rl = i0.y;
+    }
+    const int npairs = i0.x, nitems = i0.z, diag = i0.w;
+    // the work item of this lane and the row's entries (4 * npairs bytes, word aligned)
+    unsigned item = 0;
+    if (lane < nitems) item = m.pat_item[i1.z + lane];
+    if (lane < npairs) ent_s[grp][lane] = reinterpret_cast<const unsigned*>(m.pat_ent + i1.y)[lane];
+    double r[DPN];
+#pragma unroll
+    for (int i = 0; i < DPN; ++i) r[i] = 0.0;
+    if (lane < npairs) {
+        CellEval c;
+        const int cell = m.pcell[p0 + lane];
+        c.a = m.pat_a[i1.x + lane];
+        const int4 nd = m.conn[cell];
+        c.nodes[0] = nd.x;
+        c.nodes[1] = nd.y;
+        c.nodes[2] = nd.z;
+        c.nodes[3] = nd.w;
+        double4 x[4];
+#pragma unroll
+        for (int n = 0; n < 4; ++n) x[n] = m.x4[c.nodes[n]];
+        c.absdet = tet_gradients(x, c.g);
+        c.rho = (((z[c.nodes[0]] + z[c.nodes[1]]) + z[c.nodes[2]]) + z[c.nodes[3]]) / 4.0;
+        const double s = penal(c.rho, mt.p, mt.emin);
+        const double w = s * (c.absdet / 6.0);
+        double ga[3];
+        pick_row(c.g, c.a, ga);
+        double* P = &pd[grp][lane * PD];
+        P[0] = w * ga[0];
+        P[1] = w * ga[1];
+        P[2] = w * ga[2];
+#pragma unroll
+        for (int n = 0; n < 3; ++n)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) P[3 + 3 * n + d] = c.g[n][d];
+        if constexpr (RES) {
+            if constexpr (DPN == 3) {
+                double eps[6], sig[6], out[3];
+                cell_strain(c, u, eps);
+                iso_stress(mt, eps, sig);
+                btsig(sig, ga, out);
+                r[0] = w * out[0];
+                r[1] = w * out[1];
+                r[2] = w * out[2];
+            } else {
+                double gT[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+                for (int n = 0; n < 4; ++n) {
+                    const double T = u[c.nodes[n]];
+                    gT[0] += T * c.g[n][0];
+                    gT[1] += T * c.g[n][1];
+                    gT[2] += T * c.g[n][2];
+                }
+                r[0] = w * mt.k * (gT[0] * ga[0] + gT[1] * ga[1] + gT[2] * ga[2]);
+                r[0] += -1.0 * (c.absdet / 6.0) * body * 0.25 * c.rho;
+            }
+        }
+    }
+    if constexpr (RES) {
+#pragma unroll
+        for (int i = 0; i < DPN; ++i) r[i] = group_sum<G>(r[i]);
+        if (active && lane == 0) {
+#pragma unroll
+            for (int i = 0; i < DPN; ++i) R[(long)row * DPN + i] = r[i];
+        }
+    }
+    __syncthreads();
+    const int slot = item & 0xff;
+    double M[MS];
+#pragma unroll
+    for (int q = 0; q < MS; ++q) M[q] = 0.0;
+    if (lane < nitems) {
+        const int start = (item >> 8) & 0xfff, count = item >> 20;
+        const unsigned char* ent = reinterpret_cast<const unsigned char*>(ent_s[grp]) + start;
+        for (int e = 0; e < count; ++e) {
+            const int pe = ent[e];
+            const double* P = &pd[grp][(pe >> 2) * PD];
+            const int b = pe & 3;
+            const double a0 = P[0], a1 = P[1], a2 = P[2];
+            double b0, b1, b2;
+            if (b < 3) {
+                b0 = P[3 + 3 * b];
+                b1 = P[4 + 3 * b];
+                b2 = P[5 + 3 * b];
+            } else {
+                b0 = -(P[3] + P[6] + P[9]);
+                b1 = -(P[4] + P[7] + P[10]);
+                b2 = -(P[5] + P[8] + P[11]);
+            }
+            if constexpr (DPN == 3) {
+                M[0] += a0 * b0; M[1] += a0 * b1; M[2] += a0 * b2;
+                M[3] += a1 * b0; M[4] += a1 * b1; M[5] += a1 * b2;
+                M[6] += a2 * b0; M[7] += a2 * b1; M[8] += a2 * b2;
+            } else {
+                M[0] += (a0 * b0 + a1 * b1) + a2 * b2;
+            }
+        }
+    }
+    __syncthreads();  // the pair buffer now stages the output row
+    double* kout = pd[grp];
+    if (lane < nitems) {
+        if (slot != diag) {
+#pragma unroll
+            for (int q = 0; q < MS; ++q) kout[slot * MS + q] = block_entry<DPN>(mt, M, q);
+        } else {
+#pragma unroll
+            for (int q = 0; q < MS; ++q) dg[grp][lane - i1.w][q] = M[q];
+        }
+    }
+    __syncthreads();
+    if (active && lane < MS) {
+        const int nd = npairs > 0 ? (npairs + 7) / 8 : 1;
+        double Md[MS];
+#pragma unroll
+        for (int q = 0; q < MS; ++q) {
+            double s = 0.0;
+            for (int k = 0; k < nd; ++k) s += dg[grp][k][q];
+            Md[q] = s;
+        }
+        kout[diag * MS + lane] = block_entry<DPN>(mt, Md, lane);
+    }
+    __syncthreads();
+    if (active) {
+        VT* dst = vals + (long)rs * MS;
+        for (int t = lane; t < rl * MS; t += G) dst[t] = (VT)kout[t];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // gradient_z rows: block (row = control node a, col = state node b) holds
 // dR_b/dz_a (1 x DPN) = sum over cells with a, b of (1/4) s'(rho) (V) B_b^T C eps
 // (+ thermal body load derivative).
@@ -508,6 +692,65 @@ __global__ __launch_bounds__(kBlock) void k_energy_gradz_rows(RowMesh m, Materia
 }
 
 // ---------------------------------------------------------------------------
+// Criterion gradient with an adjoint, fused: for row (control node) a,
+//   dF/dz_a = sum over incident cells of
+//     (1/4) s_F'(rho) V * f_density(u)            (criterion, its own SIMP)
+//   + (1/4) s_R'(rho) V * <lambda, B^T C eps(u)>   (PDE, lambda = lam_scale u)
+//   + thermal body load: -(1/6)|det J| f (1/4)(1/4) sum_b lambda_b.
+// The reference forms the same sum as MatrixTimesVectorPlusVector(gradient_z,
+// lambda, dfdz) (src/elliptic/Problem.hpp:421-448).
+// ---------------------------------------------------------------------------
+template <int DPN, int G>
+__global__ __launch_bounds__(kBlock) void k_adjoint_gradz_rows(RowMesh m, Material mc, Material mp,
+                                                               const double* __restrict__ u, double lam_scale,
+                                                               const double* __restrict__ z, double body,
+                                                               double* __restrict__ out) {
+    constexpr int ROWS = kBlock / G;
+    const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+    const int row = xcd_block() * ROWS + grp;
+    if (row >= m.N) return;
+    const int p0 = m.n2c_off[row], p1 = m.n2c_off[row + 1];
+    double acc = 0.0;
+    for (int p = p0 + lane; p < p1; p += G) {
+        CellEval c;
+        load_cell(m, z, p, c);
+        const double V = c.absdet / 6.0;
+        const double dsF = dpenal(c.rho, mc.p, mc.emin) * 0.25;
+        const double dsR = dpenal(c.rho, mp.p, mp.emin) * 0.25;
+        if constexpr (DPN == 3) {
+            double eps[6], sigF[6], sigR[6];
+            cell_strain(c, u, eps);
+            iso_stress(mc, eps, sigF);
+            iso_stress(mp, eps, sigR);
+            double fF = 0.0, fR = 0.0;
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                fF += sigF[i] * eps[i];
+                fR += sigR[i] * eps[i];
+            }
+            // eps(lambda) = lam_scale * eps(u)
+            acc += dsF * V * (0.5 * fF) + dsR * V * (lam_scale * fR);
+        } else {
+            double gT[3] = {0.0, 0.0, 0.0};
+            double us = 0.0;
+#pragma unroll
+            for (int n = 0; n < 4; ++n) {
+                const double T = u[c.nodes[n]];
+                us += T;
+                gT[0] += T * c.g[n][0];
+                gT[1] += T * c.g[n][1];
+                gT[2] += T * c.g[n][2];
+            }
+            const double gg = gT[0] * gT[0] + gT[1] * gT[1] + gT[2] * gT[2];
+            acc += dsF * V * (mc.k * gg) + dsR * V * (mp.k * lam_scale * gg);
+            acc += -1.0 * (c.absdet / 6.0) * body * 0.25 * 0.25 * (lam_scale * us);
+        }
+    }
+    acc = group_sum<G>(acc);
+    if (lane == 0) out[row] = acc;
+}
+
+// ---------------------------------------------------------------------------
 // Uniform natural BC on a side set, in boundary-node order:
 // R[node] += sum over faces (basis 1/3) * value * scale * (1/2) * |cross|.
 // ---------------------------------------------------------------------------
@@ -644,6 +887,12 @@ RowMesh row_mesh(const pa_mesh* m) {
     r.rowmap = m->d_rowmap.p;
     r.n2c_off = m->d_n2c_off.p;
     r.pairs = m->d_pairs.p;
+    r.pcell = m->d_pcell.p;
+    r.row_pat = m->d_row_pat.p;
+    r.pat_info = m->d_pat_info.p;
+    r.pat_a = m->d_pat_a.p;
+    r.pat_ent = m->d_pat_ent.p;
+    r.pat_item = m->d_pat_item.p;
     return r;
 }
 
@@ -699,11 +948,50 @@ void dispatch_jac(const pa_vector_function* vf, const double* u, const double* z
     else throw Error(PA_ERR_LIMIT, "gradient_u: node degree above 63 is not supported");
 }
 
+// Residual-type row sums: one incident cell per lane when the node degree
+// allows (32 lanes per row), else 8 lanes per row looping over the cells.
+void residual_rows(const pa_mesh* m, int dpn, const Material& mt, const double* u, const double* z, double body,
+                   double scale, double* R, hipStream_t st) {
+    if (m->max_cells_per_node <= 32) {
+        if (dpn == 3)
+            k_residual_rows<3, 32><<<row_blocks(m->N, 32), kBlock, 0, st>>>(row_mesh(m), mt, u, z, body, scale, R);
+        else
+            k_residual_rows<1, 32><<<row_blocks(m->N, 32), kBlock, 0, st>>>(row_mesh(m), mt, u, z, body, scale, R);
+    } else {
+        if (dpn == 3)
+            k_residual_rows<3, kG><<<row_blocks(m->N, kG), kBlock, 0, st>>>(row_mesh(m), mt, u, z, body, scale, R);
+        else
+            k_residual_rows<1, kG><<<row_blocks(m->N, kG), kBlock, 0, st>>>(row_mesh(m), mt, u, z, body, scale, R);
+    }
+    check_launch();
+}
+
 template <class VT, bool RES, bool TR>
 void jacobian(const pa_vector_function* vf, const double* u, const double* z, VT* vals, double* R, hipStream_t st) {
-    PA_REQUIRE(vf->mesh->max_row_len <= 64, PA_ERR_LIMIT, "gradient_u: node degree above 63 is not supported");
-    if (vf->dpn == 3) dispatch_jac<3, VT, RES, TR>(vf, u, z, vals, R, st);
-    else dispatch_jac<1, VT, RES, TR>(vf, u, z, vals, R, st);
+    const pa_mesh* m = vf->mesh;
+    if (m->has_patterns) {
+        // The operators are self-adjoint: the transposed assembly equals the
+        // direct one (block (i,j) = K_ji^T = K_ij) up to rounding.
+        const Material mt = material_of(vf->physics, vf->material, vf->penalty);
+        const double body = vf->has_body ? vf->body_value : 0.0;
+        const int nb = row_blocks(m->N, 32);
+        if (m->max_cells_per_node <= 24) {
+            if (vf->dpn == 3)
+                k_jacobian_pattern<3, VT, RES, 24><<<nb, kBlock, 0, st>>>(row_mesh(m), mt, u, z, 0.0, vals, R);
+            else
+                k_jacobian_pattern<1, VT, RES, 24><<<nb, kBlock, 0, st>>>(row_mesh(m), mt, u, z, body, vals, R);
+        } else {
+            if (vf->dpn == 3)
+                k_jacobian_pattern<3, VT, RES, 32><<<nb, kBlock, 0, st>>>(row_mesh(m), mt, u, z, 0.0, vals, R);
+            else
+                k_jacobian_pattern<1, VT, RES, 32><<<nb, kBlock, 0, st>>>(row_mesh(m), mt, u, z, body, vals, R);
+        }
+        check_launch();
+    } else {
+        PA_REQUIRE(m->max_row_len <= 64, PA_ERR_LIMIT, "gradient_u: node degree above 63 is not supported");
+        if (vf->dpn == 3) dispatch_jac<3, VT, RES, TR>(vf, u, z, vals, R, st);
+        else dispatch_jac<1, VT, RES, TR>(vf, u, z, vals, R, st);
+    }
     if (RES) natural_bcs(vf, R, st);
 }
 
@@ -711,11 +999,7 @@ void residual(const pa_vector_function* vf, const double* u, const double* z, do
     const pa_mesh* m = vf->mesh;
     const Material mt = material_of(vf->physics, vf->material, vf->penalty);
     const double body = vf->has_body ? vf->body_value : 0.0;
-    if (vf->dpn == 3)
-        k_residual_rows<3, kG><<<row_blocks(m->N, kG), kBlock, 0, st>>>(row_mesh(m), mt, u, z, 0.0, 1.0, R);
-    else
-        k_residual_rows<1, kG><<<row_blocks(m->N, kG), kBlock, 0, st>>>(row_mesh(m), mt, u, z, body, 1.0, R);
-    check_launch();
+    residual_rows(m, vf->dpn, mt, u, z, vf->dpn == 3 ? 0.0 : body, 1.0, R, st);
     natural_bcs(vf, R, st);
 }
 
@@ -725,6 +1009,31 @@ void check_vf(const pa_vector_function* vf, const void* a, const void* b, const 
 }
 
 }  // namespace
+
+void self_adjoint_gradient(const pa_vector_function* vf, const pa_scalar_function* sf, const double* u,
+                           const double* z, double* out, hipStream_t st) {
+    const pa_mesh* m = vf->mesh;
+    const Material mp = material_of(vf->physics, vf->material, vf->penalty);
+    const Material mc = material_of(vf->physics, sf->material, sf->penalty);
+    const double body = vf->has_body ? vf->body_value : 0.0;
+    if (m->max_cells_per_node <= 32) {
+        if (vf->dpn == 3)
+            k_adjoint_gradz_rows<3, 32><<<row_blocks(m->N, 32), kBlock, 0, st>>>(row_mesh(m), mc, mp, u, -1.0, z,
+                                                                                  0.0, out);
+        else
+            k_adjoint_gradz_rows<1, 32><<<row_blocks(m->N, 32), kBlock, 0, st>>>(row_mesh(m), mc, mp, u, -1.0, z,
+                                                                                  body, out);
+    } else {
+        if (vf->dpn == 3)
+            k_adjoint_gradz_rows<3, kG><<<row_blocks(m->N, kG), kBlock, 0, st>>>(row_mesh(m), mc, mp, u, -1.0, z,
+                                                                                  0.0, out);
+        else
+            k_adjoint_gradz_rows<1, kG><<<row_blocks(m->N, kG), kBlock, 0, st>>>(row_mesh(m), mc, mp, u, -1.0, z,
+                                                                                  body, out);
+    }
+    check_launch();
+}
+
 }  // namespace pa
 
 using pa::guarded;
@@ -925,13 +1234,8 @@ int pa_scalar_function_gradient_u(const pa_scalar_function* sf, const double* u,
         const int phys = sf->dpn == 3 ? PA_MECHANICS : PA_THERMAL;
         const pa::Material mt = pa::material_of(phys, sf->material, sf->penalty);
         const hipStream_t st = (hipStream_t)stream;
-        const int nb = pa::row_blocks(m->N, pa::kG);
         // d/du of 1/2 u.K u is K u; of the thermal energy u.K u it is 2 K u.
-        if (sf->dpn == 3)
-            pa::k_residual_rows<3, pa::kG><<<nb, pa::kBlock, 0, st>>>(pa::row_mesh(m), mt, u, z, 0.0, 1.0, g);
-        else
-            pa::k_residual_rows<1, pa::kG><<<nb, pa::kBlock, 0, st>>>(pa::row_mesh(m), mt, u, z, 0.0, 2.0, g);
-        pa::check_launch();
+        pa::residual_rows(m, sf->dpn, mt, u, z, 0.0, sf->dpn == 3 ? 1.0 : 2.0, g, st);
     });
 }
 

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstring>
 #include <thread>
+#include <unordered_map>
 
 #include "pa_internal.h"
 
@@ -29,6 +30,90 @@ static void parallel_for(long n, F&& f) {
         th.emplace_back([&f, b, e] { f(b, e); });
     }
     for (auto& t : th) t.join();
+}
+
+// Row patterns.  A row's schedule is fixed by its sequence of incident cells'
+// (local index of the row node, column slots of the four nodes); rows with the
+// same sequence (all interior rows of a structured mesh, for instance) share
+// one table, so the per-row index data the kernels read is one pattern id
+// plus the incident cell ids.  For each pattern:
+//   a[p]           local index of the row node in pair p's cell;
+//   entries        (p << 2 | b) grouped by column slot, pairs ascending;
+//   items          one per off-diagonal slot, and the diagonal slot cut into
+//                  chunks of kDiagChunk entries (slot | start << 8 | count << 20).
+// Limits of the pattern kernels: <= 32 pairs, <= 32 slots, <= 32 items,
+// <= 4 diagonal chunks per row; a mesh outside them keeps the generic kernels.
+static void build_row_patterns(pa_mesh* m, const std::vector<int>& off, const std::vector<int2>& pairs) {
+    constexpr int kMaxPairs = 32, kMaxSlots = 32, kMaxItems = 32, kDiagChunk = 8;
+    const int N = m->N;
+    m->has_patterns = false;
+    if (m->max_cells_per_node > kMaxPairs || m->max_row_len > kMaxSlots) return;
+    std::unordered_map<std::string, int> index;
+    std::vector<int> row_pat(N);
+    std::vector<std::string> sigs;
+    std::string sig;
+    for (int i = 0; i < N; ++i) {
+        const int np = off[i + 1] - off[i];
+        const int rl = m->h_rowmap[i + 1] - m->h_rowmap[i];
+        sig.assign(2 + 5 * (size_t)np, '\0');
+        sig[0] = (char)np;
+        sig[1] = (char)rl;
+        for (int p = 0; p < np; ++p) {
+            const int2 pr = pairs[off[i] + p];
+            sig[2 + 5 * p] = (char)(pr.x & 3);
+            for (int b = 0; b < 4; ++b) sig[3 + 5 * p + b] = (char)(((unsigned)pr.y >> (8 * b)) & 0xff);
+        }
+        auto it = index.find(sig);
+        if (it == index.end()) {
+            it = index.emplace(sig, (int)sigs.size()).first;
+            sigs.push_back(sig);
+        }
+        row_pat[i] = it->second;
+    }
+    std::vector<int4> info;
+    std::vector<unsigned char> pa_a, pa_ent;
+    std::vector<unsigned> pa_item;
+    for (const std::string& s : sigs) {
+        const int np = (unsigned char)s[0], rl = (unsigned char)s[1];
+        auto A = [&](int p) { return (int)(unsigned char)s[2 + 5 * p]; };
+        auto S = [&](int p, int b) { return (int)(unsigned char)s[3 + 5 * p + b]; };
+        const int diag = np ? S(0, A(0)) : 0;
+        while (pa_ent.size() % 4) pa_ent.push_back(0);  // kernels read the entries as 32-bit words
+        const int a_off = (int)pa_a.size(), ent_off = (int)pa_ent.size(), item_off = (int)pa_item.size();
+        for (int p = 0; p < np; ++p) pa_a.push_back((unsigned char)A(p));
+        int first_diag = -1;
+        for (int slot = 0; slot < rl; ++slot) {
+            const int start = (int)pa_ent.size() - ent_off;
+            for (int p = 0; p < np; ++p)
+                for (int b = 0; b < 4; ++b)
+                    if (S(p, b) == slot) pa_ent.push_back((unsigned char)(p << 2 | b));
+            const int count = (int)pa_ent.size() - ent_off - start;
+            if (slot == diag) {
+                first_diag = (int)pa_item.size() - item_off;
+                for (int c = 0; c < count; c += kDiagChunk)
+                    pa_item.push_back((unsigned)slot | (unsigned)(start + c) << 8 |
+                                      (unsigned)std::min(kDiagChunk, count - c) << 20);
+                if (count == 0) pa_item.push_back((unsigned)slot | (unsigned)start << 8);
+            } else {
+                pa_item.push_back((unsigned)slot | (unsigned)start << 8 | (unsigned)count << 20);
+            }
+        }
+        const int nitems = (int)pa_item.size() - item_off;
+        const int ndiag = (np + kDiagChunk - 1) / kDiagChunk;
+        if (nitems > kMaxItems || ndiag > 4) return;
+        info.push_back(make_int4(np, rl, nitems, diag));
+        info.push_back(make_int4(a_off, ent_off, item_off, first_diag));
+    }
+    m->num_patterns = (int)sigs.size();
+    std::vector<int> pcell(pairs.size());
+    for (size_t p = 0; p < pairs.size(); ++p) pcell[p] = pairs[p].x >> 2;
+    m->d_pcell.upload(pcell.data(), pcell.size());
+    m->d_row_pat.upload(row_pat.data(), row_pat.size());
+    m->d_pat_info.upload(info.data(), info.size());
+    m->d_pat_a.upload(pa_a.data(), pa_a.size());
+    m->d_pat_ent.upload(pa_ent.data(), pa_ent.size());
+    m->d_pat_item.upload(pa_item.data(), pa_item.size());
+    m->has_patterns = true;
 }
 
 }  // namespace pa
@@ -209,6 +294,7 @@ int pa_mesh_create(const double* coords, int N, const int* conn, int C, pa_mesh*
             m->d_colidx.upload(m->h_colidx.data(), m->h_colidx.size());
             m->d_n2c_off.upload(off.data(), off.size());
             m->d_pairs.upload(pairs.data(), pairs.size());
+            pa::build_row_patterns(m, off, pairs);
         } catch (...) {
             delete m;
             throw;

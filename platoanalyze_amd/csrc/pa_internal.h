@@ -103,6 +103,16 @@ struct pa_mesh {
     pa::DevBuf<int> d_colidx;      // nnzb
     pa::DevBuf<int> d_n2c_off;     // N+1
     pa::DevBuf<int2> d_pairs;      // 4C: x = cell<<2 | local index of the row node, y = 4 packed uint8 column slots
+    // Row patterns: rows whose (incident-cell local index, column slots)
+    // sequences coincide share one gather/contraction schedule.
+    bool has_patterns = false;      // every pattern fits the pattern kernels' limits
+    int num_patterns = 0;
+    pa::DevBuf<int> d_pcell;        // 4C: cell id of each (row, incident cell) pair
+    pa::DevBuf<int> d_row_pat;      // N: pattern id of each row
+    pa::DevBuf<int4> d_pat_info;    // 2 per pattern: {npairs, rowlen, nitems, diag_slot}, {a_off, ent_off, item_off, first_diag_item}
+    pa::DevBuf<unsigned char> d_pat_a;    // per pattern pair: local index of the row node in the cell
+    pa::DevBuf<unsigned char> d_pat_ent;  // contributions (pair << 2 | local node b), grouped by column slot
+    pa::DevBuf<unsigned> d_pat_item;      // work items: slot | start << 8 | count << 20
     std::map<std::string, pa::SideSet> side_sets;
     std::map<std::string, std::vector<int>> node_sets;
 };
@@ -123,6 +133,13 @@ struct pa_vector_function {
     int body_dof = 0;
     double body_value = 0.0;
 };
+
+namespace pa {
+// Fused self-adjoint criterion gradient (lambda = -u), defined with the
+// assembly kernels; used by the problem driver.
+void self_adjoint_gradient(const pa_vector_function* vf, const pa_scalar_function* sf, const double* u,
+                           const double* z, double* out, hipStream_t st);
+}  // namespace pa
 
 struct pa_scalar_function {
     pa_mesh* mesh = nullptr;

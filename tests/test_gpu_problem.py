@@ -1,0 +1,152 @@
+"""GPU parity of the caller side: PCG, Problem.solution (Newton step with
+symmetric Dirichlet elimination), criterion value and the self-adjoint
+criterion gradient, against the oracle + scipy direct solves and central
+finite differences.
+
+Tolerances: solution vs direct solve 1e-7 relative at tol 1e-12; mixed
+FP32/FP64-refined solve must reach the 1e-8 FP64 residual and agree with the
+FP64 path to 1e-6; fused gradient vs the reference's composite formula
+(dfdz + gradient_z^T lambda, lambda = -u) 1e-12; total gradient vs central
+differences 1e-5.
+"""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a, b = np.asarray(a, float).ravel(), np.asarray(b, float).ravel()
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def pa():
+    import platoanalyze_amd
+    assert platoanalyze_amd.device_count() >= 1
+    return platoanalyze_amd
+
+
+def build(pa, dims, L, physics, seed, lo=0.3, p=3.0, emin=1e-9):
+    coords, conn = pa.box_mesh(*dims, *L)
+    mesh = pa.Mesh.box(*dims, *L)
+    N = mesh.num_nodes
+    z = np.random.default_rng(seed).uniform(lo, 1.0, N)
+    fc, fl = oracle.side_set(coords, conn, 0, L[0])
+    if physics == "elastic":
+        vf = pa.VectorFunction(mesh, "Mechanics", youngs=1.0, poisson=0.3, penalty_exponent=p, penalty_min=emin)
+        vf.add_uniform_natural_bc("x+", (0.0, 0.0, -1.0))
+        sf = pa.PhysicsScalarFunction(mesh, "Internal Elastic Energy", youngs=1.0, poisson=0.3, penalty_exponent=p,
+                                      penalty_min=emin)
+        om = oracle.Model(coords, conn, youngs=1.0, poisson=0.3, penalty_exponent=p, penalty_min=emin,
+                          face_cells=fc, face_lnodes=fl, flux=(0.0, 0.0, -1.0))
+        dofs = (3 * mesh.node_set("x-")[:, None] + np.arange(3)).ravel().astype(np.int32)
+    else:
+        vf = pa.VectorFunction(mesh, "Thermal", conductivity=1.0, penalty_exponent=p, penalty_min=emin)
+        vf.add_body_load(0, 1.0)
+        sf = pa.PhysicsScalarFunction(mesh, "Internal Thermal Energy", conductivity=1.0, penalty_exponent=p,
+                                      penalty_min=emin)
+        om = oracle.Model(coords, conn, conductivity=1.0, penalty_exponent=p, penalty_min=emin, body=(0, 1.0))
+        dofs = mesh.node_set("z-").astype(np.int32)
+    return mesh, vf, sf, om, z, dofs
+
+
+def direct_solution(om, physics, z, dofs):
+    dpn = 3 if physics == "elastic" else 1
+    n = om.N * dpn
+    vals = om.gradient_u(physics, np.zeros(n), z)
+    rhs = -om.value(physics, np.zeros(n), z)
+    oracle.apply_block_constraints(dpn, om.rowmap, om.colidx, vals, rhs, dofs, np.zeros(dofs.size))
+    K = oracle.to_scipy(dpn, dpn, om.rowmap, om.colidx, vals).tocsc()
+    return spla.spsolve(K, rhs)
+
+
+@pytest.mark.parametrize("physics", ["elastic", "thermal"])
+def test_solution_matches_direct_solve(pa, physics):
+    print(f"case: problem solution, 12x6x6 {physics}", flush=True)
+    mesh, vf, sf, om, z, dofs = build(pa, (12, 6, 6), (2.0, 1.0, 1.0), physics, 4)
+    P = pa.Problem(vf, sf, dofs, mode="fp64", tol=1e-10, max_iter=20000)
+    u = P.solution(z).numpy()
+    st = P.last_stats
+    print(st, flush=True)
+    assert st["converged"] and st["rel_residual"] <= 1e-10, st
+    ud = direct_solution(om, physics, z, dofs)
+    assert rel(u, ud) < 1e-6
+    F = P.criterion_value(z, u)
+    Fref = om.energy_value(physics, ud, z)
+    assert abs(F - Fref) < 1e-8 * abs(Fref)
+
+
+@pytest.mark.parametrize("physics", ["elastic", "thermal"])
+def test_fused_gradient_equals_reference_composite(pa, physics):
+    print(f"case: self-adjoint gradient, 10x5x5 {physics}", flush=True)
+    mesh, vf, sf, om, z, dofs = build(pa, (10, 5, 5), (2.0, 1.0, 1.0), physics, 6)
+    P = pa.Problem(vf, sf, dofs, mode="fp64", tol=1e-10)
+    u = P.solution(z).numpy()
+    g = P.criterion_gradient(z, u).numpy()
+    # reference formula: dfdz + gradient_z^T lambda with lambda = -u (src/elliptic/Problem.hpp:421-448)
+    dpn = 3 if physics == "elastic" else 1
+    comp = om.energy_gradient_z(physics, u, z) + oracle.block_spmv(1, dpn, om.rowmap, om.colidx,
+                                                                   om.gradient_z(physics, u, z), -u)
+    assert rel(g, comp) < 1e-12
+    # and through the GPU's own gradient_z matrix + block SpMV
+    Gz = vf.gradient_z(u, z)
+    y = pa.DeviceArray(mesh.num_nodes)
+    Gz.spmv(pa.DeviceArray.from_numpy(-u), y)
+    comp2 = sf.gradient_z(u, z).numpy() + y.numpy()
+    assert rel(g, comp2) < 1e-12
+
+
+def test_total_compliance_gradient_central_differences(pa):
+    print("case: compliance gradient vs central differences, 6x3x3", flush=True)
+    mesh, vf, sf, om, z, dofs = build(pa, (6, 3, 3), (2.0, 1.0, 1.0), "elastic", 8)
+    P = pa.Problem(vf, sf, dofs, mode="fp64", tol=1e-11, max_iter=50000)
+
+    def F(zz):
+        u = P.solution(zz)
+        return P.criterion_value(zz, u)
+
+    u = P.solution(z)
+    g = P.criterion_gradient(z, u).numpy()
+    h = 1e-5
+    for i in [0, 17, 40, mesh.num_nodes - 1]:
+        zp, zm = z.copy(), z.copy()
+        zp[i] += h
+        zm[i] -= h
+        d = (F(zp) - F(zm)) / (2 * h)
+        assert abs(d - g[i]) <= 1e-5 * np.max(np.abs(g)), (i, d, g[i])
+
+
+def test_mixed_precision_reaches_fp64_residual(pa):
+    print("case: mixed FP32 assembly + FP64 refinement, 20x10x10 elastic", flush=True)
+    mesh, vf, sf, om, z, dofs = build(pa, (20, 10, 10), (2.0, 1.0, 1.0), "elastic", 1)
+    P = pa.Problem(vf, sf, dofs, mode="fp64", tol=1e-8)
+    u64 = P.solution(z).numpy()
+    st64 = P.last_stats
+    P.set_options(mode="fp32", inner_tol=1e-5, max_sweeps=10)
+    u32 = P.solution(z).numpy()
+    st32 = P.last_stats
+    assert st64["converged"] and st32["converged"], (st64, st32)
+    assert st32["rel_residual"] <= 1e-8
+    assert rel(u32, u64) < 1e-6
+    g64 = P.criterion_gradient(z, pa.DeviceArray.from_numpy(u64)).numpy()
+    g32 = P.criterion_gradient(z, pa.DeviceArray.from_numpy(u32)).numpy()
+    assert rel(g32, g64) < 1e-6
+
+
+def test_pcg_solver_direct(pa):
+    mesh, vf, sf, om, z, dofs = build(pa, (8, 4, 4), (2.0, 1.0, 1.0), "elastic", 9)
+    n = 3 * mesh.num_nodes
+    K = vf.gradient_u(np.zeros(n), z)
+    rhs = pa.DeviceArray.from_numpy(np.ones(n))
+    pa.apply_block_constraints(K, rhs, dofs, np.zeros(dofs.size))
+    x = pa.DeviceArray.from_numpy(np.zeros(n))
+    for prec in (pa.PRECOND_NONE, pa.PRECOND_JACOBI, pa.PRECOND_BLOCK_JACOBI):
+        x.zero()
+        st = pa.Solver(mesh, 3, prec).solve(K, rhs, x, tol=1e-11)
+        assert st["converged"], (prec, st)
+        Ks = oracle.to_scipy(3, 3, *mesh.graph(), K.entries())
+        assert rel(Ks @ x.numpy(), rhs.numpy()) < 1e-9
