@@ -49,6 +49,7 @@ struct RowMesh {
     const unsigned char* __restrict__ pat_a;
     const unsigned char* __restrict__ pat_ent;
     const unsigned* __restrict__ pat_item;
+    const int4* __restrict__ pat_delta;
 };
 
 // Blocks are dispatched round-robin over the 8 XCDs; renumber them so each
@@ -387,9 +388,85 @@ __device__ inline double block_entry(const Material& mt, const double* M, int q)
     }
 }
 
+// Node records (x, y, z, density) packed per call so one 32-byte gather
+// brings everything the geometry and the penalty need.
+__global__ __launch_bounds__(kBlock) void k_pack_nodes(int N, const double4* __restrict__ x4,
+                                                       const double* __restrict__ z, double4* __restrict__ rec) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < N) {
+        double4 v = x4[i];
+        v.w = z[i];
+        rec[i] = v;
+    }
+}
+
+// Residual rows through row patterns: one incident cell per lane (32 lanes per
+// row), nodes from the pattern offsets, geometry and density from the packed
+// node records; also serves criterion gradient_u (scale 2 for the thermal
+// energy, no loads).
+template <int DPN>
+__global__ __launch_bounds__(kBlock) void k_residual_pattern(RowMesh m, const double4* __restrict__ nrec, Material mt,
+                                                             const double* __restrict__ u, double body, double scale,
+                                                             double* __restrict__ R) {
+    constexpr int G = 32, ROWS = kBlock / G;
+    const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+    const int row = xcd_block() * ROWS + grp;
+    if (row >= m.N) return;  // whole group exits together
+    const int pid = m.row_pat[row];
+    const int npairs = m.pat_info[2 * pid].x;
+    const int a_off = m.pat_info[2 * pid + 1].x;
+    double r[DPN];
+#pragma unroll
+    for (int i = 0; i < DPN; ++i) r[i] = 0.0;
+    if (lane < npairs) {
+        CellEval c;
+        const int4 dl = m.pat_delta[a_off + lane];
+        c.a = m.pat_a[a_off + lane];
+        c.nodes[0] = row + dl.x;
+        c.nodes[1] = row + dl.y;
+        c.nodes[2] = row + dl.z;
+        c.nodes[3] = row + dl.w;
+        double4 x[4];
+#pragma unroll
+        for (int n = 0; n < 4; ++n) x[n] = nrec[c.nodes[n]];
+        c.absdet = tet_gradients(x, c.g);
+        c.rho = (((x[0].w + x[1].w) + x[2].w) + x[3].w) / 4.0;
+        const double s = penal(c.rho, mt.p, mt.emin);
+        const double w = s * (c.absdet / 6.0);
+        double ga[3];
+        pick_row(c.g, c.a, ga);
+        if constexpr (DPN == 3) {
+            double eps[6], sig[6], out[3];
+            cell_strain(c, u, eps);
+            iso_stress(mt, eps, sig);
+            btsig(sig, ga, out);
+            r[0] = w * out[0];
+            r[1] = w * out[1];
+            r[2] = w * out[2];
+        } else {
+            double gT[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int n = 0; n < 4; ++n) {
+                const double T = u[c.nodes[n]];
+                gT[0] += T * c.g[n][0];
+                gT[1] += T * c.g[n][1];
+                gT[2] += T * c.g[n][2];
+            }
+            r[0] = w * mt.k * (gT[0] * ga[0] + gT[1] * ga[1] + gT[2] * ga[2]);
+            r[0] += -1.0 * (c.absdet / 6.0) * body * 0.25 * c.rho;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < DPN; ++i) r[i] = group_sum<G>(r[i]);
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < DPN; ++i) R[(long)row * DPN + i] = scale * r[i];
+    }
+}
+
 template <int DPN, class VT, bool RES, int PMAX>
-__global__ __launch_bounds__(kBlock) void k_jacobian_pattern(RowMesh m, Material mt, const double* __restrict__ u,
-                                                             const double* __restrict__ z, double body,
+__global__ __launch_bounds__(kBlock) void k_jacobian_pattern(RowMesh m, const double4* __restrict__ nrec, Material mt,
+                                                             const double* __restrict__ u, double body,
                                                              VT* __restrict__ vals, double* __restrict__ R) {
     constexpr int G = 32, ROWS = kBlock / G;
     constexpr int MS = DPN == 3 ? 9 : 1;  // accumulator entries (block size)
@@ -402,11 +479,10 @@ __global__ __launch_bounds__(kBlock) void k_jacobian_pattern(RowMesh m, Material
     const int row = xcd_block() * ROWS + grp;
     const bool active = row < m.N;
     int4 i0 = make_int4(0, 0, 0, 0), i1 = make_int4(0, 0, 0, 0);
-    int rs = 0, rl = 0, p0 = 0;
+    int rs = 0, rl = 0;
     if (active) {
         const int pid = m.row_pat[row];
         rs = m.rowmap[row];
-        p0 = m.n2c_off[row];
         i0 = m.pat_info[2 * pid];
         i1 = m.pat_info[2 * pid + 1];
         rl = i0.y;
@@ -421,18 +497,18 @@ __global__ __launch_bounds__(kBlock) void k_jacobian_pattern(RowMesh m, Material
     for (int i = 0; i < DPN; ++i) r[i] = 0.0;
     if (lane < npairs) {
         CellEval c;
-        const int cell = m.pcell[p0 + lane];
+        // the cell's nodes are the row node plus the pattern's offsets
+        const int4 dl = m.pat_delta[i1.x + lane];
         c.a = m.pat_a[i1.x + lane];
-        const int4 nd = m.conn[cell];
-        c.nodes[0] = nd.x;
-        c.nodes[1] = nd.y;
-        c.nodes[2] = nd.z;
-        c.nodes[3] = nd.w;
-        double4 x[4];
+        c.nodes[0] = row + dl.x;
+        c.nodes[1] = row + dl.y;
+        c.nodes[2] = row + dl.z;
+        c.nodes[3] = row + dl.w;
+        double4 x[4];  // packed (x, y, z, density) records
 #pragma unroll
-        for (int n = 0; n < 4; ++n) x[n] = m.x4[c.nodes[n]];
+        for (int n = 0; n < 4; ++n) x[n] = nrec[c.nodes[n]];
         c.absdet = tet_gradients(x, c.g);
-        c.rho = (((z[c.nodes[0]] + z[c.nodes[1]]) + z[c.nodes[2]]) + z[c.nodes[3]]) / 4.0;
+        c.rho = (((x[0].w + x[1].w) + x[2].w) + x[3].w) / 4.0;
         const double s = penal(c.rho, mt.p, mt.emin);
         const double w = s * (c.absdet / 6.0);
         double ga[3];
@@ -893,6 +969,7 @@ RowMesh row_mesh(const pa_mesh* m) {
     r.pat_a = m->d_pat_a.p;
     r.pat_ent = m->d_pat_ent.p;
     r.pat_item = m->d_pat_item.p;
+    r.pat_delta = m->d_pat_delta.p;
     return r;
 }
 
@@ -948,11 +1025,26 @@ void dispatch_jac(const pa_vector_function* vf, const double* u, const double* z
     else throw Error(PA_ERR_LIMIT, "gradient_u: node degree above 63 is not supported");
 }
 
-// Residual-type row sums: one incident cell per lane when the node degree
-// allows (32 lanes per row), else 8 lanes per row looping over the cells.
+const double4* pack_nodes(const pa_mesh* m, const double* z, hipStream_t st) {
+    if (m->d_nrec.n != (size_t)m->N) m->d_nrec.alloc(m->N);
+    k_pack_nodes<<<(m->N + kBlock - 1) / kBlock, kBlock, 0, st>>>(
+        m->N, reinterpret_cast<const double4*>(m->d_coords4.p), z, m->d_nrec.p);
+    check_launch();
+    return m->d_nrec.p;
+}
+
+// Residual-type row sums: through row patterns and packed node records when
+// the mesh has them; else one incident cell per lane (32 lanes per row) when
+// the node degree allows, else 8 lanes per row looping over the cells.
 void residual_rows(const pa_mesh* m, int dpn, const Material& mt, const double* u, const double* z, double body,
                    double scale, double* R, hipStream_t st) {
-    if (m->max_cells_per_node <= 32) {
+    if (m->has_patterns) {
+        const double4* rec = pack_nodes(m, z, st);
+        if (dpn == 3)
+            k_residual_pattern<3><<<row_blocks(m->N, 32), kBlock, 0, st>>>(row_mesh(m), rec, mt, u, body, scale, R);
+        else
+            k_residual_pattern<1><<<row_blocks(m->N, 32), kBlock, 0, st>>>(row_mesh(m), rec, mt, u, body, scale, R);
+    } else if (m->max_cells_per_node <= 32) {
         if (dpn == 3)
             k_residual_rows<3, 32><<<row_blocks(m->N, 32), kBlock, 0, st>>>(row_mesh(m), mt, u, z, body, scale, R);
         else
@@ -975,16 +1067,17 @@ void jacobian(const pa_vector_function* vf, const double* u, const double* z, VT
         const Material mt = material_of(vf->physics, vf->material, vf->penalty);
         const double body = vf->has_body ? vf->body_value : 0.0;
         const int nb = row_blocks(m->N, 32);
+        const double4* rec = pack_nodes(m, z, st);
         if (m->max_cells_per_node <= 24) {
             if (vf->dpn == 3)
-                k_jacobian_pattern<3, VT, RES, 24><<<nb, kBlock, 0, st>>>(row_mesh(m), mt, u, z, 0.0, vals, R);
+                k_jacobian_pattern<3, VT, RES, 24><<<nb, kBlock, 0, st>>>(row_mesh(m), rec, mt, u, 0.0, vals, R);
             else
-                k_jacobian_pattern<1, VT, RES, 24><<<nb, kBlock, 0, st>>>(row_mesh(m), mt, u, z, body, vals, R);
+                k_jacobian_pattern<1, VT, RES, 24><<<nb, kBlock, 0, st>>>(row_mesh(m), rec, mt, u, body, vals, R);
         } else {
             if (vf->dpn == 3)
-                k_jacobian_pattern<3, VT, RES, 32><<<nb, kBlock, 0, st>>>(row_mesh(m), mt, u, z, 0.0, vals, R);
+                k_jacobian_pattern<3, VT, RES, 32><<<nb, kBlock, 0, st>>>(row_mesh(m), rec, mt, u, 0.0, vals, R);
             else
-                k_jacobian_pattern<1, VT, RES, 32><<<nb, kBlock, 0, st>>>(row_mesh(m), mt, u, z, body, vals, R);
+                k_jacobian_pattern<1, VT, RES, 32><<<nb, kBlock, 0, st>>>(row_mesh(m), rec, mt, u, body, vals, R);
         }
         check_launch();
     } else {

@@ -52,16 +52,24 @@ static void build_row_patterns(pa_mesh* m, const std::vector<int>& off, const st
     std::vector<int> row_pat(N);
     std::vector<std::string> sigs;
     std::string sig;
+    constexpr int kRec = 5 + 16;  // a, four slots, four int32 node deltas
+    std::vector<int> conn(4L * m->C);
+    PA_HIP(hipMemcpy(conn.data(), m->d_conn.p, conn.size() * sizeof(int), hipMemcpyDeviceToHost));
     for (int i = 0; i < N; ++i) {
         const int np = off[i + 1] - off[i];
         const int rl = m->h_rowmap[i + 1] - m->h_rowmap[i];
-        sig.assign(2 + 5 * (size_t)np, '\0');
+        sig.assign(2 + kRec * (size_t)np, '\0');
         sig[0] = (char)np;
         sig[1] = (char)rl;
         for (int p = 0; p < np; ++p) {
             const int2 pr = pairs[off[i] + p];
-            sig[2 + 5 * p] = (char)(pr.x & 3);
-            for (int b = 0; b < 4; ++b) sig[3 + 5 * p + b] = (char)(((unsigned)pr.y >> (8 * b)) & 0xff);
+            char* r = &sig[2 + kRec * p];
+            r[0] = (char)(pr.x & 3);
+            for (int b = 0; b < 4; ++b) r[1 + b] = (char)(((unsigned)pr.y >> (8 * b)) & 0xff);
+            for (int b = 0; b < 4; ++b) {
+                const int d = conn[(long)(pr.x >> 2) * 4 + b] - i;
+                std::memcpy(r + 5 + 4 * b, &d, 4);
+            }
         }
         auto it = index.find(sig);
         if (it == index.end()) {
@@ -70,17 +78,22 @@ static void build_row_patterns(pa_mesh* m, const std::vector<int>& off, const st
         }
         row_pat[i] = it->second;
     }
-    std::vector<int4> info;
+    std::vector<int4> info, pa_delta;
     std::vector<unsigned char> pa_a, pa_ent;
     std::vector<unsigned> pa_item;
     for (const std::string& s : sigs) {
         const int np = (unsigned char)s[0], rl = (unsigned char)s[1];
-        auto A = [&](int p) { return (int)(unsigned char)s[2 + 5 * p]; };
-        auto S = [&](int p, int b) { return (int)(unsigned char)s[3 + 5 * p + b]; };
+        auto A = [&](int p) { return (int)(unsigned char)s[2 + kRec * p]; };
+        auto S = [&](int p, int b) { return (int)(unsigned char)s[3 + kRec * p + b]; };
         const int diag = np ? S(0, A(0)) : 0;
         while (pa_ent.size() % 4) pa_ent.push_back(0);  // kernels read the entries as 32-bit words
         const int a_off = (int)pa_a.size(), ent_off = (int)pa_ent.size(), item_off = (int)pa_item.size();
-        for (int p = 0; p < np; ++p) pa_a.push_back((unsigned char)A(p));
+        for (int p = 0; p < np; ++p) {
+            pa_a.push_back((unsigned char)A(p));
+            int d[4];
+            std::memcpy(d, &s[2 + kRec * p + 5], 16);
+            pa_delta.push_back(make_int4(d[0], d[1], d[2], d[3]));
+        }
         int first_diag = -1;
         for (int slot = 0; slot < rl; ++slot) {
             const int start = (int)pa_ent.size() - ent_off;
@@ -111,6 +124,7 @@ static void build_row_patterns(pa_mesh* m, const std::vector<int>& off, const st
     m->d_row_pat.upload(row_pat.data(), row_pat.size());
     m->d_pat_info.upload(info.data(), info.size());
     m->d_pat_a.upload(pa_a.data(), pa_a.size());
+    m->d_pat_delta.upload(pa_delta.data(), pa_delta.size());
     m->d_pat_ent.upload(pa_ent.data(), pa_ent.size());
     m->d_pat_item.upload(pa_item.data(), pa_item.size());
     m->has_patterns = true;

@@ -111,6 +111,10 @@ struct pa_mesh {
     pa::DevBuf<int> d_row_pat;      // N: pattern id of each row
     pa::DevBuf<int4> d_pat_info;    // 2 per pattern: {npairs, rowlen, nitems, diag_slot}, {a_off, ent_off, item_off, first_diag_item}
     pa::DevBuf<unsigned char> d_pat_a;    // per pattern pair: local index of the row node in the cell
+    pa::DevBuf<int4> d_pat_delta;         // per pattern pair: node id - row id of the cell's four nodes
+    // per-call packed node records (x, y, z, density); every function on this
+    // mesh shares it, so their calls must not overlap in time
+    mutable pa::DevBuf<double4> d_nrec;
     pa::DevBuf<unsigned char> d_pat_ent;  // contributions (pair << 2 | local node b), grouped by column slot
     pa::DevBuf<unsigned> d_pat_item;      // work items: slot | start << 8 | count << 20
     std::map<std::string, pa::SideSet> side_sets;
