@@ -50,6 +50,7 @@ struct RowMesh {
     const unsigned char* __restrict__ pat_ent;
     const unsigned* __restrict__ pat_item;
     const int4* __restrict__ pat_delta;
+    const uint4* __restrict__ pat_item2;
 };
 
 // Blocks are dispatched round-robin over the 8 XCDs; renumber them so each
@@ -470,11 +471,12 @@ __global__ __launch_bounds__(kBlock) void k_jacobian_pattern(RowMesh m, const do
                                                              VT* __restrict__ vals, double* __restrict__ R) {
     constexpr int G = 32, ROWS = kBlock / G;
     constexpr int MS = DPN == 3 ? 9 : 1;  // accumulator entries (block size)
-    constexpr int PD = 12;                // per pair: w g_a (0..2), g_0..g_2 (3..11); g_3 = -(g_0+g_1+g_2)
+    // per pair: w g_a (0..2), g_0..g_3 (3..14); the odd stride of 15 doubles
+    // spreads lanes reading different pairs over the LDS banks
+    constexpr int PD = 15;
     static_assert(PMAX * PD >= 32 * MS, "the pair buffer also stages the output row");
     __shared__ double pd[ROWS][PMAX * PD];
     __shared__ double dg[ROWS][4][MS];          // diagonal chunk partial sums
-    __shared__ unsigned ent_s[ROWS][PMAX];      // the row's contribution entries, 4 per word
     const int grp = threadIdx.x / G, lane = threadIdx.x % G;
     const int row = xcd_block() * ROWS + grp;
     const bool active = row < m.N;
@@ -488,10 +490,9 @@ __global__ __launch_bounds__(kBlock) void k_jacobian_pattern(RowMesh m, const do
         rl = i0.y;
     }
     const int npairs = i0.x, nitems = i0.z, diag = i0.w;
-    // the work item of this lane and the row's entries (4 * npairs bytes, word aligned)
-    unsigned item = 0;
-    if (lane < nitems) item = m.pat_item[i1.z + lane];
-    if (lane < npairs) ent_s[grp][lane] = reinterpret_cast<const unsigned*>(m.pat_ent + i1.y)[lane];
+    // the work item of this lane with its contribution entries
+    uint4 item = make_uint4(0u, 0u, 0u, 0u);
+    if (lane < nitems) item = m.pat_item2[i1.z + lane];
     double r[DPN];
 #pragma unroll
     for (int i = 0; i < DPN; ++i) r[i] = 0.0;
@@ -518,7 +519,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobian_pattern(RowMesh m, const do
         P[1] = w * ga[1];
         P[2] = w * ga[2];
 #pragma unroll
-        for (int n = 0; n < 3; ++n)
+        for (int n = 0; n < 4; ++n)
 #pragma unroll
             for (int d = 0; d < 3; ++d) P[3 + 3 * n + d] = c.g[n][d];
         if constexpr (RES) {
@@ -553,28 +554,21 @@ __global__ __launch_bounds__(kBlock) void k_jacobian_pattern(RowMesh m, const do
         }
     }
     __syncthreads();
-    const int slot = item & 0xff;
+    const int slot = item.x & 0xff;
     double M[MS];
 #pragma unroll
     for (int q = 0; q < MS; ++q) M[q] = 0.0;
     if (lane < nitems) {
-        const int start = (item >> 8) & 0xfff, count = item >> 20;
-        const unsigned char* ent = reinterpret_cast<const unsigned char*>(ent_s[grp]) + start;
-        for (int e = 0; e < count; ++e) {
-            const int pe = ent[e];
+        const int count = (item.x >> 8) & 0xff;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            if (e >= count) break;
+            const unsigned word = e < 4 ? item.y : item.z;
+            const int pe = (word >> (8 * (e % 4))) & 0xff;
             const double* P = &pd[grp][(pe >> 2) * PD];
-            const int b = pe & 3;
+            const double* gb = P + 3 + 3 * (pe & 3);
             const double a0 = P[0], a1 = P[1], a2 = P[2];
-            double b0, b1, b2;
-            if (b < 3) {
-                b0 = P[3 + 3 * b];
-                b1 = P[4 + 3 * b];
-                b2 = P[5 + 3 * b];
-            } else {
-                b0 = -(P[3] + P[6] + P[9]);
-                b1 = -(P[4] + P[7] + P[10]);
-                b2 = -(P[5] + P[8] + P[11]);
-            }
+            const double b0 = gb[0], b1 = gb[1], b2 = gb[2];
             if constexpr (DPN == 3) {
                 M[0] += a0 * b0; M[1] += a0 * b1; M[2] += a0 * b2;
                 M[3] += a1 * b0; M[4] += a1 * b1; M[5] += a1 * b2;
@@ -970,6 +964,7 @@ RowMesh row_mesh(const pa_mesh* m) {
     r.pat_ent = m->d_pat_ent.p;
     r.pat_item = m->d_pat_item.p;
     r.pat_delta = m->d_pat_delta.p;
+    r.pat_item2 = m->d_pat_item2.p;
     return r;
 }
 

@@ -81,6 +81,7 @@ static void build_row_patterns(pa_mesh* m, const std::vector<int>& off, const st
     std::vector<int4> info, pa_delta;
     std::vector<unsigned char> pa_a, pa_ent;
     std::vector<unsigned> pa_item;
+    std::vector<uint4> pa_item2;
     for (const std::string& s : sigs) {
         const int np = (unsigned char)s[0], rl = (unsigned char)s[1];
         auto A = [&](int p) { return (int)(unsigned char)s[2 + kRec * p]; };
@@ -95,25 +96,34 @@ static void build_row_patterns(pa_mesh* m, const std::vector<int>& off, const st
             pa_delta.push_back(make_int4(d[0], d[1], d[2], d[3]));
         }
         int first_diag = -1;
+        std::vector<unsigned char> slot_ent;
+        auto push_item = [&](int slot, int first, int count) {
+            // 16-byte item: slot | count << 8, then up to 8 entry bytes
+            unsigned w[4] = {(unsigned)slot | (unsigned)count << 8, 0u, 0u, 0u};
+            for (int e = 0; e < count; ++e) w[1 + e / 4] |= (unsigned)slot_ent[first + e] << (8 * (e % 4));
+            pa_item2.push_back(make_uint4(w[0], w[1], w[2], w[3]));
+            pa_item.push_back((unsigned)slot | (unsigned)count << 20);
+        };
+        bool fits = true;
         for (int slot = 0; slot < rl; ++slot) {
-            const int start = (int)pa_ent.size() - ent_off;
+            slot_ent.clear();
             for (int p = 0; p < np; ++p)
                 for (int b = 0; b < 4; ++b)
-                    if (S(p, b) == slot) pa_ent.push_back((unsigned char)(p << 2 | b));
-            const int count = (int)pa_ent.size() - ent_off - start;
+                    if (S(p, b) == slot) slot_ent.push_back((unsigned char)(p << 2 | b));
+            for (unsigned char e : slot_ent) pa_ent.push_back(e);
+            const int count = (int)slot_ent.size();
             if (slot == diag) {
                 first_diag = (int)pa_item.size() - item_off;
-                for (int c = 0; c < count; c += kDiagChunk)
-                    pa_item.push_back((unsigned)slot | (unsigned)(start + c) << 8 |
-                                      (unsigned)std::min(kDiagChunk, count - c) << 20);
-                if (count == 0) pa_item.push_back((unsigned)slot | (unsigned)start << 8);
+                for (int c = 0; c < count; c += kDiagChunk) push_item(slot, c, std::min(kDiagChunk, count - c));
+                if (count == 0) push_item(slot, 0, 0);
             } else {
-                pa_item.push_back((unsigned)slot | (unsigned)start << 8 | (unsigned)count << 20);
+                if (count > 8) fits = false;
+                push_item(slot, 0, std::min(count, 8));
             }
         }
         const int nitems = (int)pa_item.size() - item_off;
         const int ndiag = (np + kDiagChunk - 1) / kDiagChunk;
-        if (nitems > kMaxItems || ndiag > 4) return;
+        if (!fits || nitems > kMaxItems || ndiag > 4) return;
         info.push_back(make_int4(np, rl, nitems, diag));
         info.push_back(make_int4(a_off, ent_off, item_off, first_diag));
     }
@@ -125,6 +135,7 @@ static void build_row_patterns(pa_mesh* m, const std::vector<int>& off, const st
     m->d_pat_info.upload(info.data(), info.size());
     m->d_pat_a.upload(pa_a.data(), pa_a.size());
     m->d_pat_delta.upload(pa_delta.data(), pa_delta.size());
+    m->d_pat_item2.upload(pa_item2.data(), pa_item2.size());
     m->d_pat_ent.upload(pa_ent.data(), pa_ent.size());
     m->d_pat_item.upload(pa_item.data(), pa_item.size());
     m->has_patterns = true;

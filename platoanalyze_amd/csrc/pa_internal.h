@@ -112,6 +112,7 @@ struct pa_mesh {
     pa::DevBuf<int4> d_pat_info;    // 2 per pattern: {npairs, rowlen, nitems, diag_slot}, {a_off, ent_off, item_off, first_diag_item}
     pa::DevBuf<unsigned char> d_pat_a;    // per pattern pair: local index of the row node in the cell
     pa::DevBuf<int4> d_pat_delta;         // per pattern pair: node id - row id of the cell's four nodes
+    pa::DevBuf<uint4> d_pat_item2;        // work items with their (<= 8) entries: slot | count << 8, entry bytes
     // per-call packed node records (x, y, z, density); every function on this
     // mesh shares it, so their calls must not overlap in time
     mutable pa::DevBuf<double4> d_nrec;
