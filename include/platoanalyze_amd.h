@@ -232,17 +232,18 @@ int pa_solver_pcg(pa_solver* solver, const void* values_dev, int precision, cons
  * dF/dz = dF/dz|_u + (dR/dz)^T lambda with lambda = -u.
  *
  * mode PA_FP64: FP64 assembly and FP64-matrix PCG to `tol`.
- * mode PA_FP32 (mixed): FP32 assembly and FP32-matrix PCG sweeps to
- * `inner_tol`, each followed by an FP64 matrix-free residual b - K(z) x
- * computed by the assembly kernels, until ||r|| <= tol ||b||.
+ * mode PA_FP32 (mixed): FP32 assembly and PCG on the FP32 matrix with
+ * reliable updates: every `replace_every` iterations and at apparent
+ * convergence the recursive residual is replaced by the exact FP64 residual
+ * b - K(z) x, computed matrix-free by the assembly kernels; the solve ends
+ * when that FP64 residual satisfies ||r|| <= tol ||b||.
  */
 typedef struct {
     int mode;          /* PA_FP64 or PA_FP32 */
     int precond;       /* PA_PRECOND_* */
     double tol;        /* relative residual target (FP64 residual) */
-    double inner_tol;  /* relative target of one FP32 sweep (mixed mode) */
-    int max_iter;      /* CG iteration cap per sweep */
-    int max_sweeps;    /* refinement sweep cap (mixed mode) */
+    int max_iter;      /* CG iteration cap */
+    int replace_every; /* mixed mode: iterations between exact-residual replacements (0: only at convergence) */
     int check_every;   /* convergence check period (iterations) */
 } pa_problem_options;
 

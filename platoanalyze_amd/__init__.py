@@ -48,8 +48,7 @@ class SolveStats(ctypes.Structure):
 
 class ProblemOptions(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("precond", ctypes.c_int), ("tol", ctypes.c_double),
-                ("inner_tol", ctypes.c_double), ("max_iter", ctypes.c_int), ("max_sweeps", ctypes.c_int),
-                ("check_every", ctypes.c_int)]
+                ("max_iter", ctypes.c_int), ("replace_every", ctypes.c_int), ("check_every", ctypes.c_int)]
 
 
 def _load():
@@ -503,12 +502,11 @@ class Problem:
     (src/elliptic/Problem.hpp:237-450), self-adjoint criterion."""
 
     def __init__(self, vector_function, criterion, bc_dofs, bc_values=None, *, mode=FP64,
-                 precond=PRECOND_BLOCK_JACOBI, tol=1e-8, inner_tol=1e-5, max_iter=20000, max_sweeps=10,
-                 check_every=10):
+                 precond=PRECOND_BLOCK_JACOBI, tol=1e-8, max_iter=20000, replace_every=200, check_every=10):
         self.vf, self.sf = vector_function, criterion
         dofs = np.ascontiguousarray(bc_dofs, np.int32)
         vals = np.zeros(dofs.size) if bc_values is None else np.ascontiguousarray(bc_values, np.float64)
-        self.options = ProblemOptions(_prec(mode)[0], precond, tol, inner_tol, max_iter, max_sweeps, check_every)
+        self.options = ProblemOptions(_prec(mode)[0], precond, tol, max_iter, replace_every, check_every)
         h = ctypes.c_void_p()
         _check(lib.pa_problem_create(self.vf.handle, self.sf.handle, dofs.size, _np_ptr(dofs), _np_ptr(vals),
                                      ctypes.byref(self.options), ctypes.byref(h)))

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <functional>
 #include <vector>
 
 #include "pa_internal.h"
@@ -297,9 +298,38 @@ double norm2(pa_solver* s, const double* v, hipStream_t st) {
     return read_scalar(s, S_N - 1, st);
 }
 
+// z = M r and the r.z, r.r partial sums after the residual was replaced.
+template <int D>
+__global__ __launch_bounds__(kB) void k_pcg_reprec(int N, int prec, const double* __restrict__ minv,
+                                                   const double* __restrict__ r, double* __restrict__ zv,
+                                                   double* __restrict__ part, int nparts) {
+    __shared__ double sh[kB / 64];
+    double rz = 0.0, rr = 0.0;
+    for (long node = (long)blockIdx.x * kB + threadIdx.x; node < N; node += (long)gridDim.x * kB) {
+        double rv[D], zz[D];
+#pragma unroll
+        for (int i = 0; i < D; ++i) rv[i] = r[node * D + i];
+        apply_prec<D>(prec, minv, node, rv, zz);
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            zv[node * D + i] = zz[i];
+            rz += rv[i] * zz[i];
+            rr += rv[i] * rv[i];
+        }
+    }
+    const double a = block_sum(rz, sh), c = block_sum(rr, sh);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = a;
+        part[nparts + blockIdx.x] = c;
+    }
+}
+
+// Exact residual supplier for reliable updates: writes b - A_exact x into r.
+using ExactResidual = std::function<void(const double* x, double* r)>;
+
 template <int D, class VT>
 void pcg_impl(pa_solver* s, const VT* vals, const double* b, double* x, double tol, int max_iter, int check_every,
-              pa_solve_stats* stats, hipStream_t st) {
+              pa_solve_stats* stats, hipStream_t st, const ExactResidual* exact = nullptr, int replace_every = 0) {
     const pa_mesh* m = s->mesh;
     const int N = m->N;
     const long n = (long)N * D;
@@ -336,12 +366,29 @@ void pcg_impl(pa_solver* s, const VT* vals, const double* b, double* x, double t
         it += burst;
         rr = read_scalar(s, S_RR, st);
         if (!std::isfinite(rr)) break;
+        const bool converged = std::sqrt(rr) <= target;
+        if (exact && (converged || (replace_every > 0 && it % replace_every == 0))) {
+            // reliable update: swap the recursive residual for the exact one,
+            // keep the search direction
+            (*exact)(x, s->r.p);
+            k_pcg_reprec<D><<<nvb, kB, 0, st>>>(N, s->precond, s->d_minv.p, s->r.p, s->zv.p, s->part.p, nvb);
+            check();
+            k_reduce<<<1, 1024, 0, st>>>(s->part.p, nvb, nvb, 2, S_RZ, S_RR, 0, 0, s->scal.p);
+            check();
+            rr = read_scalar(s, S_RR, st);
+        }
     }
-    // FP64 residual of the matrix in use
-    spmv_dot<D, VT>(s, vals, x, s->q.p, st);
-    k_axpy<<<nvb, kB, 0, st>>>(n, -1.0, b, s->q.p);  // q = A x - b
-    check();
-    const double rn = std::sqrt(norm2(s, s->q.p, st));
+    double rn;
+    if (exact) {
+        (*exact)(x, s->q.p);
+        rn = std::sqrt(norm2(s, s->q.p, st));
+    } else {
+        // FP64 residual of the matrix in use
+        spmv_dot<D, VT>(s, vals, x, s->q.p, st);
+        k_axpy<<<nvb, kB, 0, st>>>(n, -1.0, b, s->q.p);  // q = A x - b
+        check();
+        rn = std::sqrt(norm2(s, s->q.p, st));
+    }
     stats->iterations = it;
     stats->sweeps = 1;
     stats->rel_residual = bnorm > 0.0 ? rn / bnorm : 0.0;
@@ -506,34 +553,25 @@ int pa_problem_solution(pa_problem* P, const double* z, double* u, pa_solve_stat
             pa::pcg(P->solver, K, PA_FP64, P->rhs.p, u, tol, P->opt.max_iter, P->opt.check_every, stats, st);
             return;
         }
-        // mixed: FP32-matrix sweeps on the FP64 residual of the true operator
-        const double bnorm = std::sqrt(pa::norm2(P->solver, P->rhs.p, st));
-        PA_HIP(hipMemcpyAsync(P->res.p, P->rhs.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
-        pa_solve_stats total{0, 0, 1.0, 0};
-        const int sweeps = P->opt.max_sweeps > 0 ? P->opt.max_sweeps : 1;
-        for (int k = 0; k < sweeps; ++k) {
-            PA_HIP(hipMemsetAsync(P->dx.p, 0, n * sizeof(double), st));
-            pa_solve_stats in{};
-            pa::pcg(P->solver, K, PA_FP32, P->res.p, P->dx.p, P->opt.inner_tol, P->opt.max_iter, P->opt.check_every,
-                    &in, st);
-            total.iterations += in.iterations;
-            total.sweeps += 1;
-            pa::k_axpy<<<nvb, pa::kB, 0, st>>>(n, 1.0, P->dx.p, u);
-            pa::check();
-            // r = -R(u) (FP64, matrix-free), constrained rows g - u
-            pa::check_status(pa_vector_function_value(vf, u, z, P->res.p, stream));
-            pa::k_neg<<<nvb, pa::kB, 0, st>>>(n, P->res.p);
+        // mixed: CG on the FP32 matrix with reliable updates, the recursive
+        // residual replaced by the exact FP64 one, -R(u) from the
+        // matrix-free assembly kernels (constrained rows g - u)
+        const pa::ExactResidual exact = [&](const double* x, double* r) {
+            pa::check_status(pa_vector_function_value(vf, x, z, r, stream));
+            pa::k_neg<<<nvb, pa::kB, 0, st>>>(n, r);
             pa::check();
             if (P->nbc) {
-                pa::k_bc_residual<<<(P->nbc + 255) / 256, 256, 0, st>>>(P->nbc, P->bc_dofs.p, P->bc_vals.p, u,
-                                                                          P->res.p);
+                pa::k_bc_residual<<<(P->nbc + 255) / 256, 256, 0, st>>>(P->nbc, P->bc_dofs.p, P->bc_vals.p, x, r);
                 pa::check();
             }
-            total.rel_residual = bnorm > 0.0 ? std::sqrt(pa::norm2(P->solver, P->res.p, st)) / bnorm : 0.0;
-            if (total.rel_residual <= tol) break;
-        }
-        total.converged = total.rel_residual <= tol ? 1 : 0;
-        *stats = total;
+        };
+        const int every = P->opt.replace_every > 0 ? P->opt.replace_every : 0;
+        if (dpn == 3)
+            pa::pcg_impl<3, float>(P->solver, (const float*)K, P->rhs.p, u, tol, P->opt.max_iter,
+                                   P->opt.check_every, stats, st, &exact, every);
+        else
+            pa::pcg_impl<1, float>(P->solver, (const float*)K, P->rhs.p, u, tol, P->opt.max_iter,
+                                   P->opt.check_every, stats, st, &exact, every);
     });
 }
 

@@ -126,7 +126,7 @@ def test_mixed_precision_reaches_fp64_residual(pa):
     P = pa.Problem(vf, sf, dofs, mode="fp64", tol=1e-8)
     u64 = P.solution(z).numpy()
     st64 = P.last_stats
-    P.set_options(mode="fp32", inner_tol=1e-5, max_sweeps=10)
+    P.set_options(mode="fp32", replace_every=100)
     u32 = P.solution(z).numpy()
     st32 = P.last_stats
     assert st64["converged"] and st32["converged"], (st64, st32)
