@@ -203,7 +203,40 @@ int pa_block_spmv(const pa_mesh* mesh, int bi, int bj, const void* values_dev, i
  * PCG, src/alg/AmgXConfigs.cpp:110-134).  Vectors and all reductions are
  * FP64; the matrix is FP64 or FP32 per `precision`.
  */
-enum { PA_PRECOND_NONE = 0, PA_PRECOND_JACOBI = 1, PA_PRECOND_BLOCK_JACOBI = 2 };
+enum { PA_PRECOND_NONE = 0, PA_PRECOND_JACOBI = 1, PA_PRECOND_BLOCK_JACOBI = 2, PA_PRECOND_MULTIGRID = 3 };
+
+/*
+ * Geometric multigrid preconditioner for meshes built by pa_box_mesh (nx x ny
+ * x nz hexes, 6 tets each): levels halve the grid while the cell counts are
+ * even; trilinear transfer; Galerkin coarse operators P^T A P stored as
+ * 27-point block stencils; damped block-Jacobi smoothing; dense inverse on the
+ * coarsest level.  FP32 throughout (a preconditioner).  No counterpart in the
+ * reference (its AmgX path, src/alg/AmgXConfigs.cpp, is algebraic multigrid).
+ */
+typedef struct {
+    int max_levels;       /* 0: coarsen while possible */
+    int pre_sweeps;       /* block-Jacobi sweeps before the coarse correction */
+    int post_sweeps;      /* ... and after */
+    double omega;         /* Jacobi damping (0: 0.6) */
+    int max_coarse_nodes; /* limit of the dense coarsest level (0: 1000) */
+} pa_multigrid_options;
+
+typedef struct pa_multigrid pa_multigrid;
+int pa_multigrid_create(const pa_mesh* mesh, int nx, int ny, int nz, int dpn, const pa_multigrid_options* options,
+                        pa_multigrid** mg);
+int pa_multigrid_destroy(pa_multigrid* mg);
+int pa_multigrid_levels(const pa_multigrid* mg, int* levels, int* coarse_dofs);
+/* Build the hierarchy from the (constrained) fine matrix; constrained dofs are masked out of the transfer. */
+int pa_multigrid_setup(pa_multigrid* mg, const void* values_dev, int precision, int num_bcs, const int* bc_dofs_dev,
+                       void* stream);
+/* z = M^-1 r by one V-cycle. */
+int pa_multigrid_apply(pa_multigrid* mg, const double* r_dev, double* z_dev, void* stream);
+/*
+ * Copy the Galerkin operator of `level` (>= 1) to the host as 27-point block
+ * stencils: nodes x 27 x dpn*dpn floats, offset index (dx+1)*9 + (dy+1)*3 +
+ * (dz+1).  Synchronizes.  (Diagnostic: lets tests check P^T A P.)
+ */
+int pa_multigrid_level_operator(const pa_multigrid* mg, int level, float* stencil_host, int* nx, int* ny, int* nz);
 
 typedef struct pa_solver pa_solver;
 typedef struct {
@@ -215,6 +248,8 @@ typedef struct {
 
 int pa_solver_create(const pa_mesh* mesh, int dpn, int precond, pa_solver** solver);
 int pa_solver_destroy(pa_solver* solver);
+/* Use `mg` (set up by the caller) as the preconditioner; the solver does not own it. */
+int pa_solver_set_multigrid(pa_solver* solver, pa_multigrid* mg);
 /*
  * Solve A x = b starting from the x given.  Stops when the recursive residual
  * satisfies ||r|| <= tol ||b|| (checked every `check_every` iterations; the
@@ -252,6 +287,12 @@ int pa_problem_create(pa_vector_function* vf, pa_scalar_function* criterion, int
                       const double* bc_values_host, const pa_problem_options* options, pa_problem** problem);
 int pa_problem_destroy(pa_problem* problem);
 int pa_problem_set_options(pa_problem* problem, const pa_problem_options* options);
+/*
+ * Precondition the solves with geometric multigrid on the nx x ny x nz box
+ * (rebuilt from each new Jacobian inside pa_problem_solution); selects
+ * PA_PRECOND_MULTIGRID.
+ */
+int pa_problem_set_multigrid(pa_problem* problem, int nx, int ny, int nz, const pa_multigrid_options* options);
 /* State u (N*dpn) for control z; statistics of the linear solve. Synchronizes. */
 int pa_problem_solution(pa_problem* problem, const double* z_dev, double* u_dev, pa_solve_stats* stats, void* stream);
 /* Criterion value at (u, z), one double at value_dev. */

@@ -46,6 +46,14 @@ class SolveStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+PRECOND_MULTIGRID = 3
+
+
+class MultigridOptions(ctypes.Structure):
+    _fields_ = [("max_levels", ctypes.c_int), ("pre_sweeps", ctypes.c_int), ("post_sweeps", ctypes.c_int),
+                ("omega", ctypes.c_double), ("max_coarse_nodes", ctypes.c_int)]
+
+
 class ProblemOptions(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("precond", ctypes.c_int), ("tol", ctypes.c_double),
                 ("max_iter", ctypes.c_int), ("replace_every", ctypes.c_int), ("check_every", ctypes.c_int)]
@@ -111,6 +119,14 @@ def _load():
         "pa_problem_solution": ([vp, vp, vp, ctypes.POINTER(SolveStats), vp], i),
         "pa_problem_criterion_value": ([vp, vp, vp, vp, vp], i),
         "pa_problem_criterion_gradient": ([vp, vp, vp, vp, vp], i),
+        "pa_problem_set_multigrid": ([vp, i, i, i, ctypes.POINTER(MultigridOptions)], i),
+        "pa_solver_set_multigrid": ([vp, vp], i),
+        "pa_multigrid_create": ([vp, i, i, i, i, ctypes.POINTER(MultigridOptions), ctypes.POINTER(vp)], i),
+        "pa_multigrid_destroy": ([vp], i),
+        "pa_multigrid_levels": ([vp, ip, ip], i),
+        "pa_multigrid_setup": ([vp, vp, i, i, vp, vp], i),
+        "pa_multigrid_apply": ([vp, vp, vp, vp], i),
+        "pa_multigrid_level_operator": ([vp, i, vp, ip, ip, ip], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -518,6 +534,13 @@ class Problem:
             setattr(self.options, k, _prec(v)[0] if k == "mode" else v)
         _check(lib.pa_problem_set_options(self.handle, ctypes.byref(self.options)))
 
+    def set_multigrid(self, nx, ny, nz, *, max_levels=0, pre_sweeps=2, post_sweeps=2, omega=0.6,
+                      max_coarse_nodes=1000):
+        """Geometric multigrid preconditioner on the nx x ny x nz box mesh."""
+        opts = MultigridOptions(max_levels, pre_sweeps, post_sweeps, omega, max_coarse_nodes)
+        _check(lib.pa_problem_set_multigrid(self.handle, nx, ny, nz, ctypes.byref(opts)))
+        self.options.precond = PRECOND_MULTIGRID
+
     def solution(self, z, out=None, stream=None):
         z = _dev(z)
         out = out if out is not None else DeviceArray(self.vf.size())
@@ -541,5 +564,46 @@ class Problem:
     def __del__(self):
         try:
             lib.pa_problem_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class Multigrid:
+    """Geometric multigrid preconditioner on a box mesh (FP32 V-cycles)."""
+
+    def __init__(self, mesh, nx, ny, nz, dpn, *, max_levels=0, pre_sweeps=2, post_sweeps=2, omega=0.6,
+                 max_coarse_nodes=1000):
+        opts = MultigridOptions(max_levels, pre_sweeps, post_sweeps, omega, max_coarse_nodes)
+        h = ctypes.c_void_p()
+        _check(lib.pa_multigrid_create(mesh.handle, nx, ny, nz, dpn, ctypes.byref(opts), ctypes.byref(h)))
+        self.handle, self.mesh, self.dpn = h, mesh, dpn
+
+    def levels(self):
+        L, c = ctypes.c_int(), ctypes.c_int()
+        _check(lib.pa_multigrid_levels(self.handle, ctypes.byref(L), ctypes.byref(c)))
+        return L.value, c.value
+
+    def setup(self, matrix, bc_dofs=None, stream=None):
+        dofs = DeviceArray.from_numpy(np.asarray(bc_dofs if bc_dofs is not None else [], np.int32), np.int32)
+        _check(lib.pa_multigrid_setup(self.handle, matrix.values.ptr, matrix.precision, dofs.size, dofs.ptr,
+                                      _sh(stream)))
+        self._dofs = dofs
+
+    def apply(self, r, z, stream=None):
+        _check(lib.pa_multigrid_apply(self.handle, r.ptr, z.ptr, _sh(stream)))
+        return z
+
+    def level_operator(self, level):
+        nx, ny, nz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib.pa_multigrid_level_operator(self.handle, level, None, ctypes.byref(nx), ctypes.byref(ny),
+                                               ctypes.byref(nz)))
+        n = (nx.value + 1) * (ny.value + 1) * (nz.value + 1)
+        out = np.empty(n * 27 * self.dpn * self.dpn, np.float32)
+        _check(lib.pa_multigrid_level_operator(self.handle, level, _np_ptr(out), None, None, None))
+        return (nx.value, ny.value, nz.value), out.reshape(n, 27, self.dpn, self.dpn)
+
+    def __del__(self):
+        try:
+            lib.pa_multigrid_destroy(self.handle)
         except Exception:
             pass

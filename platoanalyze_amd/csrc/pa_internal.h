@@ -144,6 +144,8 @@ namespace pa {
 // assembly kernels; used by the problem driver.
 void self_adjoint_gradient(const pa_vector_function* vf, const pa_scalar_function* sf, const double* u,
                            const double* z, double* out, hipStream_t st);
+// z = M^-1 r by one multigrid V-cycle (pa_multigrid.hip)
+void multigrid_apply(pa_multigrid* mg, const double* r, double* z, hipStream_t st);
 }  // namespace pa
 
 struct pa_scalar_function {

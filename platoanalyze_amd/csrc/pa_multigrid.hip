@@ -1,0 +1,644 @@
+// Geometric multigrid preconditioner for the structured Kuhn box meshes.
+//
+// Levels coarsen the hex grid by two in every direction while all cell counts
+// stay even.  Transfer is trilinear interpolation on the grid vertices
+// (fine vertex i takes weight 1 from the coarse vertex at i/2 when i is even,
+// 1/2 from each neighbour when i is odd, per direction).  Coarse operators
+// are Galerkin products A_{l+1} = P^T A_l P, computed on the device and
+// stored as 27-point block stencils; the finest level uses the assembled
+// block-CRS matrix itself (its Dirichlet dofs masked out of P).  Smoother:
+// damped block Jacobi, symmetric pre/post sweeps, so the V-cycle is an SPD
+// preconditioner for CG.  Coarsest level: dense inverse (FP64 Gauss-Jordan in
+// one workgroup).  Level matrices and vectors are FP32.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "pa_internal.h"
+
+namespace pa {
+namespace {
+
+constexpr int kT = 256;
+
+struct Grid {
+    int nx, ny, nz;  // cells
+    __host__ __device__ int nodes() const { return (nx + 1) * (ny + 1) * (nz + 1); }
+    __host__ __device__ int id(int i, int j, int k) const { return (i * (ny + 1) + j) * (nz + 1) + k; }
+    __host__ __device__ void ijk(int n, int& i, int& j, int& k) const {
+        k = n % (nz + 1);
+        j = (n / (nz + 1)) % (ny + 1);
+        i = n / ((nz + 1) * (ny + 1));
+    }
+    __host__ __device__ bool in(int i, int j, int k) const {
+        return i >= 0 && i <= nx && j >= 0 && j <= ny && k >= 0 && k <= nz;
+    }
+};
+
+__device__ inline int sidx(int dx, int dy, int dz) { return (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1); }
+
+// 3x3 (or 1x1) block inverse in double, stored as float.
+template <int D>
+__device__ inline void block_inverse(const double* a, float* out) {
+    if constexpr (D == 3) {
+        const double c00 = a[4] * a[8] - a[5] * a[7], c01 = a[2] * a[7] - a[1] * a[8], c02 = a[1] * a[5] - a[2] * a[4];
+        const double c10 = a[5] * a[6] - a[3] * a[8], c11 = a[0] * a[8] - a[2] * a[6], c12 = a[2] * a[3] - a[0] * a[5];
+        const double c20 = a[3] * a[7] - a[4] * a[6], c21 = a[1] * a[6] - a[0] * a[7], c22 = a[0] * a[4] - a[1] * a[3];
+        const double det = a[0] * c00 + a[1] * c10 + a[2] * c20;
+        const double id = det != 0.0 ? 1.0 / det : 0.0;
+        out[0] = (float)(c00 * id); out[1] = (float)(c01 * id); out[2] = (float)(c02 * id);
+        out[3] = (float)(c10 * id); out[4] = (float)(c11 * id); out[5] = (float)(c12 * id);
+        out[6] = (float)(c20 * id); out[7] = (float)(c21 * id); out[8] = (float)(c22 * id);
+    } else {
+        out[0] = a[0] != 0.0 ? (float)(1.0 / a[0]) : 0.0f;
+    }
+}
+
+// ----------------------------------------------------------- fine level (CRS)
+template <int D, class VT>
+__global__ void k_diag_inv_crs(int N, const int* __restrict__ diag, const VT* __restrict__ vals,
+                               float* __restrict__ dinv) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    double a[D * D];
+#pragma unroll
+    for (int q = 0; q < D * D; ++q) a[q] = (double)vals[(long)diag[n] * D * D + q];
+    block_inverse<D>(a, dinv + (long)n * D * D);
+}
+
+template <int D>
+__global__ void k_diag_inv_st(int N, const float* __restrict__ A, float* __restrict__ dinv) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    double a[D * D];
+#pragma unroll
+    for (int q = 0; q < D * D; ++q) a[q] = (double)A[((long)n * 27 + 13) * D * D + q];
+    block_inverse<D>(a, dinv + (long)n * D * D);
+}
+
+// y = b - A x on the fine CRS matrix (16 lanes per row); masked dofs get 0.
+// zero_x: x is zero (y = b).  With omega != 0 the result is the Jacobi update
+// x_out = x + omega Dinv (b - A x) instead.
+template <int D, class VT>
+__global__ __launch_bounds__(kT) void k_crs_resid_jac(int N, const int* __restrict__ rowmap,
+                                                      const int* __restrict__ colidx, const VT* __restrict__ vals,
+                                                      const float* __restrict__ dinv, const unsigned char* __restrict__ mask,
+                                                      const float* __restrict__ b, const float* __restrict__ x,
+                                                      float* __restrict__ out, float omega, int zero_x) {
+    constexpr int G = 16, ROWS = kT / G;
+    const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+    const int row = blockIdx.x * ROWS + grp;
+    if (row >= N) return;
+    float acc[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) acc[i] = 0.0f;
+    if (!zero_x) {
+        for (int e = rowmap[row] + lane; e < rowmap[row + 1]; e += G) {
+            const long c = colidx[e];
+            const VT* v = vals + (long)e * D * D;
+#pragma unroll
+            for (int i = 0; i < D; ++i)
+#pragma unroll
+                for (int j = 0; j < D; ++j) acc[i] += (float)v[i * D + j] * x[c * D + j];
+        }
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int o = G / 2; o > 0; o >>= 1) acc[i] += __shfl_xor(acc[i], o, G);
+    }
+    if (lane == 0) {
+        float r[D];
+#pragma unroll
+        for (int i = 0; i < D; ++i) r[i] = b[(long)row * D + i] - acc[i];
+        if (omega != 0.0f) {
+            const float* m = dinv + (long)row * D * D;
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                float s = 0.0f;
+#pragma unroll
+                for (int j = 0; j < D; ++j) s += m[i * D + j] * r[j];
+                const float xo = zero_x ? 0.0f : x[(long)row * D + i];
+                out[(long)row * D + i] = mask[(long)row * D + i] ? xo + omega * s : 0.0f;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < D; ++i) out[(long)row * D + i] = mask[(long)row * D + i] ? r[i] : 0.0f;
+        }
+    }
+}
+
+// ------------------------------------------------------ coarse levels (stencil)
+template <int D>
+__global__ __launch_bounds__(kT) void k_st_resid_jac(Grid g, const float* __restrict__ A, const float* __restrict__ dinv,
+                                                     const float* __restrict__ b, const float* __restrict__ x,
+                                                     float* __restrict__ out, float omega, int zero_x) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nodes()) return;
+    int i, j, k;
+    g.ijk(n, i, j, k);
+    float acc[D];
+#pragma unroll
+    for (int q = 0; q < D; ++q) acc[q] = 0.0f;
+    if (!zero_x) {
+        const float* a = A + (long)n * 27 * D * D;
+        for (int dx = -1; dx <= 1; ++dx)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dz = -1; dz <= 1; ++dz) {
+                    if (!g.in(i + dx, j + dy, k + dz)) continue;
+                    const long c = g.id(i + dx, j + dy, k + dz);
+                    const float* v = a + sidx(dx, dy, dz) * D * D;
+#pragma unroll
+                    for (int p = 0; p < D; ++p)
+#pragma unroll
+                        for (int q = 0; q < D; ++q) acc[p] += v[p * D + q] * x[c * D + q];
+                }
+    }
+    float r[D];
+#pragma unroll
+    for (int p = 0; p < D; ++p) r[p] = b[(long)n * D + p] - acc[p];
+    if (omega != 0.0f) {
+        const float* m = dinv + (long)n * D * D;
+#pragma unroll
+        for (int p = 0; p < D; ++p) {
+            float s = 0.0f;
+#pragma unroll
+            for (int q = 0; q < D; ++q) s += m[p * D + q] * r[q];
+            out[(long)n * D + p] = (zero_x ? 0.0f : x[(long)n * D + p]) + omega * s;
+        }
+    } else {
+#pragma unroll
+        for (int p = 0; p < D; ++p) out[(long)n * D + p] = r[p];
+    }
+}
+
+// ------------------------------------------------------------------ transfer
+// weight of coarse coordinate C for fine coordinate f (|f - 2C| <= 1)
+__device__ inline float w1(int f, int C) {
+    const int d = f - 2 * C;
+    return d == 0 ? 1.0f : ((d == 1 || d == -1) ? 0.5f : 0.0f);
+}
+
+// b_c[J] = sum_i w_iJ mask_i r_f[i]
+template <int D>
+__global__ void k_restrict(Grid gf, Grid gc, const float* __restrict__ rf, const unsigned char* __restrict__ mask,
+                           float* __restrict__ bc) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= gc.nodes()) return;
+    int I, J, K;
+    gc.ijk(n, I, J, K);
+    float acc[D];
+#pragma unroll
+    for (int p = 0; p < D; ++p) acc[p] = 0.0f;
+    for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dz = -1; dz <= 1; ++dz) {
+                const int i = 2 * I + dx, j = 2 * J + dy, k = 2 * K + dz;
+                if (!gf.in(i, j, k)) continue;
+                const float w = w1(i, I) * w1(j, J) * w1(k, K);
+                const long f = gf.id(i, j, k);
+#pragma unroll
+                for (int p = 0; p < D; ++p)
+                    acc[p] += (mask == nullptr || mask[f * D + p]) ? w * rf[f * D + p] : 0.0f;
+            }
+#pragma unroll
+    for (int p = 0; p < D; ++p) bc[(long)n * D + p] = acc[p];
+}
+
+// x_f[i] += mask_i sum_J w_iJ x_c[J]
+template <int D>
+__global__ void k_prolong_add(Grid gf, Grid gc, const float* __restrict__ xc, const unsigned char* __restrict__ mask,
+                              float* __restrict__ xf) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= gf.nodes()) return;
+    int i, j, k;
+    gf.ijk(n, i, j, k);
+    const int I0 = i >> 1, I1 = (i + 1) >> 1, J0 = j >> 1, J1 = (j + 1) >> 1, K0 = k >> 1, K1 = (k + 1) >> 1;
+    float acc[D];
+#pragma unroll
+    for (int p = 0; p < D; ++p) acc[p] = 0.0f;
+    for (int I = I0; I <= I1; ++I)
+        for (int J = J0; J <= J1; ++J)
+            for (int K = K0; K <= K1; ++K) {
+                const float w = w1(i, I) * w1(j, J) * w1(k, K);
+                const long c = gc.id(I, J, K);
+#pragma unroll
+                for (int p = 0; p < D; ++p) acc[p] += w * xc[c * D + p];
+            }
+#pragma unroll
+    for (int p = 0; p < D; ++p)
+        if (mask == nullptr || mask[(long)n * D + p]) xf[(long)n * D + p] += acc[p];
+}
+
+// --------------------------------------------------------- Galerkin products
+// A_c[I][s] = sum_{i in supp(I)} sum_{j ~ i} sum_{J in supp(j), J - I = s}
+//             w_iI w_jJ mask_i A_ij mask_j.
+// 32 lanes per coarse node: lane t < 27 owns the fine vertex 2I + offset(t);
+// contributions are summed in an LDS copy of the coarse row (FP32 atomics).
+template <int D, class VT, bool CRS>
+__global__ __launch_bounds__(kT) void k_rap(Grid gf, Grid gc, const int* __restrict__ rowmap,
+                                            const int* __restrict__ colidx, const VT* __restrict__ vals_crs,
+                                            const float* __restrict__ A_st, const unsigned char* __restrict__ mask,
+                                            float* __restrict__ Ac) {
+    constexpr int G = 32, ROWS = kT / G, BS = D * D;
+    __shared__ float acc[ROWS][27 * BS];
+    const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+    const int cn = blockIdx.x * ROWS + grp;
+    const bool active = cn < gc.nodes();
+    for (int t = lane; t < 27 * BS; t += G) acc[grp][t] = 0.0f;
+    __syncthreads();
+    if (active && lane < 27) {
+        int I, J, K;
+        gc.ijk(cn, I, J, K);
+        const int ox = lane / 9 - 1, oy = (lane / 3) % 3 - 1, oz = lane % 3 - 1;
+        const int i = 2 * I + ox, j = 2 * J + oy, k = 2 * K + oz;
+        if (gf.in(i, j, k)) {
+            const float wi = w1(i, I) * w1(j, J) * w1(k, K);
+            const long fi = gf.id(i, j, k);
+            float mi[D];
+#pragma unroll
+            for (int p = 0; p < D; ++p) mi[p] = (mask == nullptr || mask[fi * D + p]) ? wi : 0.0f;
+            // fine neighbours of fi with their blocks
+            auto visit = [&](int jj, int jx, int jy, int jz, const float* blk) {
+                float mj[D];
+#pragma unroll
+                for (int q = 0; q < D; ++q) mj[q] = (mask == nullptr || mask[(long)jj * D + q]) ? 1.0f : 0.0f;
+                for (int X = jx >> 1; X <= (jx + 1) >> 1; ++X)
+                    for (int Y = jy >> 1; Y <= (jy + 1) >> 1; ++Y)
+                        for (int Z = jz >> 1; Z <= (jz + 1) >> 1; ++Z) {
+                            const float wj = w1(jx, X) * w1(jy, Y) * w1(jz, Z);
+                            const int s = sidx(X - I, Y - J, Z - K);
+                            float* dst = &acc[grp][s * BS];
+#pragma unroll
+                            for (int p = 0; p < D; ++p)
+#pragma unroll
+                                for (int q = 0; q < D; ++q) {
+                                    const float v = mi[p] * blk[p * D + q] * mj[q] * wj;
+                                    if (v != 0.0f) atomicAdd(&dst[p * D + q], v);
+                                }
+                        }
+            };
+            if constexpr (CRS) {
+                for (int e = rowmap[fi]; e < rowmap[fi + 1]; ++e) {
+                    const int jj = colidx[e];
+                    int jx, jy, jz;
+                    gf.ijk(jj, jx, jy, jz);
+                    float blk[BS];
+#pragma unroll
+                    for (int q = 0; q < BS; ++q) blk[q] = (float)vals_crs[(long)e * BS + q];
+                    visit(jj, jx, jy, jz, blk);
+                }
+            } else {
+                for (int dx = -1; dx <= 1; ++dx)
+                    for (int dy = -1; dy <= 1; ++dy)
+                        for (int dz = -1; dz <= 1; ++dz) {
+                            if (!gf.in(i + dx, j + dy, k + dz)) continue;
+                            const float* blk = A_st + (fi * 27 + sidx(dx, dy, dz)) * BS;
+                            visit(gf.id(i + dx, j + dy, k + dz), i + dx, j + dy, k + dz, blk);
+                        }
+            }
+        }
+    }
+    __syncthreads();
+    if (active)
+        for (int t = lane; t < 27 * BS; t += G) Ac[(long)cn * 27 * BS + t] = acc[grp][t];
+}
+
+// ------------------------------------------------------------ coarsest level
+template <int D>
+__global__ void k_dense_from_stencil(Grid g, const float* __restrict__ A, double* __restrict__ dense) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nd = g.nodes() * D;
+    if (n >= g.nodes()) return;
+    int i, j, k;
+    g.ijk(n, i, j, k);
+    for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dz = -1; dz <= 1; ++dz) {
+                if (!g.in(i + dx, j + dy, k + dz)) continue;
+                const int c = g.id(i + dx, j + dy, k + dz);
+                const float* v = A + ((long)n * 27 + sidx(dx, dy, dz)) * D * D;
+                for (int p = 0; p < D; ++p)
+                    for (int q = 0; q < D; ++q) dense[(long)(n * D + p) * nd + c * D + q] = v[p * D + q];
+            }
+}
+
+// In-place Gauss-Jordan inverse of an SPD matrix (no pivoting), one workgroup.
+__global__ __launch_bounds__(1024) void k_gauss_jordan(int n, double* __restrict__ a, double* __restrict__ inv) {
+    for (long t = threadIdx.x; t < (long)n * n; t += blockDim.x) inv[t] = (t / n == t % n) ? 1.0 : 0.0;
+    __syncthreads();
+    __shared__ double piv;
+    for (int k = 0; k < n; ++k) {
+        if (threadIdx.x == 0) piv = a[(long)k * n + k];
+        __syncthreads();
+        const double ip = piv != 0.0 ? 1.0 / piv : 0.0;
+        for (int c = threadIdx.x; c < n; c += blockDim.x) {
+            a[(long)k * n + c] *= ip;
+            inv[(long)k * n + c] *= ip;
+        }
+        __syncthreads();
+        for (long t = threadIdx.x; t < (long)n * n; t += blockDim.x) {
+            const int r = (int)(t / n), c = (int)(t % n);
+            if (r == k) continue;
+            const double f = a[(long)r * n + k];
+            if (c != k) a[t] -= f * a[(long)k * n + c];
+            inv[t] -= f * inv[(long)k * n + c];
+        }
+        __syncthreads();
+        for (int r = threadIdx.x; r < n; r += blockDim.x)
+            if (r != k) a[(long)r * n + k] = 0.0;
+        __syncthreads();
+    }
+}
+
+__global__ void k_dense_apply(int n, const double* __restrict__ inv, const float* __restrict__ b,
+                              float* __restrict__ x) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    double s = 0.0;
+    for (int c = 0; c < n; ++c) s += inv[(long)r * n + c] * (double)b[c];
+    x[r] = (float)s;
+}
+
+// ----------------------------------------------------------------- helpers
+__global__ void k_mask_init(long n, unsigned char* m) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) m[i] = 1;
+}
+__global__ void k_mask_bc(int nbc, const int* __restrict__ dofs, unsigned char* m) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nbc) m[dofs[i]] = 0;
+}
+__global__ void k_to_f32(long n, const double* __restrict__ a, float* __restrict__ b) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+__global__ void k_to_f64(long n, const float* __restrict__ a, double* __restrict__ b) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (double)a[i];
+}
+
+inline int nblk(long n, int t = kT) { return (int)((n + t - 1) / t); }
+void chk() { PA_HIP(hipGetLastError()); }
+
+}  // namespace
+}  // namespace pa
+
+struct pa_multigrid {
+    const pa_mesh* mesh = nullptr;
+    int dpn = 3;
+    pa_multigrid_options opt{};
+    std::vector<pa::Grid> grids;  // grids[0] is the finest
+    pa::DevBuf<int> d_diag;
+    pa::DevBuf<unsigned char> mask;
+    std::vector<pa::DevBuf<float>*> A, dinv, b, x, t;  // per level (A[0] unused: CRS)
+    pa::DevBuf<double> dense, dense_inv;
+    const void* fine_vals = nullptr;
+    int fine_prec = PA_FP32;
+    bool ready = false;
+    ~pa_multigrid() {
+        for (auto* v : {&A, &dinv, &b, &x, &t})
+            for (auto* p : *v) delete p;
+    }
+};
+
+namespace pa {
+namespace {
+
+template <int D>
+void vcycle(pa_multigrid* mg, int l, hipStream_t st) {
+    const int L = (int)mg->grids.size();
+    const Grid g = mg->grids[l];
+    const long n = (long)g.nodes() * D;
+    float* b = mg->b[l]->p;
+    float* x = mg->x[l]->p;
+    float* t = mg->t[l]->p;
+    const float w = (float)mg->opt.omega;
+    const pa_mesh* m = mg->mesh;
+    if (l == L - 1) {
+        k_dense_apply<<<nblk(n), kT, 0, st>>>((int)n, mg->dense_inv.p, b, x);
+        chk();
+        return;
+    }
+    auto sweep = [&](const float* xin, float* xout, float om, int zero) {
+        if (l == 0) {
+            if (mg->fine_prec == PA_FP64)
+                k_crs_resid_jac<D, double><<<nblk(g.nodes(), kT / 16), kT, 0, st>>>(
+                    g.nodes(), m->d_rowmap.p, m->d_colidx.p, (const double*)mg->fine_vals, mg->dinv[0]->p,
+                    mg->mask.p, b, xin, xout, om, zero);
+            else
+                k_crs_resid_jac<D, float><<<nblk(g.nodes(), kT / 16), kT, 0, st>>>(
+                    g.nodes(), m->d_rowmap.p, m->d_colidx.p, (const float*)mg->fine_vals, mg->dinv[0]->p,
+                    mg->mask.p, b, xin, xout, om, zero);
+        } else {
+            k_st_resid_jac<D><<<nblk(g.nodes()), kT, 0, st>>>(g, mg->A[l]->p, mg->dinv[l]->p, b, xin, xout, om, zero);
+        }
+        chk();
+    };
+    // pre-smoothing (ping-pong x <-> t; the first sweep starts from zero)
+    const int pre = std::max(1, mg->opt.pre_sweeps);
+    sweep(x, x, w, 1);
+    for (int s = 1; s < pre; ++s) {
+        sweep(x, t, w, 0);
+        std::swap(mg->x[l], mg->t[l]);
+        x = mg->x[l]->p;
+        t = mg->t[l]->p;
+    }
+    // residual, restriction, coarse correction
+    sweep(x, t, 0.0f, 0);
+    const Grid gc = mg->grids[l + 1];
+    k_restrict<D><<<nblk(gc.nodes()), kT, 0, st>>>(g, gc, t, l == 0 ? mg->mask.p : nullptr, mg->b[l + 1]->p);
+    chk();
+    vcycle<D>(mg, l + 1, st);
+    k_prolong_add<D><<<nblk(g.nodes()), kT, 0, st>>>(g, gc, mg->x[l + 1]->p, l == 0 ? mg->mask.p : nullptr, x);
+    chk();
+    // post-smoothing
+    for (int s = 0; s < std::max(1, mg->opt.post_sweeps); ++s) {
+        sweep(x, t, w, 0);
+        std::swap(mg->x[l], mg->t[l]);
+        x = mg->x[l]->p;
+        t = mg->t[l]->p;
+    }
+}
+
+template <int D, class VT>
+void setup_impl(pa_multigrid* mg, const VT* vals, hipStream_t st) {
+    const pa_mesh* m = mg->mesh;
+    const int L = (int)mg->grids.size();
+    const Grid g0 = mg->grids[0];
+    k_diag_inv_crs<D, VT><<<nblk(g0.nodes()), kT, 0, st>>>(g0.nodes(), mg->d_diag.p, vals, mg->dinv[0]->p);
+    chk();
+    for (int l = 1; l < L; ++l) {
+        const Grid gf = mg->grids[l - 1], gc = mg->grids[l];
+        const int nb = nblk(gc.nodes(), kT / 32);
+        if (l == 1)
+            k_rap<D, VT, true><<<nb, kT, 0, st>>>(gf, gc, m->d_rowmap.p, m->d_colidx.p, vals, nullptr, mg->mask.p,
+                                                   mg->A[1]->p);
+        else
+            k_rap<D, float, false><<<nb, kT, 0, st>>>(gf, gc, nullptr, nullptr, nullptr, mg->A[l - 1]->p, nullptr,
+                                                       mg->A[l]->p);
+        chk();
+        if (l < L - 1) {
+            k_diag_inv_st<D><<<nblk(gc.nodes()), kT, 0, st>>>(gc.nodes(), mg->A[l]->p, mg->dinv[l]->p);
+            chk();
+        }
+    }
+    const Grid gl = mg->grids[L - 1];
+    const long nd = (long)gl.nodes() * D;
+    PA_HIP(hipMemsetAsync(mg->dense.p, 0, nd * nd * sizeof(double), st));
+    k_dense_from_stencil<D><<<nblk(gl.nodes()), kT, 0, st>>>(gl, mg->A[L - 1]->p, mg->dense.p);
+    chk();
+    k_gauss_jordan<<<1, 1024, 0, st>>>((int)nd, mg->dense.p, mg->dense_inv.p);
+    chk();
+}
+
+}  // namespace
+
+// used by the PCG of pa_solver.hip
+void multigrid_apply(pa_multigrid* mg, const double* r, double* z, hipStream_t st) {
+    PA_REQUIRE(mg->ready, PA_ERR_STATE, "multigrid: setup has not been called");
+    const long n = (long)mg->grids[0].nodes() * mg->dpn;
+    k_to_f32<<<nblk(n), kT, 0, st>>>(n, r, mg->b[0]->p);
+    chk();
+    if (mg->dpn == 3) vcycle<3>(mg, 0, st);
+    else vcycle<1>(mg, 0, st);
+    k_to_f64<<<nblk(n), kT, 0, st>>>(n, mg->x[0]->p, z);
+    chk();
+}
+
+}  // namespace pa
+
+using pa::guarded;
+
+extern "C" {
+
+int pa_multigrid_create(const pa_mesh* mesh, int nx, int ny, int nz, int dpn, const pa_multigrid_options* opt,
+                        pa_multigrid** out) {
+    return guarded([&] {
+        PA_REQUIRE(mesh && opt && out, PA_ERR_ARG, "pa_multigrid_create: null argument");
+        PA_REQUIRE(dpn == 1 || dpn == 3, PA_ERR_ARG, "pa_multigrid_create: dpn must be 1 or 3");
+        pa::Grid g0{nx, ny, nz};
+        PA_REQUIRE(nx > 0 && ny > 0 && nz > 0 && g0.nodes() == mesh->N, PA_ERR_ARG,
+                   "pa_multigrid_create: grid does not match the mesh node count");
+        // the graph must be a vertex-neighbourhood graph of this grid
+        for (int r = 0; r < mesh->N; ++r) {
+            int i, j, k;
+            g0.ijk(r, i, j, k);
+            for (int e = mesh->h_rowmap[r]; e < mesh->h_rowmap[r + 1]; ++e) {
+                int a, b, c;
+                g0.ijk(mesh->h_colidx[e], a, b, c);
+                PA_REQUIRE(std::abs(a - i) <= 1 && std::abs(b - j) <= 1 && std::abs(c - k) <= 1, PA_ERR_ARG,
+                           "pa_multigrid_create: mesh graph is not a structured-grid neighbourhood graph");
+            }
+        }
+        auto* mg = new pa_multigrid();
+        try {
+            mg->mesh = mesh;
+            mg->dpn = dpn;
+            mg->opt = *opt;
+            if (mg->opt.omega <= 0.0) mg->opt.omega = 0.6;
+            mg->grids.push_back(g0);
+            const int maxc = opt->max_coarse_nodes > 0 ? opt->max_coarse_nodes : 2000;
+            while (true) {
+                const pa::Grid& g = mg->grids.back();
+                if (g.nx % 2 || g.ny % 2 || g.nz % 2) break;
+                if (opt->max_levels > 0 && (int)mg->grids.size() >= opt->max_levels) break;
+                mg->grids.push_back(pa::Grid{g.nx / 2, g.ny / 2, g.nz / 2});
+            }
+            PA_REQUIRE(mg->grids.size() >= 2, PA_ERR_ARG, "pa_multigrid_create: the grid cannot be coarsened (odd cell counts)");
+            PA_REQUIRE(mg->grids.back().nodes() <= maxc, PA_ERR_LIMIT,
+                       "pa_multigrid_create: coarsest level too large for the dense solve");
+            std::vector<int> diag(mesh->N);
+            for (int i = 0; i < mesh->N; ++i)
+                for (int e = mesh->h_rowmap[i]; e < mesh->h_rowmap[i + 1]; ++e)
+                    if (mesh->h_colidx[e] == i) diag[i] = e;
+            mg->d_diag.upload(diag.data(), diag.size());
+            const int L = (int)mg->grids.size();
+            const int bs = dpn * dpn;
+            for (int l = 0; l < L; ++l) {
+                const long nn = mg->grids[l].nodes();
+                mg->A.push_back(new pa::DevBuf<float>());
+                mg->dinv.push_back(new pa::DevBuf<float>());
+                mg->b.push_back(new pa::DevBuf<float>());
+                mg->x.push_back(new pa::DevBuf<float>());
+                mg->t.push_back(new pa::DevBuf<float>());
+                if (l > 0) mg->A[l]->alloc(nn * 27 * bs);
+                mg->dinv[l]->alloc(nn * bs);
+                mg->b[l]->alloc(nn * dpn);
+                mg->x[l]->alloc(nn * dpn);
+                mg->t[l]->alloc(nn * dpn);
+            }
+            const long nd = (long)mg->grids.back().nodes() * dpn;
+            mg->dense.alloc(nd * nd);
+            mg->dense_inv.alloc(nd * nd);
+            mg->mask.alloc((long)mesh->N * dpn);
+        } catch (...) {
+            delete mg;
+            throw;
+        }
+        *out = mg;
+    });
+}
+
+int pa_multigrid_destroy(pa_multigrid* mg) {
+    return guarded([&] { delete mg; });
+}
+
+int pa_multigrid_levels(const pa_multigrid* mg, int* levels, int* coarse_dofs) {
+    return guarded([&] {
+        PA_REQUIRE(mg, PA_ERR_ARG, "pa_multigrid_levels: null argument");
+        if (levels) *levels = (int)mg->grids.size();
+        if (coarse_dofs) *coarse_dofs = mg->grids.back().nodes() * mg->dpn;
+    });
+}
+
+int pa_multigrid_setup(pa_multigrid* mg, const void* vals, int precision, int nbc, const int* bc_dofs, void* stream) {
+    return guarded([&] {
+        PA_REQUIRE(mg && vals, PA_ERR_ARG, "pa_multigrid_setup: null argument");
+        PA_REQUIRE(precision == PA_FP64 || precision == PA_FP32, PA_ERR_ARG, "pa_multigrid_setup: unknown precision");
+        const hipStream_t st = (hipStream_t)stream;
+        const long n = (long)mg->mesh->N * mg->dpn;
+        pa::k_mask_init<<<pa::nblk(n), pa::kT, 0, st>>>(n, mg->mask.p);
+        pa::chk();
+        if (nbc > 0) {
+            PA_REQUIRE(bc_dofs, PA_ERR_ARG, "pa_multigrid_setup: null constraint list");
+            pa::k_mask_bc<<<pa::nblk(nbc), pa::kT, 0, st>>>(nbc, bc_dofs, mg->mask.p);
+            pa::chk();
+        }
+        mg->fine_vals = vals;
+        mg->fine_prec = precision;
+        if (mg->dpn == 3) {
+            if (precision == PA_FP64) pa::setup_impl<3, double>(mg, (const double*)vals, st);
+            else pa::setup_impl<3, float>(mg, (const float*)vals, st);
+        } else {
+            if (precision == PA_FP64) pa::setup_impl<1, double>(mg, (const double*)vals, st);
+            else pa::setup_impl<1, float>(mg, (const float*)vals, st);
+        }
+        mg->ready = true;
+    });
+}
+
+int pa_multigrid_level_operator(const pa_multigrid* mg, int level, float* out, int* nx, int* ny, int* nz) {
+    return guarded([&] {
+        PA_REQUIRE(mg, PA_ERR_ARG, "pa_multigrid_level_operator: null argument");
+        PA_REQUIRE(level >= 1 && level < (int)mg->grids.size(), PA_ERR_ARG, "pa_multigrid_level_operator: bad level");
+        const pa::Grid g = mg->grids[level];
+        if (nx) *nx = g.nx;
+        if (ny) *ny = g.ny;
+        if (nz) *nz = g.nz;
+        if (out) {
+            PA_REQUIRE(mg->ready, PA_ERR_STATE, "pa_multigrid_level_operator: setup has not been called");
+            PA_HIP(hipDeviceSynchronize());
+            PA_HIP(hipMemcpy(out, mg->A[level]->p, mg->A[level]->n * sizeof(float), hipMemcpyDeviceToHost));
+        }
+    });
+}
+
+int pa_multigrid_apply(pa_multigrid* mg, const double* r, double* z, void* stream) {
+    return guarded([&] {
+        PA_REQUIRE(mg && r && z, PA_ERR_ARG, "pa_multigrid_apply: null argument");
+        pa::multigrid_apply(mg, r, z, (hipStream_t)stream);
+    });
+}
+
+}  // extern "C"

@@ -26,6 +26,7 @@ struct pa_solver {
     pa::DevBuf<double> part;      // partial sums: 4 x max(nvb, nsb)
     pa::DevBuf<double> scal;      // device scalars
     std::vector<double> h_scal;
+    pa_multigrid* mg = nullptr;   // PA_PRECOND_MULTIGRID (not owned)
 };
 
 struct pa_problem {
@@ -39,6 +40,7 @@ struct pa_problem {
     pa::DevBuf<double> K64;
     pa::DevBuf<float> K32;
     pa::DevBuf<double> rhs, res, dx, zero;
+    pa_multigrid* mg = nullptr;  // owned
 };
 
 namespace pa {
@@ -334,15 +336,31 @@ void pcg_impl(pa_solver* s, const VT* vals, const double* b, double* x, double t
     const int N = m->N;
     const long n = (long)N * D;
     const int nvb = s->nvb;
-    if (s->precond != PA_PRECOND_NONE) {
-        k_prec_setup<D, VT><<<(N + 255) / 256, 256, 0, st>>>(N, s->precond, s->d_diag.p, vals, s->d_minv.p);
+    // multigrid: the fused kernels run unpreconditioned (z = r), then z = M r
+    // by a V-cycle and r.z are recomputed
+    const bool mgp = s->precond == PA_PRECOND_MULTIGRID;
+    PA_REQUIRE(!mgp || s->mg, PA_ERR_STATE, "pcg: multigrid preconditioner selected but not set");
+    const int kprec = mgp ? PA_PRECOND_NONE : s->precond;
+    auto mg_rz = [&]() {
+        multigrid_apply(s->mg, s->r.p, s->zv.p, st);
+        k_dot_part<<<nvb, kB, 0, st>>>(n, s->r.p, s->zv.p, s->part.p);
+        check();
+        k_reduce<<<1, 1024, 0, st>>>(s->part.p, nvb, nvb, 1, S_RZ, 0, 0, 0, s->scal.p);
+        check();
+    };
+    if (kprec != PA_PRECOND_NONE) {
+        k_prec_setup<D, VT><<<(N + 255) / 256, 256, 0, st>>>(N, kprec, s->d_diag.p, vals, s->d_minv.p);
         check();
     }
     spmv_dot<D, VT>(s, vals, x, s->q.p, st);
-    k_pcg_init<D><<<nvb, kB, 0, st>>>(N, s->precond, s->d_minv.p, b, s->q.p, s->r.p, s->zv.p, s->p.p, s->part.p, nvb);
+    k_pcg_init<D><<<nvb, kB, 0, st>>>(N, kprec, s->d_minv.p, b, s->q.p, s->r.p, s->zv.p, s->p.p, s->part.p, nvb);
     check();
     k_reduce<<<1, 1024, 0, st>>>(s->part.p, nvb, nvb, 3, S_RZ, S_RR, S_BB, 0, s->scal.p);
     check();
+    if (mgp) {
+        mg_rz();
+        PA_HIP(hipMemcpyAsync(s->p.p, s->zv.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    }
     const double bb = read_scalar(s, S_BB, st);
     const double bnorm = std::sqrt(bb);
     int it = 0;
@@ -355,11 +373,12 @@ void pcg_impl(pa_solver* s, const VT* vals, const double* b, double* x, double t
             spmv_dot<D, VT>(s, vals, s->p.p, s->q.p, st);
             k_reduce<<<1, 1024, 0, st>>>(s->part.p, s->nsb, s->nsb, 1, S_PQ, 0, 0, 0, s->scal.p);
             check();
-            k_pcg_update<D><<<nvb, kB, 0, st>>>(N, s->precond, s->d_minv.p, s->scal.p, s->p.p, s->q.p, x, s->r.p,
+            k_pcg_update<D><<<nvb, kB, 0, st>>>(N, kprec, s->d_minv.p, s->scal.p, s->p.p, s->q.p, x, s->r.p,
                                                 s->zv.p, s->part.p, nvb);
             check();
             k_reduce<<<1, 1024, 0, st>>>(s->part.p, nvb, nvb, 2, S_RZ, S_RR, 0, 1, s->scal.p);
             check();
+            if (mgp) mg_rz();
             k_pcg_dir<<<nvb, kB, 0, st>>>(n, s->scal.p, s->zv.p, s->p.p);
             check();
         }
@@ -371,10 +390,11 @@ void pcg_impl(pa_solver* s, const VT* vals, const double* b, double* x, double t
             // reliable update: swap the recursive residual for the exact one,
             // keep the search direction
             (*exact)(x, s->r.p);
-            k_pcg_reprec<D><<<nvb, kB, 0, st>>>(N, s->precond, s->d_minv.p, s->r.p, s->zv.p, s->part.p, nvb);
+            k_pcg_reprec<D><<<nvb, kB, 0, st>>>(N, kprec, s->d_minv.p, s->r.p, s->zv.p, s->part.p, nvb);
             check();
             k_reduce<<<1, 1024, 0, st>>>(s->part.p, nvb, nvb, 2, S_RZ, S_RR, 0, 0, s->scal.p);
             check();
+            if (mgp) mg_rz();
             rr = read_scalar(s, S_RR, st);
         }
     }
@@ -422,7 +442,7 @@ int pa_solver_create(const pa_mesh* mesh, int dpn, int precond, pa_solver** out)
     return guarded([&] {
         PA_REQUIRE(mesh && out, PA_ERR_ARG, "pa_solver_create: null argument");
         PA_REQUIRE(dpn == 1 || dpn == 3, PA_ERR_ARG, "pa_solver_create: dpn must be 1 or 3");
-        PA_REQUIRE(precond >= PA_PRECOND_NONE && precond <= PA_PRECOND_BLOCK_JACOBI, PA_ERR_ARG,
+        PA_REQUIRE(precond >= PA_PRECOND_NONE && precond <= PA_PRECOND_MULTIGRID, PA_ERR_ARG,
                    "pa_solver_create: unknown preconditioner");
         auto* s = new pa_solver();
         try {
@@ -459,6 +479,14 @@ int pa_solver_create(const pa_mesh* mesh, int dpn, int precond, pa_solver** out)
 
 int pa_solver_destroy(pa_solver* s) {
     return guarded([&] { delete s; });
+}
+
+int pa_solver_set_multigrid(pa_solver* s, pa_multigrid* mg) {
+    return guarded([&] {
+        PA_REQUIRE(s && mg, PA_ERR_ARG, "pa_solver_set_multigrid: null argument");
+        s->mg = mg;
+        s->precond = PA_PRECOND_MULTIGRID;
+    });
 }
 
 int pa_solver_pcg(pa_solver* s, const void* vals, int precision, const double* b, double* x, double tol,
@@ -504,6 +532,7 @@ int pa_problem_destroy(pa_problem* P) {
     return guarded([&] {
         if (!P) return;
         if (P->solver) pa_solver_destroy(P->solver);
+        if (P->mg) pa_multigrid_destroy(P->mg);
         delete P;
     });
 }
@@ -511,13 +540,32 @@ int pa_problem_destroy(pa_problem* P) {
 int pa_problem_set_options(pa_problem* P, const pa_problem_options* opt) {
     return guarded([&] {
         PA_REQUIRE(P && opt, PA_ERR_ARG, "pa_problem_set_options: null argument");
+        PA_REQUIRE(opt->precond != PA_PRECOND_MULTIGRID || P->mg, PA_ERR_ARG,
+                   "pa_problem_set_options: multigrid needs pa_problem_set_multigrid");
         if (opt->precond != P->opt.precond) {
             pa_solver* s = nullptr;
             pa::check_status(pa_solver_create(P->vf->mesh, P->vf->dpn, opt->precond, &s));
+            if (opt->precond == PA_PRECOND_MULTIGRID) s->mg = P->mg;
             pa_solver_destroy(P->solver);
             P->solver = s;
         }
         P->opt = *opt;
+    });
+}
+
+int pa_problem_set_multigrid(pa_problem* P, int nx, int ny, int nz, const pa_multigrid_options* mopt) {
+    return guarded([&] {
+        PA_REQUIRE(P && mopt, PA_ERR_ARG, "pa_problem_set_multigrid: null argument");
+        pa_multigrid* mg = nullptr;
+        pa::check_status(pa_multigrid_create(P->vf->mesh, nx, ny, nz, P->vf->dpn, mopt, &mg));
+        if (P->mg) pa_multigrid_destroy(P->mg);
+        P->mg = mg;
+        pa_solver* s = nullptr;
+        pa::check_status(pa_solver_create(P->vf->mesh, P->vf->dpn, PA_PRECOND_MULTIGRID, &s));
+        s->mg = mg;
+        pa_solver_destroy(P->solver);
+        P->solver = s;
+        P->opt.precond = PA_PRECOND_MULTIGRID;
     });
 }
 
@@ -548,6 +596,10 @@ int pa_problem_solution(pa_problem* P, const double* z, double* u, pa_solve_stat
         if (P->nbc)
             pa::check_status(pa_apply_block_constraints(m, dpn, K, P->opt.mode, P->rhs.p, P->nbc, P->bc_dofs.p,
                                                         P->bc_vals.p, 1.0, stream));
+        if (P->opt.precond == PA_PRECOND_MULTIGRID) {
+            PA_REQUIRE(P->mg, PA_ERR_STATE, "pa_problem_solution: multigrid selected but not set");
+            pa::check_status(pa_multigrid_setup(P->mg, K, P->opt.mode, P->nbc, P->bc_dofs.p, stream));
+        }
         const double tol = P->opt.tol;
         if (!mixed) {
             pa::pcg(P->solver, K, PA_FP64, P->rhs.p, u, tol, P->opt.max_iter, P->opt.check_every, stats, st);
