@@ -365,6 +365,7 @@ void pcg_impl(pa_solver* s, const VT* vals, const double* b, double* x, double t
     const double bnorm = std::sqrt(bb);
     int it = 0;
     double rr = read_scalar(s, S_RR, st);
+    stats->sweeps = 1;
     const double target = tol * bnorm;
     check_every = check_every > 0 ? check_every : 10;
     while (it < max_iter && std::sqrt(rr) > target && bnorm > 0.0) {
@@ -387,15 +388,19 @@ void pcg_impl(pa_solver* s, const VT* vals, const double* b, double* x, double t
         if (!std::isfinite(rr)) break;
         const bool converged = std::sqrt(rr) <= target;
         if (exact && (converged || (replace_every > 0 && it % replace_every == 0))) {
-            // reliable update: swap the recursive residual for the exact one,
-            // keep the search direction
+            // residual replacement: swap the recursive residual for the exact
+            // one and restart the directions from it (p = M r), since the
+            // recursive residual of the FP32 operator may have drifted far
+            // below the exact one
             (*exact)(x, s->r.p);
             k_pcg_reprec<D><<<nvb, kB, 0, st>>>(N, kprec, s->d_minv.p, s->r.p, s->zv.p, s->part.p, nvb);
             check();
             k_reduce<<<1, 1024, 0, st>>>(s->part.p, nvb, nvb, 2, S_RZ, S_RR, 0, 0, s->scal.p);
             check();
             if (mgp) mg_rz();
+            PA_HIP(hipMemcpyAsync(s->p.p, s->zv.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
             rr = read_scalar(s, S_RR, st);
+            ++stats->sweeps;
         }
     }
     double rn;
@@ -410,7 +415,6 @@ void pcg_impl(pa_solver* s, const VT* vals, const double* b, double* x, double t
         rn = std::sqrt(norm2(s, s->q.p, st));
     }
     stats->iterations = it;
-    stats->sweeps = 1;
     stats->rel_residual = bnorm > 0.0 ? rn / bnorm : 0.0;
     stats->converged = stats->rel_residual <= tol ? 1 : 0;
 }
