@@ -68,14 +68,59 @@ def cpu_baseline(nx_slab=32):
                       f"{dims[0]}x{dims[1]}x{dims[2]} slab of the same grid ({C} cells), median of 3"}
 
 
+def compliance_gradient(pa, mesh, vf, z, dims):
+    """Second metric: time from the density on the device to the compliance
+    and its gradient on the device at ||r|| <= 1e-8 ||b|| (FP64 residual),
+    for the plain FP64 path (FP64 assembly, block-Jacobi PCG from zero) and
+    the multigrid path (FP64 assembly and outer PCG, FP32 multigrid V-cycles
+    as the preconditioner).  Errors are against the plain FP64 result."""
+    sf = pa.PhysicsScalarFunction(mesh, "Internal Elastic Energy", youngs=1.0, poisson=0.3, penalty_exponent=3.0,
+                                  penalty_min=1e-9)
+    dofs = (3 * mesh.node_set("x-")[:, None] + np.arange(3)).ravel()
+    res = {}
+    ref = None
+    for name, reps in (("plain_fp64_block_jacobi", 1), ("fp64_pcg_fp32_multigrid", 3)):
+        P = pa.Problem(vf, sf, dofs, mode="fp64", precond=pa.PRECOND_BLOCK_JACOBI, tol=1e-8, max_iter=100000,
+                       check_every=50)
+        if name != "plain_fp64_block_jacobi":
+            P.set_multigrid(*dims, pre_sweeps=2, post_sweeps=2, omega=0.6)
+            P.set_options(check_every=5)
+        u = P.solution(z)  # warm-up (allocations, code objects)
+        times = []
+        for _ in range(reps):
+            pa.synchronize()
+            t0 = time.perf_counter()
+            u = P.solution(z)
+            Fd = P.criterion_value(z, u, out=pa.DeviceArray(1))
+            g = P.criterion_gradient(z, u)
+            pa.synchronize()
+            times.append(time.perf_counter() - t0)
+        F = float(Fd.numpy()[0])
+        gh = g.numpy()
+        entry = {"s": float(np.median(times)), "iterations": P.last_stats["iterations"],
+                 "rel_residual": P.last_stats["rel_residual"], "compliance": F}
+        if ref is None:
+            ref = (F, gh)
+        else:
+            entry["compliance_rel_err_vs_plain_fp64"] = abs(F - ref[0]) / abs(ref[0])
+            entry["gradient_rel_err_vs_plain_fp64"] = float(np.linalg.norm(gh - ref[1]) / np.linalg.norm(ref[1]))
+        res[name] = entry
+        log(f"bench: compliance+gradient {name}: {entry}")
+    res["speedup_vs_plain_fp64"] = res["plain_fp64_block_jacobi"]["s"] / res["fp64_pcg_fp32_multigrid"]["s"]
+    res["unit"] = "s per compliance+gradient @1e-8"
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--mode", default="bench", choices=["bench", "copy", "profile"],
-                    help="copy: run only the calibration copy kernel; profile: kernels only, no CPU leg")
+    ap.add_argument("--no-solve", action="store_true", help="skip the compliance + gradient leg")
+    ap.add_argument("--mode", default="bench", choices=["bench", "copy", "profile", "solve"],
+                    help="copy: run only the calibration copy kernel; profile: assembly kernels only; "
+                         "solve: one multigrid compliance + gradient solve only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -112,6 +157,18 @@ def main():
     K64 = pa.DeviceArray(9 * nnzb, np.float64)
     K32 = pa.DeviceArray(9 * nnzb, np.float32)
     log(f"bench: mesh + graph built in {setup_s:.2f}s (C={C}, N={N}, nnzb={nnzb})")
+    if args.mode == "solve":
+        sf = pa.PhysicsScalarFunction(mesh, "Internal Elastic Energy", youngs=1.0, poisson=0.3,
+                                      penalty_exponent=3.0, penalty_min=1e-9)
+        dofs = (3 * mesh.node_set("x-")[:, None] + np.arange(3)).ravel()
+        P = pa.Problem(vf, sf, dofs, mode="fp64", tol=1e-8, max_iter=2000, check_every=5)
+        P.set_multigrid(*dims, pre_sweeps=2, post_sweeps=2, omega=0.6)
+        for _ in range(max(1, args.steps)):
+            u = P.solution(z)
+            P.criterion_gradient(z, u)
+        pa.synchronize()
+        log(f"bench: multigrid solve {P.last_stats}")
+        return
 
     def step():
         vf.value(u, z, out=R, stream=stream)
@@ -184,6 +241,8 @@ def main():
         "fused_cells_per_s": {"fp64": C / (t_fused * 1e-3), "fp32": C / (t_fused32 * 1e-3)},
         "setup_s": setup_s,
     }
+    if not args.no_solve:
+        out["compliance_gradient"] = compliance_gradient(pa, mesh, vf, z, dims)
     if not args.no_cpu:
         log("bench: CPU baseline (oracle, OpenMP)")
         out["cpu_baseline"] = cpu_baseline()
