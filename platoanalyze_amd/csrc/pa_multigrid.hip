@@ -373,6 +373,10 @@ __global__ void k_to_f32(long n, const double* __restrict__ a, float* __restrict
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = (float)a[i];
 }
+__global__ void k_vals_to_f32(long n, const double* __restrict__ a, float* __restrict__ b) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
+}
 __global__ void k_to_f64(long n, const float* __restrict__ a, double* __restrict__ b) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = (double)a[i];
@@ -393,6 +397,7 @@ struct pa_multigrid {
     pa::DevBuf<unsigned char> mask;
     std::vector<pa::DevBuf<float>*> A, dinv, b, x, t;  // per level (A[0] unused: CRS)
     pa::DevBuf<double> dense, dense_inv;
+    pa::DevBuf<float> fine32;  // FP32 copy of an FP64 fine matrix
     const void* fine_vals = nullptr;
     int fine_prec = PA_FP32;
     bool ready = false;
@@ -607,13 +612,18 @@ int pa_multigrid_setup(pa_multigrid* mg, const void* vals, int precision, int nb
         }
         mg->fine_vals = vals;
         mg->fine_prec = precision;
-        if (mg->dpn == 3) {
-            if (precision == PA_FP64) pa::setup_impl<3, double>(mg, (const double*)vals, st);
-            else pa::setup_impl<3, float>(mg, (const float*)vals, st);
-        } else {
-            if (precision == PA_FP64) pa::setup_impl<1, double>(mg, (const double*)vals, st);
-            else pa::setup_impl<1, float>(mg, (const float*)vals, st);
+        if (precision == PA_FP64) {
+            // the smoother reads an FP32 copy of the fine matrix (half the bytes per sweep)
+            const long nv = (long)mg->mesh->nnzb * mg->dpn * mg->dpn;
+            if (mg->fine32.n != (size_t)nv) mg->fine32.alloc(nv);
+            pa::k_vals_to_f32<<<pa::nblk(nv), pa::kT, 0, st>>>(nv, (const double*)vals, mg->fine32.p);
+            pa::chk();
+            mg->fine_vals = mg->fine32.p;
+            mg->fine_prec = PA_FP32;
         }
+        const float* v32 = (const float*)mg->fine_vals;
+        if (mg->dpn == 3) pa::setup_impl<3, float>(mg, v32, st);
+        else pa::setup_impl<1, float>(mg, v32, st);
         mg->ready = true;
     });
 }
