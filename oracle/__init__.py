@@ -63,6 +63,32 @@ class Problem(ctypes.Structure):
     ]
 
 
+class J2Problem(ctypes.Structure):
+    _fields_ = [
+        ("num_nodes", ctypes.c_int),
+        ("num_cells", ctypes.c_int),
+        ("coords", ctypes.c_void_p),
+        ("conn", ctypes.c_void_p),
+        ("youngs", ctypes.c_double),
+        ("poisson", ctypes.c_double),
+        ("hardening_isotropic", ctypes.c_double),
+        ("hardening_kinematic", ctypes.c_double),
+        ("initial_yield_stress", ctypes.c_double),
+        ("elastic_penalty_exponent", ctypes.c_double),
+        ("elastic_min_ersatz", ctypes.c_double),
+        ("plastic_penalty_exponent", ctypes.c_double),
+        ("plastic_min_ersatz", ctypes.c_double),
+        ("global_penalty_exponent", ctypes.c_double),
+        ("global_min_ersatz", ctypes.c_double),
+        ("pressure_scaling", ctypes.c_double),
+        ("projection_penalty", ctypes.c_int),
+        ("projection_penalty_exponent", ctypes.c_double),
+        ("projection_min_ersatz", ctypes.c_double),
+        ("work_penalty_exponent", ctypes.c_double),
+        ("work_min_ersatz", ctypes.c_double),
+    ]
+
+
 def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
@@ -90,6 +116,20 @@ def _declare(L):
         f.restype = d
     L.orc_apply_block_constraints.argtypes = [i, i, vp, vp, vp, vp, i, vp, vp, d]
     L.orc_block_spmv.argtypes = [i, i, i, vp, vp, vp, vp, vp]
+    J = ctypes.POINTER(J2Problem)
+    L.orc_j2_update_local_state.argtypes = [J, vp, vp, vp, vp]
+    L.orc_j2_local_value.argtypes = [J, vp, vp, vp, vp, vp]
+    L.orc_j2_local_gradient.argtypes = [J, i, vp, vp, vp, vp, vp]
+    L.orc_j2_global_value_ws.argtypes = [J, vp, vp, vp, vp, vp]
+    L.orc_j2_global_gradient_ws.argtypes = [J, i, vp, vp, vp, vp, vp]
+    L.orc_batched_inverse.argtypes = [i, i, vp, vp]
+    L.orc_j2_condensed_system.argtypes = [J, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.orc_j2_global_value.argtypes = [J, vp, vp, vp, vp, vp]
+    L.orc_j2_projection.argtypes = [J, vp, vp, vp]
+    L.orc_j2_plastic_work.argtypes = [J, vp, vp, vp, vp, vp]
+    L.orc_j2_plastic_work.restype = d
+    L.orc_j2_plastic_work_partials.argtypes = [J, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.orc_j2_cell_geometry.argtypes = [J, vp, vp]
 
 
 def set_threads(n):
@@ -236,3 +276,308 @@ def to_scipy(dpn_r, dpn_c, rowmap, colidx, values):
     bj = np.repeat(colidx[:, None] * dpn_c, dpn_r * dpn_c, axis=1) + np.tile(np.arange(dpn_c), dpn_r)[None, :]
     ncols = (colidx.max() + 1) * dpn_c if nnzb else 0
     return sp.csr_matrix((values.reshape(nnzb, -1).ravel(), (bi.ravel(), bj.ravel())), shape=(N * dpn_r, N * dpn_c))
+
+
+# ---------------------------------------------------------------------------
+# J2 plasticity (stabilized mixed u-p, 4 dofs per node, 14 local states/cell)
+# ---------------------------------------------------------------------------
+def gather_nodal(conn, field, width):
+    """Cell workset C x (4*width) of a nodal field (node*width + k)."""
+    f = np.asarray(field, np.float64).reshape(-1, width)
+    return np.ascontiguousarray(f[conn.reshape(-1, 4)].reshape(-1, 4 * width))
+
+
+def batched_inverse(A):
+    A = np.ascontiguousarray(A, np.float64)
+    n = A.shape[-1]
+    out = np.zeros_like(A)
+    lib().orc_batched_inverse(A.size // (n * n), n, _p(A), _p(out))
+    return out
+
+
+class J2Model:
+    """Holds a J2Problem and the arrays it points to.
+
+    Parameter names follow the reference's input lists:
+    J2 Plasticity (hardening moduli, initial yield stress, elastic/plastic
+    properties penalty exponent and minimum ersatz), the global residual's
+    Elliptic / Penalty Function (exponent, minimum value), Pressure Scaling,
+    the projection's penalty (None = NoPenalty) and the plastic work
+    criterion's penalty."""
+
+    def __init__(self, coords, conn, *, youngs, poisson, hardening_isotropic, hardening_kinematic,
+                 initial_yield_stress, elastic_penalty_exponent=1.0, elastic_min_ersatz=1e-9,
+                 plastic_penalty_exponent=0.5, plastic_min_ersatz=1e-4, global_penalty_exponent=1.0,
+                 global_min_ersatz=1e-9, pressure_scaling=1.0, projection_penalty=None, work_penalty=(3.0, 1e-9)):
+        self.coords = np.ascontiguousarray(coords, np.float64)
+        self.conn = np.ascontiguousarray(conn, np.int32)
+        self.N = self.coords.size // 3
+        self.C = self.conn.size // 4
+        P = J2Problem()
+        P.num_nodes, P.num_cells = self.N, self.C
+        P.coords, P.conn = _p(self.coords).value, _p(self.conn).value
+        P.youngs, P.poisson = youngs, poisson
+        P.hardening_isotropic, P.hardening_kinematic = hardening_isotropic, hardening_kinematic
+        P.initial_yield_stress = initial_yield_stress
+        P.elastic_penalty_exponent, P.elastic_min_ersatz = elastic_penalty_exponent, elastic_min_ersatz
+        P.plastic_penalty_exponent, P.plastic_min_ersatz = plastic_penalty_exponent, plastic_min_ersatz
+        P.global_penalty_exponent, P.global_min_ersatz = global_penalty_exponent, global_min_ersatz
+        P.pressure_scaling = pressure_scaling
+        if projection_penalty is None:
+            P.projection_penalty = 0
+        else:
+            P.projection_penalty = 1
+            P.projection_penalty_exponent, P.projection_min_ersatz = projection_penalty
+        P.work_penalty_exponent, P.work_min_ersatz = work_penalty
+        self.P = P
+        self.rowmap, self.colidx = graph(self.N, self.conn)
+        self.nnzb = self.colidx.size
+
+    @staticmethod
+    def _a(x):
+        return _p(np.ascontiguousarray(x, np.float64))
+
+    def update_local_state(self, u, cp, z):
+        c = np.zeros(self.C * 14)
+        u, cp, z = [np.ascontiguousarray(x, np.float64) for x in (u, cp, z)]
+        lib().orc_j2_update_local_state(ctypes.byref(self.P), _p(u), _p(cp), _p(z), _p(c))
+        return c
+
+    def local_value(self, u, c, cp, z):
+        H = np.zeros(self.C * 14)
+        args = [np.ascontiguousarray(x, np.float64) for x in (u, c, cp, z)]
+        lib().orc_j2_local_value(ctypes.byref(self.P), *[_p(a) for a in args], _p(H))
+        return H
+
+    def local_gradient(self, which, u, c, cp, z):
+        """which: 'c', 'u', 'z', 'cp' -> C x 14 x W."""
+        k = {"c": 0, "u": 1, "z": 2, "cp": 3}[which]
+        W = {0: 14, 1: 16, 2: 4, 3: 14}[k]
+        out = np.zeros(self.C * 14 * W)
+        args = [np.ascontiguousarray(x, np.float64) for x in (u, c, cp, z)]
+        lib().orc_j2_local_gradient(ctypes.byref(self.P), k, *[_p(a) for a in args], _p(out))
+        return out.reshape(self.C, 14, W)
+
+    def global_value_ws(self, u, c, pi_ws, z):
+        out = np.zeros(self.C * 16)
+        args = [np.ascontiguousarray(x, np.float64) for x in (u, c, pi_ws, z)]
+        lib().orc_j2_global_value_ws(ctypes.byref(self.P), *[_p(a) for a in args], _p(out))
+        return out.reshape(self.C, 16)
+
+    def global_gradient_ws(self, which, u, c, pi_ws, z):
+        """which: 'u', 'c', 'z', 'pi' -> C x 16 x W."""
+        k = {"u": 0, "c": 1, "z": 2, "pi": 3}[which]
+        W = {0: 16, 1: 14, 2: 4, 3: 12}[k]
+        out = np.zeros(self.C * 16 * W)
+        args = [np.ascontiguousarray(x, np.float64) for x in (u, c, pi_ws, z)]
+        lib().orc_j2_global_gradient_ws(ctypes.byref(self.P), k, *[_p(a) for a in args], _p(out))
+        return out.reshape(self.C, 16, W)
+
+    def global_value(self, u, c, pi, z):
+        R = np.zeros(self.N * 4)
+        args = [np.ascontiguousarray(x, np.float64) for x in (u, c, pi, z)]
+        lib().orc_j2_global_value(ctypes.byref(self.P), *[_p(a) for a in args], _p(R))
+        return R
+
+    def condensed_system(self, u, c, cp, pi, z):
+        """(block-CRS values nnzb x 16, residual 4N) of one Newton linearization."""
+        vals = np.zeros(self.nnzb * 16)
+        R = np.zeros(self.N * 4)
+        args = [np.ascontiguousarray(x, np.float64) for x in (u, c, cp, pi, z)]
+        lib().orc_j2_condensed_system(ctypes.byref(self.P), *[_p(a) for a in args], _p(self.rowmap),
+                                      _p(self.colidx), _p(vals), _p(R))
+        return vals, R
+
+    def projection(self, u, z):
+        pi = np.zeros(self.N * 3)
+        u, z = np.ascontiguousarray(u, np.float64), np.ascontiguousarray(z, np.float64)
+        lib().orc_j2_projection(ctypes.byref(self.P), _p(u), _p(z), _p(pi))
+        return pi
+
+    def plastic_work(self, u, up, c, cp, z):
+        args = [np.ascontiguousarray(x, np.float64) for x in (u, up, c, cp, z)]
+        return lib().orc_j2_plastic_work(ctypes.byref(self.P), *[_p(a) for a in args])
+
+    def plastic_work_partials(self, u, up, c, cp, z):
+        gu, gup = np.zeros(self.N * 4), np.zeros(self.N * 4)
+        gc, gcp = np.zeros(self.C * 14), np.zeros(self.C * 14)
+        gz = np.zeros(self.N)
+        args = [np.ascontiguousarray(x, np.float64) for x in (u, up, c, cp, z)]
+        lib().orc_j2_plastic_work_partials(ctypes.byref(self.P), *[_p(a) for a in args], _p(gu), _p(gup), _p(gc),
+                                           _p(gcp), _p(gz))
+        return gu, gup, gc, gcp, gz
+
+    def solve(self, z, bc_dofs, bc_values, num_steps=1, max_newton=10, stop_tol=1e-8, current_norm_tol=1e-8):
+        """Pseudo-time loop with the condensed Newton solve.
+
+        src/PlasticityProblem.hpp:784-868 (steps, load factor t_k = (k+1)/K,
+        Dirichlet increment on the first Newton iteration, projection update
+        after each step) and src/NewtonRaphsonSolver.hpp:555-610 (elastic
+        trial update, assemble -R_t and K_t, constraints, absolute-measure
+        stopping test, u += du, local update).  Linear solves: scipy direct.
+        Returns per-step lists of u (4N), c (C*14), pi (N*3) and the Newton
+        iteration counts."""
+        import scipy.sparse.linalg as spla
+        z = np.ascontiguousarray(z, np.float64)
+        bc_dofs = np.ascontiguousarray(bc_dofs, np.int32)
+        bc_values = np.ascontiguousarray(bc_values, np.float64)
+        us, cs, pis, iters = [], [], [], []
+        up = np.zeros(self.N * 4)
+        cp = np.zeros(self.C * 14)
+        pi = np.zeros(self.N * 3)
+        g_prev = np.zeros_like(bc_values)
+        for k in range(num_steps):
+            t = (1.0 / num_steps) * (k + 1)
+            g = bc_values * t
+            dg = g - g_prev if k > 0 else g.copy()
+            g_prev = g
+            u = up.copy()
+            c = self.update_local_state(u, cp, z)
+            it = 0
+            ref = np.finfo(float).max
+            measure = np.finfo(float).max
+            while True:
+                vals, R = self.condensed_system(u, c, cp, pi, z)
+                rhs = -R
+                apply_block_constraints(4, self.rowmap, self.colidx, vals, rhs, bc_dofs,
+                                        dg if it == 0 else np.zeros_like(dg))
+                norm = float(np.linalg.norm(rhs))
+                if it == 0:
+                    ref = norm
+                else:
+                    measure = abs(ref - norm)
+                    ref = norm
+                if measure < stop_tol or norm < current_norm_tol or it >= max_newton:
+                    break
+                K = to_scipy(4, 4, self.rowmap, self.colidx, vals)
+                du = spla.spsolve(K.tocsc(), rhs)
+                u = u + du
+                c = self.update_local_state(u, cp, z)
+                it += 1
+            us.append(u)
+            cs.append(c)
+            iters.append(it)
+            pis.append(pi)
+            if k + 1 < num_steps:
+                pi = self.projection(u, z)
+            up, cp = u, c
+        return us, cs, pis, iters
+
+    # -- criterion and its path-dependent adjoint ---------------------------
+    def cell_geometry(self):
+        g = np.zeros(self.C * 12)
+        v = np.zeros(self.C)
+        lib().orc_j2_cell_geometry(ctypes.byref(self.P), _p(g), _p(v))
+        return g.reshape(self.C, 4, 3), v
+
+    def _projection_weight(self, z):
+        """Cell weight w(rho) of the projection and its derivative dw/drho."""
+        rho = np.asarray(z)[self.conn.reshape(-1, 4)].mean(axis=1)
+        if not self.P.projection_penalty:
+            return np.ones(self.C), np.zeros(self.C)
+        p, emin = self.P.projection_penalty_exponent, self.P.projection_min_ersatz
+        return emin + (1 - emin) * rho ** p, (1 - emin) * p * rho ** (p - 1)
+
+    def criterion_value(self, z, us, cs, multiplier=1.0):
+        """Plastic work summed over the steps (src/PlasticityProblem.hpp:925-956)."""
+        total = 0.0
+        for k in range(len(us)):
+            up = us[k - 1] if k else np.zeros(self.N * 4)
+            cp = cs[k - 1] if k else np.zeros(self.C * 14)
+            total += self.plastic_work(us[k], up, cs[k], cp, z)
+        return multiplier * total
+
+    def criterion_gradient_z(self, z, us, cs, pis, bc_dofs, multiplier=1.0):
+        """dF/dz by the backward recursion of src/PathDependentAdjointSolver.hpp
+        (:277-351 local RHS, :394-469 global RHS, :737-770 projection adjoint,
+        :772-798 transposed condensed tangent solve, :824-880 local adjoint,
+        :517-551 control partials) driven as in src/PlasticityProblem.hpp:1035-1066,
+        plus the criterion's own control partial summed over steps.
+
+        The projection control partial is evaluated with the step's own
+        pressure, as the reference's forward-state bookkeeping passes it
+        (src/PlasticityProblem.hpp:1095-1112); it vanishes for NoPenalty."""
+        import scipy.sparse.linalg as spla
+        z = np.ascontiguousarray(z, np.float64)
+        K, N, C = len(us), self.N, self.C
+        cn = self.conn.reshape(-1, 4)
+        grads, vols = self.cell_geometry()
+        w, dw = self._projection_weight(z)
+        ps = self.P.pressure_scaling
+        rowsum = np.zeros(N)
+        np.add.at(rowsum, cn.ravel(), np.repeat(0.25 * vols * w, 4))
+        zeroC, zeroN = np.zeros(C * 14), np.zeros(N * 4)
+
+        def assemble(ws, width):  # cell workset C x (4*width) -> nodal
+            out = np.zeros(N * width)
+            idx = (cn[:, :, None] * width + np.arange(width)).reshape(C, -1)
+            np.add.at(out, idx.ravel(), ws.reshape(C, -1).ravel())
+            return out
+
+        def partials(k):
+            up = us[k - 1] if k else zeroN
+            cp = cs[k - 1] if k else zeroC
+            return [multiplier * a for a in self.plastic_work_partials(us[k], up, cs[k], cp, z)]
+
+        dfdz = np.zeros(N)
+        lam1 = mu1 = gam1 = None
+        for k in range(K - 1, -1, -1):
+            u, c, pi = us[k], cs[k], pis[k]
+            cp = cs[k - 1] if k else zeroC
+            pi_ws = gather_nodal(self.conn, pi, 3)
+            invJ = batched_inverse(self.local_gradient("c", u, c, cp, z))
+            dHdu = self.local_gradient("u", u, c, cp, z)
+            dHdz = self.local_gradient("z", u, c, cp, z)
+            dRdc = self.global_gradient_ws("c", u, c, pi_ws, z)
+            dRdz = self.global_gradient_ws("z", u, c, pi_ws, z)
+            dRdpi = self.global_gradient_ws("pi", u, c, pi_ws, z)
+            gu, gup, gc, gcp, gz = partials(k)
+            gc = gc.reshape(C, 14)
+            b_local = gc.copy()
+            rhs = gu.copy()
+            if k < K - 1:
+                gu1, gup1, gc1, gcp1, gz1 = partials(k + 1)
+                dHdcp1 = self.local_gradient("cp", us[k + 1], cs[k + 1], c, z)
+                b_local += gcp1.reshape(C, 14) + np.einsum("eij,ei->ej", dHdcp1, mu1)
+                rhs += gup1
+                # DpDn^T gamma_{k+1}: dQ_{a,d}/dp_b = -1/4 V w ps dN_b/dx_d
+                gw = gather_nodal(self.conn, gam1, 3).reshape(C, 4, 3).sum(axis=1)  # sum_a gamma_{a,d}
+                dp = -0.25 * (vols * w * ps)[:, None] * np.einsum("ebd,ed->eb", grads, gw)
+                pr = np.zeros(N)
+                np.add.at(pr, cn.ravel(), dp.ravel())
+                rhs[3::4] += pr
+            t = np.einsum("eji,ej->ei", invJ, b_local)
+            rhs -= assemble(np.einsum("eji,ej->ei", dHdu, t), 4)
+            rhs = -rhs
+            vals, _ = self.condensed_system(u, c, cp, pi, z)
+            # transposed assembly (assemble_jacobian_transpose_pod): block (r, c)
+            # of K^T is block (c, r) of K transposed; the graph is symmetric
+            rmap, cidx = self.rowmap, self.colidx
+            rows = np.repeat(np.arange(N, dtype=np.int64), np.diff(rmap))
+            pos = np.searchsorted(rows * N + cidx, cidx.astype(np.int64) * N + rows)
+            bvals = np.ascontiguousarray(vals.reshape(-1, 4, 4)[pos].transpose(0, 2, 1)).ravel()
+            apply_block_constraints(4, rmap, cidx, bvals, rhs, bc_dofs, np.zeros(len(bc_dofs)))
+            lam = spla.spsolve(to_scipy(4, 4, rmap, cidx, bvals).tocsc(), rhs)
+            lam_ws = gather_nodal(self.conn, lam, 4)
+            b2 = gc + np.einsum("eji,ej->ei", dRdc, lam_ws)
+            if k < K - 1:
+                b2 += gcp1.reshape(C, 14) + np.einsum("eij,ei->ej", dHdcp1, mu1)
+            mu = -np.einsum("eji,ej->ei", invJ, b2)
+            if k == 0:
+                gam = np.zeros(N * 3)
+            else:
+                r = -assemble(np.einsum("eji,ej->ei", dRdpi, lam_ws), 3)
+                gam = (r.reshape(N, 3) / rowsum[:, None]).ravel()
+            dfdz += assemble(np.einsum("eji,ej->ei", dRdz, lam_ws), 1)
+            dfdz += assemble(np.einsum("eji,ej->ei", dHdz, mu), 1)
+            if self.P.projection_penalty and k > 0:
+                # dQ_{a,d}/dz_b = 1/4 * 1/4 V dw (Pi_gp - ps grad p_k)_d
+                pk = u[3::4][cn]
+                gp = ps * np.einsum("eb,ebd->ed", pk, grads)
+                pig = pi_ws.reshape(C, 4, 3).mean(axis=1)
+                gam_ws = gather_nodal(self.conn, gam, 3).reshape(C, 4, 3).sum(axis=1)
+                q = 0.25 * 0.25 * vols * dw * np.einsum("ed,ed->e", pig - gp, gam_ws)
+                dfdz += assemble(np.repeat(q[:, None], 4, axis=1), 1)
+            dfdz += gz
+            lam1, mu1, gam1 = lam, mu, gam
+        return dfdz
