@@ -307,6 +307,123 @@ int pa_problem_criterion_value(pa_problem* problem, const double* z_dev, const d
 int pa_problem_criterion_gradient(pa_problem* problem, const double* z_dev, const double* u_dev, double* grad_dev,
                                   void* stream);
 
+/* -------------------------------------------------------- J2 plasticity */
+/*
+ * Infinitesimal-strain J2 plasticity in the reference's stabilized mixed u-p
+ * form (Physics "Plasticity", src/SimplexPlasticity.hpp:25-31): 4 global dofs
+ * per node (ux, uy, uz, p) in u[node*4 + d]; 14 local states per cell in
+ * c[cell*14 + k] (accumulated plastic strain, plastic multiplier increment,
+ * plastic strain xx yy zz yz xz xy, backstress xx yy zz yz xz xy;
+ * src/J2PlasticityLocalResidual.hpp:99-119); the nodal projected pressure
+ * gradient pi[node*3 + d].  Cell worksets are [cell][row][column] with the
+ * local global dof node*4 + d, as the reference's ScalarArray3D worksets.
+ */
+typedef struct {
+    double youngs, poisson;                                  /* Isotropic Linear Elastic */
+    double hardening_isotropic, hardening_kinematic;         /* J2 Plasticity */
+    double initial_yield_stress;
+    double elastic_penalty_exponent, elastic_min_ersatz;     /* defaults 1, 1e-9 */
+    double plastic_penalty_exponent, plastic_min_ersatz;     /* defaults 0.5, 1e-4 */
+    double global_penalty_exponent, global_min_ersatz;       /* Elliptic / Penalty Function: 1, 1e-9 */
+    double pressure_scaling;                                 /* Material Models / Pressure Scaling: 1 */
+    int projection_penalty;                                  /* State Gradient Projection penalty: 0 none, 1 SIMP */
+    double projection_penalty_exponent, projection_min_ersatz;
+    double work_penalty_exponent, work_min_ersatz;           /* plastic work criterion: 3, 1e-9 */
+} pa_j2_params;
+/* The reference's defaults (src/J2PlasticityLocalResidual.hpp:195-220,
+ * src/InfinitesimalStrainPlasticityResidual.hpp:142-160,214,
+ * src/PlasticWorkCriterion.hpp:295-303); moduli and yield stress zero. */
+void pa_j2_params_default(pa_j2_params* params);
+
+/* Which partial derivative a workset Jacobian holds. */
+enum { PA_WRT_U = 0, PA_WRT_C = 1, PA_WRT_Z = 2, PA_WRT_C_PREV = 3, PA_WRT_PI = 4 };
+
+typedef struct pa_plasticity pa_plasticity;
+int pa_plasticity_create(const pa_mesh* mesh, const pa_j2_params* params, pa_plasticity** plasticity);
+int pa_plasticity_destroy(pa_plasticity* plasticity);
+
+/* LocalVectorFunctionInc::updateLocalState, closed-form radial return
+ * (src/LocalVectorFunctionInc.hpp:236-300, src/J2PlasticityLocalResidual.hpp:440-543). */
+int pa_j2_update_local_state(pa_plasticity* p, const double* u_dev, const double* c_prev_dev, const double* z_dev,
+                             double* c_dev, void* stream);
+/* LocalVectorFunctionInc::valueWorkSet: H[cell*14 + k]
+ * (src/LocalVectorFunctionInc.hpp:302-370, src/J2PlasticityLocalResidual.hpp:310-428). */
+int pa_j2_local_value(pa_plasticity* p, const double* u_dev, const double* c_dev, const double* c_prev_dev,
+                      const double* z_dev, double* H_dev, void* stream);
+/* LocalVectorFunctionInc::gradient_{u,c,z,cp} worksets [cell][14][W], W = 16 (U), 14 (C, C_PREV), 4 (Z)
+ * (src/LocalVectorFunctionInc.hpp:372-874), hand-derived in place of the FAD evaluation types. */
+int pa_j2_local_gradient(pa_plasticity* p, int wrt, const double* u_dev, const double* c_dev,
+                         const double* c_prev_dev, const double* z_dev, double* out_dev, void* stream);
+/* GlobalVectorFunctionInc::value: assembled R[node*4 + d]
+ * (src/GlobalVectorFunctionInc.hpp:919-960, src/InfinitesimalStrainPlasticityResidual.hpp:393-521). */
+int pa_j2_global_value(pa_plasticity* p, const double* u_dev, const double* c_dev, const double* pi_dev,
+                       const double* z_dev, double* R_dev, void* stream);
+/* GlobalVectorFunctionInc::gradient_{u,c,z,n} worksets [cell][16][W], W = 16 (U), 14 (C), 4 (Z), 12 (PI)
+ * (src/GlobalVectorFunctionInc.hpp:962-1155). */
+int pa_j2_global_gradient(pa_plasticity* p, int wrt, const double* u_dev, const double* c_dev, const double* pi_dev,
+                          const double* z_dev, double* out_dev, void* stream);
+/*
+ * One Newton linearization of the condensed system
+ * (NewtonRaphsonSolver::assembleTangentMatrix / assembleResidual,
+ * src/NewtonRaphsonSolver.hpp:204-330): block-CRS 4x4 values on the mesh graph
+ *   K_t = dR/du - dR/dc (dH/dc)^-1 dH/du,   R_t = R - dR/dc (dH/dc)^-1 H.
+ * transpose != 0 assembles K_t^T (the adjoint's operator,
+ * src/PathDependentAdjointSolver.hpp:232-275); R_dev may then be null.
+ */
+int pa_j2_condensed_system(pa_plasticity* p, const double* u_dev, const double* c_dev, const double* c_prev_dev,
+                           const double* pi_dev, const double* z_dev, double* values_dev, double* R_dev,
+                           int transpose, void* stream);
+/* Projected pressure gradient from the pressure in u (PlasticityProblem::updateProjectedPressureGradient,
+ * src/PlasticityProblem.hpp:872-891; lumped solve src/Plato_Solve.hpp:22-45). */
+int pa_j2_projection(pa_plasticity* p, const double* u_dev, const double* z_dev, double* pi_dev, void* stream);
+/* PlasticWorkCriterion::evaluate of one step summed over cells, one double at value_dev
+ * (src/PlasticWorkCriterion.hpp:163-265). */
+int pa_j2_plastic_work(pa_plasticity* p, const double* u_dev, const double* u_prev_dev, const double* c_dev,
+                       const double* c_prev_dev, const double* z_dev, double* value_dev, void* stream);
+
+/*
+ * PlasticityProblem (src/PlasticityProblem.hpp:784-868, 1035-1068): pseudo-time
+ * steps with load factor (k+1)/K on the Dirichlet values, the condensed Newton
+ * solve per step (elastic trial state, Dirichlet increment on the first
+ * iteration, absolute-residual stopping test, src/NewtonRaphsonSolver.hpp:555-610)
+ * with BiCGStab + 4x4 block-Jacobi linear solves, the projection update after
+ * each step, every step's states kept on the device (the checkpoints of the
+ * adjoint), the plastic-work criterion summed over steps and its total
+ * gradient by the path-dependent adjoint (src/PathDependentAdjointSolver.hpp).
+ */
+typedef struct {
+    int num_steps;                  /* Initial Num. Pseudo Time Steps (default 20) */
+    int max_newton;                 /* Maximum Number Iterations (default 10) */
+    double newton_stop_tol;         /* Stopping Tolerance (default 1e-8) */
+    double newton_current_norm_tol; /* Current Residual Norm Stopping Tolerance (default 1e-8) */
+    int linear_max_iter;            /* BiCGStab iteration cap (default 20000) */
+    double linear_tol;              /* BiCGStab relative residual (default 1e-12) */
+} pa_plasticity_options;
+void pa_plasticity_options_default(pa_plasticity_options* options);
+
+typedef struct pa_plasticity_problem pa_plasticity_problem;
+int pa_plasticity_problem_create(pa_plasticity* p, int num_bcs, const int* bc_dofs_host,
+                                 const double* bc_values_host, const pa_plasticity_options* options,
+                                 pa_plasticity_problem** problem);
+int pa_plasticity_problem_destroy(pa_plasticity_problem* problem);
+/* Forward solve for control z; returns PA_OK with *converged = 0 if a step's
+ * Newton solve stopped without converging.  Synchronizes. */
+int pa_plasticity_problem_solution(pa_plasticity_problem* problem, const double* z_dev, int* converged,
+                                   void* stream);
+/* Device pointers to the states of step k (valid until the next solution call). */
+int pa_plasticity_problem_state(const pa_plasticity_problem* problem, int step, const double** u_dev,
+                                const double** c_dev, const double** pi_dev);
+/* Newton iterations, summed linear iterations and final residual norm of step k. */
+int pa_plasticity_problem_stats(const pa_plasticity_problem* problem, int step, int* newton_iterations,
+                                int* linear_iterations, double* residual_norm);
+/* multiplier * sum over steps of the plastic work (host value).  Synchronizes. */
+int pa_plasticity_problem_criterion_value(pa_plasticity_problem* problem, const double* z_dev, double multiplier,
+                                          double* value_host, void* stream);
+/* Total derivative of the criterion with respect to z (N doubles) by the
+ * backward recursion over the stored steps.  Synchronizes. */
+int pa_plasticity_problem_criterion_gradient(pa_plasticity_problem* problem, const double* z_dev,
+                                             double multiplier, double* grad_dev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,6 +59,25 @@ class ProblemOptions(ctypes.Structure):
                 ("max_iter", ctypes.c_int), ("replace_every", ctypes.c_int), ("check_every", ctypes.c_int)]
 
 
+class J2Params(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "youngs", "poisson", "hardening_isotropic", "hardening_kinematic", "initial_yield_stress",
+        "elastic_penalty_exponent", "elastic_min_ersatz", "plastic_penalty_exponent", "plastic_min_ersatz",
+        "global_penalty_exponent", "global_min_ersatz", "pressure_scaling")] + [
+        ("projection_penalty", ctypes.c_int), ("projection_penalty_exponent", ctypes.c_double),
+        ("projection_min_ersatz", ctypes.c_double), ("work_penalty_exponent", ctypes.c_double),
+        ("work_min_ersatz", ctypes.c_double)]
+
+
+class PlasticityOptions(ctypes.Structure):
+    _fields_ = [("num_steps", ctypes.c_int), ("max_newton", ctypes.c_int), ("newton_stop_tol", ctypes.c_double),
+                ("newton_current_norm_tol", ctypes.c_double), ("linear_max_iter", ctypes.c_int),
+                ("linear_tol", ctypes.c_double)]
+
+
+WRT_U, WRT_C, WRT_Z, WRT_C_PREV, WRT_PI = 0, 1, 2, 3, 4
+
+
 def _load():
     if not LIB_PATH.exists():
         raise ImportError(f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`")
@@ -127,6 +146,25 @@ def _load():
         "pa_multigrid_setup": ([vp, vp, i, i, vp, vp], i),
         "pa_multigrid_apply": ([vp, vp, vp, vp], i),
         "pa_multigrid_level_operator": ([vp, i, vp, ip, ip, ip], i),
+        "pa_j2_params_default": ([ctypes.POINTER(J2Params)], None),
+        "pa_plasticity_create": ([vp, ctypes.POINTER(J2Params), ctypes.POINTER(vp)], i),
+        "pa_plasticity_destroy": ([vp], i),
+        "pa_j2_update_local_state": ([vp, vp, vp, vp, vp, vp], i),
+        "pa_j2_local_value": ([vp, vp, vp, vp, vp, vp, vp], i),
+        "pa_j2_local_gradient": ([vp, i, vp, vp, vp, vp, vp, vp], i),
+        "pa_j2_global_value": ([vp, vp, vp, vp, vp, vp, vp], i),
+        "pa_j2_global_gradient": ([vp, i, vp, vp, vp, vp, vp, vp], i),
+        "pa_j2_condensed_system": ([vp, vp, vp, vp, vp, vp, vp, vp, i, vp], i),
+        "pa_j2_projection": ([vp, vp, vp, vp, vp], i),
+        "pa_j2_plastic_work": ([vp, vp, vp, vp, vp, vp, vp, vp], i),
+        "pa_plasticity_options_default": ([ctypes.POINTER(PlasticityOptions)], None),
+        "pa_plasticity_problem_create": ([vp, i, vp, vp, ctypes.POINTER(PlasticityOptions), ctypes.POINTER(vp)], i),
+        "pa_plasticity_problem_destroy": ([vp], i),
+        "pa_plasticity_problem_solution": ([vp, vp, ip, vp], i),
+        "pa_plasticity_problem_state": ([vp, i, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp)], i),
+        "pa_plasticity_problem_stats": ([vp, i, ip, ip, ctypes.POINTER(d)], i),
+        "pa_plasticity_problem_criterion_value": ([vp, vp, d, ctypes.POINTER(d), vp], i),
+        "pa_plasticity_problem_criterion_gradient": ([vp, vp, d, vp, vp], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -605,5 +643,160 @@ class Multigrid:
     def __del__(self):
         try:
             lib.pa_multigrid_destroy(self.handle)
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# J2 plasticity
+# ---------------------------------------------------------------------------
+def j2_params(**kw):
+    """pa_j2_params with the reference's defaults, overridden by keywords."""
+    p = J2Params()
+    lib.pa_j2_params_default(ctypes.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise PlatoError(f"unknown J2 parameter {k!r}")
+        setattr(p, k, v)
+    return p
+
+
+class Plasticity:
+    """Stabilized mixed u-p J2 plasticity on one mesh: the reference's
+    GlobalVectorFunctionInc (src/GlobalVectorFunctionInc.hpp) and
+    LocalVectorFunctionInc (src/LocalVectorFunctionInc.hpp) for
+    Physics 'Plasticity', plus the projection and the plastic-work criterion.
+
+    u: 4 dofs per node (ux, uy, uz, p); c: 14 local states per cell;
+    pi: 3 per node.  Worksets come back as (C, rows, W) numpy arrays."""
+
+    def __init__(self, mesh, **params):
+        self.mesh = mesh
+        self.params = j2_params(**params)
+        h = ctypes.c_void_p()
+        _check(lib.pa_plasticity_create(mesh.handle, ctypes.byref(self.params), ctypes.byref(h)))
+        self.handle = h
+        self.N, self.C = mesh.num_nodes, mesh.num_cells
+
+    # LocalVectorFunctionInc
+    def update_local_state(self, u, c_prev, z, out=None, stream=None):
+        u, cp, z = _dev(u), _dev(c_prev), _dev(z)
+        out = out if out is not None else DeviceArray(self.C * 14)
+        _check(lib.pa_j2_update_local_state(self.handle, u.ptr, cp.ptr, z.ptr, out.ptr, _sh(stream)))
+        return out
+
+    def local_value(self, u, c, c_prev, z, out=None, stream=None):
+        u, c, cp, z = _dev(u), _dev(c), _dev(c_prev), _dev(z)
+        out = out if out is not None else DeviceArray(self.C * 14)
+        _check(lib.pa_j2_local_value(self.handle, u.ptr, c.ptr, cp.ptr, z.ptr, out.ptr, _sh(stream)))
+        return out
+
+    def local_gradient(self, wrt, u, c, c_prev, z, stream=None):
+        W = {WRT_C: 14, WRT_U: 16, WRT_Z: 4, WRT_C_PREV: 14}[wrt]
+        u, c, cp, z = _dev(u), _dev(c), _dev(c_prev), _dev(z)
+        out = DeviceArray(self.C * 14 * W)
+        _check(lib.pa_j2_local_gradient(self.handle, wrt, u.ptr, c.ptr, cp.ptr, z.ptr, out.ptr, _sh(stream)))
+        return out.numpy().reshape(self.C, 14, W)
+
+    # GlobalVectorFunctionInc
+    def global_value(self, u, c, pi, z, out=None, stream=None):
+        u, c, pi, z = _dev(u), _dev(c), _dev(pi), _dev(z)
+        out = out if out is not None else DeviceArray(self.N * 4)
+        _check(lib.pa_j2_global_value(self.handle, u.ptr, c.ptr, pi.ptr, z.ptr, out.ptr, _sh(stream)))
+        return out
+
+    def global_gradient(self, wrt, u, c, pi, z, stream=None):
+        W = {WRT_U: 16, WRT_C: 14, WRT_Z: 4, WRT_PI: 12}[wrt]
+        u, c, pi, z = _dev(u), _dev(c), _dev(pi), _dev(z)
+        out = DeviceArray(self.C * 16 * W)
+        _check(lib.pa_j2_global_gradient(self.handle, wrt, u.ptr, c.ptr, pi.ptr, z.ptr, out.ptr, _sh(stream)))
+        return out.numpy().reshape(self.C, 16, W)
+
+    def condensed_system(self, u, c, c_prev, pi, z, transpose=False, values=None, R=None, stream=None):
+        """(CrsMatrix 4x4 of K_t or K_t^T, DeviceArray R_t or None)."""
+        u, c, cp, pi, z = _dev(u), _dev(c), _dev(c_prev), _dev(pi), _dev(z)
+        values = values if values is not None else DeviceArray(self.mesh.num_blocks * 16)
+        R = None if transpose else (R if R is not None else DeviceArray(self.N * 4))
+        _check(lib.pa_j2_condensed_system(self.handle, u.ptr, c.ptr, cp.ptr, pi.ptr, z.ptr, values.ptr,
+                                          R.ptr if R is not None else None, 1 if transpose else 0, _sh(stream)))
+        return CrsMatrix(self.mesh, 4, 4, values, FP64), R
+
+    def projection(self, u, z, out=None, stream=None):
+        u, z = _dev(u), _dev(z)
+        out = out if out is not None else DeviceArray(self.N * 3)
+        _check(lib.pa_j2_projection(self.handle, u.ptr, z.ptr, out.ptr, _sh(stream)))
+        return out
+
+    def plastic_work(self, u, u_prev, c, c_prev, z, stream=None):
+        u, up, c, cp, z = _dev(u), _dev(u_prev), _dev(c), _dev(c_prev), _dev(z)
+        out = DeviceArray(1)
+        _check(lib.pa_j2_plastic_work(self.handle, u.ptr, up.ptr, c.ptr, cp.ptr, z.ptr, out.ptr, _sh(stream)))
+        return float(out.numpy()[0])
+
+    def __del__(self):
+        try:
+            lib.pa_plasticity_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class PlasticityProblem:
+    """PlasticityProblem (src/PlasticityProblem.hpp): pseudo-time steps with the
+    condensed Newton solve, the plastic-work criterion and its path-dependent
+    adjoint gradient.  Dirichlet values are the end values of the load ramp."""
+
+    def __init__(self, plasticity, bc_dofs, bc_values=None, *, num_steps=20, max_newton=10, newton_stop_tol=1e-8,
+                 newton_current_norm_tol=1e-8, linear_max_iter=20000, linear_tol=1e-12):
+        self.pl = plasticity
+        dofs = np.ascontiguousarray(bc_dofs, np.int32)
+        vals = np.zeros(dofs.size) if bc_values is None else np.ascontiguousarray(bc_values, np.float64)
+        self.options = PlasticityOptions(num_steps, max_newton, newton_stop_tol, newton_current_norm_tol,
+                                         linear_max_iter, linear_tol)
+        h = ctypes.c_void_p()
+        _check(lib.pa_plasticity_problem_create(plasticity.handle, dofs.size, _np_ptr(dofs), _np_ptr(vals),
+                                                ctypes.byref(self.options), ctypes.byref(h)))
+        self.handle = h
+        self.num_steps = num_steps
+
+    def solution(self, z, stream=None):
+        z = _dev(z)
+        conv = ctypes.c_int()
+        _check(lib.pa_plasticity_problem_solution(self.handle, z.ptr, ctypes.byref(conv), _sh(stream)))
+        return bool(conv.value)
+
+    def state(self, step):
+        """(u, c, pi) of a step as numpy arrays."""
+        pu, pc, pp = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        _check(lib.pa_plasticity_problem_state(self.handle, step, ctypes.byref(pu), ctypes.byref(pc),
+                                               ctypes.byref(pp)))
+        out = []
+        for ptr, n in ((pu, self.pl.N * 4), (pc, self.pl.C * 14), (pp, self.pl.N * 3)):
+            a = np.empty(n)
+            _check(lib.pa_memcpy_d2h(_np_ptr(a), ptr, a.nbytes))
+            out.append(a)
+        return tuple(out)
+
+    def stats(self, step):
+        n, l, r = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(lib.pa_plasticity_problem_stats(self.handle, step, ctypes.byref(n), ctypes.byref(l),
+                                               ctypes.byref(r)))
+        return {"newton_iterations": n.value, "linear_iterations": l.value, "residual_norm": r.value}
+
+    def criterion_value(self, z, multiplier=1.0, stream=None):
+        z = _dev(z)
+        v = ctypes.c_double()
+        _check(lib.pa_plasticity_problem_criterion_value(self.handle, z.ptr, multiplier, ctypes.byref(v),
+                                                         _sh(stream)))
+        return v.value
+
+    def criterion_gradient(self, z, multiplier=1.0, out=None, stream=None):
+        z = _dev(z)
+        out = out if out is not None else DeviceArray(self.pl.N)
+        _check(lib.pa_plasticity_problem_criterion_gradient(self.handle, z.ptr, multiplier, out.ptr, _sh(stream)))
+        return out
+
+    def __del__(self):
+        try:
+            lib.pa_plasticity_problem_destroy(self.handle)
         except Exception:
             pass

@@ -1,0 +1,614 @@
+// PlasticityProblem on the device: pseudo-time steps, the condensed Newton
+// solve of every step with BiCGStab linear solves, the projection update, the
+// plastic-work criterion and its path-dependent adjoint gradient.
+//
+// BiCGStab keeps its scalars on the device (fixed-order reductions, so runs
+// are deterministic); the host reads ||r||^2 every `check_every` iterations.
+// Preconditioner: inverses of the 4x4 diagonal blocks (partial pivoting),
+// i.e. the nodal (ux, uy, uz, p) blocks of the saddle-point tangent.
+//
+// Every step's u, c and pi stay on the device: they are the checkpoints the
+// backward recursion reads (a 64x32x32-hex problem with 10 steps holds about
+// 0.5 GB).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "pa_j2.h"
+
+struct pa_plasticity_problem {
+    pa_plasticity* p = nullptr;
+    pa_plasticity_options opt{};
+    int nbc = 0;
+    std::vector<int> h_dofs;
+    std::vector<double> h_vals;
+    pa::DevBuf<int> bc_dofs;
+    pa::DevBuf<double> bc_inc, bc_zero;
+    // states of every step
+    pa::DevBuf<double> U, Cs, PI;
+    pa::DevBuf<double> zeroU, zeroC;
+    // Newton / Krylov workspace
+    pa::DevBuf<double> vals, R, rhs, du;
+    pa::DevBuf<double> r, rh, pv, v, y, s, zz, t, minv, part, scal;
+    // adjoint workspace
+    pa::DevBuf<double> wu, wu_next, gc, gcp, gcp_next, zc, zc_out, blocal, mu, mu_next, lam, gam, gam_next;
+    std::vector<int> newton_its, linear_its;
+    std::vector<double> res_norms;
+    bool solved = false;
+};
+
+namespace pa {
+namespace j2 {
+namespace {
+
+constexpr int kB = 256;
+void check() { PA_HIP(hipGetLastError()); }
+int blocks(long n) { return (int)((n + kB - 1) / kB); }
+
+enum { S_RHO = 0, S_RHO_OLD, S_RV, S_TS, S_TT, S_RR, S_BB, S_NORM, S_N = 16 };
+
+__device__ inline double block_sum(double v, double* sh) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (threadIdx.x % 64 == 0) sh[threadIdx.x / 64] = v;
+    __syncthreads();
+    double tt = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x / 64); ++i) tt += sh[i];
+    __syncthreads();
+    return tt;
+}
+
+// Fixed-order sums of `count` partial arrays (stride nparts) into scal[dst_k];
+// with shift, scal[S_RHO_OLD] = scal[S_RHO] first.
+__global__ void k_reduce(const double* __restrict__ part, int n, int nparts, int count, int d0, int d1, int shift,
+                         double* __restrict__ scal) {
+    __shared__ double sh[1024 / 64];
+    const int dst[2] = {d0, d1};
+    for (int kk = 0; kk < count; ++kk) {
+        double a = 0.0;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) a += part[(long)kk * nparts + i];
+        const double tt = block_sum(a, sh);
+        if (threadIdx.x == 0) {
+            if (shift && kk == 0) scal[S_RHO_OLD] = scal[S_RHO];
+            scal[dst[kk]] = tt;
+        }
+        __syncthreads();
+    }
+}
+
+// y = A x (4x4 blocks, 16 lanes per block row); partial sums of y.a1 and y.a2.
+__global__ __launch_bounds__(kB) void k_spmv4(int N, const int* __restrict__ rowmap, const int* __restrict__ colidx,
+                                              const double* __restrict__ vals, const double* __restrict__ x,
+                                              double* __restrict__ y, const double* __restrict__ a1,
+                                              const double* __restrict__ a2, double* __restrict__ part,
+                                              int nparts) {
+    constexpr int G = 16, ROWS = kB / G;
+    __shared__ double sh[kB / 64];
+    const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+    const int row = blockIdx.x * ROWS + grp;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (row < N) {
+        for (int e = rowmap[row] + lane; e < rowmap[row + 1]; e += G) {
+            const long c = colidx[e];
+            const double xv[4] = {x[c * 4], x[c * 4 + 1], x[c * 4 + 2], x[c * 4 + 3]};
+            const double* b = vals + (long)e * 16;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i] += b[i * 4 + j] * xv[j];
+        }
+    }
+    double d1 = 0.0, d2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) acc[i] += __shfl_xor(acc[i], o, G);
+        if (row < N && lane == 0) {
+            const long o = (long)row * 4 + i;
+            y[o] = acc[i];
+            if (a1) d1 += acc[i] * a1[o];
+            if (a2) d2 += acc[i] * a2[o];
+        }
+    }
+    const double s1 = block_sum(d1, sh), s2 = block_sum(d2, sh);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = s1;
+        part[nparts + blockIdx.x] = s2;
+    }
+}
+
+// Inverse of each 4x4 diagonal block, Gauss-Jordan with partial pivoting.
+__global__ void k_block_inverse4(int N, const int* __restrict__ rowmap, const int* __restrict__ colidx,
+                                 const double* __restrict__ vals, double* __restrict__ minv) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    int e = rowmap[n];
+    while (colidx[e] != n) ++e;
+    double A[4][8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            A[i][j] = vals[(long)e * 16 + i * 4 + j];
+            A[i][4 + j] = (i == j) ? 1.0 : 0.0;
+        }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        int piv = k;
+        double best = fabs(A[k][k]);
+#pragma unroll
+        for (int i = k + 1; i < 4; ++i)
+            if (fabs(A[i][k]) > best) {
+                best = fabs(A[i][k]);
+                piv = i;
+            }
+#pragma unroll
+        for (int i = k + 1; i < 4; ++i)
+            if (i == piv)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const double tmp = A[k][j];
+                    A[k][j] = A[i][j];
+                    A[i][j] = tmp;
+                }
+        const double inv = A[k][k] != 0.0 ? 1.0 / A[k][k] : 0.0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) A[k][j] *= inv;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (i != k) {
+                const double f = A[i][k];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) A[i][j] -= f * A[k][j];
+            }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) minv[(long)n * 16 + i * 4 + j] = A[i][4 + j];
+}
+
+__device__ inline void prec4(const double* __restrict__ minv, long n, const double in[4], double out[4]) {
+    const double* m = minv + n * 16;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) out[i] = m[i * 4] * in[0] + m[i * 4 + 1] * in[1] + m[i * 4 + 2] * in[2] + m[i * 4 + 3] * in[3];
+}
+
+// r = b - q, rh = r, p = v = 0; partials of r.r (-> rho and rr) and b.b.
+__global__ __launch_bounds__(kB) void k_bicg_init(long n, const double* __restrict__ b, const double* __restrict__ q,
+                                                  double* __restrict__ r, double* __restrict__ rh,
+                                                  double* __restrict__ p, double* __restrict__ v,
+                                                  double* __restrict__ part, int nparts) {
+    __shared__ double sh[kB / 64];
+    double rr = 0.0, bb = 0.0;
+    for (long i = (long)blockIdx.x * kB + threadIdx.x; i < n; i += (long)gridDim.x * kB) {
+        const double ri = b[i] - q[i];
+        r[i] = ri;
+        rh[i] = ri;
+        p[i] = 0.0;
+        v[i] = 0.0;
+        rr += ri * ri;
+        bb += b[i] * b[i];
+    }
+    const double s1 = block_sum(rr, sh), s2 = block_sum(bb, sh);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = s1;
+        part[nparts + blockIdx.x] = s2;
+    }
+}
+
+// p = r + beta (p - omega v); y = M p; beta = (rho / rho_old)(alpha / omega).
+__global__ __launch_bounds__(kB) void k_bicg_dir(int N, const double* __restrict__ scal,
+                                                 const double* __restrict__ minv, const double* __restrict__ r,
+                                                 const double* __restrict__ v, double* __restrict__ p,
+                                                 double* __restrict__ y) {
+    const double alpha = scal[S_RHO_OLD] / scal[S_RV];
+    const double omega = scal[S_TS] / scal[S_TT];
+    const double beta = (scal[S_RHO] / scal[S_RHO_OLD]) * (alpha / omega);
+    for (long n = (long)blockIdx.x * kB + threadIdx.x; n < N; n += (long)gridDim.x * kB) {
+        double pn[4], yn[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const long o = n * 4 + i;
+            pn[i] = r[o] + beta * (p[o] - omega * v[o]);
+            p[o] = pn[i];
+        }
+        prec4(minv, n, pn, yn);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) y[n * 4 + i] = yn[i];
+    }
+}
+
+// s = r - alpha v; z = M s; alpha = rho / (rh.v).
+__global__ __launch_bounds__(kB) void k_bicg_s(int N, const double* __restrict__ scal, const double* __restrict__ minv,
+                                               const double* __restrict__ r, const double* __restrict__ v,
+                                               double* __restrict__ s, double* __restrict__ zz) {
+    const double alpha = scal[S_RHO] / scal[S_RV];
+    for (long n = (long)blockIdx.x * kB + threadIdx.x; n < N; n += (long)gridDim.x * kB) {
+        double sn[4], zn[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const long o = n * 4 + i;
+            sn[i] = r[o] - alpha * v[o];
+            s[o] = sn[i];
+        }
+        prec4(minv, n, sn, zn);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) zz[n * 4 + i] = zn[i];
+    }
+}
+
+// x += alpha y + omega z; r = s - omega t; partials of rh.r and r.r.
+__global__ __launch_bounds__(kB) void k_bicg_update(long n, const double* __restrict__ scal,
+                                                    const double* __restrict__ y, const double* __restrict__ zz,
+                                                    const double* __restrict__ s, const double* __restrict__ t,
+                                                    const double* __restrict__ rh, double* __restrict__ x,
+                                                    double* __restrict__ r, double* __restrict__ part, int nparts) {
+    __shared__ double sh[kB / 64];
+    const double alpha = scal[S_RHO] / scal[S_RV];
+    const double omega = scal[S_TS] / scal[S_TT];
+    double d1 = 0.0, d2 = 0.0;
+    for (long i = (long)blockIdx.x * kB + threadIdx.x; i < n; i += (long)gridDim.x * kB) {
+        x[i] += alpha * y[i] + omega * zz[i];
+        const double ri = s[i] - omega * t[i];
+        r[i] = ri;
+        d1 += rh[i] * ri;
+        d2 += ri * ri;
+    }
+    const double a = block_sum(d1, sh), b = block_sum(d2, sh);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = a;
+        part[nparts + blockIdx.x] = b;
+    }
+}
+
+__global__ void k_set_scalars(double* scal) {
+    scal[S_RHO_OLD] = 1.0;
+    scal[S_RV] = 1.0;
+    scal[S_TS] = 1.0;
+    scal[S_TT] = 1.0;
+}
+
+__global__ void k_neg_copy(long n, const double* __restrict__ a, double* __restrict__ b) {
+    for (long i = (long)blockIdx.x * kB + threadIdx.x; i < n; i += (long)gridDim.x * kB) b[i] = -a[i];
+}
+
+__global__ void k_axpy(long n, double a, const double* __restrict__ x, double* __restrict__ y) {
+    for (long i = (long)blockIdx.x * kB + threadIdx.x; i < n; i += (long)gridDim.x * kB) y[i] += a * x[i];
+}
+
+__global__ __launch_bounds__(kB) void k_dot_part(long n, const double* __restrict__ a, double* __restrict__ part) {
+    __shared__ double sh[kB / 64];
+    double s = 0.0;
+    for (long i = (long)blockIdx.x * kB + threadIdx.x; i < n; i += (long)gridDim.x * kB) s += a[i] * a[i];
+    const double tt = block_sum(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = tt;
+}
+
+double read_scal(pa_plasticity_problem* P, int idx, hipStream_t st) {
+    double v = 0.0;
+    PA_HIP(hipMemcpyAsync(&v, P->scal.p + idx, sizeof(double), hipMemcpyDeviceToHost, st));
+    PA_HIP(hipStreamSynchronize(st));
+    return v;
+}
+
+int vec_blocks(const pa_plasticity_problem* P) { return std::min(blocks(4L * P->p->mesh->N), 2048); }
+int row_blocks(const pa_plasticity_problem* P) { return (P->p->mesh->N + 15) / 16; }
+
+double norm(pa_plasticity_problem* P, const double* x, hipStream_t st) {
+    const long n = 4L * P->p->mesh->N;
+    const int nb = vec_blocks(P);
+    k_dot_part<<<nb, kB, 0, st>>>(n, x, P->part.p);
+    check();
+    k_reduce<<<1, 1024, 0, st>>>(P->part.p, nb, nb, 1, S_NORM, 0, 0, P->scal.p);
+    check();
+    return std::sqrt(read_scal(P, S_NORM, st));
+}
+
+// BiCGStab with block-Jacobi preconditioning on the 4x4 block matrix P->vals:
+// solves A x = b from the x given; returns the iterations.
+int bicgstab(pa_plasticity_problem* P, const double* b, double* x, hipStream_t st) {
+    const pa_mesh* m = P->p->mesh;
+    const int N = m->N;
+    const long n = 4L * N;
+    const int nvb = vec_blocks(P), nsb = row_blocks(P);
+    const int np = std::max(nvb, nsb);
+    const double* A = P->vals.p;
+    k_block_inverse4<<<blocks(N), kB, 0, st>>>(N, m->d_rowmap.p, m->d_colidx.p, A, P->minv.p);
+    check();
+    k_spmv4<<<nsb, kB, 0, st>>>(N, m->d_rowmap.p, m->d_colidx.p, A, x, P->t.p, nullptr, nullptr, P->part.p, np);
+    check();
+    k_bicg_init<<<nvb, kB, 0, st>>>(n, b, P->t.p, P->r.p, P->rh.p, P->pv.p, P->v.p, P->part.p, np);
+    check();
+    k_set_scalars<<<1, 1, 0, st>>>(P->scal.p);
+    check();
+    k_reduce<<<1, 1024, 0, st>>>(P->part.p, nvb, np, 2, S_RHO, S_BB, 0, P->scal.p);
+    check();
+    const double bnorm = std::sqrt(read_scal(P, S_BB, st));
+    double rr = read_scal(P, S_RHO, st);
+    const double target = P->opt.linear_tol * bnorm;
+    const int check_every = 10;
+    int it = 0;
+    while (it < P->opt.linear_max_iter && bnorm > 0.0 && std::sqrt(rr) > target) {
+        const int burst = std::min(check_every, P->opt.linear_max_iter - it);
+        for (int q = 0; q < burst; ++q) {
+            k_bicg_dir<<<nvb, kB, 0, st>>>(N, P->scal.p, P->minv.p, P->r.p, P->v.p, P->pv.p, P->y.p);
+            check();
+            k_spmv4<<<nsb, kB, 0, st>>>(N, m->d_rowmap.p, m->d_colidx.p, A, P->y.p, P->v.p, P->rh.p, nullptr,
+                                        P->part.p, np);
+            check();
+            k_reduce<<<1, 1024, 0, st>>>(P->part.p, nsb, np, 1, S_RV, 0, 0, P->scal.p);
+            check();
+            k_bicg_s<<<nvb, kB, 0, st>>>(N, P->scal.p, P->minv.p, P->r.p, P->v.p, P->s.p, P->zz.p);
+            check();
+            k_spmv4<<<nsb, kB, 0, st>>>(N, m->d_rowmap.p, m->d_colidx.p, A, P->zz.p, P->t.p, P->s.p, P->t.p,
+                                        P->part.p, np);
+            check();
+            k_reduce<<<1, 1024, 0, st>>>(P->part.p, nsb, np, 2, S_TS, S_TT, 0, P->scal.p);
+            check();
+            k_bicg_update<<<nvb, kB, 0, st>>>(n, P->scal.p, P->y.p, P->zz.p, P->s.p, P->t.p, P->rh.p, x, P->r.p,
+                                              P->part.p, np);
+            check();
+            k_reduce<<<1, 1024, 0, st>>>(P->part.p, nvb, np, 2, S_RHO, S_RR, 1, P->scal.p);
+            check();
+        }
+        it += burst;
+        rr = read_scal(P, S_RR, st);
+        if (!std::isfinite(rr)) break;
+    }
+    return it;
+}
+
+double* step_u(pa_plasticity_problem* P, int k) { return P->U.p + (long)k * 4 * P->p->mesh->N; }
+double* step_c(pa_plasticity_problem* P, int k) { return P->Cs.p + (long)k * 14 * P->p->mesh->C; }
+double* step_pi(pa_plasticity_problem* P, int k) { return P->PI.p + (long)k * 3 * P->p->mesh->N; }
+
+void constraints(pa_plasticity_problem* P, double* rhs, const double* bcv, hipStream_t st) {
+    if (P->nbc == 0) return;
+    const int rc = pa_apply_block_constraints(P->p->mesh, 4, P->vals.p, PA_FP64, rhs, P->nbc, P->bc_dofs.p, bcv, 1.0,
+                                              st);
+    if (rc != PA_OK) throw Error(rc, pa_last_error());
+}
+
+// Forward solve (src/PlasticityProblem.hpp:784-868, src/NewtonRaphsonSolver.hpp:555-610).
+bool solve_forward(pa_plasticity_problem* P, const double* z, hipStream_t st) {
+    const pa_mesh* m = P->p->mesh;
+    const int K = P->opt.num_steps;
+    const long nu = 4L * m->N;
+    P->newton_its.assign(K, 0);
+    P->linear_its.assign(K, 0);
+    P->res_norms.assign(K, 0.0);
+    PA_HIP(hipMemsetAsync(step_pi(P, 0), 0, 3L * m->N * sizeof(double), st));
+    const double dt = 1.0 / K;
+    std::vector<double> inc(P->nbc);
+    bool all_converged = true;
+    for (int k = 0; k < K; ++k) {
+        double* uk = step_u(P, k);
+        double* ck = step_c(P, k);
+        const double* up = k ? step_u(P, k - 1) : P->zeroU.p;
+        const double* cp = k ? step_c(P, k - 1) : P->zeroC.p;
+        const double* pik = step_pi(P, k);
+        // Dirichlet increment of this step: g(t_k) - g(t_{k-1}), t_k = (k+1)/K
+        const double tk = dt * (k + 1), tp = dt * k;
+        for (int j = 0; j < P->nbc; ++j) inc[j] = P->h_vals[j] * tk - (k > 0 ? P->h_vals[j] * tp : 0.0);
+        if (P->nbc)
+            PA_HIP(hipMemcpyAsync(P->bc_inc.p, inc.data(), P->nbc * sizeof(double), hipMemcpyHostToDevice, st));
+        PA_HIP(hipMemcpyAsync(uk, up, nu * sizeof(double), hipMemcpyDeviceToDevice, st));
+        update_local_state(P->p, uk, cp, z, ck, st);
+        int it = 0, lin = 0;
+        double ref = 0.0, measure = HUGE_VAL, nrm = 0.0;
+        bool converged = false;
+        while (true) {
+            condensed_system(P->p, uk, ck, cp, pik, z, P->vals.p, P->R.p, false, st);
+            k_neg_copy<<<vec_blocks(P), kB, 0, st>>>(nu, P->R.p, P->rhs.p);
+            check();
+            constraints(P, P->rhs.p, it == 0 ? P->bc_inc.p : P->bc_zero.p, st);
+            nrm = norm(P, P->rhs.p, st);
+            if (it == 0) {
+                ref = nrm;
+            } else {
+                measure = std::fabs(ref - nrm);
+                ref = nrm;
+            }
+            if (measure < P->opt.newton_stop_tol || nrm < P->opt.newton_current_norm_tol) {
+                converged = true;
+                break;
+            }
+            if (nrm >= 1e20 || it >= P->opt.max_newton || !std::isfinite(nrm)) break;
+            PA_HIP(hipMemsetAsync(P->du.p, 0, nu * sizeof(double), st));
+            lin += bicgstab(P, P->rhs.p, P->du.p, st);
+            k_axpy<<<vec_blocks(P), kB, 0, st>>>(nu, 1.0, P->du.p, uk);
+            check();
+            update_local_state(P->p, uk, cp, z, ck, st);
+            ++it;
+        }
+        P->newton_its[k] = it;
+        P->linear_its[k] = lin;
+        P->res_norms[k] = nrm;
+        all_converged = all_converged && converged;
+        if (k + 1 < K) projection(P->p, uk, z, step_pi(P, k + 1), st);
+    }
+    PA_HIP(hipStreamSynchronize(st));
+    P->solved = true;
+    return all_converged;
+}
+
+double criterion_value(pa_plasticity_problem* P, const double* z, double mult, hipStream_t st) {
+    double total = 0.0;
+    for (int k = 0; k < P->opt.num_steps; ++k) {
+        plastic_work(P->p, step_u(P, k), k ? step_u(P, k - 1) : P->zeroU.p, step_c(P, k),
+                     k ? step_c(P, k - 1) : P->zeroC.p, z, P->scal.p + S_NORM, st);
+        total += read_scal(P, S_NORM, st);
+    }
+    return mult * total;
+}
+
+// Backward recursion (src/PlasticityProblem.hpp:1035-1066,
+// src/PathDependentAdjointSolver.hpp:277-917).
+void criterion_gradient(pa_plasticity_problem* P, const double* z, double mult, double* grad, hipStream_t st) {
+    const pa_mesh* m = P->p->mesh;
+    const int K = P->opt.num_steps;
+    const long nu = 4L * m->N;
+    PA_HIP(hipMemsetAsync(grad, 0, m->N * sizeof(double), st));
+    for (int k = K - 1; k >= 0; --k) {
+        const bool last = (k == K - 1);
+        const double* u = step_u(P, k);
+        const double* c = step_c(P, k);
+        const double* up = k ? step_u(P, k - 1) : P->zeroU.p;
+        const double* cp = k ? step_c(P, k - 1) : P->zeroC.p;
+        const double* pi = step_pi(P, k);
+        AdjointWork w{P->wu.p, P->gc.p, P->gcp.p, P->zc.p};
+        work_partials(P->p, u, up, c, cp, z, mult, w, st);
+        adjoint_local_rhs(P->p, u, c, cp, z, P->gc.p, last ? nullptr : P->gcp_next.p, last ? nullptr : P->mu_next.p,
+                          P->wu.p, last ? nullptr : P->wu_next.p, last ? nullptr : P->gam_next.p, P->blocal.p, st);
+        gather_vcell(P->p, -1.0, P->rhs.p, st);
+        condensed_system(P->p, u, c, cp, pi, z, P->vals.p, nullptr, true, st);
+        constraints(P, P->rhs.p, P->bc_zero.p, st);
+        PA_HIP(hipMemsetAsync(P->lam.p, 0, nu * sizeof(double), st));
+        bicgstab(P, P->rhs.p, P->lam.p, st);
+        adjoint_local_update(P->p, u, c, cp, pi, z, P->lam.p, P->blocal.p, P->zc.p, P->mu.p, P->zc_out.p, st);
+        if (k == 0) {
+            PA_HIP(hipMemsetAsync(P->gam.p, 0, 3L * m->N * sizeof(double), st));
+        } else {
+            gather_uniform(P->p, 3, P->p->vcell.p, 16, -1.0, false, P->gam.p, st);
+            divide_rowsum(P->p, P->gam.p, z, st);
+            if (P->p->prm.projection_penalty) projection_control_term(P->p, u, pi, P->gam.p, z, P->zc_out.p, st);
+        }
+        gather_uniform(P->p, 1, P->zc_out.p, 1.0, true, grad, st);
+        std::swap(P->wu.p, P->wu_next.p);
+        std::swap(P->gcp.p, P->gcp_next.p);
+        std::swap(P->mu.p, P->mu_next.p);
+        std::swap(P->gam.p, P->gam_next.p);
+    }
+    PA_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace
+}  // namespace j2
+}  // namespace pa
+
+using pa::guarded;
+
+extern "C" {
+
+void pa_plasticity_options_default(pa_plasticity_options* o) {
+    if (!o) return;
+    o->num_steps = 20;
+    o->max_newton = 10;
+    o->newton_stop_tol = 1e-8;
+    o->newton_current_norm_tol = 1e-8;
+    o->linear_max_iter = 20000;
+    o->linear_tol = 1e-12;
+}
+
+int pa_plasticity_problem_create(pa_plasticity* p, int nbc, const int* dofs, const double* vals,
+                                 const pa_plasticity_options* opt, pa_plasticity_problem** out) {
+    return guarded([&] {
+        PA_REQUIRE(p && out && (nbc == 0 || (dofs && vals)) && nbc >= 0, PA_ERR_ARG,
+                   "pa_plasticity_problem_create: bad argument");
+        auto* P = new pa_plasticity_problem();
+        try {
+            P->p = p;
+            if (opt) P->opt = *opt;
+            else pa_plasticity_options_default(&P->opt);
+            PA_REQUIRE(P->opt.num_steps >= 1 && P->opt.max_newton >= 0 && P->opt.linear_max_iter >= 1, PA_ERR_ARG,
+                       "pa_plasticity_problem_create: bad options");
+            const pa_mesh* m = p->mesh;
+            const long nu = 4L * m->N;
+            for (int j = 0; j < nbc; ++j)
+                PA_REQUIRE(dofs[j] >= 0 && dofs[j] < nu, PA_ERR_ARG, "pa_plasticity_problem_create: dof out of range");
+            P->nbc = nbc;
+            P->h_dofs.assign(dofs, dofs + nbc);
+            P->h_vals.assign(vals, vals + nbc);
+            if (nbc) {
+                P->bc_dofs.upload(dofs, nbc);
+                P->bc_inc.alloc(nbc);
+                P->bc_zero.alloc(nbc);
+                PA_HIP(hipMemset(P->bc_zero.p, 0, nbc * sizeof(double)));
+            }
+            const int K = P->opt.num_steps;
+            P->U.alloc(nu * K);
+            P->Cs.alloc(14L * m->C * K);
+            P->PI.alloc(3L * m->N * K);
+            P->zeroU.alloc(nu);
+            P->zeroC.alloc(14L * m->C);
+            PA_HIP(hipMemset(P->zeroU.p, 0, nu * sizeof(double)));
+            PA_HIP(hipMemset(P->zeroC.p, 0, 14L * m->C * sizeof(double)));
+            P->vals.alloc(16L * m->nnzb);
+            for (auto* b : {&P->R, &P->rhs, &P->du, &P->r, &P->rh, &P->pv, &P->v, &P->y, &P->s, &P->zz, &P->t,
+                            &P->lam})
+                b->alloc(nu);
+            P->minv.alloc(16L * m->N);
+            const int np = std::max(pa::j2::vec_blocks(P), pa::j2::row_blocks(P));
+            P->part.alloc(2L * np);
+            P->scal.alloc(pa::j2::S_N);
+            for (auto* b : {&P->wu, &P->wu_next}) b->alloc(16L * m->C);
+            for (auto* b : {&P->gc, &P->gcp, &P->gcp_next, &P->blocal, &P->mu, &P->mu_next}) b->alloc(14L * m->C);
+            P->zc.alloc(m->C);
+            P->zc_out.alloc(m->C);
+            P->gam.alloc(3L * m->N);
+            P->gam_next.alloc(3L * m->N);
+        } catch (...) {
+            delete P;
+            throw;
+        }
+        *out = P;
+    });
+}
+
+int pa_plasticity_problem_destroy(pa_plasticity_problem* P) {
+    return guarded([&] { delete P; });
+}
+
+int pa_plasticity_problem_solution(pa_plasticity_problem* P, const double* z, int* converged, void* stream) {
+    return guarded([&] {
+        PA_REQUIRE(P && z, PA_ERR_ARG, "pa_plasticity_problem_solution: null argument");
+        const bool ok = pa::j2::solve_forward(P, z, (hipStream_t)stream);
+        if (converged) *converged = ok ? 1 : 0;
+    });
+}
+
+int pa_plasticity_problem_state(const pa_plasticity_problem* P, int step, const double** u, const double** c,
+                                const double** pi) {
+    return guarded([&] {
+        PA_REQUIRE(P && step >= 0 && step < P->opt.num_steps, PA_ERR_ARG, "pa_plasticity_problem_state: bad step");
+        PA_REQUIRE(P->solved, PA_ERR_STATE, "pa_plasticity_problem_state: no solution yet");
+        auto* Q = const_cast<pa_plasticity_problem*>(P);
+        if (u) *u = pa::j2::step_u(Q, step);
+        if (c) *c = pa::j2::step_c(Q, step);
+        if (pi) *pi = pa::j2::step_pi(Q, step);
+    });
+}
+
+int pa_plasticity_problem_stats(const pa_plasticity_problem* P, int step, int* newton, int* linear, double* res) {
+    return guarded([&] {
+        PA_REQUIRE(P && step >= 0 && step < P->opt.num_steps, PA_ERR_ARG, "pa_plasticity_problem_stats: bad step");
+        PA_REQUIRE(P->solved, PA_ERR_STATE, "pa_plasticity_problem_stats: no solution yet");
+        if (newton) *newton = P->newton_its[step];
+        if (linear) *linear = P->linear_its[step];
+        if (res) *res = P->res_norms[step];
+    });
+}
+
+int pa_plasticity_problem_criterion_value(pa_plasticity_problem* P, const double* z, double mult, double* value,
+                                          void* stream) {
+    return guarded([&] {
+        PA_REQUIRE(P && z && value, PA_ERR_ARG, "pa_plasticity_problem_criterion_value: null argument");
+        PA_REQUIRE(P->solved, PA_ERR_STATE, "pa_plasticity_problem_criterion_value: no solution yet");
+        *value = pa::j2::criterion_value(P, z, mult, (hipStream_t)stream);
+    });
+}
+
+int pa_plasticity_problem_criterion_gradient(pa_plasticity_problem* P, const double* z, double mult, double* grad,
+                                             void* stream) {
+    return guarded([&] {
+        PA_REQUIRE(P && z && grad, PA_ERR_ARG, "pa_plasticity_problem_criterion_gradient: null argument");
+        PA_REQUIRE(P->solved, PA_ERR_STATE, "pa_plasticity_problem_criterion_gradient: no solution yet");
+        pa::j2::criterion_gradient(P, z, mult, grad, (hipStream_t)stream);
+    });
+}
+
+}  // extern "C"
