@@ -111,6 +111,95 @@ def compliance_gradient(pa, mesh, vf, z, dims):
     return res
 
 
+J2_DIMS, J2_LENGTHS = (64, 32, 32), (2.0, 1.0, 1.0)
+J2_PARAMS = dict(youngs=1e3, poisson=0.3, hardening_isotropic=10.0, hardening_kinematic=0.0, initial_yield_stress=1.0)
+J2_END_DISPLACEMENT, J2_STEPS = 0.008, 10
+
+
+def j2_plasticity(pa, stream, with_cpu=True):
+    """J2 plasticity (the reference's stabilized mixed u-p formulation, 4 dofs
+    per node, 14 local states per cell) on the 64x32x32 hex grid x6 Tet4
+    (393,216 cells, 70,785 nodes, 283,140 dofs): E=1e3, nu=0.3, yield 1,
+    isotropic hardening 10, x- clamped, ux ramped to 0.008 on x+ over 10
+    pseudo-time steps, z = 1.  Reports the forward solve, the plastic-work
+    criterion with its path-dependent adjoint gradient, the condensed
+    assembly kernel (cells/s) and the oracle's condensed assembly on the host."""
+    mesh = pa.Mesh.box(*J2_DIMS, *J2_LENGTHS)
+    N, C, nnzb = mesh.num_nodes, mesh.num_cells, mesh.num_blocks
+    pl = pa.Plasticity(mesh, **J2_PARAMS)
+    xm, xp = mesh.node_set("x-"), mesh.node_set("x+")
+    dofs = np.concatenate([xm * 4, xm * 4 + 1, xm * 4 + 2, xp * 4])
+    vals = np.concatenate([np.zeros(3 * xm.size), np.full(xp.size, J2_END_DISPLACEMENT)])
+    z = pa.DeviceArray.from_numpy(np.ones(N))
+    # flexible GMRES(200) to 1e-8 relative, preconditioned by the 4x4-block
+    # geometric multigrid with 4 pre- and post-smoothing sweeps
+    prob = pa.PlasticityProblem(pl, dofs, vals, num_steps=J2_STEPS, max_newton=10, linear_tol=1e-8, restart=200)
+    if os.environ.get("PA_J2_PRECOND", "multigrid") == "multigrid":
+        prob.set_multigrid(*J2_DIMS, pre_sweeps=4, post_sweeps=4)
+    pa.synchronize()
+    t0 = time.perf_counter()
+    converged = prob.solution(z)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    F = prob.criterion_value(z, 1.0)
+    g = prob.criterion_gradient(z, 1.0)
+    pa.synchronize()
+    t_adj = time.perf_counter() - t0
+    stats = [prob.stats(k) for k in range(J2_STEPS)]
+    u, c, pi = prob.state(J2_STEPS - 1)
+    cp = prob.state(J2_STEPS - 2)[1]
+    plastic = int((c.reshape(-1, 14)[:, 1] > 0).sum())
+    du, dc, dcp, dpi = (pa.DeviceArray.from_numpy(a) for a in (u, c, cp, pi))
+    values = pa.DeviceArray(16 * nnzb)
+    R = pa.DeviceArray(4 * N)
+
+    def cond():
+        pl.condensed_system(du, dc, dcp, dpi, z, values=values, R=R, stream=stream)
+
+    cond()
+    stream.synchronize()
+    a, b = pa.Event(), pa.Event()
+    reps = 10
+    a.record(stream)
+    for _ in range(reps):
+        cond()
+    b.record(stream)
+    stream.synchronize()
+    t_cond = a.elapsed_ms(b) / reps
+    # algorithmic bytes of one condensed assembly: u, c, c_prev, z, coords,
+    # connectivity, pi in; block values, graph indices and R out
+    alg = C * (16 + 2 * 112) + N * (32 + 8 + 32 + 24 + 32) + nnzb * (128 + 4) + 4 * (N + 1)
+    out = {"workload": f"J2 plasticity mixed u-p Tet4, {J2_DIMS[0]}x{J2_DIMS[1]}x{J2_DIMS[2]} hex x6 "
+                       f"({C} cells, {N} nodes, {4 * N} dof), {J2_STEPS} steps to ux={J2_END_DISPLACEMENT}",
+           "solver": "condensed Newton; FGMRES(200) to 1e-8, 4x4-block geometric multigrid (4+4 sweeps)",
+           "forward_solve_s": t_fwd, "converged": converged,
+           "criterion_value_and_adjoint_gradient_s": t_adj, "plastic_work": F,
+           "gradient_norm": float(np.linalg.norm(g.numpy())),
+           "newton_iterations": [s["newton_iterations"] for s in stats],
+           "linear_iterations": [s["linear_iterations"] for s in stats],
+           "plastic_cells_last_step": plastic,
+           "condensed_assembly_ms": t_cond, "condensed_assembly_cells_per_s": C / (t_cond * 1e-3),
+           "condensed_assembly_algorithmic_GBs": alg / (t_cond * 1e-3) / 1e9}
+    log(f"bench: J2 {out}")
+    if with_cpu:
+        import oracle
+        threads = oracle.set_threads(int(os.environ.get("PA_CPU_THREADS", min(16, os.cpu_count() or 1))))
+        coords, conn = oracle.box_mesh(*J2_DIMS, *J2_LENGTHS)
+        om = oracle.J2Model(coords, conn, **J2_PARAMS)
+        zh = np.ones(N)
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            om.condensed_system(u, c, cp, pi, zh)
+            times.append(time.perf_counter() - t0)
+        tc = float(np.median(times))
+        out["cpu_baseline_condensed"] = {"value": C / tc, "unit": "cells/s", "cores": threads, "kind": "port",
+                                         "sample": "oracle condensed Newton system (FAD Jacobians, LU inverse, "
+                                                   "row-search atomic scatter) at the last step's state, "
+                                                   "full grid, median of 3"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,9 +207,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-solve", action="store_true", help="skip the compliance + gradient leg")
-    ap.add_argument("--mode", default="bench", choices=["bench", "copy", "profile", "solve"],
+    ap.add_argument("--no-j2", action="store_true", help="skip the J2 plasticity leg")
+    ap.add_argument("--mode", default="bench", choices=["bench", "copy", "profile", "solve", "j2"],
                     help="copy: run only the calibration copy kernel; profile: assembly kernels only; "
-                         "solve: one multigrid compliance + gradient solve only")
+                         "solve: one multigrid compliance + gradient solve only; j2: the J2 plasticity leg only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -140,6 +230,9 @@ def main():
             b.copy_kernel_from(a, stream)
         stream.synchronize()
         log(f"copy kernel: {args.warmup + args.steps} x {nbytes} bytes")
+        return
+    if args.mode == "j2":
+        print(json.dumps(j2_plasticity(pa, stream, with_cpu=not args.no_cpu)), flush=True)
         return
 
     name = "elastic_128x64x64"
@@ -243,6 +336,8 @@ def main():
     }
     if not args.no_solve:
         out["compliance_gradient"] = compliance_gradient(pa, mesh, vf, z, dims)
+    if not args.no_j2:
+        out["j2_plasticity"] = j2_plasticity(pa, stream, with_cpu=not args.no_cpu)
     if not args.no_cpu:
         log("bench: CPU baseline (oracle, OpenMP)")
         out["cpu_baseline"] = cpu_baseline()

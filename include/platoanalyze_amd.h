@@ -391,13 +391,16 @@ int pa_j2_plastic_work(pa_plasticity* p, const double* u_dev, const double* u_pr
  * adjoint), the plastic-work criterion summed over steps and its total
  * gradient by the path-dependent adjoint (src/PathDependentAdjointSolver.hpp).
  */
+enum { PA_KRYLOV_GMRES = 0, PA_KRYLOV_BICGSTAB = 1 };
 typedef struct {
     int num_steps;                  /* Initial Num. Pseudo Time Steps (default 20) */
     int max_newton;                 /* Maximum Number Iterations (default 10) */
     double newton_stop_tol;         /* Stopping Tolerance (default 1e-8) */
     double newton_current_norm_tol; /* Current Residual Norm Stopping Tolerance (default 1e-8) */
-    int linear_max_iter;            /* BiCGStab iteration cap (default 20000) */
-    double linear_tol;              /* BiCGStab relative residual (default 1e-12) */
+    int linear_max_iter;            /* Krylov iteration cap (default 20000) */
+    double linear_tol;              /* Krylov relative residual (default 1e-12) */
+    int krylov;                     /* PA_KRYLOV_GMRES (flexible, restarted; default) or PA_KRYLOV_BICGSTAB */
+    int restart;                    /* GMRES restart length (default 50) */
 } pa_plasticity_options;
 void pa_plasticity_options_default(pa_plasticity_options* options);
 
@@ -406,6 +409,12 @@ int pa_plasticity_problem_create(pa_plasticity* p, int num_bcs, const int* bc_do
                                  const double* bc_values_host, const pa_plasticity_options* options,
                                  pa_plasticity_problem** problem);
 int pa_plasticity_problem_destroy(pa_plasticity_problem* problem);
+/* Precondition the BiCGStab solves with geometric multigrid on the nx x ny x nz
+ * box mesh (4x4 nodal blocks: damped block-Jacobi smoothing, Galerkin coarse
+ * operators, pivoted dense coarsest solve), rebuilt from every new tangent.
+ * No counterpart in the reference (its solver stacks are external). */
+int pa_plasticity_problem_set_multigrid(pa_plasticity_problem* problem, int nx, int ny, int nz,
+                                        const pa_multigrid_options* options);
 /* Forward solve for control z; returns PA_OK with *converged = 0 if a step's
  * Newton solve stopped without converging.  Synchronizes. */
 int pa_plasticity_problem_solution(pa_plasticity_problem* problem, const double* z_dev, int* converged,

@@ -72,7 +72,10 @@ class J2Params(ctypes.Structure):
 class PlasticityOptions(ctypes.Structure):
     _fields_ = [("num_steps", ctypes.c_int), ("max_newton", ctypes.c_int), ("newton_stop_tol", ctypes.c_double),
                 ("newton_current_norm_tol", ctypes.c_double), ("linear_max_iter", ctypes.c_int),
-                ("linear_tol", ctypes.c_double)]
+                ("linear_tol", ctypes.c_double), ("krylov", ctypes.c_int), ("restart", ctypes.c_int)]
+
+
+KRYLOV_GMRES, KRYLOV_BICGSTAB = 0, 1
 
 
 WRT_U, WRT_C, WRT_Z, WRT_C_PREV, WRT_PI = 0, 1, 2, 3, 4
@@ -160,6 +163,7 @@ def _load():
         "pa_plasticity_options_default": ([ctypes.POINTER(PlasticityOptions)], None),
         "pa_plasticity_problem_create": ([vp, i, vp, vp, ctypes.POINTER(PlasticityOptions), ctypes.POINTER(vp)], i),
         "pa_plasticity_problem_destroy": ([vp], i),
+        "pa_plasticity_problem_set_multigrid": ([vp, i, i, i, ctypes.POINTER(MultigridOptions)], i),
         "pa_plasticity_problem_solution": ([vp, vp, ip, vp], i),
         "pa_plasticity_problem_state": ([vp, i, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp)], i),
         "pa_plasticity_problem_stats": ([vp, i, ip, ip, ctypes.POINTER(d)], i),
@@ -746,17 +750,24 @@ class PlasticityProblem:
     adjoint gradient.  Dirichlet values are the end values of the load ramp."""
 
     def __init__(self, plasticity, bc_dofs, bc_values=None, *, num_steps=20, max_newton=10, newton_stop_tol=1e-8,
-                 newton_current_norm_tol=1e-8, linear_max_iter=20000, linear_tol=1e-12):
+                 newton_current_norm_tol=1e-8, linear_max_iter=20000, linear_tol=1e-12, krylov=KRYLOV_GMRES,
+                 restart=50):
         self.pl = plasticity
         dofs = np.ascontiguousarray(bc_dofs, np.int32)
         vals = np.zeros(dofs.size) if bc_values is None else np.ascontiguousarray(bc_values, np.float64)
         self.options = PlasticityOptions(num_steps, max_newton, newton_stop_tol, newton_current_norm_tol,
-                                         linear_max_iter, linear_tol)
+                                         linear_max_iter, linear_tol, krylov, restart)
         h = ctypes.c_void_p()
         _check(lib.pa_plasticity_problem_create(plasticity.handle, dofs.size, _np_ptr(dofs), _np_ptr(vals),
                                                 ctypes.byref(self.options), ctypes.byref(h)))
         self.handle = h
         self.num_steps = num_steps
+
+    def set_multigrid(self, nx, ny, nz, *, max_levels=0, pre_sweeps=2, post_sweeps=2, omega=0.6,
+                      max_coarse_nodes=1000):
+        """Geometric multigrid (4x4 nodal blocks) as the BiCGStab preconditioner."""
+        opts = MultigridOptions(max_levels, pre_sweeps, post_sweeps, omega, max_coarse_nodes)
+        _check(lib.pa_plasticity_problem_set_multigrid(self.handle, nx, ny, nz, ctypes.byref(opts)))
 
     def solution(self, z, stream=None):
         z = _dev(z)

@@ -38,10 +38,55 @@ struct Grid {
 
 __device__ inline int sidx(int dx, int dy, int dz) { return (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1); }
 
-// 3x3 (or 1x1) block inverse in double, stored as float.
+// Block inverse in double, stored as float: cofactors for 3x3, Gauss-Jordan
+// with partial pivoting for 4x4 (the nodal blocks of the mixed u-p tangent
+// are indefinite), reciprocal for 1x1.
 template <int D>
 __device__ inline void block_inverse(const double* a, float* out) {
-    if constexpr (D == 3) {
+    if constexpr (D == 4) {
+        double A[4][8];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                A[i][j] = a[i * 4 + j];
+                A[i][4 + j] = (i == j) ? 1.0 : 0.0;
+            }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            int piv = k;
+            double best = fabs(A[k][k]);
+#pragma unroll
+            for (int i = k + 1; i < 4; ++i)
+                if (fabs(A[i][k]) > best) {
+                    best = fabs(A[i][k]);
+                    piv = i;
+                }
+#pragma unroll
+            for (int i = k + 1; i < 4; ++i)
+                if (i == piv)
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const double tmp = A[k][j];
+                        A[k][j] = A[i][j];
+                        A[i][j] = tmp;
+                    }
+            const double inv = A[k][k] != 0.0 ? 1.0 / A[k][k] : 0.0;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) A[k][j] *= inv;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (i != k) {
+                    const double f = A[i][k];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) A[i][j] -= f * A[k][j];
+                }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) out[i * 4 + j] = (float)A[i][4 + j];
+    } else if constexpr (D == 3) {
         const double c00 = a[4] * a[8] - a[5] * a[7], c01 = a[2] * a[7] - a[1] * a[8], c02 = a[1] * a[5] - a[2] * a[4];
         const double c10 = a[5] * a[6] - a[3] * a[8], c11 = a[0] * a[8] - a[2] * a[6], c12 = a[2] * a[3] - a[0] * a[5];
         const double c20 = a[3] * a[7] - a[4] * a[6], c21 = a[1] * a[6] - a[0] * a[7], c22 = a[0] * a[4] - a[1] * a[3];
@@ -323,12 +368,50 @@ __global__ void k_dense_from_stencil(Grid g, const float* __restrict__ A, double
             }
 }
 
-// In-place Gauss-Jordan inverse of an SPD matrix (no pivoting), one workgroup.
+// In-place Gauss-Jordan inverse with partial pivoting (rows swapped in both
+// matrices; ties go to the lower row, so the result is deterministic), one
+// workgroup of 1024 threads.
 __global__ __launch_bounds__(1024) void k_gauss_jordan(int n, double* __restrict__ a, double* __restrict__ inv) {
     for (long t = threadIdx.x; t < (long)n * n; t += blockDim.x) inv[t] = (t / n == t % n) ? 1.0 : 0.0;
     __syncthreads();
     __shared__ double piv;
+    __shared__ double sval[1024];
+    __shared__ int sidx_[1024];
     for (int k = 0; k < n; ++k) {
+        double best = -1.0;
+        int bi = k;
+        for (int r = k + threadIdx.x; r < n; r += blockDim.x) {
+            const double v = fabs(a[(long)r * n + k]);
+            if (v > best) {
+                best = v;
+                bi = r;
+            }
+        }
+        sval[threadIdx.x] = best;
+        sidx_[threadIdx.x] = bi;
+        __syncthreads();
+        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) {
+                const double o = sval[threadIdx.x + s];
+                const int oi = sidx_[threadIdx.x + s];
+                if (o > sval[threadIdx.x] || (o == sval[threadIdx.x] && oi < sidx_[threadIdx.x])) {
+                    sval[threadIdx.x] = o;
+                    sidx_[threadIdx.x] = oi;
+                }
+            }
+            __syncthreads();
+        }
+        const int pr = sidx_[0];
+        if (pr != k)
+            for (int c = threadIdx.x; c < n; c += blockDim.x) {
+                double t0 = a[(long)k * n + c];
+                a[(long)k * n + c] = a[(long)pr * n + c];
+                a[(long)pr * n + c] = t0;
+                t0 = inv[(long)k * n + c];
+                inv[(long)k * n + c] = inv[(long)pr * n + c];
+                inv[(long)pr * n + c] = t0;
+            }
+        __syncthreads();
         if (threadIdx.x == 0) piv = a[(long)k * n + k];
         __syncthreads();
         const double ip = piv != 0.0 ? 1.0 / piv : 0.0;
@@ -506,6 +589,7 @@ void multigrid_apply(pa_multigrid* mg, const double* r, double* z, hipStream_t s
     k_to_f32<<<nblk(n), kT, 0, st>>>(n, r, mg->b[0]->p);
     chk();
     if (mg->dpn == 3) vcycle<3>(mg, 0, st);
+    else if (mg->dpn == 4) vcycle<4>(mg, 0, st);
     else vcycle<1>(mg, 0, st);
     k_to_f64<<<nblk(n), kT, 0, st>>>(n, mg->x[0]->p, z);
     chk();
@@ -521,7 +605,7 @@ int pa_multigrid_create(const pa_mesh* mesh, int nx, int ny, int nz, int dpn, co
                         pa_multigrid** out) {
     return guarded([&] {
         PA_REQUIRE(mesh && opt && out, PA_ERR_ARG, "pa_multigrid_create: null argument");
-        PA_REQUIRE(dpn == 1 || dpn == 3, PA_ERR_ARG, "pa_multigrid_create: dpn must be 1 or 3");
+        PA_REQUIRE(dpn == 1 || dpn == 3 || dpn == 4, PA_ERR_ARG, "pa_multigrid_create: dpn must be 1, 3 or 4");
         pa::Grid g0{nx, ny, nz};
         PA_REQUIRE(nx > 0 && ny > 0 && nz > 0 && g0.nodes() == mesh->N, PA_ERR_ARG,
                    "pa_multigrid_create: grid does not match the mesh node count");
@@ -623,6 +707,7 @@ int pa_multigrid_setup(pa_multigrid* mg, const void* vals, int precision, int nb
         }
         const float* v32 = (const float*)mg->fine_vals;
         if (mg->dpn == 3) pa::setup_impl<3, float>(mg, v32, st);
+        else if (mg->dpn == 4) pa::setup_impl<4, float>(mg, v32, st);
         else pa::setup_impl<1, float>(mg, v32, st);
         mg->ready = true;
     });

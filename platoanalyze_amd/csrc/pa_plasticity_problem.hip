@@ -14,6 +14,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include "pa_j2.h"
@@ -32,11 +34,17 @@ struct pa_plasticity_problem {
     // Newton / Krylov workspace
     pa::DevBuf<double> vals, R, rhs, du;
     pa::DevBuf<double> r, rh, pv, v, y, s, zz, t, minv, part, scal;
+    pa::DevBuf<double> gV, gZ, gpart, gh, gy;  // GMRES basis, preconditioned basis, partials, Hessenberg column, y
     // adjoint workspace
     pa::DevBuf<double> wu, wu_next, gc, gcp, gcp_next, zc, zc_out, blocal, mu, mu_next, lam, gam, gam_next;
     std::vector<int> newton_its, linear_its;
     std::vector<double> res_norms;
     bool solved = false;
+    bool verbose = false;  // PA_PLASTICITY_VERBOSE set: one stderr line per Newton iteration
+    pa_multigrid* mg = nullptr;  // owned; preconditioner of the linear solves when set
+    ~pa_plasticity_problem() {
+        if (mg) pa_multigrid_destroy(mg);
+    }
 };
 
 namespace pa {
@@ -216,6 +224,7 @@ __global__ __launch_bounds__(kB) void k_bicg_dir(int N, const double* __restrict
             pn[i] = r[o] + beta * (p[o] - omega * v[o]);
             p[o] = pn[i];
         }
+        if (!minv) continue;  // multigrid: y = M p is applied afterwards
         prec4(minv, n, pn, yn);
 #pragma unroll
         for (int i = 0; i < 4; ++i) y[n * 4 + i] = yn[i];
@@ -235,6 +244,7 @@ __global__ __launch_bounds__(kB) void k_bicg_s(int N, const double* __restrict__
             sn[i] = r[o] - alpha * v[o];
             s[o] = sn[i];
         }
+        if (!minv) continue;  // multigrid: z = M s is applied afterwards
         prec4(minv, n, sn, zn);
 #pragma unroll
         for (int i = 0; i < 4; ++i) zz[n * 4 + i] = zn[i];
@@ -317,8 +327,14 @@ int bicgstab(pa_plasticity_problem* P, const double* b, double* x, hipStream_t s
     const int nvb = vec_blocks(P), nsb = row_blocks(P);
     const int np = std::max(nvb, nsb);
     const double* A = P->vals.p;
-    k_block_inverse4<<<blocks(N), kB, 0, st>>>(N, m->d_rowmap.p, m->d_colidx.p, A, P->minv.p);
-    check();
+    const double* minv = P->mg ? nullptr : P->minv.p;
+    if (P->mg) {
+        const int rc = pa_multigrid_setup(P->mg, A, PA_FP64, P->nbc, P->bc_dofs.p, st);
+        if (rc != PA_OK) throw Error(rc, pa_last_error());
+    } else {
+        k_block_inverse4<<<blocks(N), kB, 0, st>>>(N, m->d_rowmap.p, m->d_colidx.p, A, P->minv.p);
+        check();
+    }
     k_spmv4<<<nsb, kB, 0, st>>>(N, m->d_rowmap.p, m->d_colidx.p, A, x, P->t.p, nullptr, nullptr, P->part.p, np);
     check();
     k_bicg_init<<<nvb, kB, 0, st>>>(n, b, P->t.p, P->r.p, P->rh.p, P->pv.p, P->v.p, P->part.p, np);
@@ -335,15 +351,17 @@ int bicgstab(pa_plasticity_problem* P, const double* b, double* x, hipStream_t s
     while (it < P->opt.linear_max_iter && bnorm > 0.0 && std::sqrt(rr) > target) {
         const int burst = std::min(check_every, P->opt.linear_max_iter - it);
         for (int q = 0; q < burst; ++q) {
-            k_bicg_dir<<<nvb, kB, 0, st>>>(N, P->scal.p, P->minv.p, P->r.p, P->v.p, P->pv.p, P->y.p);
+            k_bicg_dir<<<nvb, kB, 0, st>>>(N, P->scal.p, minv, P->r.p, P->v.p, P->pv.p, P->y.p);
             check();
+            if (P->mg) multigrid_apply(P->mg, P->pv.p, P->y.p, st);
             k_spmv4<<<nsb, kB, 0, st>>>(N, m->d_rowmap.p, m->d_colidx.p, A, P->y.p, P->v.p, P->rh.p, nullptr,
                                         P->part.p, np);
             check();
             k_reduce<<<1, 1024, 0, st>>>(P->part.p, nsb, np, 1, S_RV, 0, 0, P->scal.p);
             check();
-            k_bicg_s<<<nvb, kB, 0, st>>>(N, P->scal.p, P->minv.p, P->r.p, P->v.p, P->s.p, P->zz.p);
+            k_bicg_s<<<nvb, kB, 0, st>>>(N, P->scal.p, minv, P->r.p, P->v.p, P->s.p, P->zz.p);
             check();
+            if (P->mg) multigrid_apply(P->mg, P->s.p, P->zz.p, st);
             k_spmv4<<<nsb, kB, 0, st>>>(N, m->d_rowmap.p, m->d_colidx.p, A, P->zz.p, P->t.p, P->s.p, P->t.p,
                                         P->part.p, np);
             check();
@@ -360,6 +378,207 @@ int bicgstab(pa_plasticity_problem* P, const double* b, double* x, hipStream_t s
         if (!std::isfinite(rr)) break;
     }
     return it;
+}
+
+// ---------------------------------------------------------------------------
+// flexible restarted GMRES (right preconditioning; the FP32 multigrid cycle
+// need not be an exactly linear operator), classical Gram-Schmidt with one
+// re-orthogonalization; the (m+1) x m Hessenberg least-squares problem and
+// its Givens rotations live on the host
+// ---------------------------------------------------------------------------
+// part[i*nparts + block] = partial sums of w . V_i, i < k (V_i at V + i*n)
+__global__ __launch_bounds__(kB) void k_mdot(long n, int k, const double* __restrict__ V,
+                                             const double* __restrict__ w, double* __restrict__ part, int nparts) {
+    __shared__ double sh[kB / 64];
+    for (int i = 0; i < k; ++i) {
+        const double* vi = V + (long)i * n;
+        double s = 0.0;
+        for (long t = (long)blockIdx.x * kB + threadIdx.x; t < n; t += (long)gridDim.x * kB) s += w[t] * vi[t];
+        const double tt = block_sum(s, sh);
+        if (threadIdx.x == 0) part[(long)i * nparts + blockIdx.x] = tt;
+    }
+}
+
+// h[i] (+)= sum over the nb partials of array i; one block per i, fixed order.
+__global__ void k_reduce_rows(const double* __restrict__ part, int nb, int nparts, int accumulate,
+                              double* __restrict__ h) {
+    __shared__ double sh[1024 / 64];
+    const int i = blockIdx.x;
+    double s = 0.0;
+    for (int t = threadIdx.x; t < nb; t += blockDim.x) s += part[(long)i * nparts + t];
+    const double tt = block_sum(s, sh);
+    if (threadIdx.x == 0) h[i] = accumulate ? h[i] + tt : tt;
+}
+
+// w -= sum_{i<k} h[i] V_i
+__global__ __launch_bounds__(kB) void k_mupdate(long n, int k, const double* __restrict__ V,
+                                                const double* __restrict__ h, double* __restrict__ w) {
+    for (long t = (long)blockIdx.x * kB + threadIdx.x; t < n; t += (long)gridDim.x * kB) {
+        double s = w[t];
+        for (int i = 0; i < k; ++i) s -= h[i] * V[(long)i * n + t];
+        w[t] = s;
+    }
+}
+
+// x += sum_{i<k} y[i] Z_i
+__global__ __launch_bounds__(kB) void k_madd(long n, int k, const double* __restrict__ Z,
+                                             const double* __restrict__ y, double* __restrict__ x) {
+    for (long t = (long)blockIdx.x * kB + threadIdx.x; t < n; t += (long)gridDim.x * kB) {
+        double s = x[t];
+        for (int i = 0; i < k; ++i) s += y[i] * Z[(long)i * n + t];
+        x[t] = s;
+    }
+}
+
+__global__ void k_scale(long n, double f, double* v) {
+    for (long t = (long)blockIdx.x * kB + threadIdx.x; t < n; t += (long)gridDim.x * kB) v[t] *= f;
+}
+
+__global__ void k_sub(long n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ c) {
+    for (long t = (long)blockIdx.x * kB + threadIdx.x; t < n; t += (long)gridDim.x * kB) c[t] = a[t] - b[t];
+}
+
+__global__ void k_block_prec4(int N, const double* __restrict__ minv, const double* __restrict__ r,
+                              double* __restrict__ z) {
+    for (long n = (long)blockIdx.x * kB + threadIdx.x; n < N; n += (long)gridDim.x * kB) {
+        double in[4] = {r[n * 4], r[n * 4 + 1], r[n * 4 + 2], r[n * 4 + 3]}, out[4];
+        prec4(minv, n, in, out);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) z[n * 4 + i] = out[i];
+    }
+}
+
+void preconditioner_setup(pa_plasticity_problem* P, hipStream_t st) {
+    const pa_mesh* m = P->p->mesh;
+    if (P->mg) {
+        const int rc = pa_multigrid_setup(P->mg, P->vals.p, PA_FP64, P->nbc, P->bc_dofs.p, st);
+        if (rc != PA_OK) throw Error(rc, pa_last_error());
+    } else {
+        k_block_inverse4<<<blocks(m->N), kB, 0, st>>>(m->N, m->d_rowmap.p, m->d_colidx.p, P->vals.p, P->minv.p);
+        check();
+    }
+}
+
+void apply_preconditioner(pa_plasticity_problem* P, const double* r, double* z, hipStream_t st) {
+    if (P->mg) {
+        multigrid_apply(P->mg, r, z, st);
+    } else {
+        k_block_prec4<<<vec_blocks(P), kB, 0, st>>>(P->p->mesh->N, P->minv.p, r, z);
+        check();
+    }
+}
+
+int gmres(pa_plasticity_problem* P, const double* b, double* x, hipStream_t st) {
+    const pa_mesh* m = P->p->mesh;
+    const int N = m->N;
+    const long n = 4L * N;
+    const int M = P->opt.restart;
+    const int nvb = vec_blocks(P), nsb = row_blocks(P), np = std::max(nvb, nsb);
+    double* V = P->gV.p;
+    double* Z = P->gZ.p;
+    preconditioner_setup(P, st);
+    const double bnorm = norm(P, b, st);
+    if (!(bnorm > 0.0)) return 0;
+    const double target = P->opt.linear_tol * bnorm;
+    std::vector<double> H((M + 1) * M), cs(M), sn(M), g(M + 1), y(M), hcol(M + 1);
+    int it = 0;
+    while (it < P->opt.linear_max_iter) {
+        // r = b - A x into V_0
+        k_spmv4<<<nsb, kB, 0, st>>>(N, m->d_rowmap.p, m->d_colidx.p, P->vals.p, x, P->t.p, nullptr, nullptr,
+                                    P->part.p, np);
+        check();
+        k_sub<<<nvb, kB, 0, st>>>(n, b, P->t.p, V);
+        check();
+        const double beta = norm(P, V, st);
+        if (!std::isfinite(beta)) return it;
+        if (beta <= target) break;
+        k_scale<<<nvb, kB, 0, st>>>(n, 1.0 / beta, V);  // V_0 = r / beta
+        check();
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int j = 0;
+        bool done = false;
+        for (; j < M && it < P->opt.linear_max_iter; ++j, ++it) {
+            double* vj = V + (long)j * n;
+            double* zj = Z + (long)j * n;
+            double* w = V + (long)(j + 1) * n;
+            apply_preconditioner(P, vj, zj, st);
+            k_spmv4<<<nsb, kB, 0, st>>>(N, m->d_rowmap.p, m->d_colidx.p, P->vals.p, zj, w, nullptr, nullptr,
+                                        P->part.p, np);
+            check();
+            // two passes of classical Gram-Schmidt against V_0..V_j
+            for (int pass = 0; pass < 2; ++pass) {
+                k_mdot<<<nvb, kB, 0, st>>>(n, j + 1, V, w, P->gpart.p, nvb);
+                check();
+                k_reduce_rows<<<j + 1, 1024, 0, st>>>(P->gpart.p, nvb, nvb, 0, P->gh.p + (pass ? M + 1 : 0));
+                check();
+                k_mupdate<<<nvb, kB, 0, st>>>(n, j + 1, V, P->gh.p + (pass ? M + 1 : 0), w);
+                check();
+            }
+            const double hn = norm(P, w, st);  // synchronizes
+            std::vector<double> h1(j + 1), h2(j + 1);
+            PA_HIP(hipMemcpy(h1.data(), P->gh.p, (j + 1) * sizeof(double), hipMemcpyDeviceToHost));
+            PA_HIP(hipMemcpy(h2.data(), P->gh.p + M + 1, (j + 1) * sizeof(double), hipMemcpyDeviceToHost));
+            for (int i = 0; i <= j; ++i) hcol[i] = h1[i] + h2[i];
+            hcol[j + 1] = hn;
+            if (!std::isfinite(hn)) return it;
+            if (hn > 0.0) {
+                k_scale<<<nvb, kB, 0, st>>>(n, 1.0 / hn, w);  // V_{j+1} = w / hn
+                check();
+            }
+            // apply the previous rotations, then a new one zeroing H[j+1][j]
+            for (int i = 0; i < j; ++i) {
+                const double a = hcol[i], c = hcol[i + 1];
+                hcol[i] = cs[i] * a + sn[i] * c;
+                hcol[i + 1] = -sn[i] * a + cs[i] * c;
+            }
+            const double a = hcol[j], c = hcol[j + 1];
+            const double rr = std::hypot(a, c);
+            cs[j] = rr > 0.0 ? a / rr : 1.0;
+            sn[j] = rr > 0.0 ? c / rr : 0.0;
+            hcol[j] = rr;
+            hcol[j + 1] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            for (int i = 0; i <= j; ++i) H[i * M + j] = hcol[i];
+            if (std::fabs(g[j + 1]) <= target || hn == 0.0) {
+                ++j;
+                ++it;
+                done = true;
+                break;
+            }
+        }
+        // y = R^-1 g (upper triangular j x j), x += Z y
+        for (int i = j - 1; i >= 0; --i) {
+            double s = g[i];
+            for (int k2 = i + 1; k2 < j; ++k2) s -= H[i * M + k2] * y[k2];
+            y[i] = s / H[i * M + i];
+        }
+        PA_HIP(hipMemcpyAsync(P->gy.p, y.data(), j * sizeof(double), hipMemcpyHostToDevice, st));
+        k_madd<<<nvb, kB, 0, st>>>(n, j, Z, P->gy.p, x);
+        check();
+        PA_HIP(hipStreamSynchronize(st));
+        if (done) break;
+    }
+    return it;
+}
+
+__global__ void k_copy_bc(int nbc, const int* __restrict__ dofs, const double* __restrict__ b, double* __restrict__ x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nbc) x[dofs[i]] = b[dofs[i]];
+}
+
+// A x = b for the constrained tangent.  The Dirichlet rows are identity rows
+// (their columns zeroed) whose scale is unrelated to the stiffness rows, so a
+// relative residual test cannot see their error; x starts at b on those dofs
+// and every search direction is zero there (both preconditioners and A map
+// such vectors to such vectors), so they stay exact.
+int linear_solve(pa_plasticity_problem* P, const double* b, double* x, hipStream_t st) {
+    if (P->nbc) {
+        k_copy_bc<<<(P->nbc + 255) / 256, 256, 0, st>>>(P->nbc, P->bc_dofs.p, b, x);
+        check();
+    }
+    return P->opt.krylov == PA_KRYLOV_BICGSTAB ? bicgstab(P, b, x, st) : gmres(P, b, x, st);
 }
 
 double* step_u(pa_plasticity_problem* P, int k) { return P->U.p + (long)k * 4 * P->p->mesh->N; }
@@ -419,7 +638,9 @@ bool solve_forward(pa_plasticity_problem* P, const double* z, hipStream_t st) {
             }
             if (nrm >= 1e20 || it >= P->opt.max_newton || !std::isfinite(nrm)) break;
             PA_HIP(hipMemsetAsync(P->du.p, 0, nu * sizeof(double), st));
-            lin += bicgstab(P, P->rhs.p, P->du.p, st);
+            const int li = linear_solve(P, P->rhs.p, P->du.p, st);
+            lin += li;
+            if (P->verbose) std::fprintf(stderr, "plasticity: step %d newton %d |r| %.6e linear %d\n", k, it, nrm, li);
             k_axpy<<<vec_blocks(P), kB, 0, st>>>(nu, 1.0, P->du.p, uk);
             check();
             update_local_state(P->p, uk, cp, z, ck, st);
@@ -468,7 +689,7 @@ void criterion_gradient(pa_plasticity_problem* P, const double* z, double mult, 
         condensed_system(P->p, u, c, cp, pi, z, P->vals.p, nullptr, true, st);
         constraints(P, P->rhs.p, P->bc_zero.p, st);
         PA_HIP(hipMemsetAsync(P->lam.p, 0, nu * sizeof(double), st));
-        bicgstab(P, P->rhs.p, P->lam.p, st);
+        linear_solve(P, P->rhs.p, P->lam.p, st);
         adjoint_local_update(P->p, u, c, cp, pi, z, P->lam.p, P->blocal.p, P->zc.p, P->mu.p, P->zc_out.p, st);
         if (k == 0) {
             PA_HIP(hipMemsetAsync(P->gam.p, 0, 3L * m->N * sizeof(double), st));
@@ -502,6 +723,8 @@ void pa_plasticity_options_default(pa_plasticity_options* o) {
     o->newton_current_norm_tol = 1e-8;
     o->linear_max_iter = 20000;
     o->linear_tol = 1e-12;
+    o->krylov = PA_KRYLOV_GMRES;
+    o->restart = 50;
 }
 
 int pa_plasticity_problem_create(pa_plasticity* p, int nbc, const int* dofs, const double* vals,
@@ -512,9 +735,12 @@ int pa_plasticity_problem_create(pa_plasticity* p, int nbc, const int* dofs, con
         auto* P = new pa_plasticity_problem();
         try {
             P->p = p;
+            P->verbose = std::getenv("PA_PLASTICITY_VERBOSE") != nullptr;
             if (opt) P->opt = *opt;
             else pa_plasticity_options_default(&P->opt);
-            PA_REQUIRE(P->opt.num_steps >= 1 && P->opt.max_newton >= 0 && P->opt.linear_max_iter >= 1, PA_ERR_ARG,
+            PA_REQUIRE(P->opt.num_steps >= 1 && P->opt.max_newton >= 0 && P->opt.linear_max_iter >= 1 &&
+                           P->opt.restart >= 1 && P->opt.restart <= 1000 &&
+                           (P->opt.krylov == PA_KRYLOV_GMRES || P->opt.krylov == PA_KRYLOV_BICGSTAB), PA_ERR_ARG,
                        "pa_plasticity_problem_create: bad options");
             const pa_mesh* m = p->mesh;
             const long nu = 4L * m->N;
@@ -542,6 +768,14 @@ int pa_plasticity_problem_create(pa_plasticity* p, int nbc, const int* dofs, con
                             &P->lam})
                 b->alloc(nu);
             P->minv.alloc(16L * m->N);
+            {
+                const long M = P->opt.restart;
+                P->gV.alloc((M + 1) * nu);
+                P->gZ.alloc(M * nu);
+                P->gpart.alloc((M + 1) * pa::j2::vec_blocks(P));
+                P->gh.alloc(2 * (M + 1));
+                P->gy.alloc(M);
+            }
             const int np = std::max(pa::j2::vec_blocks(P), pa::j2::row_blocks(P));
             P->part.alloc(2L * np);
             P->scal.alloc(pa::j2::S_N);
@@ -561,6 +795,25 @@ int pa_plasticity_problem_create(pa_plasticity* p, int nbc, const int* dofs, con
 
 int pa_plasticity_problem_destroy(pa_plasticity_problem* P) {
     return guarded([&] { delete P; });
+}
+
+int pa_plasticity_problem_set_multigrid(pa_plasticity_problem* P, int nx, int ny, int nz,
+                                        const pa_multigrid_options* opt) {
+    return guarded([&] {
+        PA_REQUIRE(P, PA_ERR_ARG, "pa_plasticity_problem_set_multigrid: null argument");
+        pa_multigrid_options o{};
+        if (opt) o = *opt;
+        else {
+            o.pre_sweeps = 2;
+            o.post_sweeps = 2;
+            o.omega = 0.6;
+        }
+        pa_multigrid* mg = nullptr;
+        const int rc = pa_multigrid_create(P->p->mesh, nx, ny, nz, 4, &o, &mg);
+        if (rc != PA_OK) throw pa::Error(rc, pa_last_error());
+        if (P->mg) pa_multigrid_destroy(P->mg);
+        P->mg = mg;
+    });
 }
 
 int pa_plasticity_problem_solution(pa_plasticity_problem* P, const double* z, int* converged, void* stream) {

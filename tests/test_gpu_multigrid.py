@@ -166,3 +166,90 @@ def test_multigrid_thermal(pa):
     u2 = P2.solution(z).numpy()
     assert P2.last_stats["converged"], P2.last_stats
     assert rel(u2, u1) < 1e-7
+
+
+def j2_tangent(pa, dims):
+    """Condensed J2 tangent (4x4 blocks) of the first elastic Newton step on a
+    box clamped at x- with ux prescribed at x+, constrained."""
+    mesh = pa.Mesh.box(*dims, 2.0, 1.0, 1.0)
+    coords, conn = oracle.box_mesh(*dims, 2.0, 1.0, 1.0)
+    om = oracle.J2Model(coords, conn, youngs=1e3, poisson=0.3, hardening_isotropic=10.0, hardening_kinematic=0.0,
+                        initial_yield_stress=1.0)
+    N = om.N
+    xm, xp = mesh.node_set("x-"), mesh.node_set("x+")
+    dofs = np.concatenate([xm * 4, xm * 4 + 1, xm * 4 + 2, xp * 4]).astype(np.int32)
+    g = np.concatenate([np.zeros(3 * xm.size), np.full(xp.size, 8e-4)])
+    z = np.ones(N)
+    c = om.update_local_state(np.zeros(4 * N), np.zeros(om.C * 14), z)
+    vals, R = om.condensed_system(np.zeros(4 * N), c, np.zeros(om.C * 14), np.zeros(3 * N), z)
+    rhs = -R
+    oracle.apply_block_constraints(4, om.rowmap, om.colidx, vals, rhs, dofs, g)
+    return mesh, om, vals, rhs, dofs
+
+
+def test_multigrid_4x4_blocks_galerkin_and_vcycle(pa):
+    print("case: 4x4-block multigrid on the condensed J2 tangent, 8x4x4", flush=True)
+    dims = (8, 4, 4)
+    mesh, om, vals, rhs, dofs = j2_tangent(pa, dims)
+    K = pa.CrsMatrix(mesh, 4, 4, pa.DeviceArray.from_numpy(vals), pa.FP64)
+    mg = pa.Multigrid(mesh, *dims, 4)
+    mg.setup(K, dofs)
+    cdims, st = mg.level_operator(1)
+    mask = np.ones(4 * om.N, bool)
+    mask[dofs] = False
+    P = prolongation4(dims, mask)
+    A = oracle.to_scipy(4, 4, om.rowmap, om.colidx, vals)
+    ref = (P.T @ A @ P).toarray()
+    got = stencil_to_dense(cdims, st)
+    assert rel(got, ref) < 1e-5
+    z = mg.apply(pa.DeviceArray.from_numpy(rhs), pa.DeviceArray(rhs.size)).numpy()
+    assert np.all(np.isfinite(z))
+
+
+def prolongation4(dims, mask):
+    nx, ny, nz = dims
+    cx, cy, cz = nx // 2, ny // 2, nz // 2
+    fid = lambda i, j, k: (i * (ny + 1) + j) * (nz + 1) + k
+    cid = lambda i, j, k: (i * (cy + 1) + j) * (cz + 1) + k
+
+    def w1(f):
+        return [(f // 2, 1.0)] if f % 2 == 0 else [((f - 1) // 2, 0.5), ((f + 1) // 2, 0.5)]
+
+    rows, cols, vals = [], [], []
+    for i in range(nx + 1):
+        for j in range(ny + 1):
+            for k in range(nz + 1):
+                for I, wi in w1(i):
+                    for J, wj in w1(j):
+                        for K, wk in w1(k):
+                            for d in range(4):
+                                f = fid(i, j, k) * 4 + d
+                                if mask[f]:
+                                    rows.append(f)
+                                    cols.append(cid(I, J, K) * 4 + d)
+                                    vals.append(wi * wj * wk)
+    nf = (nx + 1) * (ny + 1) * (nz + 1) * 4
+    nc = (cx + 1) * (cy + 1) * (cz + 1) * 4
+    return sp.csr_matrix((vals, (rows, cols)), shape=(nf, nc))
+
+
+def test_multigrid_preconditioned_plasticity_newton(pa):
+    print("case: J2 10-step Newton with multigrid vs block Jacobi, 16x8x8", flush=True)
+    dims = (16, 8, 8)
+    mesh = pa.Mesh.box(*dims, 2.0, 1.0, 1.0)
+    xm, xp = mesh.node_set("x-"), mesh.node_set("x+")
+    dofs = np.concatenate([xm * 4, xm * 4 + 1, xm * 4 + 2, xp * 4])
+    # ramp to 0.004 in 10 steps: plastic from step 4 on, within the reference Newton's range
+    g = np.concatenate([np.zeros(3 * xm.size), np.full(xp.size, 0.004)])
+    z = np.ones(mesh.num_nodes)
+    sols = []
+    for use_mg in (False, True):
+        pl = pa.Plasticity(mesh, youngs=1e3, poisson=0.3, hardening_isotropic=10.0, initial_yield_stress=1.0)
+        prob = pa.PlasticityProblem(pl, dofs, g, num_steps=10, max_newton=10, linear_tol=1e-10)
+        if use_mg:
+            prob.set_multigrid(*dims)
+        ok = prob.solution(z)
+        print("multigrid" if use_mg else "block jacobi", [prob.stats(k) for k in range(10)], flush=True)
+        assert ok
+        sols.append(prob.state(9)[0])
+    assert rel(sols[1], sols[0]) < 1e-7
