@@ -2,7 +2,8 @@
  * platoanalyze_amd.h — C ABI of the MI355X-native Tet4 assembly library.
  *
  * Drop-in boundary for Plato Analyze's workset residual / Jacobian evaluation
- * and assembly on linear tetrahedra (SIMP elastostatics and steady thermal).
+ * and assembly on linear tetrahedra (SIMP elastostatics, steady thermal and
+ * J2 plasticity).
  * Every entry point returns an int status (PA_OK = 0) and never throws; the
  * message of the last failure on the calling thread is pa_last_error().
  * Array arguments named *_dev are device pointers (hipMalloc'd, e.g. through
@@ -386,7 +387,8 @@ int pa_j2_plastic_work(pa_plasticity* p, const double* u_dev, const double* u_pr
  * steps with load factor (k+1)/K on the Dirichlet values, the condensed Newton
  * solve per step (elastic trial state, Dirichlet increment on the first
  * iteration, absolute-residual stopping test, src/NewtonRaphsonSolver.hpp:555-610)
- * with BiCGStab + 4x4 block-Jacobi linear solves, the projection update after
+ * with flexible GMRES (or BiCGStab) linear solves preconditioned by 4x4 block
+ * Jacobi or geometric multigrid, the projection update after
  * each step, every step's states kept on the device (the checkpoints of the
  * adjoint), the plastic-work criterion summed over steps and its total
  * gradient by the path-dependent adjoint (src/PathDependentAdjointSolver.hpp).

@@ -1,11 +1,11 @@
 // PlasticityProblem on the device: pseudo-time steps, the condensed Newton
-// solve of every step with BiCGStab linear solves, the projection update, the
-// plastic-work criterion and its path-dependent adjoint gradient.
+// solve of every step, the projection update, the plastic-work criterion and
+// its path-dependent adjoint gradient.
 //
-// BiCGStab keeps its scalars on the device (fixed-order reductions, so runs
-// are deterministic); the host reads ||r||^2 every `check_every` iterations.
-// Preconditioner: inverses of the 4x4 diagonal blocks (partial pivoting),
-// i.e. the nodal (ux, uy, uz, p) blocks of the saddle-point tangent.
+// Linear solves: restarted flexible GMRES (default) or BiCGStab, with the
+// inverses of the 4x4 nodal (ux, uy, uz, p) diagonal blocks of the
+// saddle-point tangent or the geometric multigrid on 4x4 blocks as the
+// preconditioner.  Reductions are fixed-order, so runs are deterministic.
 //
 // Every step's u, c and pi stay on the device: they are the checkpoints the
 // backward recursion reads (a 64x32x32-hex problem with 10 steps holds about
