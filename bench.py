@@ -306,7 +306,7 @@ def main():
     jac_bytes = algorithmic_bytes(C, N, nnzb, 3, 8, with_residual=False)
     step_bytes = algorithmic_bytes(C, N, nnzb, 3, 8)
     achieved = jac_bytes / (t_jac * 1e-3) / 1e9
-    traffic = read_traffic("k_jacobian_pattern_fp64_elastic_128x64x64")
+    traffic = read_traffic("k_tile_jacobian_fp64_elastic_128x64x64")
     cells_s = C / (ms_step * 1e-3)
     out = {
         "metric": "cells/s res+Jac assembly",
@@ -326,7 +326,7 @@ def main():
                    "cells": C, "nodes": N, "blocks": nnzb, "precision": "fp64"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_jacobian_pattern<3,double,false,24> (gradient_u, FP64)",
+                     "kernel": "k_tile_jacobian<3,double,false,6> (gradient_u, FP64)",
                      "algorithmic_bytes_per_launch": jac_bytes, "avg_launch_ms": t_jac},
         "kernels_ms": {"gradient_u_fp64": t_jac, "value": t_val, "gradient_u_fp32": t_jac32,
                        "value_gradient_u_fused_fp64": t_fused, "value_gradient_u_fused_fp32": t_fused32},

@@ -88,6 +88,13 @@ int pa_mesh_create(const double* coords_host, int num_nodes, const int* conn_hos
 int pa_mesh_destroy(pa_mesh* mesh);
 /* Sizes: nodes, cells, CRS blocks. */
 int pa_mesh_sizes(const pa_mesh* mesh, int* num_nodes, int* num_cells, int64_t* num_blocks);
+/*
+ * Assembly schedule of the mesh (diagnostic; no reference counterpart):
+ * number of cell tiles and distinct tile schedules (0 when the node
+ * numbering is not a lexicographic grid, or PA_TILES=0 was set at creation),
+ * and of distinct row patterns (0 when the generic row kernels are used).
+ */
+int pa_mesh_assembly_info(const pa_mesh* mesh, int* num_tiles, int* num_tile_patterns, int* num_row_patterns);
 /* Copy the block-CRS graph to the host (rowmap: N+1, colidx: nnzb). */
 int pa_mesh_graph(const pa_mesh* mesh, int* rowmap_host, int* colidx_host);
 /* Device pointers of the graph (owned by the mesh). */

@@ -111,6 +111,7 @@ def _load():
         "pa_mesh_destroy": ([vp], i),
         "pa_mesh_sizes": ([vp, ip, ip, ctypes.POINTER(ctypes.c_int64)], i),
         "pa_mesh_graph": ([vp, vp, vp], i),
+        "pa_mesh_assembly_info": ([vp, ip, ip, ip], i),
         "pa_mesh_graph_dev": ([vp, ctypes.POINTER(vp), ctypes.POINTER(vp)], i),
         "pa_mesh_add_side_set": ([vp, ctypes.c_char_p, i, vp, vp], i),
         "pa_mesh_add_node_set": ([vp, ctypes.c_char_p, i, vp], i),
@@ -356,6 +357,12 @@ class Mesh:
                     X = coords.reshape(-1, 3)
                     m.add_node_set(name + sign, np.nonzero(np.abs(X[:, axis] - val) < 1e-12)[0])
         return m
+
+    def assembly_info(self):
+        """(cell tiles, distinct tile schedules, distinct row patterns) of the assembly kernels."""
+        t, tp, rp = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib.pa_mesh_assembly_info(self.handle, ctypes.byref(t), ctypes.byref(tp), ctypes.byref(rp)))
+        return t.value, tp.value, rp.value
 
     def graph(self):
         rowmap = np.empty(self.num_nodes + 1, np.int32)

@@ -609,6 +609,324 @@ __global__ __launch_bounds__(kBlock) void k_jacobian_pattern(RowMesh m, const do
 }
 
 // ---------------------------------------------------------------------------
+// Cell-tile assembly: one workgroup per tile, a block of up to 64 rows of the
+// node grid (schedules built once per mesh on the host).
+//  phase 1: every distinct cell incident to the tile's rows is evaluated
+//           once into LDS: the Jacobian kernel parks sqrt(w) g_n (w = s(rho)
+//           |det J| / 6), so a block is M = sum g_a (x) g_b over the cells
+//           of its edge; the residual kernel parks the four nodes' cell
+//           residuals.
+//  phase 2: work units (row, column slot) sum their block from LDS and store
+//           it; 8 lanes per row sum the residual (the addition tree of a
+//           32-lane butterfly over the incident cells).
+// Each cell is evaluated about twice (once per tile touching it) instead of
+// four times (once per node) as in the row kernels.
+// ---------------------------------------------------------------------------
+struct TileMesh {
+    const int4* __restrict__ tile;  // 2 per tile
+    const int4* __restrict__ rows;
+    const int4* __restrict__ cells;
+    const unsigned short* __restrict__ pairs;
+    const uint2* __restrict__ units;
+    const uint4* __restrict__ ents;
+    const int* __restrict__ rowmap;
+};
+
+// A tile's first row, the offsets of its schedule's tables and their sizes.
+struct TileHead {
+    int base, row_off, cell_off, unit_off, nrows, ncells, nunits;
+};
+
+__device__ inline TileHead tile_head(const TileMesh& t) {
+    const int b = xcd_block();
+    const int4 a = t.tile[2 * b], c = t.tile[2 * b + 1];
+    return TileHead{a.x, a.y, a.z, a.w, c.x, c.y, c.z};
+}
+
+// Node ids and (x, y, z, density) records of tile cell c.
+__device__ inline void tile_cell_load(const TileMesh& t, const double4* __restrict__ nrec, const TileHead& h, int c,
+                                      CellEval& ce, double4 x[4]) {
+    const int4 d = t.cells[h.cell_off + c];
+    ce.nodes[0] = h.base + d.x;
+    ce.nodes[1] = h.base + d.y;
+    ce.nodes[2] = h.base + d.z;
+    ce.nodes[3] = h.base + d.w;
+#pragma unroll
+    for (int n = 0; n < 4; ++n) x[n] = nrec[ce.nodes[n]];
+}
+
+__device__ inline void tile_cell_geometry(CellEval& ce, const double4 x[4]) {
+    ce.absdet = tet_gradients(x, ce.g);
+    ce.rho = (((x[0].w + x[1].w) + x[2].w) + x[3].w) / 4.0;
+}
+
+// Phase 1 over the tile's cells, two per thread (a tile has at most
+// 2 * kTileThreads cells), with both cells' gathers issued before either is
+// used.
+template <class F>
+__device__ inline void tile_cells(const TileMesh& t, const double4* __restrict__ nrec, const TileHead& h, F&& eval) {
+    static_assert(kTileCellsMax <= 2 * kTileThreads, "two cells per thread");
+    const int c0 = threadIdx.x, c1 = threadIdx.x + kTileThreads;
+    CellEval e0, e1;
+    double4 x0[4], x1[4];
+    if (c0 < h.ncells) tile_cell_load(t, nrec, h, c0, e0, x0);
+    if (c1 < h.ncells) tile_cell_load(t, nrec, h, c1, e1, x1);
+    if (c0 < h.ncells) {
+        tile_cell_geometry(e0, x0);
+        eval(c0, e0);
+    }
+    if (c1 < h.ncells) {
+        tile_cell_geometry(e1, x1);
+        eval(c1, e1);
+    }
+}
+
+// Residual rows: 8 lanes per row; lane s takes incident cells s, s + 16,
+// s + 8, s + 24 (absent ones as zero) and the lanes are combined by a
+// butterfly, which reproduces a 32-lane butterfly's addition tree.  The
+// row's pair entries are fetched ahead (before the cells are evaluated).
+struct TileRowFetch {
+    int row_delta;
+    unsigned pe[4];  // tile cell << 2 | a, or ~0u for an absent cell
+};
+
+__device__ inline void tile_row_fetch(const TileMesh& t, const TileHead& h, int q, TileRowFetch& f) {
+    const int r = q >> 3, s = q & 7;
+    const int4 ri = t.rows[h.row_off + r];
+    f.row_delta = ri.x;
+    const int ps[4] = {s, s + 16, s + 8, s + 24};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) f.pe[j] = ps[j] < ri.y ? (unsigned)t.pairs[ri.z + ps[j]] : ~0u;
+}
+
+template <int DPN, class F>
+__device__ inline void tile_residual_rows(const TileMesh& t, const TileHead& h, int q, TileRowFetch& f, double scale,
+                                          double* __restrict__ R, F&& contrib) {
+    while (q < h.nrows * 8) {
+        double v[4][DPN];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (f.pe[j] != ~0u) {
+                contrib(f.pe[j], v[j]);
+            } else {
+#pragma unroll
+                for (int i = 0; i < DPN; ++i) v[j][i] = 0.0;
+            }
+        }
+        double b[DPN];
+#pragma unroll
+        for (int i = 0; i < DPN; ++i) {
+            b[i] = (v[0][i] + v[1][i]) + (v[2][i] + v[3][i]);
+#pragma unroll
+            for (int o = 4; o > 0; o >>= 1) b[i] += __shfl_xor(b[i], o, 8);
+        }
+        if ((q & 7) == 0) {
+#pragma unroll
+            for (int i = 0; i < DPN; ++i) R[(long)(h.base + f.row_delta) * DPN + i] = scale * b[i];
+        }
+        q += kTileThreads;
+        if (q < h.nrows * 8) tile_row_fetch(t, h, q, f);
+    }
+}
+
+template <int DPN>
+__global__ __launch_bounds__(kTileThreads) void k_tile_residual(TileMesh t, const double4* __restrict__ nrec,
+                                                                Material mt, const double* __restrict__ u,
+                                                                double body, double scale, double* __restrict__ R) {
+    constexpr int FS = 4 * DPN + 1;  // the four nodes' residuals, one pad
+    extern __shared__ double lds[];
+    const TileHead h = tile_head(t);
+    TileRowFetch rf;
+    if ((int)threadIdx.x < h.nrows * 8) tile_row_fetch(t, h, threadIdx.x, rf);
+    tile_cells(t, nrec, h, [&](int c, const CellEval& ce) {
+        const double s = penal(ce.rho, mt.p, mt.emin);
+        const double w = s * (ce.absdet / 6.0);
+        double* F = lds + c * FS;
+        if constexpr (DPN == 3) {
+            double eps[6], sig[6], out[3];
+            cell_strain(ce, u, eps);
+            iso_stress(mt, eps, sig);
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                btsig(sig, ce.g[a], out);
+                F[3 * a + 0] = w * out[0];
+                F[3 * a + 1] = w * out[1];
+                F[3 * a + 2] = w * out[2];
+            }
+        } else {
+            double gT[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int n = 0; n < 4; ++n) {
+                const double T = u[ce.nodes[n]];
+                gT[0] += T * ce.g[n][0];
+                gT[1] += T * ce.g[n][1];
+                gT[2] += T * ce.g[n][2];
+            }
+            const double load = -1.0 * (ce.absdet / 6.0) * body * 0.25 * ce.rho;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                double r = w * mt.k * (gT[0] * ce.g[a][0] + gT[1] * ce.g[a][1] + gT[2] * ce.g[a][2]);
+                r += load;
+                F[a] = r;
+            }
+        }
+    });
+    __syncthreads();
+    tile_residual_rows<DPN>(t, h, threadIdx.x, rf, scale, R, [&](unsigned e, double* v) {
+        const double* F = lds + (e >> 2) * FS + (e & 3) * DPN;
+#pragma unroll
+        for (int i = 0; i < DPN; ++i) v[i] = F[i];
+    });
+}
+
+// One work unit (contributions to one block, or to a part of a diagonal
+// block) with its entry slots, fetched ahead of use.
+struct TileUnitFetch {
+    uint2 unit;
+    uint4 ew[kTileUnitEntries / 4];
+};
+
+__device__ inline void tile_unit_fetch(const TileMesh& t, const TileHead& h, int it, TileUnitFetch& f) {
+    f.unit = t.units[h.unit_off + it];
+#pragma unroll
+    for (int k = 0; k < kTileUnitEntries / 4; ++k) f.ew[k] = t.ents[f.unit.y + k];
+}
+
+// M = sum over the unit's UE entry slots of g_a (x) g_b (gradients scaled by
+// sqrt(w)); unused slots hold the zero cell, so the loop has no branches and
+// its LDS reads pipeline.
+template <int DPN, int UE>
+__device__ inline void tile_unit_block(const double* lds, const TileUnitFetch& f, double* M) {
+    constexpr int MS = DPN == 3 ? 9 : 1;
+#pragma unroll
+    for (int q = 0; q < MS; ++q) M[q] = 0.0;
+#pragma unroll
+    for (int e = 0; e < UE; ++e) {
+        const uint4 w4 = f.ew[e / 4];
+        const unsigned en = (e & 3) == 0 ? w4.x : (e & 3) == 1 ? w4.y : (e & 3) == 2 ? w4.z : w4.w;
+        const double* A = lds + (en & 0xffffu);
+        const double* B = lds + (en >> 16);
+        const double a0 = A[0], a1 = A[1], a2 = A[2];
+        const double b0 = B[0], b1 = B[1], b2 = B[2];
+        if constexpr (DPN == 3) {
+            M[0] += a0 * b0; M[1] += a0 * b1; M[2] += a0 * b2;
+            M[3] += a1 * b0; M[4] += a1 * b1; M[5] += a1 * b2;
+            M[6] += a2 * b0; M[7] += a2 * b1; M[8] += a2 * b2;
+        } else {
+            M[0] += (a0 * b0 + a1 * b1) + a2 * b2;
+        }
+    }
+}
+
+// Jacobian (optionally with the residual) of one tile.  Phase 1 parks
+// sqrt(w) g_n per cell (kTileStride doubles), with the residual also
+// sqrt(w) sigma (thermal: sqrt(w) grad T and the body load term).  Phase 2:
+// each thread evaluates up to three work units; the kTileDiagParts parts of
+// a diagonal block sit on adjacent lanes and are summed by a butterfly;
+// blocks are stored as lambda M + mu M^T + mu tr(M) I (thermal: k M).
+template <int DPN, class VT, bool RES, int UE>
+__global__ __launch_bounds__(kTileThreads) void k_tile_jacobian(TileMesh t, const double4* __restrict__ nrec,
+                                                                Material mt, const double* __restrict__ u,
+                                                                double body, int cmax, VT* __restrict__ vals,
+                                                                double* __restrict__ R) {
+    constexpr int MS = DPN == 3 ? 9 : 1;
+    constexpr int SS = DPN == 3 ? 7 : 5;  // residual data per cell, one pad
+    constexpr int NU = kTileUnitsMax / kTileThreads;
+    static_assert(kTileStride <= kTileThreads && kTileDiagParts == 4, "zero cell by one pass; 4-lane butterfly");
+    extern __shared__ double lds[];
+    __shared__ int rstart[64];  // rowmap of the tile's rows
+    double* S = lds + (long)cmax * kTileStride;
+    const TileHead h = tile_head(t);
+    // the thread's units and residual row are fetched while the cells load
+    TileUnitFetch f[NU];
+#pragma unroll
+    for (int k = 0; k < NU; ++k)
+        if ((int)threadIdx.x + k * kTileThreads < h.nunits) tile_unit_fetch(t, h, threadIdx.x + k * kTileThreads, f[k]);
+    TileRowFetch rf;
+    if (RES && (int)threadIdx.x < h.nrows * 8) tile_row_fetch(t, h, threadIdx.x, rf);
+    if ((int)threadIdx.x < h.nrows) rstart[threadIdx.x] = t.rowmap[h.base + t.rows[h.row_off + threadIdx.x].x];
+    if ((int)threadIdx.x < kTileStride) lds[h.ncells * kTileStride + threadIdx.x] = 0.0;  // the zero cell
+    tile_cells(t, nrec, h, [&](int c, const CellEval& ce) {
+        const double s = penal(ce.rho, mt.p, mt.emin);
+        const double w = s * (ce.absdet / 6.0);
+        const double sw = sqrt(w);
+        double* Q = lds + c * kTileStride;
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+            Q[3 * n + 0] = sw * ce.g[n][0];
+            Q[3 * n + 1] = sw * ce.g[n][1];
+            Q[3 * n + 2] = sw * ce.g[n][2];
+        }
+        if constexpr (RES) {
+            double* Sc = S + c * SS;
+            if constexpr (DPN == 3) {
+                double eps[6], sig[6];
+                cell_strain(ce, u, eps);
+                iso_stress(mt, eps, sig);
+#pragma unroll
+                for (int v = 0; v < 6; ++v) Sc[v] = sw * sig[v];
+            } else {
+                double gT[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+                for (int n = 0; n < 4; ++n) {
+                    const double T = u[ce.nodes[n]];
+                    gT[0] += T * ce.g[n][0];
+                    gT[1] += T * ce.g[n][1];
+                    gT[2] += T * ce.g[n][2];
+                }
+                Sc[0] = sw * gT[0];
+                Sc[1] = sw * gT[1];
+                Sc[2] = sw * gT[2];
+                Sc[3] = -1.0 * (ce.absdet / 6.0) * body * 0.25 * ce.rho;
+            }
+        }
+    });
+    __syncthreads();
+    if constexpr (RES) {
+        tile_residual_rows<DPN>(t, h, threadIdx.x, rf, 1.0, R, [&](unsigned e, double* v) {
+            const double* ga = lds + (e >> 2) * kTileStride + 3 * (e & 3);
+            const double* Sc = S + (e >> 2) * SS;
+            if constexpr (DPN == 3) {
+                btsig(Sc, ga, v);
+            } else {
+                double r = mt.k * (Sc[0] * ga[0] + Sc[1] * ga[1] + Sc[2] * ga[2]);
+                r += Sc[3];
+                v[0] = r;
+            }
+        });
+    }
+#pragma unroll
+    for (int k = 0; k < NU; ++k) {
+        if (k * kTileThreads >= h.nunits) break;  // uniform
+        const int it = threadIdx.x + k * kTileThreads;
+        const bool live = it < h.nunits;
+        double M[MS];
+        if (live) {
+            tile_unit_block<DPN, UE>(lds, f[k], M);
+        } else {
+#pragma unroll
+            for (int q = 0; q < MS; ++q) M[q] = 0.0;
+        }
+        const unsigned uw = live ? f[k].unit.x : 0u;
+        const bool diag = ((uw >> 24) & 1u) != 0u;
+        // parts of a diagonal block: lanes 4j..4j+3, summed (p0 + p1) + (p2 + p3)
+#pragma unroll
+        for (int q = 0; q < MS; ++q) {
+            const double o1 = __shfl_xor(M[q], 1);
+            if (diag) M[q] += o1;
+            const double o2 = __shfl_xor(M[q], 2);
+            if (diag) M[q] += o2;
+        }
+        if (live && (!diag || (uw >> 25) == 0u)) {
+            const int r = (int)(uw & 0xffu), slot = (int)((uw >> 8) & 0xffu);
+            VT* dst = vals + ((long)rstart[r] + slot) * MS;
+#pragma unroll
+            for (int q = 0; q < MS; ++q) dst[q] = (VT)block_entry<DPN>(mt, M, q);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // gradient_z rows: block (row = control node a, col = state node b) holds
 // dR_b/dz_a (1 x DPN) = sum over cells with a, b of (1/4) s'(rho) (V) B_b^T C eps
 // (+ thermal body load derivative).
@@ -968,6 +1286,33 @@ RowMesh row_mesh(const pa_mesh* m) {
     return r;
 }
 
+TileMesh tile_mesh(const pa_mesh* m) {
+    TileMesh t;
+    t.tile = m->d_tile.p;
+    t.rows = m->d_tp_row.p;
+    t.cells = m->d_tp_cell.p;
+    t.pairs = m->d_tp_pair.p;
+    t.units = m->d_tp_unit.p;
+    t.ents = reinterpret_cast<const uint4*>(m->d_tp_ent.p);
+    t.rowmap = m->d_rowmap.p;
+    return t;
+}
+
+// Dynamic LDS of a tile kernel, or 0 when the device cannot give it (the
+// caller then takes the row kernels).  Above 64 KiB the kernel is opted in.
+template <class K>
+size_t tile_lds(K kernel, size_t bytes) {
+    static int limit = -1;
+    if (limit < 0) {
+        int dev = 0;
+        PA_HIP(hipGetDevice(&dev));
+        PA_HIP(hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    }
+    if (bytes + 64 * sizeof(int) > (size_t)limit) return 0;
+    if (bytes > 65536) PA_HIP(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    return bytes;
+}
+
 constexpr int kG = 8;  // lanes per row
 
 int row_blocks(int N, int G) { return (N + kBlock / G - 1) / (kBlock / G); }
@@ -1033,6 +1378,19 @@ const double4* pack_nodes(const pa_mesh* m, const double* z, hipStream_t st) {
 // the node degree allows, else 8 lanes per row looping over the cells.
 void residual_rows(const pa_mesh* m, int dpn, const Material& mt, const double* u, const double* z, double body,
                    double scale, double* R, hipStream_t st) {
+    if (m->has_tiles) {
+        const size_t lds = (size_t)m->tile_cells_max * (4 * dpn + 1) * sizeof(double);
+        const size_t ok = dpn == 3 ? tile_lds(k_tile_residual<3>, lds) : tile_lds(k_tile_residual<1>, lds);
+        if (ok) {
+            const double4* rec = pack_nodes(m, z, st);
+            if (dpn == 3)
+                k_tile_residual<3><<<m->num_tiles, kTileThreads, lds, st>>>(tile_mesh(m), rec, mt, u, body, scale, R);
+            else
+                k_tile_residual<1><<<m->num_tiles, kTileThreads, lds, st>>>(tile_mesh(m), rec, mt, u, body, scale, R);
+            check_launch();
+            return;
+        }
+    }
     if (m->has_patterns) {
         const double4* rec = pack_nodes(m, z, st);
         if (dpn == 3)
@@ -1056,6 +1414,29 @@ void residual_rows(const pa_mesh* m, int dpn, const Material& mt, const double* 
 template <class VT, bool RES, bool TR>
 void jacobian(const pa_vector_function* vf, const double* u, const double* z, VT* vals, double* R, hipStream_t st) {
     const pa_mesh* m = vf->mesh;
+    if (m->has_tiles) {
+        // self-adjoint operators: the transposed assembly is the direct one
+        const Material mt = material_of(vf->physics, vf->material, vf->penalty);
+        const double body = vf->has_body ? vf->body_value : 0.0;
+        const int ss = !RES ? 0 : (vf->dpn == 3 ? 7 : 5);
+        const int cmax = m->tile_cells_max + 1;  // + the zero cell
+        const size_t lds = (size_t)cmax * (kTileStride + ss) * sizeof(double);
+        auto run = [&](auto kernel, double b) {
+            if (!tile_lds(kernel, lds)) return false;
+            const double4* rec = pack_nodes(m, z, st);
+            kernel<<<m->num_tiles, kTileThreads, lds, st>>>(tile_mesh(m), rec, mt, u, b, cmax, vals, R);
+            check_launch();
+            if (RES) natural_bcs(vf, R, st);
+            return true;
+        };
+        const bool six = m->tile_unit_entries <= 6;
+        bool done;
+        if (vf->dpn == 3)
+            done = six ? run(k_tile_jacobian<3, VT, RES, 6>, 0.0) : run(k_tile_jacobian<3, VT, RES, 8>, 0.0);
+        else
+            done = six ? run(k_tile_jacobian<1, VT, RES, 6>, body) : run(k_tile_jacobian<1, VT, RES, 8>, body);
+        if (done) return;
+    }
     if (m->has_patterns) {
         // The operators are self-adjoint: the transposed assembly equals the
         // direct one (block (i,j) = K_ji^T = K_ij) up to rounding.

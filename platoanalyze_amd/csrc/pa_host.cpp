@@ -2,6 +2,8 @@
 // mesh, and the one-time construction of the assembly graph and the
 // node -> cell incidence the node-ordered kernels read.
 #include <algorithm>
+#include <array>
+#include <cstdlib>
 #include <cstring>
 #include <thread>
 #include <unordered_map>
@@ -16,9 +18,9 @@ void set_last_error(const std::string& msg) { g_last_error = msg; }
 
 // Split [0, n) over host threads.
 template <class F>
-static void parallel_for(long n, F&& f) {
+static void parallel_for(long n, F&& f, long serial_below = 65536) {
     unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    if (n < 65536 || nt == 1) {
+    if (n < serial_below || nt == 1) {
         f(0L, n);
         return;
     }
@@ -139,6 +141,209 @@ static void build_row_patterns(pa_mesh* m, const std::vector<int>& off, const st
     m->d_pat_ent.upload(pa_ent.data(), pa_ent.size());
     m->d_pat_item.upload(pa_item.data(), pa_item.size());
     m->has_patterns = true;
+}
+
+// Node grid of a lexicographically numbered mesh (z fastest): the run of
+// nodes sharing node 0's (x, y), and the run sharing its x.  A numbering
+// without such runs gets no tiles; one with them but not a true grid still
+// gets correct tiles (arbitrary row groups), only fewer repeated ones.
+static bool detect_grid(const double* xyz, int N, int d[3]) {
+    int nz = 1;
+    while (nz < N && xyz[3L * nz] == xyz[0] && xyz[3L * nz + 1] == xyz[1]) ++nz;
+    int nyz = nz;
+    while (nyz < N && xyz[3L * nyz] == xyz[0]) ++nyz;
+    if (nz < 2 || nyz % nz != 0 || nyz / nz < 2 || N % nyz != 0) return false;
+    d[0] = N / nyz, d[1] = nyz / nz, d[2] = nz;
+    return true;
+}
+
+// One tile's schedule, relative to its first row.  The byte image of the
+// tables is the tile's signature: tiles that are translates of each other
+// (same row offsets, same cell node offsets, same column slots) coincide.
+//   rows   row - first row, number of incident cells, offset of its pairs
+//   cells  node ids - first row of each distinct incident cell, sorted
+//   pairs  per row and incident cell (incidence order): tile cell << 2 | a
+//   units  (row, column slot) blocks: row | slot << 8 | count << 16 |
+//          diagonal << 24 | part << 25, entry offset (16-byte groups); a
+//          diagonal block is cut into kTileDiagParts parts on adjacent units
+//   ents   per unit, kTileUnitEntries slots: LDS offsets (doubles) of the
+//          scaled gradients g_a and g_b, 16 bits each; unused slots point at
+//          the tile's zero cell (index = number of cells)
+struct TileSchedule {
+    std::vector<int4> rows, cells;
+    std::vector<unsigned short> pairs;
+    std::vector<uint2> units;
+    std::vector<unsigned> ents;
+    int max_count = 0;
+    std::string sig;
+};
+
+static bool tile_schedule(const pa_mesh* m, const std::vector<int>& conn, const std::vector<int>& off,
+                          const std::vector<int2>& pairs, const std::vector<int>& rows, TileSchedule& t) {
+    const int base = rows[0];
+    std::vector<int> cells;
+    for (int r : rows)
+        for (int p = off[r]; p < off[r + 1]; ++p) cells.push_back(pairs[p].x >> 2);
+    std::sort(cells.begin(), cells.end());
+    cells.erase(std::unique(cells.begin(), cells.end()), cells.end());
+    if ((int)cells.size() + 1 > kTileCellsMax) return false;  // + the zero cell
+    auto key = [&](int c) {
+        const int* q = &conn[4L * c];
+        return std::array<int, 4>{q[0] - base, q[1] - base, q[2] - base, q[3] - base};
+    };
+    std::sort(cells.begin(), cells.end(), [&](int a, int b) { return key(a) < key(b); });
+    std::vector<std::pair<int, int>> tc_of;  // (cell id, tile cell index)
+    for (int i = 0; i < (int)cells.size(); ++i) {
+        const auto k = key(cells[i]);
+        t.cells.push_back(make_int4(k[0], k[1], k[2], k[3]));
+        tc_of.push_back({cells[i], i});
+    }
+    std::sort(tc_of.begin(), tc_of.end());
+    auto tc = [&](int cell) { return std::lower_bound(tc_of.begin(), tc_of.end(), std::make_pair(cell, -1))->second; };
+    const unsigned zc = (unsigned)cells.size() * kTileStride;
+    std::vector<uint2> diag_units, off_units;
+    std::vector<std::vector<unsigned>> diag_ents, off_ents;
+    for (int ri = 0; ri < (int)rows.size(); ++ri) {
+        const int r = rows[ri];
+        const int np = off[r + 1] - off[r], rl = m->h_rowmap[r + 1] - m->h_rowmap[r];
+        if (np > 32 || rl > 255) return false;
+        t.rows.push_back(make_int4(r - base, np, (int)t.pairs.size(), 0));
+        for (int p = off[r]; p < off[r + 1]; ++p)
+            t.pairs.push_back((unsigned short)(tc(pairs[p].x >> 2) << 2 | (pairs[p].x & 3)));
+        const int diag = np ? (int)(((unsigned)pairs[off[r]].y >> (8 * (pairs[off[r]].x & 3))) & 0xff) : 0;
+        for (int slot = 0; slot < rl; ++slot) {
+            std::vector<unsigned> ent;
+            for (int p = off[r]; p < off[r + 1]; ++p)
+                for (int b = 0; b < 4; ++b)
+                    if ((int)(((unsigned)pairs[p].y >> (8 * b)) & 0xff) == slot) {
+                        const unsigned c = (unsigned)tc(pairs[p].x >> 2), a = (unsigned)(pairs[p].x & 3);
+                        ent.push_back((c * kTileStride + 3 * a) | (c * kTileStride + 3 * (unsigned)b) << 16);
+                    }
+            const unsigned head = (unsigned)ri | (unsigned)slot << 8;
+            if (slot == diag) {
+                const int per = ((int)ent.size() + kTileDiagParts - 1) / kTileDiagParts;
+                for (int k = 0; k < kTileDiagParts; ++k) {
+                    const int b0 = std::min((int)ent.size(), k * per), b1 = std::min((int)ent.size(), b0 + per);
+                    diag_units.push_back(make_uint2(head | (unsigned)(b1 - b0) << 16 | 1u << 24 | (unsigned)k << 25, 0u));
+                    diag_ents.emplace_back(ent.begin() + b0, ent.begin() + b1);
+                    t.max_count = std::max(t.max_count, b1 - b0);
+                }
+            } else {
+                off_units.push_back(make_uint2(head | (unsigned)ent.size() << 16, 0u));
+                t.max_count = std::max(t.max_count, (int)ent.size());
+                off_ents.push_back(std::move(ent));
+            }
+        }
+    }
+    if (t.max_count > kTileUnitEntries) return false;
+    // diagonal parts first: their groups of kTileDiagParts stay on adjacent
+    // lanes of one pass (the thread count is a multiple of the group size)
+    static_assert(kTileThreads % kTileDiagParts == 0, "diagonal groups within one pass");
+    auto add = [&](std::vector<uint2>& un, std::vector<std::vector<unsigned>>& en) {
+        for (size_t i = 0; i < un.size(); ++i) {
+            un[i].y = (unsigned)t.ents.size() / 4;
+            t.ents.insert(t.ents.end(), en[i].begin(), en[i].end());
+            for (size_t k = en[i].size(); k < (size_t)kTileUnitEntries; ++k) t.ents.push_back(zc | zc << 16);
+            t.units.push_back(un[i]);
+        }
+    };
+    add(diag_units, diag_ents);
+    add(off_units, off_ents);
+    if ((int)t.units.size() > kTileUnitsMax) return false;
+    auto put = [&](const void* p, size_t n) { t.sig.append(static_cast<const char*>(p), n); };
+    const int counts[5] = {(int)t.rows.size(), (int)t.cells.size(), (int)t.pairs.size(), (int)t.units.size(),
+                           (int)t.ents.size()};
+    put(counts, sizeof(counts));
+    put(t.rows.data(), t.rows.size() * sizeof(int4));
+    put(t.cells.data(), t.cells.size() * sizeof(int4));
+    put(t.pairs.data(), t.pairs.size() * sizeof(unsigned short));
+    put(t.units.data(), t.units.size() * sizeof(uint2));
+    put(t.ents.data(), t.ents.size() * sizeof(unsigned));
+    return true;
+}
+
+static void build_tiles(pa_mesh* m, const double* xyz, const std::vector<int>& conn, const std::vector<int>& off,
+                        const std::vector<int2>& pairs) {
+    m->has_tiles = false;
+    const char* env = std::getenv("PA_TILES");
+    if (env && env[0] == '0') return;
+    if (m->max_cells_per_node > 32) return;
+    int d[3];
+    if (!detect_grid(xyz, m->N, d)) return;
+    const int tx = (d[0] + kTileX - 1) / kTileX, ty = (d[1] + kTileY - 1) / kTileY, tz = (d[2] + kTileZ - 1) / kTileZ;
+    const long nt = (long)tx * ty * tz;
+    auto first_row = [&](long t) {
+        const int i0 = (int)(t / ((long)ty * tz)) * kTileX, j0 = (int)((t / tz) % ty) * kTileY,
+                  k0 = (int)(t % tz) * kTileZ;
+        return std::array<int, 3>{i0, j0, k0};
+    };
+    std::unordered_map<std::string, int> index;
+    std::vector<int4> tiles(2 * nt), info, trow, tcell;
+    std::vector<unsigned short> tpair;
+    std::vector<uint2> tunit;
+    std::vector<unsigned> tent;
+    int cmax = 0, rmax = 0, emax = 0;
+    // schedules are built in parallel batches and deduplicated in tile order
+    constexpr long kBatch = 2048;
+    std::vector<TileSchedule> sched(std::min(nt, kBatch));
+    std::vector<char> ok(sched.size());
+    for (long t0 = 0; t0 < nt; t0 += kBatch) {
+        const long nb = std::min(kBatch, nt - t0);
+        std::fill(ok.begin(), ok.end(), 1);
+        parallel_for(nb, [&](long b, long e) {
+            std::vector<int> rows;
+            for (long q = b; q < e; ++q) {
+                const auto f = first_row(t0 + q);
+                rows.clear();
+                for (int i = f[0]; i < std::min(f[0] + kTileX, d[0]); ++i)
+                    for (int j = f[1]; j < std::min(f[1] + kTileY, d[1]); ++j)
+                        for (int k = f[2]; k < std::min(f[2] + kTileZ, d[2]); ++k)
+                            rows.push_back((i * d[1] + j) * d[2] + k);
+                sched[q] = TileSchedule();
+                ok[q] = tile_schedule(m, conn, off, pairs, rows, sched[q]);
+            }
+        }, 64);
+        for (long q = 0; q < nb; ++q) {
+            if (!ok[q]) return;
+            TileSchedule& s = sched[q];
+            auto it = index.find(s.sig);
+            if (it == index.end()) {
+                if ((int)index.size() >= kTilePatternsMax) return;
+                it = index.emplace(s.sig, (int)index.size()).first;
+                const unsigned eoff = (unsigned)tent.size() / 4;
+                for (auto& w : s.units) w.y += eoff;
+                info.push_back(make_int4(0, (int)trow.size(), (int)tcell.size(), (int)tunit.size()));
+                info.push_back(make_int4((int)s.rows.size(), (int)s.cells.size(), (int)s.units.size(), 0));
+                const int poff = (int)tpair.size();
+                for (auto r : s.rows) trow.push_back(make_int4(r.x, r.y, r.z + poff, 0));
+                tcell.insert(tcell.end(), s.cells.begin(), s.cells.end());
+                tpair.insert(tpair.end(), s.pairs.begin(), s.pairs.end());
+                tunit.insert(tunit.end(), s.units.begin(), s.units.end());
+                tent.insert(tent.end(), s.ents.begin(), s.ents.end());
+                cmax = std::max(cmax, (int)s.cells.size());
+                rmax = std::max(rmax, (int)s.rows.size());
+                emax = std::max(emax, s.max_count);
+            }
+            const auto f = first_row(t0 + q);
+            tiles[2 * (t0 + q)] = info[2 * it->second];
+            tiles[2 * (t0 + q)].x = (f[0] * d[1] + f[1]) * d[2] + f[2];
+            tiles[2 * (t0 + q) + 1] = info[2 * it->second + 1];
+        }
+    }
+    if (tent.size() >= (1u << 31) || rmax > 64) return;
+    m->num_tiles = (int)nt;
+    m->num_tile_patterns = (int)index.size();
+    m->tile_cells_max = cmax;
+    m->tile_rows_max = rmax;
+    m->tile_unit_entries = emax <= 6 ? 6 : 8;
+    m->grid_dims[0] = d[0], m->grid_dims[1] = d[1], m->grid_dims[2] = d[2];
+    m->d_tile.upload(tiles.data(), tiles.size());
+    m->d_tp_row.upload(trow.data(), trow.size());
+    m->d_tp_cell.upload(tcell.data(), tcell.size());
+    m->d_tp_pair.upload(tpair.data(), tpair.size());
+    m->d_tp_unit.upload(tunit.data(), tunit.size());
+    m->d_tp_ent.upload(tent.data(), tent.size());
+    m->has_tiles = true;
 }
 
 }  // namespace pa
@@ -320,6 +525,10 @@ int pa_mesh_create(const double* coords, int N, const int* conn, int C, pa_mesh*
             m->d_n2c_off.upload(off.data(), off.size());
             m->d_pairs.upload(pairs.data(), pairs.size());
             pa::build_row_patterns(m, off, pairs);
+            {
+                std::vector<int> cv(conn, conn + 4L * C);
+                pa::build_tiles(m, coords, cv, off, pairs);
+            }
         } catch (...) {
             delete m;
             throw;
@@ -338,6 +547,15 @@ int pa_mesh_sizes(const pa_mesh* m, int* N, int* C, int64_t* nnzb) {
         if (N) *N = m->N;
         if (C) *C = m->C;
         if (nnzb) *nnzb = m->nnzb;
+    });
+}
+
+int pa_mesh_assembly_info(const pa_mesh* m, int* tiles, int* tile_patterns, int* row_patterns) {
+    return guarded([&] {
+        PA_REQUIRE(m, PA_ERR_ARG, "pa_mesh_assembly_info: null mesh");
+        if (tiles) *tiles = m->has_tiles ? m->num_tiles : 0;
+        if (tile_patterns) *tile_patterns = m->has_tiles ? m->num_tile_patterns : 0;
+        if (row_patterns) *row_patterns = m->has_patterns ? m->num_patterns : 0;
     });
 }
 

@@ -86,6 +86,16 @@ struct SideSet {
     int num_bnodes = 0;
 };
 
+// Cell tiles (shared by the schedule builder and the assembly kernels).
+constexpr int kTileX = 2, kTileY = 4, kTileZ = 4;  // rows per tile along the node grid
+constexpr int kTileStride = 13;     // doubles per cell in LDS: sqrt(w) g_n for n = 0..3, one pad
+constexpr int kTileCellsMax = 512;  // cells per tile the kernels' LDS is sized for
+constexpr int kTilePatternsMax = 4096;
+constexpr int kTileUnitEntries = 8;   // entry slots per work unit (contributions to one block)
+constexpr int kTileDiagParts = 4;     // units per diagonal block, summed across adjacent lanes
+constexpr int kTileThreads = 256;     // threads per tile workgroup
+constexpr int kTileUnitsMax = 3 * kTileThreads;  // work units per tile: three per thread
+
 }  // namespace pa
 
 // Mesh with the assembly graph and the node -> cell incidence used by the
@@ -118,6 +128,21 @@ struct pa_mesh {
     mutable pa::DevBuf<double4> d_nrec;
     pa::DevBuf<unsigned char> d_pat_ent;  // contributions (pair << 2 | local node b), grouped by column slot
     pa::DevBuf<unsigned> d_pat_item;      // work items: slot | start << 8 | count << 20
+    // Cell tiles: rows grouped into blocks of the lexicographic node grid; a
+    // block's incident cells are evaluated once each and its rows summed from
+    // LDS.  Blocks with the same structure relative to their first row share
+    // one schedule (tile pattern).
+    bool has_tiles = false;
+    int num_tiles = 0, num_tile_patterns = 0, tile_cells_max = 0, tile_rows_max = 0;
+    int tile_unit_entries = 0;        // entries per work unit the kernels evaluate (6 or 8)
+    int grid_dims[3] = {0, 0, 0};     // detected node grid (x, y, z node counts)
+    pa::DevBuf<int4> d_tile;          // 2 per tile: {first row, row_off, cell_off, unit_off}, {rows, cells, units, 0}
+    pa::DevBuf<int4> d_tp_row;        // per pattern row: {row - first row, incident cells, pair offset, 0}
+    pa::DevBuf<int4> d_tp_cell;       // per pattern cell: its four node ids - first row
+    pa::DevBuf<unsigned short> d_tp_pair;  // per (row, incident cell): tile cell << 2 | local index of the row node
+    pa::DevBuf<uint2> d_tp_unit;      // (row, column slot) work units: row | slot << 8 | count << 16 |
+                                      // diagonal << 24 | part << 25, entry offset in 16-byte groups
+    pa::DevBuf<unsigned> d_tp_ent;    // unit entries (kTileUnitEntries per unit): LDS offsets of g_a and g_b
     std::map<std::string, pa::SideSet> side_sets;
     std::map<std::string, std::vector<int>> node_sets;
 };
