@@ -623,8 +623,9 @@ __global__ __launch_bounds__(kBlock) void k_jacobian_pattern(RowMesh m, const do
 // four times (once per node) as in the row kernels.
 // ---------------------------------------------------------------------------
 struct TileMesh {
-    const int4* __restrict__ tile;  // 2 per tile
+    const int4* __restrict__ tile;  // 3 per tile
     const int4* __restrict__ rows;
+    const int4* __restrict__ runs;
     const int4* __restrict__ cells;
     const unsigned short* __restrict__ pairs;
     const uint2* __restrict__ units;
@@ -634,13 +635,13 @@ struct TileMesh {
 
 // A tile's first row, the offsets of its schedule's tables and their sizes.
 struct TileHead {
-    int base, row_off, cell_off, unit_off, nrows, ncells, nunits;
+    int base, row_off, cell_off, unit_off, nrows, ncells, nunits, nruns, run_off;
 };
 
 __device__ inline TileHead tile_head(const TileMesh& t) {
     const int b = xcd_block();
-    const int4 a = t.tile[2 * b], c = t.tile[2 * b + 1];
-    return TileHead{a.x, a.y, a.z, a.w, c.x, c.y, c.z};
+    const int4 a = t.tile[3 * b], c = t.tile[3 * b + 1], e = t.tile[3 * b + 2];
+    return TileHead{a.x, a.y, a.z, a.w, c.x, c.y, c.z, c.w, e.x};
 }
 
 // Node ids and (x, y, z, density) records of tile cell c.
@@ -789,32 +790,48 @@ struct TileUnitFetch {
 __device__ inline void tile_unit_fetch(const TileMesh& t, const TileHead& h, int it, TileUnitFetch& f) {
     f.unit = t.units[h.unit_off + it];
 #pragma unroll
-    for (int k = 0; k < kTileUnitEntries / 4; ++k) f.ew[k] = t.ents[f.unit.y + k];
+    for (int k = 0; k < kTileUnitEntries / 4; ++k) f.ew[k] = t.ents[(f.unit.y & 0x3fffffu) + k];
 }
 
-// M = sum over the unit's UE entry slots of g_a (x) g_b (gradients scaled by
-// sqrt(w)); unused slots hold the zero cell, so the loop has no branches and
-// its LDS reads pipeline.
-template <int DPN, int UE>
+// M = sum over a unit's first NE entry slots of g_a (x) g_b (gradients
+// scaled by sqrt(w)); slots beyond the unit's count hold the zero cell, so
+// the loop has no branches and its LDS reads pipeline.  SAME: the entries are
+// diagonal (g_b = g_a), read once, with the six distinct products mirrored
+// (a_i a_n = a_n a_i exactly).
+template <int DPN, int NE, bool SAME>
 __device__ inline void tile_unit_block(const double* lds, const TileUnitFetch& f, double* M) {
     constexpr int MS = DPN == 3 ? 9 : 1;
 #pragma unroll
     for (int q = 0; q < MS; ++q) M[q] = 0.0;
 #pragma unroll
-    for (int e = 0; e < UE; ++e) {
+    for (int e = 0; e < NE; ++e) {
         const uint4 w4 = f.ew[e / 4];
         const unsigned en = (e & 3) == 0 ? w4.x : (e & 3) == 1 ? w4.y : (e & 3) == 2 ? w4.z : w4.w;
         const double* A = lds + (en & 0xffffu);
-        const double* B = lds + (en >> 16);
         const double a0 = A[0], a1 = A[1], a2 = A[2];
-        const double b0 = B[0], b1 = B[1], b2 = B[2];
-        if constexpr (DPN == 3) {
-            M[0] += a0 * b0; M[1] += a0 * b1; M[2] += a0 * b2;
-            M[3] += a1 * b0; M[4] += a1 * b1; M[5] += a1 * b2;
-            M[6] += a2 * b0; M[7] += a2 * b1; M[8] += a2 * b2;
+        if constexpr (SAME) {
+            if constexpr (DPN == 3) {
+                M[0] += a0 * a0; M[1] += a0 * a1; M[2] += a0 * a2;
+                M[4] += a1 * a1; M[5] += a1 * a2; M[8] += a2 * a2;
+            } else {
+                M[0] += (a0 * a0 + a1 * a1) + a2 * a2;
+            }
         } else {
-            M[0] += (a0 * b0 + a1 * b1) + a2 * b2;
+            const double* B = lds + (en >> 16);
+            const double b0 = B[0], b1 = B[1], b2 = B[2];
+            if constexpr (DPN == 3) {
+                M[0] += a0 * b0; M[1] += a0 * b1; M[2] += a0 * b2;
+                M[3] += a1 * b0; M[4] += a1 * b1; M[5] += a1 * b2;
+                M[6] += a2 * b0; M[7] += a2 * b1; M[8] += a2 * b2;
+            } else {
+                M[0] += (a0 * b0 + a1 * b1) + a2 * b2;
+            }
         }
+    }
+    if constexpr (SAME && DPN == 3) {
+        M[3] = M[1];
+        M[6] = M[2];
+        M[7] = M[5];
     }
 }
 
@@ -834,7 +851,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_jacobian(TileMesh t, cons
     constexpr int NU = kTileUnitsMax / kTileThreads;
     static_assert(kTileStride <= kTileThreads && kTileDiagParts == 4, "zero cell by one pass; 4-lane butterfly");
     extern __shared__ double lds[];
-    __shared__ int rstart[64];  // rowmap of the tile's rows
+    __shared__ int rstart[64], rstage[64];  // rowmap and staging offset of the tile's rows
     double* S = lds + (long)cmax * kTileStride;
     const TileHead h = tile_head(t);
     // the thread's units and residual row are fetched while the cells load
@@ -844,7 +861,11 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_jacobian(TileMesh t, cons
         if ((int)threadIdx.x + k * kTileThreads < h.nunits) tile_unit_fetch(t, h, threadIdx.x + k * kTileThreads, f[k]);
     TileRowFetch rf;
     if (RES && (int)threadIdx.x < h.nrows * 8) tile_row_fetch(t, h, threadIdx.x, rf);
-    if ((int)threadIdx.x < h.nrows) rstart[threadIdx.x] = t.rowmap[h.base + t.rows[h.row_off + threadIdx.x].x];
+    if ((int)threadIdx.x < h.nrows) {
+        const int4 ri = t.rows[h.row_off + threadIdx.x];
+        rstart[threadIdx.x] = t.rowmap[h.base + ri.x];
+        rstage[threadIdx.x] = ri.w;
+    }
     if ((int)threadIdx.x < kTileStride) lds[h.ncells * kTileStride + threadIdx.x] = 0.0;  // the zero cell
     tile_cells(t, nrec, h, [&](int c, const CellEval& ce) {
         const double s = penal(ce.rho, mt.p, mt.emin);
@@ -895,20 +916,34 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_jacobian(TileMesh t, cons
             }
         });
     }
+    // blocks are kept in registers, then staged in LDS (the cell data is
+    // dead by then) in CRS order and written run by run with contiguous stores
+    double K[NU][MS];
+    unsigned ux[NU];
+    bool put[NU], mirror[NU];
+#pragma unroll
+    for (int k = 0; k < NU; ++k) put[k] = mirror[k] = false;
 #pragma unroll
     for (int k = 0; k < NU; ++k) {
         if (k * kTileThreads >= h.nunits) break;  // uniform
         const int it = threadIdx.x + k * kTileThreads;
         const bool live = it < h.nunits;
+        const uint2 u = live ? f[k].unit : make_uint2(0u, 0u);
         double M[MS];
-        if (live) {
-            tile_unit_block<DPN, UE>(lds, f[k], M);
-        } else {
+        // the wave shape is uniform over the wave's live lanes
+        switch (u.y >> 30) {
+            case 1: tile_unit_block<DPN, 6, true>(lds, f[k], M); break;
+            case 2: tile_unit_block<DPN, 6, false>(lds, f[k], M); break;
+            case 3: tile_unit_block<DPN, 4, false>(lds, f[k], M); break;
+            default:
+                if (live) {
+                    tile_unit_block<DPN, UE, false>(lds, f[k], M);
+                } else {
 #pragma unroll
-            for (int q = 0; q < MS; ++q) M[q] = 0.0;
+                    for (int q = 0; q < MS; ++q) M[q] = 0.0;
+                }
         }
-        const unsigned uw = live ? f[k].unit.x : 0u;
-        const bool diag = ((uw >> 24) & 1u) != 0u;
+        const bool diag = ((u.y >> 26) & 1u) != 0u;
         // parts of a diagonal block: lanes 4j..4j+3, summed (p0 + p1) + (p2 + p3)
 #pragma unroll
         for (int q = 0; q < MS; ++q) {
@@ -917,12 +952,39 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_jacobian(TileMesh t, cons
             const double o2 = __shfl_xor(M[q], 2);
             if (diag) M[q] += o2;
         }
-        if (live && (!diag || (uw >> 25) == 0u)) {
-            const int r = (int)(uw & 0xffu), slot = (int)((uw >> 8) & 0xffu);
-            VT* dst = vals + ((long)rstart[r] + slot) * MS;
 #pragma unroll
-            for (int q = 0; q < MS; ++q) dst[q] = (VT)block_entry<DPN>(mt, M, q);
+        for (int q = 0; q < MS; ++q) K[k][q] = block_entry<DPN>(mt, M, q);
+        ux[k] = u.x;
+        put[k] = live && (!diag || ((u.y >> 27) & 3u) == 0u);
+        mirror[k] = ((u.y >> 29) & 1u) != 0u;
+    }
+    __syncthreads();
+    VT* stage = reinterpret_cast<VT*>(lds);
+#pragma unroll
+    for (int k = 0; k < NU; ++k) {
+        if (!put[k]) continue;
+        VT* dst = stage + (rstage[ux[k] & 0xffu] + ((ux[k] >> 8) & 0xffu)) * MS;
+#pragma unroll
+        for (int q = 0; q < MS; ++q) dst[q] = (VT)K[k][q];
+        if (mirror[k]) {  // the mirror block K_ji = K_ij^T
+            VT* dm = stage + (rstage[(ux[k] >> 16) & 0xffu] + (ux[k] >> 24)) * MS;
+            if constexpr (DPN == 3) {
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int n = 0; n < 3; ++n) dm[3 * i + n] = (VT)K[k][3 * n + i];
+            } else {
+                dm[0] = (VT)K[k][0];
+            }
         }
+    }
+    __syncthreads();
+    for (int j = 0; j < h.nruns; ++j) {
+        const int4 run = t.runs[h.run_off + j];
+        VT* dst = vals + (long)rstart[run.x] * MS;
+        const VT* src = stage + run.y * MS;
+        const int len = run.z * MS;
+        for (int i = threadIdx.x; i < len; i += kTileThreads) dst[i] = src[i];
     }
 }
 
@@ -1290,6 +1352,7 @@ TileMesh tile_mesh(const pa_mesh* m) {
     TileMesh t;
     t.tile = m->d_tile.p;
     t.rows = m->d_tp_row.p;
+    t.runs = m->d_tp_run.p;
     t.cells = m->d_tp_cell.p;
     t.pairs = m->d_tp_pair.p;
     t.units = m->d_tp_unit.p;
@@ -1420,7 +1483,8 @@ void jacobian(const pa_vector_function* vf, const double* u, const double* z, VT
         const double body = vf->has_body ? vf->body_value : 0.0;
         const int ss = !RES ? 0 : (vf->dpn == 3 ? 7 : 5);
         const int cmax = m->tile_cells_max + 1;  // + the zero cell
-        const size_t lds = (size_t)cmax * (kTileStride + ss) * sizeof(double);
+        const size_t lds = std::max((size_t)cmax * (kTileStride + ss) * sizeof(double),
+                                    (size_t)m->tile_blocks_max * vf->dpn * vf->dpn * sizeof(VT));
         auto run = [&](auto kernel, double b) {
             if (!tile_lds(kernel, lds)) return false;
             const double4* rec = pack_nodes(m, z, st);

@@ -160,17 +160,25 @@ static bool detect_grid(const double* xyz, int N, int d[3]) {
 // One tile's schedule, relative to its first row.  The byte image of the
 // tables is the tile's signature: tiles that are translates of each other
 // (same row offsets, same cell node offsets, same column slots) coincide.
-//   rows   row - first row, number of incident cells, offset of its pairs
+//   rows   row - first row, number of incident cells, offset of its pairs,
+//          staging offset (blocks) of its output
+//   runs   rows with consecutive ids (adjacent CRS ranges): first row index,
+//          staging offset, blocks
 //   cells  node ids - first row of each distinct incident cell, sorted
 //   pairs  per row and incident cell (incidence order): tile cell << 2 | a
-//   units  (row, column slot) blocks: row | slot << 8 | count << 16 |
-//          diagonal << 24 | part << 25, entry offset (16-byte groups); a
-//          diagonal block is cut into kTileDiagParts parts on adjacent units
+//   units  (row, column slot) blocks: x = row | slot << 8 | mirror row << 16
+//          | mirror slot << 24, y = entry offset (16-byte groups) | count << 22
+//          | diagonal << 26 | part << 27 | mirror << 29 | wave shape << 30.
+//          A diagonal block is cut into kTileDiagParts parts on adjacent
+//          units; a block whose column node is also a row of the tile is
+//          computed once (by the lower row) and also stored transposed
+//          (its mirror); the wave shape (per aligned group of 64 units)
+//          selects an unpadded evaluation.
 //   ents   per unit, kTileUnitEntries slots: LDS offsets (doubles) of the
 //          scaled gradients g_a and g_b, 16 bits each; unused slots point at
 //          the tile's zero cell (index = number of cells)
 struct TileSchedule {
-    std::vector<int4> rows, cells;
+    std::vector<int4> rows, cells, runs;
     std::vector<unsigned short> pairs;
     std::vector<uint2> units;
     std::vector<unsigned> ents;
@@ -201,17 +209,33 @@ static bool tile_schedule(const pa_mesh* m, const std::vector<int>& conn, const 
     std::sort(tc_of.begin(), tc_of.end());
     auto tc = [&](int cell) { return std::lower_bound(tc_of.begin(), tc_of.end(), std::make_pair(cell, -1))->second; };
     const unsigned zc = (unsigned)cells.size() * kTileStride;
-    std::vector<uint2> diag_units, off_units;
-    std::vector<std::vector<unsigned>> diag_ents, off_ents;
+    struct Unit {
+        int r, slot, mr, ms, part;
+        bool diag, mirror;
+        std::vector<unsigned> ent;
+    };
+    std::vector<Unit> units;
+    int stage = 0;  // staging offset (blocks) of the current row
+    auto row_index = [&](int node) {
+        auto it = std::lower_bound(rows.begin(), rows.end(), node);
+        return (it != rows.end() && *it == node) ? (int)(it - rows.begin()) : -1;
+    };
     for (int ri = 0; ri < (int)rows.size(); ++ri) {
         const int r = rows[ri];
         const int np = off[r + 1] - off[r], rl = m->h_rowmap[r + 1] - m->h_rowmap[r];
         if (np > 32 || rl > 255) return false;
-        t.rows.push_back(make_int4(r - base, np, (int)t.pairs.size(), 0));
+        t.rows.push_back(make_int4(r - base, np, (int)t.pairs.size(), stage));
+        // rows with consecutive ids have adjacent CRS ranges: one output run
+        if (ri > 0 && r == rows[ri - 1] + 1) t.runs.back().z += rl;
+        else t.runs.push_back(make_int4(ri, stage, rl, 0));
+        stage += rl;
         for (int p = off[r]; p < off[r + 1]; ++p)
             t.pairs.push_back((unsigned short)(tc(pairs[p].x >> 2) << 2 | (pairs[p].x & 3)));
         const int diag = np ? (int)(((unsigned)pairs[off[r]].y >> (8 * (pairs[off[r]].x & 3))) & 0xff) : 0;
         for (int slot = 0; slot < rl; ++slot) {
+            const int col = m->h_colidx[m->h_rowmap[r] + slot];
+            const int cj = slot == diag ? -1 : row_index(col);
+            if (cj >= 0 && col < r) continue;  // computed by row col's unit and mirrored
             std::vector<unsigned> ent;
             for (int p = off[r]; p < off[r + 1]; ++p)
                 for (int b = 0; b < 4; ++b)
@@ -219,42 +243,63 @@ static bool tile_schedule(const pa_mesh* m, const std::vector<int>& conn, const 
                         const unsigned c = (unsigned)tc(pairs[p].x >> 2), a = (unsigned)(pairs[p].x & 3);
                         ent.push_back((c * kTileStride + 3 * a) | (c * kTileStride + 3 * (unsigned)b) << 16);
                     }
-            const unsigned head = (unsigned)ri | (unsigned)slot << 8;
             if (slot == diag) {
                 const int per = ((int)ent.size() + kTileDiagParts - 1) / kTileDiagParts;
                 for (int k = 0; k < kTileDiagParts; ++k) {
                     const int b0 = std::min((int)ent.size(), k * per), b1 = std::min((int)ent.size(), b0 + per);
-                    diag_units.push_back(make_uint2(head | (unsigned)(b1 - b0) << 16 | 1u << 24 | (unsigned)k << 25, 0u));
-                    diag_ents.emplace_back(ent.begin() + b0, ent.begin() + b1);
-                    t.max_count = std::max(t.max_count, b1 - b0);
+                    units.push_back({ri, slot, 0, 0, k, true, false, std::vector<unsigned>(ent.begin() + b0, ent.begin() + b1)});
                 }
             } else {
-                off_units.push_back(make_uint2(head | (unsigned)ent.size() << 16, 0u));
-                t.max_count = std::max(t.max_count, (int)ent.size());
-                off_ents.push_back(std::move(ent));
+                int ms = 0;
+                if (cj >= 0) {  // slot of row r in row col
+                    const int* c0 = &m->h_colidx[m->h_rowmap[col]];
+                    ms = (int)(std::lower_bound(c0, c0 + (m->h_rowmap[col + 1] - m->h_rowmap[col]), r) - c0);
+                }
+                units.push_back({ri, slot, std::max(cj, 0), ms, 0, false, cj >= 0, std::move(ent)});
             }
         }
     }
+    // diagonal parts first (their groups of kTileDiagParts stay on adjacent
+    // lanes of one pass), then the others by decreasing entry count, so
+    // most waves hold units of one shape
+    static_assert(kTileThreads % kTileDiagParts == 0 && kTileThreads % 64 == 0, "groups within one pass");
+    std::stable_sort(units.begin(), units.end(), [](const Unit& a, const Unit& b) {
+        if (a.diag != b.diag) return a.diag;
+        if (a.diag) return false;
+        return a.ent.size() > b.ent.size();
+    });
+    for (const Unit& u : units) t.max_count = std::max(t.max_count, (int)u.ent.size());
     if (t.max_count > kTileUnitEntries) return false;
-    // diagonal parts first: their groups of kTileDiagParts stay on adjacent
-    // lanes of one pass (the thread count is a multiple of the group size)
-    static_assert(kTileThreads % kTileDiagParts == 0, "diagonal groups within one pass");
-    auto add = [&](std::vector<uint2>& un, std::vector<std::vector<unsigned>>& en) {
-        for (size_t i = 0; i < un.size(); ++i) {
-            un[i].y = (unsigned)t.ents.size() / 4;
-            t.ents.insert(t.ents.end(), en[i].begin(), en[i].end());
-            for (size_t k = en[i].size(); k < (size_t)kTileUnitEntries; ++k) t.ents.push_back(zc | zc << 16);
-            t.units.push_back(un[i]);
-        }
+    // shape of each wave (aligned group of 64 units): 1 all diagonal parts of
+    // 6 entries, 2 all off-diagonal with 6, 3 all off-diagonal with 4, else 0
+    auto shape = [](const Unit& u) {
+        if (u.diag) return u.ent.size() == 6 ? 1u : 0u;
+        return u.ent.size() == 6 ? 2u : (u.ent.size() == 4 ? 3u : 0u);
     };
-    add(diag_units, diag_ents);
-    add(off_units, off_ents);
+    for (size_t g = 0; g < units.size(); g += 64) {
+        const size_t e = std::min(units.size(), g + 64);
+        unsigned cls = shape(units[g]);
+        for (size_t i = g; i < e; ++i)
+            if (shape(units[i]) != cls) cls = 0;
+        for (size_t i = g; i < e; ++i) {
+            const Unit& u = units[i];
+            const unsigned eoff = (unsigned)t.ents.size() / 4;
+            if (eoff >= (1u << 22)) return false;
+            t.ents.insert(t.ents.end(), u.ent.begin(), u.ent.end());
+            for (size_t k = u.ent.size(); k < (size_t)kTileUnitEntries; ++k) t.ents.push_back(zc | zc << 16);
+            const unsigned x = (unsigned)u.r | (unsigned)u.slot << 8 | (unsigned)u.mr << 16 | (unsigned)u.ms << 24;
+            const unsigned y = eoff | (unsigned)u.ent.size() << 22 | (u.diag ? 1u : 0u) << 26 | (unsigned)u.part << 27 |
+                               (u.mirror ? 1u : 0u) << 29 | cls << 30;
+            t.units.push_back(make_uint2(x, y));
+        }
+    }
     if ((int)t.units.size() > kTileUnitsMax) return false;
     auto put = [&](const void* p, size_t n) { t.sig.append(static_cast<const char*>(p), n); };
-    const int counts[5] = {(int)t.rows.size(), (int)t.cells.size(), (int)t.pairs.size(), (int)t.units.size(),
-                           (int)t.ents.size()};
+    const int counts[6] = {(int)t.rows.size(), (int)t.cells.size(), (int)t.pairs.size(), (int)t.units.size(),
+                           (int)t.ents.size(), (int)t.runs.size()};
     put(counts, sizeof(counts));
     put(t.rows.data(), t.rows.size() * sizeof(int4));
+    put(t.runs.data(), t.runs.size() * sizeof(int4));
     put(t.cells.data(), t.cells.size() * sizeof(int4));
     put(t.pairs.data(), t.pairs.size() * sizeof(unsigned short));
     put(t.units.data(), t.units.size() * sizeof(uint2));
@@ -278,11 +323,11 @@ static void build_tiles(pa_mesh* m, const double* xyz, const std::vector<int>& c
         return std::array<int, 3>{i0, j0, k0};
     };
     std::unordered_map<std::string, int> index;
-    std::vector<int4> tiles(2 * nt), info, trow, tcell;
+    std::vector<int4> tiles(3 * nt), info, trow, tcell, trun;
     std::vector<unsigned short> tpair;
     std::vector<uint2> tunit;
     std::vector<unsigned> tent;
-    int cmax = 0, rmax = 0, emax = 0;
+    int cmax = 0, rmax = 0, emax = 0, bmax = 0;
     // schedules are built in parallel batches and deduplicated in tile order
     constexpr long kBatch = 2048;
     std::vector<TileSchedule> sched(std::min(nt, kBatch));
@@ -311,11 +356,17 @@ static void build_tiles(pa_mesh* m, const double* xyz, const std::vector<int>& c
                 if ((int)index.size() >= kTilePatternsMax) return;
                 it = index.emplace(s.sig, (int)index.size()).first;
                 const unsigned eoff = (unsigned)tent.size() / 4;
-                for (auto& w : s.units) w.y += eoff;
+                if (eoff + s.ents.size() / 4 >= (1u << 22)) return;  // entry offsets are 22-bit
+                for (auto& w : s.units) w.y = (w.y & ~0x3fffffu) | ((w.y & 0x3fffffu) + eoff);
+                const int nblk = s.runs.empty() ? 0 : s.runs.back().y + s.runs.back().z;
                 info.push_back(make_int4(0, (int)trow.size(), (int)tcell.size(), (int)tunit.size()));
-                info.push_back(make_int4((int)s.rows.size(), (int)s.cells.size(), (int)s.units.size(), 0));
+                info.push_back(make_int4((int)s.rows.size(), (int)s.cells.size(), (int)s.units.size(),
+                                         (int)s.runs.size()));
+                info.push_back(make_int4((int)trun.size(), nblk, 0, 0));
+                trun.insert(trun.end(), s.runs.begin(), s.runs.end());
+                bmax = std::max(bmax, nblk);
                 const int poff = (int)tpair.size();
-                for (auto r : s.rows) trow.push_back(make_int4(r.x, r.y, r.z + poff, 0));
+                for (auto r : s.rows) trow.push_back(make_int4(r.x, r.y, r.z + poff, r.w));
                 tcell.insert(tcell.end(), s.cells.begin(), s.cells.end());
                 tpair.insert(tpair.end(), s.pairs.begin(), s.pairs.end());
                 tunit.insert(tunit.end(), s.units.begin(), s.units.end());
@@ -325,9 +376,8 @@ static void build_tiles(pa_mesh* m, const double* xyz, const std::vector<int>& c
                 emax = std::max(emax, s.max_count);
             }
             const auto f = first_row(t0 + q);
-            tiles[2 * (t0 + q)] = info[2 * it->second];
-            tiles[2 * (t0 + q)].x = (f[0] * d[1] + f[1]) * d[2] + f[2];
-            tiles[2 * (t0 + q) + 1] = info[2 * it->second + 1];
+            for (int k = 0; k < 3; ++k) tiles[3 * (t0 + q) + k] = info[3 * it->second + k];
+            tiles[3 * (t0 + q)].x = (f[0] * d[1] + f[1]) * d[2] + f[2];
         }
     }
     if (tent.size() >= (1u << 31) || rmax > 64) return;
@@ -336,10 +386,12 @@ static void build_tiles(pa_mesh* m, const double* xyz, const std::vector<int>& c
     m->tile_cells_max = cmax;
     m->tile_rows_max = rmax;
     m->tile_unit_entries = emax <= 6 ? 6 : 8;
+    m->tile_blocks_max = bmax;
     m->grid_dims[0] = d[0], m->grid_dims[1] = d[1], m->grid_dims[2] = d[2];
     m->d_tile.upload(tiles.data(), tiles.size());
     m->d_tp_row.upload(trow.data(), trow.size());
     m->d_tp_cell.upload(tcell.data(), tcell.size());
+    m->d_tp_run.upload(trun.data(), trun.size());
     m->d_tp_pair.upload(tpair.data(), tpair.size());
     m->d_tp_unit.upload(tunit.data(), tunit.size());
     m->d_tp_ent.upload(tent.data(), tent.size());

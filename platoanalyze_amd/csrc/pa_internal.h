@@ -135,13 +135,19 @@ struct pa_mesh {
     bool has_tiles = false;
     int num_tiles = 0, num_tile_patterns = 0, tile_cells_max = 0, tile_rows_max = 0;
     int tile_unit_entries = 0;        // entries per work unit the kernels evaluate (6 or 8)
+    int tile_blocks_max = 0;          // output blocks of a tile (its staging size)
     int grid_dims[3] = {0, 0, 0};     // detected node grid (x, y, z node counts)
-    pa::DevBuf<int4> d_tile;          // 2 per tile: {first row, row_off, cell_off, unit_off}, {rows, cells, units, 0}
-    pa::DevBuf<int4> d_tp_row;        // per pattern row: {row - first row, incident cells, pair offset, 0}
+    pa::DevBuf<int4> d_tile;          // 3 per tile: {first row, row_off, cell_off, unit_off},
+                                      // {rows, cells, units, runs}, {run_off, blocks, 0, 0}
+    pa::DevBuf<int4> d_tp_row;        // per pattern row: {row - first row, incident cells, pair offset,
+                                      // staging offset}
+    pa::DevBuf<int4> d_tp_run;        // per pattern output run (rows with consecutive ids): {first row index,
+                                      // staging offset, blocks, 0}
     pa::DevBuf<int4> d_tp_cell;       // per pattern cell: its four node ids - first row
     pa::DevBuf<unsigned short> d_tp_pair;  // per (row, incident cell): tile cell << 2 | local index of the row node
-    pa::DevBuf<uint2> d_tp_unit;      // (row, column slot) work units: row | slot << 8 | count << 16 |
-                                      // diagonal << 24 | part << 25, entry offset in 16-byte groups
+    pa::DevBuf<uint2> d_tp_unit;      // (row, column slot) work units: {row | slot << 8 | mirror row << 16 |
+                                      // mirror slot << 24, entry offset | count << 22 | diagonal << 26 |
+                                      // part << 27 | mirror << 29 | wave shape << 30}
     pa::DevBuf<unsigned> d_tp_ent;    // unit entries (kTileUnitEntries per unit): LDS offsets of g_a and g_b
     std::map<std::string, pa::SideSet> side_sets;
     std::map<std::string, std::vector<int>> node_sets;
