@@ -116,6 +116,61 @@ J2_PARAMS = dict(youngs=1e3, poisson=0.3, hardening_isotropic=10.0, hardening_ki
 J2_END_DISPLACEMENT, J2_STEPS = 0.008, 10
 
 
+def thermal_bandwidth(pa, stream, reps=10):
+    """Steady thermal 192^3 hex x 6 Tet4 (42.5M cells, 7.1M nodes; k = 1, SIMP
+    p = 3, z ~ U[0.1, 1] seed 3, body source 1): the bandwidth-bound case.
+    Times value and gradient_u with FP64 and FP32 matrix values (inputs
+    resident in HBM) and reports cells/s and algorithmic GB/s against the
+    8 TB/s HBM peak."""
+    from platoanalyze_amd.workloads import WORKLOADS, algorithmic_bytes, density
+    name = "thermal_192x192x192"
+    physics, dims, L, dspec = WORKLOADS[name]
+    log(f"bench: building {name}")
+    t0 = time.perf_counter()
+    mesh = pa.Mesh.box(*dims, *L, side_sets=False)
+    setup_s = time.perf_counter() - t0
+    N, C, nnzb = mesh.num_nodes, mesh.num_cells, mesh.num_blocks
+    z = pa.DeviceArray.from_numpy(density(dims, dspec))
+    T = pa.DeviceArray.from_numpy(np.random.default_rng(13).standard_normal(N))
+    vf = pa.VectorFunction(mesh, physics, conductivity=1.0, penalty_exponent=3.0, penalty_min=1e-9)
+    vf.add_body_load(0, 1.0)
+    R = pa.DeviceArray(N)
+    K64 = pa.DeviceArray(nnzb, np.float64)
+    K32 = pa.DeviceArray(nnzb, np.float32)
+
+    def timed(fn):
+        fn()
+        stream.synchronize()
+        a, b = pa.Event(), pa.Event()
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        stream.synchronize()
+        return a.elapsed_ms(b) / reps
+
+    t_val = timed(lambda: vf.value(T, z, out=R, stream=stream))
+    t_j64 = timed(lambda: vf.gradient_u(T, z, out=K64, stream=stream))
+    t_j32 = timed(lambda: vf.gradient_u(T, z, precision=pa.FP32, out=K32, stream=stream))
+    b_val = algorithmic_bytes(C, N, nnzb, 1, 8, with_jacobian=False)
+    b_j64 = algorithmic_bytes(C, N, nnzb, 1, 8, with_residual=False)
+    b_j32 = algorithmic_bytes(C, N, nnzb, 1, 4, with_residual=False)
+
+    def leg(t, nbytes):
+        gbs = nbytes / (t * 1e-3) / 1e9
+        return {"ms": t, "cells_per_s": C / (t * 1e-3), "algorithmic_bytes": nbytes, "algorithmic_GBs": gbs,
+                "frac_of_hbm_peak": gbs / HBM_PEAK_GBS}
+
+    out = {"workload": "steady thermal Tet4, 192x192x192 hex x6 (%d cells, %d nodes, %d blocks), k=1, SIMP p=3, "
+                       "z ~ U[0.1,1] seed 3, body source 1" % (C, N, nnzb),
+           "setup_s": setup_s, "value": leg(t_val, b_val), "gradient_u_fp64": leg(t_j64, b_j64),
+           "gradient_u_fp32": leg(t_j32, b_j32),
+           "value_plus_gradient_u_fp64_cells_per_s": C / ((t_val + t_j64) * 1e-3)}
+    log(f"bench: thermal 192^3 value {t_val:.3f} ms, gradient_u fp64 {t_j64:.3f} ms, fp32 {t_j32:.3f} ms")
+    del K64, K32, R, T, z, vf, mesh
+    return out
+
+
 def j2_plasticity(pa, stream, with_cpu=True):
     """J2 plasticity (the reference's stabilized mixed u-p formulation, 4 dofs
     per node, 14 local states per cell) on the 64x32x32 hex grid x6 Tet4
@@ -208,9 +263,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-solve", action="store_true", help="skip the compliance + gradient leg")
     ap.add_argument("--no-j2", action="store_true", help="skip the J2 plasticity leg")
-    ap.add_argument("--mode", default="bench", choices=["bench", "copy", "profile", "solve", "j2"],
+    ap.add_argument("--no-thermal", action="store_true", help="skip the 192^3 thermal assembly leg")
+    ap.add_argument("--mode", default="bench", choices=["bench", "copy", "profile", "solve", "j2", "thermal"],
                     help="copy: run only the calibration copy kernel; profile: assembly kernels only; "
-                         "solve: one multigrid compliance + gradient solve only; j2: the J2 plasticity leg only")
+                         "solve: one multigrid compliance + gradient solve only; j2: the J2 plasticity leg only; "
+                         "thermal: the 192^3 thermal assembly leg only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -233,6 +290,9 @@ def main():
         return
     if args.mode == "j2":
         print(json.dumps(j2_plasticity(pa, stream, with_cpu=not args.no_cpu)), flush=True)
+        return
+    if args.mode == "thermal":
+        print(json.dumps(thermal_bandwidth(pa, stream, reps=max(1, args.steps))), flush=True)
         return
 
     name = "elastic_128x64x64"
@@ -326,7 +386,7 @@ def main():
                    "cells": C, "nodes": N, "blocks": nnzb, "precision": "fp64"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_tile_jacobian<3,double,false,6> (gradient_u, FP64)",
+                     "kernel": "gradient_u call (FP64): k_pack_nodes + k_tile_jacobian<3,double,false,6>",
                      "algorithmic_bytes_per_launch": jac_bytes, "avg_launch_ms": t_jac},
         "kernels_ms": {"gradient_u_fp64": t_jac, "value": t_val, "gradient_u_fp32": t_jac32,
                        "value_gradient_u_fused_fp64": t_fused, "value_gradient_u_fused_fp32": t_fused32},
@@ -338,6 +398,8 @@ def main():
         out["compliance_gradient"] = compliance_gradient(pa, mesh, vf, z, dims)
     if not args.no_j2:
         out["j2_plasticity"] = j2_plasticity(pa, stream, with_cpu=not args.no_cpu)
+    if not args.no_thermal:
+        out["thermal_192"] = thermal_bandwidth(pa, stream)
     if not args.no_cpu:
         log("bench: CPU baseline (oracle, OpenMP)")
         out["cpu_baseline"] = cpu_baseline()
