@@ -86,6 +86,7 @@ class J2Problem(ctypes.Structure):
         ("projection_min_ersatz", ctypes.c_double),
         ("work_penalty_exponent", ctypes.c_double),
         ("work_min_ersatz", ctypes.c_double),
+        ("formulation", ctypes.c_int),
     ]
 
 
@@ -303,12 +304,21 @@ class J2Model:
     properties penalty exponent and minimum ersatz), the global residual's
     Elliptic / Penalty Function (exponent, minimum value), Pressure Scaling,
     the projection's penalty (None = NoPenalty) and the plastic work
-    criterion's penalty."""
+    criterion's penalty.
+
+    formulation "mixed" is the reference's stabilized u-p formulation (4 dof
+    per node: ux, uy, uz, p); "displacement" is a pure-displacement J2 (3 dof
+    per node, sigma = Dev(mu)(eps - eps_p) + kappa tr(eps - eps_p) I, no
+    stabilization or projection), which has no reference counterpart.  Cell
+    worksets keep the node*4 + dim layout in both (the pressure slot is zero
+    in the displacement formulation); nodal vectors and matrix blocks have
+    `dpn` = 4 or 3 components."""
 
     def __init__(self, coords, conn, *, youngs, poisson, hardening_isotropic, hardening_kinematic,
                  initial_yield_stress, elastic_penalty_exponent=1.0, elastic_min_ersatz=1e-9,
                  plastic_penalty_exponent=0.5, plastic_min_ersatz=1e-4, global_penalty_exponent=1.0,
-                 global_min_ersatz=1e-9, pressure_scaling=1.0, projection_penalty=None, work_penalty=(3.0, 1e-9)):
+                 global_min_ersatz=1e-9, pressure_scaling=1.0, projection_penalty=None, work_penalty=(3.0, 1e-9),
+                 formulation="mixed"):
         self.coords = np.ascontiguousarray(coords, np.float64)
         self.conn = np.ascontiguousarray(conn, np.int32)
         self.N = self.coords.size // 3
@@ -329,6 +339,8 @@ class J2Model:
             P.projection_penalty = 1
             P.projection_penalty_exponent, P.projection_min_ersatz = projection_penalty
         P.work_penalty_exponent, P.work_min_ersatz = work_penalty
+        P.formulation = {"mixed": 0, "displacement": 1}[formulation]
+        self.dpn = 4 if P.formulation == 0 else 3
         self.P = P
         self.rowmap, self.colidx = graph(self.N, self.conn)
         self.nnzb = self.colidx.size
@@ -374,15 +386,15 @@ class J2Model:
         return out.reshape(self.C, 16, W)
 
     def global_value(self, u, c, pi, z):
-        R = np.zeros(self.N * 4)
+        R = np.zeros(self.N * self.dpn)
         args = [np.ascontiguousarray(x, np.float64) for x in (u, c, pi, z)]
         lib().orc_j2_global_value(ctypes.byref(self.P), *[_p(a) for a in args], _p(R))
         return R
 
     def condensed_system(self, u, c, cp, pi, z):
-        """(block-CRS values nnzb x 16, residual 4N) of one Newton linearization."""
-        vals = np.zeros(self.nnzb * 16)
-        R = np.zeros(self.N * 4)
+        """(block-CRS values nnzb x dpn^2, residual dpn N) of one Newton linearization."""
+        vals = np.zeros(self.nnzb * self.dpn * self.dpn)
+        R = np.zeros(self.N * self.dpn)
         args = [np.ascontiguousarray(x, np.float64) for x in (u, c, cp, pi, z)]
         lib().orc_j2_condensed_system(ctypes.byref(self.P), *[_p(a) for a in args], _p(self.rowmap),
                                       _p(self.colidx), _p(vals), _p(R))
@@ -399,7 +411,7 @@ class J2Model:
         return lib().orc_j2_plastic_work(ctypes.byref(self.P), *[_p(a) for a in args])
 
     def plastic_work_partials(self, u, up, c, cp, z):
-        gu, gup = np.zeros(self.N * 4), np.zeros(self.N * 4)
+        gu, gup = np.zeros(self.N * self.dpn), np.zeros(self.N * self.dpn)
         gc, gcp = np.zeros(self.C * 14), np.zeros(self.C * 14)
         gz = np.zeros(self.N)
         args = [np.ascontiguousarray(x, np.float64) for x in (u, up, c, cp, z)]
@@ -415,14 +427,16 @@ class J2Model:
         after each step) and src/NewtonRaphsonSolver.hpp:555-610 (elastic
         trial update, assemble -R_t and K_t, constraints, absolute-measure
         stopping test, u += du, local update).  Linear solves: scipy direct.
-        Returns per-step lists of u (4N), c (C*14), pi (N*3) and the Newton
+        Returns per-step lists of u (dpn N), c (C*14), pi (N*3; zero in the
+        displacement formulation, which has no projection) and the Newton
         iteration counts."""
         import scipy.sparse.linalg as spla
         z = np.ascontiguousarray(z, np.float64)
         bc_dofs = np.ascontiguousarray(bc_dofs, np.int32)
         bc_values = np.ascontiguousarray(bc_values, np.float64)
         us, cs, pis, iters = [], [], [], []
-        up = np.zeros(self.N * 4)
+        D = self.dpn
+        up = np.zeros(self.N * D)
         cp = np.zeros(self.C * 14)
         pi = np.zeros(self.N * 3)
         g_prev = np.zeros_like(bc_values)
@@ -439,7 +453,7 @@ class J2Model:
             while True:
                 vals, R = self.condensed_system(u, c, cp, pi, z)
                 rhs = -R
-                apply_block_constraints(4, self.rowmap, self.colidx, vals, rhs, bc_dofs,
+                apply_block_constraints(D, self.rowmap, self.colidx, vals, rhs, bc_dofs,
                                         dg if it == 0 else np.zeros_like(dg))
                 norm = float(np.linalg.norm(rhs))
                 if it == 0:
@@ -449,7 +463,7 @@ class J2Model:
                     ref = norm
                 if measure < stop_tol or norm < current_norm_tol or it >= max_newton:
                     break
-                K = to_scipy(4, 4, self.rowmap, self.colidx, vals)
+                K = to_scipy(D, D, self.rowmap, self.colidx, vals)
                 du = spla.spsolve(K.tocsc(), rhs)
                 u = u + du
                 c = self.update_local_state(u, cp, z)
@@ -458,7 +472,7 @@ class J2Model:
             cs.append(c)
             iters.append(it)
             pis.append(pi)
-            if k + 1 < num_steps:
+            if k + 1 < num_steps and D == 4:
                 pi = self.projection(u, z)
             up, cp = u, c
         return us, cs, pis, iters
@@ -482,7 +496,7 @@ class J2Model:
         """Plastic work summed over the steps (src/PlasticityProblem.hpp:925-956)."""
         total = 0.0
         for k in range(len(us)):
-            up = us[k - 1] if k else np.zeros(self.N * 4)
+            up = us[k - 1] if k else np.zeros(self.N * self.dpn)
             cp = cs[k - 1] if k else np.zeros(self.C * 14)
             total += self.plastic_work(us[k], up, cs[k], cp, z)
         return multiplier * total
@@ -499,20 +513,29 @@ class J2Model:
         (src/PlasticityProblem.hpp:1095-1112); it vanishes for NoPenalty."""
         import scipy.sparse.linalg as spla
         z = np.ascontiguousarray(z, np.float64)
-        K, N, C = len(us), self.N, self.C
+        K, N, C, D = len(us), self.N, self.C, self.dpn
+        mixed = D == 4
         cn = self.conn.reshape(-1, 4)
         grads, vols = self.cell_geometry()
         w, dw = self._projection_weight(z)
         ps = self.P.pressure_scaling
         rowsum = np.zeros(N)
         np.add.at(rowsum, cn.ravel(), np.repeat(0.25 * vols * w, 4))
-        zeroC, zeroN = np.zeros(C * 14), np.zeros(N * 4)
+        zeroC, zeroN = np.zeros(C * 14), np.zeros(N * D)
 
         def assemble(ws, width):  # cell workset C x (4*width) -> nodal
             out = np.zeros(N * width)
             idx = (cn[:, :, None] * width + np.arange(width)).reshape(C, -1)
             np.add.at(out, idx.ravel(), ws.reshape(C, -1).ravel())
             return out
+
+        def to_nodal_layout(ws16):  # C x 16 (node*4 + dim) -> C x 4*D
+            return ws16.reshape(C, 4, 4)[:, :, :D].reshape(C, 4 * D)
+
+        def to_cell_layout(wsD):  # C x 4*D -> C x 16, pressure slot zero
+            out = np.zeros((C, 4, 4))
+            out[:, :, :D] = wsD.reshape(C, 4, D)
+            return out.reshape(C, 16)
 
         def partials(k):
             up = us[k - 1] if k else zeroN
@@ -540,14 +563,15 @@ class J2Model:
                 dHdcp1 = self.local_gradient("cp", us[k + 1], cs[k + 1], c, z)
                 b_local += gcp1.reshape(C, 14) + np.einsum("eij,ei->ej", dHdcp1, mu1)
                 rhs += gup1
-                # DpDn^T gamma_{k+1}: dQ_{a,d}/dp_b = -1/4 V w ps dN_b/dx_d
-                gw = gather_nodal(self.conn, gam1, 3).reshape(C, 4, 3).sum(axis=1)  # sum_a gamma_{a,d}
-                dp = -0.25 * (vols * w * ps)[:, None] * np.einsum("ebd,ed->eb", grads, gw)
-                pr = np.zeros(N)
-                np.add.at(pr, cn.ravel(), dp.ravel())
-                rhs[3::4] += pr
+                if mixed:
+                    # DpDn^T gamma_{k+1}: dQ_{a,d}/dp_b = -1/4 V w ps dN_b/dx_d
+                    gw = gather_nodal(self.conn, gam1, 3).reshape(C, 4, 3).sum(axis=1)  # sum_a gamma_{a,d}
+                    dp = -0.25 * (vols * w * ps)[:, None] * np.einsum("ebd,ed->eb", grads, gw)
+                    pr = np.zeros(N)
+                    np.add.at(pr, cn.ravel(), dp.ravel())
+                    rhs[3::4] += pr
             t = np.einsum("eji,ej->ei", invJ, b_local)
-            rhs -= assemble(np.einsum("eji,ej->ei", dHdu, t), 4)
+            rhs -= assemble(to_nodal_layout(np.einsum("eji,ej->ei", dHdu, t)), D)
             rhs = -rhs
             vals, _ = self.condensed_system(u, c, cp, pi, z)
             # transposed assembly (assemble_jacobian_transpose_pod): block (r, c)
@@ -555,22 +579,22 @@ class J2Model:
             rmap, cidx = self.rowmap, self.colidx
             rows = np.repeat(np.arange(N, dtype=np.int64), np.diff(rmap))
             pos = np.searchsorted(rows * N + cidx, cidx.astype(np.int64) * N + rows)
-            bvals = np.ascontiguousarray(vals.reshape(-1, 4, 4)[pos].transpose(0, 2, 1)).ravel()
-            apply_block_constraints(4, rmap, cidx, bvals, rhs, bc_dofs, np.zeros(len(bc_dofs)))
-            lam = spla.spsolve(to_scipy(4, 4, rmap, cidx, bvals).tocsc(), rhs)
-            lam_ws = gather_nodal(self.conn, lam, 4)
+            bvals = np.ascontiguousarray(vals.reshape(-1, D, D)[pos].transpose(0, 2, 1)).ravel()
+            apply_block_constraints(D, rmap, cidx, bvals, rhs, bc_dofs, np.zeros(len(bc_dofs)))
+            lam = spla.spsolve(to_scipy(D, D, rmap, cidx, bvals).tocsc(), rhs)
+            lam_ws = to_cell_layout(gather_nodal(self.conn, lam, D))
             b2 = gc + np.einsum("eji,ej->ei", dRdc, lam_ws)
             if k < K - 1:
                 b2 += gcp1.reshape(C, 14) + np.einsum("eij,ei->ej", dHdcp1, mu1)
             mu = -np.einsum("eji,ej->ei", invJ, b2)
-            if k == 0:
+            if k == 0 or not mixed:
                 gam = np.zeros(N * 3)
             else:
                 r = -assemble(np.einsum("eji,ej->ei", dRdpi, lam_ws), 3)
                 gam = (r.reshape(N, 3) / rowsum[:, None]).ravel()
             dfdz += assemble(np.einsum("eji,ej->ei", dRdz, lam_ws), 1)
             dfdz += assemble(np.einsum("eji,ej->ei", dHdz, mu), 1)
-            if self.P.projection_penalty and k > 0:
+            if mixed and self.P.projection_penalty and k > 0:
                 # dQ_{a,d}/dz_b = 1/4 * 1/4 V dw (Pi_gp - ps grad p_k)_d
                 pk = u[3::4][cn]
                 gp = ps * np.einsum("eb,ebd->ed", pk, grads)

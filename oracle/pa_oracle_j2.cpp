@@ -131,6 +131,11 @@ typedef struct {
     double projection_penalty_exponent, projection_min_ersatz;
     // plastic work criterion penalty (src/PlasticWorkCriterion.hpp:295-303)
     double work_penalty_exponent, work_min_ersatz;
+    // 0: the reference's stabilized mixed u-p formulation (4 dof per node);
+    // 1: pure-displacement J2 (3 dof per node), which has no reference
+    //    counterpart: sigma = Dev(mu)(eps - eps_p) + kappa tr(eps - eps_p) I,
+    //    no pressure unknown, stabilization or projection.
+    int formulation;
 } orc_j2_problem;
 
 }  // extern "C"
@@ -154,6 +159,7 @@ inline CellGeom cell_geom(const orc_j2_problem* P, int c) {
 }
 
 inline double shear_modulus(const orc_j2_problem* P) { return P->youngs / (2.0 * (1.0 + P->poisson)); }
+inline int dofs(const orc_j2_problem* P) { return P->formulation ? 3 : 4; }
 inline double bulk_modulus(const orc_j2_problem* P) { return P->youngs / (3.0 * (1.0 - 2.0 * P->poisson)); }
 
 // Strain<3, 4>, src/Strain.hpp:56-95: Voigt (xx, yy, zz, yz, xz, xy),
@@ -263,6 +269,24 @@ void update_cell(const orc_j2_problem* P, const CellGeom& G, const double u[16],
 template <class T>
 void global_residual_cell(const orc_j2_problem* P, const CellGeom& G, const T u[16], const T c[14], const T pi[12],
                           const T z[4], T R[16]) {
+    if (P->formulation) {
+        // pure displacement: R_{n,i} = V sum_j sigma_ij dN_n/dx_j, with the
+        // global residual's penalty on both moduli
+        const double V = G.vol;
+        T eps[6], ee[6], s[6];
+        total_strain(G.g, u, eps);
+        for (int i = 0; i < 6; ++i) ee[i] = eps[i] - c[2 + i];
+        T pen = msimp(cell_density(z), P->global_penalty_exponent, P->global_min_ersatz);
+        T mu = pen * shear_modulus(P);
+        deviatoric_stress(mu, ee, s);
+        T vol = (pen * bulk_modulus(P)) * ((ee[0] + ee[1]) + ee[2]);
+        for (int i = 0; i < 3; ++i) s[i] = s[i] + vol;
+        for (int k = 0; k < 16; ++k) R[k] = 0.0;
+        for (int i = 0; i < 3; ++i)
+            for (int n = 0; n < 4; ++n)
+                for (int j = 0; j < 3; ++j) R[n * 4 + i] += V * s[kVoigt[i][j]] * G.g[n][j];
+        return;
+    }
     const double ps = P->pressure_scaling;
     const double V = G.vol;
     T eps[6], ee[6], s[6];
@@ -326,7 +350,12 @@ T plastic_work_cell(const orc_j2_problem* P, const CellGeom& G, const T u[16], c
         eec[i] = epsc[i] - c[2 + i];
     }
     T rho = cell_density(z);
-    T mu = msimp(rho, P->work_penalty_exponent, P->work_min_ersatz) * shear_modulus(P);
+    const T wpen = msimp(rho, P->work_penalty_exponent, P->work_min_ersatz);
+    if (P->formulation) {  // the volumetric stress takes the pressure's place
+        pp = (wpen * bulk_modulus(P)) * ((eep[0] + eep[1]) + eep[2]);
+        pc = (wpen * bulk_modulus(P)) * ((eec[0] + eec[1]) + eec[2]);
+    }
+    T mu = wpen * shear_modulus(P);
     deviatoric_stress(mu, eep, sp);
     deviatoric_stress(mu, eec, sc);
     T w = 0.0;
@@ -339,9 +368,11 @@ T plastic_work_cell(const orc_j2_problem* P, const CellGeom& G, const T u[16], c
 }
 
 // gathers of one cell
+// cell layout node*4 + dim; the pressure slot is zero in the displacement formulation
 inline void gather_u(const orc_j2_problem* P, const double* u, int c, double out[16]) {
+    const int D = dofs(P);
     for (int n = 0; n < 4; ++n)
-        for (int d = 0; d < 4; ++d) out[n * 4 + d] = u[(size_t)P->conn[(size_t)c * 4 + n] * 4 + d];
+        for (int d = 0; d < 4; ++d) out[n * 4 + d] = d < D ? u[(size_t)P->conn[(size_t)c * 4 + n] * D + d] : 0.0;
 }
 inline void gather_z(const orc_j2_problem* P, const double* z, int c, double out[4]) {
     for (int n = 0; n < 4; ++n) out[n] = z[P->conn[(size_t)c * 4 + n]];
@@ -430,7 +461,7 @@ inline void atomic_add(double* dst, double x) {
 
 extern "C" {
 
-// LocalVectorFunctionInc::updateLocalState. u: 4N, c_prev / c: C*14, z: N.
+// LocalVectorFunctionInc::updateLocalState. u: D N, c_prev / c: C*14, z: N.
 void orc_j2_update_local_state(const orc_j2_problem* P, const double* u, const double* cp, const double* z,
                                double* c) {
     for_cells(P->num_cells, [&](int e) {
@@ -516,8 +547,9 @@ void orc_batched_inverse(int count, int n, const double* A, double* Ainv) {
 
 // One Newton linearization of the condensed system
 // (src/NewtonRaphsonSolver.hpp:204-330):
-//   K_t = dR/du - dR/dc (dH/dc)^-1 dH/du   -> block CRS, 4x4 blocks
-//   R_t = R - dR/dc (dH/dc)^-1 H           -> 4N
+//   K_t = dR/du - dR/dc (dH/dc)^-1 dH/du   -> block CRS, D x D blocks
+//   R_t = R - dR/dc (dH/dc)^-1 H           -> D N
+// (D = 4 in the mixed formulation, 3 in the displacement one).
 // pi: nodal projected pressure gradient (N*3).  Scatter by row search and
 // atomic add (src/Assembly.hpp:1222-1521).  The caller's values/R must be zero.
 void orc_j2_condensed_system(const orc_j2_problem* P, const double* u, const double* c, const double* cp,
@@ -562,19 +594,21 @@ void orc_j2_condensed_system(const orc_j2_problem* P, const double* u, const dou
             for (int k = 0; k < 14; ++k) s += dRdc[i * 14 + k] * y[k];
             Rt[i] = Re[i] - s;
         }
+        const int D = dofs(P);
         for (int a = 0; a < 4; ++a) {
             const int row = P->conn[(size_t)e * 4 + a];
-            for (int i = 0; i < 4; ++i) atomic_add(&R[(size_t)row * 4 + i], Rt[a * 4 + i]);
+            for (int i = 0; i < D; ++i) atomic_add(&R[(size_t)row * D + i], Rt[a * 4 + i]);
             for (int b = 0; b < 4; ++b) {
                 const long ent = entry_ordinal(rowmap, colidx, row, P->conn[(size_t)e * 4 + b]);
-                for (int i = 0; i < 4; ++i)
-                    for (int j = 0; j < 4; ++j) atomic_add(&values[ent * 16 + i * 4 + j], Kt[(a * 4 + i) * 16 + b * 4 + j]);
+                for (int i = 0; i < D; ++i)
+                    for (int j = 0; j < D; ++j)
+                        atomic_add(&values[ent * D * D + i * D + j], Kt[(a * 4 + i) * 16 + b * 4 + j]);
             }
         }
     });
 }
 
-// Assembled global residual R(u, c, pi, z) (4N), without condensation.
+// Assembled global residual R(u, c, pi, z) (D N), without condensation.
 void orc_j2_global_value(const orc_j2_problem* P, const double* u, const double* c, const double* pi,
                          const double* z, double* R) {
     for_cells(P->num_cells, [&](int e) {
@@ -585,8 +619,9 @@ void orc_j2_global_value(const orc_j2_problem* P, const double* u, const double*
         for (int n = 0; n < 4; ++n)
             for (int d = 0; d < 3; ++d) pe[n * 3 + d] = pi[(size_t)P->conn[(size_t)e * 4 + n] * 3 + d];
         global_residual_cell<double>(P, G, ue, c + (size_t)e * 14, pe, ze, Re);
+        const int D = dofs(P);
         for (int a = 0; a < 4; ++a)
-            for (int i = 0; i < 4; ++i) atomic_add(&R[(size_t)P->conn[(size_t)e * 4 + a] * 4 + i], Re[a * 4 + i]);
+            for (int i = 0; i < D; ++i) atomic_add(&R[(size_t)P->conn[(size_t)e * 4 + a] * D + i], Re[a * 4 + i]);
     });
 }
 
@@ -653,7 +688,7 @@ double orc_j2_plastic_work(const orc_j2_problem* P, const double* u, const doubl
 }
 
 // Partials of one step's plastic work (dual numbers), each accumulated into
-// the caller's zeroed arrays: dW/du (4N), dW/du_prev (4N), dW/dc (C*14),
+// the caller's zeroed arrays: dW/du (D N), dW/du_prev (D N), dW/dc (C*14),
 // dW/dc_prev (C*14), dW/dz (N).
 void orc_j2_plastic_work_partials(const orc_j2_problem* P, const double* u, const double* up, const double* c,
                                   const double* cp, const double* z, double* gu, double* gup, double* gc,
@@ -672,11 +707,12 @@ void orc_j2_plastic_work_partials(const orc_j2_problem* P, const double* u, cons
         for (int k = 0; k < 14; ++k) { tc[k] = ce[k]; seed(tc[k], 32 + k); tcp[k] = cpe[k]; seed(tcp[k], 46 + k); }
         for (int k = 0; k < 4; ++k) { tz[k] = ze[k]; seed(tz[k], 60 + k); }
         T w = plastic_work_cell(P, G, tu, tup, tc, tcp, tz);
+        const int D = dofs(P);
         for (int a = 0; a < 4; ++a) {
             const int node = P->conn[(size_t)e * 4 + a];
-            for (int d = 0; d < 4; ++d) {
-                atomic_add(&gu[(size_t)node * 4 + d], w.d[a * 4 + d]);
-                atomic_add(&gup[(size_t)node * 4 + d], w.d[16 + a * 4 + d]);
+            for (int d = 0; d < D; ++d) {
+                atomic_add(&gu[(size_t)node * D + d], w.d[a * 4 + d]);
+                atomic_add(&gup[(size_t)node * D + d], w.d[16 + a * 4 + d]);
             }
             atomic_add(&gz[node], w.d[60 + a]);
         }
