@@ -337,7 +337,19 @@ typedef struct {
     int projection_penalty;                                  /* State Gradient Projection penalty: 0 none, 1 SIMP */
     double projection_penalty_exponent, projection_min_ersatz;
     double work_penalty_exponent, work_min_ersatz;           /* plastic work criterion: 3, 1e-9 */
+    int formulation;                                         /* PA_J2_MIXED (default) or PA_J2_DISPLACEMENT */
 } pa_j2_params;
+/*
+ * PA_J2_MIXED: the reference's stabilized mixed u-p form above (4 dofs per node).
+ * PA_J2_DISPLACEMENT: pure-displacement J2 with no reference counterpart,
+ * sigma = Dev(mu~)(eps - eps_p) + kappa~ tr(eps - eps_p) I with the global
+ * penalty on both moduli, 3 dofs per node (u[node*3 + d], R, lambda, Dirichlet
+ * dofs and 3x3 blocks of the condensed operator), no pressure, stabilization or
+ * projection (pi is read and ignored); the plastic work uses kappa~_w tr(eps_e)
+ * in place of the pressure.  Cell worksets keep the 16 rows/columns node*4 + d,
+ * with the pressure slots zero.
+ */
+enum { PA_J2_MIXED = 0, PA_J2_DISPLACEMENT = 1 };
 /* The reference's defaults (src/J2PlasticityLocalResidual.hpp:195-220,
  * src/InfinitesimalStrainPlasticityResidual.hpp:142-160,214,
  * src/PlasticWorkCriterion.hpp:295-303); moduli and yield stress zero. */
@@ -362,7 +374,7 @@ int pa_j2_local_value(pa_plasticity* p, const double* u_dev, const double* c_dev
  * (src/LocalVectorFunctionInc.hpp:372-874), hand-derived in place of the FAD evaluation types. */
 int pa_j2_local_gradient(pa_plasticity* p, int wrt, const double* u_dev, const double* c_dev,
                          const double* c_prev_dev, const double* z_dev, double* out_dev, void* stream);
-/* GlobalVectorFunctionInc::value: assembled R[node*4 + d]
+/* GlobalVectorFunctionInc::value: assembled R[node*dpn + d] (dpn 4 mixed, 3 displacement)
  * (src/GlobalVectorFunctionInc.hpp:919-960, src/InfinitesimalStrainPlasticityResidual.hpp:393-521). */
 int pa_j2_global_value(pa_plasticity* p, const double* u_dev, const double* c_dev, const double* pi_dev,
                        const double* z_dev, double* R_dev, void* stream);
@@ -373,7 +385,7 @@ int pa_j2_global_gradient(pa_plasticity* p, int wrt, const double* u_dev, const 
 /*
  * One Newton linearization of the condensed system
  * (NewtonRaphsonSolver::assembleTangentMatrix / assembleResidual,
- * src/NewtonRaphsonSolver.hpp:204-330): block-CRS 4x4 values on the mesh graph
+ * src/NewtonRaphsonSolver.hpp:204-330): block-CRS dpn x dpn values on the mesh graph
  *   K_t = dR/du - dR/dc (dH/dc)^-1 dH/du,   R_t = R - dR/dc (dH/dc)^-1 H.
  * transpose != 0 assembles K_t^T (the adjoint's operator,
  * src/PathDependentAdjointSolver.hpp:232-275); R_dev may then be null.
@@ -394,8 +406,8 @@ int pa_j2_plastic_work(pa_plasticity* p, const double* u_dev, const double* u_pr
  * steps with load factor (k+1)/K on the Dirichlet values, the condensed Newton
  * solve per step (elastic trial state, Dirichlet increment on the first
  * iteration, absolute-residual stopping test, src/NewtonRaphsonSolver.hpp:555-610)
- * with flexible GMRES (or BiCGStab) linear solves preconditioned by 4x4 block
- * Jacobi or geometric multigrid, the projection update after
+ * with flexible GMRES (or BiCGStab) linear solves preconditioned by dpn x dpn block
+ * Jacobi or geometric multigrid, the projection update (mixed form) after
  * each step, every step's states kept on the device (the checkpoints of the
  * adjoint), the plastic-work criterion summed over steps and its total
  * gradient by the path-dependent adjoint (src/PathDependentAdjointSolver.hpp).

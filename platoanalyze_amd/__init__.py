@@ -66,7 +66,10 @@ class J2Params(ctypes.Structure):
         "global_penalty_exponent", "global_min_ersatz", "pressure_scaling")] + [
         ("projection_penalty", ctypes.c_int), ("projection_penalty_exponent", ctypes.c_double),
         ("projection_min_ersatz", ctypes.c_double), ("work_penalty_exponent", ctypes.c_double),
-        ("work_min_ersatz", ctypes.c_double)]
+        ("work_min_ersatz", ctypes.c_double), ("formulation", ctypes.c_int)]
+
+
+J2_MIXED, J2_DISPLACEMENT = 0, 1
 
 
 class PlasticityOptions(ctypes.Structure):
@@ -662,10 +665,16 @@ class Multigrid:
 # J2 plasticity
 # ---------------------------------------------------------------------------
 def j2_params(**kw):
-    """pa_j2_params with the reference's defaults, overridden by keywords."""
+    """pa_j2_params with the reference's defaults, overridden by keywords.
+    formulation: "mixed" (the reference's stabilized u-p form, 4 dofs per node)
+    or "displacement" (pure-displacement J2, 3 dofs per node)."""
     p = J2Params()
     lib.pa_j2_params_default(ctypes.byref(p))
     for k, v in kw.items():
+        if k == "formulation" and isinstance(v, str):
+            if v not in ("mixed", "displacement"):
+                raise PlatoError(f"unknown J2 formulation {v!r}")
+            v = J2_MIXED if v == "mixed" else J2_DISPLACEMENT
         if not hasattr(p, k):
             raise PlatoError(f"unknown J2 parameter {k!r}")
         setattr(p, k, v)
@@ -678,8 +687,10 @@ class Plasticity:
     LocalVectorFunctionInc (src/LocalVectorFunctionInc.hpp) for
     Physics 'Plasticity', plus the projection and the plastic-work criterion.
 
-    u: 4 dofs per node (ux, uy, uz, p); c: 14 local states per cell;
-    pi: 3 per node.  Worksets come back as (C, rows, W) numpy arrays."""
+    u: 4 dofs per node (ux, uy, uz, p), or 3 (ux, uy, uz) with
+    formulation="displacement"; c: 14 local states per cell; pi: 3 per node.
+    Worksets come back as (C, rows, W) numpy arrays with the local dof
+    node*4 + d in either formulation."""
 
     def __init__(self, mesh, **params):
         self.mesh = mesh
@@ -688,6 +699,7 @@ class Plasticity:
         _check(lib.pa_plasticity_create(mesh.handle, ctypes.byref(self.params), ctypes.byref(h)))
         self.handle = h
         self.N, self.C = mesh.num_nodes, mesh.num_cells
+        self.dpn = 4 if self.params.formulation == J2_MIXED else 3
 
     # LocalVectorFunctionInc
     def update_local_state(self, u, c_prev, z, out=None, stream=None):
@@ -712,7 +724,7 @@ class Plasticity:
     # GlobalVectorFunctionInc
     def global_value(self, u, c, pi, z, out=None, stream=None):
         u, c, pi, z = _dev(u), _dev(c), _dev(pi), _dev(z)
-        out = out if out is not None else DeviceArray(self.N * 4)
+        out = out if out is not None else DeviceArray(self.N * self.dpn)
         _check(lib.pa_j2_global_value(self.handle, u.ptr, c.ptr, pi.ptr, z.ptr, out.ptr, _sh(stream)))
         return out
 
@@ -724,13 +736,14 @@ class Plasticity:
         return out.numpy().reshape(self.C, 16, W)
 
     def condensed_system(self, u, c, c_prev, pi, z, transpose=False, values=None, R=None, stream=None):
-        """(CrsMatrix 4x4 of K_t or K_t^T, DeviceArray R_t or None)."""
+        """(CrsMatrix dpn x dpn of K_t or K_t^T, DeviceArray R_t or None)."""
         u, c, cp, pi, z = _dev(u), _dev(c), _dev(c_prev), _dev(pi), _dev(z)
-        values = values if values is not None else DeviceArray(self.mesh.num_blocks * 16)
-        R = None if transpose else (R if R is not None else DeviceArray(self.N * 4))
+        D = self.dpn
+        values = values if values is not None else DeviceArray(self.mesh.num_blocks * D * D)
+        R = None if transpose else (R if R is not None else DeviceArray(self.N * D))
         _check(lib.pa_j2_condensed_system(self.handle, u.ptr, c.ptr, cp.ptr, pi.ptr, z.ptr, values.ptr,
                                           R.ptr if R is not None else None, 1 if transpose else 0, _sh(stream)))
-        return CrsMatrix(self.mesh, 4, 4, values, FP64), R
+        return CrsMatrix(self.mesh, D, D, values, FP64), R
 
     def projection(self, u, z, out=None, stream=None):
         u, z = _dev(u), _dev(z)
@@ -772,7 +785,7 @@ class PlasticityProblem:
 
     def set_multigrid(self, nx, ny, nz, *, max_levels=0, pre_sweeps=2, post_sweeps=2, omega=0.6,
                       max_coarse_nodes=1000):
-        """Geometric multigrid (4x4 nodal blocks) as the BiCGStab preconditioner."""
+        """Geometric multigrid (dpn x dpn nodal blocks) as the Krylov preconditioner."""
         opts = MultigridOptions(max_levels, pre_sweeps, post_sweeps, omega, max_coarse_nodes)
         _check(lib.pa_plasticity_problem_set_multigrid(self.handle, nx, ny, nz, ctypes.byref(opts)))
 
@@ -788,7 +801,7 @@ class PlasticityProblem:
         _check(lib.pa_plasticity_problem_state(self.handle, step, ctypes.byref(pu), ctypes.byref(pc),
                                                ctypes.byref(pp)))
         out = []
-        for ptr, n in ((pu, self.pl.N * 4), (pc, self.pl.C * 14), (pp, self.pl.N * 3)):
+        for ptr, n in ((pu, self.pl.N * self.pl.dpn), (pc, self.pl.C * 14), (pp, self.pl.N * 3)):
             a = np.empty(n)
             _check(lib.pa_memcpy_d2h(_np_ptr(a), ptr, a.nbytes))
             out.append(a)

@@ -15,6 +15,7 @@ struct pa_plasticity {
     pa::DevBuf<double> rowsum;  // N: lumped projection mass
     pa::DevBuf<double> tmp4;    // N*4: projection accumulators
     pa::DevBuf<double> zero4;   // N*4 zeros
+    int dpn = 4;                // global dofs per node of the formulation
 };
 
 namespace pa {
@@ -30,6 +31,8 @@ struct KP {
     int proj;                          // projection penalty on/off
     double pj_e, pj_m;                 // projection penalty
     double pw_e, pw_m;                 // plastic work penalty
+    int mixed;                         // 1: stabilized mixed u-p; 0: pure displacement
+    int dpn;                           // global dofs per node: 4 (mixed) or 3
 };
 
 KP constants(const pa_j2_params& p);
@@ -73,7 +76,7 @@ void gather_uniform(pa_plasticity* p, int W, const double* ws, double scale, boo
 // the same with a cell stride other than W
 void gather_uniform(pa_plasticity* p, int W, const double* ws, int stride, double scale, bool accumulate,
                     double* out, hipStream_t st);
-// node gather of per-local-node cell vectors (vcell, 4 dofs per node): out[node*4 + d] = scale * sum
+// node gather of per-local-node cell vectors (vcell, layout node*4 + d): out[node*dpn + d] = scale * sum, d < dpn
 void gather_vcell(pa_plasticity* p, double scale, double* out, hipStream_t st);
 // gamma = v / rowsum (node-wise, 3 components), rowsum of the projection's lumped mass
 void divide_rowsum(pa_plasticity* p, double* gamma, const double* z, hipStream_t st);

@@ -137,16 +137,17 @@ __device__ inline void cell_geometry(const double4* __restrict__ X, const int nn
     V = fabs(det) * (1.0 / 6.0);
 }
 
-__device__ inline void strain_of(const double g[4][3], const double* __restrict__ u, const int nn[4], double eps[6],
-                                 double p[4]) {
+// u holds D dofs per node: (ux, uy, uz, p) for D = 4, (ux, uy, uz) and p = 0 for D = 3.
+__device__ inline void strain_of(const double g[4][3], const double* __restrict__ u, int D, const int nn[4],
+                                 double eps[6], double p[4]) {
     double uu[4][3];
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
-        const double* un = u + (long)nn[n] * 4;
+        const double* un = u + (long)nn[n] * D;
         uu[n][0] = un[0];
         uu[n][1] = un[1];
         uu[n][2] = un[2];
-        p[n] = un[3];
+        p[n] = D == 4 ? un[3] : 0.0;
     }
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -168,14 +169,15 @@ __device__ inline void strain_of(const double g[4][3], const double* __restrict_
 }
 
 __device__ inline void load_cell(const int* __restrict__ conn, int e, const double4* __restrict__ X,
-                                 const double* __restrict__ u, const double* __restrict__ z, CellIn& C, int nn[4]) {
+                                 const double* __restrict__ u, int D, const double* __restrict__ z, CellIn& C,
+                                 int nn[4]) {
     const int4 q = reinterpret_cast<const int4*>(conn)[e];
     nn[0] = q.x;
     nn[1] = q.y;
     nn[2] = q.z;
     nn[3] = q.w;
     cell_geometry(X, nn, C.g, C.V);
-    strain_of(C.g, u, nn, C.eps, C.p);
+    strain_of(C.g, u, D, nn, C.eps, C.p);
     C.rho = (z[nn[0]] + z[nn[1]] + z[nn[2]] + z[nn[3]]) / 4.0;
 }
 
@@ -309,7 +311,8 @@ struct Glob {
     double pg, dpg, kG;
     double sG[6];  // Dev(pg mu)(eps - eps_p)
     double s0[6];  // Dev(mu)(eps - eps_p)
-    double pgp;    // ps * interpolated pressure
+    double pgp;    // mixed: ps * interpolated pressure; displacement: kG tr(eps - eps_p)
+    double tre;    // tr(eps - eps_p)
     double stab[3];
     double tau, volstrain;
 };
@@ -333,6 +336,14 @@ __device__ inline void global_ctx(const KP& k, const CellIn& C, const double c[1
 #pragma unroll
     for (int n = 0; n < 4; ++n) p += 0.25 * C.p[n];
     G.kG = G.pg * k.kappa;
+    G.tre = (ee[0] + ee[1]) + ee[2];
+    if (!k.mixed) {  // the volumetric stress takes the pressure's place; no stabilization
+        G.pgp = G.kG * G.tre;
+        G.volstrain = G.tau = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) G.stab[d] = 0.0;
+        return;
+    }
     G.pgp = p * k.ps;
     const double div = C.eps[0] + C.eps[1] + C.eps[2];
     G.volstrain = k.ps * (div - G.pgp / G.kG);
@@ -341,8 +352,8 @@ __device__ inline void global_ctx(const KP& k, const CellIn& C, const double c[1
     for (int d = 0; d < 3; ++d) G.stab[d] = k.ps * G.tau * ((k.ps * gradp[d]) - pig[d]);
 }
 
-// R_{a,i}, row r = a*4 + i.
-__device__ inline double global_residual_row(const CellIn& C, const Glob& G, const double ga[3], int i) {
+// R_{a,i}, row r = a*4 + i (the pressure row i = 3 is empty in the displacement formulation).
+__device__ inline double global_residual_row(const KP& k, const CellIn& C, const Glob& G, const double ga[3], int i) {
     if (i < 3) {
         double srow[3];
         tensor_row(G.sG, i, srow);
@@ -352,6 +363,7 @@ __device__ inline double global_residual_row(const CellIn& C, const Glob& G, con
         R += C.V * G.pgp * sel<3>(ga, i);
         return R;
     }
+    if (!k.mixed) return 0.0;
     double R = 0.0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) R += -1.0 * G.stab[d] * ga[d] * C.V;
@@ -368,7 +380,17 @@ __device__ inline void dr_du_row(const KP& k, const CellIn& C, const Glob& G, co
     for (int b = 0; b < 4; ++b) {
         const double* gb = C.g[b];
         const double dot = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
-        if (i < 3) {
+        if (!k.mixed) {
+            // V (muG (dot delta_ij + g_aj g_bi - 2/3 g_ai g_bj) + kG g_ai g_bj); no pressure row or column
+            const double gbi = sel<3>(C.g[b], i);
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                out[b * 4 + j] =
+                    i < 3 ? C.V * muG * ((i == j ? dot : 0.0) + ga[j] * gbi - (2.0 / 3.0) * gai * gb[j]) +
+                                C.V * G.kG * gai * gb[j]
+                          : 0.0;
+            out[b * 4 + 3] = 0.0;
+        } else if (i < 3) {
             const double gbi = sel<3>(C.g[b], i);
 #pragma unroll
             for (int j = 0; j < 3; ++j)
@@ -398,6 +420,9 @@ __device__ inline void dr_dep_row(const KP& k, const CellIn& C, const Glob& G, c
     out[3] = -C.V * muG * (i == 1 ? ga[2] : (i == 2 ? ga[1] : 0.0));
     out[4] = -C.V * muG * (i == 0 ? ga[2] : (i == 2 ? ga[0] : 0.0));
     out[5] = -C.V * muG * (i == 0 ? ga[1] : (i == 1 ? ga[0] : 0.0));
+    if (!k.mixed)
+#pragma unroll
+        for (int w = 0; w < 3; ++w) out[w] -= C.V * G.kG * gai;
 }
 
 // d R_{a,i} / d rho.
@@ -405,8 +430,10 @@ __device__ inline double dr_drho_row(const KP& k, const CellIn& C, const Glob& G
     if (i < 3) {
         double srow[3];
         tensor_row(G.s0, i, srow);
-        return C.V * G.dpg * (srow[0] * ga[0] + srow[1] * ga[1] + srow[2] * ga[2]);
+        const double d = C.V * G.dpg * (srow[0] * ga[0] + srow[1] * ga[1] + srow[2] * ga[2]);
+        return k.mixed ? d : d + C.V * G.dpg * k.kappa * G.tre * sel<3>(ga, i);
     }
+    if (!k.mixed) return 0.0;
     return 0.25 * C.V * k.ps * G.pgp * G.dpg / (G.pg * G.pg * k.kappa);
 }
 
@@ -439,7 +466,7 @@ __global__ __launch_bounds__(kB) void k_update(int C, KP k, const int* __restric
     if (e >= C) return;
     CellIn Ci;
     int nn[4];
-    load_cell(conn, e, X, u, z, Ci, nn);
+    load_cell(conn, e, X, u, k.dpn, z, Ci, nn);
     double c[14];
     load14(cp, e, c);
     c[1] = 0.0;
@@ -473,7 +500,7 @@ __global__ __launch_bounds__(kB) void k_local(int C, int mode, int wrt, KP k, co
     if (e >= C) return;
     CellIn Ci;
     int nn[4];
-    load_cell(conn, e, X, u, z, Ci, nn);
+    load_cell(conn, e, X, u, k.dpn, z, Ci, nn);
     double c[14], cp[14];
     load14(c_in, e, c);
     load14(cp_in, e, cp);
@@ -526,7 +553,7 @@ __global__ __launch_bounds__(kB) void k_global(int C, int mode, int wrt, KP k, c
     const int a = r / 4, i = r % 4;
     CellIn Ci;
     int nn[4];
-    load_cell(conn, e, X, u, z, Ci, nn);
+    load_cell(conn, e, X, u, k.dpn, z, Ci, nn);
     double c[14];
     load14(c_in, e, c);
     double pig[3];
@@ -536,7 +563,7 @@ __global__ __launch_bounds__(kB) void k_global(int C, int mode, int wrt, KP k, c
     double ga[3];
     pick_node(Ci.g, a, ga);
     if (mode == 0) {
-        out[(long)e * 16 + r] = global_residual_row(Ci, G, ga, i);
+        out[(long)e * 16 + r] = global_residual_row(k, Ci, G, ga, i);
         return;
     }
     if (wrt == PA_WRT_U) {
@@ -566,7 +593,7 @@ __global__ __launch_bounds__(kB) void k_global(int C, int mode, int wrt, KP k, c
 #pragma unroll
         for (int b = 0; b < 4; ++b)
 #pragma unroll
-            for (int d = 0; d < 3; ++d) o[b * 3 + d] = (i == 3) ? Ci.V * k.ps * G.tau * 0.25 * ga[d] : 0.0;
+            for (int d = 0; d < 3; ++d) o[b * 3 + d] = (i == 3 && k.mixed) ? Ci.V * k.ps * G.tau * 0.25 * ga[d] : 0.0;
     }
 }
 
@@ -619,7 +646,7 @@ __global__ __launch_bounds__(kB) void k_condensed(int C, KP k, const int* __rest
     const int e = valid ? e0 : C - 1;
     CellIn Ci;
     int nn[4];
-    load_cell(conn, e, X, u, z, Ci, nn);
+    load_cell(conn, e, X, u, k.dpn, z, Ci, nn);
     double c[14], cp[14];
     load14(c_in, e, c);
     load14(cp_in, e, cp);
@@ -654,7 +681,7 @@ __global__ __launch_bounds__(kB) void k_condensed(int C, KP k, const int* __rest
     pick_node(Ci.g, a, ga);
     double K[16];
     dr_du_row(k, Ci, Gl, ga, i, K);
-    double Rt = global_residual_row(Ci, Gl, ga, i);
+    double Rt = global_residual_row(k, Ci, Gl, ga, i);
     double dep[6];
     dr_dep_row(k, Ci, Gl, ga, i, dep);
 #pragma unroll
@@ -724,7 +751,7 @@ __global__ __launch_bounds__(kB) void k_adj_local_rhs(int C, KP k, const int* __
     const int e = valid ? e0 : C - 1;
     CellIn Ci;
     int nn[4];
-    load_cell(conn, e, X, u, z, Ci, nn);
+    load_cell(conn, e, X, u, k.dpn, z, Ci, nn);
     double c[14];
     load14(c_in, e, c);
     Loc L;
@@ -755,7 +782,7 @@ __global__ __launch_bounds__(kB) void k_adj_local_rhs(int C, KP k, const int* __
     double out = wu[(long)e * 16 + lane] - F;
     if (wu_next) {
         out += wu_next[(long)e * 16 + lane];
-        if (i == 3) {
+        if (i == 3 && k.mixed) {
             // DpDn^T gamma: dQ_{a',d}/dp_b = -1/4 V w ps dN_b/dx_d (src/PressureGradientProjectionResidual.hpp:136-160)
             double w = 1.0, dw;
             if (k.proj) msimp(Ci.rho, k.pj_e, k.pj_m, w, dw);
@@ -796,7 +823,7 @@ __global__ __launch_bounds__(kB) void k_adj_local_update(int C, KP k, const int*
     const int e = valid ? e0 : C - 1;
     CellIn Ci;
     int nn[4];
-    load_cell(conn, e, X, u, z, Ci, nn);
+    load_cell(conn, e, X, u, k.dpn, z, Ci, nn);
     double c[14];
     load14(c_in, e, c);
     Loc L;
@@ -809,7 +836,7 @@ __global__ __launch_bounds__(kB) void k_adj_local_update(int C, KP k, const int*
     const int a = lane / 4, i = lane % 4;
     double ga[3];
     pick_node(Ci.g, a, ga);
-    const double lam = lambda[(long)sel_int4(nn, a) * 4 + i];
+    const double lam = i < k.dpn ? lambda[(long)sel_int4(nn, a) * k.dpn + i] : 0.0;
     double dep[6];
     dr_dep_row(k, Ci, Gl, ga, i, dep);
     double drl[6];
@@ -841,7 +868,9 @@ __global__ __launch_bounds__(kB) void k_adj_local_update(int C, KP k, const int*
 // ---------------------------------------------------------------------------
 // node-ordered gathers (fixed summation order: incident cells ascending)
 // ---------------------------------------------------------------------------
-// Block-CRS values (4x4) and the residual of each graph row; lane = block entry.
+// Block-CRS values (D x D blocks of the 4x4 cell blocks: D = 3 drops the
+// pressure row and column) and the residual of each graph row; lane = 4x4 block entry.
+template <int D>
 __global__ __launch_bounds__(kB) void k_gather_matrix(int N, const int* __restrict__ rowmap,
                                                       const int* __restrict__ n2c_off,
                                                       const int2* __restrict__ pairs,
@@ -871,19 +900,20 @@ __global__ __launch_bounds__(kB) void k_gather_matrix(int N, const int* __restri
         }
         if (lane < 4) racc += vcell[e * 16 + a * 4 + lane];
     }
-    double* out = values + (long)rowmap[row] * 16;
-    for (int s = 0; s < len; ++s) out[s * 16 + lane] = my[s * 16 + lane];
-    if (R && lane < 4) R[(long)row * 4 + lane] = racc;
+    double* out = values + (long)rowmap[row] * (D * D);
+    if (i < D && j < D)
+        for (int s = 0; s < len; ++s) out[s * (D * D) + i * D + j] = my[s * 16 + lane];
+    if (R && lane < D) R[(long)row * D + lane] = racc;
 }
 
-// out[node*4 + d] = scale * sum over incident cells of vcell[cell*16 + a*4 + d].
-__global__ __launch_bounds__(kB) void k_gather_vcell(int N, const int* __restrict__ n2c_off,
+// out[node*D + d] = scale * sum over incident cells of vcell[cell*16 + a*4 + d], d < D.
+__global__ __launch_bounds__(kB) void k_gather_vcell(int N, int D, const int* __restrict__ n2c_off,
                                                      const int2* __restrict__ pairs,
                                                      const double* __restrict__ vcell, double scale,
                                                      double* __restrict__ out) {
     const long t = (long)blockIdx.x * kB + threadIdx.x;
-    if (t >= (long)N * 4) return;
-    const int node = (int)(t / 4), d = (int)(t % 4);
+    if (t >= (long)N * D) return;
+    const int node = (int)(t / D), d = (int)(t % D);
     double s = 0.0;
     for (int p = n2c_off[node]; p < n2c_off[node + 1]; ++p) {
         const int2 pr = pairs[p];
@@ -913,7 +943,7 @@ __global__ __launch_bounds__(kB) void k_proj_cells(int C, KP k, const int* __res
     if (e >= C) return;
     CellIn Ci;
     int nn[4];
-    load_cell(conn, e, X, u, z, Ci, nn);
+    load_cell(conn, e, X, u, k.dpn, z, Ci, nn);
     double w = 1.0, dw;
     if (k.proj) msimp(Ci.rho, k.pj_e, k.pj_m, w, dw);
     double* o = vcell + (long)e * 16;
@@ -956,7 +986,7 @@ __global__ __launch_bounds__(kB) void k_proj_control(int C, KP k, const int* __r
     if (e >= C) return;
     CellIn Ci;
     int nn[4];
-    load_cell(conn, e, X, u, z, Ci, nn);
+    load_cell(conn, e, X, u, k.dpn, z, Ci, nn);
     double w, dw;
     msimp(Ci.rho, k.pj_e, k.pj_m, w, dw);
     double pig[3];
@@ -1009,12 +1039,18 @@ __device__ inline void work_ctx(const KP& k, const CellIn& Cc, const double epsp
     }
     Pp = Pp * k.ps;
     Pc = Pc * k.ps;
+    // displacement formulation: the volumetric stress kappa~_w tr(eps_e) takes the pressure's place
+    const double trc = (eec[0] + eec[1]) + eec[2], trp = (eep[0] + eep[1]) + eep[2];
+    if (!k.mixed) {
+        Pp = (W.pw * k.kappa) * trp;
+        Pc = (W.pw * k.kappa) * trc;
+    }
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
         const double sigc = i < 3 ? sc[i] + Pc : sc[i];
         const double sigp = i < 3 ? sp[i] + Pp : sp[i];
         W.sigsum[i] = sigc + sigp;
-        W.s0sum[i] = s0c[i] + s0p[i];
+        W.s0sum[i] = s0c[i] + s0p[i] + ((!k.mixed && i < 3) ? k.kappa * (trc + trp) : 0.0);
     }
 }
 
@@ -1024,9 +1060,9 @@ __device__ inline void load_work_cell(const KP& k, const int* __restrict__ conn,
                                       const double* __restrict__ cp_in, const double* __restrict__ z, CellIn& Ci,
                                       double c[14], double cp[14], Work& W) {
     int nn[4];
-    load_cell(conn, e, X, u, z, Ci, nn);
+    load_cell(conn, e, X, u, k.dpn, z, Ci, nn);
     double epsp[6], pp[4];
-    strain_of(Ci.g, up, nn, epsp, pp);
+    strain_of(Ci.g, up, k.dpn, nn, epsp, pp);
     load14(c_in, e, c);
     load14(cp_in, e, cp);
     work_ctx(k, Ci, epsp, pp, c, cp, W);
@@ -1089,10 +1125,14 @@ __global__ __launch_bounds__(kB) void k_work_partials(int C, KP k, double mult, 
     double c[14], cp[14];
     Work W;
     load_work_cell(k, conn, e, X, u, up, c_in, cp_in, z, Ci, c, cp, W);
+    // m = (d sigma / d eps_e)^T delta (the elastic operator is symmetric)
     double m[6];
     dev(W.pw * k.mu, W.delta, m);
     const double hv = 0.5 * W.V * mult;
     const double trd = W.delta[0] + W.delta[1] + W.delta[2];
+    if (!k.mixed)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) m[i] += (W.pw * k.kappa) * trd;
     double* o = wu + (long)e * 16;
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
@@ -1101,7 +1141,7 @@ __global__ __launch_bounds__(kB) void k_work_partials(int C, KP k, double mult, 
         o[b * 4 + 0] = hv * v[0];
         o[b * 4 + 1] = hv * v[1];
         o[b * 4 + 2] = hv * v[2];
-        o[b * 4 + 3] = hv * k.ps * 0.25 * trd;
+        o[b * 4 + 3] = k.mixed ? hv * k.ps * 0.25 * trd : 0.0;
     }
     double* oc = gc + (long)e * 14;
     double* op = gcp + (long)e * 14;
@@ -1142,6 +1182,8 @@ KP constants(const pa_j2_params& p) {
     k.pj_m = p.projection_min_ersatz;
     k.pw_e = p.work_penalty_exponent;
     k.pw_m = p.work_min_ersatz;
+    k.mixed = p.formulation == PA_J2_MIXED ? 1 : 0;
+    k.dpn = k.mixed ? 4 : 3;
     return k;
 }
 
@@ -1158,7 +1200,8 @@ void update_local_state(pa_plasticity* p, const double* u, const double* cp, con
 
 void gather_vcell(pa_plasticity* p, double scale, double* out, hipStream_t st) {
     const pa_mesh* m = p->mesh;
-    k_gather_vcell<<<blocks(4L * m->N), kB, 0, st>>>(m->N, m->d_n2c_off.p, m->d_pairs.p, p->vcell.p, scale, out);
+    k_gather_vcell<<<blocks((long)p->dpn * m->N), kB, 0, st>>>(m->N, p->dpn, m->d_n2c_off.p, m->d_pairs.p,
+                                                               p->vcell.p, scale, out);
     check();
 }
 
@@ -1183,9 +1226,10 @@ void condensed_system(pa_plasticity* p, const double* u, const double* c, const 
     check();
     const int maxlen = m->max_row_len;
     const size_t lds = (size_t)CPB * maxlen * 16 * sizeof(double);
-    k_gather_matrix<<<(m->N + CPB - 1) / CPB, kB, lds, st>>>(m->N, m->d_rowmap.p, m->d_n2c_off.p, m->d_pairs.p,
-                                                             p->kcell.p, p->vcell.p, transpose ? 1 : 0, maxlen,
-                                                             values, transpose ? nullptr : R);
+    auto* gather = p->dpn == 4 ? k_gather_matrix<4> : k_gather_matrix<3>;
+    gather<<<(m->N + CPB - 1) / CPB, kB, lds, st>>>(m->N, m->d_rowmap.p, m->d_n2c_off.p, m->d_pairs.p, p->kcell.p,
+                                                    p->vcell.p, transpose ? 1 : 0, maxlen, values,
+                                                    transpose ? nullptr : R);
     check();
 }
 
@@ -1283,6 +1327,7 @@ void pa_j2_params_default(pa_j2_params* p) {
     p->projection_min_ersatz = 1e-9;
     p->work_penalty_exponent = 3.0;
     p->work_min_ersatz = 1e-9;
+    p->formulation = PA_J2_MIXED;
 }
 
 int pa_plasticity_create(const pa_mesh* mesh, const pa_j2_params* params, pa_plasticity** out) {
@@ -1290,12 +1335,15 @@ int pa_plasticity_create(const pa_mesh* mesh, const pa_j2_params* params, pa_pla
         PA_REQUIRE(mesh && params && out, PA_ERR_ARG, "pa_plasticity_create: null argument");
         PA_REQUIRE(params->youngs > 0.0 && params->poisson > -1.0 && params->poisson < 0.5, PA_ERR_ARG,
                    "pa_plasticity_create: elastic constants out of range");
+        PA_REQUIRE(params->formulation == PA_J2_MIXED || params->formulation == PA_J2_DISPLACEMENT, PA_ERR_ARG,
+                   "pa_plasticity_create: formulation must be PA_J2_MIXED or PA_J2_DISPLACEMENT");
         PA_REQUIRE(mesh->max_row_len <= 32, PA_ERR_LIMIT,
                    "pa_plasticity_create: graph rows above 32 blocks are not supported by the gather kernel");
         auto* p = new pa_plasticity();
         try {
             p->mesh = mesh;
             p->prm = *params;
+            p->dpn = params->formulation == PA_J2_MIXED ? 4 : 3;
             p->kcell.alloc(256L * mesh->C);
             p->vcell.alloc(16L * mesh->C);
             p->part.alloc((size_t)std::max(1L, (mesh->C + 255L) / 256));

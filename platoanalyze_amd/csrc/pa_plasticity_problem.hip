@@ -3,9 +3,10 @@
 // its path-dependent adjoint gradient.
 //
 // Linear solves: restarted flexible GMRES (default) or BiCGStab, with the
-// inverses of the 4x4 nodal (ux, uy, uz, p) diagonal blocks of the
-// saddle-point tangent or the geometric multigrid on 4x4 blocks as the
-// preconditioner.  Reductions are fixed-order, so runs are deterministic.
+// inverses of the D x D nodal diagonal blocks of the tangent (D = 4: ux, uy,
+// uz, p of the saddle-point form; D = 3 in the pure-displacement form) or the
+// geometric multigrid on D x D blocks as the preconditioner.  Reductions are
+// fixed-order, so runs are deterministic.
 //
 // Every step's u, c and pi stay on the device: they are the checkpoints the
 // backward recursion reads (a 64x32x32-hex problem with 10 steps holds about
@@ -87,35 +88,38 @@ __global__ void k_reduce(const double* __restrict__ part, int n, int nparts, int
     }
 }
 
-// y = A x (4x4 blocks, 16 lanes per block row); partial sums of y.a1 and y.a2.
-__global__ __launch_bounds__(kB) void k_spmv4(int N, const int* __restrict__ rowmap, const int* __restrict__ colidx,
-                                              const double* __restrict__ vals, const double* __restrict__ x,
-                                              double* __restrict__ y, const double* __restrict__ a1,
-                                              const double* __restrict__ a2, double* __restrict__ part,
-                                              int nparts) {
+// y = A x (D x D blocks, 16 lanes per block row); partial sums of y.a1 and y.a2.
+template <int D>
+__global__ __launch_bounds__(kB) void k_spmv(int N, const int* __restrict__ rowmap, const int* __restrict__ colidx,
+                                             const double* __restrict__ vals, const double* __restrict__ x,
+                                             double* __restrict__ y, const double* __restrict__ a1,
+                                             const double* __restrict__ a2, double* __restrict__ part,
+                                             int nparts) {
     constexpr int G = 16, ROWS = kB / G;
     __shared__ double sh[kB / 64];
     const int grp = threadIdx.x / G, lane = threadIdx.x % G;
     const int row = blockIdx.x * ROWS + grp;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    double acc[D] = {};
     if (row < N) {
         for (int e = rowmap[row] + lane; e < rowmap[row + 1]; e += G) {
             const long c = colidx[e];
-            const double xv[4] = {x[c * 4], x[c * 4 + 1], x[c * 4 + 2], x[c * 4 + 3]};
-            const double* b = vals + (long)e * 16;
+            double xv[D];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < D; ++j) xv[j] = x[c * D + j];
+            const double* b = vals + (long)e * (D * D);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i] += b[i * 4 + j] * xv[j];
+            for (int i = 0; i < D; ++i)
+#pragma unroll
+                for (int j = 0; j < D; ++j) acc[i] += b[i * D + j] * xv[j];
         }
     }
     double d1 = 0.0, d2 = 0.0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < D; ++i) {
 #pragma unroll
         for (int o = G / 2; o > 0; o >>= 1) acc[i] += __shfl_xor(acc[i], o, G);
         if (row < N && lane == 0) {
-            const long o = (long)row * 4 + i;
+            const long o = (long)row * D + i;
             y[o] = acc[i];
             if (a1) d1 += acc[i] * a1[o];
             if (a2) d2 += acc[i] * a2[o];
@@ -128,61 +132,68 @@ __global__ __launch_bounds__(kB) void k_spmv4(int N, const int* __restrict__ row
     }
 }
 
-// Inverse of each 4x4 diagonal block, Gauss-Jordan with partial pivoting.
-__global__ void k_block_inverse4(int N, const int* __restrict__ rowmap, const int* __restrict__ colidx,
-                                 const double* __restrict__ vals, double* __restrict__ minv) {
+// Inverse of each D x D diagonal block, Gauss-Jordan with partial pivoting.
+template <int D>
+__global__ void k_block_inverse(int N, const int* __restrict__ rowmap, const int* __restrict__ colidx,
+                                const double* __restrict__ vals, double* __restrict__ minv) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     int e = rowmap[n];
     while (colidx[e] != n) ++e;
-    double A[4][8];
+    double A[D][2 * D];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < D; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            A[i][j] = vals[(long)e * 16 + i * 4 + j];
-            A[i][4 + j] = (i == j) ? 1.0 : 0.0;
+        for (int j = 0; j < D; ++j) {
+            A[i][j] = vals[(long)e * (D * D) + i * D + j];
+            A[i][D + j] = (i == j) ? 1.0 : 0.0;
         }
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < D; ++k) {
         int piv = k;
         double best = fabs(A[k][k]);
 #pragma unroll
-        for (int i = k + 1; i < 4; ++i)
+        for (int i = k + 1; i < D; ++i)
             if (fabs(A[i][k]) > best) {
                 best = fabs(A[i][k]);
                 piv = i;
             }
 #pragma unroll
-        for (int i = k + 1; i < 4; ++i)
+        for (int i = k + 1; i < D; ++i)
             if (i == piv)
 #pragma unroll
-                for (int j = 0; j < 8; ++j) {
+                for (int j = 0; j < 2 * D; ++j) {
                     const double tmp = A[k][j];
                     A[k][j] = A[i][j];
                     A[i][j] = tmp;
                 }
         const double inv = A[k][k] != 0.0 ? 1.0 / A[k][k] : 0.0;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) A[k][j] *= inv;
+        for (int j = 0; j < 2 * D; ++j) A[k][j] *= inv;
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < D; ++i)
             if (i != k) {
                 const double f = A[i][k];
 #pragma unroll
-                for (int j = 0; j < 8; ++j) A[i][j] -= f * A[k][j];
+                for (int j = 0; j < 2 * D; ++j) A[i][j] -= f * A[k][j];
             }
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < D; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) minv[(long)n * 16 + i * 4 + j] = A[i][4 + j];
+        for (int j = 0; j < D; ++j) minv[(long)n * (D * D) + i * D + j] = A[i][D + j];
 }
 
-__device__ inline void prec4(const double* __restrict__ minv, long n, const double in[4], double out[4]) {
-    const double* m = minv + n * 16;
+template <int D>
+__device__ inline void prec_block(const double* __restrict__ minv, long n, const double in[D], double out[D]) {
+    const double* m = minv + n * (D * D);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) out[i] = m[i * 4] * in[0] + m[i * 4 + 1] * in[1] + m[i * 4 + 2] * in[2] + m[i * 4 + 3] * in[3];
+    for (int i = 0; i < D; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < D; ++j) s += m[i * D + j] * in[j];
+        out[i] = s;
+    }
 }
 
 // r = b - q, rh = r, p = v = 0; partials of r.r (-> rho and rr) and b.b.
@@ -209,6 +220,7 @@ __global__ __launch_bounds__(kB) void k_bicg_init(long n, const double* __restri
 }
 
 // p = r + beta (p - omega v); y = M p; beta = (rho / rho_old)(alpha / omega).
+template <int D>
 __global__ __launch_bounds__(kB) void k_bicg_dir(int N, const double* __restrict__ scal,
                                                  const double* __restrict__ minv, const double* __restrict__ r,
                                                  const double* __restrict__ v, double* __restrict__ p,
@@ -217,37 +229,38 @@ __global__ __launch_bounds__(kB) void k_bicg_dir(int N, const double* __restrict
     const double omega = scal[S_TS] / scal[S_TT];
     const double beta = (scal[S_RHO] / scal[S_RHO_OLD]) * (alpha / omega);
     for (long n = (long)blockIdx.x * kB + threadIdx.x; n < N; n += (long)gridDim.x * kB) {
-        double pn[4], yn[4];
+        double pn[D], yn[D];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const long o = n * 4 + i;
+        for (int i = 0; i < D; ++i) {
+            const long o = n * D + i;
             pn[i] = r[o] + beta * (p[o] - omega * v[o]);
             p[o] = pn[i];
         }
         if (!minv) continue;  // multigrid: y = M p is applied afterwards
-        prec4(minv, n, pn, yn);
+        prec_block<D>(minv, n, pn, yn);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) y[n * 4 + i] = yn[i];
+        for (int i = 0; i < D; ++i) y[n * D + i] = yn[i];
     }
 }
 
 // s = r - alpha v; z = M s; alpha = rho / (rh.v).
+template <int D>
 __global__ __launch_bounds__(kB) void k_bicg_s(int N, const double* __restrict__ scal, const double* __restrict__ minv,
                                                const double* __restrict__ r, const double* __restrict__ v,
                                                double* __restrict__ s, double* __restrict__ zz) {
     const double alpha = scal[S_RHO] / scal[S_RV];
     for (long n = (long)blockIdx.x * kB + threadIdx.x; n < N; n += (long)gridDim.x * kB) {
-        double sn[4], zn[4];
+        double sn[D], zn[D];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const long o = n * 4 + i;
+        for (int i = 0; i < D; ++i) {
+            const long o = n * D + i;
             sn[i] = r[o] - alpha * v[o];
             s[o] = sn[i];
         }
         if (!minv) continue;  // multigrid: z = M s is applied afterwards
-        prec4(minv, n, sn, zn);
+        prec_block<D>(minv, n, sn, zn);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) zz[n * 4 + i] = zn[i];
+        for (int i = 0; i < D; ++i) zz[n * D + i] = zn[i];
     }
 }
 
@@ -305,11 +318,13 @@ double read_scal(pa_plasticity_problem* P, int idx, hipStream_t st) {
     return v;
 }
 
-int vec_blocks(const pa_plasticity_problem* P) { return std::min(blocks(4L * P->p->mesh->N), 2048); }
+// global unknowns of the formulation (dpn per node)
+long num_dofs(const pa_plasticity_problem* P) { return (long)P->p->dpn * P->p->mesh->N; }
+int vec_blocks(const pa_plasticity_problem* P) { return std::min(blocks(num_dofs(P)), 2048); }
 int row_blocks(const pa_plasticity_problem* P) { return (P->p->mesh->N + 15) / 16; }
 
 double norm(pa_plasticity_problem* P, const double* x, hipStream_t st) {
-    const long n = 4L * P->p->mesh->N;
+    const long n = num_dofs(P);
     const int nb = vec_blocks(P);
     k_dot_part<<<nb, kB, 0, st>>>(n, x, P->part.p);
     check();
@@ -318,12 +333,30 @@ double norm(pa_plasticity_problem* P, const double* x, hipStream_t st) {
     return std::sqrt(read_scal(P, S_NORM, st));
 }
 
-// BiCGStab with block-Jacobi preconditioning on the 4x4 block matrix P->vals:
+template <int D>
+__global__ void k_block_prec(int N, const double* __restrict__ minv, const double* __restrict__ r,
+                             double* __restrict__ z);
+
+// Kernels of the block size D of the formulation.
+struct BlockKernels {
+    decltype(&k_spmv<4>) spmv;
+    decltype(&k_block_inverse<4>) inverse;
+    decltype(&k_bicg_dir<4>) dir;
+    decltype(&k_bicg_s<4>) s;
+    decltype(&k_block_prec<4>) prec;
+};
+BlockKernels block_kernels(const pa_plasticity_problem* P) {
+    if (P->p->dpn == 4) return {k_spmv<4>, k_block_inverse<4>, k_bicg_dir<4>, k_bicg_s<4>, k_block_prec<4>};
+    return {k_spmv<3>, k_block_inverse<3>, k_bicg_dir<3>, k_bicg_s<3>, k_block_prec<3>};
+}
+
+// BiCGStab with block-Jacobi preconditioning on the D x D block matrix P->vals:
 // solves A x = b from the x given; returns the iterations.
 int bicgstab(pa_plasticity_problem* P, const double* b, double* x, hipStream_t st) {
     const pa_mesh* m = P->p->mesh;
     const int N = m->N;
-    const long n = 4L * N;
+    const long n = num_dofs(P);
+    const BlockKernels K = block_kernels(P);
     const int nvb = vec_blocks(P), nsb = row_blocks(P);
     const int np = std::max(nvb, nsb);
     const double* A = P->vals.p;
@@ -332,10 +365,10 @@ int bicgstab(pa_plasticity_problem* P, const double* b, double* x, hipStream_t s
         const int rc = pa_multigrid_setup(P->mg, A, PA_FP64, P->nbc, P->bc_dofs.p, st);
         if (rc != PA_OK) throw Error(rc, pa_last_error());
     } else {
-        k_block_inverse4<<<blocks(N), kB, 0, st>>>(N, m->d_rowmap.p, m->d_colidx.p, A, P->minv.p);
+        K.inverse<<<blocks(N), kB, 0, st>>>(N, m->d_rowmap.p, m->d_colidx.p, A, P->minv.p);
         check();
     }
-    k_spmv4<<<nsb, kB, 0, st>>>(N, m->d_rowmap.p, m->d_colidx.p, A, x, P->t.p, nullptr, nullptr, P->part.p, np);
+    K.spmv<<<nsb, kB, 0, st>>>(N, m->d_rowmap.p, m->d_colidx.p, A, x, P->t.p, nullptr, nullptr, P->part.p, np);
     check();
     k_bicg_init<<<nvb, kB, 0, st>>>(n, b, P->t.p, P->r.p, P->rh.p, P->pv.p, P->v.p, P->part.p, np);
     check();
@@ -351,19 +384,19 @@ int bicgstab(pa_plasticity_problem* P, const double* b, double* x, hipStream_t s
     while (it < P->opt.linear_max_iter && bnorm > 0.0 && std::sqrt(rr) > target) {
         const int burst = std::min(check_every, P->opt.linear_max_iter - it);
         for (int q = 0; q < burst; ++q) {
-            k_bicg_dir<<<nvb, kB, 0, st>>>(N, P->scal.p, minv, P->r.p, P->v.p, P->pv.p, P->y.p);
+            K.dir<<<nvb, kB, 0, st>>>(N, P->scal.p, minv, P->r.p, P->v.p, P->pv.p, P->y.p);
             check();
             if (P->mg) multigrid_apply(P->mg, P->pv.p, P->y.p, st);
-            k_spmv4<<<nsb, kB, 0, st>>>(N, m->d_rowmap.p, m->d_colidx.p, A, P->y.p, P->v.p, P->rh.p, nullptr,
-                                        P->part.p, np);
+            K.spmv<<<nsb, kB, 0, st>>>(N, m->d_rowmap.p, m->d_colidx.p, A, P->y.p, P->v.p, P->rh.p, nullptr,
+                                       P->part.p, np);
             check();
             k_reduce<<<1, 1024, 0, st>>>(P->part.p, nsb, np, 1, S_RV, 0, 0, P->scal.p);
             check();
-            k_bicg_s<<<nvb, kB, 0, st>>>(N, P->scal.p, minv, P->r.p, P->v.p, P->s.p, P->zz.p);
+            K.s<<<nvb, kB, 0, st>>>(N, P->scal.p, minv, P->r.p, P->v.p, P->s.p, P->zz.p);
             check();
             if (P->mg) multigrid_apply(P->mg, P->s.p, P->zz.p, st);
-            k_spmv4<<<nsb, kB, 0, st>>>(N, m->d_rowmap.p, m->d_colidx.p, A, P->zz.p, P->t.p, P->s.p, P->t.p,
-                                        P->part.p, np);
+            K.spmv<<<nsb, kB, 0, st>>>(N, m->d_rowmap.p, m->d_colidx.p, A, P->zz.p, P->t.p, P->s.p, P->t.p,
+                                       P->part.p, np);
             check();
             k_reduce<<<1, 1024, 0, st>>>(P->part.p, nsb, np, 2, S_TS, S_TT, 0, P->scal.p);
             check();
@@ -438,13 +471,16 @@ __global__ void k_sub(long n, const double* __restrict__ a, const double* __rest
     for (long t = (long)blockIdx.x * kB + threadIdx.x; t < n; t += (long)gridDim.x * kB) c[t] = a[t] - b[t];
 }
 
-__global__ void k_block_prec4(int N, const double* __restrict__ minv, const double* __restrict__ r,
-                              double* __restrict__ z) {
+template <int D>
+__global__ void k_block_prec(int N, const double* __restrict__ minv, const double* __restrict__ r,
+                             double* __restrict__ z) {
     for (long n = (long)blockIdx.x * kB + threadIdx.x; n < N; n += (long)gridDim.x * kB) {
-        double in[4] = {r[n * 4], r[n * 4 + 1], r[n * 4 + 2], r[n * 4 + 3]}, out[4];
-        prec4(minv, n, in, out);
+        double in[D], out[D];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) z[n * 4 + i] = out[i];
+        for (int i = 0; i < D; ++i) in[i] = r[n * D + i];
+        prec_block<D>(minv, n, in, out);
+#pragma unroll
+        for (int i = 0; i < D; ++i) z[n * D + i] = out[i];
     }
 }
 
@@ -454,7 +490,8 @@ void preconditioner_setup(pa_plasticity_problem* P, hipStream_t st) {
         const int rc = pa_multigrid_setup(P->mg, P->vals.p, PA_FP64, P->nbc, P->bc_dofs.p, st);
         if (rc != PA_OK) throw Error(rc, pa_last_error());
     } else {
-        k_block_inverse4<<<blocks(m->N), kB, 0, st>>>(m->N, m->d_rowmap.p, m->d_colidx.p, P->vals.p, P->minv.p);
+        block_kernels(P).inverse<<<blocks(m->N), kB, 0, st>>>(m->N, m->d_rowmap.p, m->d_colidx.p, P->vals.p,
+                                                               P->minv.p);
         check();
     }
 }
@@ -463,7 +500,7 @@ void apply_preconditioner(pa_plasticity_problem* P, const double* r, double* z, 
     if (P->mg) {
         multigrid_apply(P->mg, r, z, st);
     } else {
-        k_block_prec4<<<vec_blocks(P), kB, 0, st>>>(P->p->mesh->N, P->minv.p, r, z);
+        block_kernels(P).prec<<<vec_blocks(P), kB, 0, st>>>(P->p->mesh->N, P->minv.p, r, z);
         check();
     }
 }
@@ -471,8 +508,9 @@ void apply_preconditioner(pa_plasticity_problem* P, const double* r, double* z, 
 int gmres(pa_plasticity_problem* P, const double* b, double* x, hipStream_t st) {
     const pa_mesh* m = P->p->mesh;
     const int N = m->N;
-    const long n = 4L * N;
+    const long n = num_dofs(P);
     const int M = P->opt.restart;
+    const auto spmv = block_kernels(P).spmv;
     const int nvb = vec_blocks(P), nsb = row_blocks(P), np = std::max(nvb, nsb);
     double* V = P->gV.p;
     double* Z = P->gZ.p;
@@ -484,8 +522,8 @@ int gmres(pa_plasticity_problem* P, const double* b, double* x, hipStream_t st) 
     int it = 0;
     while (it < P->opt.linear_max_iter) {
         // r = b - A x into V_0
-        k_spmv4<<<nsb, kB, 0, st>>>(N, m->d_rowmap.p, m->d_colidx.p, P->vals.p, x, P->t.p, nullptr, nullptr,
-                                    P->part.p, np);
+        spmv<<<nsb, kB, 0, st>>>(N, m->d_rowmap.p, m->d_colidx.p, P->vals.p, x, P->t.p, nullptr, nullptr,
+                                 P->part.p, np);
         check();
         k_sub<<<nvb, kB, 0, st>>>(n, b, P->t.p, V);
         check();
@@ -503,8 +541,8 @@ int gmres(pa_plasticity_problem* P, const double* b, double* x, hipStream_t st) 
             double* zj = Z + (long)j * n;
             double* w = V + (long)(j + 1) * n;
             apply_preconditioner(P, vj, zj, st);
-            k_spmv4<<<nsb, kB, 0, st>>>(N, m->d_rowmap.p, m->d_colidx.p, P->vals.p, zj, w, nullptr, nullptr,
-                                        P->part.p, np);
+            spmv<<<nsb, kB, 0, st>>>(N, m->d_rowmap.p, m->d_colidx.p, P->vals.p, zj, w, nullptr, nullptr,
+                                     P->part.p, np);
             check();
             // two passes of classical Gram-Schmidt against V_0..V_j
             for (int pass = 0; pass < 2; ++pass) {
@@ -581,14 +619,14 @@ int linear_solve(pa_plasticity_problem* P, const double* b, double* x, hipStream
     return P->opt.krylov == PA_KRYLOV_BICGSTAB ? bicgstab(P, b, x, st) : gmres(P, b, x, st);
 }
 
-double* step_u(pa_plasticity_problem* P, int k) { return P->U.p + (long)k * 4 * P->p->mesh->N; }
+double* step_u(pa_plasticity_problem* P, int k) { return P->U.p + (long)k * num_dofs(P); }
 double* step_c(pa_plasticity_problem* P, int k) { return P->Cs.p + (long)k * 14 * P->p->mesh->C; }
 double* step_pi(pa_plasticity_problem* P, int k) { return P->PI.p + (long)k * 3 * P->p->mesh->N; }
 
 void constraints(pa_plasticity_problem* P, double* rhs, const double* bcv, hipStream_t st) {
     if (P->nbc == 0) return;
-    const int rc = pa_apply_block_constraints(P->p->mesh, 4, P->vals.p, PA_FP64, rhs, P->nbc, P->bc_dofs.p, bcv, 1.0,
-                                              st);
+    const int rc = pa_apply_block_constraints(P->p->mesh, P->p->dpn, P->vals.p, PA_FP64, rhs, P->nbc, P->bc_dofs.p,
+                                              bcv, 1.0, st);
     if (rc != PA_OK) throw Error(rc, pa_last_error());
 }
 
@@ -596,11 +634,13 @@ void constraints(pa_plasticity_problem* P, double* rhs, const double* bcv, hipSt
 bool solve_forward(pa_plasticity_problem* P, const double* z, hipStream_t st) {
     const pa_mesh* m = P->p->mesh;
     const int K = P->opt.num_steps;
-    const long nu = 4L * m->N;
+    const long nu = num_dofs(P);
+    const bool mixed = P->p->dpn == 4;
     P->newton_its.assign(K, 0);
     P->linear_its.assign(K, 0);
     P->res_norms.assign(K, 0.0);
-    PA_HIP(hipMemsetAsync(step_pi(P, 0), 0, 3L * m->N * sizeof(double), st));
+    // the pure-displacement form has no projection: every step's pi stays zero
+    PA_HIP(hipMemsetAsync(step_pi(P, 0), 0, 3L * m->N * (mixed ? 1 : K) * sizeof(double), st));
     const double dt = 1.0 / K;
     std::vector<double> inc(P->nbc);
     bool all_converged = true;
@@ -650,7 +690,7 @@ bool solve_forward(pa_plasticity_problem* P, const double* z, hipStream_t st) {
         P->linear_its[k] = lin;
         P->res_norms[k] = nrm;
         all_converged = all_converged && converged;
-        if (k + 1 < K) projection(P->p, uk, z, step_pi(P, k + 1), st);
+        if (mixed && k + 1 < K) projection(P->p, uk, z, step_pi(P, k + 1), st);
     }
     PA_HIP(hipStreamSynchronize(st));
     P->solved = true;
@@ -672,7 +712,8 @@ double criterion_value(pa_plasticity_problem* P, const double* z, double mult, h
 void criterion_gradient(pa_plasticity_problem* P, const double* z, double mult, double* grad, hipStream_t st) {
     const pa_mesh* m = P->p->mesh;
     const int K = P->opt.num_steps;
-    const long nu = 4L * m->N;
+    const long nu = num_dofs(P);
+    const bool mixed = P->p->dpn == 4;
     PA_HIP(hipMemsetAsync(grad, 0, m->N * sizeof(double), st));
     for (int k = K - 1; k >= 0; --k) {
         const bool last = (k == K - 1);
@@ -691,7 +732,7 @@ void criterion_gradient(pa_plasticity_problem* P, const double* z, double mult, 
         PA_HIP(hipMemsetAsync(P->lam.p, 0, nu * sizeof(double), st));
         linear_solve(P, P->rhs.p, P->lam.p, st);
         adjoint_local_update(P->p, u, c, cp, pi, z, P->lam.p, P->blocal.p, P->zc.p, P->mu.p, P->zc_out.p, st);
-        if (k == 0) {
+        if (k == 0 || !mixed) {  // no projection adjoint in the first step or the displacement form
             PA_HIP(hipMemsetAsync(P->gam.p, 0, 3L * m->N * sizeof(double), st));
         } else {
             gather_uniform(P->p, 3, P->p->vcell.p, 16, -1.0, false, P->gam.p, st);
@@ -743,7 +784,7 @@ int pa_plasticity_problem_create(pa_plasticity* p, int nbc, const int* dofs, con
                            (P->opt.krylov == PA_KRYLOV_GMRES || P->opt.krylov == PA_KRYLOV_BICGSTAB), PA_ERR_ARG,
                        "pa_plasticity_problem_create: bad options");
             const pa_mesh* m = p->mesh;
-            const long nu = 4L * m->N;
+            const long nu = (long)p->dpn * m->N;
             for (int j = 0; j < nbc; ++j)
                 PA_REQUIRE(dofs[j] >= 0 && dofs[j] < nu, PA_ERR_ARG, "pa_plasticity_problem_create: dof out of range");
             P->nbc = nbc;
@@ -763,11 +804,11 @@ int pa_plasticity_problem_create(pa_plasticity* p, int nbc, const int* dofs, con
             P->zeroC.alloc(14L * m->C);
             PA_HIP(hipMemset(P->zeroU.p, 0, nu * sizeof(double)));
             PA_HIP(hipMemset(P->zeroC.p, 0, 14L * m->C * sizeof(double)));
-            P->vals.alloc(16L * m->nnzb);
+            P->vals.alloc((long)p->dpn * p->dpn * m->nnzb);
             for (auto* b : {&P->R, &P->rhs, &P->du, &P->r, &P->rh, &P->pv, &P->v, &P->y, &P->s, &P->zz, &P->t,
                             &P->lam})
                 b->alloc(nu);
-            P->minv.alloc(16L * m->N);
+            P->minv.alloc((long)p->dpn * p->dpn * m->N);
             {
                 const long M = P->opt.restart;
                 P->gV.alloc((M + 1) * nu);
@@ -809,7 +850,7 @@ int pa_plasticity_problem_set_multigrid(pa_plasticity_problem* P, int nx, int ny
             o.omega = 0.6;
         }
         pa_multigrid* mg = nullptr;
-        const int rc = pa_multigrid_create(P->p->mesh, nx, ny, nz, 4, &o, &mg);
+        const int rc = pa_multigrid_create(P->p->mesh, nx, ny, nz, P->p->dpn, &o, &mg);
         if (rc != PA_OK) throw pa::Error(rc, pa_last_error());
         if (P->mg) pa_multigrid_destroy(P->mg);
         P->mg = mg;
