@@ -171,22 +171,27 @@ def thermal_bandwidth(pa, stream, reps=10):
     return out
 
 
-def j2_plasticity(pa, stream, with_cpu=True):
-    """J2 plasticity (the reference's stabilized mixed u-p formulation, 4 dofs
-    per node, 14 local states per cell) on the 64x32x32 hex grid x6 Tet4
-    (393,216 cells, 70,785 nodes, 283,140 dofs): E=1e3, nu=0.3, yield 1,
-    isotropic hardening 10, x- clamped, ux ramped to 0.008 on x+ over 10
-    pseudo-time steps, z = 1.  Reports the forward solve, the plastic-work
-    criterion with its path-dependent adjoint gradient, the condensed
-    assembly kernel (cells/s) and the oracle's condensed assembly on the host."""
+def j2_plasticity(pa, stream, with_cpu=True, formulation="mixed"):
+    """J2 plasticity on the 64x32x32 hex grid x6 Tet4 (393,216 cells, 70,785
+    nodes, 14 local states per cell): E=1e3, nu=0.3, yield 1, isotropic
+    hardening 10, x- clamped, ux ramped to 0.008 on x+ over 10 pseudo-time
+    steps, z = 1.  formulation "mixed" is the reference's stabilized u-p form
+    (4 dofs per node, 283,140 dofs); "displacement" the pure-displacement J2
+    (3 dofs per node, 212,355 dofs).  Reports the forward solve, the
+    plastic-work criterion with its path-dependent adjoint gradient, the
+    condensed assembly kernel (cells/s) and the oracle's condensed assembly
+    on the host."""
     mesh = pa.Mesh.box(*J2_DIMS, *J2_LENGTHS)
     N, C, nnzb = mesh.num_nodes, mesh.num_cells, mesh.num_blocks
-    pl = pa.Plasticity(mesh, **J2_PARAMS)
+    pl = pa.Plasticity(mesh, formulation=formulation, **J2_PARAMS)
+    D = pl.dpn
     xm, xp = mesh.node_set("x-"), mesh.node_set("x+")
-    dofs = np.concatenate([xm * 4, xm * 4 + 1, xm * 4 + 2, xp * 4])
+    dofs = np.concatenate([xm * D, xm * D + 1, xm * D + 2, xp * D])
     vals = np.concatenate([np.zeros(3 * xm.size), np.full(xp.size, J2_END_DISPLACEMENT)])
+    order = np.argsort(dofs)
+    dofs, vals = dofs[order], vals[order]
     z = pa.DeviceArray.from_numpy(np.ones(N))
-    # flexible GMRES(200) to 1e-8 relative, preconditioned by the 4x4-block
+    # flexible GMRES(200) to 1e-8 relative, preconditioned by the DxD-block
     # geometric multigrid with 4 pre- and post-smoothing sweeps
     prob = pa.PlasticityProblem(pl, dofs, vals, num_steps=J2_STEPS, max_newton=10, linear_tol=1e-8, restart=200)
     if os.environ.get("PA_J2_PRECOND", "multigrid") == "multigrid":
@@ -205,8 +210,8 @@ def j2_plasticity(pa, stream, with_cpu=True):
     cp = prob.state(J2_STEPS - 2)[1]
     plastic = int((c.reshape(-1, 14)[:, 1] > 0).sum())
     du, dc, dcp, dpi = (pa.DeviceArray.from_numpy(a) for a in (u, c, cp, pi))
-    values = pa.DeviceArray(16 * nnzb)
-    R = pa.DeviceArray(4 * N)
+    values = pa.DeviceArray(D * D * nnzb)
+    R = pa.DeviceArray(D * N)
 
     def cond():
         pl.condensed_system(du, dc, dcp, dpi, z, values=values, R=R, stream=stream)
@@ -222,11 +227,13 @@ def j2_plasticity(pa, stream, with_cpu=True):
     stream.synchronize()
     t_cond = a.elapsed_ms(b) / reps
     # algorithmic bytes of one condensed assembly: u, c, c_prev, z, coords,
-    # connectivity, pi in; block values, graph indices and R out
-    alg = C * (16 + 2 * 112) + N * (32 + 8 + 32 + 24 + 32) + nnzb * (128 + 4) + 4 * (N + 1)
-    out = {"workload": f"J2 plasticity mixed u-p Tet4, {J2_DIMS[0]}x{J2_DIMS[1]}x{J2_DIMS[2]} hex x6 "
-                       f"({C} cells, {N} nodes, {4 * N} dof), {J2_STEPS} steps to ux={J2_END_DISPLACEMENT}",
-           "solver": "condensed Newton; FGMRES(200) to 1e-8, 4x4-block geometric multigrid (4+4 sweeps)",
+    # connectivity, pi (mixed form) in; block values, graph indices and R out
+    alg = C * (16 + 2 * 112) + N * (8 * D + 8 + 32 + (24 if D == 4 else 0) + 8 * D) + nnzb * (8 * D * D + 4) \
+        + 4 * (N + 1)
+    name = "mixed u-p" if D == 4 else "pure-displacement"
+    out = {"workload": f"J2 plasticity {name} Tet4, {J2_DIMS[0]}x{J2_DIMS[1]}x{J2_DIMS[2]} hex x6 "
+                       f"({C} cells, {N} nodes, {D * N} dof), {J2_STEPS} steps to ux={J2_END_DISPLACEMENT}",
+           "solver": f"condensed Newton; FGMRES(200) to 1e-8, {D}x{D}-block geometric multigrid (4+4 sweeps)",
            "forward_solve_s": t_fwd, "converged": converged,
            "criterion_value_and_adjoint_gradient_s": t_adj, "plastic_work": F,
            "gradient_norm": float(np.linalg.norm(g.numpy())),
@@ -240,7 +247,7 @@ def j2_plasticity(pa, stream, with_cpu=True):
         import oracle
         threads = oracle.set_threads(int(os.environ.get("PA_CPU_THREADS", min(16, os.cpu_count() or 1))))
         coords, conn = oracle.box_mesh(*J2_DIMS, *J2_LENGTHS)
-        om = oracle.J2Model(coords, conn, **J2_PARAMS)
+        om = oracle.J2Model(coords, conn, formulation=formulation, **J2_PARAMS)
         zh = np.ones(N)
         times = []
         for _ in range(3):
@@ -262,7 +269,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-solve", action="store_true", help="skip the compliance + gradient leg")
-    ap.add_argument("--no-j2", action="store_true", help="skip the J2 plasticity leg")
+    ap.add_argument("--no-j2", action="store_true", help="skip the J2 plasticity legs")
+    ap.add_argument("--j2-formulation", default="mixed", choices=["mixed", "displacement"],
+                    help="J2 formulation of --mode j2")
     ap.add_argument("--no-thermal", action="store_true", help="skip the 192^3 thermal assembly leg")
     ap.add_argument("--mode", default="bench", choices=["bench", "copy", "profile", "solve", "j2", "thermal"],
                     help="copy: run only the calibration copy kernel; profile: assembly kernels only; "
@@ -289,7 +298,8 @@ def main():
         log(f"copy kernel: {args.warmup + args.steps} x {nbytes} bytes")
         return
     if args.mode == "j2":
-        print(json.dumps(j2_plasticity(pa, stream, with_cpu=not args.no_cpu)), flush=True)
+        print(json.dumps(j2_plasticity(pa, stream, with_cpu=not args.no_cpu, formulation=args.j2_formulation)),
+              flush=True)
         return
     if args.mode == "thermal":
         print(json.dumps(thermal_bandwidth(pa, stream, reps=max(1, args.steps))), flush=True)
@@ -398,6 +408,8 @@ def main():
         out["compliance_gradient"] = compliance_gradient(pa, mesh, vf, z, dims)
     if not args.no_j2:
         out["j2_plasticity"] = j2_plasticity(pa, stream, with_cpu=not args.no_cpu)
+        out["j2_plasticity_displacement"] = j2_plasticity(pa, stream, with_cpu=not args.no_cpu,
+                                                          formulation="displacement")
     if not args.no_thermal:
         out["thermal_192"] = thermal_bandwidth(pa, stream)
     if not args.no_cpu:
