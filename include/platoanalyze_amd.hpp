@@ -1,0 +1,1435 @@
+/*
+ * platoanalyze_amd.hpp — C++ host layer over the C ABI (header-only, C++17).
+ *
+ * Re-creates, for this path, the reference's class names, method names,
+ * argument meaning and error behaviour, so that its call sites and unit tests
+ * switch by namespace (Plato:: -> Plato::amd::):
+ *
+ *   Elliptic::VectorFunction<Mechanics<3> | Thermal<3>>
+ *       value, gradient_u, gradient_u_T, gradient_z     src/elliptic/VectorFunction.hpp:115-710
+ *   Elliptic::PhysicsScalarFunction<...>
+ *       value, gradient_u, gradient_z                   src/elliptic/PhysicsScalarFunction.hpp:225-494
+ *   Elliptic::Problem<...>
+ *       solution, criterionValue, criterionGradient     src/elliptic/Problem.hpp:237-450
+ *   LocalVectorFunctionInc<InfinitesimalStrainPlasticity<3>>
+ *       updateLocalState, value, gradient_{u,c,cp,z}    src/LocalVectorFunctionInc.hpp:236-874
+ *   GlobalVectorFunctionInc<InfinitesimalStrainPlasticity<3>>
+ *       value, gradient_{u,up,c,cp,z,n}                 src/GlobalVectorFunctionInc.hpp:919-1155
+ *   PlasticityProblem<InfinitesimalStrainPlasticity<3>>
+ *       solution, criterionValue, criterionGradient     src/PlasticityProblem.hpp:784-1068
+ *   CrsMatrix (rowMap, columnIndices, entries, ...)     src/alg/CrsMatrix.hpp:20-113
+ *
+ * Inputs are the reference's parameter lists (the Teuchos XML of its input
+ * decks and unit tests), read by a small built-in parser
+ * (getParametersFromXmlString) with the reference's key names and defaults.
+ * Only what this path implements is accepted; any other material model, load
+ * type or criterion throws, naming the key.
+ *
+ * Errors throw std::runtime_error carrying the library's message, as the
+ * reference's ANALYZE_THROWERR (src/AnalyzeMacros.hpp:29-33).  Arrays are
+ * device arrays shared by reference, like Kokkos views; `toHost` copies one to
+ * a std::vector.  Every call runs on the default stream and returns after its
+ * work has been enqueued; reading a result to the host synchronizes.
+ *
+ * Link with -lplatoanalyze_amd (the library that implements the C ABI).
+ */
+#ifndef PLATOANALYZE_AMD_HPP
+#define PLATOANALYZE_AMD_HPP
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdlib>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "platoanalyze_amd.h"
+
+namespace Plato {
+namespace amd {
+
+using Scalar = double;
+using OrdinalType = int;
+
+[[noreturn]] inline void throw_error(const std::string& aMsg) { throw std::runtime_error(aMsg); }
+
+inline void check(int aStatus) {
+    if (aStatus != PA_OK) throw_error(pa_last_error());
+}
+
+// ---------------------------------------------------------------- arrays
+// Device array shared by reference (Kokkos::View<T*> in the reference).
+template <class T>
+class View {
+  public:
+    View() = default;
+    View(const std::string& aLabel, size_t aSize) : mLabel(aLabel), mSize(aSize) {
+        void* tPtr = nullptr;
+        if (aSize) {
+            check(pa_malloc(&tPtr, aSize * sizeof(T)));
+            mPtr = std::shared_ptr<T>(static_cast<T*>(tPtr), [](T* p) { pa_free(p); });
+            check(pa_memset(tPtr, 0, aSize * sizeof(T), nullptr));
+        }
+    }
+    // a view of memory another object owns, kept alive through aOwner
+    static View alias(std::shared_ptr<void> aOwner, T* aPtr, size_t aSize, const std::string& aLabel) {
+        View tView;
+        tView.mPtr = std::shared_ptr<T>(aOwner, aPtr);
+        tView.mSize = aSize;
+        tView.mLabel = aLabel;
+        return tView;
+    }
+    T* data() const { return mPtr.get(); }
+    size_t extent(int aDim) const { return aDim == 0 ? mSize : 1; }
+    size_t size() const { return mSize; }
+    const std::string& label() const { return mLabel; }
+
+  private:
+    std::string mLabel;
+    std::shared_ptr<T> mPtr;
+    size_t mSize = 0;
+};
+
+using ScalarVector = View<Scalar>;
+using OrdinalVector = View<OrdinalType>;
+
+template <class T>
+std::vector<T> toHost(const View<T>& aView) {
+    std::vector<T> tOut(aView.size());
+    if (aView.size()) check(pa_memcpy_d2h(tOut.data(), aView.data(), aView.size() * sizeof(T)));
+    return tOut;
+}
+
+template <class T>
+View<T> toDevice(const std::vector<T>& aHost, const std::string& aLabel = "") {
+    View<T> tView(aLabel, aHost.size());
+    if (aHost.size()) check(pa_memcpy_h2d(tView.data(), aHost.data(), aHost.size() * sizeof(T)));
+    return tView;
+}
+
+// Plato::blas1::fill
+template <class T>
+void fill(T aValue, const View<T>& aView) {
+    std::vector<T> tHost(aView.size(), aValue);
+    if (aView.size()) check(pa_memcpy_h2d(aView.data(), tHost.data(), tHost.size() * sizeof(T)));
+}
+
+// Cell workset (ScalarArray3D in the reference): [cell][row][column], LayoutRight.
+struct ScalarArray3D {
+    View<Scalar> values;
+    size_t dims[3] = {0, 0, 0};
+    ScalarArray3D() = default;
+    ScalarArray3D(const std::string& aLabel, size_t a, size_t b, size_t c) : values(aLabel, a * b * c), dims{a, b, c} {}
+    Scalar* data() const { return values.data(); }
+    size_t extent(int aDim) const { return dims[aDim]; }
+    size_t size() const { return values.size(); }
+};
+
+// Named state arrays of a solve (Plato::Solutions, src/Solutions.hpp).
+class Solutions {
+  public:
+    void set(const std::string& aName, const ScalarVector& aState) { mStates[aName] = {aState}; }
+    void set(const std::string& aName, const std::vector<ScalarVector>& aStates) { mStates[aName] = aStates; }
+    const ScalarVector& get(const std::string& aName, size_t aStep = 0) const {
+        auto tIt = mStates.find(aName);
+        if (tIt == mStates.end() || aStep >= tIt->second.size())
+            throw_error("Solutions: no state '" + aName + "' at step " + std::to_string(aStep));
+        return tIt->second[aStep];
+    }
+    size_t numSteps(const std::string& aName = "State") const {
+        auto tIt = mStates.find(aName);
+        return tIt == mStates.end() ? 0 : tIt->second.size();
+    }
+    bool empty() const { return mStates.empty(); }
+
+  private:
+    std::map<std::string, std::vector<ScalarVector>> mStates;
+};
+
+// ------------------------------------------------------- parameter lists
+// Subset of Teuchos::ParameterList: typed scalar, string and array entries
+// and nested sublists, in insertion order.
+class ParameterList {
+  public:
+    struct Entry {
+        std::string type;  // "double", "int", "bool", "string", "Array(double)", "Array(int)", "Array(string)", "list"
+        std::string text;  // the value as written
+        std::shared_ptr<ParameterList> list;
+    };
+
+    explicit ParameterList(std::string aName = "ANONYMOUS") : mName(std::move(aName)) {}
+    const std::string& name() const { return mName; }
+
+    bool isParameter(const std::string& aName) const {
+        const Entry* e = find(aName);
+        return e && e->type != "list";
+    }
+    bool isSublist(const std::string& aName) const {
+        const Entry* e = find(aName);
+        return e && e->type == "list";
+    }
+    template <class T>
+    bool isType(const std::string& aName) const {
+        const Entry* e = find(aName);
+        return e && matches<T>(e->type);
+    }
+    template <class T>
+    T get(const std::string& aName) const {
+        const Entry* e = find(aName);
+        if (!e || e->type == "list")
+            throw_error("ParameterList '" + mName + "': parameter '" + aName + "' does not exist");
+        if (!matches<T>(e->type))
+            throw_error("ParameterList '" + mName + "': parameter '" + aName + "' is not of the type asked for (" +
+                        e->type + ")");
+        return convert<T>(e->text, aName);
+    }
+    template <class T>
+    T get(const std::string& aName, const T& aDefault) const {
+        return isParameter(aName) ? get<T>(aName) : aDefault;
+    }
+    std::string get(const std::string& aName, const char* aDefault) const {
+        return get<std::string>(aName, std::string(aDefault));
+    }
+    const ParameterList& sublist(const std::string& aName) const {
+        const Entry* e = find(aName);
+        if (!e || e->type != "list") throw_error("ParameterList '" + mName + "': no sublist '" + aName + "'");
+        return *e->list;
+    }
+    ParameterList& sublist(const std::string& aName) {
+        Entry* e = find(aName);
+        if (e && e->type != "list") throw_error("ParameterList '" + mName + "': '" + aName + "' is not a sublist");
+        if (!e) {
+            mEntries.push_back({aName, {"list", "", std::make_shared<ParameterList>(aName)}});
+            e = &mEntries.back().second;
+        }
+        return *e->list;
+    }
+    void set(const std::string& aName, const std::string& aType, const std::string& aText) {
+        Entry* e = find(aName);
+        if (e) *e = {aType, aText, nullptr};
+        else mEntries.push_back({aName, {aType, aText, nullptr}});
+    }
+    void set(const std::string& aName, double aValue) {
+        std::ostringstream s;
+        s.precision(17);
+        s << aValue;
+        set(aName, "double", s.str());
+    }
+    void set(const std::string& aName, int aValue) { set(aName, "int", std::to_string(aValue)); }
+    void set(const std::string& aName, bool aValue) { set(aName, "bool", aValue ? "true" : "false"); }
+    void set(const std::string& aName, const std::string& aValue) { set(aName, "string", aValue); }
+    void set(const std::string& aName, const char* aValue) { set(aName, "string", std::string(aValue)); }
+    // names of the sublists, in order
+    std::vector<std::string> sublistNames() const {
+        std::vector<std::string> tOut;
+        for (const auto& kv : mEntries)
+            if (kv.second.type == "list") tOut.push_back(kv.first);
+        return tOut;
+    }
+
+  private:
+    const Entry* find(const std::string& aName) const {
+        for (const auto& kv : mEntries)
+            if (kv.first == aName) return &kv.second;
+        return nullptr;
+    }
+    Entry* find(const std::string& aName) {
+        for (auto& kv : mEntries)
+            if (kv.first == aName) return &kv.second;
+        return nullptr;
+    }
+    template <class T>
+    static bool matches(const std::string& aType) {
+        if constexpr (std::is_same_v<T, double>) return aType == "double" || aType == "int";
+        else if constexpr (std::is_same_v<T, int>) return aType == "int";
+        else if constexpr (std::is_same_v<T, bool>) return aType == "bool";
+        else if constexpr (std::is_same_v<T, std::string>) return aType == "string";
+        else if constexpr (std::is_same_v<T, std::vector<double>>)
+            return aType == "Array(double)" || aType == "Array(int)";
+        else if constexpr (std::is_same_v<T, std::vector<int>>) return aType == "Array(int)";
+        else if constexpr (std::is_same_v<T, std::vector<std::string>>) return aType == "Array(string)";
+        else return false;
+    }
+    static std::vector<std::string> array_items(const std::string& aText) {
+        std::string t = aText;
+        const size_t a = t.find('{'), b = t.rfind('}');
+        if (a == std::string::npos || b == std::string::npos || b < a) throw_error("malformed array value '" + aText + "'");
+        t = t.substr(a + 1, b - a - 1);
+        std::vector<std::string> tOut;
+        std::string tItem;
+        std::istringstream s(t);
+        while (std::getline(s, tItem, ',')) {
+            const size_t i = tItem.find_first_not_of(" \t\n\r"), j = tItem.find_last_not_of(" \t\n\r");
+            if (i != std::string::npos) tOut.push_back(tItem.substr(i, j - i + 1));
+        }
+        return tOut;
+    }
+    static double to_double(const std::string& aText, const std::string& aName) {
+        char* tEnd = nullptr;
+        const double v = std::strtod(aText.c_str(), &tEnd);
+        if (tEnd == aText.c_str()) throw_error("parameter '" + aName + "': '" + aText + "' is not a number");
+        return v;
+    }
+    template <class T>
+    static T convert(const std::string& aText, const std::string& aName) {
+        if constexpr (std::is_same_v<T, double>) return to_double(aText, aName);
+        else if constexpr (std::is_same_v<T, int>) return (int)std::lround(to_double(aText, aName));
+        else if constexpr (std::is_same_v<T, bool>) {
+            std::string t = aText;
+            for (auto& ch : t) ch = (char)std::tolower((unsigned char)ch);
+            return t == "true" || t == "1" || t == "yes";
+        } else if constexpr (std::is_same_v<T, std::string>) return aText;
+        else if constexpr (std::is_same_v<T, std::vector<double>>) {
+            std::vector<double> tOut;
+            for (const auto& s : array_items(aText)) tOut.push_back(to_double(s, aName));
+            return tOut;
+        } else if constexpr (std::is_same_v<T, std::vector<int>>) {
+            std::vector<int> tOut;
+            for (const auto& s : array_items(aText)) tOut.push_back((int)std::lround(to_double(s, aName)));
+            return tOut;
+        } else {
+            return array_items(aText);
+        }
+    }
+
+    std::string mName;
+    std::vector<std::pair<std::string, Entry>> mEntries;
+};
+
+namespace detail {
+
+inline std::string xml_unescape(const std::string& s) {
+    std::string o;
+    for (size_t i = 0; i < s.size(); ++i) {
+        if (s[i] != '&') {
+            o += s[i];
+            continue;
+        }
+        const size_t j = s.find(';', i);
+        const std::string e = j == std::string::npos ? "" : s.substr(i + 1, j - i - 1);
+        if (e == "lt") o += '<';
+        else if (e == "gt") o += '>';
+        else if (e == "amp") o += '&';
+        else if (e == "quot") o += '"';
+        else if (e == "apos") o += '\'';
+        else {
+            o += s[i];
+            continue;
+        }
+        i = j;
+    }
+    return o;
+}
+
+// attributes of one tag body: name='value' or name="value"
+inline std::map<std::string, std::string> xml_attributes(const std::string& aBody) {
+    std::map<std::string, std::string> tOut;
+    size_t i = 0;
+    while (i < aBody.size()) {
+        while (i < aBody.size() && std::isspace((unsigned char)aBody[i])) ++i;
+        size_t k = i;
+        while (k < aBody.size() && (std::isalnum((unsigned char)aBody[k]) || aBody[k] == '_' || aBody[k] == '-')) ++k;
+        if (k == i) break;
+        const std::string tKey = aBody.substr(i, k - i);
+        while (k < aBody.size() && std::isspace((unsigned char)aBody[k])) ++k;
+        if (k >= aBody.size() || aBody[k] != '=') throw_error("XML: attribute '" + tKey + "' has no value");
+        ++k;
+        while (k < aBody.size() && std::isspace((unsigned char)aBody[k])) ++k;
+        if (k >= aBody.size() || (aBody[k] != '\'' && aBody[k] != '"')) throw_error("XML: unquoted attribute");
+        const char q = aBody[k];
+        const size_t e = aBody.find(q, k + 1);
+        if (e == std::string::npos) throw_error("XML: unterminated attribute");
+        tOut[tKey] = xml_unescape(aBody.substr(k + 1, e - k - 1));
+        i = e + 1;
+    }
+    return tOut;
+}
+
+}  // namespace detail
+
+// Teuchos::getParametersFromXmlString for <ParameterList>/<Parameter> documents.
+inline std::shared_ptr<ParameterList> getParametersFromXmlString(const std::string& aXml) {
+    std::shared_ptr<ParameterList> tRoot;
+    std::vector<ParameterList*> tStack;
+    size_t i = 0;
+    while ((i = aXml.find('<', i)) != std::string::npos) {
+        if (aXml.compare(i, 4, "<!--") == 0) {
+            const size_t e = aXml.find("-->", i);
+            if (e == std::string::npos) throw_error("XML: unterminated comment");
+            i = e + 3;
+            continue;
+        }
+        const size_t e = aXml.find('>', i);
+        if (e == std::string::npos) throw_error("XML: unterminated tag");
+        std::string tTag = aXml.substr(i + 1, e - i - 1);
+        i = e + 1;
+        if (!tTag.empty() && tTag[0] == '?') continue;  // <?xml ...?>
+        const bool tClosing = !tTag.empty() && tTag[0] == '/';
+        bool tSelfClosing = !tTag.empty() && tTag.back() == '/';
+        if (tSelfClosing) tTag.pop_back();
+        if (tClosing) {
+            if (tStack.empty()) throw_error("XML: unbalanced </ParameterList>");
+            tStack.pop_back();
+            continue;
+        }
+        size_t k = 0;
+        while (k < tTag.size() && !std::isspace((unsigned char)tTag[k])) ++k;
+        const std::string tElement = tTag.substr(0, k);
+        auto tAttr = detail::xml_attributes(tTag.substr(k));
+        if (tElement == "ParameterList") {
+            const std::string tName = tAttr.count("name") ? tAttr["name"] : "ANONYMOUS";
+            ParameterList* tList;
+            if (tStack.empty()) {
+                if (tRoot) throw_error("XML: more than one top-level ParameterList");
+                tRoot = std::make_shared<ParameterList>(tName);
+                tList = tRoot.get();
+            } else {
+                tList = &tStack.back()->sublist(tName);
+            }
+            if (!tSelfClosing) tStack.push_back(tList);
+        } else if (tElement == "Parameter") {
+            if (tStack.empty()) throw_error("XML: Parameter outside a ParameterList");
+            if (!tAttr.count("name") || !tAttr.count("type") || !tAttr.count("value"))
+                throw_error("XML: Parameter needs name, type and value");
+            tStack.back()->set(tAttr["name"], tAttr["type"], tAttr["value"]);
+        } else {
+            throw_error("XML: unexpected element <" + tElement + ">");
+        }
+    }
+    if (!tRoot) throw_error("XML: no ParameterList");
+    if (!tStack.empty()) throw_error("XML: unterminated ParameterList '" + tStack.back()->name() + "'");
+    return tRoot;
+}
+
+// Constant or linear-in-time expressions of the input's Function strings
+// (numbers, t, + - * / ^, parentheses).  The reference evaluates them with
+// Omega_h's expression reader (src/PlatoMeshExpr.hpp:16-50); functions of
+// x, y, z are not on this path and throw.
+class Expression {
+  public:
+    explicit Expression(std::string aText) : mText(std::move(aText)) {}
+    double operator()(double t) const {
+        mPos = 0;
+        mT = t;
+        const double v = sum();
+        skip();
+        if (mPos != mText.size()) fail();
+        return v;
+    }
+    const std::string& text() const { return mText; }
+
+  private:
+    [[noreturn]] void fail() const {
+        throw_error("expression '" + mText + "': only numbers, t, + - * / ^ and parentheses are supported here");
+    }
+    void skip() const {
+        while (mPos < mText.size() && std::isspace((unsigned char)mText[mPos])) ++mPos;
+    }
+    double sum() const {
+        double v = product();
+        for (;;) {
+            skip();
+            if (mPos < mText.size() && (mText[mPos] == '+' || mText[mPos] == '-')) {
+                const char op = mText[mPos++];
+                const double r = product();
+                v = op == '+' ? v + r : v - r;
+            } else {
+                return v;
+            }
+        }
+    }
+    double product() const {
+        double v = power();
+        for (;;) {
+            skip();
+            if (mPos < mText.size() && (mText[mPos] == '*' || mText[mPos] == '/')) {
+                const char op = mText[mPos++];
+                const double r = power();
+                v = op == '*' ? v * r : v / r;
+            } else {
+                return v;
+            }
+        }
+    }
+    double power() const {
+        const double b = unary();
+        skip();
+        if (mPos < mText.size() && mText[mPos] == '^') {
+            ++mPos;
+            return std::pow(b, power());
+        }
+        return b;
+    }
+    double unary() const {
+        skip();
+        if (mPos < mText.size() && (mText[mPos] == '-' || mText[mPos] == '+')) {
+            const char op = mText[mPos++];
+            const double v = unary();
+            return op == '-' ? -v : v;
+        }
+        return atom();
+    }
+    double atom() const {
+        skip();
+        if (mPos >= mText.size()) fail();
+        if (mText[mPos] == '(') {
+            ++mPos;
+            const double v = sum();
+            skip();
+            if (mPos >= mText.size() || mText[mPos] != ')') fail();
+            ++mPos;
+            return v;
+        }
+        if (mText[mPos] == 't' && (mPos + 1 == mText.size() || !std::isalnum((unsigned char)mText[mPos + 1]))) {
+            ++mPos;
+            return mT;
+        }
+        const char* s = mText.c_str() + mPos;
+        char* tEnd = nullptr;
+        const double v = std::strtod(s, &tEnd);
+        if (tEnd == s) fail();
+        mPos += (size_t)(tEnd - s);
+        return v;
+    }
+    std::string mText;
+    mutable size_t mPos = 0;
+    mutable double mT = 0.0;
+};
+
+// ------------------------------------------------------------------ mesh
+// Tet4 mesh with its assembly graph, side sets and node sets (the reference's
+// Omega_h mesh behind Plato::SpatialModel).
+class Mesh {
+  public:
+    Mesh(const std::vector<double>& aCoords, const std::vector<int>& aConn) {
+        if (aCoords.size() % 3 || aConn.size() % 4) throw_error("Mesh: coords must be N*3 and conn C*4");
+        pa_mesh* tMesh = nullptr;
+        check(pa_mesh_create(aCoords.data(), (int)(aCoords.size() / 3), aConn.data(), (int)(aConn.size() / 4), &tMesh));
+        mMesh = std::shared_ptr<pa_mesh>(tMesh, [](pa_mesh* m) { pa_mesh_destroy(m); });
+        mCoords = aCoords;
+        mConn = aConn;
+    }
+    int NumNodes() const { return (int)(mCoords.size() / 3); }
+    int NumElements() const { return (int)(mConn.size() / 4); }
+    long long NumBlocks() const {
+        int n, c;
+        int64_t b;
+        check(pa_mesh_sizes(mMesh.get(), &n, &c, &b));
+        return (long long)b;
+    }
+    pa_mesh* handle() const { return mMesh.get(); }
+    std::shared_ptr<pa_mesh> shared() const { return mMesh; }
+    const std::vector<double>& coords() const { return mCoords; }
+    const std::vector<int>& connectivity() const { return mConn; }
+    void addSideSet(const std::string& aName, const std::vector<int>& aFaceCells, const std::vector<int>& aFaceLocalNodes) {
+        if (aFaceLocalNodes.size() != 3 * aFaceCells.size()) throw_error("Mesh: side set needs 3 local nodes per face");
+        check(pa_mesh_add_side_set(mMesh.get(), aName.c_str(), (int)aFaceCells.size(), aFaceCells.data(),
+                                   aFaceLocalNodes.data()));
+    }
+    void addNodeSet(const std::string& aName, const std::vector<int>& aNodes) {
+        check(pa_mesh_add_node_set(mMesh.get(), aName.c_str(), (int)aNodes.size(), aNodes.data()));
+    }
+    std::vector<int> getNodeSetNodes(const std::string& aName) const {
+        int n = 0;
+        check(pa_mesh_node_set(mMesh.get(), aName.c_str(), &n, nullptr));
+        std::vector<int> tOut(n);
+        check(pa_mesh_node_set(mMesh.get(), aName.c_str(), &n, tOut.data()));
+        return tOut;
+    }
+    // hex counts of a box built by getBoxMesh (0 otherwise): the multigrid needs them
+    int boxDims[3] = {0, 0, 0};
+
+  private:
+    std::shared_ptr<pa_mesh> mMesh;
+    std::vector<double> mCoords;
+    std::vector<int> mConn;
+};
+
+// Box [0,lx]x[0,ly]x[0,lz] of nx*ny*nz hexes, 6 Tet4 each, with the side sets
+// and node sets x-, x+, y-, y+, z-, z+ of the reference's BamG test meshes
+// (unit_tests/PlatoTestHelpers.hpp:31-84).
+inline std::shared_ptr<Mesh> getBoxMesh(int nx, int ny, int nz, double lx = 1.0, double ly = 1.0, double lz = 1.0) {
+    if (nx < 1 || ny < 1 || nz < 1) throw_error("getBoxMesh: interval counts must be positive");
+    const long N = (long)(nx + 1) * (ny + 1) * (nz + 1), C = 6L * nx * ny * nz;
+    std::vector<double> tCoords(N * 3);
+    std::vector<int> tConn(C * 4);
+    check(pa_box_mesh(nx, ny, nz, lx, ly, lz, tCoords.data(), tConn.data()));
+    auto tMesh = std::make_shared<Mesh>(tCoords, tConn);
+    tMesh->boxDims[0] = nx;
+    tMesh->boxDims[1] = ny;
+    tMesh->boxDims[2] = nz;
+    const double L[3] = {lx, ly, lz};
+    const char* tAxes = "xyz";
+    static const int kFaces[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+    for (int ax = 0; ax < 3; ++ax)
+        for (int side = 0; side < 2; ++side) {
+            const double v = side ? L[ax] : 0.0;
+            auto on = [&](int node) { return std::fabs(tCoords[(size_t)node * 3 + ax] - v) < 1e-12; };
+            std::vector<int> tCells, tLocal, tNodes;
+            for (long c = 0; c < C; ++c)
+                for (const auto& f : kFaces)
+                    if (on(tConn[c * 4 + f[0]]) && on(tConn[c * 4 + f[1]]) && on(tConn[c * 4 + f[2]])) {
+                        tCells.push_back((int)c);
+                        tLocal.insert(tLocal.end(), f, f + 3);
+                    }
+            for (long n = 0; n < N; ++n)
+                if (on((int)n)) tNodes.push_back((int)n);
+            const std::string tName = std::string(1, tAxes[ax]) + (side ? "+" : "-");
+            tMesh->addSideSet(tName, tCells, tLocal);
+            tMesh->addNodeSet(tName, tNodes);
+        }
+    return tMesh;
+}
+
+// PlatoUtestHelpers::getBoxMesh("TET4", meshWidth): the unit cube with
+// meshWidth intervals per direction.
+inline std::shared_ptr<Mesh> getBoxMesh(const std::string& aType, int aMeshWidth) {
+    if (aType != "TET4") throw_error("getBoxMesh: element type '" + aType + "' is not supported (TET4 only)");
+    return getBoxMesh(aMeshWidth, aMeshWidth, aMeshWidth, 1.0, 1.0, 1.0);
+}
+
+// ------------------------------------------------------------ CrsMatrix
+// Block CRS matrix on the mesh graph (src/alg/CrsMatrix.hpp): row = node,
+// sorted column node ids with the diagonal; block entry e*bi*bj + i*bj + j.
+template <class T = Scalar>
+class CrsMatrix {
+  public:
+    CrsMatrix(const Mesh& aMesh, int aNumRowsPerBlock, int aNumColsPerBlock, const std::string& aLabel)
+        : mNumRowsPerBlock(aNumRowsPerBlock), mNumColsPerBlock(aNumColsPerBlock),
+          mIsBlockMatrix(aNumRowsPerBlock * aNumColsPerBlock > 1) {
+        const int* r = nullptr;
+        const int* c = nullptr;
+        check(pa_mesh_graph_dev(aMesh.handle(), &r, &c));
+        const long long nnzb = aMesh.NumBlocks();
+        auto tOwner = std::static_pointer_cast<void>(aMesh.shared());
+        mRowMap = OrdinalVector::alias(tOwner, const_cast<int*>(r), (size_t)aMesh.NumNodes() + 1, "row map");
+        mColumnIndices = OrdinalVector::alias(tOwner, const_cast<int*>(c), (size_t)nnzb, "column indices");
+        mEntries = View<T>(aLabel, (size_t)nnzb * aNumRowsPerBlock * aNumColsPerBlock);
+        mNumRows = aMesh.NumNodes() * aNumRowsPerBlock;
+        mNumCols = aMesh.NumNodes() * aNumColsPerBlock;
+    }
+    const OrdinalVector& rowMap() const { return mRowMap; }
+    const OrdinalVector& columnIndices() const { return mColumnIndices; }
+    const View<T>& entries() const { return mEntries; }
+    int numRows() const { return mNumRows; }
+    int numCols() const { return mNumCols; }
+    int numRowsPerBlock() const { return mNumRowsPerBlock; }
+    int numColsPerBlock() const { return mNumColsPerBlock; }
+    bool isBlockMatrix() const { return mIsBlockMatrix; }
+
+  private:
+    OrdinalVector mRowMap, mColumnIndices;
+    View<T> mEntries;
+    int mNumRows = 0, mNumCols = 0, mNumRowsPerBlock = 1, mNumColsPerBlock = 1;
+    bool mIsBlockMatrix = false;
+};
+
+using CrsMatrixType = CrsMatrix<Scalar>;
+
+// --------------------------------------------------------------- physics
+template <int SpaceDim>
+struct Mechanics {
+    static_assert(SpaceDim == 3, "this path is three-dimensional");
+    static constexpr int mNumSpatialDims = SpaceDim;
+    static constexpr int mNumDofsPerNode = SpaceDim;
+    static constexpr int mPhysics = PA_MECHANICS;
+    static constexpr int mEnergy = PA_INTERNAL_ELASTIC_ENERGY;
+    static constexpr const char* mEnergyName = "Internal Elastic Energy";
+    static std::vector<std::string> dofNames() { return {"Displacement X", "Displacement Y", "Displacement Z"}; }
+};
+
+template <int SpaceDim>
+struct Thermal {
+    static_assert(SpaceDim == 3, "this path is three-dimensional");
+    static constexpr int mNumSpatialDims = SpaceDim;
+    static constexpr int mNumDofsPerNode = 1;
+    static constexpr int mPhysics = PA_THERMAL;
+    static constexpr int mEnergy = PA_INTERNAL_THERMAL_ENERGY;
+    static constexpr const char* mEnergyName = "Internal Thermal Energy";
+    static std::vector<std::string> dofNames() { return {"Temperature"}; }
+};
+
+// Physics "Plasticity": the stabilized mixed u-p J2 form (4 dofs per node).
+template <int SpaceDim>
+struct InfinitesimalStrainPlasticity {
+    static_assert(SpaceDim == 3, "this path is three-dimensional");
+    static constexpr int mNumSpatialDims = SpaceDim;
+    static constexpr int mNumDofsPerNode = SpaceDim + 1;
+    static constexpr int mNumLocalDofsPerCell = 14;
+    static constexpr int mFormulation = PA_J2_MIXED;
+};
+
+// Pure-displacement J2 (3 dofs per node); no counterpart in the reference.
+template <int SpaceDim>
+struct DisplacementPlasticity {
+    static_assert(SpaceDim == 3, "this path is three-dimensional");
+    static constexpr int mNumSpatialDims = SpaceDim;
+    static constexpr int mNumDofsPerNode = SpaceDim;
+    static constexpr int mNumLocalDofsPerCell = 14;
+    static constexpr int mFormulation = PA_J2_DISPLACEMENT;
+};
+
+namespace detail {
+
+// Material model sublist of the (single) domain: Spatial Model/Domains/<d>/
+// Material Model names an entry of Material Models (src/SpatialModel.hpp).
+inline const ParameterList& domain_material(const ParameterList& aParams) {
+    if (!aParams.isSublist("Material Models")) throw_error("input: no 'Material Models' sublist");
+    const auto& tModels = aParams.sublist("Material Models");
+    std::string tName;
+    if (aParams.isSublist("Spatial Model") && aParams.sublist("Spatial Model").isSublist("Domains")) {
+        const auto& tDomains = aParams.sublist("Spatial Model").sublist("Domains");
+        const auto tNames = tDomains.sublistNames();
+        if (tNames.size() != 1) throw_error("input: exactly one domain is supported on this path");
+        tName = tDomains.sublist(tNames[0]).get<std::string>("Material Model");
+    } else {
+        const auto tNames = tModels.sublistNames();
+        if (tNames.size() != 1) throw_error("input: name the material model through 'Spatial Model/Domains'");
+        tName = tNames[0];
+    }
+    if (!tModels.isSublist(tName)) throw_error("input: material model '" + tName + "' is not defined");
+    return tModels.sublist(tName);
+}
+
+// SIMP penalty of a 'Penalty Function' sublist (src/Simp.hpp:41-52).
+inline void simp(const ParameterList& aList, double aExponentDefault, double aMinDefault, double aOut[2]) {
+    const std::string tType = aList.get<std::string>("Type", std::string("SIMP"));
+    if (tType != "SIMP") throw_error("Penalty Function type '" + tType + "' is not supported (SIMP only)");
+    aOut[0] = aList.get<double>("Exponent", aExponentDefault);
+    aOut[1] = aList.get<double>("Minimum Value", aMinDefault);
+}
+
+inline double constant_function(const std::string& aText, const std::string& aWhat) {
+    Expression e(aText);
+    const double v0 = e(0.0), v1 = e(1.0);
+    if (v0 != v1) throw_error(aWhat + ": function '" + aText + "' is not constant");
+    return v0;
+}
+
+// Dirichlet dofs and end values of 'Essential Boundary Conditions'
+// (src/EssentialBCs.hpp:81-99, src/EssentialBC.hpp:31-65), sorted, the later
+// entry winning on a repeated dof.  allowTime: 'Time Dependent' functions
+// linear in t with f(0) = 0 give their t = 1 value (the load ramp scales it).
+inline void essential_bcs(const ParameterList& aParams, const Mesh& aMesh, int aDofsPerNode, bool aAllowTime,
+                          std::vector<int>& aDofs, std::vector<double>& aValues) {
+    std::map<int, double> tBcs;
+    if (aParams.isSublist("Essential Boundary Conditions")) {
+        const auto& tList = aParams.sublist("Essential Boundary Conditions");
+        for (const auto& tName : tList.sublistNames()) {
+            const auto& bc = tList.sublist(tName);
+            const std::string tType = bc.get<std::string>("Type");
+            const int tIndex = bc.get<int>("Index", 0);
+            if (tIndex < 0 || tIndex >= aDofsPerNode) throw_error("essential BC '" + tName + "': Index out of range");
+            double tValue = 0.0;
+            if (tType == "Zero Value") {
+                tValue = 0.0;
+            } else if (tType == "Fixed Value") {
+                tValue = bc.isType<std::string>("Function")
+                             ? constant_function(bc.get<std::string>("Function"), "essential BC '" + tName + "'")
+                             : bc.get<double>("Value");
+                if (aAllowTime && tValue != 0.0)
+                    throw_error("essential BC '" + tName + "': a constant non-zero value is not supported by the "
+                                "pseudo-time driver (use 'Time Dependent' with a function of t)");
+            } else if (tType == "Time Dependent" && aAllowTime) {
+                Expression e(bc.get<std::string>("Function"));
+                const double f0 = e(0.0), f1 = e(1.0), fh = e(0.5);
+                if (f0 != 0.0 || std::fabs(fh - 0.5 * f1) > 1e-12 * std::fabs(f1))
+                    throw_error("essential BC '" + tName + "': only functions a*t are supported");
+                tValue = f1;
+            } else {
+                throw_error("essential BC '" + tName + "': type '" + tType + "' is not supported on this path");
+            }
+            for (int n : aMesh.getNodeSetNodes(bc.get<std::string>("Sides")))
+                tBcs[n * aDofsPerNode + tIndex] = tValue;
+        }
+    }
+    aDofs.clear();
+    aValues.clear();
+    for (const auto& kv : tBcs) {
+        aDofs.push_back(kv.first);
+        aValues.push_back(kv.second);
+    }
+}
+
+}  // namespace detail
+
+namespace Elliptic {
+
+// ----------------------------------------------------- VectorFunction
+// Elliptic::VectorFunction (src/elliptic/VectorFunction.hpp:115-710) with the
+// residual of Mechanics (ElastostaticResidual) or Thermal
+// (ThermostaticResidual), SIMP from '<type>/Penalty Function', uniform
+// natural BCs and constant body loads.
+template <class PhysicsT>
+class VectorFunction {
+  public:
+    static constexpr int mNumDofsPerNode = PhysicsT::mNumDofsPerNode;
+
+    VectorFunction(std::shared_ptr<Mesh> aMesh, const ParameterList& aProblemParams,
+                   const std::string& aType = "Elliptic")
+        : mMesh(std::move(aMesh)) {
+        if (aType != "Elliptic") throw_error("VectorFunction: PDE type '" + aType + "' is not supported (Elliptic)");
+        const auto& tMaterial = detail::domain_material(aProblemParams);
+        double tMat[2] = {0.0, 0.0}, tPen[2];
+        if (PhysicsT::mPhysics == PA_MECHANICS) {
+            if (!tMaterial.isSublist("Isotropic Linear Elastic"))
+                throw_error("VectorFunction: the material needs an 'Isotropic Linear Elastic' sublist");
+            const auto& e = tMaterial.sublist("Isotropic Linear Elastic");
+            tMat[0] = e.get<double>("Youngs Modulus");
+            tMat[1] = e.get<double>("Poissons Ratio");
+        } else {
+            if (!tMaterial.isSublist("Thermal Conduction"))
+                throw_error("VectorFunction: the material needs a 'Thermal Conduction' sublist");
+            tMat[0] = tMaterial.sublist("Thermal Conduction").get<double>("Thermal Conductivity");
+        }
+        const ParameterList tEmpty;
+        const auto& tRes = aProblemParams.isSublist(aType) ? aProblemParams.sublist(aType) : tEmpty;
+        detail::simp(tRes.isSublist("Penalty Function") ? tRes.sublist("Penalty Function") : tEmpty, 3.0, 1e-9, tPen);
+        pa_vector_function* tVf = nullptr;
+        check(pa_vector_function_create(mMesh->handle(), PhysicsT::mPhysics, tMat, tPen, &tVf));
+        mVf = std::shared_ptr<pa_vector_function>(tVf, [](pa_vector_function* v) { pa_vector_function_destroy(v); });
+        if (aProblemParams.isSublist("Natural Boundary Conditions")) readNaturalBCs(aProblemParams.sublist("Natural Boundary Conditions"));
+        if (aProblemParams.isSublist("Body Loads")) readBodyLoads(aProblemParams.sublist("Body Loads"));
+    }
+
+    ScalarVector value(const ScalarVector& aState, const ScalarVector& aControl, Scalar /*aTimeStep*/ = 0.0) const {
+        checkSizes(aState, aControl);
+        ScalarVector tResidual("Residual", size());
+        check(pa_vector_function_value(mVf.get(), aState.data(), aControl.data(), tResidual.data(), nullptr));
+        return tResidual;
+    }
+    std::shared_ptr<CrsMatrixType> gradient_u(const ScalarVector& aState, const ScalarVector& aControl,
+                                              Scalar /*aTimeStep*/ = 0.0) const {
+        checkSizes(aState, aControl);
+        auto tJ = std::make_shared<CrsMatrixType>(*mMesh, mNumDofsPerNode, mNumDofsPerNode, "Jacobian");
+        check(pa_vector_function_gradient_u(mVf.get(), aState.data(), aControl.data(), tJ->entries().data(), PA_FP64,
+                                            nullptr));
+        return tJ;
+    }
+    // gradient_u with FP32 values (the mixed-precision solve's matrix); no reference counterpart
+    std::shared_ptr<CrsMatrix<float>> gradient_u_fp32(const ScalarVector& aState, const ScalarVector& aControl) const {
+        checkSizes(aState, aControl);
+        auto tJ = std::make_shared<CrsMatrix<float>>(*mMesh, mNumDofsPerNode, mNumDofsPerNode, "Jacobian");
+        check(pa_vector_function_gradient_u(mVf.get(), aState.data(), aControl.data(), tJ->entries().data(), PA_FP32,
+                                            nullptr));
+        return tJ;
+    }
+    std::shared_ptr<CrsMatrixType> gradient_u_T(const ScalarVector& aState, const ScalarVector& aControl,
+                                                Scalar /*aTimeStep*/ = 0.0) const {
+        checkSizes(aState, aControl);
+        auto tJ = std::make_shared<CrsMatrixType>(*mMesh, mNumDofsPerNode, mNumDofsPerNode, "Jacobian Transpose");
+        check(pa_vector_function_gradient_u_T(mVf.get(), aState.data(), aControl.data(), tJ->entries().data(),
+                                              PA_FP64, nullptr));
+        return tJ;
+    }
+    // (dR/dz)^T on the node graph with 1 x dpn blocks
+    std::shared_ptr<CrsMatrixType> gradient_z(const ScalarVector& aState, const ScalarVector& aControl,
+                                              Scalar /*aTimeStep*/ = 0.0) const {
+        checkSizes(aState, aControl);
+        auto tJ = std::make_shared<CrsMatrixType>(*mMesh, 1, mNumDofsPerNode, "Gradient Z");
+        check(pa_vector_function_gradient_z(mVf.get(), aState.data(), aControl.data(), tJ->entries().data(), nullptr));
+        return tJ;
+    }
+    OrdinalType size() const { return mMesh->NumNodes() * mNumDofsPerNode; }
+    OrdinalType numNodes() const { return mMesh->NumNodes(); }
+    OrdinalType numCells() const { return mMesh->NumElements(); }
+    OrdinalType numDofsPerNode() const { return mNumDofsPerNode; }
+    OrdinalType numNodesPerCell() const { return 4; }
+    OrdinalType numDofsPerCell() const { return 4 * mNumDofsPerNode; }
+    OrdinalType numControlsPerNode() const { return 1; }
+    std::vector<std::string> getDofNames() const { return PhysicsT::dofNames(); }
+    pa_vector_function* handle() const { return mVf.get(); }
+    const std::shared_ptr<Mesh>& mesh() const { return mMesh; }
+
+  private:
+    void checkSizes(const ScalarVector& aState, const ScalarVector& aControl) const {
+        if ((OrdinalType)aState.size() != size() || (OrdinalType)aControl.size() != numNodes())
+            throw_error("VectorFunction: state must hold N*dofs and control N values");
+    }
+    // NaturalBCs::appendNaturalBC (src/NaturalBCs.hpp:110-224): 'Uniform' with
+    // 'Values' (one per dof) or 'Value' at 'Index'; 'Uniform Component' with
+    // 'Value' at 'Component' x|y|z.  Values may be constant expressions.
+    void readNaturalBCs(const ParameterList& aList) {
+        for (const auto& tName : aList.sublistNames()) {
+            const auto& bc = aList.sublist(tName);
+            const std::string tType = bc.get<std::string>("Type");
+            std::vector<double> tVec(mNumDofsPerNode, 0.0);
+            auto scalar = [&](const std::string& aKey) {
+                return bc.isType<std::string>(aKey) ? detail::constant_function(bc.get<std::string>(aKey), tName)
+                                                    : bc.get<double>(aKey);
+            };
+            if (tType == "Uniform") {
+                const bool tHasValues = bc.isParameter("Values"), tHasValue = bc.isParameter("Value");
+                if (tHasValues == tHasValue)
+                    throw_error("natural BC '" + tName + "': give exactly one of 'Values' and 'Value'");
+                if (tHasValues) {
+                    std::vector<double> v;
+                    if (bc.isType<std::vector<std::string>>("Values")) {
+                        for (const auto& s : bc.get<std::vector<std::string>>("Values"))
+                            v.push_back(detail::constant_function(s, tName));
+                    } else {
+                        v = bc.get<std::vector<double>>("Values");
+                    }
+                    if ((int)v.size() != mNumDofsPerNode) throw_error("natural BC '" + tName + "': wrong 'Values' length");
+                    tVec = v;
+                } else {
+                    const int tDof = bc.get<int>("Index", 0);
+                    if (tDof < 0 || tDof >= mNumDofsPerNode) throw_error("natural BC '" + tName + "': Index out of range");
+                    tVec[tDof] = scalar("Value");
+                }
+            } else if (tType == "Uniform Component") {
+                const std::string c = bc.get<std::string>("Component");
+                const int tDof = c == "x" || c == "X" ? 0 : c == "y" || c == "Y" ? 1 : c == "z" || c == "Z" ? 2 : -1;
+                if (tDof < 0 || tDof >= mNumDofsPerNode) throw_error("natural BC '" + tName + "': bad Component");
+                tVec[tDof] = scalar("Value");
+            } else {
+                throw_error("Natural Boundary Condition Type '" + tType + "' is not supported on this path");
+            }
+            check(pa_vector_function_add_uniform_natural_bc(mVf.get(), bc.get<std::string>("Sides").c_str(),
+                                                           tVec.data()));
+        }
+    }
+    // BodyLoads (src/BodyLoads.hpp:40-122): constant 'Function' at 'Index'.
+    void readBodyLoads(const ParameterList& aList) {
+        for (const auto& tName : aList.sublistNames()) {
+            const auto& bl = aList.sublist(tName);
+            const double v = detail::constant_function(bl.get<std::string>("Function"), "body load '" + tName + "'");
+            check(pa_vector_function_add_body_load(mVf.get(), bl.get<int>("Index", 0), v));
+        }
+    }
+
+    std::shared_ptr<Mesh> mMesh;
+    std::shared_ptr<pa_vector_function> mVf;
+};
+
+// ------------------------------------------------ PhysicsScalarFunction
+// Elliptic::PhysicsScalarFunction with 'Internal Elastic Energy' or
+// 'Internal Thermal Energy' (src/elliptic/PhysicsScalarFunction.hpp:60-494),
+// SIMP from 'Criteria/<name>/Penalty Function'.
+template <class PhysicsT>
+class PhysicsScalarFunction {
+  public:
+    PhysicsScalarFunction(std::shared_ptr<Mesh> aMesh, const ParameterList& aProblemParams,
+                          const std::string& aFunctionName)
+        : mMesh(std::move(aMesh)), mName(aFunctionName) {
+        if (!aProblemParams.isSublist("Criteria") || !aProblemParams.sublist("Criteria").isSublist(aFunctionName))
+            throw_error("PhysicsScalarFunction: no criterion '" + aFunctionName + "'");
+        const auto& tCrit = aProblemParams.sublist("Criteria").sublist(aFunctionName);
+        const std::string tType = tCrit.get<std::string>("Scalar Function Type", std::string(""));
+        if (tType != PhysicsT::mEnergyName)
+            throw_error("PhysicsScalarFunction: scalar function type '" + tType + "' is not supported for this physics ('" +
+                        PhysicsT::mEnergyName + "')");
+        const auto& tMaterial = detail::domain_material(aProblemParams);
+        double tMat[2] = {0.0, 0.0}, tPen[2];
+        if (PhysicsT::mPhysics == PA_MECHANICS) {
+            const auto& e = tMaterial.sublist("Isotropic Linear Elastic");
+            tMat[0] = e.get<double>("Youngs Modulus");
+            tMat[1] = e.get<double>("Poissons Ratio");
+        } else {
+            tMat[0] = tMaterial.sublist("Thermal Conduction").get<double>("Thermal Conductivity");
+        }
+        const ParameterList tEmpty;
+        detail::simp(tCrit.isSublist("Penalty Function") ? tCrit.sublist("Penalty Function") : tEmpty, 3.0, 1e-9, tPen);
+        pa_scalar_function* tSf = nullptr;
+        check(pa_scalar_function_create(mMesh->handle(), PhysicsT::mEnergy, tMat, tPen, &tSf));
+        mSf = std::shared_ptr<pa_scalar_function>(tSf, [](pa_scalar_function* s) { pa_scalar_function_destroy(s); });
+    }
+
+    Scalar value(const ScalarVector& aState, const ScalarVector& aControl, Scalar /*aTimeStep*/ = 0.0) const {
+        ScalarVector tValue("value", 1);
+        check(pa_scalar_function_value(mSf.get(), aState.data(), aControl.data(), tValue.data(), nullptr));
+        return toHost(tValue)[0];
+    }
+    Scalar value(const Solutions& aSolution, const ScalarVector& aControl, Scalar aTimeStep = 0.0) const {
+        return value(aSolution.get("State"), aControl, aTimeStep);
+    }
+    ScalarVector gradient_u(const ScalarVector& aState, const ScalarVector& aControl, Scalar = 0.0) const {
+        ScalarVector tOut("gradient_u", aState.size());
+        check(pa_scalar_function_gradient_u(mSf.get(), aState.data(), aControl.data(), tOut.data(), nullptr));
+        return tOut;
+    }
+    ScalarVector gradient_u(const Solutions& aSolution, const ScalarVector& aControl, OrdinalType aStepIndex = 0,
+                            Scalar aTimeStep = 0.0) const {
+        return gradient_u(aSolution.get("State", aStepIndex), aControl, aTimeStep);
+    }
+    ScalarVector gradient_z(const ScalarVector& aState, const ScalarVector& aControl, Scalar = 0.0) const {
+        ScalarVector tOut("gradient_z", aControl.size());
+        check(pa_scalar_function_gradient_z(mSf.get(), aState.data(), aControl.data(), tOut.data(), nullptr));
+        return tOut;
+    }
+    ScalarVector gradient_z(const Solutions& aSolution, const ScalarVector& aControl, Scalar aTimeStep = 0.0) const {
+        return gradient_z(aSolution.get("State"), aControl, aTimeStep);
+    }
+    std::string name() const { return mName; }
+    pa_scalar_function* handle() const { return mSf.get(); }
+
+  private:
+    std::shared_ptr<Mesh> mMesh;
+    std::string mName;
+    std::shared_ptr<pa_scalar_function> mSf;
+};
+
+// -------------------------------------------------------------- Problem
+// Elliptic::Problem (src/elliptic/Problem.hpp:96-450): one Newton step per
+// solution (R(0, z), K, symmetric Dirichlet elimination, K du = -R), the
+// criteria of 'Criteria' and their self-adjoint gradients ('Self-Adjoint'
+// must be true: the general adjoint of this path is lambda = -u).
+// 'Linear Solver' keys: Tolerance (default 1e-14, as AztecOO's), Iterations
+// (1000); and, with no reference counterpart, Preconditioner ('Block Jacobi',
+// 'Jacobi', 'None', 'Multigrid') and Precision ('Double' | 'Mixed').
+template <class PhysicsT>
+class Problem {
+  public:
+    Problem(std::shared_ptr<Mesh> aMesh, const ParameterList& aProblemParams)
+        : mMesh(aMesh), mPDE(std::make_shared<VectorFunction<PhysicsT>>(
+                            aMesh, aProblemParams, aProblemParams.get<std::string>("PDE Constraint"))) {
+        if (!aProblemParams.get<bool>("Self-Adjoint", false))
+            throw_error("Problem: only self-adjoint criteria ('Self-Adjoint' = true) are on this path");
+        detail::essential_bcs(aProblemParams, *aMesh, PhysicsT::mNumDofsPerNode, false, mBcDofs, mBcValues);
+        const ParameterList tEmpty;
+        const auto& ls = aProblemParams.isSublist("Linear Solver") ? aProblemParams.sublist("Linear Solver") : tEmpty;
+        mOptions.mode = ls.get<std::string>("Precision", std::string("Double")) == "Mixed" ? PA_FP32 : PA_FP64;
+        mOptions.tol = ls.get<double>("Tolerance", 1e-14);
+        mOptions.max_iter = ls.get<int>("Iterations", 1000);
+        mOptions.replace_every = ls.get<int>("Replace Every", 200);
+        mOptions.check_every = ls.get<int>("Check Every", 10);
+        const std::string p = ls.get<std::string>("Preconditioner", std::string("Block Jacobi"));
+        mMultigrid = p == "Multigrid";
+        mOptions.precond = p == "None" ? PA_PRECOND_NONE : p == "Jacobi" ? PA_PRECOND_JACOBI : PA_PRECOND_BLOCK_JACOBI;
+        if (aProblemParams.isSublist("Criteria")) {
+            const auto& cl = aProblemParams.sublist("Criteria");
+            for (const auto& tName : cl.sublistNames()) {
+                if (cl.sublist(tName).get<std::string>("Type", std::string("Scalar Function")) != "Scalar Function")
+                    throw_error("Problem: criterion '" + tName + "' is not a 'Scalar Function'");
+                auto tSf = std::make_shared<PhysicsScalarFunction<PhysicsT>>(aMesh, aProblemParams, tName);
+                pa_problem* tP = nullptr;
+                check(pa_problem_create(mPDE->handle(), tSf->handle(), (int)mBcDofs.size(), mBcDofs.data(),
+                                        mBcValues.data(), &mOptions, &tP));
+                auto tProblem = std::shared_ptr<pa_problem>(tP, [](pa_problem* q) { pa_problem_destroy(q); });
+                if (mMultigrid) {
+                    if (!mMesh->boxDims[0]) throw_error("Problem: the multigrid preconditioner needs a box mesh");
+                    pa_multigrid_options o{};
+                    o.pre_sweeps = o.post_sweeps = 2;
+                    o.omega = 0.6;
+                    check(pa_problem_set_multigrid(tP, mMesh->boxDims[0], mMesh->boxDims[1], mMesh->boxDims[2], &o));
+                }
+                mCriteria[tName] = {tSf, tProblem};
+                if (mFirst.empty()) mFirst = tName;
+            }
+        }
+        if (mFirst.empty()) throw_error("Problem: no criteria");
+    }
+
+    Solutions solution(const ScalarVector& aControl) {
+        ScalarVector tState("State", (size_t)mPDE->size());
+        check(pa_problem_solution(mCriteria.at(mFirst).problem.get(), aControl.data(), tState.data(), &mStats, nullptr));
+        Solutions tS;
+        tS.set("State", tState);
+        mLast = tS;
+        return tS;
+    }
+    Scalar criterionValue(const ScalarVector& aControl, const Solutions& aSolution, const std::string& aName) {
+        ScalarVector tValue("value", 1);
+        check(pa_problem_criterion_value(criterion(aName).problem.get(), aControl.data(), aSolution.get("State").data(),
+                                         tValue.data(), nullptr));
+        return toHost(tValue)[0];
+    }
+    Scalar criterionValue(const ScalarVector& aControl, const std::string& aName) {
+        return criterionValue(aControl, lastSolution(aControl), aName);
+    }
+    ScalarVector criterionGradient(const ScalarVector& aControl, const Solutions& aSolution, const std::string& aName) {
+        ScalarVector tGrad("criterion gradient", aControl.size());
+        check(pa_problem_criterion_gradient(criterion(aName).problem.get(), aControl.data(),
+                                            aSolution.get("State").data(), tGrad.data(), nullptr));
+        return tGrad;
+    }
+    ScalarVector criterionGradient(const ScalarVector& aControl, const std::string& aName) {
+        return criterionGradient(aControl, lastSolution(aControl), aName);
+    }
+    const pa_solve_stats& solverStats() const { return mStats; }
+    const std::vector<int>& bcDofs() const { return mBcDofs; }
+    const VectorFunction<PhysicsT>& getPDE() const { return *mPDE; }
+
+  private:
+    struct Criterion {
+        std::shared_ptr<PhysicsScalarFunction<PhysicsT>> function;
+        std::shared_ptr<pa_problem> problem;
+    };
+    const Criterion& criterion(const std::string& aName) const {
+        auto it = mCriteria.find(aName);
+        if (it == mCriteria.end()) throw_error("Problem: no criterion '" + aName + "'");
+        return it->second;
+    }
+    const Solutions& lastSolution(const ScalarVector& aControl) {
+        if (mLast.empty()) solution(aControl);
+        return mLast;
+    }
+
+    std::shared_ptr<Mesh> mMesh;
+    std::shared_ptr<VectorFunction<PhysicsT>> mPDE;
+    std::map<std::string, Criterion> mCriteria;
+    std::string mFirst;
+    std::vector<int> mBcDofs;
+    std::vector<double> mBcValues;
+    pa_problem_options mOptions{};
+    bool mMultigrid = false;
+    pa_solve_stats mStats{};
+    Solutions mLast;
+};
+
+}  // namespace Elliptic
+
+// ----------------------------------------------------- J2 plasticity
+namespace detail {
+
+// pa_j2_params from the reference's input lists: Material Models/<m>/
+// {Isotropic Linear Elastic, Plasticity Model/J2 Plasticity}
+// (src/J2PlasticityLocalResidual.hpp:148-220), Material Models/Pressure
+// Scaling, Elliptic/Penalty Function (src/InfinitesimalStrainPlasticityResidual.hpp:142-160),
+// State Gradient Projection/Penalty Function (src/Projection.hpp:30-62) and the
+// plastic work criterion's Penalty Function (src/PlasticWorkCriterion.hpp:295-303).
+inline pa_j2_params j2_params(const ParameterList& aParams, int aFormulation, const std::string& aWorkCriterion) {
+    pa_j2_params p;
+    pa_j2_params_default(&p);
+    p.formulation = aFormulation;
+    const auto& tMaterial = domain_material(aParams);
+    if (!tMaterial.isSublist("Isotropic Linear Elastic"))
+        throw_error("plasticity: the material needs an 'Isotropic Linear Elastic' sublist");
+    const auto& e = tMaterial.sublist("Isotropic Linear Elastic");
+    p.youngs = e.get<double>("Youngs Modulus");
+    p.poisson = e.get<double>("Poissons Ratio");
+    if (!tMaterial.isSublist("Plasticity Model") || !tMaterial.sublist("Plasticity Model").isSublist("J2 Plasticity"))
+        throw_error("plasticity: the material needs a 'Plasticity Model/J2 Plasticity' sublist");
+    const auto& j = tMaterial.sublist("Plasticity Model").sublist("J2 Plasticity");
+    p.hardening_isotropic = j.get<double>("Hardening Modulus Isotropic");
+    p.hardening_kinematic = j.get<double>("Hardening Modulus Kinematic");
+    p.initial_yield_stress = j.get<double>("Initial Yield Stress");
+    p.elastic_penalty_exponent = j.get<double>("Elastic Properties Penalty Exponent", 1.0);
+    p.elastic_min_ersatz = j.get<double>("Elastic Properties Minimum Ersatz", 1e-9);
+    p.plastic_penalty_exponent = j.get<double>("Plastic Properties Penalty Exponent", 0.5);
+    p.plastic_min_ersatz = j.get<double>("Plastic Properties Minimum Ersatz", 1e-4);
+    p.pressure_scaling = aParams.sublist("Material Models").get<double>("Pressure Scaling", 1.0);
+    const ParameterList tEmpty;
+    double tPen[2];
+    const auto& el = aParams.isSublist("Elliptic") ? aParams.sublist("Elliptic") : tEmpty;
+    simp(el.isSublist("Penalty Function") ? el.sublist("Penalty Function") : tEmpty, 1.0, 1e-9, tPen);
+    p.global_penalty_exponent = tPen[0];
+    p.global_min_ersatz = tPen[1];
+    if (aParams.isSublist("State Gradient Projection") &&
+        aParams.sublist("State Gradient Projection").isSublist("Penalty Function")) {
+        const auto& pp = aParams.sublist("State Gradient Projection").sublist("Penalty Function");
+        const std::string t = pp.get<std::string>("Type", std::string("NoPenalty"));
+        if (t == "SIMP") {
+            p.projection_penalty = 1;
+            p.projection_penalty_exponent = pp.get<double>("Exponent", 3.0);
+            p.projection_min_ersatz = pp.get<double>("Minimum Value", 1e-9);
+        } else if (t != "NoPenalty") {
+            throw_error("State Gradient Projection penalty '" + t + "' is not supported (SIMP or NoPenalty)");
+        }
+    }
+    if (!aWorkCriterion.empty()) {
+        const auto& c = aParams.sublist("Criteria").sublist(aWorkCriterion);
+        simp(c.isSublist("Penalty Function") ? c.sublist("Penalty Function") : tEmpty, 3.0, 1e-9, tPen);
+        p.work_penalty_exponent = tPen[0];
+        p.work_min_ersatz = tPen[1];
+    }
+    return p;
+}
+
+// the 'Plastic Work' criterion of the input ("" if none); more than one throws
+inline std::string plastic_work_criterion(const ParameterList& aParams, double& aMultiplier) {
+    std::string tFound;
+    aMultiplier = 1.0;
+    if (!aParams.isSublist("Criteria")) return tFound;
+    const auto& cl = aParams.sublist("Criteria");
+    for (const auto& n : cl.sublistNames()) {
+        const std::string t = cl.sublist(n).get<std::string>("Scalar Function Type", std::string(""));
+        if (t != "Plastic Work") throw_error("plasticity: criterion '" + n + "' of type '" + t + "' is not on this path");
+        if (!tFound.empty()) throw_error("plasticity: more than one 'Plastic Work' criterion");
+        tFound = n;
+        aMultiplier = cl.sublist(n).get<double>("Multiplier", 1.0);  // src/BasicLocalScalarFunction.hpp:714
+    }
+    return tFound;
+}
+
+}  // namespace detail
+
+// Shared J2 evaluator (the kernels behind the local and global functions).
+class PlasticityEvaluator {
+  public:
+    PlasticityEvaluator(std::shared_ptr<Mesh> aMesh, const pa_j2_params& aParams) : mMesh(std::move(aMesh)) {
+        pa_plasticity* p = nullptr;
+        check(pa_plasticity_create(mMesh->handle(), &aParams, &p));
+        mP = std::shared_ptr<pa_plasticity>(p, [](pa_plasticity* q) { pa_plasticity_destroy(q); });
+        mDofsPerNode = aParams.formulation == PA_J2_MIXED ? 4 : 3;
+    }
+    pa_plasticity* handle() const { return mP.get(); }
+    const std::shared_ptr<Mesh>& mesh() const { return mMesh; }
+    int dofsPerNode() const { return mDofsPerNode; }
+
+  private:
+    std::shared_ptr<Mesh> mMesh;
+    std::shared_ptr<pa_plasticity> mP;
+    int mDofsPerNode = 4;
+};
+
+// LocalVectorFunctionInc (src/LocalVectorFunctionInc.hpp:236-874) for the J2
+// local residual: 14 states per cell.  The previous global state and the
+// time data of the reference's signatures are accepted and not needed here.
+template <class PhysicsT>
+class LocalVectorFunctionInc {
+  public:
+    static constexpr int mNumLocalDofsPerCell = 14;
+    LocalVectorFunctionInc(std::shared_ptr<Mesh> aMesh, const ParameterList& aParams)
+        : mEval(std::make_shared<PlasticityEvaluator>(aMesh, detail::j2_params(aParams, PhysicsT::mFormulation, ""))) {}
+    explicit LocalVectorFunctionInc(std::shared_ptr<PlasticityEvaluator> aEval) : mEval(std::move(aEval)) {}
+
+    void updateLocalState(const ScalarVector& aGlobalState, const ScalarVector& /*aPrevGlobalState*/,
+                          const ScalarVector& aLocalState, const ScalarVector& aPrevLocalState,
+                          const ScalarVector& aControl, Scalar /*aTimeData*/ = 0.0) const {
+        check(pa_j2_update_local_state(mEval->handle(), aGlobalState.data(), aPrevLocalState.data(), aControl.data(),
+                                       aLocalState.data(), nullptr));
+    }
+    ScalarVector value(const ScalarVector& aGlobalState, const ScalarVector& /*aPrevGlobalState*/,
+                       const ScalarVector& aLocalState, const ScalarVector& aPrevLocalState,
+                       const ScalarVector& aControl, Scalar /*aTimeData*/ = 0.0) const {
+        ScalarVector tH("Local Residual", (size_t)numCells() * 14);
+        check(pa_j2_local_value(mEval->handle(), aGlobalState.data(), aLocalState.data(), aPrevLocalState.data(),
+                                aControl.data(), tH.data(), nullptr));
+        return tH;
+    }
+    ScalarArray3D gradient_u(const ScalarVector& u, const ScalarVector& up, const ScalarVector& c,
+                             const ScalarVector& cp, const ScalarVector& z, Scalar = 0.0) const {
+        return workset(PA_WRT_U, 16, u, c, cp, z);
+    }
+    ScalarArray3D gradient_c(const ScalarVector& u, const ScalarVector&, const ScalarVector& c, const ScalarVector& cp,
+                             const ScalarVector& z, Scalar = 0.0) const {
+        return workset(PA_WRT_C, 14, u, c, cp, z);
+    }
+    ScalarArray3D gradient_cp(const ScalarVector& u, const ScalarVector&, const ScalarVector& c,
+                              const ScalarVector& cp, const ScalarVector& z, Scalar = 0.0) const {
+        return workset(PA_WRT_C_PREV, 14, u, c, cp, z);
+    }
+    ScalarArray3D gradient_z(const ScalarVector& u, const ScalarVector&, const ScalarVector& c, const ScalarVector& cp,
+                             const ScalarVector& z, Scalar = 0.0) const {
+        return workset(PA_WRT_Z, 4, u, c, cp, z);
+    }
+    OrdinalType numCells() const { return mEval->mesh()->NumElements(); }
+
+  private:
+    ScalarArray3D workset(int aWrt, int aWidth, const ScalarVector& u, const ScalarVector& c, const ScalarVector& cp,
+                          const ScalarVector& z) const {
+        ScalarArray3D tOut("Local Jacobian", (size_t)numCells(), 14, (size_t)aWidth);
+        check(pa_j2_local_gradient(mEval->handle(), aWrt, u.data(), c.data(), cp.data(), z.data(), tOut.data(), nullptr));
+        return tOut;
+    }
+    std::shared_ptr<PlasticityEvaluator> mEval;
+};
+
+// GlobalVectorFunctionInc (src/GlobalVectorFunctionInc.hpp:919-1155) for the
+// J2 global residual.  Worksets are [cell][16][W] with the local dof node*4 + d.
+// The residual does not depend on the previous global or local state, so
+// gradient_up and gradient_cp are zero worksets.
+template <class PhysicsT>
+class GlobalVectorFunctionInc {
+  public:
+    GlobalVectorFunctionInc(std::shared_ptr<Mesh> aMesh, const ParameterList& aParams)
+        : mEval(std::make_shared<PlasticityEvaluator>(aMesh, detail::j2_params(aParams, PhysicsT::mFormulation, ""))) {}
+    explicit GlobalVectorFunctionInc(std::shared_ptr<PlasticityEvaluator> aEval) : mEval(std::move(aEval)) {}
+
+    ScalarVector value(const ScalarVector& u, const ScalarVector& /*up*/, const ScalarVector& c,
+                       const ScalarVector& /*cp*/, const ScalarVector& aProjPressGrad, const ScalarVector& z,
+                       Scalar = 0.0) const {
+        ScalarVector tR("Global Residual", (size_t)size());
+        check(pa_j2_global_value(mEval->handle(), u.data(), c.data(), aProjPressGrad.data(), z.data(), tR.data(),
+                                 nullptr));
+        return tR;
+    }
+    ScalarArray3D gradient_u(const ScalarVector& u, const ScalarVector&, const ScalarVector& c, const ScalarVector&,
+                             const ScalarVector& pi, const ScalarVector& z, Scalar = 0.0) const {
+        return workset(PA_WRT_U, 16, u, c, pi, z);
+    }
+    ScalarArray3D gradient_up(const ScalarVector&, const ScalarVector&, const ScalarVector&, const ScalarVector&,
+                              const ScalarVector&, const ScalarVector&, Scalar = 0.0) const {
+        return ScalarArray3D("Global Jacobian", (size_t)numCells(), 16, 16);
+    }
+    ScalarArray3D gradient_c(const ScalarVector& u, const ScalarVector&, const ScalarVector& c, const ScalarVector&,
+                             const ScalarVector& pi, const ScalarVector& z, Scalar = 0.0) const {
+        return workset(PA_WRT_C, 14, u, c, pi, z);
+    }
+    ScalarArray3D gradient_cp(const ScalarVector&, const ScalarVector&, const ScalarVector&, const ScalarVector&,
+                              const ScalarVector&, const ScalarVector&, Scalar = 0.0) const {
+        return ScalarArray3D("Global Jacobian", (size_t)numCells(), 16, 14);
+    }
+    ScalarArray3D gradient_z(const ScalarVector& u, const ScalarVector&, const ScalarVector& c, const ScalarVector&,
+                             const ScalarVector& pi, const ScalarVector& z, Scalar = 0.0) const {
+        return workset(PA_WRT_Z, 4, u, c, pi, z);
+    }
+    // d R / d (projected pressure gradient), [cell][16][12]
+    ScalarArray3D gradient_n(const ScalarVector& u, const ScalarVector&, const ScalarVector& c, const ScalarVector&,
+                             const ScalarVector& pi, const ScalarVector& z, Scalar = 0.0) const {
+        return workset(PA_WRT_PI, 12, u, c, pi, z);
+    }
+    // NewtonRaphsonSolver::assembleTangentMatrix + assembleResidual
+    // (src/NewtonRaphsonSolver.hpp:204-330) in one call: K_t on the node graph
+    // (dpn x dpn blocks) and R_t; transposed: K_t^T only (the adjoint's operator).
+    std::shared_ptr<CrsMatrixType> condensedSystem(const ScalarVector& u, const ScalarVector& c, const ScalarVector& cp,
+                                                   const ScalarVector& pi, const ScalarVector& z,
+                                                   ScalarVector* aResidual, bool aTranspose = false) const {
+        const int D = mEval->dofsPerNode();
+        auto tK = std::make_shared<CrsMatrixType>(*mEval->mesh(), D, D, "Tangent");
+        if (!aTranspose && !aResidual) throw_error("condensedSystem: a residual output is required");
+        if (aResidual && !aTranspose) *aResidual = ScalarVector("Condensed Residual", (size_t)size());
+        check(pa_j2_condensed_system(mEval->handle(), u.data(), c.data(), cp.data(), pi.data(), z.data(),
+                                     tK->entries().data(), aTranspose ? nullptr : aResidual->data(),
+                                     aTranspose ? 1 : 0, nullptr));
+        return tK;
+    }
+    OrdinalType size() const { return mEval->mesh()->NumNodes() * mEval->dofsPerNode(); }
+    OrdinalType numCells() const { return mEval->mesh()->NumElements(); }
+    OrdinalType numDofsPerNode() const { return mEval->dofsPerNode(); }
+
+  private:
+    ScalarArray3D workset(int aWrt, int aWidth, const ScalarVector& u, const ScalarVector& c, const ScalarVector& pi,
+                          const ScalarVector& z) const {
+        ScalarArray3D tOut("Global Jacobian", (size_t)numCells(), 16, (size_t)aWidth);
+        check(pa_j2_global_gradient(mEval->handle(), aWrt, u.data(), c.data(), pi.data(), z.data(), tOut.data(),
+                                    nullptr));
+        return tOut;
+    }
+    std::shared_ptr<PlasticityEvaluator> mEval;
+};
+
+// PlasticityProblem (src/PlasticityProblem.hpp:96-1068): 'Time Stepping/
+// Initial Num. Pseudo Time Steps' (default 20) with load factor (k+1)/K,
+// 'Newton-Raphson/{Maximum Number Iterations (10), Stopping Tolerance (1e-8),
+// Current Residual Norm Stopping Tolerance (1e-8)}', the 'Plastic Work'
+// criterion with its 'Multiplier', Dirichlet values from 'Essential Boundary
+// Conditions' (Zero Value, Time Dependent a*t).  'Linear Solver' keys, with
+// no reference counterpart: Solver ('GMRES' | 'BiCGStab'), Restart (50),
+// Tolerance (1e-12), Iterations (20000), Preconditioner ('Block Jacobi' |
+// 'Multigrid'; the multigrid needs a box mesh).
+template <class PhysicsT>
+class PlasticityProblem {
+  public:
+    PlasticityProblem(std::shared_ptr<Mesh> aMesh, const ParameterList& aParams) : mMesh(aMesh) {
+        const std::string tPhysics = aParams.get<std::string>("Physics", std::string("Plasticity"));
+        if (tPhysics != "Plasticity") throw_error("PlasticityProblem: Physics '" + tPhysics + "' is not on this path");
+        mCriterion = detail::plastic_work_criterion(aParams, mMultiplier);
+        mEval = std::make_shared<PlasticityEvaluator>(aMesh,
+                                                      detail::j2_params(aParams, PhysicsT::mFormulation, mCriterion));
+        pa_plasticity_options_default(&mOptions);
+        if (aParams.isSublist("Time Stepping"))
+            mOptions.num_steps = aParams.sublist("Time Stepping").get<int>("Initial Num. Pseudo Time Steps", 20);
+        else
+            mOptions.num_steps = 20;
+        if (aParams.isSublist("Newton-Raphson")) {
+            const auto& nr = aParams.sublist("Newton-Raphson");
+            mOptions.max_newton = nr.get<int>("Maximum Number Iterations", 10);
+            mOptions.newton_stop_tol = nr.get<double>("Stopping Tolerance", 1e-8);
+            mOptions.newton_current_norm_tol = nr.get<double>("Current Residual Norm Stopping Tolerance", 1e-8);
+        }
+        if (aParams.isSublist("Linear Solver")) {
+            const auto& ls = aParams.sublist("Linear Solver");
+            mOptions.krylov = ls.get<std::string>("Solver", std::string("GMRES")) == "BiCGStab" ? PA_KRYLOV_BICGSTAB
+                                                                                              : PA_KRYLOV_GMRES;
+            mOptions.restart = ls.get<int>("Restart", 50);
+            mOptions.linear_tol = ls.get<double>("Tolerance", 1e-12);
+            mOptions.linear_max_iter = ls.get<int>("Iterations", 20000);
+            mMultigrid = ls.get<std::string>("Preconditioner", std::string("Block Jacobi")) == "Multigrid";
+        }
+        readEssentialBoundaryConditions(aParams);
+    }
+
+    // PlasticityProblem::readEssentialBoundaryConditions
+    void readEssentialBoundaryConditions(const ParameterList& aParams) {
+        detail::essential_bcs(aParams, *mMesh, mEval->dofsPerNode(), true, mBcDofs, mBcValues);
+        pa_plasticity_problem* q = nullptr;
+        check(pa_plasticity_problem_create(mEval->handle(), (int)mBcDofs.size(), mBcDofs.data(), mBcValues.data(),
+                                           &mOptions, &q));
+        mProblem = std::shared_ptr<pa_plasticity_problem>(q, [](pa_plasticity_problem* r) {
+            pa_plasticity_problem_destroy(r);
+        });
+        if (mMultigrid) {
+            if (!mMesh->boxDims[0]) throw_error("PlasticityProblem: the multigrid preconditioner needs a box mesh");
+            pa_multigrid_options o{};
+            o.pre_sweeps = o.post_sweeps = 4;
+            o.omega = 0.6;
+            check(pa_plasticity_problem_set_multigrid(mProblem.get(), mMesh->boxDims[0], mMesh->boxDims[1],
+                                                      mMesh->boxDims[2], &o));
+        }
+    }
+
+    // Every step's global state ("State"), local state ("Local State") and
+    // projected pressure gradient ("Projected Pressure Gradient"), as device
+    // copies.
+    Solutions solution(const ScalarVector& aControl) {
+        int tConverged = 0;
+        check(pa_plasticity_problem_solution(mProblem.get(), aControl.data(), &tConverged, nullptr));
+        mConverged = tConverged != 0;
+        const int K = mOptions.num_steps;
+        std::vector<ScalarVector> us, cs, pis;
+        const size_t nu = (size_t)mMesh->NumNodes() * mEval->dofsPerNode(), nc = (size_t)mMesh->NumElements() * 14,
+                     np = (size_t)mMesh->NumNodes() * 3;
+        for (int k = 0; k < K; ++k) {
+            const double *u, *c, *pi;
+            check(pa_plasticity_problem_state(mProblem.get(), k, &u, &c, &pi));
+            us.emplace_back("State", nu);
+            cs.emplace_back("Local State", nc);
+            pis.emplace_back("Projected Pressure Gradient", np);
+            check(pa_memcpy_d2d(us.back().data(), u, nu * sizeof(double), nullptr));
+            check(pa_memcpy_d2d(cs.back().data(), c, nc * sizeof(double), nullptr));
+            check(pa_memcpy_d2d(pis.back().data(), pi, np * sizeof(double), nullptr));
+        }
+        Solutions tS;
+        tS.set("State", us);
+        tS.set("Local State", cs);
+        tS.set("Projected Pressure Gradient", pis);
+        return tS;
+    }
+    Scalar criterionValue(const ScalarVector& aControl, const std::string& aName) {
+        requireCriterion(aName);
+        double v = 0.0;
+        check(pa_plasticity_problem_criterion_value(mProblem.get(), aControl.data(), mMultiplier, &v, nullptr));
+        return v;
+    }
+    ScalarVector criterionGradient(const ScalarVector& aControl, const std::string& aName) {
+        requireCriterion(aName);
+        ScalarVector tGrad("criterion gradient", aControl.size());
+        check(pa_plasticity_problem_criterion_gradient(mProblem.get(), aControl.data(), mMultiplier, tGrad.data(),
+                                                       nullptr));
+        return tGrad;
+    }
+    bool converged() const { return mConverged; }
+    // Newton iterations, Krylov iterations and final residual norm of a step
+    void stepStats(int aStep, int& aNewton, int& aLinear, double& aResidual) const {
+        check(pa_plasticity_problem_stats(mProblem.get(), aStep, &aNewton, &aLinear, &aResidual));
+    }
+    std::shared_ptr<PlasticityEvaluator> evaluator() const { return mEval; }
+
+  private:
+    void requireCriterion(const std::string& aName) const {
+        if (aName != mCriterion) throw_error("PlasticityProblem: no criterion '" + aName + "'");
+    }
+    std::shared_ptr<Mesh> mMesh;
+    std::shared_ptr<PlasticityEvaluator> mEval;
+    std::shared_ptr<pa_plasticity_problem> mProblem;
+    pa_plasticity_options mOptions{};
+    std::string mCriterion;
+    double mMultiplier = 1.0;
+    bool mMultigrid = false;
+    bool mConverged = false;
+    std::vector<int> mBcDofs;
+    std::vector<double> mBcValues;
+};
+
+}  // namespace amd
+}  // namespace Plato
+
+#endif  // PLATOANALYZE_AMD_HPP
