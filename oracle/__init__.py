@@ -4,9 +4,10 @@ This package is the checker for the HIP library: the tests, the smoke check
 and the CPU-baseline leg of bench.py load it.  The product library never
 imports, links or calls anything here.
 
-`build()` compiles pa_oracle.cpp (and the J2 restatement) with g++/OpenMP into
-oracle/liboracle.so; `lib()` loads it with ctypes.  The numpy helpers below
-wrap the C entry points.
+`build()` compiles pa_oracle.cpp, the J2 restatement and the optimized CPU
+variant of the elastic assembly (pa_cpu_fast.cpp, a CPU baseline) with
+g++/OpenMP into oracle/liboracle.so; `lib()` loads it with ctypes.  The numpy
+helpers below wrap the C entry points.
 """
 import ctypes
 import os
@@ -17,7 +18,7 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 LIB = HERE / "liboracle.so"
-SOURCES = [HERE / "pa_oracle.cpp", HERE / "pa_oracle_j2.cpp"]
+SOURCES = [HERE / "pa_oracle.cpp", HERE / "pa_oracle_j2.cpp", HERE / "pa_cpu_fast.cpp"]
 
 _lib = None
 
@@ -135,6 +136,47 @@ def _declare(L):
 
 def set_threads(n):
     return lib().orc_set_threads(int(n))
+
+
+class FastElastic:
+    """Optimized CPU elastic value + gradient_u (pa_cpu_fast.cpp): closed-form
+    element stiffness, precomputed block ordinals, coloured atomic-free
+    OpenMP assembly.  A CPU baseline, checked against the restatement."""
+
+    def __init__(self, coords, conn, face_cells=None, face_lnodes=None, traction=(0.0, 0.0, 0.0)):
+        L = lib()
+        vp, i = ctypes.c_void_p, ctypes.c_int
+        L.cpu_fast_create.argtypes = [i, i, vp, vp, i, vp, vp, vp]
+        L.cpu_fast_create.restype = vp
+        L.cpu_fast_destroy.argtypes = [vp]
+        L.cpu_fast_nnzb.argtypes = [vp]
+        L.cpu_fast_nnzb.restype = ctypes.c_longlong
+        L.cpu_fast_colors.argtypes = [vp]
+        L.cpu_fast_colors.restype = i
+        L.cpu_fast_elastic.argtypes = [vp] + [ctypes.c_double] * 4 + [vp] * 4
+        self._keep = [np.ascontiguousarray(coords, np.float64), np.ascontiguousarray(conn, np.int32)]
+        fc = np.ascontiguousarray(face_cells if face_cells is not None else np.zeros(0), np.int32)
+        fl = np.ascontiguousarray(face_lnodes if face_lnodes is not None else np.zeros(0), np.int32)
+        t = np.ascontiguousarray(traction, np.float64)
+        self.N, self.C = self._keep[0].size // 3, self._keep[1].size // 4
+        self.h = L.cpu_fast_create(self.N, self.C, _p(self._keep[0]), _p(self._keep[1]), fc.size, _p(fc), _p(fl),
+                                   _p(t))
+        self.nnzb = L.cpu_fast_nnzb(self.h)
+        self.colors = L.cpu_fast_colors(self.h)
+        self.R = np.zeros(self.N * 3)
+        self.values = np.zeros(self.nnzb * 9)
+
+    def value_gradient_u(self, youngs, poisson, penalty_exponent, penalty_min, u, z):
+        u, z = np.ascontiguousarray(u, np.float64), np.ascontiguousarray(z, np.float64)
+        lib().cpu_fast_elastic(self.h, youngs, poisson, penalty_exponent, penalty_min, _p(u), _p(z), _p(self.R),
+                               _p(self.values))
+        return self.R, self.values
+
+    def __del__(self):
+        try:
+            lib().cpu_fast_destroy(self.h)
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------------------
