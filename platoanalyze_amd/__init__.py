@@ -361,6 +361,20 @@ class Mesh:
                     m.add_node_set(name + sign, np.nonzero(np.abs(X[:, axis] - val) < 1e-12)[0])
         return m
 
+    @classmethod
+    def from_exodus(cls, path):
+        """Mesh of a Tet4 Exodus II file with its side sets and node sets
+        (platoanalyze_amd.exodus; the reference reads meshes through Omega_h,
+        src/OmegaHMesh.cpp:13-27)."""
+        from . import exodus
+        em = exodus.read(path)
+        m = cls(em.coords, em.conn)
+        for name, (cells, ln) in em.side_sets.items():
+            m.add_side_set(name, cells, ln)
+        for name, nodes in em.node_sets.items():
+            m.add_node_set(name, nodes)
+        return m
+
     def assembly_info(self):
         """(cell tiles, distinct tile schedules, distinct row patterns) of the assembly kernels."""
         t, tp, rp = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

@@ -627,6 +627,7 @@ struct TileMesh {
     const int4* __restrict__ rows;
     const int4* __restrict__ runs;
     const int4* __restrict__ cells;
+    const uint2* __restrict__ cslots;
     const unsigned short* __restrict__ pairs;
     const uint2* __restrict__ units;
     const uint4* __restrict__ ents;
@@ -730,29 +731,47 @@ __device__ inline void tile_residual_rows(const TileMesh& t, const TileHead& h, 
     }
 }
 
+// Residual tile kernel, one cell per thread (kResThreads >= kTileCellsMax).
+// Phase 1 writes each cell's contributions to the tile's rows straight into
+// their pair slots (pair k of row r at r * 32 + k; contributions to nodes
+// outside the tile are dropped), so the LDS holds 32 * DPN doubles per row
+// instead of the cells' full residuals and more workgroups fit on a CU.
+// Phase 2 sums a row's slots with the 8-lane tree of the row kernels.
+constexpr int kResThreads = 512;
+static_assert(kTileCellsMax <= kResThreads && kTileX * kTileY * kTileZ * 8 <= kResThreads, "one cell per thread");
+
 template <int DPN>
-__global__ __launch_bounds__(kTileThreads) void k_tile_residual(TileMesh t, const double4* __restrict__ nrec,
-                                                                Material mt, const double* __restrict__ u,
-                                                                double body, double scale, double* __restrict__ R) {
-    constexpr int FS = 4 * DPN + 1;  // the four nodes' residuals, one pad
+__global__ __launch_bounds__(kResThreads) void k_tile_residual(TileMesh t, const double4* __restrict__ nrec,
+                                                               Material mt, const double* __restrict__ u,
+                                                               double body, double scale, double* __restrict__ R) {
     extern __shared__ double lds[];
     const TileHead h = tile_head(t);
-    TileRowFetch rf;
-    if ((int)threadIdx.x < h.nrows * 8) tile_row_fetch(t, h, threadIdx.x, rf);
-    tile_cells(t, nrec, h, [&](int c, const CellEval& ce) {
-        const double s = penal(ce.rho, mt.p, mt.emin);
-        const double w = s * (ce.absdet / 6.0);
-        double* F = lds + c * FS;
+    // schedule words fetched ahead: this lane's row of phase 2 and its cell's slots
+    const int q = threadIdx.x, r = q >> 3, s = q & 7;
+    int4 ri = make_int4(0, 0, 0, 0);
+    if (r < h.nrows) ri = t.rows[h.row_off + r];
+    const int c = threadIdx.x;
+    if (c < h.ncells) {
+        const uint2 sl = t.cslots[h.cell_off + c];
+        CellEval ce;
+        double4 x[4];
+        tile_cell_load(t, nrec, h, c, ce, x);
+        tile_cell_geometry(ce, x);
+        const unsigned slot[4] = {sl.x & 0xffffu, sl.x >> 16, sl.y & 0xffffu, sl.y >> 16};
+        const double sp = penal(ce.rho, mt.p, mt.emin);
+        const double w = sp * (ce.absdet / 6.0);
         if constexpr (DPN == 3) {
             double eps[6], sig[6], out[3];
             cell_strain(ce, u, eps);
             iso_stress(mt, eps, sig);
 #pragma unroll
             for (int a = 0; a < 4; ++a) {
+                if (slot[a] == 0xffffu) continue;
                 btsig(sig, ce.g[a], out);
-                F[3 * a + 0] = w * out[0];
-                F[3 * a + 1] = w * out[1];
-                F[3 * a + 2] = w * out[2];
+                double* F = lds + slot[a] * 3;
+                F[0] = w * out[0];
+                F[1] = w * out[1];
+                F[2] = w * out[2];
             }
         } else {
             double gT[3] = {0.0, 0.0, 0.0};
@@ -766,18 +785,33 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_residual(TileMesh t, cons
             const double load = -1.0 * (ce.absdet / 6.0) * body * 0.25 * ce.rho;
 #pragma unroll
             for (int a = 0; a < 4; ++a) {
-                double r = w * mt.k * (gT[0] * ce.g[a][0] + gT[1] * ce.g[a][1] + gT[2] * ce.g[a][2]);
-                r += load;
-                F[a] = r;
+                if (slot[a] == 0xffffu) continue;
+                double v = w * mt.k * (gT[0] * ce.g[a][0] + gT[1] * ce.g[a][1] + gT[2] * ce.g[a][2]);
+                v += load;
+                lds[slot[a]] = v;
             }
         }
-    });
+    }
     __syncthreads();
-    tile_residual_rows<DPN>(t, h, threadIdx.x, rf, scale, R, [&](unsigned e, double* v) {
-        const double* F = lds + (e >> 2) * FS + (e & 3) * DPN;
+    if (r >= h.nrows) return;
+    // lane s: pairs s, s + 16, s + 8, s + 24 (the 32-lane butterfly's tree)
+    const int ps[4] = {s, s + 16, s + 8, s + 24};
+    double v[4][DPN];
 #pragma unroll
-        for (int i = 0; i < DPN; ++i) v[i] = F[i];
-    });
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int i = 0; i < DPN; ++i) v[j][i] = ps[j] < ri.y ? lds[(r * 32 + ps[j]) * DPN + i] : 0.0;
+    double b[DPN];
+#pragma unroll
+    for (int i = 0; i < DPN; ++i) {
+        b[i] = (v[0][i] + v[1][i]) + (v[2][i] + v[3][i]);
+#pragma unroll
+        for (int o = 4; o > 0; o >>= 1) b[i] += __shfl_xor(b[i], o, 8);
+    }
+    if (s == 0) {
+#pragma unroll
+        for (int i = 0; i < DPN; ++i) R[(long)(h.base + ri.x) * DPN + i] = scale * b[i];
+    }
 }
 
 // One work unit (contributions to one block, or to a part of a diagonal
@@ -1354,6 +1388,7 @@ TileMesh tile_mesh(const pa_mesh* m) {
     t.rows = m->d_tp_row.p;
     t.runs = m->d_tp_run.p;
     t.cells = m->d_tp_cell.p;
+    t.cslots = m->d_tp_cslot.p;
     t.pairs = m->d_tp_pair.p;
     t.units = m->d_tp_unit.p;
     t.ents = reinterpret_cast<const uint4*>(m->d_tp_ent.p);
@@ -1441,15 +1476,15 @@ const double4* pack_nodes(const pa_mesh* m, const double* z, hipStream_t st) {
 // the node degree allows, else 8 lanes per row looping over the cells.
 void residual_rows(const pa_mesh* m, int dpn, const Material& mt, const double* u, const double* z, double body,
                    double scale, double* R, hipStream_t st) {
-    if (m->has_tiles) {
-        const size_t lds = (size_t)m->tile_cells_max * (4 * dpn + 1) * sizeof(double);
+    if (m->has_tiles && m->tile_rows_max * 8 <= kResThreads) {
+        const size_t lds = (size_t)m->tile_rows_max * 32 * dpn * sizeof(double);
         const size_t ok = dpn == 3 ? tile_lds(k_tile_residual<3>, lds) : tile_lds(k_tile_residual<1>, lds);
         if (ok) {
             const double4* rec = pack_nodes(m, z, st);
             if (dpn == 3)
-                k_tile_residual<3><<<m->num_tiles, kTileThreads, lds, st>>>(tile_mesh(m), rec, mt, u, body, scale, R);
+                k_tile_residual<3><<<m->num_tiles, kResThreads, lds, st>>>(tile_mesh(m), rec, mt, u, body, scale, R);
             else
-                k_tile_residual<1><<<m->num_tiles, kTileThreads, lds, st>>>(tile_mesh(m), rec, mt, u, body, scale, R);
+                k_tile_residual<1><<<m->num_tiles, kResThreads, lds, st>>>(tile_mesh(m), rec, mt, u, body, scale, R);
             check_launch();
             return;
         }

@@ -180,6 +180,7 @@ static bool detect_grid(const double* xyz, int N, int d[3]) {
 struct TileSchedule {
     std::vector<int4> rows, cells, runs;
     std::vector<unsigned short> pairs;
+    std::vector<uint2> cslots;  // per cell: LDS pair slot (row * 32 + k) of its 4 nodes, 0xffff if not a row
     std::vector<uint2> units;
     std::vector<unsigned> ents;
     int max_count = 0;
@@ -268,6 +269,18 @@ static bool tile_schedule(const pa_mesh* m, const std::vector<int>& conn, const 
         if (a.diag) return false;
         return a.ent.size() > b.ent.size();
     });
+    // pair slots of the residual kernel: pair k of row ri lives at ri * 32 + k
+    {
+        std::vector<unsigned short> cs(4 * cells.size(), 0xffff);
+        for (int ri = 0; ri < (int)t.rows.size(); ++ri)
+            for (int k = 0; k < t.rows[ri].y; ++k) {
+                const unsigned pe = t.pairs[t.rows[ri].z + k];
+                cs[4 * (pe >> 2) + (pe & 3)] = (unsigned short)(ri * 32 + k);
+            }
+        for (size_t c = 0; c < cells.size(); ++c)
+            t.cslots.push_back(make_uint2((unsigned)cs[4 * c] | (unsigned)cs[4 * c + 1] << 16,
+                                          (unsigned)cs[4 * c + 2] | (unsigned)cs[4 * c + 3] << 16));
+    }
     for (const Unit& u : units) t.max_count = std::max(t.max_count, (int)u.ent.size());
     if (t.max_count > kTileUnitEntries) return false;
     // shape of each wave (aligned group of 64 units): 1 all diagonal parts of
@@ -324,6 +337,7 @@ static void build_tiles(pa_mesh* m, const double* xyz, const std::vector<int>& c
     };
     std::unordered_map<std::string, int> index;
     std::vector<int4> tiles(3 * nt), info, trow, tcell, trun;
+    std::vector<uint2> tcslot;
     std::vector<unsigned short> tpair;
     std::vector<uint2> tunit;
     std::vector<unsigned> tent;
@@ -368,6 +382,7 @@ static void build_tiles(pa_mesh* m, const double* xyz, const std::vector<int>& c
                 const int poff = (int)tpair.size();
                 for (auto r : s.rows) trow.push_back(make_int4(r.x, r.y, r.z + poff, r.w));
                 tcell.insert(tcell.end(), s.cells.begin(), s.cells.end());
+                tcslot.insert(tcslot.end(), s.cslots.begin(), s.cslots.end());
                 tpair.insert(tpair.end(), s.pairs.begin(), s.pairs.end());
                 tunit.insert(tunit.end(), s.units.begin(), s.units.end());
                 tent.insert(tent.end(), s.ents.begin(), s.ents.end());
@@ -391,6 +406,7 @@ static void build_tiles(pa_mesh* m, const double* xyz, const std::vector<int>& c
     m->d_tile.upload(tiles.data(), tiles.size());
     m->d_tp_row.upload(trow.data(), trow.size());
     m->d_tp_cell.upload(tcell.data(), tcell.size());
+    m->d_tp_cslot.upload(tcslot.data(), tcslot.size());
     m->d_tp_run.upload(trun.data(), trun.size());
     m->d_tp_pair.upload(tpair.data(), tpair.size());
     m->d_tp_unit.upload(tunit.data(), tunit.size());

@@ -144,6 +144,8 @@ struct pa_mesh {
     pa::DevBuf<int4> d_tp_run;        // per pattern output run (rows with consecutive ids): {first row index,
                                       // staging offset, blocks, 0}
     pa::DevBuf<int4> d_tp_cell;       // per pattern cell: its four node ids - first row
+    pa::DevBuf<uint2> d_tp_cslot;     // per pattern cell: residual pair slot (row * 32 + k) of each node
+                                      // (16 bits each, 0xffff for a node that is not a tile row)
     pa::DevBuf<unsigned short> d_tp_pair;  // per (row, incident cell): tile cell << 2 | local index of the row node
     pa::DevBuf<uint2> d_tp_unit;      // (row, column slot) work units: {row | slot << 8 | mirror row << 16 |
                                       // mirror slot << 24, entry offset | count << 22 | diagonal << 26 |
