@@ -39,7 +39,9 @@
 #include <algorithm>
 #include <cctype>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <memory>
 #include <sstream>
@@ -584,6 +586,264 @@ inline std::shared_ptr<Mesh> getBoxMesh(int nx, int ny, int nz, double lx = 1.0,
             tMesh->addSideSet(tName, tCells, tLocal);
             tMesh->addNodeSet(tName, tNodes);
         }
+    return tMesh;
+}
+
+namespace detail {
+
+// Non-record variables of a netCDF classic (CDF-1) or 64-bit-offset (CDF-2)
+// file, the container of Exodus II meshes (the netCDF classic format
+// specification: big-endian header of dimensions, attributes and variables,
+// each variable's data contiguous at its `begin` offset).
+class NetcdfClassic {
+  public:
+    struct Var {
+        std::vector<int> dims;
+        int type = 0;  // 1 byte, 2 char, 3 short, 4 int, 5 float, 6 double
+        unsigned long long begin = 0;
+        std::map<std::string, std::string> text_attrs;
+    };
+    explicit NetcdfClassic(const std::string& aPath) : mPath(aPath) {
+        std::FILE* f = std::fopen(aPath.c_str(), "rb");
+        if (!f) throw_error("netCDF: cannot open '" + aPath + "'");
+        std::fseek(f, 0, SEEK_END);
+        mBytes.resize((size_t)std::ftell(f));
+        std::fseek(f, 0, SEEK_SET);
+        const size_t n = std::fread(mBytes.data(), 1, mBytes.size(), f);
+        std::fclose(f);
+        if (n != mBytes.size() || mBytes.size() < 8 || mBytes[0] != 'C' || mBytes[1] != 'D' || mBytes[2] != 'F')
+            throw_error("netCDF: '" + aPath + "' is not a netCDF classic file (netCDF-4/HDF5 files are not read here)");
+        const int version = mBytes[3];
+        if (version != 1 && version != 2) throw_error("netCDF: '" + aPath + "': unsupported format version");
+        mOffset64 = version == 2;
+        size_t p = 4;
+        u32(p);  // numrecs
+        // dimensions
+        if (u32(p) == 0x0A) {
+            const unsigned nd = u32(p);
+            for (unsigned i = 0; i < nd; ++i) {
+                const std::string name = text(p);
+                mDimNames.push_back(name);
+                mDims.push_back(u32(p));
+            }
+        } else {
+            u32(p);
+        }
+        skip_attrs(p, nullptr);
+        // variables
+        if (u32(p) == 0x0B) {
+            const unsigned nv = u32(p);
+            for (unsigned i = 0; i < nv; ++i) {
+                const std::string name = text(p);
+                Var v;
+                const unsigned nd = u32(p);
+                for (unsigned k = 0; k < nd; ++k) v.dims.push_back((int)u32(p));
+                skip_attrs(p, &v.text_attrs);
+                v.type = (int)u32(p);
+                u32(p);  // vsize
+                v.begin = mOffset64 ? u64(p) : u32(p);
+                mVars[name] = v;
+            }
+        } else {
+            u32(p);
+        }
+    }
+    long long dim(const std::string& aName, long long aDefault = -1) const {
+        for (size_t i = 0; i < mDimNames.size(); ++i)
+            if (mDimNames[i] == aName) return mDims[i];
+        if (aDefault < 0) throw_error("netCDF: '" + mPath + "' has no dimension '" + aName + "'");
+        return aDefault;
+    }
+    bool has(const std::string& aName) const { return mVars.count(aName) != 0; }
+    const Var& var(const std::string& aName) const {
+        auto it = mVars.find(aName);
+        if (it == mVars.end()) throw_error("netCDF: '" + mPath + "' has no variable '" + aName + "'");
+        return it->second;
+    }
+    std::vector<long long> shape(const std::string& aName) const {
+        std::vector<long long> s;
+        for (int d : var(aName).dims) {
+            if (mDims[d] == 0) throw_error("netCDF: record variable '" + aName + "' is not supported");
+            s.push_back(mDims[d]);
+        }
+        return s;
+    }
+    // numeric variable as doubles (int, short, byte, float or double)
+    std::vector<double> numbers(const std::string& aName) const {
+        const Var& v = var(aName);
+        size_t n = 1;
+        for (long long s : shape(aName)) n *= (size_t)s;
+        const int w = v.type == 6 ? 8 : (v.type == 4 || v.type == 5) ? 4 : v.type == 3 ? 2 : 1;
+        if (v.type == 2) throw_error("netCDF: variable '" + aName + "' is text");
+        if (v.begin + (unsigned long long)n * w > mBytes.size()) throw_error("netCDF: variable '" + aName + "' truncated");
+        std::vector<double> out(n);
+        size_t p = (size_t)v.begin;
+        for (size_t i = 0; i < n; ++i) {
+            if (v.type == 6) {
+                const unsigned long long b = u64(p);
+                double d;
+                std::memcpy(&d, &b, 8);
+                out[i] = d;
+            } else if (v.type == 5) {
+                const unsigned b = u32(p);
+                float f;
+                std::memcpy(&f, &b, 4);
+                out[i] = f;
+            } else if (v.type == 4) {
+                out[i] = (double)(int)u32(p);
+            } else if (v.type == 3) {
+                const int x = (mBytes[p] << 8) | mBytes[p + 1];
+                p += 2;
+                out[i] = (double)(short)x;
+            } else {
+                out[i] = (double)(signed char)mBytes[p++];
+            }
+        }
+        return out;
+    }
+    // rows of a char variable (rows x len), each cut at its first NUL
+    std::vector<std::string> strings(const std::string& aName) const {
+        const Var& v = var(aName);
+        const auto s = shape(aName);
+        if (v.type != 2 || s.size() != 2) throw_error("netCDF: '" + aName + "' is not a text array");
+        std::vector<std::string> out;
+        for (long long r = 0; r < s[0]; ++r) {
+            const char* c = reinterpret_cast<const char*>(mBytes.data() + v.begin + (size_t)(r * s[1]));
+            std::string t(c, (size_t)s[1]);
+            t = t.substr(0, t.find('\0'));
+            while (!t.empty() && std::isspace((unsigned char)t.back())) t.pop_back();
+            out.push_back(t);
+        }
+        return out;
+    }
+
+  private:
+    unsigned u32(size_t& p) const {
+        if (p + 4 > mBytes.size()) throw_error("netCDF: '" + mPath + "' truncated header");
+        const unsigned v = (unsigned)mBytes[p] << 24 | (unsigned)mBytes[p + 1] << 16 | (unsigned)mBytes[p + 2] << 8 |
+                           (unsigned)mBytes[p + 3];
+        p += 4;
+        return v;
+    }
+    unsigned long long u64(size_t& p) const {
+        const unsigned long long hi = u32(p);
+        return hi << 32 | u32(p);
+    }
+    std::string text(size_t& p) const {
+        const unsigned n = u32(p);
+        if (p + n > mBytes.size()) throw_error("netCDF: '" + mPath + "' truncated name");
+        std::string s(reinterpret_cast<const char*>(mBytes.data() + p), n);
+        p += (n + 3) / 4 * 4;
+        return s;
+    }
+    void skip_attrs(size_t& p, std::map<std::string, std::string>* aText) const {
+        if (u32(p) != 0x0C) {
+            u32(p);
+            return;
+        }
+        const unsigned na = u32(p);
+        for (unsigned i = 0; i < na; ++i) {
+            const std::string name = text(p);
+            const unsigned type = u32(p), n = u32(p);
+            const unsigned w = type == 6 ? 8 : (type == 4 || type == 5) ? 4 : type == 3 ? 2 : 1;
+            if (type == 2 && aText) {
+                std::string v(reinterpret_cast<const char*>(mBytes.data() + p), n);
+                (*aText)[name] = v.substr(0, v.find('\0'));
+            }
+            p += ((size_t)n * w + 3) / 4 * 4;
+        }
+    }
+    std::string mPath;
+    std::vector<unsigned char> mBytes;
+    bool mOffset64 = false;
+    std::vector<std::string> mDimNames;
+    std::vector<long long> mDims;
+    std::map<std::string, Var> mVars;
+};
+
+}  // namespace detail
+
+// Tet4 mesh of an Exodus II file (netCDF classic or 64-bit offset) with its
+// side sets and node sets, as the reference reads through Omega_h
+// (src/OmegaHMesh.cpp:13-27): coordx/y/z (or coord), connect<b> of TETRA4
+// blocks (1-based, blocks in order), elem_ss<i>/side_ss<i> with the TETRA
+// side numbering 1 = (1,2,4), 2 = (2,3,4), 3 = (1,4,3), 4 = (1,3,2), node_ns<i>,
+// and the names in ss_names / ns_names (the id from ss_prop1 / ns_prop1 when a
+// name is empty).
+inline std::shared_ptr<Mesh> readExodusMesh(const std::string& aPath) {
+    detail::NetcdfClassic nc(aPath);
+    if (nc.dim("num_dim", 3) != 3) throw_error("readExodusMesh: only three-dimensional meshes are on this path");
+    const long long N = nc.dim("num_nodes");
+    std::vector<double> tCoords((size_t)N * 3);
+    if (nc.has("coord")) {
+        const auto c = nc.numbers("coord");
+        for (long long n = 0; n < N; ++n)
+            for (int d = 0; d < 3; ++d) tCoords[(size_t)n * 3 + d] = c[(size_t)d * N + n];
+    } else {
+        const char* names[3] = {"coordx", "coordy", "coordz"};
+        for (int d = 0; d < 3; ++d) {
+            const auto c = nc.numbers(names[d]);
+            for (long long n = 0; n < N; ++n) tCoords[(size_t)n * 3 + d] = c[(size_t)n];
+        }
+    }
+    std::vector<int> tConn;
+    const long long nblk = nc.dim("num_el_blk", 0);
+    for (long long b = 1; b <= nblk; ++b) {
+        const std::string v = "connect" + std::to_string(b);
+        if (!nc.has(v)) continue;
+        std::string etype = nc.var(v).text_attrs.count("elem_type") ? nc.var(v).text_attrs.at("elem_type") : "TETRA4";
+        for (auto& ch : etype) ch = (char)std::toupper((unsigned char)ch);
+        const auto s = nc.shape(v);
+        if ((etype != "TETRA" && etype != "TETRA4" && etype != "TET4" && etype != "TET") || s.size() != 2 || s[1] != 4)
+            throw_error("readExodusMesh: element block " + std::to_string(b) + " is " + etype +
+                        "; only TETRA4 is on this path");
+        for (double x : nc.numbers(v)) {
+            const long long n = (long long)x - 1;
+            if (n < 0 || n >= N) throw_error("readExodusMesh: connectivity out of range");
+            tConn.push_back((int)n);
+        }
+    }
+    auto tMesh = std::make_shared<Mesh>(tCoords, tConn);
+    const long long C = (long long)tConn.size() / 4;
+    auto set_names = [&](const std::string& aNames, const std::string& aIds, long long aCount) {
+        std::vector<std::string> out(aCount);
+        const auto names = nc.has(aNames) ? nc.strings(aNames) : std::vector<std::string>();
+        const auto ids = nc.has(aIds) ? nc.numbers(aIds) : std::vector<double>();
+        for (long long i = 0; i < aCount; ++i) {
+            out[i] = i < (long long)names.size() ? names[i] : "";
+            if (out[i].empty()) out[i] = std::to_string(i < (long long)ids.size() ? (long long)ids[i] : i + 1);
+        }
+        return out;
+    };
+    static const int kSides[4][3] = {{0, 1, 3}, {1, 2, 3}, {0, 3, 2}, {0, 2, 1}};
+    const long long nss = nc.dim("num_side_sets", 0);
+    const auto ss = set_names("ss_names", "ss_prop1", nss);
+    for (long long i = 1; i <= nss; ++i) {
+        const std::string ev = "elem_ss" + std::to_string(i), sv = "side_ss" + std::to_string(i);
+        if (!nc.has(ev)) continue;
+        const auto el = nc.numbers(ev), sd = nc.numbers(sv);
+        std::vector<int> cells, local;
+        for (size_t k = 0; k < el.size(); ++k) {
+            const long long c = (long long)el[k] - 1, s = (long long)sd[k] - 1;
+            if (c < 0 || c >= C || s < 0 || s > 3) throw_error("readExodusMesh: side set '" + ss[i - 1] + "' out of range");
+            cells.push_back((int)c);
+            local.insert(local.end(), kSides[s], kSides[s] + 3);
+        }
+        tMesh->addSideSet(ss[i - 1], cells, local);
+    }
+    const long long nns = nc.dim("num_node_sets", 0);
+    const auto ns = set_names("ns_names", "ns_prop1", nns);
+    for (long long i = 1; i <= nns; ++i) {
+        const std::string nv = "node_ns" + std::to_string(i);
+        if (!nc.has(nv)) continue;
+        std::vector<int> nodes;
+        for (double x : nc.numbers(nv)) {
+            const long long n = (long long)x - 1;
+            if (n < 0 || n >= N) throw_error("readExodusMesh: node set '" + ns[i - 1] + "' out of range");
+            nodes.push_back((int)n);
+        }
+        tMesh->addNodeSet(ns[i - 1], nodes);
+    }
     return tMesh;
 }
 

@@ -8,6 +8,8 @@
 // elliptic compliance Problem and the J2 local/global worksets.
 //
 // Usage: reference_cases <case>|all   prints one JSON object per case
+//        (ExodusMesh reads the mesh file named by PA_EXODUS_MESH and is not
+//        part of all)
 //        reference_cases --list       case names (no device use)
 //        reference_cases --parser     checks of the parameter-list reader and
 //                                     the expression evaluator (no device use)
@@ -15,6 +17,7 @@
 // by the Python tests that run this program.
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <iostream>
@@ -481,6 +484,35 @@ std::string j2_worksets() {
     return j.str();
 }
 
+// the compliance deck on a mesh read from an Exodus II file (path in
+// PA_EXODUS_MESH): value, gradient_u and the solved state
+std::string exodus_problem() {
+    const char* path = std::getenv("PA_EXODUS_MESH");
+    if (!path) throw std::runtime_error("PA_EXODUS_MESH is not set");
+    auto tParams = pa::getParametersFromXmlString(kComplianceDeck);
+    auto tMesh = pa::readExodusMesh(path);
+    const int N = tMesh->NumNodes();
+    std::vector<double> zh(N), uh(3 * N);
+    for (int i = 0; i < N; ++i) zh[i] = 0.3 + 0.7 * (double)((i * 7919) % 101) / 100.0;
+    for (int i = 0; i < 3 * N; ++i) uh[i] = 1e-3 * (double)((i * 104729) % 211 - 105);
+    auto z = pa::toDevice(zh, "control");
+    auto u = pa::toDevice(uh, "state");
+    pa::Elliptic::VectorFunction<pa::Mechanics<3>> tVf(tMesh, *tParams, "Elliptic");
+    pa::Elliptic::Problem<pa::Mechanics<3>> tProblem(tMesh, *tParams);
+    auto tSolution = tProblem.solution(z);
+    Json j;
+    j.put("num_nodes", (double)N);
+    j.put("num_cells", (double)tMesh->NumElements());
+    j.put("x_minus", tMesh->getNodeSetNodes("x-"));
+    j.put("z", zh);
+    j.put("u", uh);
+    j.put("residual", pa::toHost(tVf.value(u, z)));
+    j.put("jacobian", pa::toHost(tVf.gradient_u(u, z)->entries()));
+    j.put("state", pa::toHost(tSolution.get("State")));
+    j.put("compliance", tProblem.criterionValue(z, tSolution, "Compliance"));
+    return j.str();
+}
+
 // ---------------------------------------------------------------- parser
 int parser_checks() {
     int failures = 0;
@@ -551,6 +583,7 @@ int main(int argc, char** argv) {
         {"ElastoPlasticity_ElasticSolution3D", elastic_solution_j2},
         {"ElastoPlasticity_CriterionTest_3D", criterion_test_j2},
         {"J2Worksets", j2_worksets},
+        {"ExodusMesh", exodus_problem},
     };
     if (argc < 2) {
         std::fprintf(stderr, "usage: %s <case>|all|--list|--parser\n", argv[0]);
@@ -564,7 +597,7 @@ int main(int argc, char** argv) {
     if (what == "--parser") return parser_checks();
     int ran = 0;
     for (const auto& c : cases) {
-        if (what != "all" && what != c.first) continue;
+        if (what != c.first && (what != "all" || c.first == "ExodusMesh")) continue;
         ++ran;
         std::fprintf(stderr, "case %s\n", c.first.c_str());
         try {

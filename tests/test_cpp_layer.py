@@ -10,7 +10,8 @@ HERE = Path(__file__).resolve().parent
 BIN = HERE / "cpp" / "reference_cases"
 
 CASES = ["ElastostaticResidual3D", "InternalElasticEnergy3D", "ThermostaticResidual3D", "InternalThermalEnergy3D",
-         "Compliance", "ElastoPlasticity_ElasticSolution3D", "ElastoPlasticity_CriterionTest_3D", "J2Worksets"]
+         "Compliance", "ElastoPlasticity_ElasticSolution3D", "ElastoPlasticity_CriterionTest_3D", "J2Worksets",
+         "ExodusMesh"]
 
 
 @pytest.fixture(scope="module")
