@@ -454,6 +454,76 @@ int pa_plasticity_problem_criterion_value(pa_plasticity_problem* problem, const 
 int pa_plasticity_problem_criterion_gradient(pa_plasticity_problem* problem, const double* z_dev,
                                              double multiplier, double* grad_dev, void* stream);
 
+/* ------------------------------------------------- warm-start network */
+/*
+ * Kernels of the learned warm start (no counterpart in the reference): a small
+ * 3D U-Net on the voxelised density field predicts the displacement field that
+ * starts the linear solve.  BF16 activations/weights (raw uint16 bits),
+ * FP32 accumulation, FP32 master weights.  A tensor is a channels-last view
+ * (pointer, stride): element (voxel v, channel c) at ptr[v * stride + c], voxels
+ * ordered (d, h, w) with w fastest.  Channel counts of convolutions are
+ * multiples of 16.
+ */
+/* 3x3x3 convolution, zero padding: y = act(bias + conv(x, W)), bf16 and/or FP32
+ * output (y_f32 may be null, y may be null when y_f32 is given); accumulate adds
+ * the existing bf16 y before the activation.  wfrag: weights packed by
+ * pa_cnn_pack_weights; bias may be null; relu 0/1.  D*H*W must be a multiple of 16. */
+int pa_cnn_conv3d(int D, int H, int W, const void* x_dev, int x_stride, int cin, const void* wfrag_dev,
+                  const float* bias_dev, int cout, void* y_dev, int y_stride, int relu, float* y_f32_dev,
+                  int yf_stride, int accumulate, void* stream);
+/* Pack FP32 weights W[27][cin][cout] (tap t = (dz+1)*9 + (dy+1)*3 + dx+1) into
+ * the convolution's bf16 fragments; transposed != 0 packs the data-gradient
+ * weights W'[t][cout][cin] = W[26 - t][cin][cout] (a convolution with cin and
+ * cout exchanged).  wfrag: 27*cin*cout bf16. */
+int pa_cnn_pack_weights(const float* w_dev, int cin, int cout, int transposed, void* wfrag_dev, void* stream);
+/* Weight-gradient partials parts[chunk][27][cin][cout] = sum over the chunk's
+ * voxels of x[nb(v, t)][ci] dy[v][co]; sum them with pa_cnn_sum_parts. */
+int pa_cnn_conv3d_wgrad_parts(int D, int H, int W, const void* x_dev, int x_stride, int cin, const void* dy_dev,
+                              int dy_stride, int cout, long long chunk, float* parts_dev, void* stream);
+/* dst[k] = sum_p parts[p*n + k] (fixed order). */
+int pa_cnn_sum_parts(long long n, int nparts, const float* parts_dev, float* dst_dev, void* stream);
+/* Bias-gradient partials parts[block][c] over voxel chunks. */
+int pa_cnn_bias_parts(long long V, const void* dy_dev, int dy_stride, int c, long long chunk, float* parts_dev,
+                      void* stream);
+/* 2x2x2 average pooling of a (D, H, W) tensor and its gradient (accumulate adds). */
+int pa_cnn_pool(int D, int H, int W, const void* x_dev, int x_stride, int c, void* y_dev, int y_stride, void* stream);
+int pa_cnn_pool_bwd(int D, int H, int W, const void* dy_dev, int dy_stride, int c, void* dx_dev, int dx_stride,
+                    int accumulate, void* stream);
+/* Nearest 2x upsampling to the fine grid (D, H, W), and its gradient onto the coarse grid (D, H, W). */
+int pa_cnn_upsample(int D, int H, int W, const void* x_dev, int x_stride, int c, void* y_dev, int y_stride,
+                    void* stream);
+int pa_cnn_upsample_bwd(int D, int H, int W, const void* dy_dev, int dy_stride, int c, void* dx_dev, int dx_stride,
+                        void* stream);
+/* dy = y > 0 ? dy : 0 (the ReLU gradient, y the activation). */
+int pa_cnn_relu_bwd(long long V, int c, const void* y_dev, int y_stride, void* dy_dev, int dy_stride, void* stream);
+/* Weighted mean squared error of the first nc of c FP32 output channels against
+ * target (V x nc), weight rho^3 + wmin with rho channel 0 of the bf16 input x
+ * (the voxel density; x null: weight 1): gradient (bf16, c channels) and
+ * per-256-voxel partial sums of the weighted squared error. */
+int pa_cnn_mse(long long V, int nc, int c, const float* out_dev, int out_stride, const float* target_dev,
+               const void* x_dev, int x_stride, float wmin, void* dout_dev, int dout_stride, float* part_dev,
+               void* stream);
+/* Energy-norm proxy loss of e = out - target over the first nc channels:
+ * sum_v w_v (l2 |e_v|^2 + sum over the +x/+y/+z voxel neighbours u of |e_u - e_v|^2) / (V nc),
+ * w = rho^3 + wmin from the input's density channel; gradient (bf16) and per-256-voxel partial sums. */
+int pa_cnn_energy_loss(int D, int H, int W, int nc, int c, const float* out_dev, int out_stride,
+                       const float* target_dev, const void* x_dev, int x_stride, float wmin, float l2, void* dout_dev,
+                       int dout_stride, float* part_dev, void* stream);
+/* One Adam step on n FP32 parameters (lr_t includes the bias correction). */
+int pa_cnn_adam(long long n, float lr_t, float beta1, float beta2, float eps, const float* grad_dev, float* w_dev,
+                float* m1_dev, float* m2_dev, void* stream);
+/* FP32 view -> bf16 view; bf16 y += bf16 x. */
+int pa_cnn_to_bf16(long long V, int c, const float* x_dev, int x_stride, void* y_dev, int y_stride, void* stream);
+int pa_cnn_add_bf16(long long V, int c, const void* x_dev, int x_stride, void* y_dev, int y_stride, void* stream);
+/* Box-mesh node field (N x nc FP64, N = (D+1)(H+1)(W+1), the pa_box_mesh node
+ * order) to the hex voxels (mean of 8 nodes) times scale, clipped to
+ * [-clip, clip] when clip > 0, FP32, written at channel offset oc of a view
+ * with stride os; and voxels back to nodes (mean of the adjacent voxels). */
+int pa_cnn_nodes_to_voxels(int D, int H, int W, int nc, const double* nodes_dev, double scale, double clip,
+                           float* out_dev, int out_stride, int channel_offset, void* stream);
+int pa_cnn_voxels_to_nodes(int D, int H, int W, int nc, const float* vox_dev, int vox_stride, double scale,
+                           double* nodes_dev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -89,6 +89,7 @@ def _load():
         raise ImportError(f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`")
     L = ctypes.CDLL(str(LIB_PATH))
     vp, i, d, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_size_t
+    ll, f = ctypes.c_longlong, ctypes.c_float
     ip = ctypes.POINTER(ctypes.c_int)
     sig = {
         "pa_last_error": ([], ctypes.c_char_p),
@@ -173,6 +174,24 @@ def _load():
         "pa_plasticity_problem_stats": ([vp, i, ip, ip, ctypes.POINTER(d)], i),
         "pa_plasticity_problem_criterion_value": ([vp, vp, d, ctypes.POINTER(d), vp], i),
         "pa_plasticity_problem_criterion_gradient": ([vp, vp, d, vp, vp], i),
+        # warm-start network kernels
+        "pa_cnn_conv3d": ([i, i, i, vp, i, i, vp, vp, i, vp, i, i, vp, i, i, vp], i),
+        "pa_cnn_pack_weights": ([vp, i, i, i, vp, vp], i),
+        "pa_cnn_conv3d_wgrad_parts": ([i, i, i, vp, i, i, vp, i, i, ll, vp, vp], i),
+        "pa_cnn_sum_parts": ([ll, i, vp, vp, vp], i),
+        "pa_cnn_bias_parts": ([ll, vp, i, i, ll, vp, vp], i),
+        "pa_cnn_pool": ([i, i, i, vp, i, i, vp, i, vp], i),
+        "pa_cnn_pool_bwd": ([i, i, i, vp, i, i, vp, i, i, vp], i),
+        "pa_cnn_upsample": ([i, i, i, vp, i, i, vp, i, vp], i),
+        "pa_cnn_upsample_bwd": ([i, i, i, vp, i, i, vp, i, vp], i),
+        "pa_cnn_relu_bwd": ([ll, i, vp, i, vp, i, vp], i),
+        "pa_cnn_mse": ([ll, i, i, vp, i, vp, vp, i, f, vp, i, vp, vp], i),
+        "pa_cnn_adam": ([ll, f, f, f, f, vp, vp, vp, vp, vp], i),
+        "pa_cnn_energy_loss": ([i, i, i, i, i, vp, i, vp, vp, i, f, f, vp, i, vp, vp], i),
+        "pa_cnn_to_bf16": ([ll, i, vp, i, vp, i, vp], i),
+        "pa_cnn_add_bf16": ([ll, i, vp, i, vp, i, vp], i),
+        "pa_cnn_nodes_to_voxels": ([i, i, i, i, vp, d, d, vp, i, i, vp], i),
+        "pa_cnn_voxels_to_nodes": ([i, i, i, i, vp, i, d, vp, vp], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -565,6 +584,11 @@ class Solver:
         h = ctypes.c_void_p()
         _check(lib.pa_solver_create(mesh.handle, dpn, precond, ctypes.byref(h)))
         self.handle, self.mesh, self.dpn = h, mesh, dpn
+
+    def set_multigrid(self, mg):
+        """Precondition with a Multigrid (set up by the caller; kept alive here)."""
+        _check(lib.pa_solver_set_multigrid(self.handle, mg.handle))
+        self._mg = mg
 
     def solve(self, matrix, b, x, tol=1e-8, max_iter=10000, check_every=10, stream=None):
         st = SolveStats()
