@@ -150,6 +150,30 @@ J2_PARAMS = dict(youngs=1e3, poisson=0.3, hardening_isotropic=10.0, hardening_ki
 J2_END_DISPLACEMENT, J2_STEPS = 0.008, 10
 
 
+WARMSTART_WEIGHTS = ROOT / "platoanalyze_amd" / "data" / "warmstart_96x48x48.npz"
+
+
+def warmstart_leg(pa, fields=4):
+    """Configuration 5: the learned warm start on 96x48x48 hex x6 Tet4 (1,327,104
+    cells, 0.70M dof) GRF + Heaviside (beta 8) density fields not seen in training:
+    PCG iterations and seconds to ||r|| <= 1e-8 ||b|| from zero and from the
+    network's start, block-Jacobi and multigrid preconditioning (the weights are
+    the ones committed with the library, trained on 2,000 of the library's own
+    solves by python -m platoanalyze_amd.warmstart)."""
+    from platoanalyze_amd import warmstart as ws
+    if not WARMSTART_WEIGHTS.exists():
+        return {"skipped": f"no weights at {WARMSTART_WEIGHTS.relative_to(ROOT)}"}
+    dims = (96, 48, 48)
+    net, scale = ws.load(str(WARMSTART_WEIGHTS), dims)
+    box = ws.ElasticBox(dims)
+    feat = ws.Featurizer(dims)
+    recs = ws.evaluate(net, scale, box, feat, list(range(1, 1 + fields)), log=log)
+    out = {"workload": "SIMP elastostatics Tet4, 96x48x48 hex x6 (1,327,104 cells, 232,897 nodes, 698,691 dof), GRF + tanh "
+                       f"projection beta=8, {fields} unseen fields", "summary": ws.summarize(recs), "fields": recs}
+    log(f"bench: warm start {out['summary']}")
+    return out
+
+
 def thermal_bandwidth(pa, stream, reps=10):
     """Steady thermal 192^3 hex x 6 Tet4 (42.5M cells, 7.1M nodes; k = 1, SIMP
     p = 3, z ~ U[0.1, 1] seed 3, body source 1): the bandwidth-bound case.
@@ -307,6 +331,7 @@ def main():
     ap.add_argument("--j2-formulation", default="mixed", choices=["mixed", "displacement"],
                     help="J2 formulation of --mode j2")
     ap.add_argument("--no-thermal", action="store_true", help="skip the 192^3 thermal assembly leg")
+    ap.add_argument("--no-warmstart", action="store_true", help="skip the learned warm-start leg")
     ap.add_argument("--mode", default="bench", choices=["bench", "copy", "profile", "solve", "j2", "thermal"],
                     help="copy: run only the calibration copy kernel; profile: assembly kernels only; "
                          "solve: one multigrid compliance + gradient solve only; j2: the J2 plasticity leg only; "
@@ -444,6 +469,8 @@ def main():
         out["j2_plasticity"] = j2_plasticity(pa, stream, with_cpu=not args.no_cpu)
         out["j2_plasticity_displacement"] = j2_plasticity(pa, stream, with_cpu=not args.no_cpu,
                                                           formulation="displacement")
+    if not args.no_warmstart:
+        out["warmstart_96x48x48"] = warmstart_leg(pa)
     if not args.no_thermal:
         out["thermal_192"] = thermal_bandwidth(pa, stream)
     if not args.no_cpu:
