@@ -178,27 +178,37 @@ template <int D>
 __global__ __launch_bounds__(kT) void k_st_resid_jac(Grid g, const float* __restrict__ A, const float* __restrict__ dinv,
                                                      const float* __restrict__ b, const float* __restrict__ x,
                                                      float* __restrict__ out, float omega, int zero_x) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    // One 32-lane group per node: lane s < 27 owns stencil block s, i.e. the
+    // neighbour at offset (s/9 - 1, s/3 % 3 - 1, s % 3 - 1); the D partial
+    // products are then summed across the group.  A node's 27 blocks are
+    // contiguous, so the group reads its row of A in one coalesced sweep.
+    constexpr int G = 32, NODES = kT / G;
+    const int grp = threadIdx.x / G, s = threadIdx.x % G;
+    const int n = blockIdx.x * NODES + grp;
     if (n >= g.nodes()) return;
-    int i, j, k;
-    g.ijk(n, i, j, k);
     float acc[D];
 #pragma unroll
     for (int q = 0; q < D; ++q) acc[q] = 0.0f;
     if (!zero_x) {
-        const float* a = A + (long)n * 27 * D * D;
-        for (int dx = -1; dx <= 1; ++dx)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dz = -1; dz <= 1; ++dz) {
-                    if (!g.in(i + dx, j + dy, k + dz)) continue;
-                    const long c = g.id(i + dx, j + dy, k + dz);
-                    const float* v = a + sidx(dx, dy, dz) * D * D;
+        if (s < 27) {
+            int i, j, k;
+            g.ijk(n, i, j, k);
+            const int ii = i + s / 9 - 1, jj = j + (s / 3) % 3 - 1, kk = k + s % 3 - 1;
+            if (g.in(ii, jj, kk)) {
+                const long c = g.id(ii, jj, kk);
+                const float* v = A + ((long)n * 27 + s) * D * D;
 #pragma unroll
-                    for (int p = 0; p < D; ++p)
+                for (int p = 0; p < D; ++p)
 #pragma unroll
-                        for (int q = 0; q < D; ++q) acc[p] += v[p * D + q] * x[c * D + q];
-                }
+                    for (int q = 0; q < D; ++q) acc[p] += v[p * D + q] * x[c * D + q];
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < D; ++p)
+#pragma unroll
+            for (int o = G / 2; o > 0; o >>= 1) acc[p] += __shfl_xor(acc[p], o, G);
     }
+    if (s != 0) return;
     float r[D];
 #pragma unroll
     for (int p = 0; p < D; ++p) r[p] = b[(long)n * D + p] - acc[p];
@@ -206,10 +216,10 @@ __global__ __launch_bounds__(kT) void k_st_resid_jac(Grid g, const float* __rest
         const float* m = dinv + (long)n * D * D;
 #pragma unroll
         for (int p = 0; p < D; ++p) {
-            float s = 0.0f;
+            float t = 0.0f;
 #pragma unroll
-            for (int q = 0; q < D; ++q) s += m[p * D + q] * r[q];
-            out[(long)n * D + p] = (zero_x ? 0.0f : x[(long)n * D + p]) + omega * s;
+            for (int q = 0; q < D; ++q) t += m[p * D + q] * r[q];
+            out[(long)n * D + p] = (zero_x ? 0.0f : x[(long)n * D + p]) + omega * t;
         }
     } else {
 #pragma unroll
@@ -225,29 +235,38 @@ __device__ inline float w1(int f, int C) {
 }
 
 // b_c[J] = sum_i w_iJ mask_i r_f[i]
+// One 32-lane group per coarse node: lane t < 27 owns the fine vertex
+// 2J + (t/9 - 1, t/3 % 3 - 1, t % 3 - 1); the group sums the D components.
 template <int D>
-__global__ void k_restrict(Grid gf, Grid gc, const float* __restrict__ rf, const unsigned char* __restrict__ mask,
-                           float* __restrict__ bc) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kT) void k_restrict(Grid gf, Grid gc, const float* __restrict__ rf,
+                                                 const unsigned char* __restrict__ mask, float* __restrict__ bc) {
+    constexpr int G = 32, NODES = kT / G;
+    const int grp = threadIdx.x / G, t = threadIdx.x % G;
+    const int n = blockIdx.x * NODES + grp;
     if (n >= gc.nodes()) return;
-    int I, J, K;
-    gc.ijk(n, I, J, K);
     float acc[D];
 #pragma unroll
     for (int p = 0; p < D; ++p) acc[p] = 0.0f;
-    for (int dx = -1; dx <= 1; ++dx)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dz = -1; dz <= 1; ++dz) {
-                const int i = 2 * I + dx, j = 2 * J + dy, k = 2 * K + dz;
-                if (!gf.in(i, j, k)) continue;
-                const float w = w1(i, I) * w1(j, J) * w1(k, K);
-                const long f = gf.id(i, j, k);
+    if (t < 27) {
+        int I, J, K;
+        gc.ijk(n, I, J, K);
+        const int i = 2 * I + t / 9 - 1, j = 2 * J + (t / 3) % 3 - 1, k = 2 * K + t % 3 - 1;
+        if (gf.in(i, j, k)) {
+            const float w = w1(i, I) * w1(j, J) * w1(k, K);
+            const long f = gf.id(i, j, k);
 #pragma unroll
-                for (int p = 0; p < D; ++p)
-                    acc[p] += (mask == nullptr || mask[f * D + p]) ? w * rf[f * D + p] : 0.0f;
-            }
+            for (int p = 0; p < D; ++p)
+                acc[p] = (mask == nullptr || mask[f * D + p]) ? w * rf[f * D + p] : 0.0f;
+        }
+    }
 #pragma unroll
-    for (int p = 0; p < D; ++p) bc[(long)n * D + p] = acc[p];
+    for (int p = 0; p < D; ++p)
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) acc[p] += __shfl_xor(acc[p], o, G);
+    if (t == 0) {
+#pragma unroll
+        for (int p = 0; p < D; ++p) bc[(long)n * D + p] = acc[p];
+    }
 }
 
 // x_f[i] += mask_i sum_J w_iJ x_c[J]
@@ -519,7 +538,7 @@ void vcycle(pa_multigrid* mg, int l, hipStream_t st) {
                     g.nodes(), m->d_rowmap.p, m->d_colidx.p, (const float*)mg->fine_vals, mg->dinv[0]->p,
                     mg->mask.p, b, xin, xout, om, zero);
         } else {
-            k_st_resid_jac<D><<<nblk(g.nodes()), kT, 0, st>>>(g, mg->A[l]->p, mg->dinv[l]->p, b, xin, xout, om, zero);
+            k_st_resid_jac<D><<<nblk(g.nodes(), kT / 32), kT, 0, st>>>(g, mg->A[l]->p, mg->dinv[l]->p, b, xin, xout, om, zero);
         }
         chk();
     };
@@ -535,7 +554,7 @@ void vcycle(pa_multigrid* mg, int l, hipStream_t st) {
     // residual, restriction, coarse correction
     sweep(x, t, 0.0f, 0);
     const Grid gc = mg->grids[l + 1];
-    k_restrict<D><<<nblk(gc.nodes()), kT, 0, st>>>(g, gc, t, l == 0 ? mg->mask.p : nullptr, mg->b[l + 1]->p);
+    k_restrict<D><<<nblk(gc.nodes(), kT / 32), kT, 0, st>>>(g, gc, t, l == 0 ? mg->mask.p : nullptr, mg->b[l + 1]->p);
     chk();
     vcycle<D>(mg, l + 1, st);
     k_prolong_add<D><<<nblk(g.nodes()), kT, 0, st>>>(g, gc, mg->x[l + 1]->p, l == 0 ? mg->mask.p : nullptr, x);
