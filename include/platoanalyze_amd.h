@@ -165,6 +165,15 @@ int pa_vector_function_value_gradient_u(const pa_vector_function* vf, const doub
  */
 int pa_vector_function_gradient_z(const pa_vector_function* vf, const double* u_dev, const double* z_dev,
                                   double* values_dev, void* stream);
+/*
+ * VectorFunction::gradient_x (src/elliptic/VectorFunction.hpp:337-420): the
+ * transposed dR/dx (node coordinates) as a block matrix on the node graph with
+ * 3 x dpn blocks (row = node whose position moves, col = state node); entry
+ * [i*dpn + j] of block (a, b) is dR_{b,j}/dx_{a,i}.  Includes the natural
+ * BCs' face-area terms.  values [nnzb*3*dpn] doubles.
+ */
+int pa_vector_function_gradient_x(const pa_vector_function* vf, const double* u_dev, const double* z_dev,
+                                  double* values_dev, void* stream);
 
 /* ------------------------------------------------------- scalar function */
 /*
@@ -183,6 +192,9 @@ int pa_scalar_function_gradient_u(const pa_scalar_function* sf, const double* u_
                                   double* grad_dev, void* stream);
 /* PhysicsScalarFunction::gradient_z (src/elliptic/PhysicsScalarFunction.hpp:431-494): N doubles. */
 int pa_scalar_function_gradient_z(const pa_scalar_function* sf, const double* u_dev, const double* z_dev,
+                                  double* grad_dev, void* stream);
+/* PhysicsScalarFunction::gradient_x (src/elliptic/PhysicsScalarFunction.hpp:293-355): N*3 doubles. */
+int pa_scalar_function_gradient_x(const pa_scalar_function* sf, const double* u_dev, const double* z_dev,
                                   double* grad_dev, void* stream);
 
 /* ---------------------------------------------------------- constraints */
@@ -314,6 +326,13 @@ int pa_problem_criterion_value(pa_problem* problem, const double* z_dev, const d
  */
 int pa_problem_criterion_gradient(pa_problem* problem, const double* z_dev, const double* u_dev, double* grad_dev,
                                   void* stream);
+/*
+ * Criterion gradient wrt the node coordinates (N*3 doubles) of a self-adjoint
+ * problem: the reference's criterionGradientX returns -1 times the criterion's
+ * partial derivative there (src/elliptic/Problem.hpp:492-515).
+ */
+int pa_problem_criterion_gradient_x(pa_problem* problem, const double* z_dev, const double* u_dev, double* grad_dev,
+                                    void* stream);
 
 /* -------------------------------------------------------- J2 plasticity */
 /*

@@ -6,11 +6,13 @@
  * switch by namespace (Plato:: -> Plato::amd::):
  *
  *   Elliptic::VectorFunction<Mechanics<3> | Thermal<3>>
- *       value, gradient_u, gradient_u_T, gradient_z     src/elliptic/VectorFunction.hpp:115-710
+ *       value, gradient_u, gradient_u_T, gradient_z,    src/elliptic/VectorFunction.hpp:115-710
+ *       gradient_x
  *   Elliptic::PhysicsScalarFunction<...>
- *       value, gradient_u, gradient_z                   src/elliptic/PhysicsScalarFunction.hpp:225-494
+ *       value, gradient_u, gradient_z, gradient_x       src/elliptic/PhysicsScalarFunction.hpp:225-494
  *   Elliptic::Problem<...>
- *       solution, criterionValue, criterionGradient     src/elliptic/Problem.hpp:237-450
+ *       solution, criterionValue, criterionGradient,    src/elliptic/Problem.hpp:237-542
+ *       criterionGradientX
  *   LocalVectorFunctionInc<InfinitesimalStrainPlasticity<3>>
  *       updateLocalState, value, gradient_{u,c,cp,z}    src/LocalVectorFunctionInc.hpp:236-874
  *   GlobalVectorFunctionInc<InfinitesimalStrainPlasticity<3>>
@@ -1096,6 +1098,14 @@ class VectorFunction {
         check(pa_vector_function_gradient_z(mVf.get(), aState.data(), aControl.data(), tJ->entries().data(), nullptr));
         return tJ;
     }
+    // (dR/dx)^T on the node graph with 3 x dpn blocks (src/elliptic/VectorFunction.hpp:337-420)
+    std::shared_ptr<CrsMatrixType> gradient_x(const ScalarVector& aState, const ScalarVector& aControl,
+                                              Scalar /*aTimeStep*/ = 0.0) const {
+        checkSizes(aState, aControl);
+        auto tJ = std::make_shared<CrsMatrixType>(*mMesh, 3, mNumDofsPerNode, "Gradient X");
+        check(pa_vector_function_gradient_x(mVf.get(), aState.data(), aControl.data(), tJ->entries().data(), nullptr));
+        return tJ;
+    }
     OrdinalType size() const { return mMesh->NumNodes() * mNumDofsPerNode; }
     OrdinalType numNodes() const { return mMesh->NumNodes(); }
     OrdinalType numCells() const { return mMesh->NumElements(); }
@@ -1226,6 +1236,15 @@ class PhysicsScalarFunction {
     ScalarVector gradient_z(const Solutions& aSolution, const ScalarVector& aControl, Scalar aTimeStep = 0.0) const {
         return gradient_z(aSolution.get("State"), aControl, aTimeStep);
     }
+    // src/elliptic/PhysicsScalarFunction.hpp:293-355: N*3 entries, node*3 + dim
+    ScalarVector gradient_x(const ScalarVector& aState, const ScalarVector& aControl, Scalar = 0.0) const {
+        ScalarVector tOut("gradient_x", 3 * aControl.size());
+        check(pa_scalar_function_gradient_x(mSf.get(), aState.data(), aControl.data(), tOut.data(), nullptr));
+        return tOut;
+    }
+    ScalarVector gradient_x(const Solutions& aSolution, const ScalarVector& aControl, Scalar aTimeStep = 0.0) const {
+        return gradient_x(aSolution.get("State"), aControl, aTimeStep);
+    }
     std::string name() const { return mName; }
     pa_scalar_function* handle() const { return mSf.get(); }
 
@@ -1311,6 +1330,17 @@ class Problem {
     }
     ScalarVector criterionGradient(const ScalarVector& aControl, const std::string& aName) {
         return criterionGradient(aControl, lastSolution(aControl), aName);
+    }
+    // self-adjoint criterionGradientX: -df/dx (src/elliptic/Problem.hpp:492-515)
+    ScalarVector criterionGradientX(const ScalarVector& aControl, const Solutions& aSolution,
+                                    const std::string& aName) {
+        ScalarVector tGrad("criterion gradient x", 3 * aControl.size());
+        check(pa_problem_criterion_gradient_x(criterion(aName).problem.get(), aControl.data(),
+                                              aSolution.get("State").data(), tGrad.data(), nullptr));
+        return tGrad;
+    }
+    ScalarVector criterionGradientX(const ScalarVector& aControl, const std::string& aName) {
+        return criterionGradientX(aControl, lastSolution(aControl), aName);
     }
     const pa_solve_stats& solverStats() const { return mStats; }
     const std::vector<int>& bcDofs() const { return mBcDofs; }

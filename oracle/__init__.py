@@ -108,7 +108,8 @@ def _declare(L):
     for name in ["elastic_value", "thermal_value", "elastic_gradient_u", "thermal_gradient_u",
                  "elastic_gradient_u_T", "thermal_gradient_u_T", "elastic_gradient_z", "thermal_gradient_z",
                  "elastic_energy_gradient_u", "thermal_energy_gradient_u", "elastic_energy_gradient_z",
-                 "thermal_energy_gradient_z"]:
+                 "thermal_energy_gradient_z", "elastic_gradient_x", "thermal_gradient_x",
+                 "elastic_energy_gradient_x", "thermal_energy_gradient_x"]:
         f = getattr(L, "orc_" + name)
         f.argtypes = [P, vp, vp, vp]
         f.restype = None
@@ -271,6 +272,11 @@ class Model:
         dpn = 3 if physics == "elastic" else 1
         return self._vec(physics + "_gradient_z", u, z, self.nnzb * dpn)
 
+    def gradient_x(self, physics, u, z):
+        """(dR/dx)^T on the graph, 3 x dpn blocks: block (a, b)[i][j] = dR_{b,j}/dx_{a,i}."""
+        dpn = 3 if physics == "elastic" else 1
+        return self._vec(physics + "_gradient_x", u, z, self.nnzb * 3 * dpn)
+
     # PhysicsScalarFunction (internal energy)
     def energy_value(self, physics, u, z):
         f = getattr(lib(), f"orc_{physics}_energy_value")
@@ -282,6 +288,9 @@ class Model:
 
     def energy_gradient_z(self, physics, u, z):
         return self._vec(physics + "_energy_gradient_z", u, z, self.N)
+
+    def energy_gradient_x(self, physics, u, z):
+        return self._vec(physics + "_energy_gradient_x", u, z, self.N * 3)
 
     def cell_chain(self, u):
         C = self.C

@@ -19,15 +19,17 @@
 //
 // Pinning (what fixes each function independently of this file):
 //   mesh, graph, gradients, strain, stress, residual, Jacobian, gradient_z,
-//   energies  -> unit_tests/DerivativeTests.cpp goldens (tests/golden/);
+//   gradient_x, energies -> unit_tests/DerivativeTests.cpp goldens (tests/golden/);
 //   Jacobian  -> also the analytic B^T C B closed form and central FD;
-//   gradient_z and criterion gradients -> central finite differences;
+//   gradient_z, gradient_x and criterion gradients -> central finite
+//   differences (gradient_x also: translation invariance);
 //   Dirichlet -> symmetric elimination invariants (row/col zero, unit diag).
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <vector>
 #include <algorithm>
+#include <utility>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -62,6 +64,22 @@ template <int W> Fad<W> operator+(const Fad<W>& a, double s) { Fad<W> r(a); r.v 
 template <int W> Fad<W> operator+(double s, const Fad<W>& a) { Fad<W> r(a); r.v += s; return r; }
 template <int W> Fad<W> operator-(const Fad<W>& a, double s) { Fad<W> r(a); r.v -= s; return r; }
 template <int W> Fad<W> operator/(const Fad<W>& a, double s) { Fad<W> r(a); r /= s; return r; }
+template <int W> Fad<W> operator-(const Fad<W>& a) { Fad<W> r(a); r *= -1.0; return r; }
+template <int W> Fad<W> operator/(const Fad<W>& a, const Fad<W>& b) {
+    Fad<W> r;
+    r.v = a.v / b.v;
+    for (int i = 0; i < W; ++i) r.d[i] = (a.d[i] * b.v - a.v * b.d[i]) / (b.v * b.v);
+    return r;
+}
+inline double absval(double x) { return std::fabs(x); }
+template <int W> Fad<W> absval(const Fad<W>& a) { return a.v < 0.0 ? -a : a; }
+inline double fsqrt(double x) { return std::sqrt(x); }
+template <int W> Fad<W> fsqrt(const Fad<W>& a) {
+    Fad<W> r;
+    r.v = std::sqrt(a.v);
+    for (int i = 0; i < W; ++i) r.d[i] = a.d[i] / (2.0 * r.v);
+    return r;
+}
 template <int W> Fad<W> fpow(const Fad<W>& a, double p) {
     Fad<W> r;
     r.v = std::pow(a.v, p);
@@ -120,16 +138,18 @@ namespace {
 // Geometry: ComputeGradientWorkset<3>, src/ImplicitFunctors.hpp:169-221.
 // cfg[n][d]; J = [x_a - x_3] with a = column; grads are the rows of J^-1.
 // ---------------------------------------------------------------------------
-inline void compute_gradient(const double cfg[4][3], double g[4][3], double& vol) {
-    double j11 = cfg[0][0] - cfg[3][0], j12 = cfg[1][0] - cfg[3][0], j13 = cfg[2][0] - cfg[3][0];
-    double j21 = cfg[0][1] - cfg[3][1], j22 = cfg[1][1] - cfg[3][1], j23 = cfg[2][1] - cfg[3][1];
-    double j31 = cfg[0][2] - cfg[3][2], j32 = cfg[1][2] - cfg[3][2], j33 = cfg[2][2] - cfg[3][2];
-    double detj = j11 * j22 * j33 + j12 * j23 * j31 + j13 * j21 * j32
-                - j11 * j23 * j32 - j12 * j21 * j33 - j13 * j22 * j31;
-    double i11 = (j22 * j33 - j23 * j32) / detj, i12 = (j13 * j32 - j12 * j33) / detj, i13 = (j12 * j23 - j13 * j22) / detj;
-    double i21 = (j23 * j31 - j21 * j33) / detj, i22 = (j11 * j33 - j13 * j31) / detj, i23 = (j13 * j21 - j11 * j23) / detj;
-    double i31 = (j21 * j32 - j22 * j31) / detj, i32 = (j12 * j31 - j11 * j32) / detj, i33 = (j11 * j22 - j12 * j21) / detj;
-    vol = std::fabs(detj);
+// The config type K is double, or a config FAD for gradient_x.
+template <class K>
+inline void compute_gradient(const K cfg[4][3], K g[4][3], K& vol) {
+    K j11 = cfg[0][0] - cfg[3][0], j12 = cfg[1][0] - cfg[3][0], j13 = cfg[2][0] - cfg[3][0];
+    K j21 = cfg[0][1] - cfg[3][1], j22 = cfg[1][1] - cfg[3][1], j23 = cfg[2][1] - cfg[3][1];
+    K j31 = cfg[0][2] - cfg[3][2], j32 = cfg[1][2] - cfg[3][2], j33 = cfg[2][2] - cfg[3][2];
+    K detj = j11 * j22 * j33 + j12 * j23 * j31 + j13 * j21 * j32
+           - j11 * j23 * j32 - j12 * j21 * j33 - j13 * j22 * j31;
+    K i11 = (j22 * j33 - j23 * j32) / detj, i12 = (j13 * j32 - j12 * j33) / detj, i13 = (j12 * j23 - j13 * j22) / detj;
+    K i21 = (j23 * j31 - j21 * j33) / detj, i22 = (j11 * j33 - j13 * j31) / detj, i23 = (j13 * j21 - j11 * j23) / detj;
+    K i31 = (j21 * j32 - j22 * j31) / detj, i32 = (j12 * j31 - j11 * j32) / detj, i33 = (j11 * j22 - j12 * j21) / detj;
+    vol = absval(detj);
     g[0][0] = i11; g[0][1] = i12; g[0][2] = i13;
     g[1][0] = i21; g[1][1] = i22; g[1][2] = i23;
     g[2][0] = i31; g[2][1] = i32; g[2][2] = i33;
@@ -170,6 +190,20 @@ void workset_config(const orc_problem* P, std::vector<double>& ws) {
         for (int n = 0; n < 4; ++n)
             for (int d = 0; d < 3; ++d)
                 ws[(size_t)c * 12 + n * 3 + d] = P->coords[(size_t)P->conn[(size_t)c * 4 + n] * 3 + d];
+}
+
+// workset_config_fad, src/Assembly.hpp:1130-1165: config FAD slot node*3 + dim.
+void workset_config_fad(const orc_problem* P, std::vector<Fad<12>>& ws) {
+    const int C = P->num_cells;
+    ws.assign((size_t)C * 12, Fad<12>(0.0));
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < C; ++c)
+        for (int n = 0; n < 4; ++n)
+            for (int d = 0; d < 3; ++d) {
+                Fad<12> x(P->coords[(size_t)P->conn[(size_t)c * 4 + n] * 3 + d]);
+                x.d[n * 3 + d] = 1.0;
+                ws[(size_t)c * 12 + n * 3 + d] = x;
+            }
 }
 
 template <class S, int DPN>
@@ -245,9 +279,10 @@ inline long entry_ordinal(const int* rowmap, const int* colidx, int row, int col
 // (src/StressDivergence.hpp:70-90).  Cubature weight 1/6
 // (src/LinearTetCubRuleDegreeOne.hpp:72-78).
 // ---------------------------------------------------------------------------
-template <class S, class Z, class R>
-void elastic_evaluate(const orc_problem* P, const std::vector<double>& cfgWS, const std::vector<S>& stateWS,
+template <class S, class Z, class R, class K>
+void elastic_evaluate(const orc_problem* P, const std::vector<K>& cfgWS, const std::vector<S>& stateWS,
                       const std::vector<Z>& ctrlWS, std::vector<R>& result) {
+    using E = decltype(std::declval<S>() * std::declval<K>());  // strain type
     const int C = P->num_cells;
     double Cm[6][6];
     isotropic_c(P->youngs, P->poisson, Cm);
@@ -255,13 +290,13 @@ void elastic_evaluate(const orc_problem* P, const std::vector<double>& cfgWS, co
     result.assign((size_t)C * 12, R(0.0));
 #pragma omp parallel for schedule(static)
     for (int c = 0; c < C; ++c) {
-        double cfg[4][3], g[4][3], vol;
+        K cfg[4][3], g[4][3], vol;
         for (int n = 0; n < 4; ++n) for (int d = 0; d < 3; ++d) cfg[n][d] = cfgWS[(size_t)c * 12 + n * 3 + d];
         compute_gradient(cfg, g, vol);
         vol *= qw;
         const S* u = &stateWS[(size_t)c * 12];
-        // strain (StrainScalarType = state type since config is double)
-        S eps[6];
+        // strain (StrainScalarType: the state type, or the config type for gradient_x)
+        E eps[6];
         int v = 0;
         for (int d = 0; d < 3; ++d) {
             eps[v] = 0.0;
@@ -299,22 +334,23 @@ void elastic_evaluate(const orc_problem* P, const std::vector<double>& cfgWS, co
 // ScalarGrad (src/ScalarGrad.hpp:58-74), linear ThermalFlux q = -k grad T
 // (src/ThermalFlux.hpp:42-52), FluxDivergence with scale -1
 // (src/FluxDivergence.hpp:31-48), then body load (src/BodyLoads.hpp:54-122).
-template <class S, class Z, class R>
-void thermal_evaluate(const orc_problem* P, const std::vector<double>& cfgWS, const std::vector<S>& stateWS,
+template <class S, class Z, class R, class K>
+void thermal_evaluate(const orc_problem* P, const std::vector<K>& cfgWS, const std::vector<S>& stateWS,
                       const std::vector<Z>& ctrlWS, std::vector<R>& result) {
+    using E = decltype(std::declval<S>() * std::declval<K>());  // temperature-gradient type
     const int C = P->num_cells;
     const double qw = 1.0 / 6.0;
     const double k = P->conductivity;
     result.assign((size_t)C * 4, R(0.0));
 #pragma omp parallel for schedule(static)
     for (int c = 0; c < C; ++c) {
-        double cfg[4][3], g[4][3], vol;
+        K cfg[4][3], g[4][3], vol;
         for (int n = 0; n < 4; ++n) for (int d = 0; d < 3; ++d) cfg[n][d] = cfgWS[(size_t)c * 12 + n * 3 + d];
         compute_gradient(cfg, g, vol);
-        double detj_abs = vol;
+        K detj_abs = vol;
         vol *= qw;
         const S* T = &stateWS[(size_t)c * 4];
-        S grad[3];
+        E grad[3];
         for (int d = 0; d < 3; ++d) {
             grad[d] = 0.0;
             for (int n = 0; n < 4; ++n) grad[d] += T[n] * g[n][d];
@@ -339,7 +375,7 @@ void thermal_evaluate(const orc_problem* P, const std::vector<double>& cfgWS, co
             Z dens = 0.0;
             for (int n = 0; n < 4; ++n) dens += ctrlWS[(size_t)c * 4 + n];
             dens = dens / 4.0;
-            double w = -1.0 * qw * detj_abs;
+            K w = -1.0 * qw * detj_abs;
             for (int n = 0; n < 4; ++n) out[n] += w * P->body_value * 0.25 * dens;
         }
     }
@@ -465,9 +501,10 @@ void gradz_impl(const orc_problem* P, const double* u, const double* z, double* 
 // InternalThermalEnergy, src/elliptic/InternalThermalEnergy.hpp:96-150
 // (-1 * weighted flux . grad T * V).  Sum: local_result_sum (src/Assembly.hpp:20-33).
 // ---------------------------------------------------------------------------
-template <class S, class Z, class R, int DPN>
-void energy_evaluate(const orc_problem* P, const std::vector<double>& cfgWS, const std::vector<S>& stateWS,
+template <class S, class Z, class R, int DPN, class K = double>
+void energy_evaluate(const orc_problem* P, const std::vector<K>& cfgWS, const std::vector<S>& stateWS,
                      const std::vector<Z>& ctrlWS, std::vector<R>& result) {
+    using E = decltype(std::declval<S>() * std::declval<K>());
     const int C = P->num_cells;
     const double qw = 1.0 / 6.0;
     double Cm[6][6];
@@ -475,7 +512,7 @@ void energy_evaluate(const orc_problem* P, const std::vector<double>& cfgWS, con
     result.assign((size_t)C, R(0.0));
 #pragma omp parallel for schedule(static)
     for (int c = 0; c < C; ++c) {
-        double cfg[4][3], g[4][3], vol;
+        K cfg[4][3], g[4][3], vol;
         for (int n = 0; n < 4; ++n) for (int d = 0; d < 3; ++d) cfg[n][d] = cfgWS[(size_t)c * 12 + n * 3 + d];
         compute_gradient(cfg, g, vol);
         vol *= qw;
@@ -485,7 +522,7 @@ void energy_evaluate(const orc_problem* P, const std::vector<double>& cfgWS, con
         Z s = msimp(rho, P->penalty_exponent, P->penalty_min);
         if (DPN == 3) {
             const S* u = &stateWS[(size_t)c * 12];
-            S eps[6];
+            E eps[6];
             int v = 0;
             for (int d = 0; d < 3; ++d) {
                 eps[v] = 0.0;
@@ -509,7 +546,7 @@ void energy_evaluate(const orc_problem* P, const std::vector<double>& cfgWS, con
             result[c] += 0.5 * inc * vol;
         } else {
             const S* T = &stateWS[(size_t)c * 4];
-            S grad[3];
+            E grad[3];
             for (int d = 0; d < 3; ++d) {
                 grad[d] = 0.0;
                 for (int n = 0; n < 4; ++n) grad[d] += T[n] * g[n][d];
@@ -570,6 +607,88 @@ void energy_gradz_impl(const orc_problem* P, const double* u, const double* z, d
     std::memset(g, 0, (size_t)P->num_nodes * sizeof(double));
     for (int c = 0; c < P->num_cells; ++c)
         for (int n = 0; n < 4; ++n) g[P->conn[(size_t)c * 4 + n]] += res[c].d[n];
+}
+
+// ---------------------------------------------------------------------------
+// gradient_x (GradientX evaluation type: config FAD of width 12, state and
+// control plain).  VectorFunction::gradient_x, src/elliptic/VectorFunction.hpp:
+// 337-420: volume and boundary worksets scattered by assemble_transpose_jacobian
+// (src/Assembly.hpp:1457-1521) through BlockMatrixEntryOrdinal<3, 3, dpn>
+// (src/ImplicitFunctors.hpp:730-766): block (config node, state node) holds
+// dR_{state node, j} / dx_{config node, i} at offset i*dpn + j.
+// ---------------------------------------------------------------------------
+
+// Boundary workset of the uniform natural BC with config FAD (the same steps
+// as boundary_evaluate, the face Jacobian taken from the config workset).
+void boundary_evaluate_x(const orc_problem* P, int dpn, double scale, const std::vector<Fad<12>>& cfgWS,
+                         std::vector<Fad<12>>& result) {
+    const int C = P->num_cells;
+    result.assign((size_t)C * 4 * dpn, Fad<12>(0.0));
+    const double cubw = 0.5, basis = 1.0 / 3.0;
+    for (int f = 0; f < P->num_faces; ++f) {
+        int c = P->face_cells[f];
+        int ln[3] = {P->face_lnodes[f * 3], P->face_lnodes[f * 3 + 1], P->face_lnodes[f * 3 + 2]};
+        Fad<12> J[2][3];
+        for (int a = 0; a < 2; ++a)
+            for (int d = 0; d < 3; ++d)
+                J[a][d] = cfgWS[(size_t)c * 12 + ln[a] * 3 + d] - cfgWS[(size_t)c * 12 + ln[2] * 3 + d];
+        Fad<12> j23 = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+        Fad<12> j31 = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+        Fad<12> j12 = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        Fad<12> area = (scale * cubw) * fsqrt(j23 * j23 + j31 * j31 + j12 * j12);
+        for (int n = 0; n < 3; ++n)
+            for (int d = 0; d < dpn; ++d)
+                result[(size_t)c * 4 * dpn + ln[n] * dpn + d] += basis * P->flux[d] * area;
+    }
+}
+
+template <int DPN>
+void gradx_scatter(const orc_problem* P, const int* rowmap, const int* colidx, const std::vector<Fad<12>>& res,
+                   double* values) {
+    const int C = P->num_cells;
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < C; ++c)
+        for (int r = 0; r < 4 * DPN; ++r)
+            for (int col = 0; col < 12; ++col) {
+                // entry(cell, icellDof = col (config dof), jcellDof = r (residual dof))
+                const int ni = P->conn[(size_t)c * 4 + col / 3], id = col % 3;
+                const int nj = P->conn[(size_t)c * 4 + r / DPN], jd = r % DPN;
+                const long e = entry_ordinal(rowmap, colidx, ni, nj);
+                atomic_add(&values[e * 3 * DPN + id * DPN + jd], res[(size_t)c * 4 * DPN + r].d[col]);
+            }
+}
+
+template <int DPN>
+void gradx_impl(const orc_problem* P, const double* u, const double* z, double* values) {
+    std::vector<int> rowmap, colidx;
+    build_graph(P->num_nodes, P->num_cells, P->conn, rowmap, colidx);
+    std::memset(values, 0, (size_t)rowmap[P->num_nodes] * 3 * DPN * sizeof(double));
+    std::vector<double> stateWS, ctrlWS;
+    std::vector<Fad<12>> cfgWS, res, bres;
+    workset_config_fad(P, cfgWS);
+    workset_state<double, DPN>(P, u, stateWS);
+    workset_control<double>(P, z, ctrlWS);
+    if (DPN == 3) elastic_evaluate(P, cfgWS, stateWS, ctrlWS, res);
+    else thermal_evaluate(P, cfgWS, stateWS, ctrlWS, res);
+    gradx_scatter<DPN>(P, rowmap.data(), colidx.data(), res, values);
+    boundary_evaluate_x(P, DPN, DPN == 3 ? -1.0 : 1.0, cfgWS, bres);
+    gradx_scatter<DPN>(P, rowmap.data(), colidx.data(), bres, values);
+}
+
+// PhysicsScalarFunction::gradient_x, src/elliptic/PhysicsScalarFunction.hpp:293-355;
+// scatter assemble_vector_gradient_fad (VectorEntryOrdinal: node*3 + dim).
+template <int DPN>
+void energy_gradx_impl(const orc_problem* P, const double* u, const double* z, double* g) {
+    std::vector<double> stateWS, ctrlWS;
+    std::vector<Fad<12>> cfgWS, res;
+    workset_state<double, DPN>(P, u, stateWS);
+    workset_control<double>(P, z, ctrlWS);
+    workset_config_fad(P, cfgWS);
+    energy_evaluate<double, double, Fad<12>, DPN, Fad<12>>(P, cfgWS, stateWS, ctrlWS, res);
+    std::memset(g, 0, (size_t)P->num_nodes * 3 * sizeof(double));
+    for (int c = 0; c < P->num_cells; ++c)
+        for (int n = 0; n < 4; ++n)
+            for (int d = 0; d < 3; ++d) g[(size_t)P->conn[(size_t)c * 4 + n] * 3 + d] += res[c].d[n * 3 + d];
 }
 
 }  // namespace
@@ -682,6 +801,8 @@ void orc_elastic_gradient_u_T(const orc_problem* P, const double* u, const doubl
 void orc_thermal_gradient_u_T(const orc_problem* P, const double* u, const double* z, double* v) { jacobian_impl<1>(P, u, z, v, true); }
 void orc_elastic_gradient_z(const orc_problem* P, const double* u, const double* z, double* v) { gradz_impl<3>(P, u, z, v); }
 void orc_thermal_gradient_z(const orc_problem* P, const double* u, const double* z, double* v) { gradz_impl<1>(P, u, z, v); }
+void orc_elastic_gradient_x(const orc_problem* P, const double* u, const double* z, double* v) { gradx_impl<3>(P, u, z, v); }
+void orc_thermal_gradient_x(const orc_problem* P, const double* u, const double* z, double* v) { gradx_impl<1>(P, u, z, v); }
 
 double orc_elastic_energy_value(const orc_problem* P, const double* u, const double* z) { return energy_value_impl<3>(P, u, z); }
 double orc_thermal_energy_value(const orc_problem* P, const double* u, const double* z) { return energy_value_impl<1>(P, u, z); }
@@ -689,6 +810,8 @@ void orc_elastic_energy_gradient_u(const orc_problem* P, const double* u, const 
 void orc_thermal_energy_gradient_u(const orc_problem* P, const double* u, const double* z, double* g) { energy_gradu_impl<1>(P, u, z, g); }
 void orc_elastic_energy_gradient_z(const orc_problem* P, const double* u, const double* z, double* g) { energy_gradz_impl<3>(P, u, z, g); }
 void orc_thermal_energy_gradient_z(const orc_problem* P, const double* u, const double* z, double* g) { energy_gradz_impl<1>(P, u, z, g); }
+void orc_elastic_energy_gradient_x(const orc_problem* P, const double* u, const double* z, double* g) { energy_gradx_impl<3>(P, u, z, g); }
+void orc_thermal_energy_gradient_x(const orc_problem* P, const double* u, const double* z, double* g) { energy_gradx_impl<1>(P, u, z, g); }
 
 // applyBlockConstraints, src/ApplyConstraints.hpp:40-112 (symmetric
 // elimination: zero row and column, unit diagonal, RHS lift b -= A g, b_bc = g).

@@ -6,10 +6,10 @@ with ctypes and mirrors the reference's interface for this path:
 
   * ``VectorFunction``  ~ Plato::Elliptic::VectorFunction<PhysicsT>
     (src/elliptic/VectorFunction.hpp:115-710): value, gradient_u,
-    gradient_u_T, gradient_z;
+    gradient_u_T, gradient_z, gradient_x;
   * ``PhysicsScalarFunction`` ~ Plato::Elliptic::PhysicsScalarFunction
     (src/elliptic/PhysicsScalarFunction.hpp:225-494): value, gradient_u,
-    gradient_z;
+    gradient_z, gradient_x;
   * ``apply_block_constraints`` ~ Plato::applyBlockConstraints
     (src/ApplyConstraints.hpp:40-112).
 
@@ -130,11 +130,14 @@ def _load():
         "pa_vector_function_gradient_u_T": ([vp, vp, vp, vp, i, vp], i),
         "pa_vector_function_value_gradient_u": ([vp, vp, vp, vp, vp, i, vp], i),
         "pa_vector_function_gradient_z": ([vp, vp, vp, vp, vp], i),
+        "pa_vector_function_gradient_x": ([vp, vp, vp, vp, vp], i),
         "pa_scalar_function_create": ([vp, i, vp, vp, ctypes.POINTER(vp)], i),
         "pa_scalar_function_destroy": ([vp], i),
         "pa_scalar_function_value": ([vp, vp, vp, vp, vp], i),
         "pa_scalar_function_gradient_u": ([vp, vp, vp, vp, vp], i),
         "pa_scalar_function_gradient_z": ([vp, vp, vp, vp, vp], i),
+        "pa_scalar_function_gradient_x": ([vp, vp, vp, vp, vp], i),
+        "pa_problem_criterion_gradient_x": ([vp, vp, vp, vp, vp], i),
         "pa_apply_block_constraints": ([vp, i, vp, i, vp, i, vp, vp, d, vp], i),
         "pa_block_spmv": ([vp, i, i, vp, i, vp, d, vp, vp], i),
         "pa_solver_create": ([vp, i, i, ctypes.POINTER(vp)], i),
@@ -520,6 +523,14 @@ class VectorFunction:
         _check(lib.pa_vector_function_gradient_z(self.handle, u.ptr, z.ptr, out.ptr, _sh(stream)))
         return CrsMatrix(self.mesh, 1, self.dpn, out, FP64)
 
+    def gradient_x(self, u, z, out=None, stream=None):
+        """(dR/dx)^T on the node graph, 3 x dpn blocks: block (a, b)[i][j] = dR_{b,j}/dx_{a,i}
+        (src/elliptic/VectorFunction.hpp:337-420)."""
+        u, z = _dev(u), _dev(z)
+        out = out if out is not None else DeviceArray(self.mesh.num_blocks * 3 * self.dpn)
+        _check(lib.pa_vector_function_gradient_x(self.handle, u.ptr, z.ptr, out.ptr, _sh(stream)))
+        return CrsMatrix(self.mesh, 3, self.dpn, out, FP64)
+
     def __del__(self):
         try:
             lib.pa_vector_function_destroy(self.handle)
@@ -560,6 +571,13 @@ class PhysicsScalarFunction:
         u, z = _dev(u), _dev(z)
         out = out if out is not None else DeviceArray(self.mesh.num_nodes)
         _check(lib.pa_scalar_function_gradient_z(self.handle, u.ptr, z.ptr, out.ptr, _sh(stream)))
+        return out
+
+    def gradient_x(self, u, z, out=None, stream=None):
+        """df/dx, N*3 doubles (src/elliptic/PhysicsScalarFunction.hpp:293-355)."""
+        u, z = _dev(u), _dev(z)
+        out = out if out is not None else DeviceArray(self.mesh.num_nodes * 3)
+        _check(lib.pa_scalar_function_gradient_x(self.handle, u.ptr, z.ptr, out.ptr, _sh(stream)))
         return out
 
     def __del__(self):
@@ -649,6 +667,14 @@ class Problem:
         z, u = _dev(z), _dev(u)
         out = out if out is not None else DeviceArray(self.vf.mesh.num_nodes)
         _check(lib.pa_problem_criterion_gradient(self.handle, z.ptr, u.ptr, out.ptr, _sh(stream)))
+        return out
+
+    def criterion_gradient_x(self, z, u, out=None, stream=None):
+        """criterionGradientX of a self-adjoint problem: -df/dx, N*3 doubles
+        (src/elliptic/Problem.hpp:492-542)."""
+        z, u = _dev(z), _dev(u)
+        out = out if out is not None else DeviceArray(self.vf.mesh.num_nodes * 3)
+        _check(lib.pa_problem_criterion_gradient_x(self.handle, z.ptr, u.ptr, out.ptr, _sh(stream)))
         return out
 
     def __del__(self):

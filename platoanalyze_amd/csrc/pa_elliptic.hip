@@ -1090,6 +1090,178 @@ __global__ __launch_bounds__(kBlock) void k_gradz_rows(RowMesh m, Material mt, c
 }
 
 // ---------------------------------------------------------------------------
+// gradient_x rows (shape derivatives): block (row = node a whose position
+// moves, col = state node b) holds dR_b/dx_a, 3 x DPN, entry [i][j] =
+// dR_{b,j}/dx_{a,i}.  With g_c = grad N_c and V the cell volume,
+//   d g_c / dx_{a,i} = -g_a g_{c,i},   d V / dx_{a,i} = V g_{a,i},
+// so with H = grad u (H_dn = sum_c u_{c,d} g_{c,n}) and sigma0 = C eps(H):
+//   elastic: w [ g_{a,i} (sigma0 g_b)_j - g_{b,i} (sigma0 g_a)_j
+//                + (C deps_i g_b)_j ],  deps_i = sym(-H e_i (x) g_a),
+//   thermal: w k [ g_{a,i} (gT.g_b) - gT_i (g_a.g_b) - g_{b,i} (gT.g_a) ]
+//            - V g_{a,i} f rho / 4 (body source),
+// w = s(rho) V.  The reference obtains the same numbers by forward-mode
+// differentiation in the cell coordinates (GradientX evaluation type).
+// ---------------------------------------------------------------------------
+__device__ inline void grad_u_cell(const CellEval& c, const double* __restrict__ u, double H[3][3]) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int n = 0; n < 3; ++n) H[d][n] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double u0 = u[3L * c.nodes[k] + 0], u1 = u[3L * c.nodes[k] + 1], u2 = u[3L * c.nodes[k] + 2];
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            H[0][n] += u0 * c.g[k][n];
+            H[1][n] += u1 * c.g[k][n];
+            H[2][n] += u2 * c.g[k][n];
+        }
+    }
+}
+
+// Voigt strain (engineering shear) of a displacement gradient.
+__device__ inline void voigt_strain(const double H[3][3], double eps[6]) {
+    eps[0] = H[0][0];
+    eps[1] = H[1][1];
+    eps[2] = H[2][2];
+    eps[3] = H[1][2] + H[2][1];
+    eps[4] = H[0][2] + H[2][0];
+    eps[5] = H[0][1] + H[1][0];
+}
+
+template <int DPN, int G, int KMAX>
+__global__ __launch_bounds__(kBlock) void k_gradx_rows(RowMesh m, Material mt, const double* __restrict__ u,
+                                                       const double* __restrict__ z, double body,
+                                                       double* __restrict__ vals) {
+    constexpr int ROWS = kBlock / G, BS = 3 * DPN;
+    __shared__ double acc[ROWS][KMAX * BS];
+    const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+    const int row = xcd_block() * ROWS + grp;
+    const bool active = row < m.N;
+    int rs = 0, rl = 0, p0 = 0, p1 = 0;
+    if (active) {
+        rs = m.rowmap[row];
+        rl = m.rowmap[row + 1] - rs;
+        p0 = m.n2c_off[row];
+        p1 = m.n2c_off[row + 1];
+    }
+    for (int t = lane; t < rl * BS; t += G) acc[grp][t] = 0.0;
+    __syncthreads();
+    for (int p = p0 + lane; p < p1; p += G) {
+        CellEval c;
+        load_cell(m, z, p, c);
+        const double V = c.absdet / 6.0;
+        const double w = penal(c.rho, mt.p, mt.emin) * V;
+        double ga[3];
+        pick_row(c.g, c.a, ga);
+        double* racc = acc[grp];
+        if constexpr (DPN == 3) {
+            double H[3][3], eps[6], sig[6], sa[3];
+            grad_u_cell(c, u, H);
+            voigt_strain(H, eps);
+            iso_stress(mt, eps, sig);
+            btsig(sig, ga, sa);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                double dH[3][3], de[6], ds[6];
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+#pragma unroll
+                    for (int n = 0; n < 3; ++n) dH[d][n] = -H[d][i] * ga[n];
+                voigt_strain(dH, de);
+                iso_stress(mt, de, ds);
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const int slot = (c.slots >> (8 * b)) & 0xff;
+                    double sb[3], db[3];
+                    btsig(sig, c.g[b], sb);
+                    btsig(ds, c.g[b], db);
+#pragma unroll
+                    for (int j = 0; j < 3; ++j)
+                        lds_add(&racc[slot * BS + i * 3 + j], w * ((ga[i] * sb[j] - c.g[b][i] * sa[j]) + db[j]));
+                }
+            }
+        } else {
+            double gT[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int n = 0; n < 4; ++n) {
+                const double T = u[c.nodes[n]];
+                gT[0] += T * c.g[n][0];
+                gT[1] += T * c.g[n][1];
+                gT[2] += T * c.g[n][2];
+            }
+            const double wk = w * mt.k;
+            const double ta = gT[0] * ga[0] + gT[1] * ga[1] + gT[2] * ga[2];
+            const double bl = -1.0 * V * body * 0.25 * c.rho;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int slot = (c.slots >> (8 * b)) & 0xff;
+                const double* gb = c.g[b];
+                const double tb = gT[0] * gb[0] + gT[1] * gb[1] + gT[2] * gb[2];
+                const double ab = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    lds_add(&racc[slot * BS + i], wk * ((ga[i] * tb - gT[i] * ab) - gb[i] * ta) + bl * ga[i]);
+            }
+        }
+    }
+    __syncthreads();
+    if (active) {
+        double* dst = vals + (long)rs * BS;
+        for (int t = lane; t < rl * BS; t += G) dst[t] = acc[grp][t];
+    }
+}
+
+// Shape derivative of a uniform natural BC, added to the gradient_x blocks of
+// the boundary rows: R_b += (1/3) v scale A_f over the faces f holding b, and
+// for face (A, B, C) with unit normal n of (B - A) x (C - A) the area moves as
+// dA/dx_A = 1/2 (x_B - x_C) x n (cyclic for B, C).  One thread per boundary
+// node a; it alone updates row a, after the volume kernel.
+template <int DPN>
+__global__ void k_gradx_natural_bc(int nb, const int* __restrict__ bnodes, const int* __restrict__ boff,
+                                   const int* __restrict__ bface, const int* __restrict__ fcells,
+                                   const int* __restrict__ flnodes, const int4* __restrict__ conn,
+                                   const double4* __restrict__ x4, const int* __restrict__ rowmap,
+                                   const int* __restrict__ colidx, double v0, double v1, double v2, double scale,
+                                   double* __restrict__ vals) {
+    constexpr int BS = 3 * DPN;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nb) return;
+    const int a = bnodes[t];
+    const int r0 = rowmap[a], r1 = rowmap[a + 1];
+    const double v[3] = {v0, v1, v2};
+    const double basis = 1.0 / 3.0;
+    for (int k = boff[t]; k < boff[t + 1]; ++k) {
+        const int f = bface[k] >> 2, pos = bface[k] & 3;
+        const int4 nd = conn[fcells[f]];
+        const int cn[4] = {nd.x, nd.y, nd.z, nd.w};
+        const int fn[3] = {cn[flnodes[3 * f + 0]], cn[flnodes[3 * f + 1]], cn[flnodes[3 * f + 2]]};
+        const double4 A = x4[fn[0]], B = x4[fn[1]], C = x4[fn[2]];
+        const double e1[3] = {B.x - A.x, B.y - A.y, B.z - A.z}, e2[3] = {C.x - A.x, C.y - A.y, C.z - A.z};
+        double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        const double inv = 1.0 / sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        n[0] *= inv;
+        n[1] *= inv;
+        n[2] *= inv;
+        // edge opposite to this node, oriented cyclically: x_next - x_prev
+        const double4 P = pos == 0 ? B : (pos == 1 ? C : A), Q = pos == 0 ? C : (pos == 1 ? A : B);
+        const double q[3] = {P.x - Q.x, P.y - Q.y, P.z - Q.z};
+        const double dA[3] = {0.5 * (q[1] * n[2] - q[2] * n[1]), 0.5 * (q[2] * n[0] - q[0] * n[2]),
+                              0.5 * (q[0] * n[1] - q[1] * n[0])};
+        for (int b = 0; b < 3; ++b) {
+            int e = r0;
+            while (e < r1 && colidx[e] != fn[b]) ++e;
+            if (e == r1) continue;  // not reached: face nodes share a cell
+            double* blk = vals + (long)e * BS;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < DPN; ++j) blk[i * DPN + j] += basis * v[j] * scale * dA[i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Internal energy: per-cell values (block partial sums), nodal gradient_z.
 // ---------------------------------------------------------------------------
 template <int DPN>
@@ -1154,6 +1326,62 @@ __global__ void k_sum_partials(int n, const double* __restrict__ partials, doubl
     for (int i = threadIdx.x; i < n; i += 64) acc += partials[i];
     acc = group_sum<64>(acc);
     if (threadIdx.x == 0) *out = acc;
+}
+
+// Criterion gradient_x rows: for node a, sum over incident cells of
+//   elastic: s V [ 1/2 g_{a,i} (sigma0 : eps) - sum_d H_{d,i} (sigma0 g_a)_d ],
+//   thermal: s k V [ g_{a,i} |gT|^2 - 2 gT_i (gT . g_a) ],
+// the derivatives of 1/2 s sigma0:eps V and s k |grad T|^2 V (see k_gradx_rows).
+template <int DPN, int G>
+__global__ __launch_bounds__(kBlock) void k_energy_gradx_rows(RowMesh m, Material mt, const double* __restrict__ u,
+                                                              const double* __restrict__ z, double scale,
+                                                              double* __restrict__ gx) {
+    constexpr int ROWS = kBlock / G;
+    const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+    const int row = xcd_block() * ROWS + grp;
+    if (row >= m.N) return;
+    const int p0 = m.n2c_off[row], p1 = m.n2c_off[row + 1];
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int p = p0 + lane; p < p1; p += G) {
+        CellEval c;
+        load_cell(m, z, p, c);
+        const double w = penal(c.rho, mt.p, mt.emin) * (c.absdet / 6.0);
+        double ga[3];
+        pick_row(c.g, c.a, ga);
+        if constexpr (DPN == 3) {
+            double H[3][3], eps[6], sig[6], sa[3];
+            grad_u_cell(c, u, H);
+            voigt_strain(H, eps);
+            iso_stress(mt, eps, sig);
+            btsig(sig, ga, sa);
+            double se = 0.0;
+#pragma unroll
+            for (int v = 0; v < 6; ++v) se += sig[v] * eps[v];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                acc[i] += w * (0.5 * ga[i] * se - ((H[0][i] * sa[0] + H[1][i] * sa[1]) + H[2][i] * sa[2]));
+        } else {
+            double gT[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int n = 0; n < 4; ++n) {
+                const double T = u[c.nodes[n]];
+                gT[0] += T * c.g[n][0];
+                gT[1] += T * c.g[n][1];
+                gT[2] += T * c.g[n][2];
+            }
+            const double tt = gT[0] * gT[0] + gT[1] * gT[1] + gT[2] * gT[2];
+            const double ta = gT[0] * ga[0] + gT[1] * ga[1] + gT[2] * ga[2];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) acc[i] += w * mt.k * (ga[i] * tt - 2.0 * gT[i] * ta);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) acc[i] = group_sum<G>(acc[i]);
+    if (lane == 0) {
+        gx[3L * row + 0] = scale * acc[0];
+        gx[3L * row + 1] = scale * acc[1];
+        gx[3L * row + 2] = scale * acc[2];
+    }
 }
 
 template <int DPN, int G>
@@ -1578,6 +1806,17 @@ void check_vf(const pa_vector_function* vf, const void* a, const void* b, const 
 
 }  // namespace
 
+void criterion_gradient_x(const pa_scalar_function* sf, const double* u, const double* z, double scale, double* g,
+                          hipStream_t st) {
+    const pa_mesh* m = sf->mesh;
+    const int phys = sf->dpn == 3 ? PA_MECHANICS : PA_THERMAL;
+    const Material mt = material_of(phys, sf->material, sf->penalty);
+    const int nb = row_blocks(m->N, kG);
+    if (sf->dpn == 3) k_energy_gradx_rows<3, kG><<<nb, kBlock, 0, st>>>(row_mesh(m), mt, u, z, scale, g);
+    else k_energy_gradx_rows<1, kG><<<nb, kBlock, 0, st>>>(row_mesh(m), mt, u, z, scale, g);
+    check_launch();
+}
+
 void self_adjoint_gradient(const pa_vector_function* vf, const pa_scalar_function* sf, const double* u,
                            const double* z, double* out, hipStream_t st) {
     const pa_mesh* m = vf->mesh;
@@ -1747,6 +1986,51 @@ int pa_vector_function_gradient_z(const pa_vector_function* vf, const double* u,
     });
 }
 
+int pa_vector_function_gradient_x(const pa_vector_function* vf, const double* u, const double* z, double* vals,
+                                  void* stream) {
+    return guarded([&] {
+        pa::check_vf(vf, u, z, vals);
+        const pa_mesh* m = vf->mesh;
+        PA_REQUIRE(m->max_row_len <= 64, PA_ERR_LIMIT, "gradient_x: node degree above 63 is not supported");
+        const pa::Material mt = pa::material_of(vf->physics, vf->material, vf->penalty);
+        const double body = vf->has_body ? vf->body_value : 0.0;
+        const hipStream_t st = (hipStream_t)stream;
+        const int nb8 = pa::row_blocks(m->N, pa::kG), nb32 = pa::row_blocks(m->N, 32);
+        if (vf->dpn == 3) {
+            if (m->max_row_len <= 16)
+                pa::k_gradx_rows<3, pa::kG, 16><<<nb8, pa::kBlock, 0, st>>>(pa::row_mesh(m), mt, u, z, 0.0, vals);
+            else
+                pa::k_gradx_rows<3, 32, 64><<<nb32, pa::kBlock, 0, st>>>(pa::row_mesh(m), mt, u, z, 0.0, vals);
+        } else {
+            if (m->max_row_len <= 16)
+                pa::k_gradx_rows<1, pa::kG, 16><<<nb8, pa::kBlock, 0, st>>>(pa::row_mesh(m), mt, u, z, body, vals);
+            else
+                pa::k_gradx_rows<1, 32, 64><<<nb32, pa::kBlock, 0, st>>>(pa::row_mesh(m), mt, u, z, body, vals);
+        }
+        pa::check_launch();
+        // evaluate_boundary: the same scale as the residual's natural BCs
+        const double scale = vf->physics == PA_MECHANICS ? -1.0 : 1.0;
+        for (const auto& bc : vf->natural_bcs) {
+            auto it = m->side_sets.find(bc.side_set);
+            PA_REQUIRE(it != m->side_sets.end(), PA_ERR_ARG, "natural BC: unknown side set " + bc.side_set);
+            const pa::SideSet& ss = it->second;
+            if (ss.num_bnodes == 0) continue;
+            const int nb = ss.num_bnodes;
+            const auto* c4 = reinterpret_cast<const int4*>(m->d_conn.p);
+            const auto* x4 = reinterpret_cast<const double4*>(m->d_coords4.p);
+            if (vf->dpn == 3)
+                pa::k_gradx_natural_bc<3><<<(nb + 255) / 256, 256, 0, st>>>(
+                    nb, ss.d_bnodes.p, ss.d_bface_off.p, ss.d_bface.p, ss.d_cells.p, ss.d_lnodes.p, c4, x4,
+                    m->d_rowmap.p, m->d_colidx.p, bc.values[0], bc.values[1], bc.values[2], scale, vals);
+            else
+                pa::k_gradx_natural_bc<1><<<(nb + 255) / 256, 256, 0, st>>>(
+                    nb, ss.d_bnodes.p, ss.d_bface_off.p, ss.d_bface.p, ss.d_cells.p, ss.d_lnodes.p, c4, x4,
+                    m->d_rowmap.p, m->d_colidx.p, bc.values[0], 0.0, 0.0, scale, vals);
+            pa::check_launch();
+        }
+    });
+}
+
 int pa_scalar_function_create(pa_mesh* mesh, int type, const double* material, const double* penalty,
                               pa_scalar_function** out) {
     return guarded([&] {
@@ -1819,6 +2103,14 @@ int pa_scalar_function_gradient_z(const pa_scalar_function* sf, const double* u,
         if (sf->dpn == 3) pa::k_energy_gradz_rows<3, pa::kG><<<nb, pa::kBlock, 0, st>>>(pa::row_mesh(m), mt, u, z, g);
         else pa::k_energy_gradz_rows<1, pa::kG><<<nb, pa::kBlock, 0, st>>>(pa::row_mesh(m), mt, u, z, g);
         pa::check_launch();
+    });
+}
+
+int pa_scalar_function_gradient_x(const pa_scalar_function* sf, const double* u, const double* z, double* g,
+                                  void* stream) {
+    return guarded([&] {
+        PA_REQUIRE(sf && u && z && g, PA_ERR_ARG, "pa_scalar_function_gradient_x: null argument");
+        pa::criterion_gradient_x(sf, u, z, 1.0, g, (hipStream_t)stream);
     });
 }
 

@@ -173,6 +173,10 @@ struct pa_vector_function {
 };
 
 namespace pa {
+// Criterion gradient_x (N*3 doubles), scaled (the self-adjoint total
+// gradient wrt the coordinates is -1 times it).
+void criterion_gradient_x(const pa_scalar_function* sf, const double* u, const double* z, double scale, double* g,
+                          hipStream_t st);
 // Fused self-adjoint criterion gradient (lambda = -u), defined with the
 // assembly kernels; used by the problem driver.
 void self_adjoint_gradient(const pa_vector_function* vf, const pa_scalar_function* sf, const double* u,

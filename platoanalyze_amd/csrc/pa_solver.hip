@@ -645,4 +645,12 @@ int pa_problem_criterion_gradient(pa_problem* P, const double* z, const double* 
     });
 }
 
+int pa_problem_criterion_gradient_x(pa_problem* P, const double* z, const double* u, double* grad, void* stream) {
+    return guarded([&] {
+        PA_REQUIRE(P && z && u && grad, PA_ERR_ARG, "pa_problem_criterion_gradient_x: null argument");
+        // self-adjoint: dF/dx = -(df/dx), src/elliptic/Problem.hpp:512-515
+        pa::criterion_gradient_x(P->sf, u, z, -1.0, grad, (hipStream_t)stream);
+    });
+}
+
 }  // extern "C"

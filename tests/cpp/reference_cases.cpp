@@ -322,6 +322,9 @@ std::string elastostatic_residual() {
     j.put("num_rows", (double)jacobian->numRows());
     j.put("block", (double)jacobian->numRowsPerBlock());
     j.put("gz_block_cols", (double)gradient_z->numColsPerBlock());
+    auto gradient_x = esVectorFunction.gradient_x(u, z);
+    j.put("gradient_x", pa::toHost(gradient_x->entries()));
+    j.put("gx_block", (double)gradient_x->numRowsPerBlock() * 10 + gradient_x->numColsPerBlock());
     return j.str();
 }
 
@@ -337,6 +340,7 @@ std::string internal_elastic_energy() {
     j.put("value", eeScalarFunction.value(tSolution, z));
     j.put("gradient_u", pa::toHost(eeScalarFunction.gradient_u(tSolution, z, 0)));
     j.put("gradient_z", pa::toHost(eeScalarFunction.gradient_z(tSolution, z)));
+    j.put("gradient_x", pa::toHost(eeScalarFunction.gradient_x(tSolution, z)));
     return j.str();
 }
 
@@ -351,6 +355,7 @@ std::string thermostatic_residual() {
     j.put("residual", pa::toHost(tsVectorFunction.value(u, z)));
     j.put("jacobian", pa::toHost(tsVectorFunction.gradient_u(u, z)->entries()));
     j.put("gradient_z", pa::toHost(tsVectorFunction.gradient_z(u, z)->entries()));
+    j.put("gradient_x", pa::toHost(tsVectorFunction.gradient_x(u, z)->entries()));
     return j.str();
 }
 
@@ -364,6 +369,7 @@ std::string internal_thermal_energy() {
     j.put("value", tsScalarFunction.value(u, z));
     j.put("gradient_u", pa::toHost(tsScalarFunction.gradient_u(u, z)));
     j.put("gradient_z", pa::toHost(tsScalarFunction.gradient_z(u, z)));
+    j.put("gradient_x", pa::toHost(tsScalarFunction.gradient_x(u, z)));
     return j.str();
 }
 
@@ -378,7 +384,9 @@ std::string compliance_problem() {
     auto tSolution = tProblem.solution(z);
     const double F = tProblem.criterionValue(z, tSolution, "Compliance");
     auto dF = tProblem.criterionGradient(z, tSolution, "Compliance");
+    auto dFdx = tProblem.criterionGradientX(z, tSolution, "Compliance");
     Json j;
+    j.put("gradient_x", pa::toHost(dFdx));
     j.put("z", zh);
     j.put("u", pa::toHost(tSolution.get("State")));
     j.put("value", F);

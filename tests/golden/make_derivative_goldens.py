@@ -68,19 +68,23 @@ def main(ref_root):
                "result": "result_gold"},
         {"youngs": 1.0e6, "poisson": 0.3, "state": "u_k = 1e-4*(k+1)"})
     add("ElastostaticResidual3D", {"residual": "residual_gold", "jacobian_head": "gold_jac_entries",
-                                   "gradient_z_head": "gold_grad_entries"},
+                                   "gradient_z_head": "gold_grad_entries",
+                                   "gradient_x_head": "gold_grad_x_entries"},
         {"youngs": 1.0e6, "poisson": 0.3, "penalty_exponent": 1.0, "penalty_min": 0.0,
          "state": "u_k = 1e-4*(k+1)", "control": 1.0})
     add("InternalElasticEnergy3D", {"value": r"re:value_gold = ([-+0-9.eE]+);",
-                                    "gradient_u": "grad_u_gold", "gradient_z": "grad_z_gold"},
+                                    "gradient_u": "grad_u_gold", "gradient_z": "grad_z_gold",
+                                    "gradient_x": "grad_x_gold"},
         {"youngs": 1.0e6, "poisson": 0.3, "penalty_exponent": 1.0, "penalty_min": 0.0,
          "state": "u_k = 1e-4*(k+1)", "control": 1.0})
     add("ThermostaticResidual3D", {"residual": "residual_gold", "jacobian_head": "gold_jac_entries",
-                                   "gradient_z_head": "gold_grad_entries"},
+                                   "gradient_z_head": "gold_grad_entries",
+                                   "gradient_x_head": "gold_grad_x_entries"},
         {"conductivity": 100.0, "penalty_exponent": 1.0, "penalty_min": 0.0,
          "state": "T_k = 0.1*(k+1)", "control": 1.0})
     add("InternalThermalEnergy3D", {"value": r"re:value_gold = ([-+0-9.eE]+);",
-                                    "gradient_u": "grad_u_gold", "gradient_z": "grad_z_gold"},
+                                    "gradient_u": "grad_u_gold", "gradient_z": "grad_z_gold",
+                                    "gradient_x": "grad_x_gold"},
         {"conductivity": 100.0, "penalty_exponent": 1.0, "penalty_min": 0.0,
          "state": "T_k = 0.1*(k+1)", "control": 1.0})
     dst = Path(__file__).with_name("derivative_tests.json")

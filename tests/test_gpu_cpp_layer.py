@@ -76,6 +76,8 @@ def test_elastostatic_residual(results):
     om = model("elastic")
     assert np.array_equal(o["rowmap"], om.rowmap) and np.array_equal(o["colidx"], om.colidx)
     assert o["num_rows"] == 81 and o["block"] == 3 and o["gz_block_cols"] == 3
+    assert reference_equal(o["gradient_x"], gold("ElastostaticResidual3D", "gradient_x_head"), 1e-13)
+    assert o["gx_block"] == 33
     u = 1e-4 * np.arange(1, 82)
     z = np.ones(27)
     assert rel(o["jacobian"], om.gradient_u("elastic", u, z)) < 1e-12
@@ -90,6 +92,10 @@ def test_internal_elastic_energy(results):
     assert abs(o["value"] - 48.15) <= 1e-13 * 48.15
     assert reference_equal(o["gradient_u"], gold("InternalElasticEnergy3D", "gradient_u"), 1e-13)
     assert reference_equal(o["gradient_z"], gold("InternalElasticEnergy3D", "gradient_z"), 1e-13)
+    gx, ref = np.asarray(o["gradient_x"]), gold("InternalElasticEnergy3D", "gradient_x")
+    nz = ref != 0.0  # zero entries are checked as |value| < 1e-13, as the reference does
+    assert reference_equal(gx[:ref.size][nz], ref[nz], 1e-13)
+    assert np.max(np.abs(gx[:ref.size][~nz])) < 1e-13
 
 
 def test_thermostatic_residual_and_energy(results):
@@ -97,10 +103,12 @@ def test_thermostatic_residual_and_energy(results):
     assert reference_equal(o["residual"], gold("ThermostaticResidual3D", "residual"), 1e-13)
     assert reference_equal(o["jacobian"], gold("ThermostaticResidual3D", "jacobian_head"), 1e-15, 1e-9)
     assert reference_equal(o["gradient_z"], gold("ThermostaticResidual3D", "gradient_z_head"), 1e-14)
+    assert reference_equal(o["gradient_x"], gold("ThermostaticResidual3D", "gradient_x_head"), 1e-13)
     e = results["InternalThermalEnergy3D"]
     assert abs(e["value"] - float(gold("InternalThermalEnergy3D", "value"))) < 1e-11
     assert reference_equal(e["gradient_u"], gold("InternalThermalEnergy3D", "gradient_u"), 1e-13)
     assert reference_equal(e["gradient_z"], gold("InternalThermalEnergy3D", "gradient_z"), 1e-13)
+    assert reference_equal(e["gradient_x"], gold("InternalThermalEnergy3D", "gradient_x"), 1e-13)
 
 
 def test_compliance_problem(results):
@@ -126,6 +134,8 @@ def test_compliance_problem(results):
     comp = om.energy_gradient_z("elastic", u, z) + oracle.block_spmv(1, 3, om.rowmap, om.colidx,
                                                                      om.gradient_z("elastic", u, z), -u)
     assert rel(o["gradient"], comp) < 1e-12
+    # self-adjoint criterionGradientX = -df/dx
+    assert rel(o["gradient_x"], -om.energy_gradient_x("elastic", u, z)) < 1e-12
 
 
 def test_j2_elastic_solution_golden(results):

@@ -182,3 +182,97 @@ def test_config2_full_size_properties(pa):
                                   penalty_min=1e-9)
     e = sf.value(u, z)
     assert abs(e - 0.5 * u @ Ku.numpy()) < 1e-11 * abs(e)
+
+
+# ------------------------------------------------- gradient_x (shape derivatives)
+def test_gradient_x_goldens_through_abi(pa):
+    print("case: gradient_x goldens, 2x2x2 box", flush=True)
+    mesh = pa.Mesh.box(2, 2, 2)
+    N = mesh.num_nodes
+    u = 1e-4 * np.arange(1, 3 * N + 1)
+    T = 0.1 * np.arange(1, N + 1)
+    z = np.ones(N)
+    vf = pa.VectorFunction(mesh, "Mechanics", youngs=1e6, poisson=0.3, penalty_exponent=1.0, penalty_min=0.0)
+    Gx = vf.gradient_x(u, z)
+    assert (Gx.bi, Gx.bj) == (3, 3)
+    assert rel(Gx.entries()[:36], gold("ElastostaticResidual3D", "gradient_x_head")) < 1e-13
+    sf = pa.PhysicsScalarFunction(mesh, "Internal Elastic Energy", youngs=1e6, poisson=0.3, penalty_exponent=1.0,
+                                  penalty_min=0.0)
+    gx = sf.gradient_x(u, z).numpy()
+    ref = gold("InternalElasticEnergy3D", "gradient_x")
+    nz = ref != 0.0
+    assert rel(gx[:ref.size][nz], ref[nz]) < 1e-13
+    assert np.max(np.abs(gx[:ref.size][~nz])) < 1e-13
+    tf = pa.VectorFunction(mesh, "Thermal", conductivity=100.0, penalty_exponent=1.0, penalty_min=0.0)
+    assert rel(tf.gradient_x(T, z).entries()[:12], gold("ThermostaticResidual3D", "gradient_x_head")) < 1e-13
+    ts = pa.PhysicsScalarFunction(mesh, "Internal Thermal Energy", conductivity=100.0, penalty_exponent=1.0,
+                                  penalty_min=0.0)
+    assert rel(ts.gradient_x(T, z).numpy()[:36], gold("InternalThermalEnergy3D", "gradient_x")) < 1e-13
+
+
+def _skewed_mesh(pa, dims, L, seed):
+    coords, conn = pa.box_mesh(*dims, *L)
+    X = coords.reshape(-1, 3).copy()
+    rng = np.random.default_rng(seed)
+    h = min(L[0] / dims[0], L[1] / dims[1], L[2] / dims[2])
+    X += rng.uniform(-0.12 * h, 0.12 * h, X.shape)
+    X[np.isclose(coords.reshape(-1, 3)[:, 0], L[0]), 0] = L[0]
+    coords = X.ravel()
+    fc, fl = oracle.side_set(coords, conn, 0, L[0])
+    mesh = pa.Mesh(coords, conn)
+    mesh.add_side_set("x+", fc, fl)
+    return coords, conn, fc, fl, mesh, rng
+
+
+@pytest.mark.parametrize("physics", ["elastic", "thermal"])
+def test_gradient_x_parity_skewed_mesh(pa, physics):
+    """gradient_x of the vector function (natural BC and body source included),
+    of the criterion, and the problem's criterionGradientX against the oracle
+    (config FAD) on a 12x6x6 box with every node moved off the grid."""
+    print(f"case: gradient_x, skewed 12x6x6 box, {physics}, seed 3", flush=True)
+    coords, conn, fc, fl, mesh, rng = _skewed_mesh(pa, (12, 6, 6), (2.0, 1.0, 1.0), 3)
+    N = mesh.num_nodes
+    z = rng.uniform(0.3, 1.0, N)
+    if physics == "elastic":
+        st = rng.standard_normal(3 * N) * 1e-2
+        vf = pa.VectorFunction(mesh, "Mechanics", youngs=2.0, poisson=0.3, penalty_exponent=3.0, penalty_min=1e-3)
+        vf.add_uniform_natural_bc("x+", (0.1, 0.2, -1.0))
+        om = oracle.Model(coords, conn, youngs=2.0, poisson=0.3, penalty_exponent=3.0, penalty_min=1e-3,
+                          face_cells=fc, face_lnodes=fl, flux=(0.1, 0.2, -1.0))
+        sf = pa.PhysicsScalarFunction(mesh, "Internal Elastic Energy", youngs=2.0, poisson=0.3,
+                                      penalty_exponent=3.0, penalty_min=1e-3)
+    else:
+        st = rng.standard_normal(N)
+        vf = pa.VectorFunction(mesh, "Thermal", conductivity=1.5, penalty_exponent=3.0, penalty_min=1e-3)
+        vf.add_uniform_natural_bc("x+", (0.7,))
+        vf.add_body_load(0, 1.0)
+        om = oracle.Model(coords, conn, conductivity=1.5, penalty_exponent=3.0, penalty_min=1e-3, face_cells=fc,
+                          face_lnodes=fl, flux=(0.7, 0.0, 0.0), body=(0, 1.0))
+        sf = pa.PhysicsScalarFunction(mesh, "Internal Thermal Energy", conductivity=1.5, penalty_exponent=3.0,
+                                      penalty_min=1e-3)
+    assert rel(vf.gradient_x(st, z).entries(), om.gradient_x(physics, st, z)) < TOL64
+    gref = om.energy_gradient_x(physics, st, z)
+    assert rel(sf.gradient_x(st, z).numpy(), gref) < TOL64
+    # criterionGradientX, self-adjoint: -df/dx exactly
+    dofs = np.arange(3 if physics == "elastic" else 1)
+    P = pa.Problem(vf, sf, dofs, mode="fp64", tol=1e-10, max_iter=10)
+    assert np.array_equal(P.criterion_gradient_x(z, st).numpy(), -sf.gradient_x(st, z).numpy())
+    # (dR/dx)^T lambda through the 3 x dpn block SpMV against scipy
+    Gx = vf.gradient_x(st, z)
+    lam = pa.DeviceArray.from_numpy(rng.standard_normal(st.size))
+    y = pa.DeviceArray(3 * N)
+    Gx.spmv(lam, y)
+    dpn = 3 if physics == "elastic" else 1
+    Gs = oracle.to_scipy(3, dpn, om.rowmap, om.colidx, om.gradient_x(physics, st, z))
+    assert rel(y.numpy(), Gs @ lam.numpy()) < TOL64
+
+
+def test_gradient_x_config1_parity(pa):
+    print("case: gradient_x, 20x10x10 box, elastic and thermal, seed 1", flush=True)
+    for physics in ("elastic", "thermal"):
+        mesh, vf, om, u, z = _cfg(pa, (20, 10, 10), (2.0, 1.0, 1.0), 1, physics)
+        assert rel(vf.gradient_x(u, z).entries(), om.gradient_x(physics, u, z)) < TOL64
+        kind = "Internal Elastic Energy" if physics == "elastic" else "Internal Thermal Energy"
+        mat = dict(youngs=1.0, poisson=0.3) if physics == "elastic" else dict(conductivity=1.0)
+        sf = pa.PhysicsScalarFunction(mesh, kind, penalty_exponent=3.0, penalty_min=1e-9, **mat)
+        assert rel(sf.gradient_x(u, z).numpy(), om.energy_gradient_x(physics, u, z)) < TOL64

@@ -78,6 +78,18 @@ def test_internal_thermal_energy_goldens(box2):
     assert rel(th.energy_gradient_z("thermal", T, z), gold("InternalThermalEnergy3D", "gradient_z")) < 1e-13
 
 
+def test_gradient_x_goldens(box2):
+    el, th, u, T, z = box2["el"], box2["th"], box2["u"], box2["T"], box2["z"]
+    assert rel(el.gradient_x("elastic", u, z)[:36], gold("ElastostaticResidual3D", "gradient_x_head")) < 1e-13
+    assert rel(th.gradient_x("thermal", T, z)[:12], gold("ThermostaticResidual3D", "gradient_x_head")) < 1e-13
+    ge = el.energy_gradient_x("elastic", u, z)
+    ref = gold("InternalElasticEnergy3D", "gradient_x")
+    nz = ref != 0.0  # the reference checks its zero entries as |value| < 1e-13
+    assert rel(ge[:ref.size][nz], ref[nz]) < 1e-13
+    assert np.max(np.abs(ge[:ref.size][~nz])) < 1e-13
+    assert rel(th.energy_gradient_x("thermal", T, z)[:36], gold("InternalThermalEnergy3D", "gradient_x")) < 1e-13
+
+
 def test_graph_rows_match_reference_structure(box2):
     rowmap, colidx = box2["el"].rowmap, box2["el"].colidx
     # DerivativeTests.cpp:1651-1656 (thermal Jacobian head) implies these rows
@@ -236,6 +248,74 @@ def test_energy_gradients_central_differences(small):
         assert rel(gu[iu], fd(lambda x: m.energy_value(phys, x, z), st, 1e-6, iu)) < 1e-7
         iz = [1, 6, 17]
         assert rel(gz[iz], fd(lambda x: m.energy_value(phys, st, x), z, 1e-6, iz)) < 1e-7
+
+
+def _small_models(coords, conn, fc, fl):
+    el = oracle.Model(coords, conn, youngs=2.0, poisson=0.3, penalty_exponent=3.0, penalty_min=1e-3,
+                      face_cells=fc, face_lnodes=fl, flux=(0.1, 0.2, -1.0))
+    th = oracle.Model(coords, conn, conductivity=1.5, penalty_exponent=3.0, penalty_min=1e-3, face_cells=fc,
+                      face_lnodes=fl, flux=(0.7, 0, 0), body=(0, 1.0))
+    return el, th
+
+
+@pytest.fixture(scope="module")
+def skewed():
+    # the small box with interior and boundary nodes moved off the grid, so no
+    # cell is a right-angled Kuhn tet
+    coords, conn = oracle.box_mesh(3, 2, 2, 1.5, 1.0, 1.0)
+    X = coords.reshape(-1, 3).copy()
+    rng = np.random.default_rng(5)
+    X += rng.uniform(-0.06, 0.06, X.shape)
+    X[np.isclose(coords.reshape(-1, 3)[:, 0], 1.5), 0] = 1.5  # x+ face stays planar
+    coords = X.ravel()
+    fc, fl = oracle.side_set(coords, conn, 0, 1.5)
+    N = X.shape[0]
+    z = rng.uniform(0.3, 1.0, N)
+    u = rng.standard_normal(3 * N) * 1e-2
+    T = rng.standard_normal(N)
+    return dict(coords=coords, conn=conn, fc=fc, fl=fl, N=N, z=z, u=u, T=T)
+
+
+def test_gradient_x_central_differences(skewed):
+    """Rows of (dR/dx)^T and df/dx against central differences of the oracle's
+    value in the node coordinates, natural BC and body source included; the
+    nodes cover a corner, the loaded x+ face and the interior."""
+    S = skewed
+    X = S["coords"]
+    el, th = _small_models(X, S["conn"], S["fc"], S["fl"])
+    X3 = X.reshape(-1, 3)
+    loaded = int(np.flatnonzero(np.isclose(X3[:, 0], 1.5))[5])
+    nodes = [0, loaded, 17]
+    h = 1e-6
+    for phys, st, dpn in (("elastic", S["u"], 3), ("thermal", S["T"], 1)):
+        m = el if phys == "elastic" else th
+        Gx = oracle.to_scipy(3, dpn, m.rowmap, m.colidx, m.gradient_x(phys, st, S["z"])).toarray()
+        gx = m.energy_gradient_x(phys, st, S["z"])
+        for a in nodes:
+            for i in range(3):
+                Xp, Xm = X.copy(), X.copy()
+                Xp[3 * a + i] += h
+                Xm[3 * a + i] -= h
+                mp = _small_models(Xp, S["conn"], S["fc"], S["fl"])[0 if phys == "elastic" else 1]
+                mm = _small_models(Xm, S["conn"], S["fc"], S["fl"])[0 if phys == "elastic" else 1]
+                dR = (mp.value(phys, st, S["z"]) - mm.value(phys, st, S["z"])) / (2 * h)
+                assert rel(Gx[3 * a + i], dR) < 1e-7, (phys, a, i)
+                df = (mp.energy_value(phys, st, S["z"]) - mm.energy_value(phys, st, S["z"])) / (2 * h)
+                assert abs(gx[3 * a + i] - df) < 1e-7 * np.max(np.abs(gx)), (phys, a, i)
+
+
+def test_gradient_x_translation_invariance(skewed):
+    """A rigid translation of the whole mesh changes neither R nor f: the
+    coordinate derivatives summed over all nodes vanish per direction."""
+    S = skewed
+    el, th = _small_models(S["coords"], S["conn"], S["fc"], S["fl"])
+    for m, phys, st, dpn in ((el, "elastic", S["u"], 3), (th, "thermal", S["T"], 1)):
+        Gx = oracle.to_scipy(3, dpn, m.rowmap, m.colidx, m.gradient_x(phys, st, S["z"])).toarray()
+        col = np.abs(Gx).max()
+        for i in range(3):
+            assert np.max(np.abs(Gx[i::3].sum(0))) < 1e-12 * col
+        gx = m.energy_gradient_x(phys, st, S["z"]).reshape(-1, 3)
+        assert np.max(np.abs(gx.sum(0))) < 1e-12 * np.abs(gx).max()
 
 
 def test_elastic_energy_is_half_u_K_u(small):
