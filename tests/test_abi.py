@@ -34,6 +34,25 @@ def test_error_path_without_device_work():
     assert "positive" in pa.lib.pa_last_error().decode()
 
 
+def test_mesh_validation_errors_without_device_work():
+    """pa_mesh_create validates its input on the host before any device
+    allocation: out-of-range connectivity and empty meshes are errors."""
+    import platoanalyze_amd as pa
+    h = ctypes.c_void_p()
+    coords = np.zeros(12)
+    bad = np.array([0, 1, 2, 4], np.int32)  # node 4 of a 4-node mesh
+    st = pa.lib.pa_mesh_create(coords.ctypes.data_as(ctypes.c_void_p), 4, bad.ctypes.data_as(ctypes.c_void_p), 1,
+                               ctypes.byref(h))
+    assert st != 0 and "out of range" in pa.lib.pa_last_error().decode()
+    neg = np.array([0, 1, -1, 3], np.int32)
+    st = pa.lib.pa_mesh_create(coords.ctypes.data_as(ctypes.c_void_p), 4, neg.ctypes.data_as(ctypes.c_void_p), 1,
+                               ctypes.byref(h))
+    assert st != 0 and "out of range" in pa.lib.pa_last_error().decode()
+    st = pa.lib.pa_mesh_create(coords.ctypes.data_as(ctypes.c_void_p), 4, bad.ctypes.data_as(ctypes.c_void_p), 0,
+                               ctypes.byref(h))
+    assert st != 0 and "empty" in pa.lib.pa_last_error().decode()
+
+
 def test_box_mesh_matches_oracle():
     import oracle
     import platoanalyze_amd as pa
