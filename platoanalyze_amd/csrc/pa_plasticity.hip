@@ -876,11 +876,11 @@ __global__ __launch_bounds__(kB) void k_gather_matrix(int N, const int* __restri
                                                       const int2* __restrict__ pairs,
                                                       const double* __restrict__ kcell,
                                                       const double* __restrict__ vcell, int transpose, int maxlen,
-                                                      double* __restrict__ values, double* __restrict__ R) {
+                                                      int rpb, double* __restrict__ values, double* __restrict__ R) {
     extern __shared__ double acc[];
     const int grp = threadIdx.x / G16, lane = threadIdx.x % G16;
-    const int row = blockIdx.x * CPB + grp;
-    if (row >= N) return;
+    const int row = blockIdx.x * rpb + grp;
+    if (grp >= rpb || row >= N) return;  // whole 16-lane groups exit together
     double* my = acc + (long)grp * maxlen * 16;
     const int len = rowmap[row + 1] - rowmap[row];
     for (int s = 0; s < len; ++s) my[s * 16 + lane] = 0.0;
@@ -1224,11 +1224,13 @@ void condensed_system(pa_plasticity* p, const double* u, const double* c, const 
     k_condensed<<<(m->C + CPB - 1) / CPB, kB, 0, st>>>(m->C, constants(p->prm), m->d_conn.p, coords4(p), u, c, cp,
                                                        pi, z, p->kcell.p, p->vcell.p);
     check();
+    // rows per workgroup: as many 16-lane groups as 64 KB of row accumulators allow
     const int maxlen = m->max_row_len;
-    const size_t lds = (size_t)CPB * maxlen * 16 * sizeof(double);
+    const int rpb = std::min(CPB, (int)(65536 / ((size_t)maxlen * 16 * sizeof(double))));
+    const size_t lds = (size_t)rpb * maxlen * 16 * sizeof(double);
     auto* gather = p->dpn == 4 ? k_gather_matrix<4> : k_gather_matrix<3>;
-    gather<<<(m->N + CPB - 1) / CPB, kB, lds, st>>>(m->N, m->d_rowmap.p, m->d_n2c_off.p, m->d_pairs.p, p->kcell.p,
-                                                    p->vcell.p, transpose ? 1 : 0, maxlen, values,
+    gather<<<(m->N + rpb - 1) / rpb, kB, lds, st>>>(m->N, m->d_rowmap.p, m->d_n2c_off.p, m->d_pairs.p, p->kcell.p,
+                                                    p->vcell.p, transpose ? 1 : 0, maxlen, rpb, values,
                                                     transpose ? nullptr : R);
     check();
 }
@@ -1337,8 +1339,8 @@ int pa_plasticity_create(const pa_mesh* mesh, const pa_j2_params* params, pa_pla
                    "pa_plasticity_create: elastic constants out of range");
         PA_REQUIRE(params->formulation == PA_J2_MIXED || params->formulation == PA_J2_DISPLACEMENT, PA_ERR_ARG,
                    "pa_plasticity_create: formulation must be PA_J2_MIXED or PA_J2_DISPLACEMENT");
-        PA_REQUIRE(mesh->max_row_len <= 32, PA_ERR_LIMIT,
-                   "pa_plasticity_create: graph rows above 32 blocks are not supported by the gather kernel");
+        PA_REQUIRE(mesh->max_row_len <= 64, PA_ERR_LIMIT,
+                   "pa_plasticity_create: graph rows above 64 blocks are not supported by the gather kernel");
         auto* p = new pa_plasticity();
         try {
             p->mesh = mesh;

@@ -93,3 +93,46 @@ def test_fan_mesh_all_entry_points(pa, physics):
     assert rel(sf.gradient_u(st, z).numpy(), om.energy_gradient_u(physics, st, z)) < TOL64
     assert rel(sf.gradient_z(st, z).numpy(), om.energy_gradient_z(physics, st, z)) < TOL64
     assert rel(sf.gradient_x(st, z).numpy(), om.energy_gradient_x(physics, st, z)) < TOL64
+
+
+def test_j2_fan_mesh_all_entry_points(pa):
+    """J2 plasticity (mixed u-p) on the fan: a 41-block row takes the gather
+    kernel's reduced rows-per-workgroup launch."""
+    print("case: J2 on the 76-tet fan mesh (41-block rows)", flush=True)
+    coords, conn, _, _, rng = fan_mesh()
+    coords = coords[:-3]  # drop the unused node: the lumped projection needs every node in a cell
+    mesh = pa.Mesh(coords, conn)
+    mat = dict(youngs=1e3, poisson=0.3, hardening_isotropic=10.0, hardening_kinematic=5.0, initial_yield_stress=1.0,
+               elastic_penalty_exponent=3.0, elastic_min_ersatz=1e-9, plastic_penalty_exponent=2.5,
+               plastic_min_ersatz=1e-9, global_penalty_exponent=3.0, global_min_ersatz=1e-9, pressure_scaling=2.0)
+    pl = pa.Plasticity(mesh, projection_penalty=1, projection_penalty_exponent=3.0, projection_min_ersatz=1e-3,
+                       work_penalty_exponent=3.0, work_min_ersatz=1e-9, **mat)
+    om = oracle.J2Model(coords, conn, projection_penalty=(3.0, 1e-3), work_penalty=(3.0, 1e-9), **mat)
+    N, C = om.N, om.C
+    u = rng.uniform(-1, 1, N * 4) * 3e-4  # about 30 of the 76 cells plastic
+    u[3::4] *= 100.0
+    cp = np.zeros((C, 14))
+    cp[:, 2:8] = rng.uniform(-1, 1, (C, 6)) * 1e-4
+    cp[:, 8:] = rng.uniform(-1, 1, (C, 6)) * 0.05
+    cp[:, 2:5] -= cp[:, 2:5].mean(axis=1, keepdims=True)
+    cp[:, 8:11] -= cp[:, 8:11].mean(axis=1, keepdims=True)
+    cp = cp.ravel()
+    z = rng.uniform(0.5, 1.0, N)
+    pi = rng.uniform(-1, 1, N * 3)
+    c = om.update_local_state(u, cp, z)
+    nplastic = (c.reshape(-1, 14)[:, 1] > 0).sum()
+    assert 5 < nplastic < C - 5
+    assert rel(pl.update_local_state(u, cp, z).numpy(), c) < TOL64
+    for wrt, code in (("c", pa.WRT_C), ("u", pa.WRT_U), ("z", pa.WRT_Z), ("cp", pa.WRT_C_PREV)):
+        assert rel(pl.local_gradient(code, u, c, cp, z), om.local_gradient(wrt, u, c, cp, z)) < TOL64, wrt
+    assert rel(pl.global_value(u, c, pi, z).numpy(), om.global_value(u, c, pi, z)) < TOL64
+    K, R = pl.condensed_system(u, c, cp, pi, z)
+    vals, Rref = om.condensed_system(u, c, cp, pi, z)
+    assert rel(K.entries(), vals) < TOL64 and rel(R.numpy(), Rref) < TOL64
+    KT, _ = pl.condensed_system(u, c, cp, pi, z, transpose=True)
+    A = oracle.to_scipy(4, 4, om.rowmap, om.colidx, vals)
+    AT = oracle.to_scipy(4, 4, om.rowmap, om.colidx, KT.entries())
+    assert abs(AT - A.T).max() <= 1e-12 * abs(A).max()
+    assert rel(pl.projection(u, z).numpy(), om.projection(u, z)) < TOL64
+    w = om.plastic_work(u, 0.5 * u, c, cp, z)
+    assert abs(pl.plastic_work(u, 0.5 * u, c, cp, z) - w) <= TOL64 * abs(w)
