@@ -628,10 +628,12 @@ struct TileMesh {
     const int4* __restrict__ runs;
     const int4* __restrict__ cells;
     const uint2* __restrict__ cslots;
+    const unsigned* __restrict__ hslots;
     const unsigned short* __restrict__ pairs;
     const uint2* __restrict__ units;
     const uint4* __restrict__ ents;
     const int* __restrict__ rowmap;
+    int d0, d1, d2;  // node grid (x, y, z counts; z fastest)
 };
 
 // A tile's first row, the offsets of its schedule's tables and their sizes.
@@ -645,9 +647,33 @@ __device__ inline TileHead tile_head(const TileMesh& t) {
     return TileHead{a.x, a.y, a.z, a.w, c.x, c.y, c.z, c.w, e.x};
 }
 
-// Node ids and (x, y, z, density) records of tile cell c.
-__device__ inline void tile_cell_load(const TileMesh& t, const double4* __restrict__ nrec, const TileHead& h, int c,
-                                      CellEval& ce, double4 x[4]) {
+// The tile's halo (kTileHalo nodes: its row block grown by one node on each
+// side, z fastest) into LDS: (x, y, z, density) records and, with NU > 0, the
+// NU state values of each node.  Halo nodes outside the mesh stay unset; no
+// cell refers to them.
+template <int NU>
+__device__ inline void tile_halo_load(const TileMesh& t, const TileHead& h, const double4* __restrict__ x4,
+                                      const double* __restrict__ z, const double* __restrict__ u, double4* hrec,
+                                      double* hu) {
+    const int i0 = h.base / (t.d1 * t.d2), j0 = (h.base / t.d2) % t.d1, k0 = h.base % t.d2;
+    for (int s = threadIdx.x; s < kTileHalo; s += blockDim.x) {
+        const int a = s / ((kTileY + 2) * (kTileZ + 2)), b = (s / (kTileZ + 2)) % (kTileY + 2), c = s % (kTileZ + 2);
+        const int i = i0 - 1 + a, j = j0 - 1 + b, k = k0 - 1 + c;
+        if (i < 0 || i >= t.d0 || j < 0 || j >= t.d1 || k < 0 || k >= t.d2) continue;
+        const long node = ((long)i * t.d1 + j) * t.d2 + k;
+        double4 r = x4[node];
+        r.w = z[node];
+        hrec[s] = r;
+#pragma unroll
+        for (int q = 0; q < NU; ++q) hu[s * NU + q] = u[node * NU + q];
+    }
+}
+
+// Node ids and (x, y, z, density) of tile cell c from packed node records
+// in global memory (the Jacobian alone: its block phase keeps the LDS busy,
+// so the cell records come through the vector memory path instead).
+__device__ inline void tile_cell_gather(const TileMesh& t, const double4* __restrict__ nrec, const TileHead& h, int c,
+                                        CellEval& ce, double4 x[4]) {
     const int4 d = t.cells[h.cell_off + c];
     ce.nodes[0] = h.base + d.x;
     ce.nodes[1] = h.base + d.y;
@@ -655,6 +681,18 @@ __device__ inline void tile_cell_load(const TileMesh& t, const double4* __restri
     ce.nodes[3] = h.base + d.w;
 #pragma unroll
     for (int n = 0; n < 4; ++n) x[n] = nrec[ce.nodes[n]];
+}
+
+// (x, y, z, density) of tile cell c from the halo; ce.nodes holds the halo
+// slots of its nodes (the index of their state values in the halo).
+__device__ inline void tile_cell_load(const TileMesh& t, const double4* hrec, const TileHead& h, int c,
+                                      CellEval& ce, double4 x[4]) {
+    const unsigned hs = t.hslots[h.cell_off + c];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+        ce.nodes[n] = (int)((hs >> (8 * n)) & 0xffu);
+        x[n] = hrec[ce.nodes[n]];
+    }
 }
 
 __device__ inline void tile_cell_geometry(CellEval& ce, const double4 x[4]) {
@@ -665,14 +703,19 @@ __device__ inline void tile_cell_geometry(CellEval& ce, const double4 x[4]) {
 // Phase 1 over the tile's cells, two per thread (a tile has at most
 // 2 * kTileThreads cells), with both cells' gathers issued before either is
 // used.
-template <class F>
-__device__ inline void tile_cells(const TileMesh& t, const double4* __restrict__ nrec, const TileHead& h, F&& eval) {
+template <bool HALO, class F>
+__device__ inline void tile_cells(const TileMesh& t, const double4* src, const TileHead& h, F&& eval) {
     static_assert(kTileCellsMax <= 2 * kTileThreads, "two cells per thread");
     const int c0 = threadIdx.x, c1 = threadIdx.x + kTileThreads;
     CellEval e0, e1;
     double4 x0[4], x1[4];
-    if (c0 < h.ncells) tile_cell_load(t, nrec, h, c0, e0, x0);
-    if (c1 < h.ncells) tile_cell_load(t, nrec, h, c1, e1, x1);
+    if constexpr (HALO) {
+        if (c0 < h.ncells) tile_cell_load(t, src, h, c0, e0, x0);
+        if (c1 < h.ncells) tile_cell_load(t, src, h, c1, e1, x1);
+    } else {
+        if (c0 < h.ncells) tile_cell_gather(t, src, h, c0, e0, x0);
+        if (c1 < h.ncells) tile_cell_gather(t, src, h, c1, e1, x1);
+    }
     if (c0 < h.ncells) {
         tile_cell_geometry(e0, x0);
         eval(c0, e0);
@@ -741,28 +784,36 @@ constexpr int kResThreads = 512;
 static_assert(kTileCellsMax <= kResThreads && kTileX * kTileY * kTileZ * 8 <= kResThreads, "one cell per thread");
 
 template <int DPN>
-__global__ __launch_bounds__(kResThreads) void k_tile_residual(TileMesh t, const double4* __restrict__ nrec,
-                                                               Material mt, const double* __restrict__ u,
-                                                               double body, double scale, double* __restrict__ R) {
+__global__ __launch_bounds__(kResThreads) void k_tile_residual(TileMesh t, const double4* __restrict__ x4,
+                                                               const double* __restrict__ z, Material mt,
+                                                               const double* __restrict__ u, double body,
+                                                               double scale, int hoff, double* __restrict__ R) {
+    // LDS: the pair slots (rows x 32 x DPN doubles), then at hoff the halo
+    // records and the halo's state values
     extern __shared__ double lds[];
+    double4* hrec = reinterpret_cast<double4*>(lds + hoff);
+    double* hu = lds + hoff + 4 * kTileHalo;
     const TileHead h = tile_head(t);
     // schedule words fetched ahead: this lane's row of phase 2 and its cell's slots
     const int q = threadIdx.x, r = q >> 3, s = q & 7;
     int4 ri = make_int4(0, 0, 0, 0);
     if (r < h.nrows) ri = t.rows[h.row_off + r];
     const int c = threadIdx.x;
+    uint2 sl = make_uint2(0u, 0u);
+    if (c < h.ncells) sl = t.cslots[h.cell_off + c];
+    tile_halo_load<DPN>(t, h, x4, z, u, hrec, hu);
+    __syncthreads();
     if (c < h.ncells) {
-        const uint2 sl = t.cslots[h.cell_off + c];
         CellEval ce;
         double4 x[4];
-        tile_cell_load(t, nrec, h, c, ce, x);
+        tile_cell_load(t, hrec, h, c, ce, x);
         tile_cell_geometry(ce, x);
         const unsigned slot[4] = {sl.x & 0xffffu, sl.x >> 16, sl.y & 0xffffu, sl.y >> 16};
         const double sp = penal(ce.rho, mt.p, mt.emin);
         const double w = sp * (ce.absdet / 6.0);
         if constexpr (DPN == 3) {
             double eps[6], sig[6], out[3];
-            cell_strain(ce, u, eps);
+            cell_strain(ce, hu, eps);
             iso_stress(mt, eps, sig);
 #pragma unroll
             for (int a = 0; a < 4; ++a) {
@@ -777,7 +828,7 @@ __global__ __launch_bounds__(kResThreads) void k_tile_residual(TileMesh t, const
             double gT[3] = {0.0, 0.0, 0.0};
 #pragma unroll
             for (int n = 0; n < 4; ++n) {
-                const double T = u[ce.nodes[n]];
+                const double T = hu[ce.nodes[n]];
                 gT[0] += T * ce.g[n][0];
                 gT[1] += T * ce.g[n][1];
                 gT[2] += T * ce.g[n][2];
@@ -876,9 +927,10 @@ __device__ inline void tile_unit_block(const double* lds, const TileUnitFetch& f
 // a diagonal block sit on adjacent lanes and are summed by a butterfly;
 // blocks are stored as lambda M + mu M^T + mu tr(M) I (thermal: k M).
 template <int DPN, class VT, bool RES, int UE>
-__global__ __launch_bounds__(kTileThreads) void k_tile_jacobian(TileMesh t, const double4* __restrict__ nrec,
-                                                                Material mt, const double* __restrict__ u,
-                                                                double body, int cmax, VT* __restrict__ vals,
+__global__ __launch_bounds__(kTileThreads) void k_tile_jacobian(TileMesh t, const double4* __restrict__ x4,
+                                                                const double* __restrict__ z, Material mt,
+                                                                const double* __restrict__ u, double body, int cmax,
+                                                                int hoff, VT* __restrict__ vals,
                                                                 double* __restrict__ R) {
     constexpr int MS = DPN == 3 ? 9 : 1;
     constexpr int SS = DPN == 3 ? 7 : 5;  // residual data per cell, one pad
@@ -901,7 +953,15 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_jacobian(TileMesh t, cons
         rstage[threadIdx.x] = ri.w;
     }
     if ((int)threadIdx.x < kTileStride) lds[h.ncells * kTileStride + threadIdx.x] = 0.0;  // the zero cell
-    tile_cells(t, nrec, h, [&](int c, const CellEval& ce) {
+    // with the residual: the halo (at hoff, after the cell data), records and
+    // state values; the Jacobian alone gathers packed records (x4 holds them)
+    double4* hrec = reinterpret_cast<double4*>(lds + hoff);
+    double* hu = lds + hoff + 4 * kTileHalo;
+    if constexpr (RES) {
+        tile_halo_load<DPN>(t, h, x4, z, u, hrec, hu);
+        __syncthreads();
+    }
+    tile_cells<RES>(t, RES ? hrec : x4, h, [&](int c, const CellEval& ce) {
         const double s = penal(ce.rho, mt.p, mt.emin);
         const double w = s * (ce.absdet / 6.0);
         const double sw = sqrt(w);
@@ -916,7 +976,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_jacobian(TileMesh t, cons
             double* Sc = S + c * SS;
             if constexpr (DPN == 3) {
                 double eps[6], sig[6];
-                cell_strain(ce, u, eps);
+                cell_strain(ce, hu, eps);
                 iso_stress(mt, eps, sig);
 #pragma unroll
                 for (int v = 0; v < 6; ++v) Sc[v] = sw * sig[v];
@@ -924,7 +984,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_jacobian(TileMesh t, cons
                 double gT[3] = {0.0, 0.0, 0.0};
 #pragma unroll
                 for (int n = 0; n < 4; ++n) {
-                    const double T = u[ce.nodes[n]];
+                    const double T = hu[ce.nodes[n]];
                     gT[0] += T * ce.g[n][0];
                     gT[1] += T * ce.g[n][1];
                     gT[2] += T * ce.g[n][2];
@@ -1617,10 +1677,14 @@ TileMesh tile_mesh(const pa_mesh* m) {
     t.runs = m->d_tp_run.p;
     t.cells = m->d_tp_cell.p;
     t.cslots = m->d_tp_cslot.p;
+    t.hslots = m->d_tp_hslot.p;
     t.pairs = m->d_tp_pair.p;
     t.units = m->d_tp_unit.p;
     t.ents = reinterpret_cast<const uint4*>(m->d_tp_ent.p);
     t.rowmap = m->d_rowmap.p;
+    t.d0 = m->grid_dims[0];
+    t.d1 = m->grid_dims[1];
+    t.d2 = m->grid_dims[2];
     return t;
 }
 
@@ -1705,14 +1769,17 @@ const double4* pack_nodes(const pa_mesh* m, const double* z, hipStream_t st) {
 void residual_rows(const pa_mesh* m, int dpn, const Material& mt, const double* u, const double* z, double body,
                    double scale, double* R, hipStream_t st) {
     if (m->has_tiles && m->tile_rows_max * 8 <= kResThreads) {
-        const size_t lds = (size_t)m->tile_rows_max * 32 * dpn * sizeof(double);
+        const int hoff = m->tile_rows_max * 32 * dpn;  // doubles; a multiple of 32 keeps the records aligned
+        const size_t lds = (size_t)(hoff + (4 + dpn) * kTileHalo) * sizeof(double);
         const size_t ok = dpn == 3 ? tile_lds(k_tile_residual<3>, lds) : tile_lds(k_tile_residual<1>, lds);
         if (ok) {
-            const double4* rec = pack_nodes(m, z, st);
+            const double4* x4 = reinterpret_cast<const double4*>(m->d_coords4.p);
             if (dpn == 3)
-                k_tile_residual<3><<<m->num_tiles, kResThreads, lds, st>>>(tile_mesh(m), rec, mt, u, body, scale, R);
+                k_tile_residual<3><<<m->num_tiles, kResThreads, lds, st>>>(tile_mesh(m), x4, z, mt, u, body, scale,
+                                                                           hoff, R);
             else
-                k_tile_residual<1><<<m->num_tiles, kResThreads, lds, st>>>(tile_mesh(m), rec, mt, u, body, scale, R);
+                k_tile_residual<1><<<m->num_tiles, kResThreads, lds, st>>>(tile_mesh(m), x4, z, mt, u, body, scale,
+                                                                           hoff, R);
             check_launch();
             return;
         }
@@ -1746,12 +1813,16 @@ void jacobian(const pa_vector_function* vf, const double* u, const double* z, VT
         const double body = vf->has_body ? vf->body_value : 0.0;
         const int ss = !RES ? 0 : (vf->dpn == 3 ? 7 : 5);
         const int cmax = m->tile_cells_max + 1;  // + the zero cell
-        const size_t lds = std::max((size_t)cmax * (kTileStride + ss) * sizeof(double),
+        // cell data, then (with the residual) the halo at an even, 16-byte
+        // aligned offset; without it the kernel gathers packed node records
+        const int hoff = (cmax * (kTileStride + ss) + 1) & ~1;
+        const int halo = RES ? (4 + vf->dpn) * kTileHalo : 0;
+        const size_t lds = std::max((size_t)(hoff + halo) * sizeof(double),
                                     (size_t)m->tile_blocks_max * vf->dpn * vf->dpn * sizeof(VT));
         auto run = [&](auto kernel, double b) {
             if (!tile_lds(kernel, lds)) return false;
-            const double4* rec = pack_nodes(m, z, st);
-            kernel<<<m->num_tiles, kTileThreads, lds, st>>>(tile_mesh(m), rec, mt, u, b, cmax, vals, R);
+            const double4* x4 = RES ? reinterpret_cast<const double4*>(m->d_coords4.p) : pack_nodes(m, z, st);
+            kernel<<<m->num_tiles, kTileThreads, lds, st>>>(tile_mesh(m), x4, z, mt, u, b, cmax, hoff, vals, R);
             check_launch();
             if (RES) natural_bcs(vf, R, st);
             return true;

@@ -165,6 +165,10 @@ static bool detect_grid(const double* xyz, int N, int d[3]) {
 //   runs   rows with consecutive ids (adjacent CRS ranges): first row index,
 //          staging offset, blocks
 //   cells  node ids - first row of each distinct incident cell, sorted
+//   hslots per cell, the halo slots of its 4 nodes (8 bits each): the tile's
+//          rows span a kTileX x kTileY x kTileZ block of the node grid and
+//          every cell node lies in the block grown by one node on each side
+//          (kTileHalo nodes, numbered like the grid, z fastest)
 //   pairs  per row and incident cell (incidence order): tile cell << 2 | a
 //   units  (row, column slot) blocks: x = row | slot << 8 | mirror row << 16
 //          | mirror slot << 24, y = entry offset (16-byte groups) | count << 22
@@ -181,6 +185,7 @@ struct TileSchedule {
     std::vector<int4> rows, cells, runs;
     std::vector<unsigned short> pairs;
     std::vector<uint2> cslots;  // per cell: LDS pair slot (row * 32 + k) of its 4 nodes, 0xffff if not a row
+    std::vector<unsigned> hslots;  // per cell: halo slots of its 4 nodes, 8 bits each
     std::vector<uint2> units;
     std::vector<unsigned> ents;
     int max_count = 0;
@@ -188,7 +193,8 @@ struct TileSchedule {
 };
 
 static bool tile_schedule(const pa_mesh* m, const std::vector<int>& conn, const std::vector<int>& off,
-                          const std::vector<int2>& pairs, const std::vector<int>& rows, TileSchedule& t) {
+                          const std::vector<int2>& pairs, const std::vector<int>& rows, const int d[3],
+                          TileSchedule& t) {
     const int base = rows[0];
     std::vector<int> cells;
     for (int r : rows)
@@ -202,10 +208,19 @@ static bool tile_schedule(const pa_mesh* m, const std::vector<int>& conn, const 
     };
     std::sort(cells.begin(), cells.end(), [&](int a, int b) { return key(a) < key(b); });
     std::vector<std::pair<int, int>> tc_of;  // (cell id, tile cell index)
+    const int i0 = base / (d[1] * d[2]), j0 = (base / d[2]) % d[1], k0 = base % d[2];
     for (int i = 0; i < (int)cells.size(); ++i) {
         const auto k = key(cells[i]);
         t.cells.push_back(make_int4(k[0], k[1], k[2], k[3]));
         tc_of.push_back({cells[i], i});
+        unsigned hs = 0;
+        for (int n = 0; n < 4; ++n) {
+            const int node = conn[4L * cells[i] + n];
+            const int a = node / (d[1] * d[2]) - i0 + 1, b = (node / d[2]) % d[1] - j0 + 1, c = node % d[2] - k0 + 1;
+            if (a < 0 || a >= kTileX + 2 || b < 0 || b >= kTileY + 2 || c < 0 || c >= kTileZ + 2) return false;
+            hs |= (unsigned)((a * (kTileY + 2) + b) * (kTileZ + 2) + c) << (8 * n);
+        }
+        t.hslots.push_back(hs);
     }
     std::sort(tc_of.begin(), tc_of.end());
     auto tc = [&](int cell) { return std::lower_bound(tc_of.begin(), tc_of.end(), std::make_pair(cell, -1))->second; };
@@ -314,6 +329,7 @@ static bool tile_schedule(const pa_mesh* m, const std::vector<int>& conn, const 
     put(t.rows.data(), t.rows.size() * sizeof(int4));
     put(t.runs.data(), t.runs.size() * sizeof(int4));
     put(t.cells.data(), t.cells.size() * sizeof(int4));
+    put(t.hslots.data(), t.hslots.size() * sizeof(unsigned));
     put(t.pairs.data(), t.pairs.size() * sizeof(unsigned short));
     put(t.units.data(), t.units.size() * sizeof(uint2));
     put(t.ents.data(), t.ents.size() * sizeof(unsigned));
@@ -338,6 +354,7 @@ static void build_tiles(pa_mesh* m, const double* xyz, const std::vector<int>& c
     std::unordered_map<std::string, int> index;
     std::vector<int4> tiles(3 * nt), info, trow, tcell, trun;
     std::vector<uint2> tcslot;
+    std::vector<unsigned> thslot;
     std::vector<unsigned short> tpair;
     std::vector<uint2> tunit;
     std::vector<unsigned> tent;
@@ -359,7 +376,7 @@ static void build_tiles(pa_mesh* m, const double* xyz, const std::vector<int>& c
                         for (int k = f[2]; k < std::min(f[2] + kTileZ, d[2]); ++k)
                             rows.push_back((i * d[1] + j) * d[2] + k);
                 sched[q] = TileSchedule();
-                ok[q] = tile_schedule(m, conn, off, pairs, rows, sched[q]);
+                ok[q] = tile_schedule(m, conn, off, pairs, rows, d, sched[q]);
             }
         }, 64);
         for (long q = 0; q < nb; ++q) {
@@ -383,6 +400,7 @@ static void build_tiles(pa_mesh* m, const double* xyz, const std::vector<int>& c
                 for (auto r : s.rows) trow.push_back(make_int4(r.x, r.y, r.z + poff, r.w));
                 tcell.insert(tcell.end(), s.cells.begin(), s.cells.end());
                 tcslot.insert(tcslot.end(), s.cslots.begin(), s.cslots.end());
+                thslot.insert(thslot.end(), s.hslots.begin(), s.hslots.end());
                 tpair.insert(tpair.end(), s.pairs.begin(), s.pairs.end());
                 tunit.insert(tunit.end(), s.units.begin(), s.units.end());
                 tent.insert(tent.end(), s.ents.begin(), s.ents.end());
@@ -407,6 +425,7 @@ static void build_tiles(pa_mesh* m, const double* xyz, const std::vector<int>& c
     m->d_tp_row.upload(trow.data(), trow.size());
     m->d_tp_cell.upload(tcell.data(), tcell.size());
     m->d_tp_cslot.upload(tcslot.data(), tcslot.size());
+    m->d_tp_hslot.upload(thslot.data(), thslot.size());
     m->d_tp_run.upload(trun.data(), trun.size());
     m->d_tp_pair.upload(tpair.data(), tpair.size());
     m->d_tp_unit.upload(tunit.data(), tunit.size());

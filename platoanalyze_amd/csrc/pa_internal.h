@@ -88,6 +88,7 @@ struct SideSet {
 
 // Cell tiles (shared by the schedule builder and the assembly kernels).
 constexpr int kTileX = 2, kTileY = 4, kTileZ = 4;  // rows per tile along the node grid
+constexpr int kTileHalo = (kTileX + 2) * (kTileY + 2) * (kTileZ + 2);  // nodes a tile's cells reach
 constexpr int kTileStride = 13;     // doubles per cell in LDS: sqrt(w) g_n for n = 0..3, one pad
 constexpr int kTileCellsMax = 512;  // cells per tile the kernels' LDS is sized for
 constexpr int kTilePatternsMax = 4096;
@@ -145,6 +146,7 @@ struct pa_mesh {
                                       // staging offset, blocks, 0}
     pa::DevBuf<int4> d_tp_cell;       // per pattern cell: its four node ids - first row
     pa::DevBuf<uint2> d_tp_cslot;     // per pattern cell: residual pair slot (row * 32 + k) of each node
+    pa::DevBuf<unsigned> d_tp_hslot;  // per tile cell: halo slots of its 4 nodes
                                       // (16 bits each, 0xffff for a node that is not a tile row)
     pa::DevBuf<unsigned short> d_tp_pair;  // per (row, incident cell): tile cell << 2 | local index of the row node
     pa::DevBuf<uint2> d_tp_unit;      // (row, column slot) work units: {row | slot << 8 | mirror row << 16 |
