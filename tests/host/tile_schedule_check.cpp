@@ -159,6 +159,23 @@ int main(int argc, char** argv) {
         for (int c = 0; c < ncells; ++c) {
             const int4 d = m->d_tp_cell.p[h0.z + c];
             const int nd[4] = {base + d.x, base + d.y, base + d.z, base + d.w};
+            // the halo slots the residual kernels read the nodes through
+            {
+                const int d1 = m->grid_dims[1], d2 = m->grid_dims[2];
+                const int i0 = base / (d1 * d2), j0 = (base / d2) % d1, k0 = base % d2;
+                const unsigned hs = m->d_tp_hslot.p[h0.z + c];
+                for (int n = 0; n < 4; ++n) {
+                    const int slot = (int)((hs >> (8 * n)) & 0xffu);
+                    CHECK(slot < kTileHalo, "tile %d cell %d: halo slot %d out of range", b, c, slot);
+                    const int a = slot / ((kTileY + 2) * (kTileZ + 2)), bb = (slot / (kTileZ + 2)) % (kTileY + 2),
+                              cc = slot % (kTileZ + 2);
+                    const int i = i0 - 1 + a, j = j0 - 1 + bb, k = k0 - 1 + cc;
+                    CHECK(i >= 0 && i < m->grid_dims[0] && j >= 0 && j < d1 && k >= 0 && k < d2,
+                          "tile %d cell %d: halo node outside the mesh", b, c);
+                    CHECK((i * d1 + j) * d2 + k == nd[n], "tile %d cell %d: halo slot names node %d, not %d", b, c,
+                          (i * d1 + j) * d2 + k, nd[n]);
+                }
+            }
             eval(nd, cells[c]);
             const double sw = std::sqrt(cells[c].w);
             for (int n = 0; n < 4; ++n)
