@@ -323,8 +323,10 @@ def j2_plasticity(pa, stream, with_cpu=True, formulation="mixed"):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # the GPU ramps its clocks over the first milliseconds of work: the default
+    # untimed warm-up (about 18 ms) lets the timed steps run at steady state
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-solve", action="store_true", help="skip the compliance + gradient leg")
     ap.add_argument("--no-j2", action="store_true", help="skip the J2 plasticity legs")

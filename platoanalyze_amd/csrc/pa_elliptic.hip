@@ -614,8 +614,8 @@ __global__ __launch_bounds__(kBlock) void k_jacobian_pattern(RowMesh m, const do
 //  phase 1: every distinct cell incident to the tile's rows is evaluated
 //           once into LDS: the Jacobian kernel parks sqrt(w) g_n (w = s(rho)
 //           |det J| / 6), so a block is M = sum g_a (x) g_b over the cells
-//           of its edge; the residual kernel parks the four nodes' cell
-//           residuals.
+//           of its edge; the residual kernel (which first stages the tile's
+//           node halo in LDS) parks the cell's residuals of the tile's rows.
 //  phase 2: work units (row, column slot) sum their block from LDS and store
 //           it; 8 lanes per row sum the residual (the addition tree of a
 //           32-lane butterfly over the incident cells).
@@ -703,19 +703,14 @@ __device__ inline void tile_cell_geometry(CellEval& ce, const double4 x[4]) {
 // Phase 1 over the tile's cells, two per thread (a tile has at most
 // 2 * kTileThreads cells), with both cells' gathers issued before either is
 // used.
-template <bool HALO, class F>
-__device__ inline void tile_cells(const TileMesh& t, const double4* src, const TileHead& h, F&& eval) {
+template <class F>
+__device__ inline void tile_cells(const TileMesh& t, const double4* __restrict__ nrec, const TileHead& h, F&& eval) {
     static_assert(kTileCellsMax <= 2 * kTileThreads, "two cells per thread");
     const int c0 = threadIdx.x, c1 = threadIdx.x + kTileThreads;
     CellEval e0, e1;
     double4 x0[4], x1[4];
-    if constexpr (HALO) {
-        if (c0 < h.ncells) tile_cell_load(t, src, h, c0, e0, x0);
-        if (c1 < h.ncells) tile_cell_load(t, src, h, c1, e1, x1);
-    } else {
-        if (c0 < h.ncells) tile_cell_gather(t, src, h, c0, e0, x0);
-        if (c1 < h.ncells) tile_cell_gather(t, src, h, c1, e1, x1);
-    }
+    if (c0 < h.ncells) tile_cell_gather(t, nrec, h, c0, e0, x0);
+    if (c1 < h.ncells) tile_cell_gather(t, nrec, h, c1, e1, x1);
     if (c0 < h.ncells) {
         tile_cell_geometry(e0, x0);
         eval(c0, e0);
@@ -723,54 +718,6 @@ __device__ inline void tile_cells(const TileMesh& t, const double4* src, const T
     if (c1 < h.ncells) {
         tile_cell_geometry(e1, x1);
         eval(c1, e1);
-    }
-}
-
-// Residual rows: 8 lanes per row; lane s takes incident cells s, s + 16,
-// s + 8, s + 24 (absent ones as zero) and the lanes are combined by a
-// butterfly, which reproduces a 32-lane butterfly's addition tree.  The
-// row's pair entries are fetched ahead (before the cells are evaluated).
-struct TileRowFetch {
-    int row_delta;
-    unsigned pe[4];  // tile cell << 2 | a, or ~0u for an absent cell
-};
-
-__device__ inline void tile_row_fetch(const TileMesh& t, const TileHead& h, int q, TileRowFetch& f) {
-    const int r = q >> 3, s = q & 7;
-    const int4 ri = t.rows[h.row_off + r];
-    f.row_delta = ri.x;
-    const int ps[4] = {s, s + 16, s + 8, s + 24};
-#pragma unroll
-    for (int j = 0; j < 4; ++j) f.pe[j] = ps[j] < ri.y ? (unsigned)t.pairs[ri.z + ps[j]] : ~0u;
-}
-
-template <int DPN, class F>
-__device__ inline void tile_residual_rows(const TileMesh& t, const TileHead& h, int q, TileRowFetch& f, double scale,
-                                          double* __restrict__ R, F&& contrib) {
-    while (q < h.nrows * 8) {
-        double v[4][DPN];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            if (f.pe[j] != ~0u) {
-                contrib(f.pe[j], v[j]);
-            } else {
-#pragma unroll
-                for (int i = 0; i < DPN; ++i) v[j][i] = 0.0;
-            }
-        }
-        double b[DPN];
-#pragma unroll
-        for (int i = 0; i < DPN; ++i) {
-            b[i] = (v[0][i] + v[1][i]) + (v[2][i] + v[3][i]);
-#pragma unroll
-            for (int o = 4; o > 0; o >>= 1) b[i] += __shfl_xor(b[i], o, 8);
-        }
-        if ((q & 7) == 0) {
-#pragma unroll
-            for (int i = 0; i < DPN; ++i) R[(long)(h.base + f.row_delta) * DPN + i] = scale * b[i];
-        }
-        q += kTileThreads;
-        if (q < h.nrows * 8) tile_row_fetch(t, h, q, f);
     }
 }
 
@@ -920,48 +867,33 @@ __device__ inline void tile_unit_block(const double* lds, const TileUnitFetch& f
     }
 }
 
-// Jacobian (optionally with the residual) of one tile.  Phase 1 parks
-// sqrt(w) g_n per cell (kTileStride doubles), with the residual also
-// sqrt(w) sigma (thermal: sqrt(w) grad T and the body load term).  Phase 2:
-// each thread evaluates up to three work units; the kTileDiagParts parts of
-// a diagonal block sit on adjacent lanes and are summed by a butterfly;
-// blocks are stored as lambda M + mu M^T + mu tr(M) I (thermal: k M).
-template <int DPN, class VT, bool RES, int UE>
-__global__ __launch_bounds__(kTileThreads) void k_tile_jacobian(TileMesh t, const double4* __restrict__ x4,
-                                                                const double* __restrict__ z, Material mt,
-                                                                const double* __restrict__ u, double body, int cmax,
-                                                                int hoff, VT* __restrict__ vals,
-                                                                double* __restrict__ R) {
+// Jacobian of one tile.  Phase 1 parks sqrt(w) g_n per cell (kTileStride
+// doubles), from packed node records gathered through the vector memory
+// path (the block phase keeps the LDS busy).  Phase 2: each thread evaluates
+// up to three work units; the kTileDiagParts parts of a diagonal block sit on
+// adjacent lanes and are summed by a butterfly; blocks are stored as
+// lambda M + mu M^T + mu tr(M) I (thermal: k M).
+template <int DPN, class VT, int UE>
+__global__ __launch_bounds__(kTileThreads) void k_tile_jacobian(TileMesh t, const double4* __restrict__ nrec,
+                                                                Material mt, int cmax, VT* __restrict__ vals) {
     constexpr int MS = DPN == 3 ? 9 : 1;
-    constexpr int SS = DPN == 3 ? 7 : 5;  // residual data per cell, one pad
     constexpr int NU = kTileUnitsMax / kTileThreads;
     static_assert(kTileStride <= kTileThreads && kTileDiagParts == 4, "zero cell by one pass; 4-lane butterfly");
     extern __shared__ double lds[];
     __shared__ int rstart[64], rstage[64];  // rowmap and staging offset of the tile's rows
-    double* S = lds + (long)cmax * kTileStride;
     const TileHead h = tile_head(t);
-    // the thread's units and residual row are fetched while the cells load
+    // the thread's units are fetched while the cells load
     TileUnitFetch f[NU];
 #pragma unroll
     for (int k = 0; k < NU; ++k)
         if ((int)threadIdx.x + k * kTileThreads < h.nunits) tile_unit_fetch(t, h, threadIdx.x + k * kTileThreads, f[k]);
-    TileRowFetch rf;
-    if (RES && (int)threadIdx.x < h.nrows * 8) tile_row_fetch(t, h, threadIdx.x, rf);
     if ((int)threadIdx.x < h.nrows) {
         const int4 ri = t.rows[h.row_off + threadIdx.x];
         rstart[threadIdx.x] = t.rowmap[h.base + ri.x];
         rstage[threadIdx.x] = ri.w;
     }
     if ((int)threadIdx.x < kTileStride) lds[h.ncells * kTileStride + threadIdx.x] = 0.0;  // the zero cell
-    // with the residual: the halo (at hoff, after the cell data), records and
-    // state values; the Jacobian alone gathers packed records (x4 holds them)
-    double4* hrec = reinterpret_cast<double4*>(lds + hoff);
-    double* hu = lds + hoff + 4 * kTileHalo;
-    if constexpr (RES) {
-        tile_halo_load<DPN>(t, h, x4, z, u, hrec, hu);
-        __syncthreads();
-    }
-    tile_cells<RES>(t, RES ? hrec : x4, h, [&](int c, const CellEval& ce) {
+    tile_cells(t, nrec, h, [&](int c, const CellEval& ce) {
         const double s = penal(ce.rho, mt.p, mt.emin);
         const double w = s * (ce.absdet / 6.0);
         const double sw = sqrt(w);
@@ -972,44 +904,8 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_jacobian(TileMesh t, cons
             Q[3 * n + 1] = sw * ce.g[n][1];
             Q[3 * n + 2] = sw * ce.g[n][2];
         }
-        if constexpr (RES) {
-            double* Sc = S + c * SS;
-            if constexpr (DPN == 3) {
-                double eps[6], sig[6];
-                cell_strain(ce, hu, eps);
-                iso_stress(mt, eps, sig);
-#pragma unroll
-                for (int v = 0; v < 6; ++v) Sc[v] = sw * sig[v];
-            } else {
-                double gT[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-                for (int n = 0; n < 4; ++n) {
-                    const double T = hu[ce.nodes[n]];
-                    gT[0] += T * ce.g[n][0];
-                    gT[1] += T * ce.g[n][1];
-                    gT[2] += T * ce.g[n][2];
-                }
-                Sc[0] = sw * gT[0];
-                Sc[1] = sw * gT[1];
-                Sc[2] = sw * gT[2];
-                Sc[3] = -1.0 * (ce.absdet / 6.0) * body * 0.25 * ce.rho;
-            }
-        }
     });
     __syncthreads();
-    if constexpr (RES) {
-        tile_residual_rows<DPN>(t, h, threadIdx.x, rf, 1.0, R, [&](unsigned e, double* v) {
-            const double* ga = lds + (e >> 2) * kTileStride + 3 * (e & 3);
-            const double* Sc = S + (e >> 2) * SS;
-            if constexpr (DPN == 3) {
-                btsig(Sc, ga, v);
-            } else {
-                double r = mt.k * (Sc[0] * ga[0] + Sc[1] * ga[1] + Sc[2] * ga[2]);
-                r += Sc[3];
-                v[0] = r;
-            }
-        });
-    }
     // blocks are kept in registers, then staged in LDS (the cell data is
     // dead by then) in CRS order and written run by run with contiguous stores
     double K[NU][MS];
@@ -1804,36 +1700,34 @@ void residual_rows(const pa_mesh* m, int dpn, const Material& mt, const double* 
     check_launch();
 }
 
+// Tile-kernel Jacobian; false when the mesh has no tiles or the device
+// cannot give the kernel its LDS.  Self-adjoint operators: the transposed
+// assembly is the direct one.
+template <class VT>
+bool tile_jacobian(const pa_vector_function* vf, const double* z, VT* vals, hipStream_t st) {
+    const pa_mesh* m = vf->mesh;
+    if (!m->has_tiles) return false;
+    const Material mt = material_of(vf->physics, vf->material, vf->penalty);
+    const int cmax = m->tile_cells_max + 1;  // + the zero cell
+    const size_t lds = std::max((size_t)cmax * kTileStride * sizeof(double),
+                                (size_t)m->tile_blocks_max * vf->dpn * vf->dpn * sizeof(VT));
+    auto run = [&](auto kernel) {
+        if (!tile_lds(kernel, lds)) return false;
+        const double4* rec = pack_nodes(m, z, st);
+        kernel<<<m->num_tiles, kTileThreads, lds, st>>>(tile_mesh(m), rec, mt, cmax, vals);
+        check_launch();
+        return true;
+    };
+    const bool six = m->tile_unit_entries <= 6;
+    if (vf->dpn == 3) return six ? run(k_tile_jacobian<3, VT, 6>) : run(k_tile_jacobian<3, VT, 8>);
+    return six ? run(k_tile_jacobian<1, VT, 6>) : run(k_tile_jacobian<1, VT, 8>);
+}
+
 template <class VT, bool RES, bool TR>
 void jacobian(const pa_vector_function* vf, const double* u, const double* z, VT* vals, double* R, hipStream_t st) {
     const pa_mesh* m = vf->mesh;
-    if (m->has_tiles) {
-        // self-adjoint operators: the transposed assembly is the direct one
-        const Material mt = material_of(vf->physics, vf->material, vf->penalty);
-        const double body = vf->has_body ? vf->body_value : 0.0;
-        const int ss = !RES ? 0 : (vf->dpn == 3 ? 7 : 5);
-        const int cmax = m->tile_cells_max + 1;  // + the zero cell
-        // cell data, then (with the residual) the halo at an even, 16-byte
-        // aligned offset; without it the kernel gathers packed node records
-        const int hoff = (cmax * (kTileStride + ss) + 1) & ~1;
-        const int halo = RES ? (4 + vf->dpn) * kTileHalo : 0;
-        const size_t lds = std::max((size_t)(hoff + halo) * sizeof(double),
-                                    (size_t)m->tile_blocks_max * vf->dpn * vf->dpn * sizeof(VT));
-        auto run = [&](auto kernel, double b) {
-            if (!tile_lds(kernel, lds)) return false;
-            const double4* x4 = RES ? reinterpret_cast<const double4*>(m->d_coords4.p) : pack_nodes(m, z, st);
-            kernel<<<m->num_tiles, kTileThreads, lds, st>>>(tile_mesh(m), x4, z, mt, u, b, cmax, hoff, vals, R);
-            check_launch();
-            if (RES) natural_bcs(vf, R, st);
-            return true;
-        };
-        const bool six = m->tile_unit_entries <= 6;
-        bool done;
-        if (vf->dpn == 3)
-            done = six ? run(k_tile_jacobian<3, VT, RES, 6>, 0.0) : run(k_tile_jacobian<3, VT, RES, 8>, 0.0);
-        else
-            done = six ? run(k_tile_jacobian<1, VT, RES, 6>, body) : run(k_tile_jacobian<1, VT, RES, 8>, body);
-        if (done) return;
+    if constexpr (!RES) {
+        if (tile_jacobian(vf, z, vals, st)) return;
     }
     if (m->has_patterns) {
         // The operators are self-adjoint: the transposed assembly equals the
@@ -2022,6 +1916,18 @@ int pa_vector_function_value_gradient_u(const pa_vector_function* vf, const doub
     return guarded([&] {
         pa::check_vf(vf, u, z, vals);
         PA_REQUIRE(R, PA_ERR_ARG, "value_gradient_u: null residual");
+        PA_REQUIRE(precision == PA_FP64 || precision == PA_FP32, PA_ERR_ARG, "value_gradient_u: unknown precision");
+        if (vf->mesh->has_tiles) {
+            // on tile meshes the residual kernel and the Jacobian kernel one
+            // after the other beat one fused pass (whose cell data halves the
+            // workgroups per CU)
+            pa::residual(vf, u, z, R, (hipStream_t)stream);
+            if (precision == PA_FP64)
+                pa::jacobian<double, false, false>(vf, u, z, (double*)vals, nullptr, (hipStream_t)stream);
+            else
+                pa::jacobian<float, false, false>(vf, u, z, (float*)vals, nullptr, (hipStream_t)stream);
+            return;
+        }
         if (precision == PA_FP64)
             pa::jacobian<double, true, false>(vf, u, z, (double*)vals, R, (hipStream_t)stream);
         else if (precision == PA_FP32)
