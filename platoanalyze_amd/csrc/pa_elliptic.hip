@@ -703,14 +703,19 @@ __device__ inline void tile_cell_geometry(CellEval& ce, const double4 x[4]) {
 // Phase 1 over the tile's cells, two per thread (a tile has at most
 // 2 * kTileThreads cells), with both cells' gathers issued before either is
 // used.
-template <class F>
-__device__ inline void tile_cells(const TileMesh& t, const double4* __restrict__ nrec, const TileHead& h, F&& eval) {
+template <bool HALO, class F>
+__device__ inline void tile_cells(const TileMesh& t, const double4* src, const TileHead& h, F&& eval) {
     static_assert(kTileCellsMax <= 2 * kTileThreads, "two cells per thread");
     const int c0 = threadIdx.x, c1 = threadIdx.x + kTileThreads;
     CellEval e0, e1;
     double4 x0[4], x1[4];
-    if (c0 < h.ncells) tile_cell_gather(t, nrec, h, c0, e0, x0);
-    if (c1 < h.ncells) tile_cell_gather(t, nrec, h, c1, e1, x1);
+    if constexpr (HALO) {
+        if (c0 < h.ncells) tile_cell_load(t, src, h, c0, e0, x0);
+        if (c1 < h.ncells) tile_cell_load(t, src, h, c1, e1, x1);
+    } else {
+        if (c0 < h.ncells) tile_cell_gather(t, src, h, c0, e0, x0);
+        if (c1 < h.ncells) tile_cell_gather(t, src, h, c1, e1, x1);
+    }
     if (c0 < h.ncells) {
         tile_cell_geometry(e0, x0);
         eval(c0, e0);
@@ -825,6 +830,16 @@ __device__ inline void tile_unit_fetch(const TileMesh& t, const TileHead& h, int
     for (int k = 0; k < kTileUnitEntries / 4; ++k) f.ew[k] = t.ents[(f.unit.y & 0x3fffffu) + k];
 }
 
+// Slot of the product for node pair (a, b) among the ten a <= b pairs of a
+// cell, in the order (0,0) (0,1) (0,2) (0,3) (1,1) (1,2) (1,3) (2,2) (2,3) (3,3).
+// The thermal Jacobian parks a cell's ten products at this stride (the
+// schedule's entry slots still count cells in kTileStride units).
+constexpr int kThermalStride = 10;
+__device__ inline unsigned thermal_pair_index(unsigned a, unsigned b) {
+    const unsigned lo = a < b ? a : b, hi = a < b ? b : a;
+    return lo * 4 - (lo * (lo - 1)) / 2 + (hi - lo);
+}
+
 // M = sum over a unit's first NE entry slots of g_a (x) g_b (gradients
 // scaled by sqrt(w)); slots beyond the unit's count hold the zero cell, so
 // the loop has no branches and its LDS reads pipeline.  SAME: the entries are
@@ -839,24 +854,24 @@ __device__ inline void tile_unit_block(const double* lds, const TileUnitFetch& f
     for (int e = 0; e < NE; ++e) {
         const uint4 w4 = f.ew[e / 4];
         const unsigned en = (e & 3) == 0 ? w4.x : (e & 3) == 1 ? w4.y : (e & 3) == 2 ? w4.z : w4.w;
-        const double* A = lds + (en & 0xffffu);
-        const double a0 = A[0], a1 = A[1], a2 = A[2];
-        if constexpr (SAME) {
-            if constexpr (DPN == 3) {
+        if constexpr (DPN == 1) {
+            // the entry's slots name g_a and g_b of one cell (cell * kTileStride
+            // + 3 a); the cell parked the product of the pair instead
+            const unsigned oa = en & 0xffffu, ob = en >> 16;
+            const unsigned c = oa / kTileStride, a = (oa - c * kTileStride) / 3, b = (ob - c * kTileStride) / 3;
+            M[0] += lds[c * kThermalStride + thermal_pair_index(a, b)];
+        } else {
+            const double* A = lds + (en & 0xffffu);
+            const double a0 = A[0], a1 = A[1], a2 = A[2];
+            if constexpr (SAME) {
                 M[0] += a0 * a0; M[1] += a0 * a1; M[2] += a0 * a2;
                 M[4] += a1 * a1; M[5] += a1 * a2; M[8] += a2 * a2;
             } else {
-                M[0] += (a0 * a0 + a1 * a1) + a2 * a2;
-            }
-        } else {
-            const double* B = lds + (en >> 16);
-            const double b0 = B[0], b1 = B[1], b2 = B[2];
-            if constexpr (DPN == 3) {
+                const double* B = lds + (en >> 16);
+                const double b0 = B[0], b1 = B[1], b2 = B[2];
                 M[0] += a0 * b0; M[1] += a0 * b1; M[2] += a0 * b2;
                 M[3] += a1 * b0; M[4] += a1 * b1; M[5] += a1 * b2;
                 M[6] += a2 * b0; M[7] += a2 * b1; M[8] += a2 * b2;
-            } else {
-                M[0] += (a0 * b0 + a1 * b1) + a2 * b2;
             }
         }
     }
@@ -868,14 +883,20 @@ __device__ inline void tile_unit_block(const double* lds, const TileUnitFetch& f
 }
 
 // Jacobian of one tile.  Phase 1 parks sqrt(w) g_n per cell (kTileStride
-// doubles), from packed node records gathered through the vector memory
-// path (the block phase keeps the LDS busy).  Phase 2: each thread evaluates
+// doubles), from packed node records gathered through the vector memory path
+// (the 3 x 3 block phase keeps the LDS busy); thermal: the ten products
+// w g_a . g_b per cell, from the tile's node halo staged in LDS (its block
+// phase reads one product per entry and leaves the LDS room).  Phase 2: each thread evaluates
 // up to three work units; the kTileDiagParts parts of a diagonal block sit on
 // adjacent lanes and are summed by a butterfly; blocks are stored as
 // lambda M + mu M^T + mu tr(M) I (thermal: k M).
 template <int DPN, class VT, int UE>
-__global__ __launch_bounds__(kTileThreads) void k_tile_jacobian(TileMesh t, const double4* __restrict__ nrec,
-                                                                Material mt, int cmax, VT* __restrict__ vals) {
+__global__ __launch_bounds__(kTileThreads) void k_tile_jacobian(TileMesh t, const double4* __restrict__ nodes,
+                                                                const double* __restrict__ z, Material mt, int cmax,
+                                                                VT* __restrict__ vals) {
+    // nodes: packed (x, y, z, density) records (DPN 3), or the padded
+    // coordinates read with z into the halo (DPN 1)
+    constexpr bool HALO = DPN == 1;
     constexpr int MS = DPN == 3 ? 9 : 1;
     constexpr int NU = kTileUnitsMax / kTileThreads;
     static_assert(kTileStride <= kTileThreads && kTileDiagParts == 4, "zero cell by one pass; 4-lane butterfly");
@@ -892,17 +913,34 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_jacobian(TileMesh t, cons
         rstart[threadIdx.x] = t.rowmap[h.base + ri.x];
         rstage[threadIdx.x] = ri.w;
     }
-    if ((int)threadIdx.x < kTileStride) lds[h.ncells * kTileStride + threadIdx.x] = 0.0;  // the zero cell
-    tile_cells(t, nrec, h, [&](int c, const CellEval& ce) {
+    constexpr int CS = DPN == 3 ? kTileStride : kThermalStride;  // LDS doubles per cell
+    if ((int)threadIdx.x < CS) lds[h.ncells * CS + threadIdx.x] = 0.0;  // the zero cell
+    double4* hrec = reinterpret_cast<double4*>(lds + ((cmax * CS + 1) & ~1));  // after the cells, 16-byte aligned
+    if constexpr (HALO) {
+        tile_halo_load<0>(t, h, nodes, z, nullptr, hrec, nullptr);
+        __syncthreads();
+    }
+    tile_cells<HALO>(t, HALO ? hrec : nodes, h, [&](int c, const CellEval& ce) {
         const double s = penal(ce.rho, mt.p, mt.emin);
         const double w = s * (ce.absdet / 6.0);
-        const double sw = sqrt(w);
-        double* Q = lds + c * kTileStride;
+        double* Q = lds + c * CS;
+        if constexpr (DPN == 3) {
+            const double sw = sqrt(w);
 #pragma unroll
-        for (int n = 0; n < 4; ++n) {
-            Q[3 * n + 0] = sw * ce.g[n][0];
-            Q[3 * n + 1] = sw * ce.g[n][1];
-            Q[3 * n + 2] = sw * ce.g[n][2];
+            for (int n = 0; n < 4; ++n) {
+                Q[3 * n + 0] = sw * ce.g[n][0];
+                Q[3 * n + 1] = sw * ce.g[n][1];
+                Q[3 * n + 2] = sw * ce.g[n][2];
+            }
+        } else {
+            // 1 x 1 blocks: the ten products w g_a . g_b (a <= b), one LDS
+            // read per unit entry instead of two gradients
+            int q = 0;
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = a; b < 4; ++b)
+                    Q[q++] = w * ((ce.g[a][0] * ce.g[b][0] + ce.g[a][1] * ce.g[b][1]) + ce.g[a][2] * ce.g[b][2]);
         }
     });
     __syncthreads();
@@ -1709,12 +1747,13 @@ bool tile_jacobian(const pa_vector_function* vf, const double* z, VT* vals, hipS
     if (!m->has_tiles) return false;
     const Material mt = material_of(vf->physics, vf->material, vf->penalty);
     const int cmax = m->tile_cells_max + 1;  // + the zero cell
-    const size_t lds = std::max((size_t)cmax * kTileStride * sizeof(double),
-                                (size_t)m->tile_blocks_max * vf->dpn * vf->dpn * sizeof(VT));
+    // 3 x 3 blocks: cell data; 1 x 1: cell products, then the node halo
+    const size_t cells = vf->dpn == 3 ? (size_t)cmax * kTileStride : ((size_t)cmax * kThermalStride + 1) / 2 * 2 + 4 * kTileHalo;
+    const size_t lds = std::max(cells * sizeof(double), (size_t)m->tile_blocks_max * vf->dpn * vf->dpn * sizeof(VT));
     auto run = [&](auto kernel) {
         if (!tile_lds(kernel, lds)) return false;
-        const double4* rec = pack_nodes(m, z, st);
-        kernel<<<m->num_tiles, kTileThreads, lds, st>>>(tile_mesh(m), rec, mt, cmax, vals);
+        const double4* rec = vf->dpn == 3 ? pack_nodes(m, z, st) : reinterpret_cast<const double4*>(m->d_coords4.p);
+        kernel<<<m->num_tiles, kTileThreads, lds, st>>>(tile_mesh(m), rec, z, mt, cmax, vals);
         check_launch();
         return true;
     };
