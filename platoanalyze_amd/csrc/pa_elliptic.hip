@@ -739,7 +739,8 @@ template <int DPN>
 __global__ __launch_bounds__(kResThreads) void k_tile_residual(TileMesh t, const double4* __restrict__ x4,
                                                                const double* __restrict__ z, Material mt,
                                                                const double* __restrict__ u, double body,
-                                                               double scale, int hoff, double* __restrict__ R) {
+                                                               double scale, int hoff, const double* __restrict__ fext,
+                                                               double* __restrict__ R) {
     // LDS: the pair slots (rows x 32 x DPN doubles), then at hoff the halo
     // records and the halo's state values
     extern __shared__ double lds[];
@@ -812,8 +813,14 @@ __global__ __launch_bounds__(kResThreads) void k_tile_residual(TileMesh t, const
         for (int o = 4; o > 0; o >>= 1) b[i] += __shfl_xor(b[i], o, 8);
     }
     if (s == 0) {
+        const long o = (long)(h.base + ri.x) * DPN;
+        if (fext) {  // plus the natural BCs' nodal load
 #pragma unroll
-        for (int i = 0; i < DPN; ++i) R[(long)(h.base + ri.x) * DPN + i] = scale * b[i];
+            for (int i = 0; i < DPN; ++i) R[o + i] = scale * b[i] + fext[o + i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < DPN; ++i) R[o + i] = scale * b[i];
+        }
     }
 }
 
@@ -1700,24 +1707,29 @@ const double4* pack_nodes(const pa_mesh* m, const double* z, hipStream_t st) {
 // Residual-type row sums: through row patterns and packed node records when
 // the mesh has them; else one incident cell per lane (32 lanes per row) when
 // the node degree allows, else 8 lanes per row looping over the cells.
+// Residual rows by the tile kernel, plus a nodal load fext when given;
+// false when the mesh has no usable tiles.
+bool tile_residual(const pa_mesh* m, int dpn, const Material& mt, const double* u, const double* z, double body,
+                   double scale, const double* fext, double* R, hipStream_t st) {
+    if (!m->has_tiles || m->tile_rows_max * 8 > kResThreads) return false;
+    const int hoff = m->tile_rows_max * 32 * dpn;  // doubles; a multiple of 32 keeps the records aligned
+    const size_t lds = (size_t)(hoff + (4 + dpn) * kTileHalo) * sizeof(double);
+    const size_t ok = dpn == 3 ? tile_lds(k_tile_residual<3>, lds) : tile_lds(k_tile_residual<1>, lds);
+    if (!ok) return false;
+    const double4* x4 = reinterpret_cast<const double4*>(m->d_coords4.p);
+    if (dpn == 3)
+        k_tile_residual<3><<<m->num_tiles, kResThreads, lds, st>>>(tile_mesh(m), x4, z, mt, u, body, scale, hoff, fext,
+                                                                   R);
+    else
+        k_tile_residual<1><<<m->num_tiles, kResThreads, lds, st>>>(tile_mesh(m), x4, z, mt, u, body, scale, hoff, fext,
+                                                                   R);
+    check_launch();
+    return true;
+}
+
 void residual_rows(const pa_mesh* m, int dpn, const Material& mt, const double* u, const double* z, double body,
                    double scale, double* R, hipStream_t st) {
-    if (m->has_tiles && m->tile_rows_max * 8 <= kResThreads) {
-        const int hoff = m->tile_rows_max * 32 * dpn;  // doubles; a multiple of 32 keeps the records aligned
-        const size_t lds = (size_t)(hoff + (4 + dpn) * kTileHalo) * sizeof(double);
-        const size_t ok = dpn == 3 ? tile_lds(k_tile_residual<3>, lds) : tile_lds(k_tile_residual<1>, lds);
-        if (ok) {
-            const double4* x4 = reinterpret_cast<const double4*>(m->d_coords4.p);
-            if (dpn == 3)
-                k_tile_residual<3><<<m->num_tiles, kResThreads, lds, st>>>(tile_mesh(m), x4, z, mt, u, body, scale,
-                                                                           hoff, R);
-            else
-                k_tile_residual<1><<<m->num_tiles, kResThreads, lds, st>>>(tile_mesh(m), x4, z, mt, u, body, scale,
-                                                                           hoff, R);
-            check_launch();
-            return;
-        }
-    }
+    if (tile_residual(m, dpn, mt, u, z, body, scale, nullptr, R, st)) return;
     if (m->has_patterns) {
         const double4* rec = pack_nodes(m, z, st);
         if (dpn == 3)
@@ -1795,11 +1807,36 @@ void jacobian(const pa_vector_function* vf, const double* u, const double* z, VT
     if (RES) natural_bcs(vf, R, st);
 }
 
+__global__ void k_zero(long n, double* __restrict__ v) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) v[i] = 0.0;
+}
+
+// The natural BCs' nodal load of vf (null without natural BCs), built on
+// first use by the node-ordered load kernel into a zeroed vector and kept
+// until a load is added or the mesh's side sets change.  The vector is
+// cleared by a kernel, not a memset, so a first call may be captured.
+const double* natural_load(const pa_vector_function* vf, hipStream_t st) {
+    if (vf->natural_bcs.empty()) return nullptr;
+    const pa_mesh* m = vf->mesh;
+    if (vf->fext_nbc != vf->natural_bcs.size() || vf->fext_version != m->side_set_version) {
+        const long n = (long)m->N * vf->dpn;
+        if (vf->fext.n != (size_t)n) vf->fext.alloc(n);
+        k_zero<<<(int)std::min<long>(4096, (n + kBlock - 1) / kBlock), kBlock, 0, st>>>(n, vf->fext.p);
+        check_launch();
+        natural_bcs(vf, vf->fext.p, st);
+        vf->fext_nbc = vf->natural_bcs.size();
+        vf->fext_version = m->side_set_version;
+    }
+    return vf->fext.p;
+}
+
 void residual(const pa_vector_function* vf, const double* u, const double* z, double* R, hipStream_t st) {
     const pa_mesh* m = vf->mesh;
     const Material mt = material_of(vf->physics, vf->material, vf->penalty);
-    const double body = vf->has_body ? vf->body_value : 0.0;
-    residual_rows(m, vf->dpn, mt, u, z, vf->dpn == 3 ? 0.0 : body, 1.0, R, st);
+    const double body = vf->dpn == 3 ? 0.0 : (vf->has_body ? vf->body_value : 0.0);
+    // on tiles the natural BCs' load is added as the rows are written
+    if (m->has_tiles && tile_residual(m, vf->dpn, mt, u, z, body, 1.0, natural_load(vf, st), R, st)) return;
+    residual_rows(m, vf->dpn, mt, u, z, body, 1.0, R, st);
     natural_bcs(vf, R, st);
 }
 

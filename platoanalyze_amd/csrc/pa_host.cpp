@@ -667,6 +667,7 @@ int pa_mesh_add_side_set(pa_mesh* m, const char* name, int F, const int* cells, 
         PA_REQUIRE(m && name, PA_ERR_ARG, "pa_mesh_add_side_set: null argument");
         PA_REQUIRE(F >= 0 && (F == 0 || (cells && lnodes)), PA_ERR_ARG, "pa_mesh_add_side_set: bad faces");
         pa::SideSet& s = m->side_sets[name];
+        ++m->side_set_version;
         s.cells.assign(cells, cells + F);
         s.lnodes.assign(lnodes, lnodes + 3L * F);
         for (int f = 0; f < F; ++f) {

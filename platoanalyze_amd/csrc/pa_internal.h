@@ -138,6 +138,7 @@ struct pa_mesh {
     int tile_unit_entries = 0;        // entries per work unit the kernels evaluate (6 or 8)
     int tile_blocks_max = 0;          // output blocks of a tile (its staging size)
     int grid_dims[3] = {0, 0, 0};     // detected node grid (x, y, z node counts)
+    int side_set_version = 0;         // bumped by every side-set definition
     pa::DevBuf<int4> d_tile;          // 3 per tile: {first row, row_off, cell_off, unit_off},
                                       // {rows, cells, units, runs}, {run_off, blocks, 0, 0}
     pa::DevBuf<int4> d_tp_row;        // per pattern row: {row - first row, incident cells, pair offset,
@@ -172,6 +173,12 @@ struct pa_vector_function {
     bool has_body = false;
     int body_dof = 0;
     double body_value = 0.0;
+    // the natural BCs' nodal load (it depends on the coordinates and the
+    // load values only), built on first use: valid for fext_nbc loads and
+    // the mesh's side sets at fext_version
+    mutable pa::DevBuf<double> fext;
+    mutable size_t fext_nbc = 0;
+    mutable int fext_version = -1;
 };
 
 namespace pa {

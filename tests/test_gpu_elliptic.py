@@ -276,3 +276,34 @@ def test_gradient_x_config1_parity(pa):
         mat = dict(youngs=1.0, poisson=0.3) if physics == "elastic" else dict(conductivity=1.0)
         sf = pa.PhysicsScalarFunction(mesh, kind, penalty_exponent=3.0, penalty_min=1e-9, **mat)
         assert rel(sf.gradient_x(u, z).numpy(), om.energy_gradient_x(physics, u, z)) < TOL64
+
+
+def test_natural_load_cache_follows_changes(pa):
+    """value() on a tile mesh adds the natural BCs' nodal load, kept between
+    calls: a load added later, or a side set redefined, must show up."""
+    print("case: natural-BC load cache, 8x4x4 box, elastic", flush=True)
+    dims, L = (8, 4, 4), (2.0, 1.0, 1.0)
+    coords, conn = pa.box_mesh(*dims, *L)
+    mesh = pa.Mesh.box(*dims, *L)
+    assert mesh.assembly_info()[0] > 0  # tiles
+    N = mesh.num_nodes
+    rng = np.random.default_rng(5)
+    z = rng.uniform(0.3, 1.0, N)
+    u = rng.standard_normal(3 * N) * 1e-2
+    fc, fl = oracle.side_set(coords, conn, 0, L[0])
+    kw = dict(youngs=1.0, poisson=0.3, penalty_exponent=3.0, penalty_min=1e-9)
+
+    def ref(flux, faces=True):
+        m = oracle.Model(coords, conn, face_cells=fc if faces else None, face_lnodes=fl if faces else None,
+                         flux=flux, **kw)
+        return m.value("elastic", u, z)
+
+    vf = pa.VectorFunction(mesh, "Mechanics", **kw)
+    assert rel(vf.value(u, z).numpy(), ref((0.0, 0.0, 0.0), faces=False)) < TOL64
+    vf.add_uniform_natural_bc("x+", (0.0, 0.0, -1.0))
+    assert rel(vf.value(u, z).numpy(), ref((0.0, 0.0, -1.0))) < TOL64
+    assert rel(vf.value(u, z).numpy(), ref((0.0, 0.0, -1.0))) < TOL64  # cached
+    vf.add_uniform_natural_bc("x+", (0.5, 0.0, 0.0))
+    assert rel(vf.value(u, z).numpy(), ref((0.5, 0.0, -1.0))) < TOL64
+    mesh.add_side_set("x+", np.zeros(0, np.int32), np.zeros(0, np.int32))  # redefined: no faces
+    assert rel(vf.value(u, z).numpy(), ref((0.0, 0.0, 0.0), faces=False)) < TOL64
