@@ -629,7 +629,6 @@ struct TileMesh {
     const int4* __restrict__ cells;
     const uint2* __restrict__ cslots;
     const unsigned* __restrict__ hslots;
-    const unsigned short* __restrict__ pairs;
     const uint2* __restrict__ units;
     const uint4* __restrict__ ents;
     const int* __restrict__ rowmap;
@@ -1621,7 +1620,6 @@ TileMesh tile_mesh(const pa_mesh* m) {
     t.cells = m->d_tp_cell.p;
     t.cslots = m->d_tp_cslot.p;
     t.hslots = m->d_tp_hslot.p;
-    t.pairs = m->d_tp_pair.p;
     t.units = m->d_tp_unit.p;
     t.ents = reinterpret_cast<const uint4*>(m->d_tp_ent.p);
     t.rowmap = m->d_rowmap.p;

@@ -149,7 +149,7 @@ struct pa_mesh {
     pa::DevBuf<uint2> d_tp_cslot;     // per pattern cell: residual pair slot (row * 32 + k) of each node
     pa::DevBuf<unsigned> d_tp_hslot;  // per tile cell: halo slots of its 4 nodes
                                       // (16 bits each, 0xffff for a node that is not a tile row)
-    pa::DevBuf<unsigned short> d_tp_pair;  // per (row, incident cell): tile cell << 2 | local index of the row node
+    pa::DevBuf<unsigned short> d_tp_pair;  // per (row, incident cell): tile cell << 2 | local index (host checks)
     pa::DevBuf<uint2> d_tp_unit;      // (row, column slot) work units: {row | slot << 8 | mirror row << 16 |
                                       // mirror slot << 24, entry offset | count << 22 | diagonal << 26 |
                                       // part << 27 | mirror << 29 | wave shape << 30}
