@@ -153,8 +153,9 @@ int pa_vector_function_gradient_u(const pa_vector_function* vf, const double* u_
 int pa_vector_function_gradient_u_T(const pa_vector_function* vf, const double* u_dev, const double* z_dev,
                                     void* values_dev, int precision, void* stream);
 /*
- * value and gradient_u in one pass over the mesh (same results as the two
- * separate calls); r_dev: N*dpn doubles.
+ * value and gradient_u in one call (the two separate calls' results, up to
+ * rounding; one fused pass over the mesh where no cell tiles exist);
+ * r_dev: N*dpn doubles.
  */
 int pa_vector_function_value_gradient_u(const pa_vector_function* vf, const double* u_dev, const double* z_dev,
                                         double* r_dev, void* values_dev, int precision, void* stream);
