@@ -457,7 +457,7 @@ def main():
                    "cells": C, "nodes": N, "blocks": nnzb, "precision": "fp64"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "gradient_u call (FP64): k_pack_nodes + k_tile_jacobian<3,double,false,6>",
+                     "kernel": "gradient_u call (FP64): k_pack_nodes + k_tile_jacobian<3,double,6>",
                      "algorithmic_bytes_per_launch": jac_bytes, "avg_launch_ms": t_jac},
         "kernels_ms": {"gradient_u_fp64": t_jac, "value": t_val, "gradient_u_fp32": t_jac32,
                        "value_gradient_u_fused_fp64": t_fused, "value_gradient_u_fused_fp32": t_fused32},
