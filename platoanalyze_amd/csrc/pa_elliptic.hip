@@ -401,6 +401,13 @@ __global__ __launch_bounds__(kBlock) void k_pack_nodes(int N, const double4* __r
     }
 }
 
+// Later calls: the coordinates of the records stay, only the density moves in.
+__global__ __launch_bounds__(kBlock) void k_pack_density(int N, const double* __restrict__ z,
+                                                         double4* __restrict__ rec) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < N) rec[i].w = z[i];
+}
+
 // Residual rows through row patterns: one incident cell per lane (32 lanes per
 // row), nodes from the pattern offsets, geometry and density from the packed
 // node records; also serves criterion gradient_u (scale 2 for the thermal
@@ -1697,9 +1704,13 @@ void dispatch_jac(const pa_vector_function* vf, const double* u, const double* z
 }
 
 const double4* pack_nodes(const pa_mesh* m, const double* z, hipStream_t st) {
-    if (m->d_nrec.n != (size_t)m->N) m->d_nrec.alloc(m->N);
-    k_pack_nodes<<<(m->N + kBlock - 1) / kBlock, kBlock, 0, st>>>(
-        m->N, reinterpret_cast<const double4*>(m->d_coords4.p), z, m->d_nrec.p);
+    if (m->d_nrec.n != (size_t)m->N) {
+        m->d_nrec.alloc(m->N);
+        k_pack_nodes<<<(m->N + kBlock - 1) / kBlock, kBlock, 0, st>>>(
+            m->N, reinterpret_cast<const double4*>(m->d_coords4.p), z, m->d_nrec.p);
+    } else {
+        k_pack_density<<<(m->N + kBlock - 1) / kBlock, kBlock, 0, st>>>(m->N, z, m->d_nrec.p);
+    }
     check_launch();
     return m->d_nrec.p;
 }
