@@ -95,6 +95,15 @@ int pa_mesh_sizes(const pa_mesh* mesh, int* num_nodes, int* num_cells, int64_t* 
  * and of distinct row patterns (0 when the generic row kernels are used).
  */
 int pa_mesh_assembly_info(const pa_mesh* mesh, int* num_tiles, int* num_tile_patterns, int* num_row_patterns);
+/*
+ * Node grid of the streaming grid kernels (diagnostic; no reference
+ * counterpart): dims = (x, y, z) node counts when the nodes are a
+ * lexicographic grid (z fastest) and the cells are the six tets of every hex
+ * around its v0 -> v111 diagonal, else zeros (also when PA_GRID=0 was set at
+ * creation).  On such meshes value() and the thermal gradient_u run the
+ * streaming kernels.
+ */
+int pa_mesh_grid_info(const pa_mesh* mesh, int* dims);
 /* Copy the block-CRS graph to the host (rowmap: N+1, colidx: nnzb). */
 int pa_mesh_graph(const pa_mesh* mesh, int* rowmap_host, int* colidx_host);
 /* Device pointers of the graph (owned by the mesh). */

@@ -116,6 +116,7 @@ def _load():
         "pa_mesh_sizes": ([vp, ip, ip, ctypes.POINTER(ctypes.c_int64)], i),
         "pa_mesh_graph": ([vp, vp, vp], i),
         "pa_mesh_assembly_info": ([vp, ip, ip, ip], i),
+        "pa_mesh_grid_info": ([vp, ip], i),
         "pa_mesh_graph_dev": ([vp, ctypes.POINTER(vp), ctypes.POINTER(vp)], i),
         "pa_mesh_add_side_set": ([vp, ctypes.c_char_p, i, vp, vp], i),
         "pa_mesh_add_node_set": ([vp, ctypes.c_char_p, i, vp], i),
@@ -402,6 +403,13 @@ class Mesh:
         t, tp, rp = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         _check(lib.pa_mesh_assembly_info(self.handle, ctypes.byref(t), ctypes.byref(tp), ctypes.byref(rp)))
         return t.value, tp.value, rp.value
+
+    def grid_info(self):
+        """Node grid (x, y, z counts) of the streaming grid kernels, or None
+        when the mesh is not a Kuhn-split grid."""
+        d = (ctypes.c_int * 3)()
+        _check(lib.pa_mesh_grid_info(self.handle, d))
+        return tuple(d) if d[0] else None
 
     def graph(self):
         rowmap = np.empty(self.num_nodes + 1, np.int32)

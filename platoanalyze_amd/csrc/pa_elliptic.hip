@@ -22,18 +22,13 @@
 
 #include <cmath>
 
+#include "pa_cell.h"
 #include "pa_internal.h"
 
 namespace pa {
 namespace {
 
 constexpr int kBlock = 256;
-
-struct Material {
-    double lambda, mu;  // isotropic: C(0,1) and C(3,3) of the Voigt stiffness
-    double k;           // conductivity
-    double p, emin;     // MSIMP exponent, minimum value
-};
 
 struct RowMesh {
     int N;
@@ -52,36 +47,6 @@ struct RowMesh {
     const int4* __restrict__ pat_delta;
     const uint4* __restrict__ pat_item2;
 };
-
-// Blocks are dispatched round-robin over the 8 XCDs; renumber them so each
-// XCD walks a contiguous range of rows and its L2 keeps the shared nodes.
-__device__ inline int xcd_block() {
-    const int nb = gridDim.x, b = blockIdx.x;
-    const int per = (nb + 7) / 8;
-    const int x = b % 8, i = b / 8;
-    const int full = nb % 8 == 0 ? 8 : nb % 8;  // XCDs that receive `per` blocks
-    if (x < full) return x * per + i;
-    return full * per + (x - full) * (per - 1) + i;
-}
-
-__device__ inline double penal(double rho, double p, double emin) {
-    // integral exponents by products (matches pow to an ulp)
-    double r;
-    if (p == 3.0) r = rho * rho * rho;
-    else if (p == 1.0) r = rho;
-    else if (p == 2.0) r = rho * rho;
-    else r = pow(rho, p);
-    return emin + (1.0 - emin) * r;
-}
-
-__device__ inline double dpenal(double rho, double p, double emin) {
-    double r;
-    if (p == 3.0) r = 3.0 * rho * rho;
-    else if (p == 1.0) r = 1.0;
-    else if (p == 2.0) r = 2.0 * rho;
-    else r = rho == 0.0 ? 0.0 : p * pow(rho, p - 1.0);
-    return (1.0 - emin) * r;
-}
 
 // Tet4 shape-function gradients and |det J| (ComputeGradientWorkset<3>).
 // One correctly rounded reciprocal of det J replaces the reference's nine
@@ -389,31 +354,18 @@ __device__ inline double block_entry(const Material& mt, const double* M, int q)
     }
 }
 
-// Node records (x, y, z, density) packed per call so one 32-byte gather
-// brings everything the geometry and the penalty need.
-__global__ __launch_bounds__(kBlock) void k_pack_nodes(int N, const double4* __restrict__ x4,
-                                                       const double* __restrict__ z, double4* __restrict__ rec) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < N) {
-        double4 v = x4[i];
-        v.w = z[i];
-        rec[i] = v;
-    }
-}
-
-// Later calls: the coordinates of the records stay, only the density moves in.
-__global__ __launch_bounds__(kBlock) void k_pack_density(int N, const double* __restrict__ z,
-                                                         double4* __restrict__ rec) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < N) rec[i].w = z[i];
+// (x, y, z, density) of a node: the padded coordinates with the density.
+__device__ inline double4 node_rec(const double4* __restrict__ x4, const double* __restrict__ z, int n) {
+    double4 r = x4[n];
+    r.w = z[n];
+    return r;
 }
 
 // Residual rows through row patterns: one incident cell per lane (32 lanes per
-// row), nodes from the pattern offsets, geometry and density from the packed
-// node records; also serves criterion gradient_u (scale 2 for the thermal
+// row), nodes from the pattern offsets; also serves criterion gradient_u (scale 2 for the thermal
 // energy, no loads).
 template <int DPN>
-__global__ __launch_bounds__(kBlock) void k_residual_pattern(RowMesh m, const double4* __restrict__ nrec, Material mt,
+__global__ __launch_bounds__(kBlock) void k_residual_pattern(RowMesh m, const double* __restrict__ z, Material mt,
                                                              const double* __restrict__ u, double body, double scale,
                                                              double* __restrict__ R) {
     constexpr int G = 32, ROWS = kBlock / G;
@@ -436,7 +388,7 @@ __global__ __launch_bounds__(kBlock) void k_residual_pattern(RowMesh m, const do
         c.nodes[3] = row + dl.w;
         double4 x[4];
 #pragma unroll
-        for (int n = 0; n < 4; ++n) x[n] = nrec[c.nodes[n]];
+        for (int n = 0; n < 4; ++n) x[n] = node_rec(m.x4, z, c.nodes[n]);
         c.absdet = tet_gradients(x, c.g);
         c.rho = (((x[0].w + x[1].w) + x[2].w) + x[3].w) / 4.0;
         const double s = penal(c.rho, mt.p, mt.emin);
@@ -473,7 +425,7 @@ __global__ __launch_bounds__(kBlock) void k_residual_pattern(RowMesh m, const do
 }
 
 template <int DPN, class VT, bool RES, int PMAX>
-__global__ __launch_bounds__(kBlock) void k_jacobian_pattern(RowMesh m, const double4* __restrict__ nrec, Material mt,
+__global__ __launch_bounds__(kBlock) void k_jacobian_pattern(RowMesh m, const double* __restrict__ z, Material mt,
                                                              const double* __restrict__ u, double body,
                                                              VT* __restrict__ vals, double* __restrict__ R) {
     constexpr int G = 32, ROWS = kBlock / G;
@@ -512,9 +464,9 @@ __global__ __launch_bounds__(kBlock) void k_jacobian_pattern(RowMesh m, const do
         c.nodes[1] = row + dl.y;
         c.nodes[2] = row + dl.z;
         c.nodes[3] = row + dl.w;
-        double4 x[4];  // packed (x, y, z, density) records
+        double4 x[4];  // (x, y, z, density) records
 #pragma unroll
-        for (int n = 0; n < 4; ++n) x[n] = nrec[c.nodes[n]];
+        for (int n = 0; n < 4; ++n) x[n] = node_rec(m.x4, z, c.nodes[n]);
         c.absdet = tet_gradients(x, c.g);
         c.rho = (((x[0].w + x[1].w) + x[2].w) + x[3].w) / 4.0;
         const double s = penal(c.rho, mt.p, mt.emin);
@@ -675,18 +627,19 @@ __device__ inline void tile_halo_load(const TileMesh& t, const TileHead& h, cons
     }
 }
 
-// Node ids and (x, y, z, density) of tile cell c from packed node records
-// in global memory (the Jacobian alone: its block phase keeps the LDS busy,
-// so the cell records come through the vector memory path instead).
-__device__ inline void tile_cell_gather(const TileMesh& t, const double4* __restrict__ nrec, const TileHead& h, int c,
-                                        CellEval& ce, double4 x[4]) {
+// Node ids and (x, y, z, density) of tile cell c gathered from global memory
+// (the elastic Jacobian: its block phase keeps the LDS busy, so the cell
+// records come through the vector memory path instead).
+__device__ inline void tile_cell_gather(const TileMesh& t, const double4* __restrict__ x4,
+                                        const double* __restrict__ z, const TileHead& h, int c, CellEval& ce,
+                                        double4 x[4]) {
     const int4 d = t.cells[h.cell_off + c];
     ce.nodes[0] = h.base + d.x;
     ce.nodes[1] = h.base + d.y;
     ce.nodes[2] = h.base + d.z;
     ce.nodes[3] = h.base + d.w;
 #pragma unroll
-    for (int n = 0; n < 4; ++n) x[n] = nrec[ce.nodes[n]];
+    for (int n = 0; n < 4; ++n) x[n] = node_rec(x4, z, ce.nodes[n]);
 }
 
 // (x, y, z, density) of tile cell c from the halo; ce.nodes holds the halo
@@ -710,7 +663,8 @@ __device__ inline void tile_cell_geometry(CellEval& ce, const double4 x[4]) {
 // 2 * kTileThreads cells), with both cells' gathers issued before either is
 // used.
 template <bool HALO, class F>
-__device__ inline void tile_cells(const TileMesh& t, const double4* src, const TileHead& h, F&& eval) {
+__device__ inline void tile_cells(const TileMesh& t, const double4* src, const double* __restrict__ z,
+                                  const TileHead& h, F&& eval) {
     static_assert(kTileCellsMax <= 2 * kTileThreads, "two cells per thread");
     const int c0 = threadIdx.x, c1 = threadIdx.x + kTileThreads;
     CellEval e0, e1;
@@ -719,8 +673,8 @@ __device__ inline void tile_cells(const TileMesh& t, const double4* src, const T
         if (c0 < h.ncells) tile_cell_load(t, src, h, c0, e0, x0);
         if (c1 < h.ncells) tile_cell_load(t, src, h, c1, e1, x1);
     } else {
-        if (c0 < h.ncells) tile_cell_gather(t, src, h, c0, e0, x0);
-        if (c1 < h.ncells) tile_cell_gather(t, src, h, c1, e1, x1);
+        if (c0 < h.ncells) tile_cell_gather(t, src, z, h, c0, e0, x0);
+        if (c1 < h.ncells) tile_cell_gather(t, src, z, h, c1, e1, x1);
     }
     if (c0 < h.ncells) {
         tile_cell_geometry(e0, x0);
@@ -898,8 +852,8 @@ __device__ inline void tile_unit_block(const double* lds, const TileUnitFetch& f
 }
 
 // Jacobian of one tile.  Phase 1 parks sqrt(w) g_n per cell (kTileStride
-// doubles), from packed node records gathered through the vector memory path
-// (the 3 x 3 block phase keeps the LDS busy); thermal: the ten products
+// doubles), from coordinates and densities gathered through the vector memory
+// path (the 3 x 3 block phase keeps the LDS busy); thermal: the ten products
 // w g_a . g_b per cell, from the tile's node halo staged in LDS (its block
 // phase reads one product per entry and leaves the LDS room).  Phase 2: each thread evaluates
 // up to three work units; the kTileDiagParts parts of a diagonal block sit on
@@ -909,8 +863,8 @@ template <int DPN, class VT, int UE>
 __global__ __launch_bounds__(kTileThreads) void k_tile_jacobian(TileMesh t, const double4* __restrict__ nodes,
                                                                 const double* __restrict__ z, Material mt, int cmax,
                                                                 VT* __restrict__ vals) {
-    // nodes: packed (x, y, z, density) records (DPN 3), or the padded
-    // coordinates read with z into the halo (DPN 1)
+    // nodes: the padded coordinates, gathered per cell with z (DPN 3) or
+    // read with z into the halo (DPN 1)
     constexpr bool HALO = DPN == 1;
     constexpr int MS = DPN == 3 ? 9 : 1;
     constexpr int NU = kTileUnitsMax / kTileThreads;
@@ -935,7 +889,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_jacobian(TileMesh t, cons
         tile_halo_load<0>(t, h, nodes, z, nullptr, hrec, nullptr);
         __syncthreads();
     }
-    tile_cells<HALO>(t, HALO ? hrec : nodes, h, [&](int c, const CellEval& ce) {
+    tile_cells<HALO>(t, HALO ? hrec : nodes, z, h, [&](int c, const CellEval& ce) {
         const double s = penal(ce.rho, mt.p, mt.emin);
         const double w = s * (ce.absdet / 6.0);
         double* Q = lds + c * CS;
@@ -1703,17 +1657,6 @@ void dispatch_jac(const pa_vector_function* vf, const double* u, const double* z
     else throw Error(PA_ERR_LIMIT, "gradient_u: node degree above 63 is not supported");
 }
 
-const double4* pack_nodes(const pa_mesh* m, const double* z, hipStream_t st) {
-    if (m->d_nrec.n != (size_t)m->N) {
-        m->d_nrec.alloc(m->N);
-        k_pack_nodes<<<(m->N + kBlock - 1) / kBlock, kBlock, 0, st>>>(
-            m->N, reinterpret_cast<const double4*>(m->d_coords4.p), z, m->d_nrec.p);
-    } else {
-        k_pack_density<<<(m->N + kBlock - 1) / kBlock, kBlock, 0, st>>>(m->N, z, m->d_nrec.p);
-    }
-    check_launch();
-    return m->d_nrec.p;
-}
 
 // Residual-type row sums: through row patterns and packed node records when
 // the mesh has them; else one incident cell per lane (32 lanes per row) when
@@ -1740,13 +1683,13 @@ bool tile_residual(const pa_mesh* m, int dpn, const Material& mt, const double* 
 
 void residual_rows(const pa_mesh* m, int dpn, const Material& mt, const double* u, const double* z, double body,
                    double scale, double* R, hipStream_t st) {
+    if (grid_residual(m, dpn, mt, u, z, body, scale, nullptr, R, st)) return;
     if (tile_residual(m, dpn, mt, u, z, body, scale, nullptr, R, st)) return;
     if (m->has_patterns) {
-        const double4* rec = pack_nodes(m, z, st);
         if (dpn == 3)
-            k_residual_pattern<3><<<row_blocks(m->N, 32), kBlock, 0, st>>>(row_mesh(m), rec, mt, u, body, scale, R);
+            k_residual_pattern<3><<<row_blocks(m->N, 32), kBlock, 0, st>>>(row_mesh(m), z, mt, u, body, scale, R);
         else
-            k_residual_pattern<1><<<row_blocks(m->N, 32), kBlock, 0, st>>>(row_mesh(m), rec, mt, u, body, scale, R);
+            k_residual_pattern<1><<<row_blocks(m->N, 32), kBlock, 0, st>>>(row_mesh(m), z, mt, u, body, scale, R);
     } else if (m->max_cells_per_node <= 32) {
         if (dpn == 3)
             k_residual_rows<3, 32><<<row_blocks(m->N, 32), kBlock, 0, st>>>(row_mesh(m), mt, u, z, body, scale, R);
@@ -1775,8 +1718,8 @@ bool tile_jacobian(const pa_vector_function* vf, const double* z, VT* vals, hipS
     const size_t lds = std::max(cells * sizeof(double), (size_t)m->tile_blocks_max * vf->dpn * vf->dpn * sizeof(VT));
     auto run = [&](auto kernel) {
         if (!tile_lds(kernel, lds)) return false;
-        const double4* rec = vf->dpn == 3 ? pack_nodes(m, z, st) : reinterpret_cast<const double4*>(m->d_coords4.p);
-        kernel<<<m->num_tiles, kTileThreads, lds, st>>>(tile_mesh(m), rec, z, mt, cmax, vals);
+        const double4* x4 = reinterpret_cast<const double4*>(m->d_coords4.p);
+        kernel<<<m->num_tiles, kTileThreads, lds, st>>>(tile_mesh(m), x4, z, mt, cmax, vals);
         check_launch();
         return true;
     };
@@ -1789,6 +1732,10 @@ template <class VT, bool RES, bool TR>
 void jacobian(const pa_vector_function* vf, const double* u, const double* z, VT* vals, double* R, hipStream_t st) {
     const pa_mesh* m = vf->mesh;
     if constexpr (!RES) {
+        // thermal on Kuhn grids: the streaming kernel; else the tiles
+        if (vf->dpn == 1 && m->has_grid &&
+            grid_jacobian1(m, material_of(vf->physics, vf->material, vf->penalty), z, vals, st))
+            return;
         if (tile_jacobian(vf, z, vals, st)) return;
     }
     if (m->has_patterns) {
@@ -1797,17 +1744,16 @@ void jacobian(const pa_vector_function* vf, const double* u, const double* z, VT
         const Material mt = material_of(vf->physics, vf->material, vf->penalty);
         const double body = vf->has_body ? vf->body_value : 0.0;
         const int nb = row_blocks(m->N, 32);
-        const double4* rec = pack_nodes(m, z, st);
         if (m->max_cells_per_node <= 24) {
             if (vf->dpn == 3)
-                k_jacobian_pattern<3, VT, RES, 24><<<nb, kBlock, 0, st>>>(row_mesh(m), rec, mt, u, 0.0, vals, R);
+                k_jacobian_pattern<3, VT, RES, 24><<<nb, kBlock, 0, st>>>(row_mesh(m), z, mt, u, 0.0, vals, R);
             else
-                k_jacobian_pattern<1, VT, RES, 24><<<nb, kBlock, 0, st>>>(row_mesh(m), rec, mt, u, body, vals, R);
+                k_jacobian_pattern<1, VT, RES, 24><<<nb, kBlock, 0, st>>>(row_mesh(m), z, mt, u, body, vals, R);
         } else {
             if (vf->dpn == 3)
-                k_jacobian_pattern<3, VT, RES, 32><<<nb, kBlock, 0, st>>>(row_mesh(m), rec, mt, u, 0.0, vals, R);
+                k_jacobian_pattern<3, VT, RES, 32><<<nb, kBlock, 0, st>>>(row_mesh(m), z, mt, u, 0.0, vals, R);
             else
-                k_jacobian_pattern<1, VT, RES, 32><<<nb, kBlock, 0, st>>>(row_mesh(m), rec, mt, u, body, vals, R);
+                k_jacobian_pattern<1, VT, RES, 32><<<nb, kBlock, 0, st>>>(row_mesh(m), z, mt, u, body, vals, R);
         }
         check_launch();
     } else {
@@ -1822,22 +1768,26 @@ __global__ void k_zero(long n, double* __restrict__ v) {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) v[i] = 0.0;
 }
 
-// The natural BCs' nodal load of vf (null without natural BCs), built on
-// first use by the node-ordered load kernel into a zeroed vector and kept
-// until a load is added or the mesh's side sets change.  The vector is
-// cleared by a kernel, not a memset, so a first call may be captured.
-const double* natural_load(const pa_vector_function* vf, hipStream_t st) {
-    if (vf->natural_bcs.empty()) return nullptr;
+// The natural BCs' nodal load of vf (null without natural BCs): assembled by
+// the node-ordered load kernel into a zeroed vector when a load is added, and
+// again on the first call after the mesh's side sets change, each time
+// synchronously, so every later call on any stream reads a finished vector.
+void build_natural_load(const pa_vector_function* vf) {
     const pa_mesh* m = vf->mesh;
-    if (vf->fext_nbc != vf->natural_bcs.size() || vf->fext_version != m->side_set_version) {
-        const long n = (long)m->N * vf->dpn;
-        if (vf->fext.n != (size_t)n) vf->fext.alloc(n);
-        k_zero<<<(int)std::min<long>(4096, (n + kBlock - 1) / kBlock), kBlock, 0, st>>>(n, vf->fext.p);
-        check_launch();
-        natural_bcs(vf, vf->fext.p, st);
-        vf->fext_nbc = vf->natural_bcs.size();
-        vf->fext_version = m->side_set_version;
-    }
+    const long n = (long)m->N * vf->dpn;
+    if (vf->fext.n != (size_t)n) vf->fext.alloc(n);
+    k_zero<<<(int)std::min<long>(4096, (n + kBlock - 1) / kBlock), kBlock, 0, nullptr>>>(n, vf->fext.p);
+    check_launch();
+    natural_bcs(vf, vf->fext.p, nullptr);
+    PA_HIP(hipStreamSynchronize(nullptr));
+    vf->fext_nbc = vf->natural_bcs.size();
+    vf->fext_version = m->side_set_version;
+}
+
+const double* natural_load(const pa_vector_function* vf) {
+    if (vf->natural_bcs.empty()) return nullptr;
+    if (vf->fext_nbc != vf->natural_bcs.size() || vf->fext_version != vf->mesh->side_set_version)
+        build_natural_load(vf);
     return vf->fext.p;
 }
 
@@ -1845,8 +1795,9 @@ void residual(const pa_vector_function* vf, const double* u, const double* z, do
     const pa_mesh* m = vf->mesh;
     const Material mt = material_of(vf->physics, vf->material, vf->penalty);
     const double body = vf->dpn == 3 ? 0.0 : (vf->has_body ? vf->body_value : 0.0);
-    // on tiles the natural BCs' load is added as the rows are written
-    if (m->has_tiles && tile_residual(m, vf->dpn, mt, u, z, body, 1.0, natural_load(vf, st), R, st)) return;
+    // on grid and tile meshes the natural BCs' load is added as the rows are written
+    if (m->has_grid && grid_residual(m, vf->dpn, mt, u, z, body, 1.0, natural_load(vf), R, st)) return;
+    if (m->has_tiles && tile_residual(m, vf->dpn, mt, u, z, body, 1.0, natural_load(vf), R, st)) return;
     residual_rows(m, vf->dpn, mt, u, z, body, 1.0, R, st);
     natural_bcs(vf, R, st);
 }
@@ -1950,6 +1901,7 @@ int pa_vector_function_add_uniform_natural_bc(pa_vector_function* vf, const char
         bc.side_set = side_set;
         for (int i = 0; i < 3; ++i) bc.values[i] = i < vf->dpn ? values[i] : 0.0;
         vf->natural_bcs.push_back(bc);
+        pa::build_natural_load(vf);
     });
 }
 
@@ -2004,10 +1956,10 @@ int pa_vector_function_value_gradient_u(const pa_vector_function* vf, const doub
         pa::check_vf(vf, u, z, vals);
         PA_REQUIRE(R, PA_ERR_ARG, "value_gradient_u: null residual");
         PA_REQUIRE(precision == PA_FP64 || precision == PA_FP32, PA_ERR_ARG, "value_gradient_u: unknown precision");
-        if (vf->mesh->has_tiles) {
-            // on tile meshes the residual kernel and the Jacobian kernel one
-            // after the other beat one fused pass (whose cell data halves the
-            // workgroups per CU)
+        if (vf->mesh->has_tiles || vf->mesh->has_grid) {
+            // on grid and tile meshes the residual kernel and the Jacobian
+            // kernel one after the other beat one fused pass (whose cell data
+            // halves the workgroups per CU)
             pa::residual(vf, u, z, R, (hipStream_t)stream);
             if (precision == PA_FP64)
                 pa::jacobian<double, false, false>(vf, u, z, (double*)vals, nullptr, (hipStream_t)stream);

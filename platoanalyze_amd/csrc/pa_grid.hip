@@ -1,0 +1,682 @@
+// Streaming assembly on Kuhn-split hex grids (gfx950): the residual of SIMP
+// elastostatics and steady thermal diffusion, and the thermal Jacobian.
+//
+// Applies to a mesh whose nodes are a lexicographic grid (z fastest) and
+// whose cells are exactly the 6 tets of every hex around its v0 -> v7
+// diagonal (the box meshes; checked once per mesh on the host).  Hex corner
+// v = dx << 2 | dy << 1 | dz; the tets are (0, p, q, 7) with (p, q) = (6, 2),
+// (2, 3), (3, 1), (1, 5), (5, 4), (4, 6), the vertex order of the box mesh.
+//
+// Work decomposition.  A workgroup of W waves owns node rows (i, j, k) in a
+// block of lines: lane l of wave w sits on hex column (j, k) = (hj0 + w,
+// hk0 + l), and the workgroup walks the x layers of hexes.  At layer h every
+// lane evaluates its hex once (six tets, one reciprocal each) and sends the
+// hex's contributions to the rows of its eight corners: corners with
+// (dy, dz) = (0, 0) stay in the lane, (0, 1) go to lane l + 1 by a shuffle,
+// (1, 1) to lane l + 1 by a shuffle and on with the (1, 0) corners to wave
+// w + 1 through a double-buffered LDS slot.  Corners at dx = 0 complete the
+// rows of node layer h; those at dx = 1 are carried in registers to layer
+// h + 1.  Wave 0 (past the first line block) and lane 0 (past the first z
+// block) only compute the halo hexes of their neighbours, so each hex is
+// evaluated about (W / (W - 1)) (64 / 63) times, and every row is summed in
+// registers in a fixed order: no atomics, deterministic.  Rows of one line
+// are consecutive node ids, so a wave's rows form one contiguous CRS range,
+// which the Jacobian stages in LDS and stores with lane-contiguous stores.
+//
+// Per tet (nodes x_0..x_3 in the cell's order, J = [x_a - x_3]), with the
+// cofactor vectors c_a (c_0 = e_1 x e_2, c_1 = e_2 x e_0, c_2 = e_0 x e_1,
+// e_a = x_a - x_3; c_3 = -(c_0 + c_1 + c_2)), grad N_a = c_a / det J and
+// V = |det J| / 6:
+//   thermal Jacobian   K_ab = s(rho) k V grad N_a . grad N_b
+//                            = s k / (6 |det|) c_a . c_b
+//   thermal residual   R_a  = s k / (6 |det|) sum_b T_b c_b . c_a
+//                             - V f rho / 4                (body source)
+//   elastic residual   R_a  = B(c_a)^T [s / (6 |det|) C eps(u; c)]
+// the reference's chain (src/ImplicitFunctors.hpp:169-221, src/Strain.hpp:56-95,
+// src/LinearStress.hpp:102-120, src/Simp.hpp:130-134, src/StressDivergence.hpp:70-90,
+// src/ScalarGrad.hpp:58-74, src/ThermalFlux.hpp:42-52, src/FluxDivergence.hpp:31-48,
+// src/BodyLoads.hpp:103-121) with one division per tet.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <type_traits>
+#include <utility>
+
+#include "pa_cell.h"
+#include "pa_internal.h"
+
+namespace pa {
+namespace {
+
+constexpr int kGridWaves = 4;  // waves per workgroup (one halo line)
+
+// Compile-time loop: f(std::integral_constant<int, i>) for i = 0..N-1, so
+// table lookups on i are constant expressions and the hex value arrays stay
+// in registers.
+template <class F, int... Is>
+__device__ inline void sfor_impl(F&& f, std::integer_sequence<int, Is...>) {
+    (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, class F>
+__device__ inline void sfor(F&& f) {
+    sfor_impl(f, std::make_integer_sequence<int, N>{});
+}
+#define PA_CI(x) decltype(x)::value
+
+// The 15 neighbour offsets of a node of the Kuhn grid, in the order of their
+// node ids (z fastest, so the order of the CRS columns of a row).
+__host__ __device__ constexpr int nb_di(int s) { return s < 4 ? -1 : (s < 11 ? 0 : 1); }
+__host__ __device__ constexpr int nb_dj(int s) {
+    constexpr int t[15] = {-1, -1, 0, 0, -1, -1, 0, 0, 0, 1, 1, 0, 0, 1, 1};
+    return t[s];
+}
+__host__ __device__ constexpr int nb_dk(int s) {
+    constexpr int t[15] = {-1, 0, -1, 0, -1, 0, -1, 0, 1, 0, 1, 0, 1, 0, 1};
+    return t[s];
+}
+constexpr int kDiagSlot = 7;
+
+// Edges of the split (12 hex edges, 6 face diagonals, the body diagonal) as
+// corner pairs a < b; index in the hex value array is 8 + edge.
+__host__ __device__ constexpr int edge_of(int a, int b) {
+    constexpr int ea[19] = {0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 2, 2, 2, 3, 4, 4, 4, 5, 6};
+    constexpr int eb[19] = {1, 2, 3, 4, 5, 6, 7, 3, 5, 7, 3, 6, 7, 7, 5, 6, 7, 7, 7};
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    for (int e = 0; e < 19; ++e)
+        if (ea[e] == lo && eb[e] == hi) return e;
+    return -1;
+}
+
+// Index in the hex's 27 values (8 diagonals, 19 edges) of what the hex adds
+// to slot s of the row of its corner v, or -1.
+__host__ __device__ constexpr int jac_entry(int v, int s) {
+    if (s == kDiagSlot) return v;
+    const int x = (v >> 2) + nb_di(s), y = ((v >> 1) & 1) + nb_dj(s), z = (v & 1) + nb_dk(s);
+    if (x < 0 || x > 1 || y < 0 || y > 1 || z < 0 || z > 1) return -1;
+    const int e = edge_of(v, x << 2 | y << 1 | z);
+    return e < 0 ? -1 : 8 + e;
+}
+
+// Kuhn tets (0, p, q, 7)
+__host__ __device__ constexpr int tet_p(int t) {
+    constexpr int p[6] = {6, 2, 3, 1, 5, 4};
+    return p[t];
+}
+__host__ __device__ constexpr int tet_q(int t) {
+    constexpr int q[6] = {2, 3, 1, 5, 4, 6};
+    return q[t];
+}
+
+struct GridArgs {
+    int d0, d1, d2;  // node grid (x, y, z counts)
+    int nj, nk;      // line blocks and z blocks
+    int lx;          // node layers per x block
+    const double4* __restrict__ x4;
+    const double* __restrict__ z;
+};
+
+struct Corner {
+    double x, y, z, r;
+};
+
+// Cofactor vectors and det J of tet (0, p, q, 7) of a hex.
+struct TetGeom {
+    double c[4][3];
+    double det;
+    double rho;
+};
+
+__device__ inline void tet_geom(const Corner& A, const Corner& B, const Corner& Cq, const Corner& D, TetGeom& g) {
+    const double e0x = A.x - D.x, e0y = A.y - D.y, e0z = A.z - D.z;
+    const double e1x = B.x - D.x, e1y = B.y - D.y, e1z = B.z - D.z;
+    const double e2x = Cq.x - D.x, e2y = Cq.y - D.y, e2z = Cq.z - D.z;
+    g.c[0][0] = e1y * e2z - e1z * e2y;
+    g.c[0][1] = e1z * e2x - e1x * e2z;
+    g.c[0][2] = e1x * e2y - e1y * e2x;
+    g.c[1][0] = e2y * e0z - e2z * e0y;
+    g.c[1][1] = e2z * e0x - e2x * e0z;
+    g.c[1][2] = e2x * e0y - e2y * e0x;
+    g.c[2][0] = e0y * e1z - e0z * e1y;
+    g.c[2][1] = e0z * e1x - e0x * e1z;
+    g.c[2][2] = e0x * e1y - e0y * e1x;
+    g.det = (e0x * g.c[0][0] + e0y * g.c[0][1]) + e0z * g.c[0][2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) g.c[3][d] = -((g.c[0][d] + g.c[1][d]) + g.c[2][d]);
+    g.rho = (((A.r + B.r) + Cq.r) + D.r) / 4.0;
+}
+
+// ------------------------------------------------------------------ blocks
+// Line block jc: lines [J0, J1) owned, hex lines from hj0; z block kc: rows
+// [K0, K1), hexes from hk0; the last z block's last lane also owns the top
+// row when it lies one past the lanes (top).
+struct Block {
+    int J0, J1, hj0, K0, K1, hk0, X0, X1;
+    bool top;
+};
+
+__device__ inline Block grid_block(const GridArgs& g, int W) {
+    const int b = xcd_block();
+    const int kc = b % g.nk, jc = (b / g.nk) % g.nj, xc = b / (g.nk * g.nj);
+    Block B;
+    B.J0 = jc == 0 ? 0 : W + (W - 1) * (jc - 1);
+    B.J1 = min(B.J0 + (jc == 0 ? W : W - 1), g.d1);
+    B.hj0 = jc == 0 ? 0 : B.J0 - 1;
+    B.K0 = kc == 0 ? 0 : 64 + 63 * (kc - 1);
+    B.K1 = min(B.K0 + (kc == 0 ? 64 : 63), g.d2);
+    B.hk0 = kc == 0 ? 0 : B.K0 - 1;
+    B.top = B.K1 == g.d2 - 1 && B.hk0 + 64 == g.d2 - 1;
+    if (B.top) B.K1 = g.d2;
+    B.X0 = xc * g.lx;
+    B.X1 = min(B.X0 + g.lx, g.d0);
+    return B;
+}
+
+// 15-bit mask of the neighbours of node (i, j, k) inside the grid.
+__device__ inline unsigned row_exists(const GridArgs& g, int i, int j, int k) {
+    unsigned m = 0x7fffu;
+    if (i == 0) m &= ~0x000fu;
+    if (i == g.d0 - 1) m &= ~0x7800u;
+    if (j == 0) m &= ~0x0033u;
+    if (j == g.d1 - 1) m &= ~0x6600u;
+    if (k == 0) m &= ~0x0055u;
+    if (k == g.d2 - 1) m &= ~0x5500u;
+    return m;
+}
+
+__device__ inline void load_corner(const GridArgs& g, long node, Corner& c) {
+    const double4 p = g.x4[node];
+    c.x = p.x;
+    c.y = p.y;
+    c.z = p.z;
+    c.r = g.z[node];
+}
+
+// The four corners (dy, dz) of hex column (j, k) at node layer i.
+__device__ inline void load_layer(const GridArgs& g, int i, int j, int k, Corner c[4]) {
+    const long n = ((long)i * g.d1 + j) * g.d2 + k;
+    load_corner(g, n, c[0]);
+    load_corner(g, n + 1, c[1]);
+    load_corner(g, n + g.d2, c[2]);
+    load_corner(g, n + g.d2 + 1, c[3]);
+}
+
+__device__ inline double shfl_prev(double v, int lane) {
+    const double r = __shfl_up(v, 1, 64);
+    return lane == 0 ? 0.0 : r;
+}
+
+// Orders a wave's LDS writes before its own lanes' later reads.
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ----------------------------------------------------- thermal Jacobian
+// Live slots of the row of corner v (the hex adds something there).
+__host__ __device__ constexpr unsigned jac_live(int v) {
+    unsigned m = 0;
+    for (int s = 0; s < 15; ++s)
+        if (jac_entry(v, s) >= 0) m |= 1u << s;
+    return m;
+}
+
+// Slots sent to the next wave: corners 2 (layer h) and 6 (layer h + 1) of the
+// own hex plus corners 3 and 7 of the previous lane's hex (same node lines).
+__host__ __device__ constexpr unsigned jx_live(int lev) { return jac_live(lev << 2 | 2) | jac_live(lev << 2 | 3); }
+__host__ __device__ constexpr int popc15(unsigned m) {
+    int n = 0;
+    for (int s = 0; s < 15; ++s) n += (m >> s) & 1u;
+    return n;
+}
+__host__ __device__ constexpr int jx_index(int lev, int s) {
+    // position of (lev, s) among the exported slots, layer h first
+    int n = lev == 0 ? 0 : popc15(jx_live(0));
+    const unsigned m = jx_live(lev);
+    for (int t = 0; t < s; ++t) n += (m >> t) & 1u;
+    return n;
+}
+constexpr int kJX = popc15(jx_live(0)) + popc15(jx_live(1));  // 16
+// corners 3, 7 of the last lane's hex go to the next wave's top row
+__host__ __device__ constexpr int jt_index(int lev, int s) {
+    int n = lev == 0 ? 0 : popc15(jac_live(3));
+    const unsigned m = jac_live(lev << 2 | 3);
+    for (int t = 0; t < s; ++t) n += (m >> t) & 1u;
+    return n;
+}
+constexpr int kJT = popc15(jac_live(3)) + popc15(jac_live(7));  // 13
+// carried slots: corners at dx = 1
+__host__ __device__ constexpr unsigned carry_live() {
+    return jac_live(4) | jac_live(5) | jac_live(6) | jac_live(7);
+}
+constexpr int kStage = 65 * 15;  // staged entries of one wave's rows (64 rows + the top row)
+static_assert(kJX * 64 >= kStage, "a wave's staging fits its exchange slot");
+
+// The 27 values of one hex: w-scaled products c_a . c_b summed over its tets.
+template <int PK>
+__device__ inline void hex_jac1(const Corner X[8], const Material& mt, double H[27]) {
+#pragma unroll
+    for (int q = 0; q < 27; ++q) H[q] = 0.0;
+    sfor<6>([&](auto T) {
+        constexpr int t = PA_CI(T);
+        constexpr int v[4] = {0, tet_p(t), tet_q(t), 7};
+        TetGeom g;
+        tet_geom(X[v[0]], X[v[1]], X[v[2]], X[v[3]], g);
+        const double coef = (penal_k<PK>(g.rho, mt.p, mt.emin) * mt.k) / (6.0 * fabs(g.det));
+        sfor<4>([&](auto A) {
+            constexpr int a = PA_CI(A);
+            sfor<4>([&](auto Bq) {
+                constexpr int b = PA_CI(Bq);
+                if constexpr (b >= a) {
+                    constexpr int idx = a == b ? v[a] : 8 + edge_of(v[a], v[b]);
+                    H[idx] += coef * ((g.c[a][0] * g.c[b][0] + g.c[a][1] * g.c[b][1]) + g.c[a][2] * g.c[b][2]);
+                }
+            });
+        });
+        // one tet at a time: interleaving the six tets' geometry would keep
+        // all their temporaries live at once
+        __builtin_amdgcn_sched_barrier(0);
+    });
+}
+
+template <class VT>
+__device__ inline void store_row(VT* stage, int off, unsigned exist, const double row[15]) {
+    int pos = 0;
+#pragma unroll
+    for (int s = 0; s < 15; ++s) {
+        if ((exist >> s) & 1u) {
+            stage[off + pos] = (VT)row[s];
+            ++pos;
+        }
+    }
+}
+
+template <class VT, int W, int PK>
+__global__ __launch_bounds__(W * 64, 2) void k_grid_jacobian1(GridArgs g, Material mt, const int* __restrict__ rowmap,
+                                                              VT* __restrict__ vals) {
+    // xch[buf][w][slot][lane]: wave w's exports to wave w + 1; the region of
+    // wave w - 1 (wave 0: the unused region of wave W - 1) then stages wave
+    // w's rows
+    __shared__ double xch[2][W][kJX][64];
+    __shared__ double xtop[2][W][kJT];     // last lane's corners 3, 7 for the next wave's top row
+    __shared__ double topc[W][15];         // carried top row of each wave
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const Block B = grid_block(g, W);
+    const int j = B.hj0 + w, k = B.hk0 + l;
+    const bool line_owned = j >= B.J0 && j < B.J1;
+    const bool row_owned = line_owned && k >= B.K0 && k < B.K1;
+    const bool top_lane = B.top && l == 63;  // also owns row d2 - 1
+    const bool hexcol = j < g.d1 - 1 && k < g.d2 - 1;
+    const int hs = B.X0 == 0 ? 0 : B.X0 - 1;
+    const int he = B.X1 == g.d0 ? g.d0 : B.X1;  // layer d0 - 1 has no hexes: its rows are the carry
+    constexpr unsigned CL = carry_live();
+    double carry[15];
+#pragma unroll
+    for (int s = 0; s < 15; ++s) carry[s] = 0.0;
+    if (top_lane) {
+#pragma unroll
+        for (int s = 0; s < 15; ++s) topc[w][s] = 0.0;
+    }
+    Corner lo[4];
+    if (hexcol) load_layer(g, hs, j, k, lo);
+    for (int h = hs; h < he; ++h) {
+        const int buf = h & 1;
+        const bool hex = hexcol && h < g.d0 - 1;
+        double H[27];
+        if (hex) {
+            Corner X[8];
+            Corner hi[4];
+            load_layer(g, h + 1, j, k, hi);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                X[q] = lo[q];
+                X[4 + q] = hi[q];
+            }
+            hex_jac1<PK>(X, mt, H);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) lo[q] = hi[q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 27; ++q) H[q] = 0.0;
+        }
+        // corners 1, 5 of the previous lane's hex, and its corners 3, 7
+        double rk[2][15], rjk[2][15];
+        sfor<2>([&](auto LV) {
+            constexpr int lev = PA_CI(LV);
+            sfor<15>([&](auto S) {
+                constexpr int s = PA_CI(S);
+                constexpr int e1 = jac_entry(lev << 2 | 1, s), e3 = jac_entry(lev << 2 | 3, s);
+                if constexpr (e1 >= 0) rk[lev][s] = shfl_prev(H[e1], l);
+                else rk[lev][s] = 0.0;
+                if constexpr (e3 >= 0) rjk[lev][s] = shfl_prev(H[e3], l);
+                else rjk[lev][s] = 0.0;
+            });
+        });
+        if (w < W - 1) {
+            sfor<2>([&](auto LV) {
+                constexpr int lev = PA_CI(LV);
+                sfor<15>([&](auto S) {
+                    constexpr int s = PA_CI(S);
+                    if constexpr ((jx_live(lev) >> s) & 1u) {
+                        constexpr int e2 = jac_entry(lev << 2 | 2, s);
+                        double v = rjk[lev][s];
+                        if constexpr (e2 >= 0) v += H[e2];
+                        xch[buf][w][jx_index(lev, s)][l] = v;
+                    }
+                });
+            });
+            if (top_lane) {
+                sfor<2>([&](auto LV) {
+                    constexpr int lev = PA_CI(LV);
+                    sfor<15>([&](auto S) {
+                        constexpr int s = PA_CI(S);
+                        constexpr int e3 = jac_entry(lev << 2 | 3, s);
+                        if constexpr (e3 >= 0) xtop[buf][w][jt_index(lev, s)] = H[e3];
+                    });
+                });
+            }
+        }
+        __syncthreads();
+        double row[15];
+        sfor<2>([&](auto LV) {
+            constexpr int lev = PA_CI(LV);
+            sfor<15>([&](auto S) {
+                constexpr int s = PA_CI(S);
+                constexpr int e0 = jac_entry(lev << 2, s);
+                double v = rk[lev][s];
+                if constexpr (e0 >= 0) v = H[e0] + v;
+                if constexpr ((jx_live(lev) >> s) & 1u) v += w > 0 ? xch[buf][w - 1][jx_index(lev, s)][l] : 0.0;
+                if constexpr (lev == 0) row[s] = carry[s] + v;
+                else carry[s] = ((CL >> s) & 1u) ? v : 0.0;
+            });
+        });
+        double trow[15];
+        if (top_lane) {
+            sfor<2>([&](auto LV) {
+                constexpr int lev = PA_CI(LV);
+                sfor<15>([&](auto S) {
+                    constexpr int s = PA_CI(S);
+                    constexpr int e1 = jac_entry(lev << 2 | 1, s), e3 = jac_entry(lev << 2 | 3, s);
+                    double v = 0.0;
+                    if constexpr (e1 >= 0) v = H[e1];
+                    if constexpr (e3 >= 0) v += w > 0 ? xtop[buf][w - 1][jt_index(lev, s)] : 0.0;
+                    if constexpr (lev == 0) trow[s] = topc[w][s] + v;
+                    else topc[w][s] = v;
+                });
+            });
+        }
+        if (line_owned && h >= B.X0) {
+            // the wave's rows [K0, K1) of line (h, j) are one CRS range
+            const long line = ((long)h * g.d1 + j) * g.d2;
+            const int base = rowmap[line + B.K0], end = rowmap[line + B.K1];
+            double* region = &xch[buf][w == 0 ? W - 1 : w - 1][0][0];
+            VT* stage = reinterpret_cast<VT*>(region);
+            wave_lds_sync();  // the exchange reads above come first
+            if (row_owned) store_row(stage, rowmap[line + k] - base, row_exists(g, h, j, k), row);
+            if (top_lane) store_row(stage, rowmap[line + g.d2 - 1] - base, row_exists(g, h, j, g.d2 - 1), trow);
+            wave_lds_sync();
+            VT* dst = vals + base;
+            for (int t = l; t < end - base; t += 64) dst[t] = stage[t];
+        }
+    }
+}
+
+// -------------------------------------------------------------- residual
+// Per corner NV = DPN values; the hex's eight corner sums.
+template <int DPN, int PK>
+__device__ inline void hex_residual(const Corner X[8], const double U[8][DPN], const Material& mt, double body,
+                                   double H[8][DPN]) {
+#pragma unroll
+    for (int v = 0; v < 8; ++v)
+#pragma unroll
+        for (int d = 0; d < DPN; ++d) H[v][d] = 0.0;
+    sfor<6>([&](auto T) {
+        constexpr int t = PA_CI(T);
+        constexpr int v[4] = {0, tet_p(t), tet_q(t), 7};
+        TetGeom g;
+        tet_geom(X[v[0]], X[v[1]], X[v[2]], X[v[3]], g);
+        const double s = penal_k<PK>(g.rho, mt.p, mt.emin);
+        if constexpr (DPN == 3) {
+            // eps(u; c) in Voigt order xx, yy, zz, yz, xz, xy (engineering shear)
+            double e[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const double* u = U[v[a]];
+                const double* c = g.c[a];
+                e[0] += u[0] * c[0];
+                e[1] += u[1] * c[1];
+                e[2] += u[2] * c[2];
+                e[3] += u[2] * c[1] + u[1] * c[2];
+                e[4] += u[2] * c[0] + u[0] * c[2];
+                e[5] += u[1] * c[0] + u[0] * c[1];
+            }
+            const double coef = s / (6.0 * fabs(g.det));
+            const double tr = (e[0] + e[1]) + e[2];
+            const double cl = coef * mt.lambda, c2m = coef * 2.0 * mt.mu, cm = coef * mt.mu;
+            const double sg[6] = {cl * tr + c2m * e[0], cl * tr + c2m * e[1], cl * tr + c2m * e[2],
+                                  cm * e[3], cm * e[4], cm * e[5]};
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const double* c = g.c[a];
+                H[v[a]][0] += (sg[0] * c[0] + sg[5] * c[1]) + sg[4] * c[2];
+                H[v[a]][1] += (sg[5] * c[0] + sg[1] * c[1]) + sg[3] * c[2];
+                H[v[a]][2] += (sg[4] * c[0] + sg[3] * c[1]) + sg[2] * c[2];
+            }
+        } else {
+            double gt[3] = {0, 0, 0};
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) gt[d] += U[v[a]][0] * g.c[a][d];
+            const double coef = (s * mt.k) / (6.0 * fabs(g.det));
+            // body source: -(cubature weight 1/6)|det J| f (1/4) rho (unpenalised mean density)
+            const double load = -1.0 * ((1.0 / 6.0) * fabs(g.det)) * body * 0.25 * g.rho;
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+                H[v[a]][0] += coef * ((gt[0] * g.c[a][0] + gt[1] * g.c[a][1]) + gt[2] * g.c[a][2]) + load;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    });
+}
+
+template <int DPN>
+__device__ inline void load_state(const double* __restrict__ u, int i, int j, int k, const GridArgs& g,
+                                  double U[4][DPN]) {
+    const long n = ((long)i * g.d1 + j) * g.d2 + k;
+    const long nn[4] = {n, n + 1, n + g.d2, n + g.d2 + 1};
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int d = 0; d < DPN; ++d) U[q][d] = u[nn[q] * DPN + d];
+}
+
+template <int DPN, int W, int PK>
+__global__ __launch_bounds__(W * 64, 2) void k_grid_residual(GridArgs g, Material mt, const double* __restrict__ u,
+                                                             double body, double scale,
+                                                             const double* __restrict__ fext,
+                                                             double* __restrict__ R) {
+    constexpr int NX = 2 * DPN;
+    __shared__ double xch[2][W][NX][64];
+    __shared__ double xtop[2][W][NX];
+    __shared__ double topc[W][DPN];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const Block B = grid_block(g, W);
+    const int j = B.hj0 + w, k = B.hk0 + l;
+    const bool line_owned = j >= B.J0 && j < B.J1;
+    const bool row_owned = line_owned && k >= B.K0 && k < B.K1;
+    const bool top_lane = B.top && l == 63;
+    const bool hexcol = j < g.d1 - 1 && k < g.d2 - 1;
+    const int hs = B.X0 == 0 ? 0 : B.X0 - 1;
+    const int he = B.X1 == g.d0 ? g.d0 : B.X1;
+    double carry[DPN];
+#pragma unroll
+    for (int d = 0; d < DPN; ++d) carry[d] = 0.0;
+    if (top_lane) {
+#pragma unroll
+        for (int d = 0; d < DPN; ++d) topc[w][d] = 0.0;
+    }
+    Corner lo[4];
+    double ulo[4][DPN];
+    if (hexcol) {
+        load_layer(g, hs, j, k, lo);
+        load_state<DPN>(u, hs, j, k, g, ulo);
+    }
+    for (int h = hs; h < he; ++h) {
+        const int buf = h & 1;
+        const bool hex = hexcol && h < g.d0 - 1;
+        double H[8][DPN];
+        if (hex) {
+            Corner X[8], hi[4];
+            double U[8][DPN], uhi[4][DPN];
+            load_layer(g, h + 1, j, k, hi);
+            load_state<DPN>(u, h + 1, j, k, g, uhi);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                X[q] = lo[q];
+                X[4 + q] = hi[q];
+#pragma unroll
+                for (int d = 0; d < DPN; ++d) {
+                    U[q][d] = ulo[q][d];
+                    U[4 + q][d] = uhi[q][d];
+                }
+            }
+            hex_residual<DPN, PK>(X, U, mt, body, H);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                lo[q] = hi[q];
+#pragma unroll
+                for (int d = 0; d < DPN; ++d) ulo[q][d] = uhi[q][d];
+            }
+        } else {
+#pragma unroll
+            for (int v = 0; v < 8; ++v)
+#pragma unroll
+                for (int d = 0; d < DPN; ++d) H[v][d] = 0.0;
+        }
+        double rk[2][DPN], rjk[2][DPN];
+#pragma unroll
+        for (int lev = 0; lev < 2; ++lev)
+#pragma unroll
+            for (int d = 0; d < DPN; ++d) {
+                rk[lev][d] = shfl_prev(H[lev << 2 | 1][d], l);
+                rjk[lev][d] = shfl_prev(H[lev << 2 | 3][d], l);
+            }
+        if (w < W - 1) {
+#pragma unroll
+            for (int lev = 0; lev < 2; ++lev)
+#pragma unroll
+                for (int d = 0; d < DPN; ++d) xch[buf][w][lev * DPN + d][l] = H[lev << 2 | 2][d] + rjk[lev][d];
+            if (top_lane) {
+#pragma unroll
+                for (int lev = 0; lev < 2; ++lev)
+#pragma unroll
+                    for (int d = 0; d < DPN; ++d) xtop[buf][w][lev * DPN + d] = H[lev << 2 | 3][d];
+            }
+        }
+        __syncthreads();
+        double row[DPN], trow[DPN];
+#pragma unroll
+        for (int d = 0; d < DPN; ++d) {
+            const double j0 = w > 0 ? xch[buf][w - 1][d][l] : 0.0;
+            const double j1 = w > 0 ? xch[buf][w - 1][DPN + d][l] : 0.0;
+            row[d] = carry[d] + ((H[0][d] + rk[0][d]) + j0);
+            carry[d] = (H[4][d] + rk[1][d]) + j1;
+        }
+        if (top_lane) {
+#pragma unroll
+            for (int d = 0; d < DPN; ++d) {
+                const double t0 = w > 0 ? xtop[buf][w - 1][d] : 0.0;
+                const double t1 = w > 0 ? xtop[buf][w - 1][DPN + d] : 0.0;
+                trow[d] = topc[w][d] + (H[1][d] + t0);
+                topc[w][d] = H[5][d] + t1;
+            }
+        }
+        if (line_owned && h >= B.X0) {
+            const long line = ((long)h * g.d1 + j) * g.d2;
+            if (row_owned) {
+                const long o = (line + k) * DPN;
+#pragma unroll
+                for (int d = 0; d < DPN; ++d) R[o + d] = scale * row[d] + (fext ? fext[o + d] : 0.0);
+            }
+            if (top_lane) {
+                const long o = (line + g.d2 - 1) * DPN;
+#pragma unroll
+                for (int d = 0; d < DPN; ++d) R[o + d] = scale * trow[d] + (fext ? fext[o + d] : 0.0);
+            }
+        }
+    }
+}
+
+// Launch geometry: line blocks, z blocks, x blocks (about 8 workgroups per
+// CU in all, each walking at least 4 layers).
+GridArgs grid_args(const pa_mesh* m, int W) {
+    GridArgs g{};
+    g.d0 = m->kgrid[0];
+    g.d1 = m->kgrid[1];
+    g.d2 = m->kgrid[2];
+    g.nj = 1;
+    while (W + (W - 1) * (g.nj - 1) < g.d1) ++g.nj;
+    g.nk = 1;
+    for (;;) {
+        const int K0 = g.nk == 1 ? 0 : 64 + 63 * (g.nk - 2);
+        int K1 = std::min(K0 + (g.nk == 1 ? 64 : 63), g.d2);
+        const int hk0 = g.nk == 1 ? 0 : K0 - 1;
+        if (K1 == g.d2 - 1 && hk0 + 64 == g.d2 - 1) K1 = g.d2;
+        if (K1 >= g.d2) break;
+        ++g.nk;
+    }
+    const long lines = (long)g.nj * g.nk;
+    const long target = 2048;
+    int nx = (int)std::max<long>(1, std::min<long>((target + lines - 1) / lines, (g.d0 + 3) / 4));
+    g.lx = (g.d0 + nx - 1) / nx;
+    g.x4 = reinterpret_cast<const double4*>(m->d_coords4.p);
+    return g;
+}
+
+int grid_blocks(const GridArgs& g) { return g.nj * g.nk * ((g.d0 + g.lx - 1) / g.lx); }
+
+}  // namespace
+
+bool grid_residual(const pa_mesh* m, int dpn, const Material& mt, const double* u, const double* z, double body,
+                   double scale, const double* fext, double* R, hipStream_t st) {
+    if (!m->has_grid) return false;
+    GridArgs g = grid_args(m, kGridWaves);
+    g.z = z;
+    const int nb = grid_blocks(g), nt = kGridWaves * 64;
+    auto run = [&](auto kernel) { kernel<<<nb, nt, 0, st>>>(g, mt, u, body, scale, fext, R); };
+    switch (penal_kind(mt.p) * 4 + dpn) {
+        case 3 * 4 + 3: run(k_grid_residual<3, kGridWaves, 3>); break;
+        case 3 * 4 + 1: run(k_grid_residual<1, kGridWaves, 3>); break;
+        case 2 * 4 + 3: run(k_grid_residual<3, kGridWaves, 2>); break;
+        case 2 * 4 + 1: run(k_grid_residual<1, kGridWaves, 2>); break;
+        case 1 * 4 + 3: run(k_grid_residual<3, kGridWaves, 1>); break;
+        case 1 * 4 + 1: run(k_grid_residual<1, kGridWaves, 1>); break;
+        case 3: run(k_grid_residual<3, kGridWaves, 0>); break;
+        default: run(k_grid_residual<1, kGridWaves, 0>); break;
+    }
+    PA_HIP(hipGetLastError());
+    return true;
+}
+
+template <class VT>
+bool grid_jacobian1(const pa_mesh* m, const Material& mt, const double* z, VT* vals, hipStream_t st) {
+    if (!m->has_grid) return false;
+    GridArgs g = grid_args(m, kGridWaves);
+    g.z = z;
+    const int nb = grid_blocks(g), nt = kGridWaves * 64;
+    auto run = [&](auto kernel) { kernel<<<nb, nt, 0, st>>>(g, mt, m->d_rowmap.p, vals); };
+    switch (penal_kind(mt.p)) {
+        case 3: run(k_grid_jacobian1<VT, kGridWaves, 3>); break;
+        case 2: run(k_grid_jacobian1<VT, kGridWaves, 2>); break;
+        case 1: run(k_grid_jacobian1<VT, kGridWaves, 1>); break;
+        default: run(k_grid_jacobian1<VT, kGridWaves, 0>); break;
+    }
+    PA_HIP(hipGetLastError());
+    return true;
+}
+
+template bool grid_jacobian1<double>(const pa_mesh*, const Material&, const double*, double*, hipStream_t);
+template bool grid_jacobian1<float>(const pa_mesh*, const Material&, const double*, float*, hipStream_t);
+
+}  // namespace pa

@@ -433,6 +433,65 @@ static void build_tiles(pa_mesh* m, const double* xyz, const std::vector<int>& c
     m->has_tiles = true;
 }
 
+// Kuhn grid check for the streaming grid kernels: the nodes form a
+// lexicographic grid (z fastest) and the cells are exactly the six tets of
+// every hex around its v0 -> v7 diagonal (in any order, any vertex order), so
+// the graph is the 15-point stencil clipped at the boundary.
+static void build_kuhn_grid(pa_mesh* m, const double* xyz, const std::vector<int>& conn) {
+    m->has_grid = false;
+    const char* env = std::getenv("PA_GRID");
+    if (env && env[0] == '0') return;
+    int d[3];
+    if (!detect_grid(xyz, m->N, d) || d[0] < 2) return;
+    const long nh = (long)(d[0] - 1) * (d[1] - 1) * (d[2] - 1);
+    if ((long)m->C != 6 * nh) return;
+    // (only the numbering matters: the kernels take every cell's geometry
+    // from its coordinates)
+    static const int kTets[6][4] = {{0, 2, 6, 7}, {0, 2, 3, 7}, {0, 1, 3, 7}, {0, 1, 5, 7}, {0, 4, 5, 7}, {0, 4, 6, 7}};
+    std::vector<unsigned char> mask(nh, 0);
+    const long s1 = d[2], s0 = (long)d[1] * d[2];
+    for (long c = 0; c < m->C; ++c) {
+        int ii[4], jj[4], kk[4];
+        for (int a = 0; a < 4; ++a) {
+            const long n = conn[4 * c + a];
+            ii[a] = (int)(n / s0), jj[a] = (int)((n / s1) % d[1]), kk[a] = (int)(n % s1);
+        }
+        const int i0 = *std::min_element(ii, ii + 4), j0 = *std::min_element(jj, jj + 4),
+                  k0 = *std::min_element(kk, kk + 4);
+        if (i0 >= d[0] - 1 || j0 >= d[1] - 1 || k0 >= d[2] - 1) return;
+        int v[4];
+        for (int a = 0; a < 4; ++a) {
+            const int di = ii[a] - i0, dj = jj[a] - j0, dk = kk[a] - k0;
+            if (di > 1 || dj > 1 || dk > 1) return;
+            v[a] = di << 2 | dj << 1 | dk;
+        }
+        std::sort(v, v + 4);
+        int t = 0;
+        while (t < 6 && !std::equal(v, v + 4, kTets[t])) ++t;
+        if (t == 6) return;
+        const long h = ((long)i0 * (d[1] - 1) + j0) * (d[2] - 1) + k0;
+        if (mask[h] & (1u << t)) return;  // the same tet twice
+        mask[h] |= (unsigned char)(1u << t);
+    }
+    for (long h = 0; h < nh; ++h)
+        if (mask[h] != 0x3f) return;
+    // every row is the clipped 15-point stencil
+    for (int n = 0; n < m->N; ++n) {
+        const int i = n / (int)s0, j = (n / (int)s1) % d[1], k = n % (int)s1;
+        int len = 0;
+        static const int off[15][3] = {{-1, -1, -1}, {-1, -1, 0}, {-1, 0, -1}, {-1, 0, 0}, {0, -1, -1},
+                                       {0, -1, 0},   {0, 0, -1},  {0, 0, 0},   {0, 0, 1},  {0, 1, 0},
+                                       {0, 1, 1},    {1, 0, 0},   {1, 0, 1},   {1, 1, 0},  {1, 1, 1}};
+        for (const auto& o : off) {
+            const int a = i + o[0], b = j + o[1], c = k + o[2];
+            len += a >= 0 && a < d[0] && b >= 0 && b < d[1] && c >= 0 && c < d[2];
+        }
+        if (m->h_rowmap[n + 1] - m->h_rowmap[n] != len) return;
+    }
+    m->kgrid[0] = d[0], m->kgrid[1] = d[1], m->kgrid[2] = d[2];
+    m->has_grid = true;
+}
+
 }  // namespace pa
 
 using pa::guarded;
@@ -615,6 +674,7 @@ int pa_mesh_create(const double* coords, int N, const int* conn, int C, pa_mesh*
             {
                 std::vector<int> cv(conn, conn + 4L * C);
                 pa::build_tiles(m, coords, cv, off, pairs);
+                pa::build_kuhn_grid(m, coords, cv);
             }
         } catch (...) {
             delete m;
@@ -643,6 +703,13 @@ int pa_mesh_assembly_info(const pa_mesh* m, int* tiles, int* tile_patterns, int*
         if (tiles) *tiles = m->has_tiles ? m->num_tiles : 0;
         if (tile_patterns) *tile_patterns = m->has_tiles ? m->num_tile_patterns : 0;
         if (row_patterns) *row_patterns = m->has_patterns ? m->num_patterns : 0;
+    });
+}
+
+int pa_mesh_grid_info(const pa_mesh* m, int* dims) {
+    return guarded([&] {
+        PA_REQUIRE(m && dims, PA_ERR_ARG, "pa_mesh_grid_info: null argument");
+        for (int d = 0; d < 3; ++d) dims[d] = m->has_grid ? m->kgrid[d] : 0;
     });
 }
 

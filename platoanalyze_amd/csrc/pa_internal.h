@@ -124,9 +124,6 @@ struct pa_mesh {
     pa::DevBuf<unsigned char> d_pat_a;    // per pattern pair: local index of the row node in the cell
     pa::DevBuf<int4> d_pat_delta;         // per pattern pair: node id - row id of the cell's four nodes
     pa::DevBuf<uint4> d_pat_item2;        // work items with their (<= 8) entries: slot | count << 8, entry bytes
-    // per-call packed node records (x, y, z, density); every function on this
-    // mesh shares it, so their calls must not overlap in time
-    mutable pa::DevBuf<double4> d_nrec;
     pa::DevBuf<unsigned char> d_pat_ent;  // contributions (pair << 2 | local node b), grouped by column slot
     pa::DevBuf<unsigned> d_pat_item;      // work items: slot | start << 8 | count << 20
     // Cell tiles: rows grouped into blocks of the lexicographic node grid; a
@@ -138,6 +135,11 @@ struct pa_mesh {
     int tile_unit_entries = 0;        // entries per work unit the kernels evaluate (6 or 8)
     int tile_blocks_max = 0;          // output blocks of a tile (its staging size)
     int grid_dims[3] = {0, 0, 0};     // detected node grid (x, y, z node counts)
+    // Kuhn grid: nodes a lexicographic grid (z fastest) and cells exactly the
+    // six tets of every hex around its v0 -> v7 diagonal; the streaming grid
+    // kernels apply
+    bool has_grid = false;
+    int kgrid[3] = {0, 0, 0};
     int side_set_version = 0;         // bumped by every side-set definition
     pa::DevBuf<int4> d_tile;          // 3 per tile: {first row, row_off, cell_off, unit_off},
                                       // {rows, cells, units, runs}, {run_off, blocks, 0, 0}
@@ -182,6 +184,14 @@ struct pa_vector_function {
 };
 
 namespace pa {
+struct Material;
+// Streaming kernels on Kuhn grids (pa_grid.hip); false when the mesh is not one.
+// Residual rows R = scale * (assembled + body source) + fext (fext may be null).
+bool grid_residual(const pa_mesh* m, int dpn, const Material& mt, const double* u, const double* z, double body,
+                   double scale, const double* fext, double* R, hipStream_t st);
+// Thermal (1 x 1 block) Jacobian values in the mesh's CRS order.
+template <class VT>
+bool grid_jacobian1(const pa_mesh* m, const Material& mt, const double* z, VT* vals, hipStream_t st);
 // Criterion gradient_x (N*3 doubles), scaled (the self-adjoint total
 // gradient wrt the coordinates is -1 times it).
 void criterion_gradient_x(const pa_scalar_function* sf, const double* u, const double* z, double scale, double* g,

@@ -33,15 +33,19 @@ def rel(a, b):
 
 
 def _box(pa, dims, L, tiles):
-    old = os.environ.get("PA_TILES")
+    # the streaming grid kernels are off here (they take over box meshes):
+    # these tests compare the tile kernels with the row kernels
+    old = {k: os.environ.get(k) for k in ("PA_TILES", "PA_GRID")}
     os.environ["PA_TILES"] = "1" if tiles else "0"
+    os.environ["PA_GRID"] = "0"
     try:
         return pa.Mesh.box(*dims, *L)
     finally:
-        if old is None:
-            del os.environ["PA_TILES"]
-        else:
-            os.environ["PA_TILES"] = old
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
 
 
 def _functions(pa, mesh, physics):
