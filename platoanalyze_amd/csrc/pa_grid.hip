@@ -40,6 +40,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <type_traits>
 #include <utility>
 
@@ -120,30 +121,51 @@ struct Corner {
     double x, y, z, r;
 };
 
-// Cofactor vectors and det J of tet (0, p, q, 7) of a hex.
-struct TetGeom {
-    double c[4][3];
-    double det;
-    double rho;
+__device__ inline void cross(const double a[3], const double b[3], double c[3]) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+__device__ inline double dot3(const double a[3], const double b[3]) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+
+// Edge vectors and cross products shared by the six tets of a hex:
+// e[p] = x_p - x_7 (p = 0..6), F[p] = e[p] x e[0] (p = 1..6).  Tet (0, p, q, 7)
+// (local nodes 0..3 = corners 0, p, q, 7) has det J = e[0] . (e[p] x e[q]) and
+// cofactor vectors c_0 = e[p] x e[q], c_1 = e[q] x e[0] = F[q],
+// c_2 = e[0] x e[p] = -F[p], c_3 = -(c_0 + c_1 + c_2).
+struct HexGeom {
+    double e[7][3];
+    double F[7][3];  // 1..6
 };
 
-__device__ inline void tet_geom(const Corner& A, const Corner& B, const Corner& Cq, const Corner& D, TetGeom& g) {
-    const double e0x = A.x - D.x, e0y = A.y - D.y, e0z = A.z - D.z;
-    const double e1x = B.x - D.x, e1y = B.y - D.y, e1z = B.z - D.z;
-    const double e2x = Cq.x - D.x, e2y = Cq.y - D.y, e2z = Cq.z - D.z;
-    g.c[0][0] = e1y * e2z - e1z * e2y;
-    g.c[0][1] = e1z * e2x - e1x * e2z;
-    g.c[0][2] = e1x * e2y - e1y * e2x;
-    g.c[1][0] = e2y * e0z - e2z * e0y;
-    g.c[1][1] = e2z * e0x - e2x * e0z;
-    g.c[1][2] = e2x * e0y - e2y * e0x;
-    g.c[2][0] = e0y * e1z - e0z * e1y;
-    g.c[2][1] = e0z * e1x - e0x * e1z;
-    g.c[2][2] = e0x * e1y - e0y * e1x;
-    g.det = (e0x * g.c[0][0] + e0y * g.c[0][1]) + e0z * g.c[0][2];
+template <bool WITH_F = true>
+__device__ inline void hex_geom(const Corner X[8], HexGeom& h) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) g.c[3][d] = -((g.c[0][d] + g.c[1][d]) + g.c[2][d]);
-    g.rho = (((A.r + B.r) + Cq.r) + D.r) / 4.0;
+    for (int p = 0; p < 7; ++p) {
+        h.e[p][0] = X[p].x - X[7].x;
+        h.e[p][1] = X[p].y - X[7].y;
+        h.e[p][2] = X[p].z - X[7].z;
+    }
+    if constexpr (WITH_F) {
+#pragma unroll
+        for (int p = 1; p < 7; ++p) cross(h.e[p], h.e[0], h.F[p]);
+    }
+}
+
+// Mean nodal density of tet (0, p, q, 7), in the cell's node order.
+__device__ inline double tet_rho(const Corner X[8], int p, int q) {
+    return (((X[0].r + X[p].r) + X[q].r) + X[7].r) * 0.25;
+}
+
+// 1/x: the hardware estimate refined by two Newton steps (each one squares
+// the relative error), well inside an ulp-level bar.
+__device__ inline double recip(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
 }
 
 // ------------------------------------------------------------------ blocks
@@ -201,9 +223,13 @@ __device__ inline void load_layer(const GridArgs& g, int i, int j, int k, Corner
     load_corner(g, n + g.d2 + 1, c[3]);
 }
 
-__device__ inline double shfl_prev(double v, int lane) {
-    const double r = __shfl_up(v, 1, 64);
-    return lane == 0 ? 0.0 : r;
+// The value of lane l - 1 (0 on lane 0): a DPP wave shift right by one lane,
+// whose out-of-range source lane reads as the zero `old` operand.
+__device__ inline double shfl_prev(double v, int) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffull), 0x138, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), 0x138, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
 // Orders a wave's LDS writes before its own lanes' later reads.
@@ -251,44 +277,63 @@ __host__ __device__ constexpr unsigned carry_live() {
     return jac_live(4) | jac_live(5) | jac_live(6) | jac_live(7);
 }
 constexpr int kStage = 65 * 15;  // staged entries of one wave's rows (64 rows + the top row)
-static_assert(kJX * 64 >= kStage, "a wave's staging fits its exchange slot");
+static_assert(kJX * 64 >= kStage + 1, "a wave's staging (and its spare entry) fits its exchange slot");
 
 // The 27 values of one hex: w-scaled products c_a . c_b summed over its tets.
 template <int PK>
 __device__ inline void hex_jac1(const Corner X[8], const Material& mt, double H[27]) {
 #pragma unroll
     for (int q = 0; q < 27; ++q) H[q] = 0.0;
+    HexGeom hg;
+    hex_geom(X, hg);
     sfor<6>([&](auto T) {
         constexpr int t = PA_CI(T);
-        constexpr int v[4] = {0, tet_p(t), tet_q(t), 7};
-        TetGeom g;
-        tet_geom(X[v[0]], X[v[1]], X[v[2]], X[v[3]], g);
-        const double coef = (penal_k<PK>(g.rho, mt.p, mt.emin) * mt.k) / (6.0 * fabs(g.det));
+        constexpr int p = tet_p(t), q = tet_q(t);
+        constexpr int v[4] = {0, p, q, 7};
+        double G[3];  // c_0
+        cross(hg.e[p], hg.e[q], G);
+        const double det = dot3(hg.e[0], G);
+        // products c_a . c_b (a <= b); those with c_3 from c_3 = -(c_0 + c_1 + c_2)
+        double P[4][4];
+        P[0][0] = dot3(G, G);
+        P[0][1] = dot3(G, hg.F[q]);
+        P[0][2] = -dot3(G, hg.F[p]);
+        P[1][1] = dot3(hg.F[q], hg.F[q]);
+        P[1][2] = -dot3(hg.F[q], hg.F[p]);
+        P[2][2] = dot3(hg.F[p], hg.F[p]);
+        P[0][3] = -((P[0][0] + P[0][1]) + P[0][2]);
+        P[1][3] = -((P[0][1] + P[1][1]) + P[1][2]);
+        P[2][3] = -((P[0][2] + P[1][2]) + P[2][2]);
+        P[3][3] = -((P[0][3] + P[1][3]) + P[2][3]);
+        // s(rho) k V grad N_a . grad N_b = s k / (6 |det|) c_a . c_b
+        const double coef = (penal_k<PK>(tet_rho(X, p, q), mt.p, mt.emin) * mt.k) * recip(6.0 * fabs(det));
         sfor<4>([&](auto A) {
             constexpr int a = PA_CI(A);
             sfor<4>([&](auto Bq) {
                 constexpr int b = PA_CI(Bq);
                 if constexpr (b >= a) {
                     constexpr int idx = a == b ? v[a] : 8 + edge_of(v[a], v[b]);
-                    H[idx] += coef * ((g.c[a][0] * g.c[b][0] + g.c[a][1] * g.c[b][1]) + g.c[a][2] * g.c[b][2]);
+                    H[idx] = fma(coef, P[a][b], H[idx]);
                 }
             });
         });
-        // one tet at a time: interleaving the six tets' geometry would keep
-        // all their temporaries live at once
+        // one tet at a time: interleaving the six tets would keep all their
+        // temporaries live at once
         __builtin_amdgcn_sched_barrier(0);
     });
 }
 
+// A row's entries of the neighbours inside the grid, packed in column order
+// at stage[off...]; the slots of missing neighbours (zero) go to a spare
+// entry at stage[trash] instead, so there are no branches.
 template <class VT>
-__device__ inline void store_row(VT* stage, int off, unsigned exist, const double row[15]) {
-    int pos = 0;
+__device__ inline void store_row(VT* stage, int off, int trash, unsigned exist, const double row[15]) {
+    int pos = off;
 #pragma unroll
     for (int s = 0; s < 15; ++s) {
-        if ((exist >> s) & 1u) {
-            stage[off + pos] = (VT)row[s];
-            ++pos;
-        }
+        const bool in = (exist >> s) & 1u;
+        stage[in ? pos : trash] = (VT)row[s];
+        pos += in ? 1 : 0;
     }
 }
 
@@ -318,27 +363,45 @@ __global__ __launch_bounds__(W * 64, 2) void k_grid_jacobian1(GridArgs g, Materi
 #pragma unroll
         for (int s = 0; s < 15; ++s) topc[w][s] = 0.0;
     }
-    Corner lo[4];
-    if (hexcol) load_layer(g, hs, j, k, lo);
+    // corners of node layers h (lo) and h + 1 (hi); the next layer's are
+    // loaded one step ahead, with this step's row offsets, so their latency
+    // hides behind the hex arithmetic
+    Corner lo[4], hi[4];
+    if (hexcol) {
+        load_layer(g, hs, j, k, lo);
+        if (hs < g.d0 - 1) load_layer(g, hs + 1, j, k, hi);
+    }
     for (int h = hs; h < he; ++h) {
         const int buf = h & 1;
         const bool hex = hexcol && h < g.d0 - 1;
+        Corner nx[4];
+        if (hexcol && h + 1 < g.d0 - 1 && h + 1 < he) load_layer(g, h + 2, j, k, nx);
+        const bool out = line_owned && h >= B.X0;
+        const long line = ((long)h * g.d1 + j) * g.d2;
+        int rbase = 0, rend = 0, rrow = 0, rtop = 0;
+        if (out) {
+            rbase = rowmap[line + B.K0];
+            rend = rowmap[line + B.K1];
+            if (row_owned) rrow = rowmap[line + k];
+            if (top_lane) rtop = rowmap[line + g.d2 - 1];
+        }
         double H[27];
         if (hex) {
             Corner X[8];
-            Corner hi[4];
-            load_layer(g, h + 1, j, k, hi);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 X[q] = lo[q];
                 X[4 + q] = hi[q];
             }
             hex_jac1<PK>(X, mt, H);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) lo[q] = hi[q];
         } else {
 #pragma unroll
             for (int q = 0; q < 27; ++q) H[q] = 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            lo[q] = hi[q];
+            hi[q] = nx[q];
         }
         // corners 1, 5 of the previous lane's hex, and its corners 3, 7
         double rk[2][15], rjk[2][15];
@@ -406,18 +469,25 @@ __global__ __launch_bounds__(W * 64, 2) void k_grid_jacobian1(GridArgs g, Materi
                 });
             });
         }
-        if (line_owned && h >= B.X0) {
+        if (out) {
             // the wave's rows [K0, K1) of line (h, j) are one CRS range
-            const long line = ((long)h * g.d1 + j) * g.d2;
-            const int base = rowmap[line + B.K0], end = rowmap[line + B.K1];
             double* region = &xch[buf][w == 0 ? W - 1 : w - 1][0][0];
             VT* stage = reinterpret_cast<VT*>(region);
             wave_lds_sync();  // the exchange reads above come first
-            if (row_owned) store_row(stage, rowmap[line + k] - base, row_exists(g, h, j, k), row);
-            if (top_lane) store_row(stage, rowmap[line + g.d2 - 1] - base, row_exists(g, h, j, g.d2 - 1), trow);
+            if (row_owned) store_row(stage, rrow - rbase, kStage, row_exists(g, h, j, k), row);
+            if (top_lane) store_row(stage, rtop - rbase, kStage, row_exists(g, h, j, g.d2 - 1), trow);
             wave_lds_sync();
-            VT* dst = vals + base;
-            for (int t = l; t < end - base; t += 64) dst[t] = stage[t];
+            VT* dst = vals + rbase;
+            const int n = rend - rbase;
+            // four staged entries per lane read before any is stored
+            for (int t = l; t < n; t += 256) {
+                VT a[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) a[q] = t + 64 * q < n ? stage[t + 64 * q] : VT(0);
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (t + 64 * q < n) dst[t + 64 * q] = a[q];
+            }
         }
     }
 }
@@ -431,50 +501,80 @@ __device__ inline void hex_residual(const Corner X[8], const double U[8][DPN], c
     for (int v = 0; v < 8; ++v)
 #pragma unroll
         for (int d = 0; d < DPN; ++d) H[v][d] = 0.0;
+    HexGeom hg;
+    hex_geom<DPN == 1>(X, hg);
+    // state differences to corner 7: sum_b u_b (x) c_b = sum_{b < 3} (u_b - u_3) (x) c_b
+    double du[7][DPN];
+#pragma unroll
+    for (int p = 0; p < 7; ++p)
+#pragma unroll
+        for (int d = 0; d < DPN; ++d) du[p][d] = U[p][d] - U[7][d];
     sfor<6>([&](auto T) {
         constexpr int t = PA_CI(T);
-        constexpr int v[4] = {0, tet_p(t), tet_q(t), 7};
-        TetGeom g;
-        tet_geom(X[v[0]], X[v[1]], X[v[2]], X[v[3]], g);
-        const double s = penal_k<PK>(g.rho, mt.p, mt.emin);
+        constexpr int p = tet_p(t), q = tet_q(t);
+        constexpr int v[4] = {0, p, q, 7};
+        double c[3][3];  // c_0, c_1, c_2
+        cross(hg.e[p], hg.e[q], c[0]);
+        if constexpr (DPN == 3) {
+            // the elastic kernel has no registers to keep the shared F[p]
+            cross(hg.e[q], hg.e[0], c[1]);
+            cross(hg.e[0], hg.e[p], c[2]);
+        } else {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                c[1][d] = hg.F[q][d];
+                c[2][d] = -hg.F[p][d];
+            }
+        }
+        const double det = dot3(hg.e[0], c[0]);
+        const double* dl[3] = {du[0], du[p], du[q]};
+        const double rho = tet_rho(X, p, q);
+        const double s = penal_k<PK>(rho, mt.p, mt.emin);
+        double f[3][DPN];  // nodal residuals of local nodes 0..2; node 3's is minus their sum
         if constexpr (DPN == 3) {
             // eps(u; c) in Voigt order xx, yy, zz, yz, xz, xy (engineering shear)
             double e[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const double* u = U[v[a]];
-                const double* c = g.c[a];
-                e[0] += u[0] * c[0];
-                e[1] += u[1] * c[1];
-                e[2] += u[2] * c[2];
-                e[3] += u[2] * c[1] + u[1] * c[2];
-                e[4] += u[2] * c[0] + u[0] * c[2];
-                e[5] += u[1] * c[0] + u[0] * c[1];
+            for (int a = 0; a < 3; ++a) {
+                const double* u = dl[a];
+                const double* ca = c[a];
+                e[0] += u[0] * ca[0];
+                e[1] += u[1] * ca[1];
+                e[2] += u[2] * ca[2];
+                e[3] += u[2] * ca[1] + u[1] * ca[2];
+                e[4] += u[2] * ca[0] + u[0] * ca[2];
+                e[5] += u[1] * ca[0] + u[0] * ca[1];
             }
-            const double coef = s / (6.0 * fabs(g.det));
+            const double coef = s * recip(6.0 * fabs(det));
             const double tr = (e[0] + e[1]) + e[2];
             const double cl = coef * mt.lambda, c2m = coef * 2.0 * mt.mu, cm = coef * mt.mu;
             const double sg[6] = {cl * tr + c2m * e[0], cl * tr + c2m * e[1], cl * tr + c2m * e[2],
                                   cm * e[3], cm * e[4], cm * e[5]};
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const double* c = g.c[a];
-                H[v[a]][0] += (sg[0] * c[0] + sg[5] * c[1]) + sg[4] * c[2];
-                H[v[a]][1] += (sg[5] * c[0] + sg[1] * c[1]) + sg[3] * c[2];
-                H[v[a]][2] += (sg[4] * c[0] + sg[3] * c[1]) + sg[2] * c[2];
+            for (int a = 0; a < 3; ++a) {
+                const double* ca = c[a];
+                f[a][0] = (sg[0] * ca[0] + sg[5] * ca[1]) + sg[4] * ca[2];
+                f[a][1] = (sg[5] * ca[0] + sg[1] * ca[1]) + sg[3] * ca[2];
+                f[a][2] = (sg[4] * ca[0] + sg[3] * ca[1]) + sg[2] * ca[2];
             }
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) H[v[a]][d] += f[a][d];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) H[7][d] -= (f[0][d] + f[1][d]) + f[2][d];
         } else {
-            double gt[3] = {0, 0, 0};
+            double gt[3];
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int d = 0; d < 3; ++d) gt[d] += U[v[a]][0] * g.c[a][d];
-            const double coef = (s * mt.k) / (6.0 * fabs(g.det));
+            for (int d = 0; d < 3; ++d) gt[d] = (dl[0][0] * c[0][d] + dl[1][0] * c[1][d]) + dl[2][0] * c[2][d];
+            const double coef = (s * mt.k) * recip(6.0 * fabs(det));
             // body source: -(cubature weight 1/6)|det J| f (1/4) rho (unpenalised mean density)
-            const double load = -1.0 * ((1.0 / 6.0) * fabs(g.det)) * body * 0.25 * g.rho;
+            const double load = -1.0 * ((1.0 / 6.0) * fabs(det)) * body * 0.25 * rho;
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
-                H[v[a]][0] += coef * ((gt[0] * g.c[a][0] + gt[1] * g.c[a][1]) + gt[2] * g.c[a][2]) + load;
+            for (int a = 0; a < 3; ++a) f[a][0] = coef * dot3(gt, c[a]);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) H[v[a]][0] += f[a][0] + load;
+            H[7][0] += load - ((f[0][0] + f[1][0]) + f[2][0]);
         }
         __builtin_amdgcn_sched_barrier(0);
     });
@@ -516,21 +616,38 @@ __global__ __launch_bounds__(W * 64, 2) void k_grid_residual(GridArgs g, Materia
 #pragma unroll
         for (int d = 0; d < DPN; ++d) topc[w][d] = 0.0;
     }
-    Corner lo[4];
-    double ulo[4][DPN];
+    // corners and states of node layers h (lo) and h + 1 (hi).  The thermal
+    // kernel loads the next layer's one step ahead (its latency hides behind
+    // the hex arithmetic); the elastic one has no registers to spare for it
+    // and loads layer h + 1 in the step.
+    constexpr bool AHEAD = DPN == 1;
+    Corner lo[4], hi[4];
+    double ulo[4][DPN], uhi[4][DPN];
     if (hexcol) {
         load_layer(g, hs, j, k, lo);
         load_state<DPN>(u, hs, j, k, g, ulo);
+        if (AHEAD && hs < g.d0 - 1) {
+            load_layer(g, hs + 1, j, k, hi);
+            load_state<DPN>(u, hs + 1, j, k, g, uhi);
+        }
     }
     for (int h = hs; h < he; ++h) {
         const int buf = h & 1;
         const bool hex = hexcol && h < g.d0 - 1;
-        double H[8][DPN];
-        if (hex) {
-            Corner X[8], hi[4];
-            double U[8][DPN], uhi[4][DPN];
+        Corner nx[4];
+        double unx[4][DPN];
+        if (AHEAD && hexcol && h + 1 < g.d0 - 1 && h + 1 < he) {
+            load_layer(g, h + 2, j, k, nx);
+            load_state<DPN>(u, h + 2, j, k, g, unx);
+        }
+        if (!AHEAD && hex) {
             load_layer(g, h + 1, j, k, hi);
             load_state<DPN>(u, h + 1, j, k, g, uhi);
+        }
+        double H[8][DPN];
+        if (hex) {
+            Corner X[8];
+            double U[8][DPN];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 X[q] = lo[q];
@@ -542,17 +659,22 @@ __global__ __launch_bounds__(W * 64, 2) void k_grid_residual(GridArgs g, Materia
                 }
             }
             hex_residual<DPN, PK>(X, U, mt, body, H);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                lo[q] = hi[q];
-#pragma unroll
-                for (int d = 0; d < DPN; ++d) ulo[q][d] = uhi[q][d];
-            }
         } else {
 #pragma unroll
             for (int v = 0; v < 8; ++v)
 #pragma unroll
                 for (int d = 0; d < DPN; ++d) H[v][d] = 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            lo[q] = hi[q];
+#pragma unroll
+            for (int d = 0; d < DPN; ++d) ulo[q][d] = uhi[q][d];
+            if constexpr (AHEAD) {
+                hi[q] = nx[q];
+#pragma unroll
+                for (int d = 0; d < DPN; ++d) uhi[q][d] = unx[q][d];
+            }
         }
         double rk[2][DPN], rjk[2][DPN];
 #pragma unroll
@@ -636,25 +758,55 @@ GridArgs grid_args(const pa_mesh* m, int W) {
 
 int grid_blocks(const GridArgs& g) { return g.nj * g.nk * ((g.d0 + g.lx - 1) / g.lx); }
 
+// Waves per workgroup of the grid kernels: kGridWaves, or PA_GRID_WAVES=8
+// (fewer halo lines, one workgroup per CU).
+int grid_waves() {
+    static const int w = [] {
+        const char* e = std::getenv("PA_GRID_WAVES");
+        return e && std::atoi(e) == 8 ? 8 : kGridWaves;
+    }();
+    return w;
+}
+
+// Exponent classes with their own code: 3 (the configurations), 1 (the
+// reference's unit tests), any other through pow.
+template <class Run, class K3, class K1, class K0>
+void by_penalty(double p, Run&& run, K3 k3, K1 k1, K0 k0) {
+    const int pk = penal_kind(p);
+    if (pk == 3) run(k3);
+    else if (pk == 1) run(k1);
+    else run(k0);
+}
+
+template <int W>
+void grid_residual_w(const pa_mesh* m, int dpn, const Material& mt, const double* u, const double* z, double body,
+                     double scale, const double* fext, double* R, hipStream_t st) {
+    GridArgs g = grid_args(m, W);
+    g.z = z;
+    const int nb = grid_blocks(g);
+    auto run = [&](auto kernel) { kernel<<<nb, W * 64, 0, st>>>(g, mt, u, body, scale, fext, R); };
+    if (dpn == 3)
+        by_penalty(mt.p, run, k_grid_residual<3, W, 3>, k_grid_residual<3, W, 1>, k_grid_residual<3, W, 0>);
+    else
+        by_penalty(mt.p, run, k_grid_residual<1, W, 3>, k_grid_residual<1, W, 1>, k_grid_residual<1, W, 0>);
+}
+
+template <class VT, int W>
+void grid_jacobian1_w(const pa_mesh* m, const Material& mt, const double* z, VT* vals, hipStream_t st) {
+    GridArgs g = grid_args(m, W);
+    g.z = z;
+    const int nb = grid_blocks(g);
+    auto run = [&](auto kernel) { kernel<<<nb, W * 64, 0, st>>>(g, mt, m->d_rowmap.p, vals); };
+    by_penalty(mt.p, run, k_grid_jacobian1<VT, W, 3>, k_grid_jacobian1<VT, W, 1>, k_grid_jacobian1<VT, W, 0>);
+}
+
 }  // namespace
 
 bool grid_residual(const pa_mesh* m, int dpn, const Material& mt, const double* u, const double* z, double body,
                    double scale, const double* fext, double* R, hipStream_t st) {
     if (!m->has_grid) return false;
-    GridArgs g = grid_args(m, kGridWaves);
-    g.z = z;
-    const int nb = grid_blocks(g), nt = kGridWaves * 64;
-    auto run = [&](auto kernel) { kernel<<<nb, nt, 0, st>>>(g, mt, u, body, scale, fext, R); };
-    switch (penal_kind(mt.p) * 4 + dpn) {
-        case 3 * 4 + 3: run(k_grid_residual<3, kGridWaves, 3>); break;
-        case 3 * 4 + 1: run(k_grid_residual<1, kGridWaves, 3>); break;
-        case 2 * 4 + 3: run(k_grid_residual<3, kGridWaves, 2>); break;
-        case 2 * 4 + 1: run(k_grid_residual<1, kGridWaves, 2>); break;
-        case 1 * 4 + 3: run(k_grid_residual<3, kGridWaves, 1>); break;
-        case 1 * 4 + 1: run(k_grid_residual<1, kGridWaves, 1>); break;
-        case 3: run(k_grid_residual<3, kGridWaves, 0>); break;
-        default: run(k_grid_residual<1, kGridWaves, 0>); break;
-    }
+    if (grid_waves() == 8) grid_residual_w<8>(m, dpn, mt, u, z, body, scale, fext, R, st);
+    else grid_residual_w<kGridWaves>(m, dpn, mt, u, z, body, scale, fext, R, st);
     PA_HIP(hipGetLastError());
     return true;
 }
@@ -662,16 +814,8 @@ bool grid_residual(const pa_mesh* m, int dpn, const Material& mt, const double* 
 template <class VT>
 bool grid_jacobian1(const pa_mesh* m, const Material& mt, const double* z, VT* vals, hipStream_t st) {
     if (!m->has_grid) return false;
-    GridArgs g = grid_args(m, kGridWaves);
-    g.z = z;
-    const int nb = grid_blocks(g), nt = kGridWaves * 64;
-    auto run = [&](auto kernel) { kernel<<<nb, nt, 0, st>>>(g, mt, m->d_rowmap.p, vals); };
-    switch (penal_kind(mt.p)) {
-        case 3: run(k_grid_jacobian1<VT, kGridWaves, 3>); break;
-        case 2: run(k_grid_jacobian1<VT, kGridWaves, 2>); break;
-        case 1: run(k_grid_jacobian1<VT, kGridWaves, 1>); break;
-        default: run(k_grid_jacobian1<VT, kGridWaves, 0>); break;
-    }
+    if (grid_waves() == 8) grid_jacobian1_w<VT, 8>(m, mt, z, vals, st);
+    else grid_jacobian1_w<VT, kGridWaves>(m, mt, z, vals, st);
     PA_HIP(hipGetLastError());
     return true;
 }
