@@ -197,6 +197,19 @@ int pa_scalar_function_destroy(pa_scalar_function* sf);
 /* PhysicsScalarFunction::value (src/elliptic/PhysicsScalarFunction.hpp:225-284): one double at value_dev. */
 int pa_scalar_function_value(const pa_scalar_function* sf, const double* u_dev, const double* z_dev,
                              double* value_dev, void* stream);
+/*
+ * pa_scalar_function_value without allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory in the call
+ * (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  Same kernels and result as pa_scalar_function_value; its
+ * per-workgroup partial sums (at most 1024 doubles, enough for any mesh: the
+ * cell kernel strides over the cells) live in a static device array each GPU
+ * holds until the process ends, shared by every call on that GPU, so calls
+ * that overlap in time on one GPU race.  The first call on a device looks up
+ * the array's address (no allocation).
+ */
+int pa_scalar_function_value_noalloc(const pa_scalar_function* sf, const double* u_dev, const double* z_dev,
+                                     double* value_dev, void* stream);
 /* PhysicsScalarFunction::gradient_u: N*dpn doubles. */
 int pa_scalar_function_gradient_u(const pa_scalar_function* sf, const double* u_dev, const double* z_dev,
                                   double* grad_dev, void* stream);

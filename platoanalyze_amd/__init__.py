@@ -135,6 +135,7 @@ def _load():
         "pa_scalar_function_create": ([vp, i, vp, vp, ctypes.POINTER(vp)], i),
         "pa_scalar_function_destroy": ([vp], i),
         "pa_scalar_function_value": ([vp, vp, vp, vp, vp], i),
+        "pa_scalar_function_value_noalloc": ([vp, vp, vp, vp, vp], i),
         "pa_scalar_function_gradient_u": ([vp, vp, vp, vp, vp], i),
         "pa_scalar_function_gradient_z": ([vp, vp, vp, vp, vp], i),
         "pa_scalar_function_gradient_x": ([vp, vp, vp, vp, vp], i),

@@ -21,9 +21,17 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 
 #include "pa_cell.h"
 #include "pa_internal.h"
+
+// Partial sums of pa_scalar_function_value_noalloc: one per workgroup of the
+// cell kernel (at most kNoAllocPartials), kept on each GPU until the process
+// ends.  Host code uses it only through the address hipGetSymbolAddress
+// returns for the current device.
+constexpr int kNoAllocPartials = 1024;
+static __device__ __align__(256) double scratch[kNoAllocPartials];
 
 namespace pa {
 namespace {
@@ -2090,6 +2098,48 @@ int pa_scalar_function_value(const pa_scalar_function* sf, const double* u, cons
         else pa::k_energy_cells<1><<<nb, pa::kBlock, 0, st>>>(m->C, x4, c4, mt, u, z, sf->d_partials.p);
         pa::check_launch();
         pa::k_sum_partials<<<1, 64, 0, st>>>(nb, sf->d_partials.p, value);
+        pa::check_launch();
+    });
+}
+
+// Device address of `scratch` on the current device, looked up once per
+// device (each GPU holds its own copy of the array).
+static double* scratch_on_device() {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static double* ptr[kMaxDevices] = {};
+    int dev = 0;
+    PA_HIP(hipGetDevice(&dev));
+    PA_REQUIRE(dev >= 0 && dev < kMaxDevices, PA_ERR_LIMIT, "scratch: device ordinal out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        PA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        ptr[dev] = static_cast<double*>(p);
+    }
+    return ptr[dev];
+}
+
+int pa_scalar_function_value_noalloc(const pa_scalar_function* sf, const double* u, const double* z, double* value,
+                                     void* stream) {
+    return guarded([&] {
+        PA_REQUIRE(sf && u && z && value, PA_ERR_ARG, "pa_scalar_function_value_noalloc: null argument");
+        const pa_mesh* m = sf->mesh;
+        const int phys = sf->dpn == 3 ? PA_MECHANICS : PA_THERMAL;
+        const pa::Material mt = pa::material_of(phys, sf->material, sf->penalty);
+        const hipStream_t st = (hipStream_t)stream;
+        // one partial per workgroup of the grid-stride cell kernel
+        const long nb = std::min<long>(kNoAllocPartials, (m->C + pa::kBlock - 1) / pa::kBlock);
+        PA_REQUIRE(nb >= 1 && nb <= kNoAllocPartials, PA_ERR_LIMIT,
+                   "pa_scalar_function_value_noalloc: more partial sums than the scratch holds");
+        double* partials = scratch_on_device();
+        const auto* x4 = reinterpret_cast<const double4*>(m->d_coords4.p);
+        const auto* c4 = reinterpret_cast<const int4*>(m->d_conn.p);
+        // every partial the sum reads is written by this call first: nothing to clear
+        if (sf->dpn == 3) pa::k_energy_cells<3><<<(int)nb, pa::kBlock, 0, st>>>(m->C, x4, c4, mt, u, z, partials);
+        else pa::k_energy_cells<1><<<(int)nb, pa::kBlock, 0, st>>>(m->C, x4, c4, mt, u, z, partials);
+        pa::check_launch();
+        pa::k_sum_partials<<<1, 64, 0, st>>>((int)nb, partials, value);
         pa::check_launch();
     });
 }
