@@ -41,12 +41,60 @@ def read_traffic(name):
     return None
 
 
+def host_cpus():
+    """The host CPU as lscpu / sysfs describe it, and the CPUs this process may
+    run on: the affinity set, capped by the cgroup CPU quota when one is set
+    (the box gives a GPU's process a share of the host's cores)."""
+    info = {"model": None, "sockets": None, "physical_cores": None, "logical_cpus": os.cpu_count()}
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                info["model"] = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    cores, sockets = set(), set()
+    for cpu in Path("/sys/devices/system/cpu").glob("cpu[0-9]*"):
+        try:
+            pkg = (cpu / "topology" / "physical_package_id").read_text().strip()
+            core = (cpu / "topology" / "core_id").read_text().strip()
+        except OSError:
+            continue
+        sockets.add(pkg)
+        cores.add((pkg, core))
+    if cores:
+        info["sockets"], info["physical_cores"] = len(sockets), len(cores)
+    affinity = sorted(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, period = Path("/sys/fs/cgroup/cpu.max").read_text().split()
+        if q != "max":
+            quota = max(1, int(int(q) / int(period)))
+    except (OSError, ValueError):
+        pass
+    # one thread per physical core of the affinity set (hyperthreads dropped)
+    phys = set()
+    for c in affinity:
+        try:
+            t = Path(f"/sys/devices/system/cpu/cpu{c}/topology")
+            phys.add(((t / "physical_package_id").read_text().strip(), (t / "core_id").read_text().strip()))
+        except OSError:
+            phys.add(("?", str(c)))
+    usable = len(phys) if phys else len(affinity)
+    if quota is not None:
+        usable = min(usable, quota)
+    info.update({"affinity_cpus": len(affinity), "cgroup_cpu_quota": quota, "usable_physical_cores": usable})
+    return info
+
+
 def cpu_baseline(nx_slab=32):
     import oracle
     from platoanalyze_amd.workloads import grf_density
 
-    threads = int(os.environ.get("PA_CPU_THREADS", min(16, os.cpu_count() or 1)))
+    host = host_cpus()
+    threads = int(os.environ.get("PA_CPU_THREADS", host["usable_physical_cores"]))
     threads = oracle.set_threads(threads)
+    log(f"bench: host CPU {host}; CPU baseline on {threads} threads")
     dims = (nx_slab, 64, 64)
     coords, conn = oracle.box_mesh(*dims, 2.0 * nx_slab / 128, 1.0, 1.0)
     N = coords.size // 3
@@ -63,7 +111,7 @@ def cpu_baseline(nx_slab=32):
         m.gradient_u("elastic", u, z)
         times.append(time.perf_counter() - t0)
     t = float(np.median(times))
-    out = {"value": C / t, "unit": "cells/s", "cores": threads, "kind": "port",
+    out = {"value": C / t, "unit": "cells/s", "cores": threads, "kind": "port", "host": host,
            "sample": f"oracle value+gradient_u (FAD, per-call graph, row-search atomic scatter) on a "
                      f"{dims[0]}x{dims[1]}x{dims[2]} slab of the same grid ({C} cells), median of 3"}
     # the same restatement on one core, on a quarter of the slab
@@ -105,16 +153,23 @@ def cpu_baseline(nx_slab=32):
 def compliance_gradient(pa, mesh, vf, z, dims):
     """Second metric: time from the density on the device to the compliance
     and its gradient on the device at ||r|| <= 1e-8 ||b|| (FP64 residual),
-    for the plain FP64 path (FP64 assembly, block-Jacobi PCG from zero) and
-    the multigrid path (FP64 assembly and outer PCG, FP32 multigrid V-cycles
-    as the preconditioner).  Errors are against the plain FP64 result."""
+    for the plain FP64 path (FP64 assembly, block-Jacobi PCG from zero), the
+    multigrid path (FP64 assembly and outer PCG, FP32 multigrid V-cycles as
+    the preconditioner), and the mixed path (FP32 assembly and FP32-matrix PCG
+    iterations, the FP64 residual recomputed matrix-free by the FP64 assembly
+    kernels every replace_every iterations and at apparent convergence, FP32
+    multigrid V-cycles).  Errors are against the plain FP64 result.  (The
+    error against the FP64 oracle with scipy's direct solve is a GPU test at
+    the 20x10x10 configuration.)"""
     sf = pa.PhysicsScalarFunction(mesh, "Internal Elastic Energy", youngs=1.0, poisson=0.3, penalty_exponent=3.0,
                                   penalty_min=1e-9)
     dofs = (3 * mesh.node_set("x-")[:, None] + np.arange(3)).ravel()
     res = {}
     ref = None
-    for name, reps in (("plain_fp64_block_jacobi", 1), ("fp64_pcg_fp32_multigrid", 3)):
-        P = pa.Problem(vf, sf, dofs, mode="fp64", precond=pa.PRECOND_BLOCK_JACOBI, tol=1e-8, max_iter=100000,
+    legs = (("plain_fp64_block_jacobi", "fp64", 1), ("fp64_pcg_fp32_multigrid", "fp64", 3),
+            ("fp32_assembly_pcg_fp64_refined_fp32_multigrid", "fp32", 3))
+    for name, mode, reps in legs:
+        P = pa.Problem(vf, sf, dofs, mode=mode, precond=pa.PRECOND_BLOCK_JACOBI, tol=1e-8, max_iter=100000,
                        check_every=50)
         if name != "plain_fp64_block_jacobi":
             P.set_multigrid(*dims, pre_sweeps=2, post_sweeps=2, omega=0.6)
@@ -141,6 +196,8 @@ def compliance_gradient(pa, mesh, vf, z, dims):
         res[name] = entry
         log(f"bench: compliance+gradient {name}: {entry}")
     res["speedup_vs_plain_fp64"] = res["plain_fp64_block_jacobi"]["s"] / res["fp64_pcg_fp32_multigrid"]["s"]
+    res["speedup_vs_plain_fp64_fp32_assembly"] = (res["plain_fp64_block_jacobi"]["s"] /
+                                                 res["fp32_assembly_pcg_fp64_refined_fp32_multigrid"]["s"])
     res["unit"] = "s per compliance+gradient @1e-8"
     return res
 
@@ -153,7 +210,7 @@ J2_END_DISPLACEMENT, J2_STEPS = 0.008, 10
 WARMSTART_WEIGHTS = ROOT / "platoanalyze_amd" / "data" / "warmstart_96x48x48.npz"
 
 
-def warmstart_leg(pa, fields=4):
+def warmstart_leg(pa, fields=4, mg_fields=64):
     """Configuration 5: the learned warm start on 96x48x48 hex x6 Tet4 (1,327,104
     cells, 0.70M dof) GRF + Heaviside (beta 8) density fields not seen in training:
     PCG iterations and seconds to ||r|| <= 1e-8 ||b|| from zero and from the
@@ -171,6 +228,10 @@ def warmstart_leg(pa, fields=4):
     out = {"workload": "SIMP elastostatics Tet4, 96x48x48 hex x6 (1,327,104 cells, 232,897 nodes, 698,691 dof), GRF + tanh "
                        f"projection beta=8, {fields} unseen fields", "summary": ws.summarize(recs), "fields": recs}
     log(f"bench: warm start {out['summary']}")
+    # the configuration's 64 fields (seeds 100..163) with multigrid PCG only
+    recs64 = ws.evaluate(net, scale, box, feat, list(range(100, 100 + mg_fields)), block_jacobi=False)
+    out["batch_64"] = {"seeds": [100, 99 + mg_fields], "summary": ws.summarize(recs64)}
+    log(f"bench: warm start, 64 fields: {out['batch_64']['summary']}")
     return out
 
 
@@ -219,11 +280,17 @@ def thermal_bandwidth(pa, stream, reps=10):
         return {"ms": t, "cells_per_s": C / (t * 1e-3), "algorithmic_bytes": nbytes, "algorithmic_GBs": gbs,
                 "frac_of_hbm_peak": gbs / HBM_PEAK_GBS}
 
+    ach = b_j64 / (t_j64 * 1e-3) / 1e9
     out = {"workload": "steady thermal Tet4, 192x192x192 hex x6 (%d cells, %d nodes, %d blocks), k=1, SIMP p=3, "
                        "z ~ U[0.1,1] seed 3, body source 1" % (C, N, nnzb),
            "setup_s": setup_s, "value": leg(t_val, b_val), "gradient_u_fp64": leg(t_j64, b_j64),
            "gradient_u_fp32": leg(t_j32, b_j32),
-           "value_plus_gradient_u_fp64_cells_per_s": C / ((t_val + t_j64) * 1e-3)}
+           "value_plus_gradient_u_fp64_cells_per_s": C / ((t_val + t_j64) * 1e-3),
+           "thermal_roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": ach / HBM_PEAK_GBS,
+                                "traffic": read_traffic("k_grid_jacobian1_fp64_thermal_192x192x192"),
+                                "kernel": "gradient_u (FP64): k_grid_jacobian1<double,4,3>",
+                                "algorithmic_bytes_per_launch": b_j64, "avg_launch_ms": t_j64}}
     log(f"bench: thermal 192^3 value {t_val:.3f} ms, gradient_u fp64 {t_j64:.3f} ms, fp32 {t_j32:.3f} ms")
     del K64, K32, R, T, z, vf, mesh
     return out
@@ -457,7 +524,7 @@ def main():
                    "cells": C, "nodes": N, "blocks": nnzb, "precision": "fp64"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "gradient_u call (FP64): k_pack_nodes + k_tile_jacobian<3,double,6>",
+                     "kernel": "gradient_u (FP64): k_tile_jacobian<3,double,6>",
                      "algorithmic_bytes_per_launch": jac_bytes, "avg_launch_ms": t_jac},
         "kernels_ms": {"gradient_u_fp64": t_jac, "value": t_val, "gradient_u_fp32": t_jac32,
                        "value_gradient_u_fused_fp64": t_fused, "value_gradient_u_fused_fp32": t_fused32},
@@ -475,6 +542,7 @@ def main():
         out["warmstart_96x48x48"] = warmstart_leg(pa)
     if not args.no_thermal:
         out["thermal_192"] = thermal_bandwidth(pa, stream)
+        out["thermal_roofline"] = out["thermal_192"]["thermal_roofline"]
     if not args.no_cpu:
         log("bench: CPU baseline (oracle, OpenMP)")
         out["cpu_baseline"] = cpu_baseline()

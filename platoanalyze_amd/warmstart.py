@@ -566,14 +566,17 @@ def energy_error(K, x, xs):
     return float(np.sqrt(max(e.numpy() @ Ke.numpy(), 0.0) / (xs @ Ks.numpy())))
 
 
-def evaluate(net, scale, box, feat, seeds, tol=1e-8, max_iter=50000, log=None, multigrid=True):
+def evaluate(net, scale, box, feat, seeds, tol=1e-8, max_iter=50000, log=None, multigrid=True,
+             block_jacobi=True):
     """Per field of the given seeds: PCG iterations and seconds to ||r|| <= tol ||b|| from zero
     and from the network's warm start (inference, voxel-to-node map and energy scaling
-    included), with block-Jacobi and (optionally) multigrid preconditioning; plus the
-    warm start's relative error against the converged solution."""
+    included), with block-Jacobi and/or multigrid preconditioning; plus the warm start's
+    relative error against the converged solution (from zero)."""
     import platoanalyze_amd as pa
     mesh, dims = box.mesh, box.dims
-    bj = pa.Solver(mesh, 3, pa.PRECOND_BLOCK_JACOBI)
+    assert multigrid or block_jacobi
+    if block_jacobi:
+        bj = pa.Solver(mesh, 3, pa.PRECOND_BLOCK_JACOBI)
     if multigrid:
         mg = pa.Multigrid(mesh, *dims, 3)
         mgs = pa.Solver(mesh, 3, pa.PRECOND_MULTIGRID)
@@ -586,24 +589,25 @@ def evaluate(net, scale, box, feat, seeds, tol=1e-8, max_iter=50000, log=None, m
         rec = {"seed": int(seed)}
         pa.synchronize()
         t0 = time.perf_counter()
-        x = DeviceArray.from_numpy(np.zeros(n))
-        st = bj.solve(K, b, x, tol, max_iter)
-        rec["bj_zero"] = {"iterations": st["iterations"], "s": time.perf_counter() - t0,
-                          "rel_residual": st["rel_residual"]}
-        xs = x.numpy()
-        t0 = time.perf_counter()
         xp = predict(net, feat, zd, scale)
         x0, alpha = energy_scaled(K, b, xp, box.bc)
+        pa.synchronize()
         t_ws = time.perf_counter() - t0
-        xw = DeviceArray.from_numpy(x0.numpy())
-        st = bj.solve(K, b, xw, tol, max_iter)
-        rec["bj_warm"] = {"iterations": st["iterations"], "s": time.perf_counter() - t0,
-                          "rel_residual": st["rel_residual"], "warm_start_s": t_ws}
         x0h = x0.numpy()
-        rec["warm_start_rel_error"] = float(np.linalg.norm(x0h - xs) / np.linalg.norm(xs))
-        rec["warm_start_energy_error"] = energy_error(K, x0h, xs)
-        rec["alpha"] = alpha
-        rec["solutions_rel_diff"] = float(np.linalg.norm(xw.numpy() - xs) / np.linalg.norm(xs))
+        xs, xw_sol = None, None
+        if block_jacobi:
+            t0 = time.perf_counter()
+            x = DeviceArray.from_numpy(np.zeros(n))
+            st = bj.solve(K, b, x, tol, max_iter)
+            rec["bj_zero"] = {"iterations": st["iterations"], "s": time.perf_counter() - t0,
+                              "rel_residual": st["rel_residual"]}
+            xs = x.numpy()
+            t0 = time.perf_counter()
+            xw = DeviceArray.from_numpy(x0h)
+            st = bj.solve(K, b, xw, tol, max_iter)
+            rec["bj_warm"] = {"iterations": st["iterations"], "s": time.perf_counter() - t0 + t_ws,
+                              "rel_residual": st["rel_residual"], "warm_start_s": t_ws}
+            xw_sol = xw.numpy()
         if multigrid:
             t0 = time.perf_counter()
             mg.setup(K, box.bc)
@@ -611,11 +615,21 @@ def evaluate(net, scale, box, feat, seeds, tol=1e-8, max_iter=50000, log=None, m
             x = DeviceArray.from_numpy(np.zeros(n))
             t0 = time.perf_counter()
             st = mgs.solve(K, b, x, tol, max_iter)
-            rec["mg_zero"] = {"iterations": st["iterations"], "s": time.perf_counter() - t0 + t_setup}
+            rec["mg_zero"] = {"iterations": st["iterations"], "s": time.perf_counter() - t0 + t_setup,
+                              "rel_residual": st["rel_residual"]}
+            if xs is None:
+                xs = x.numpy()
             t0 = time.perf_counter()
             xw = DeviceArray.from_numpy(x0h)
             st = mgs.solve(K, b, xw, tol, max_iter)
-            rec["mg_warm"] = {"iterations": st["iterations"], "s": time.perf_counter() - t0 + t_setup + t_ws}
+            rec["mg_warm"] = {"iterations": st["iterations"], "s": time.perf_counter() - t0 + t_setup + t_ws,
+                              "rel_residual": st["rel_residual"]}
+            if xw_sol is None:
+                xw_sol = xw.numpy()
+        rec["warm_start_rel_error"] = float(np.linalg.norm(x0h - xs) / np.linalg.norm(xs))
+        rec["warm_start_energy_error"] = energy_error(K, x0h, xs)
+        rec["alpha"] = alpha
+        rec["solutions_rel_diff"] = float(np.linalg.norm(xw_sol - xs) / np.linalg.norm(xs))
         out.append(rec)
         if log:
             log(f"warmstart eval seed {seed}: " + ", ".join(

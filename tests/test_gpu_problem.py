@@ -150,3 +150,65 @@ def test_pcg_solver_direct(pa):
         assert st["converged"], (prec, st)
         Ks = oracle.to_scipy(3, 3, *mesh.graph(), K.entries())
         assert rel(Ks @ x.numpy(), rhs.numpy()) < 1e-9
+
+
+@pytest.mark.parametrize("mode", ["fp64", "fp32"])
+def test_metric2_paths_against_oracle_direct_solve(pa, mode):
+    """The second metric's fast paths at configuration 1 (20x10x10, z ~ U[0.3, 1]
+    seed 1): multigrid-preconditioned PCG to ||r|| <= 1e-8 ||b|| with FP64
+    assembly (mode fp64) or with FP32 assembly, FP32-matrix iterations and
+    FP64 reliable updates (mode fp32).  Compliance and dC/dz against the
+    oracle's assembled matrix solved by scipy's direct solver and the
+    reference's gradient formula dfdz + gradient_z^T lambda, lambda = -u
+    (src/elliptic/Problem.hpp:395-450): 1e-6 relative."""
+    print(f"case: metric-2 path ({mode} + multigrid) vs oracle direct solve, 20x10x10", flush=True)
+    dims = (20, 10, 10)
+    mesh, vf, sf, om, z, dofs = build(pa, dims, (2.0, 1.0, 1.0), "elastic", 1)
+    P = pa.Problem(vf, sf, dofs, mode=mode, tol=1e-8, max_iter=5000, check_every=5)
+    P.set_multigrid(*dims, pre_sweeps=2, post_sweeps=2, omega=0.6)
+    u = P.solution(z)
+    st = P.last_stats
+    assert st["converged"] and st["rel_residual"] <= 1e-8, st
+    F = P.criterion_value(z, u)
+    g = P.criterion_gradient(z, u).numpy()
+    ud = direct_solution(om, "elastic", z, dofs)
+    Fref = om.energy_value("elastic", ud, z)
+    gref = om.energy_gradient_z("elastic", ud, z) + oracle.block_spmv(1, 3, om.rowmap, om.colidx,
+                                                                      om.gradient_z("elastic", ud, z), -ud)
+    assert abs(F - Fref) <= 1e-6 * abs(Fref)
+    assert rel(g, gref) < 1e-6
+
+
+def test_calls_from_two_streams(pa):
+    """value and gradient_u on one mesh issued on two streams at once, right
+    after a natural BC is added, equal the single-stream results bit for bit
+    (the calls share no scratch)."""
+    print("case: value/gradient_u from two streams, 24x12x12 elastic and thermal", flush=True)
+    mesh = pa.Mesh.box(24, 12, 12, 2.0, 1.0, 1.0)
+    N = mesh.num_nodes
+    rng = np.random.default_rng(12)
+    z1, z2 = pa.DeviceArray.from_numpy(rng.uniform(0.3, 1.0, N)), pa.DeviceArray.from_numpy(rng.uniform(0.3, 1.0, N))
+    u = pa.DeviceArray.from_numpy(rng.standard_normal(3 * N) * 1e-2)
+    T = pa.DeviceArray.from_numpy(rng.standard_normal(N))
+    ef = pa.VectorFunction(mesh, "Mechanics", youngs=1.0, poisson=0.3, penalty_exponent=3.0, penalty_min=1e-9)
+    tf = pa.VectorFunction(mesh, "Thermal", conductivity=1.0, penalty_exponent=3.0, penalty_min=1e-9)
+    ref = [ef.value(u, z1).numpy(), ef.gradient_u(u, z2).entries(), tf.value(T, z2).numpy(),
+           tf.gradient_u(T, z1).entries()]
+    ef.add_uniform_natural_bc("x+", (0.0, 0.0, -1.0))
+    ref[0] = ef.value(u, z1).numpy()
+    ef2 = pa.VectorFunction(mesh, "Mechanics", youngs=1.0, poisson=0.3, penalty_exponent=3.0, penalty_min=1e-9)
+    s1, s2 = pa.Stream(), pa.Stream()
+    outs = [pa.DeviceArray(3 * N), pa.DeviceArray(9 * mesh.num_blocks), pa.DeviceArray(N),
+            pa.DeviceArray(mesh.num_blocks)]
+    ef2.add_uniform_natural_bc("x+", (0.0, 0.0, -1.0))
+    for _ in range(3):
+        ef2.value(u, z1, out=outs[0], stream=s1)
+        tf.gradient_u(T, z1, out=outs[3], stream=s2)
+        ef.gradient_u(u, z2, out=outs[1], stream=s2)
+        tf.value(T, z2, out=outs[2], stream=s1)
+        s1.synchronize()
+        s2.synchronize()
+        assert np.array_equal(outs[0].numpy(), ref[0])
+        assert np.array_equal(outs[1].numpy(), ref[1])
+        assert np.array_equal(outs[2].numpy(), ref[2])
+        assert np.array_equal(outs[3].numpy(), ref[3])
