@@ -331,6 +331,24 @@ def j2_plasticity(pa, stream, with_cpu=True, formulation="mixed"):
     pa.synchronize()
     t_adj = time.perf_counter() - t0
     stats = [prob.stats(k) for k in range(J2_STEPS)]
+    # memory-bounded adjoint: keep every m-th step (m = ceil(sqrt(K))), recompute the rest
+    m_ck = int(np.ceil(np.sqrt(J2_STEPS)))
+    pck = pa.PlasticityProblem(pl, dofs, vals, num_steps=J2_STEPS, max_newton=10, linear_tol=1e-8, restart=200,
+                               checkpoint_stride=m_ck)
+    if os.environ.get("PA_J2_PRECOND", "multigrid") == "multigrid":
+        pck.set_multigrid(*J2_DIMS, pre_sweeps=4, post_sweeps=4)
+    pck.solution(z)
+    pa.synchronize()
+    t0 = time.perf_counter()
+    F_ck = pck.criterion_value(z, 1.0)
+    g_ck = pck.criterion_gradient(z, 1.0)
+    pa.synchronize()
+    t_ck = time.perf_counter() - t0
+    checkpointing = {"stride": m_ck, "state_bytes": pck.state_bytes(), "state_bytes_every_step": prob.state_bytes(),
+                     "criterion_value_and_adjoint_gradient_s": t_ck,
+                     "value_bitwise_equal": F_ck == F,
+                     "gradient_bitwise_equal": bool(np.array_equal(g_ck.numpy(), g.numpy()))}
+    del pck
     u, c, pi = prob.state(J2_STEPS - 1)
     cp = prob.state(J2_STEPS - 2)[1]
     plastic = int((c.reshape(-1, 14)[:, 1] > 0).sum())
@@ -366,7 +384,8 @@ def j2_plasticity(pa, stream, with_cpu=True, formulation="mixed"):
            "linear_iterations": [s["linear_iterations"] for s in stats],
            "plastic_cells_last_step": plastic,
            "condensed_assembly_ms": t_cond, "condensed_assembly_cells_per_s": C / (t_cond * 1e-3),
-           "condensed_assembly_algorithmic_GBs": alg / (t_cond * 1e-3) / 1e9}
+           "condensed_assembly_algorithmic_GBs": alg / (t_cond * 1e-3) / 1e9,
+           "checkpointed_adjoint": checkpointing}
     log(f"bench: J2 {out}")
     if with_cpu:
         import oracle

@@ -464,6 +464,16 @@ typedef struct {
     double linear_tol;              /* Krylov relative residual (default 1e-12) */
     int krylov;                     /* PA_KRYLOV_GMRES (flexible, restarted; default) or PA_KRYLOV_BICGSTAB */
     int restart;                    /* GMRES restart length (default 50) */
+    /* Checkpoint stride m of the path-dependent adjoint (no counterpart in the
+     * reference, which keeps every step, src/PlasticityProblem.hpp:288-292).
+     * 1 (default): every step's (u, c, pi) stays on the device.  m > 1: only
+     * steps m-1, 2m-1, ... and the last are kept; the criterion value and the
+     * backward sweep recompute the others segment by segment from the
+     * preceding kept step (bitwise the same states: every reduction is
+     * fixed-order), holding ceil(K/m) + m - 1 step states instead of K
+     * (m = ceil(sqrt(K)) minimizes it).  The criterion calls must then get the
+     * control of the last solution. */
+    int checkpoint_stride;
 } pa_plasticity_options;
 void pa_plasticity_options_default(pa_plasticity_options* options);
 
@@ -482,7 +492,11 @@ int pa_plasticity_problem_set_multigrid(pa_plasticity_problem* problem, int nx, 
  * Newton solve stopped without converging.  Synchronizes. */
 int pa_plasticity_problem_solution(pa_plasticity_problem* problem, const double* z_dev, int* converged,
                                    void* stream);
-/* Device pointers to the states of step k (valid until the next solution call). */
+/* Device bytes the problem holds for step states (u, c, pi). */
+int pa_plasticity_problem_state_bytes(const pa_plasticity_problem* problem, size_t* bytes);
+/* Device pointers to the states of step k (valid until the next solution call);
+ * with checkpoint_stride m > 1 only the kept steps (m-1, 2m-1, ..., the last),
+ * else PA_ERR_STATE. */
 int pa_plasticity_problem_state(const pa_plasticity_problem* problem, int step, const double** u_dev,
                                 const double** c_dev, const double** pi_dev);
 /* Newton iterations, summed linear iterations and final residual norm of step k. */
@@ -492,7 +506,9 @@ int pa_plasticity_problem_stats(const pa_plasticity_problem* problem, int step, 
 int pa_plasticity_problem_criterion_value(pa_plasticity_problem* problem, const double* z_dev, double multiplier,
                                           double* value_host, void* stream);
 /* Total derivative of the criterion with respect to z (N doubles) by the
- * backward recursion over the stored steps.  Synchronizes. */
+ * backward recursion over the steps.  Returns PA_ERR_STATE (the gradient is
+ * then not to be used) when a step's adjoint linear solve stops short of
+ * linear_tol (iteration limit or a non-finite residual).  Synchronizes. */
 int pa_plasticity_problem_criterion_gradient(pa_plasticity_problem* problem, const double* z_dev,
                                              double multiplier, double* grad_dev, void* stream);
 

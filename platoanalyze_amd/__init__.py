@@ -75,7 +75,8 @@ J2_MIXED, J2_DISPLACEMENT = 0, 1
 class PlasticityOptions(ctypes.Structure):
     _fields_ = [("num_steps", ctypes.c_int), ("max_newton", ctypes.c_int), ("newton_stop_tol", ctypes.c_double),
                 ("newton_current_norm_tol", ctypes.c_double), ("linear_max_iter", ctypes.c_int),
-                ("linear_tol", ctypes.c_double), ("krylov", ctypes.c_int), ("restart", ctypes.c_int)]
+                ("linear_tol", ctypes.c_double), ("krylov", ctypes.c_int), ("restart", ctypes.c_int),
+                ("checkpoint_stride", ctypes.c_int)]
 
 
 KRYLOV_GMRES, KRYLOV_BICGSTAB = 0, 1
@@ -177,6 +178,7 @@ def _load():
         "pa_plasticity_problem_solution": ([vp, vp, ip, vp], i),
         "pa_plasticity_problem_state": ([vp, i, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp)], i),
         "pa_plasticity_problem_stats": ([vp, i, ip, ip, ctypes.POINTER(d)], i),
+        "pa_plasticity_problem_state_bytes": ([vp, ctypes.POINTER(sz)], i),
         "pa_plasticity_problem_criterion_value": ([vp, vp, d, ctypes.POINTER(d), vp], i),
         "pa_plasticity_problem_criterion_gradient": ([vp, vp, d, vp, vp], i),
         # warm-start network kernels
@@ -844,12 +846,12 @@ class PlasticityProblem:
 
     def __init__(self, plasticity, bc_dofs, bc_values=None, *, num_steps=20, max_newton=10, newton_stop_tol=1e-8,
                  newton_current_norm_tol=1e-8, linear_max_iter=20000, linear_tol=1e-12, krylov=KRYLOV_GMRES,
-                 restart=50):
+                 restart=50, checkpoint_stride=1):
         self.pl = plasticity
         dofs = np.ascontiguousarray(bc_dofs, np.int32)
         vals = np.zeros(dofs.size) if bc_values is None else np.ascontiguousarray(bc_values, np.float64)
         self.options = PlasticityOptions(num_steps, max_newton, newton_stop_tol, newton_current_norm_tol,
-                                         linear_max_iter, linear_tol, krylov, restart)
+                                         linear_max_iter, linear_tol, krylov, restart, checkpoint_stride)
         h = ctypes.c_void_p()
         _check(lib.pa_plasticity_problem_create(plasticity.handle, dofs.size, _np_ptr(dofs), _np_ptr(vals),
                                                 ctypes.byref(self.options), ctypes.byref(h)))
@@ -861,6 +863,12 @@ class PlasticityProblem:
         """Geometric multigrid (dpn x dpn nodal blocks) as the Krylov preconditioner."""
         opts = MultigridOptions(max_levels, pre_sweeps, post_sweeps, omega, max_coarse_nodes)
         _check(lib.pa_plasticity_problem_set_multigrid(self.handle, nx, ny, nz, ctypes.byref(opts)))
+
+    def state_bytes(self):
+        """Device bytes held for the step states (u, c, pi)."""
+        b = ctypes.c_size_t()
+        _check(lib.pa_plasticity_problem_state_bytes(self.handle, ctypes.byref(b)))
+        return b.value
 
     def solution(self, z, stream=None):
         z = _dev(z)

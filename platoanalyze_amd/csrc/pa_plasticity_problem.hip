@@ -8,9 +8,13 @@
 // geometric multigrid on D x D blocks as the preconditioner.  Reductions are
 // fixed-order, so runs are deterministic.
 //
-// Every step's u, c and pi stay on the device: they are the checkpoints the
-// backward recursion reads (a 64x32x32-hex problem with 10 steps holds about
-// 0.5 GB).
+// Step states (u, c, pi) live in slots on the device.  With checkpoint
+// stride 1 every step has its own slot (a 64x32x32-hex problem with 10 steps
+// holds about 0.5 GB).  With stride m > 1 only the kept steps (m-1, 2m-1, ...,
+// and the last: each ends a segment of m steps) have their own slot, and the
+// m - 1 other steps of a segment share m - 1 slots; the criterion and the
+// backward recursion recompute a segment's unkept steps from the kept step
+// before it by the same Newton solves, which reproduce the states bit for bit.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,8 +33,9 @@ struct pa_plasticity_problem {
     std::vector<double> h_vals;
     pa::DevBuf<int> bc_dofs;
     pa::DevBuf<double> bc_inc, bc_zero;
-    // states of every step
+    // step state slots (see slot())
     pa::DevBuf<double> U, Cs, PI;
+    int nslots = 0, nkept = 0;
     pa::DevBuf<double> zeroU, zeroC;
     // Newton / Krylov workspace
     pa::DevBuf<double> vals, R, rhs, du;
@@ -39,6 +44,9 @@ struct pa_plasticity_problem {
     // adjoint workspace
     pa::DevBuf<double> wu, wu_next, gc, gcp, gcp_next, zc, zc_out, blocal, mu, mu_next, lam, gam, gam_next;
     std::vector<int> newton_its, linear_its;
+    // outcome of the last Krylov solve
+    bool lin_converged = true;
+    double lin_rel = 0.0;
     std::vector<double> res_norms;
     bool solved = false;
     bool verbose = false;  // PA_PLASTICITY_VERBOSE set: one stderr line per Newton iteration
@@ -410,6 +418,8 @@ int bicgstab(pa_plasticity_problem* P, const double* b, double* x, hipStream_t s
         rr = read_scal(P, S_RR, st);
         if (!std::isfinite(rr)) break;
     }
+    P->lin_rel = bnorm > 0.0 ? std::sqrt(rr) / bnorm : 0.0;
+    P->lin_converged = std::isfinite(rr) && (bnorm == 0.0 || std::sqrt(rr) <= target);
     return it;
 }
 
@@ -516,8 +526,12 @@ int gmres(pa_plasticity_problem* P, const double* b, double* x, hipStream_t st) 
     double* Z = P->gZ.p;
     preconditioner_setup(P, st);
     const double bnorm = norm(P, b, st);
+    P->lin_converged = true;
+    P->lin_rel = 0.0;
     if (!(bnorm > 0.0)) return 0;
     const double target = P->opt.linear_tol * bnorm;
+    P->lin_converged = false;
+    P->lin_rel = 1.0;
     std::vector<double> H((M + 1) * M), cs(M), sn(M), g(M + 1), y(M), hcol(M + 1);
     int it = 0;
     while (it < P->opt.linear_max_iter) {
@@ -529,7 +543,11 @@ int gmres(pa_plasticity_problem* P, const double* b, double* x, hipStream_t st) 
         check();
         const double beta = norm(P, V, st);
         if (!std::isfinite(beta)) return it;
-        if (beta <= target) break;
+        P->lin_rel = beta / bnorm;
+        if (beta <= target) {
+            P->lin_converged = true;
+            break;
+        }
         k_scale<<<nvb, kB, 0, st>>>(n, 1.0 / beta, V);  // V_0 = r / beta
         check();
         std::fill(g.begin(), g.end(), 0.0);
@@ -579,6 +597,7 @@ int gmres(pa_plasticity_problem* P, const double* b, double* x, hipStream_t st) 
             g[j + 1] = -sn[j] * g[j];
             g[j] = cs[j] * g[j];
             for (int i = 0; i <= j; ++i) H[i * M + j] = hcol[i];
+            P->lin_rel = std::fabs(g[j + 1]) / bnorm;
             if (std::fabs(g[j + 1]) <= target || hn == 0.0) {
                 ++j;
                 ++it;
@@ -596,7 +615,10 @@ int gmres(pa_plasticity_problem* P, const double* b, double* x, hipStream_t st) 
         k_madd<<<nvb, kB, 0, st>>>(n, j, Z, P->gy.p, x);
         check();
         PA_HIP(hipStreamSynchronize(st));
-        if (done) break;
+        if (done) {
+            P->lin_converged = std::isfinite(P->lin_rel);
+            break;
+        }
     }
     return it;
 }
@@ -619,9 +641,24 @@ int linear_solve(pa_plasticity_problem* P, const double* b, double* x, hipStream
     return P->opt.krylov == PA_KRYLOV_BICGSTAB ? bicgstab(P, b, x, st) : gmres(P, b, x, st);
 }
 
-double* step_u(pa_plasticity_problem* P, int k) { return P->U.p + (long)k * num_dofs(P); }
-double* step_c(pa_plasticity_problem* P, int k) { return P->Cs.p + (long)k * 14 * P->p->mesh->C; }
-double* step_pi(pa_plasticity_problem* P, int k) { return P->PI.p + (long)k * 3 * P->p->mesh->N; }
+// Checkpoint layout: stride m, kept steps m-1, 2m-1, ... and K-1.
+int stride(const pa_plasticity_problem* P) { return std::max(1, P->opt.checkpoint_stride); }
+bool kept(const pa_plasticity_problem* P, int k) {
+    const int m = stride(P);
+    return m == 1 || k == P->opt.num_steps - 1 || k % m == m - 1;
+}
+// Slot of step k: its own for a kept step (k / m), else one of the m - 1
+// slots shared by the segments' unkept steps (nkept + k % m).
+int slot(const pa_plasticity_problem* P, int k) {
+    const int m = stride(P);
+    if (m == 1) return k;
+    return kept(P, k) ? k / m : P->nkept + k % m;
+}
+double* step_u(pa_plasticity_problem* P, int k) { return P->U.p + (long)slot(P, k) * num_dofs(P); }
+double* step_c(pa_plasticity_problem* P, int k) { return P->Cs.p + (long)slot(P, k) * 14 * P->p->mesh->C; }
+double* step_pi(pa_plasticity_problem* P, int k) { return P->PI.p + (long)slot(P, k) * 3 * P->p->mesh->N; }
+const double* prev_u(pa_plasticity_problem* P, int k) { return k ? step_u(P, k - 1) : P->zeroU.p; }
+const double* prev_c(pa_plasticity_problem* P, int k) { return k ? step_c(P, k - 1) : P->zeroC.p; }
 
 void constraints(pa_plasticity_problem* P, double* rhs, const double* bcv, hipStream_t st) {
     if (P->nbc == 0) return;
@@ -630,122 +667,180 @@ void constraints(pa_plasticity_problem* P, double* rhs, const double* bcv, hipSt
     if (rc != PA_OK) throw Error(rc, pa_last_error());
 }
 
-// Forward solve (src/PlasticityProblem.hpp:784-868, src/NewtonRaphsonSolver.hpp:555-610).
-bool solve_forward(pa_plasticity_problem* P, const double* z, hipStream_t st) {
-    const pa_mesh* m = P->p->mesh;
+// The projected pressure gradient step k uses (src/PlasticityProblem.hpp:872-891):
+// zero in the first step and in the pure-displacement form, else the
+// projection of the previous step's pressure.
+void step_projection(pa_plasticity_problem* P, int k, const double* z, hipStream_t st) {
+    if (P->p->dpn == 4 && k > 0) projection(P->p, step_u(P, k - 1), z, step_pi(P, k), st);
+    else PA_HIP(hipMemsetAsync(step_pi(P, k), 0, 3L * P->p->mesh->N * sizeof(double), st));
+}
+
+// Newton solve of step k (src/NewtonRaphsonSolver.hpp:555-610) from the states
+// of step k - 1 and the step's projection; returns convergence and, with
+// record, keeps the iteration counts.
+bool forward_step(pa_plasticity_problem* P, int k, const double* z, bool record, hipStream_t st) {
     const int K = P->opt.num_steps;
     const long nu = num_dofs(P);
-    const bool mixed = P->p->dpn == 4;
-    P->newton_its.assign(K, 0);
-    P->linear_its.assign(K, 0);
-    P->res_norms.assign(K, 0.0);
-    // the pure-displacement form has no projection: every step's pi stays zero
-    PA_HIP(hipMemsetAsync(step_pi(P, 0), 0, 3L * m->N * (mixed ? 1 : K) * sizeof(double), st));
-    const double dt = 1.0 / K;
+    double* uk = step_u(P, k);
+    double* ck = step_c(P, k);
+    const double* up = prev_u(P, k);
+    const double* cp = prev_c(P, k);
+    const double* pik = step_pi(P, k);
+    // Dirichlet increment of this step: g(t_k) - g(t_{k-1}), t_k = (k+1)/K
+    const double dt = 1.0 / K, tk = dt * (k + 1), tp = dt * k;
     std::vector<double> inc(P->nbc);
-    bool all_converged = true;
-    for (int k = 0; k < K; ++k) {
-        double* uk = step_u(P, k);
-        double* ck = step_c(P, k);
-        const double* up = k ? step_u(P, k - 1) : P->zeroU.p;
-        const double* cp = k ? step_c(P, k - 1) : P->zeroC.p;
-        const double* pik = step_pi(P, k);
-        // Dirichlet increment of this step: g(t_k) - g(t_{k-1}), t_k = (k+1)/K
-        const double tk = dt * (k + 1), tp = dt * k;
-        for (int j = 0; j < P->nbc; ++j) inc[j] = P->h_vals[j] * tk - (k > 0 ? P->h_vals[j] * tp : 0.0);
-        if (P->nbc)
-            PA_HIP(hipMemcpyAsync(P->bc_inc.p, inc.data(), P->nbc * sizeof(double), hipMemcpyHostToDevice, st));
-        PA_HIP(hipMemcpyAsync(uk, up, nu * sizeof(double), hipMemcpyDeviceToDevice, st));
-        update_local_state(P->p, uk, cp, z, ck, st);
-        int it = 0, lin = 0;
-        double ref = 0.0, measure = HUGE_VAL, nrm = 0.0;
-        bool converged = false;
-        while (true) {
-            condensed_system(P->p, uk, ck, cp, pik, z, P->vals.p, P->R.p, false, st);
-            k_neg_copy<<<vec_blocks(P), kB, 0, st>>>(nu, P->R.p, P->rhs.p);
-            check();
-            constraints(P, P->rhs.p, it == 0 ? P->bc_inc.p : P->bc_zero.p, st);
-            nrm = norm(P, P->rhs.p, st);
-            if (it == 0) {
-                ref = nrm;
-            } else {
-                measure = std::fabs(ref - nrm);
-                ref = nrm;
-            }
-            if (measure < P->opt.newton_stop_tol || nrm < P->opt.newton_current_norm_tol) {
-                converged = true;
-                break;
-            }
-            if (nrm >= 1e20 || it >= P->opt.max_newton || !std::isfinite(nrm)) break;
-            PA_HIP(hipMemsetAsync(P->du.p, 0, nu * sizeof(double), st));
-            const int li = linear_solve(P, P->rhs.p, P->du.p, st);
-            lin += li;
-            if (P->verbose) std::fprintf(stderr, "plasticity: step %d newton %d |r| %.6e linear %d\n", k, it, nrm, li);
-            k_axpy<<<vec_blocks(P), kB, 0, st>>>(nu, 1.0, P->du.p, uk);
-            check();
-            update_local_state(P->p, uk, cp, z, ck, st);
-            ++it;
+    for (int j = 0; j < P->nbc; ++j) inc[j] = P->h_vals[j] * tk - (k > 0 ? P->h_vals[j] * tp : 0.0);
+    if (P->nbc) PA_HIP(hipMemcpyAsync(P->bc_inc.p, inc.data(), P->nbc * sizeof(double), hipMemcpyHostToDevice, st));
+    PA_HIP(hipMemcpyAsync(uk, up, nu * sizeof(double), hipMemcpyDeviceToDevice, st));
+    update_local_state(P->p, uk, cp, z, ck, st);
+    int it = 0, lin = 0;
+    double ref = 0.0, measure = HUGE_VAL, nrm = 0.0;
+    bool converged = false;
+    while (true) {
+        condensed_system(P->p, uk, ck, cp, pik, z, P->vals.p, P->R.p, false, st);
+        k_neg_copy<<<vec_blocks(P), kB, 0, st>>>(nu, P->R.p, P->rhs.p);
+        check();
+        constraints(P, P->rhs.p, it == 0 ? P->bc_inc.p : P->bc_zero.p, st);
+        nrm = norm(P, P->rhs.p, st);
+        if (it == 0) {
+            ref = nrm;
+        } else {
+            measure = std::fabs(ref - nrm);
+            ref = nrm;
         }
+        if (measure < P->opt.newton_stop_tol || nrm < P->opt.newton_current_norm_tol) {
+            converged = true;
+            break;
+        }
+        if (nrm >= 1e20 || it >= P->opt.max_newton || !std::isfinite(nrm)) break;
+        PA_HIP(hipMemsetAsync(P->du.p, 0, nu * sizeof(double), st));
+        const int li = linear_solve(P, P->rhs.p, P->du.p, st);
+        lin += li;
+        if (P->verbose) std::fprintf(stderr, "plasticity: step %d newton %d |r| %.6e linear %d\n", k, it, nrm, li);
+        k_axpy<<<vec_blocks(P), kB, 0, st>>>(nu, 1.0, P->du.p, uk);
+        check();
+        update_local_state(P->p, uk, cp, z, ck, st);
+        ++it;
+    }
+    if (record) {
         P->newton_its[k] = it;
         P->linear_its[k] = lin;
         P->res_norms[k] = nrm;
-        all_converged = all_converged && converged;
-        if (mixed && k + 1 < K) projection(P->p, uk, z, step_pi(P, k + 1), st);
+    }
+    return converged;
+}
+
+// Forward solve (src/PlasticityProblem.hpp:784-868).
+bool solve_forward(pa_plasticity_problem* P, const double* z, hipStream_t st) {
+    const int K = P->opt.num_steps;
+    P->newton_its.assign(K, 0);
+    P->linear_its.assign(K, 0);
+    P->res_norms.assign(K, 0.0);
+    bool all_converged = true;
+    for (int k = 0; k < K; ++k) {
+        step_projection(P, k, z, st);
+        all_converged = forward_step(P, k, z, true, st) && all_converged;
     }
     PA_HIP(hipStreamSynchronize(st));
     P->solved = true;
     return all_converged;
 }
 
+// Segment s: steps [first, last], last a kept step.
+int num_segments(const pa_plasticity_problem* P) { return (P->opt.num_steps + stride(P) - 1) / stride(P); }
+int seg_first(const pa_plasticity_problem* P, int s) { return s * stride(P); }
+int seg_last(const pa_plasticity_problem* P, int s) {
+    return std::min(s * stride(P) + stride(P) - 1, P->opt.num_steps - 1);
+}
+
+// States of a segment's unkept steps, recomputed from the kept step before it.
+void recompute_segment(pa_plasticity_problem* P, int s, const double* z, hipStream_t st) {
+    for (int k = seg_first(P, s); k < seg_last(P, s); ++k) {
+        step_projection(P, k, z, st);
+        forward_step(P, k, z, false, st);
+    }
+}
+
+// Plastic-work criterion over the steps (src/PlasticityProblem.hpp:925-956);
+// with checkpoints each segment is recomputed first.
 double criterion_value(pa_plasticity_problem* P, const double* z, double mult, hipStream_t st) {
     double total = 0.0;
-    for (int k = 0; k < P->opt.num_steps; ++k) {
-        plastic_work(P->p, step_u(P, k), k ? step_u(P, k - 1) : P->zeroU.p, step_c(P, k),
-                     k ? step_c(P, k - 1) : P->zeroC.p, z, P->scal.p + S_NORM, st);
-        total += read_scal(P, S_NORM, st);
+    for (int s = 0; s < num_segments(P); ++s) {
+        if (stride(P) > 1) recompute_segment(P, s, z, st);
+        for (int k = seg_first(P, s); k <= seg_last(P, s); ++k) {
+            plastic_work(P->p, step_u(P, k), prev_u(P, k), step_c(P, k), prev_c(P, k), z, P->scal.p + S_NORM, st);
+            total += read_scal(P, S_NORM, st);
+        }
     }
     return mult * total;
 }
 
-// Backward recursion (src/PlasticityProblem.hpp:1035-1066,
-// src/PathDependentAdjointSolver.hpp:277-917).
-void criterion_gradient(pa_plasticity_problem* P, const double* z, double mult, double* grad, hipStream_t st) {
+// One step of the backward recursion (src/PathDependentAdjointSolver.hpp:
+// 277-917): the adjoint of step k from the adjoint of step k + 1 (kept in the
+// *_next buffers), its control terms added to grad.  Returns false when the
+// global adjoint solve stopped short of the tolerance.
+bool backward_step(pa_plasticity_problem* P, int k, const double* z, double mult, double* grad, hipStream_t st) {
     const pa_mesh* m = P->p->mesh;
     const int K = P->opt.num_steps;
     const long nu = num_dofs(P);
     const bool mixed = P->p->dpn == 4;
-    PA_HIP(hipMemsetAsync(grad, 0, m->N * sizeof(double), st));
-    for (int k = K - 1; k >= 0; --k) {
-        const bool last = (k == K - 1);
-        const double* u = step_u(P, k);
-        const double* c = step_c(P, k);
-        const double* up = k ? step_u(P, k - 1) : P->zeroU.p;
-        const double* cp = k ? step_c(P, k - 1) : P->zeroC.p;
-        const double* pi = step_pi(P, k);
-        AdjointWork w{P->wu.p, P->gc.p, P->gcp.p, P->zc.p};
-        work_partials(P->p, u, up, c, cp, z, mult, w, st);
-        adjoint_local_rhs(P->p, u, c, cp, z, P->gc.p, last ? nullptr : P->gcp_next.p, last ? nullptr : P->mu_next.p,
-                          P->wu.p, last ? nullptr : P->wu_next.p, last ? nullptr : P->gam_next.p, P->blocal.p, st);
-        gather_vcell(P->p, -1.0, P->rhs.p, st);
-        condensed_system(P->p, u, c, cp, pi, z, P->vals.p, nullptr, true, st);
-        constraints(P, P->rhs.p, P->bc_zero.p, st);
-        PA_HIP(hipMemsetAsync(P->lam.p, 0, nu * sizeof(double), st));
-        linear_solve(P, P->rhs.p, P->lam.p, st);
-        adjoint_local_update(P->p, u, c, cp, pi, z, P->lam.p, P->blocal.p, P->zc.p, P->mu.p, P->zc_out.p, st);
-        if (k == 0 || !mixed) {  // no projection adjoint in the first step or the displacement form
-            PA_HIP(hipMemsetAsync(P->gam.p, 0, 3L * m->N * sizeof(double), st));
-        } else {
-            gather_uniform(P->p, 3, P->p->vcell.p, 16, -1.0, false, P->gam.p, st);
-            divide_rowsum(P->p, P->gam.p, z, st);
-            if (P->p->prm.projection_penalty) projection_control_term(P->p, u, pi, P->gam.p, z, P->zc_out.p, st);
+    const bool last = (k == K - 1);
+    const double* u = step_u(P, k);
+    const double* c = step_c(P, k);
+    const double* up = prev_u(P, k);
+    const double* cp = prev_c(P, k);
+    const double* pi = step_pi(P, k);
+    AdjointWork w{P->wu.p, P->gc.p, P->gcp.p, P->zc.p};
+    work_partials(P->p, u, up, c, cp, z, mult, w, st);
+    adjoint_local_rhs(P->p, u, c, cp, z, P->gc.p, last ? nullptr : P->gcp_next.p, last ? nullptr : P->mu_next.p,
+                      P->wu.p, last ? nullptr : P->wu_next.p, last ? nullptr : P->gam_next.p, P->blocal.p, st);
+    gather_vcell(P->p, -1.0, P->rhs.p, st);
+    condensed_system(P->p, u, c, cp, pi, z, P->vals.p, nullptr, true, st);
+    constraints(P, P->rhs.p, P->bc_zero.p, st);
+    PA_HIP(hipMemsetAsync(P->lam.p, 0, nu * sizeof(double), st));
+    linear_solve(P, P->rhs.p, P->lam.p, st);
+    const bool ok = P->lin_converged;
+    adjoint_local_update(P->p, u, c, cp, pi, z, P->lam.p, P->blocal.p, P->zc.p, P->mu.p, P->zc_out.p, st);
+    if (k == 0 || !mixed) {  // no projection adjoint in the first step or the displacement form
+        PA_HIP(hipMemsetAsync(P->gam.p, 0, 3L * m->N * sizeof(double), st));
+    } else {
+        gather_uniform(P->p, 3, P->p->vcell.p, 16, -1.0, false, P->gam.p, st);
+        divide_rowsum(P->p, P->gam.p, z, st);
+        if (P->p->prm.projection_penalty) projection_control_term(P->p, u, pi, P->gam.p, z, P->zc_out.p, st);
+    }
+    gather_uniform(P->p, 1, P->zc_out.p, 1.0, true, grad, st);
+    std::swap(P->wu.p, P->wu_next.p);
+    std::swap(P->gcp.p, P->gcp_next.p);
+    std::swap(P->mu.p, P->mu_next.p);
+    std::swap(P->gam.p, P->gam_next.p);
+    return ok;
+}
+
+// Backward recursion (src/PlasticityProblem.hpp:1035-1066) over the segments
+// from the last, each recomputed first when checkpointed.  The recompute
+// touches only the forward workspace, not the adjoint carried between steps.
+void criterion_gradient(pa_plasticity_problem* P, const double* z, double mult, double* grad, hipStream_t st) {
+    PA_HIP(hipMemsetAsync(grad, 0, P->p->mesh->N * sizeof(double), st));
+    int failed = -1;
+    double worst = 0.0;
+    for (int s = num_segments(P) - 1; s >= 0; --s) {
+        if (stride(P) > 1) recompute_segment(P, s, z, st);
+        for (int k = seg_last(P, s); k >= seg_first(P, s); --k) {
+            if (!backward_step(P, k, z, mult, grad, st) && failed < 0) {
+                failed = k;
+                worst = P->lin_rel;
+            }
         }
-        gather_uniform(P->p, 1, P->zc_out.p, 1.0, true, grad, st);
-        std::swap(P->wu.p, P->wu_next.p);
-        std::swap(P->gcp.p, P->gcp_next.p);
-        std::swap(P->mu.p, P->mu_next.p);
-        std::swap(P->gam.p, P->gam_next.p);
     }
     PA_HIP(hipStreamSynchronize(st));
+    if (failed >= 0) {
+        char msg[160];
+        std::snprintf(msg, sizeof msg,
+                      "criterion_gradient: the adjoint linear solve of step %d stopped at relative residual %.3e "
+                      "(linear_tol %.1e, linear_max_iter %d)",
+                      failed, worst, P->opt.linear_tol, P->opt.linear_max_iter);
+        throw Error(PA_ERR_STATE, msg);
+    }
 }
 
 }  // namespace
@@ -766,6 +861,7 @@ void pa_plasticity_options_default(pa_plasticity_options* o) {
     o->linear_tol = 1e-12;
     o->krylov = PA_KRYLOV_GMRES;
     o->restart = 50;
+    o->checkpoint_stride = 1;
 }
 
 int pa_plasticity_problem_create(pa_plasticity* p, int nbc, const int* dofs, const double* vals,
@@ -780,7 +876,8 @@ int pa_plasticity_problem_create(pa_plasticity* p, int nbc, const int* dofs, con
             if (opt) P->opt = *opt;
             else pa_plasticity_options_default(&P->opt);
             PA_REQUIRE(P->opt.num_steps >= 1 && P->opt.max_newton >= 0 && P->opt.linear_max_iter >= 1 &&
-                           P->opt.restart >= 1 && P->opt.restart <= 1000 &&
+                           P->opt.restart >= 1 && P->opt.restart <= 1000 && P->opt.checkpoint_stride >= 0 &&
+                           P->opt.checkpoint_stride <= P->opt.num_steps &&
                            (P->opt.krylov == PA_KRYLOV_GMRES || P->opt.krylov == PA_KRYLOV_BICGSTAB), PA_ERR_ARG,
                        "pa_plasticity_problem_create: bad options");
             const pa_mesh* m = p->mesh;
@@ -797,9 +894,12 @@ int pa_plasticity_problem_create(pa_plasticity* p, int nbc, const int* dofs, con
                 PA_HIP(hipMemset(P->bc_zero.p, 0, nbc * sizeof(double)));
             }
             const int K = P->opt.num_steps;
-            P->U.alloc(nu * K);
-            P->Cs.alloc(14L * m->C * K);
-            P->PI.alloc(3L * m->N * K);
+            const int mstride = pa::j2::stride(P);
+            P->nkept = mstride == 1 ? K : (K + mstride - 1) / mstride;
+            P->nslots = mstride == 1 ? K : P->nkept + mstride - 1;
+            P->U.alloc(nu * P->nslots);
+            P->Cs.alloc(14L * m->C * P->nslots);
+            P->PI.alloc(3L * m->N * P->nslots);
             P->zeroU.alloc(nu);
             P->zeroC.alloc(14L * m->C);
             PA_HIP(hipMemset(P->zeroU.p, 0, nu * sizeof(double)));
@@ -870,10 +970,20 @@ int pa_plasticity_problem_state(const pa_plasticity_problem* P, int step, const 
     return guarded([&] {
         PA_REQUIRE(P && step >= 0 && step < P->opt.num_steps, PA_ERR_ARG, "pa_plasticity_problem_state: bad step");
         PA_REQUIRE(P->solved, PA_ERR_STATE, "pa_plasticity_problem_state: no solution yet");
+        PA_REQUIRE(pa::j2::kept(P, step), PA_ERR_STATE,
+                   "pa_plasticity_problem_state: step not kept (checkpoint stride " +
+                       std::to_string(pa::j2::stride(P)) + ")");
         auto* Q = const_cast<pa_plasticity_problem*>(P);
         if (u) *u = pa::j2::step_u(Q, step);
         if (c) *c = pa::j2::step_c(Q, step);
         if (pi) *pi = pa::j2::step_pi(Q, step);
+    });
+}
+
+int pa_plasticity_problem_state_bytes(const pa_plasticity_problem* P, size_t* bytes) {
+    return guarded([&] {
+        PA_REQUIRE(P && bytes, PA_ERR_ARG, "pa_plasticity_problem_state_bytes: null argument");
+        *bytes = (P->U.n + P->Cs.n + P->PI.n) * sizeof(double);
     });
 }
 
