@@ -639,7 +639,11 @@ class NetcdfClassic {
                 const std::string name = text(p);
                 Var v;
                 const unsigned nd = u32(p);
-                for (unsigned k = 0; k < nd; ++k) v.dims.push_back((int)u32(p));
+                for (unsigned k = 0; k < nd; ++k) {
+                    const unsigned d = u32(p);
+                    if (d >= mDims.size()) throw_error("netCDF: '" + aPath + "': variable '" + name + "' has an unknown dimension");
+                    v.dims.push_back((int)d);
+                }
                 skip_attrs(p, &v.text_attrs);
                 v.type = (int)u32(p);
                 u32(p);  // vsize
@@ -670,14 +674,23 @@ class NetcdfClassic {
         }
         return s;
     }
+    // Bytes of a variable of n values of width w at v.begin, checked against the file.
+    void check_extent(const std::string& aName, const Var& v, unsigned long long n, unsigned long long w) const {
+        if (n > mBytes.size() || v.begin > mBytes.size() || n * w > mBytes.size() - v.begin)
+            throw_error("netCDF: variable '" + aName + "' of '" + mPath + "' runs past the end of the file");
+    }
     // numeric variable as doubles (int, short, byte, float or double)
     std::vector<double> numbers(const std::string& aName) const {
         const Var& v = var(aName);
-        size_t n = 1;
-        for (long long s : shape(aName)) n *= (size_t)s;
+        unsigned long long n = 1;
+        for (long long s : shape(aName)) {
+            n *= (unsigned long long)s;
+            if (n > mBytes.size()) throw_error("netCDF: variable '" + aName + "' runs past the end of the file");
+        }
         const int w = v.type == 6 ? 8 : (v.type == 4 || v.type == 5) ? 4 : v.type == 3 ? 2 : 1;
         if (v.type == 2) throw_error("netCDF: variable '" + aName + "' is text");
-        if (v.begin + (unsigned long long)n * w > mBytes.size()) throw_error("netCDF: variable '" + aName + "' truncated");
+        if (v.type < 1 || v.type > 6) throw_error("netCDF: variable '" + aName + "' has an unknown type");
+        check_extent(aName, v, n, (unsigned long long)w);
         std::vector<double> out(n);
         size_t p = (size_t)v.begin;
         for (size_t i = 0; i < n; ++i) {
@@ -708,6 +721,9 @@ class NetcdfClassic {
         const Var& v = var(aName);
         const auto s = shape(aName);
         if (v.type != 2 || s.size() != 2) throw_error("netCDF: '" + aName + "' is not a text array");
+        if ((unsigned long long)s[0] > mBytes.size() || (unsigned long long)s[1] > mBytes.size())
+            throw_error("netCDF: variable '" + aName + "' runs past the end of the file");
+        check_extent(aName, v, (unsigned long long)s[0] * (unsigned long long)s[1], 1);
         std::vector<std::string> out;
         for (long long r = 0; r < s[0]; ++r) {
             const char* c = reinterpret_cast<const char*>(mBytes.data() + v.begin + (size_t)(r * s[1]));
@@ -748,6 +764,8 @@ class NetcdfClassic {
             const std::string name = text(p);
             const unsigned type = u32(p), n = u32(p);
             const unsigned w = type == 6 ? 8 : (type == 4 || type == 5) ? 4 : type == 3 ? 2 : 1;
+            const unsigned long long bytes = ((unsigned long long)n * w + 3) / 4 * 4;
+            if (bytes > mBytes.size() - p) throw_error("netCDF: '" + mPath + "' truncated attribute");
             if (type == 2 && aText) {
                 std::string v(reinterpret_cast<const char*>(mBytes.data() + p), n);
                 (*aText)[name] = v.substr(0, v.find('\0'));
@@ -772,24 +790,51 @@ class NetcdfClassic {
 // side numbering 1 = (1,2,4), 2 = (2,3,4), 3 = (1,4,3), 4 = (1,3,2), node_ns<i>,
 // and the names in ss_names / ns_names (the id from ss_prop1 / ns_prop1 when a
 // name is empty).
-inline std::shared_ptr<Mesh> readExodusMesh(const std::string& aPath) {
+// Host arrays of an Exodus II Tet4 mesh (no device work): coordinates,
+// 0-based connectivity, side sets (cells and local node ordinals per face),
+// node sets.  Every index is checked against the file's own counts.
+struct ExodusData {
+    std::vector<double> coords;
+    std::vector<int> conn;
+    std::vector<std::pair<std::string, std::pair<std::vector<int>, std::vector<int>>>> side_sets;
+    std::vector<std::pair<std::string, std::vector<int>>> node_sets;
+};
+
+// 1-based index x of a file as a 0-based one in [0, n), else -1.
+inline long long exodus_index(double x, long long n) {
+    if (!(x >= 1.0 && x <= (double)n)) return -1;
+    return (long long)x - 1;
+}
+
+inline ExodusData readExodusData(const std::string& aPath) {
     detail::NetcdfClassic nc(aPath);
+    ExodusData tOut;
+    // set and block counts are bounded by the variables they need
+    auto count = [&](const char* aDim) {
+        const long long c = nc.dim(aDim, 0);
+        if (c < 0 || c > 1000000) throw_error(std::string("readExodusMesh: bad ") + aDim);
+        return c;
+    };
     if (nc.dim("num_dim", 3) != 3) throw_error("readExodusMesh: only three-dimensional meshes are on this path");
     const long long N = nc.dim("num_nodes");
-    std::vector<double> tCoords((size_t)N * 3);
+    if (N <= 0 || N > (1LL << 31) - 1) throw_error("readExodusMesh: bad node count");
+    std::vector<double>& tCoords = tOut.coords;
+    tCoords.assign((size_t)N * 3, 0.0);
     if (nc.has("coord")) {
         const auto c = nc.numbers("coord");
+        if ((long long)c.size() < 3 * N) throw_error("readExodusMesh: coord holds fewer than 3 x num_nodes values");
         for (long long n = 0; n < N; ++n)
             for (int d = 0; d < 3; ++d) tCoords[(size_t)n * 3 + d] = c[(size_t)d * N + n];
     } else {
         const char* names[3] = {"coordx", "coordy", "coordz"};
         for (int d = 0; d < 3; ++d) {
             const auto c = nc.numbers(names[d]);
+            if ((long long)c.size() < N) throw_error(std::string("readExodusMesh: ") + names[d] + " is short");
             for (long long n = 0; n < N; ++n) tCoords[(size_t)n * 3 + d] = c[(size_t)n];
         }
     }
-    std::vector<int> tConn;
-    const long long nblk = nc.dim("num_el_blk", 0);
+    std::vector<int>& tConn = tOut.conn;
+    const long long nblk = count("num_el_blk");
     for (long long b = 1; b <= nblk; ++b) {
         const std::string v = "connect" + std::to_string(b);
         if (!nc.has(v)) continue;
@@ -800,52 +845,66 @@ inline std::shared_ptr<Mesh> readExodusMesh(const std::string& aPath) {
             throw_error("readExodusMesh: element block " + std::to_string(b) + " is " + etype +
                         "; only TETRA4 is on this path");
         for (double x : nc.numbers(v)) {
-            const long long n = (long long)x - 1;
-            if (n < 0 || n >= N) throw_error("readExodusMesh: connectivity out of range");
+            const long long n = exodus_index(x, N);
+            if (n < 0) throw_error("readExodusMesh: connectivity out of range");
             tConn.push_back((int)n);
         }
     }
-    auto tMesh = std::make_shared<Mesh>(tCoords, tConn);
     const long long C = (long long)tConn.size() / 4;
     auto set_names = [&](const std::string& aNames, const std::string& aIds, long long aCount) {
-        std::vector<std::string> out(aCount);
+        std::vector<std::string> out((size_t)aCount);
         const auto names = nc.has(aNames) ? nc.strings(aNames) : std::vector<std::string>();
         const auto ids = nc.has(aIds) ? nc.numbers(aIds) : std::vector<double>();
         for (long long i = 0; i < aCount; ++i) {
             out[i] = i < (long long)names.size() ? names[i] : "";
-            if (out[i].empty()) out[i] = std::to_string(i < (long long)ids.size() ? (long long)ids[i] : i + 1);
+            if (out[i].empty())
+                out[i] = std::to_string(i < (long long)ids.size() && std::isfinite(ids[i]) && std::fabs(ids[i]) < 1e15
+                                            ? (long long)ids[i]
+                                            : i + 1);
         }
         return out;
     };
     static const int kSides[4][3] = {{0, 1, 3}, {1, 2, 3}, {0, 3, 2}, {0, 2, 1}};
-    const long long nss = nc.dim("num_side_sets", 0);
+    const long long nss = count("num_side_sets");
     const auto ss = set_names("ss_names", "ss_prop1", nss);
     for (long long i = 1; i <= nss; ++i) {
         const std::string ev = "elem_ss" + std::to_string(i), sv = "side_ss" + std::to_string(i);
         if (!nc.has(ev)) continue;
         const auto el = nc.numbers(ev), sd = nc.numbers(sv);
+        if (el.size() != sd.size())
+            throw_error("readExodusMesh: side set '" + ss[i - 1] + "' has " + std::to_string(el.size()) +
+                        " elements and " + std::to_string(sd.size()) + " sides");
         std::vector<int> cells, local;
         for (size_t k = 0; k < el.size(); ++k) {
-            const long long c = (long long)el[k] - 1, s = (long long)sd[k] - 1;
-            if (c < 0 || c >= C || s < 0 || s > 3) throw_error("readExodusMesh: side set '" + ss[i - 1] + "' out of range");
+            const long long c = exodus_index(el[k], C), s = exodus_index(sd[k], 4);
+            if (c < 0 || s < 0) throw_error("readExodusMesh: side set '" + ss[i - 1] + "' out of range");
             cells.push_back((int)c);
             local.insert(local.end(), kSides[s], kSides[s] + 3);
         }
-        tMesh->addSideSet(ss[i - 1], cells, local);
+        tOut.side_sets.push_back({ss[i - 1], {cells, local}});
     }
-    const long long nns = nc.dim("num_node_sets", 0);
+    const long long nns = count("num_node_sets");
     const auto ns = set_names("ns_names", "ns_prop1", nns);
     for (long long i = 1; i <= nns; ++i) {
         const std::string nv = "node_ns" + std::to_string(i);
         if (!nc.has(nv)) continue;
         std::vector<int> nodes;
         for (double x : nc.numbers(nv)) {
-            const long long n = (long long)x - 1;
-            if (n < 0 || n >= N) throw_error("readExodusMesh: node set '" + ns[i - 1] + "' out of range");
+            const long long n = exodus_index(x, N);
+            if (n < 0) throw_error("readExodusMesh: node set '" + ns[i - 1] + "' out of range");
             nodes.push_back((int)n);
         }
-        tMesh->addNodeSet(ns[i - 1], nodes);
+        tOut.node_sets.push_back({ns[i - 1], nodes});
     }
+    return tOut;
+}
+
+// The mesh of an Exodus II Tet4 file with its side sets and node sets.
+inline std::shared_ptr<Mesh> readExodusMesh(const std::string& aPath) {
+    const ExodusData tData = readExodusData(aPath);
+    auto tMesh = std::make_shared<Mesh>(tData.coords, tData.conn);
+    for (const auto& s : tData.side_sets) tMesh->addSideSet(s.first, s.second.first, s.second.second);
+    for (const auto& s : tData.node_sets) tMesh->addNodeSet(s.first, s.second);
     return tMesh;
 }
 

@@ -13,6 +13,9 @@
 //        reference_cases --list       case names (no device use)
 //        reference_cases --parser     checks of the parameter-list reader and
 //                                     the expression evaluator (no device use)
+//        reference_cases --exodus <f> the host-side Exodus II reader on one file
+//                                     (no device use): counts, or status 3 and
+//                                     the reader's error
 // The outputs are compared with the reference's golden values and the oracle
 // by the Python tests that run this program.
 #include <cmath>
@@ -594,7 +597,7 @@ int main(int argc, char** argv) {
         {"ExodusMesh", exodus_problem},
     };
     if (argc < 2) {
-        std::fprintf(stderr, "usage: %s <case>|all|--list|--parser\n", argv[0]);
+        std::fprintf(stderr, "usage: %s <case>|all|--list|--parser|--exodus <file>\n", argv[0]);
         return 2;
     }
     const std::string what = argv[1];
@@ -603,6 +606,20 @@ int main(int argc, char** argv) {
         return 0;
     }
     if (what == "--parser") return parser_checks();
+    if (what == "--exodus") {
+        // host-side Exodus reader only (no device work): counts, or the
+        // reader's error with status 3
+        if (argc < 3) return 2;
+        try {
+            const Plato::amd::ExodusData d = Plato::amd::readExodusData(argv[2]);
+            std::printf("{\"nodes\": %zu, \"cells\": %zu, \"side_sets\": %zu, \"node_sets\": %zu}\n",
+                        d.coords.size() / 3, d.conn.size() / 4, d.side_sets.size(), d.node_sets.size());
+            return 0;
+        } catch (const std::exception& e) {
+            std::printf("{\"error\": \"%s\"}\n", e.what());
+            return 3;
+        }
+    }
     int ran = 0;
     for (const auto& c : cases) {
         if (what != c.first && (what != "all" || c.first == "ExodusMesh")) continue;
