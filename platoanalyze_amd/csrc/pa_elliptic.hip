@@ -1471,17 +1471,66 @@ __global__ void k_natural_bc(int nb, const int* __restrict__ bnodes, const int* 
 }
 
 // ---------------------------------------------------------------------------
-// applyBlockConstraints (src/ApplyConstraints.hpp:40-112), one thread per
-// constrained dof as in the reference, then the RHS overwrite.
+// applyBlockConstraints (src/ApplyConstraints.hpp:40-112).  The reference
+// subtracts A[r][c] * g_r from rhs[c] for every constrained dof r and every
+// other dof c of its row with one atomic add each, so the order of the adds
+// into a dof next to several constrained dofs changes from run to run.  Here
+// the same sums run in a fixed order: the constrained values are scattered
+// into a dense vector, each node next to a constrained node gathers its
+// terms from the constrained rows (the graph is symmetric: row n holds every
+// node whose row holds n), and only then are the rows and columns cleared.
 // ---------------------------------------------------------------------------
-template <class VT>
-__global__ void k_constraints_1(int dpn, const int* __restrict__ rowmap, const int* __restrict__ colidx,
-                                VT* __restrict__ vals, double* __restrict__ rhs, int nbc, const int* __restrict__ dofs,
-                                const double* __restrict__ bcv, double scale) {
+// g[dof] = scale * value of each constrained dof; touched[n] = 1 for the
+// nodes of the constrained rows.  g and touched start zeroed.
+__global__ void k_constraints_mark(int dpn, const int* __restrict__ rowmap, const int* __restrict__ colidx, int nbc,
+                                   const int* __restrict__ dofs, const double* __restrict__ bcv, double scale,
+                                   double* __restrict__ g, unsigned char* __restrict__ touched) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbc) return;
     const int rdof = dofs[b];
-    const double val = scale * bcv[b];
+    g[rdof] = scale * bcv[b];
+    const int rnode = rdof / dpn;
+    for (int off = rowmap[rnode]; off < rowmap[rnode + 1]; ++off) touched[colidx[off]] = 1;
+}
+
+// rhs[c] -= A[r][c] * g[r] over the constrained dofs r != c whose rows hold c,
+// in the order of node c's row and then of r's local dof.  One thread per node.
+template <class VT>
+__global__ void k_constraints_rhs(int N, int dpn, const int* __restrict__ rowmap, const int* __restrict__ colidx,
+                                  const VT* __restrict__ vals, const double* __restrict__ g,
+                                  const unsigned char* __restrict__ touched, double* __restrict__ rhs) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N || !touched[n]) return;
+    double acc[4];
+    for (int lc = 0; lc < dpn; ++lc) acc[lc] = rhs[(long)n * dpn + lc];
+    for (int e = rowmap[n]; e < rowmap[n + 1]; ++e) {
+        const int r = colidx[e];
+        bool any = false;
+        for (int lr = 0; lr < dpn; ++lr) any |= g[(long)r * dpn + lr] != 0.0;
+        if (!any) continue;
+        int o = rowmap[r];
+        const int o1 = rowmap[r + 1];
+        while (o < o1 && colidx[o] != n) ++o;
+        if (o == o1) continue;
+        for (int lr = 0; lr < dpn; ++lr) {
+            const double v = g[(long)r * dpn + lr];
+            if (v == 0.0) continue;
+            for (int lc = 0; lc < dpn; ++lc)
+                if (r != n || lr != lc) acc[lc] -= (double)vals[(long)dpn * dpn * o + dpn * lr + lc] * v;
+        }
+    }
+    for (int lc = 0; lc < dpn; ++lc) rhs[(long)n * dpn + lc] = acc[lc];
+}
+
+// Constrained rows become identity rows and their columns are cleared (the
+// reference's first loop without its right-hand-side update); every write is
+// a constant, so the threads' order does not matter.
+template <class VT>
+__global__ void k_constraints_1(int dpn, const int* __restrict__ rowmap, const int* __restrict__ colidx,
+                                VT* __restrict__ vals, int nbc, const int* __restrict__ dofs) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbc) return;
+    const int rdof = dofs[b];
     const int rnode = rdof / dpn, lr = rdof % dpn;
     for (int off = rowmap[rnode]; off < rowmap[rnode + 1]; ++off) {
         const int cnode = colidx[off];
@@ -1491,8 +1540,6 @@ __global__ void k_constraints_1(int dpn, const int* __restrict__ rowmap, const i
             if (cdof == rdof) {
                 vals[ent] = (VT)1.0;
             } else {
-                const double a = (double)vals[ent];
-                if (val != 0.0) atomicAdd(&rhs[cdof], -a * val);
                 vals[ent] = (VT)0.0;
                 for (int o2 = rowmap[cnode]; o2 < rowmap[cnode + 1]; ++o2)
                     if (dpn * colidx[o2] + lr == rdof) vals[(long)dpn * dpn * o2 + dpn * lc + lr] = (VT)0.0;
@@ -1852,6 +1899,33 @@ void self_adjoint_gradient(const pa_vector_function* vf, const pa_scalar_functio
     check_launch();
 }
 
+void apply_block_constraints(const pa_mesh* m, int dpn, void* vals, int precision, double* rhs, int nbc,
+                             const int* dofs, const double* bcv, double scale, double* dense, unsigned char* touched,
+                             hipStream_t st) {
+    if (nbc == 0) return;
+    PA_REQUIRE(precision == PA_FP64 || precision == PA_FP32, PA_ERR_ARG, "apply_block_constraints: unknown precision");
+    const long nd = (long)m->N * dpn;
+    PA_HIP(hipMemsetAsync(dense, 0, nd * sizeof(double), st));
+    PA_HIP(hipMemsetAsync(touched, 0, (size_t)m->N, st));
+    const dim3 grid((nbc + 255) / 256), blk(256), ngrid((m->N + 255) / 256);
+    k_constraints_mark<<<grid, blk, 0, st>>>(dpn, m->d_rowmap.p, m->d_colidx.p, nbc, dofs, bcv, scale, dense, touched);
+    check_launch();
+    if (precision == PA_FP64) {
+        k_constraints_rhs<double><<<ngrid, blk, 0, st>>>(m->N, dpn, m->d_rowmap.p, m->d_colidx.p,
+                                                         (const double*)vals, dense, touched, rhs);
+        check_launch();
+        k_constraints_1<double><<<grid, blk, 0, st>>>(dpn, m->d_rowmap.p, m->d_colidx.p, (double*)vals, nbc, dofs);
+    } else {
+        k_constraints_rhs<float><<<ngrid, blk, 0, st>>>(m->N, dpn, m->d_rowmap.p, m->d_colidx.p, (const float*)vals,
+                                                        dense, touched, rhs);
+        check_launch();
+        k_constraints_1<float><<<grid, blk, 0, st>>>(dpn, m->d_rowmap.p, m->d_colidx.p, (float*)vals, nbc, dofs);
+    }
+    check_launch();
+    k_constraints_2<<<grid, blk, 0, st>>>(nbc, dofs, bcv, scale, rhs);
+    check_launch();
+}
+
 }  // namespace pa
 
 using pa::guarded;
@@ -2187,19 +2261,16 @@ int pa_apply_block_constraints(const pa_mesh* m, int dpn, void* vals, int precis
         PA_REQUIRE(dpn >= 1 && dpn <= 4, PA_ERR_ARG, "pa_apply_block_constraints: dpn out of range");
         if (nbc == 0) return;
         PA_REQUIRE(nbc > 0 && dofs && bcv, PA_ERR_ARG, "pa_apply_block_constraints: bad constraint list");
+        PA_REQUIRE(precision == PA_FP64 || precision == PA_FP32, PA_ERR_ARG,
+                   "pa_apply_block_constraints: unknown precision");
+        // work vectors for this call, freed once the stream has run it
         const hipStream_t st = (hipStream_t)stream;
-        const dim3 grid((nbc + 255) / 256), blk(256);
-        if (precision == PA_FP64)
-            pa::k_constraints_1<double><<<grid, blk, 0, st>>>(dpn, m->d_rowmap.p, m->d_colidx.p, (double*)vals, rhs,
-                                                              nbc, dofs, bcv, scale);
-        else if (precision == PA_FP32)
-            pa::k_constraints_1<float><<<grid, blk, 0, st>>>(dpn, m->d_rowmap.p, m->d_colidx.p, (float*)vals, rhs,
-                                                             nbc, dofs, bcv, scale);
-        else
-            throw pa::Error(PA_ERR_ARG, "pa_apply_block_constraints: unknown precision");
-        pa::check_launch();
-        pa::k_constraints_2<<<grid, blk, 0, st>>>(nbc, dofs, bcv, scale, rhs);
-        pa::check_launch();
+        pa::DevBuf<double> dense;
+        pa::DevBuf<unsigned char> touched;
+        dense.alloc((size_t)m->N * dpn);
+        touched.alloc((size_t)m->N);
+        pa::apply_block_constraints(m, dpn, vals, precision, rhs, nbc, dofs, bcv, scale, dense.p, touched.p, st);
+        PA_HIP(hipStreamSynchronize(st));
     });
 }
 

@@ -200,6 +200,11 @@ void criterion_gradient_x(const pa_scalar_function* sf, const double* u, const d
 // assembly kernels; used by the problem driver.
 void self_adjoint_gradient(const pa_vector_function* vf, const pa_scalar_function* sf, const double* u,
                            const double* z, double* out, hipStream_t st);
+// applyBlockConstraints in a fixed summation order, with caller-owned work
+// vectors: dense (N * dpn doubles) and touched (N bytes).
+void apply_block_constraints(const pa_mesh* m, int dpn, void* vals, int precision, double* rhs, int nbc,
+                             const int* dofs, const double* bcv, double scale, double* dense, unsigned char* touched,
+                             hipStream_t st);
 // z = M^-1 r by one multigrid V-cycle (pa_multigrid.hip)
 void multigrid_apply(pa_multigrid* mg, const double* r, double* z, hipStream_t st);
 }  // namespace pa

@@ -33,6 +33,8 @@ struct pa_plasticity_problem {
     std::vector<double> h_vals;
     pa::DevBuf<int> bc_dofs;
     pa::DevBuf<double> bc_inc, bc_zero;
+    pa::DevBuf<double> bc_dense;  // constraint work vectors (N * dpn, N)
+    pa::DevBuf<unsigned char> bc_touch;
     // step state slots (see slot())
     pa::DevBuf<double> U, Cs, PI;
     int nslots = 0, nkept = 0;
@@ -662,9 +664,8 @@ const double* prev_c(pa_plasticity_problem* P, int k) { return k ? step_c(P, k -
 
 void constraints(pa_plasticity_problem* P, double* rhs, const double* bcv, hipStream_t st) {
     if (P->nbc == 0) return;
-    const int rc = pa_apply_block_constraints(P->p->mesh, P->p->dpn, P->vals.p, PA_FP64, rhs, P->nbc, P->bc_dofs.p,
-                                              bcv, 1.0, st);
-    if (rc != PA_OK) throw Error(rc, pa_last_error());
+    apply_block_constraints(P->p->mesh, P->p->dpn, P->vals.p, PA_FP64, rhs, P->nbc, P->bc_dofs.p, bcv, 1.0,
+                            P->bc_dense.p, P->bc_touch.p, st);
 }
 
 // The projected pressure gradient step k uses (src/PlasticityProblem.hpp:872-891):
@@ -892,6 +893,8 @@ int pa_plasticity_problem_create(pa_plasticity* p, int nbc, const int* dofs, con
                 P->bc_inc.alloc(nbc);
                 P->bc_zero.alloc(nbc);
                 PA_HIP(hipMemset(P->bc_zero.p, 0, nbc * sizeof(double)));
+                P->bc_dense.alloc((size_t)p->dpn * p->mesh->N);
+                P->bc_touch.alloc(p->mesh->N);
             }
             const int K = P->opt.num_steps;
             const int mstride = pa::j2::stride(P);

@@ -37,6 +37,8 @@ struct pa_problem {
     int nbc = 0;
     pa::DevBuf<int> bc_dofs;
     pa::DevBuf<double> bc_vals;
+    pa::DevBuf<double> bc_dense;        // constraint work vectors (N * dpn, N)
+    pa::DevBuf<unsigned char> bc_touch;
     pa::DevBuf<double> K64;
     pa::DevBuf<float> K32;
     pa::DevBuf<double> rhs, res, dx, zero;
@@ -519,6 +521,10 @@ int pa_problem_create(pa_vector_function* vf, pa_scalar_function* sf, int nbc, c
             P->nbc = nbc;
             P->bc_dofs.upload(dofs, nbc);
             P->bc_vals.upload(vals, nbc);
+            if (nbc) {
+                P->bc_dense.alloc(n);
+                P->bc_touch.alloc(vf->mesh->N);
+            }
             pa::check_status(pa_solver_create(vf->mesh, vf->dpn, opt->precond, &P->solver));
             P->rhs.alloc(n);
             P->res.alloc(n);
@@ -598,8 +604,8 @@ int pa_problem_solution(pa_problem* P, const double* z, double* u, pa_solve_stat
         pa::k_neg<<<nvb, pa::kB, 0, st>>>(n, P->rhs.p);
         pa::check();
         if (P->nbc)
-            pa::check_status(pa_apply_block_constraints(m, dpn, K, P->opt.mode, P->rhs.p, P->nbc, P->bc_dofs.p,
-                                                        P->bc_vals.p, 1.0, stream));
+            pa::apply_block_constraints(m, dpn, K, P->opt.mode, P->rhs.p, P->nbc, P->bc_dofs.p, P->bc_vals.p, 1.0,
+                                        P->bc_dense.p, P->bc_touch.p, st);
         if (P->opt.precond == PA_PRECOND_MULTIGRID) {
             PA_REQUIRE(P->mg, PA_ERR_STATE, "pa_problem_solution: multigrid selected but not set");
             pa::check_status(pa_multigrid_setup(P->mg, K, P->opt.mode, P->nbc, P->bc_dofs.p, stream));

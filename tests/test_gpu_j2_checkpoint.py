@@ -81,6 +81,30 @@ def test_checkpointed_gradient_is_bitwise_equal(pa, form):
         assert prob.criterion_value(z, -1.0) == F, m
 
 
+def test_repeat_solves_are_bitwise_equal(pa):
+    """Two problems solved from scratch, and a checkpointed third, on a grid
+    whose constraints span several wavefronts: states, criterion value and
+    gradient agree bit for bit (no order-dependent sums on the path)."""
+    print("case: J2 repeat solves, displacement, 16x8x8, 4 steps, strides 1/1/2", flush=True)
+    mesh = pa.Mesh.box(16, 8, 8, 2.0, 1.0, 1.0)
+    pl = pa.Plasticity(mesh, formulation="displacement", **PARAMS)
+    xm, xp = mesh.node_set("x-"), mesh.node_set("x+")
+    dofs = np.concatenate([xm * 3, xm * 3 + 1, xm * 3 + 2, xp * 3])
+    vals = np.concatenate([np.zeros(3 * xm.size), np.full(xp.size, 0.012)])
+    order = np.argsort(dofs)
+    dofs, vals = dofs[order].astype(np.int32), vals[order]
+    z = np.random.default_rng(4).uniform(0.6, 1.0, mesh.num_nodes)
+    out = []
+    for m in (1, 1, 2):
+        prob = pa.PlasticityProblem(pl, dofs, vals, num_steps=4, checkpoint_stride=m, max_newton=25,
+                                    newton_stop_tol=1e-11, newton_current_norm_tol=1e-11, linear_tol=1e-12)
+        assert prob.solution(z)
+        out.append((prob.state(3)[0], prob.criterion_value(z, -1.0), prob.criterion_gradient(z, -1.0).numpy()))
+    assert (out[0][0] != 0).any()
+    for u, F, g in out[1:]:
+        assert np.array_equal(u, out[0][0]) and F == out[0][1] and np.array_equal(g, out[0][2])
+
+
 def test_gradient_matches_oracle_with_checkpoints(pa):
     print("case: J2 checkpointed adjoint vs oracle, mixed, stride 3", flush=True)
     pl, dofs, vals, z = setup(pa, "mixed")
