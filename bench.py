@@ -523,7 +523,7 @@ def main():
     jac_bytes = algorithmic_bytes(C, N, nnzb, 3, 8, with_residual=False)
     step_bytes = algorithmic_bytes(C, N, nnzb, 3, 8)
     achieved = jac_bytes / (t_jac * 1e-3) / 1e9
-    traffic = read_traffic("k_tile_jacobian_fp64_elastic_128x64x64")
+    traffic = read_traffic("k_grid_jacobian3_fp64_elastic_128x64x64")
     cells_s = C / (ms_step * 1e-3)
     out = {
         "metric": "cells/s res+Jac assembly",
@@ -543,7 +543,7 @@ def main():
                    "cells": C, "nodes": N, "blocks": nnzb, "precision": "fp64"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "gradient_u (FP64): k_tile_jacobian<3,double,6>",
+                     "kernel": "gradient_u (FP64): k_grid_jacobian3<double,4,4,4,3>",
                      "algorithmic_bytes_per_launch": jac_bytes, "avg_launch_ms": t_jac},
         "kernels_ms": {"gradient_u_fp64": t_jac, "value": t_val, "gradient_u_fp32": t_jac32,
                        "value_gradient_u_fused_fp64": t_fused, "value_gradient_u_fused_fp32": t_fused32},

@@ -1787,9 +1787,13 @@ template <class VT, bool RES, bool TR>
 void jacobian(const pa_vector_function* vf, const double* u, const double* z, VT* vals, double* R, hipStream_t st) {
     const pa_mesh* m = vf->mesh;
     if constexpr (!RES) {
-        // thermal on Kuhn grids: the streaming kernel; else the tiles
+        // Kuhn grids: the grid kernels (symmetric operators: gradient_u_T is
+        // the same matrix, bit for bit); else the tiles
         if (vf->dpn == 1 && m->has_grid &&
             grid_jacobian1(m, material_of(vf->physics, vf->material, vf->penalty), z, vals, st))
+            return;
+        if (vf->dpn == 3 && m->has_grid &&
+            grid_jacobian3(m, material_of(vf->physics, vf->material, vf->penalty), z, vals, st))
             return;
         if (tile_jacobian(vf, z, vals, st)) return;
     }

@@ -1,10 +1,12 @@
-"""Streaming grid kernels (value of both physics, the thermal gradient_u) on
-Kuhn-split hex grids against the oracle and against the cell-tile kernels.
+"""Grid kernels on Kuhn-split hex grids (the streaming value of both physics
+and thermal gradient_u, the brick kernel of the elastic gradient_u) against
+the oracle and against the cell-tile kernels.
 
 The grid dimensions cover every branch of the kernels' work split: one and
 several z blocks of 64/63 rows, the z block whose last lane also owns the top
 row (65 and 128 node layers), several line blocks of 4/3 lines, several x
-blocks, and degenerate one-hex directions.  Bars: FP64 1e-12 relative to the
+blocks, and degenerate one-hex directions; for the elastic bricks (2 x 3 x 4
+rows) partial bricks in every direction.  Bars: FP64 1e-12 relative to the
 oracle (max norm), FP32 values 2e-6; against the tile kernels (same
 mathematics, different rounding order) 1e-13.
 """
@@ -33,6 +35,18 @@ def pa():
 def rel(a, b):
     a, b = np.asarray(a, float).ravel(), np.asarray(b, float).ravel()
     return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def block_transpose(rowmap, colidx, vals, d):
+    """The block CRS matrix with every block (i, j) replaced by block (j, i)^T."""
+    B = vals.reshape(-1, d, d)
+    out = np.empty_like(B)
+    for i in range(rowmap.size - 1):
+        for e in range(rowmap[i], rowmap[i + 1]):
+            j = colidx[e]
+            f = rowmap[j] + np.searchsorted(colidx[rowmap[j]:rowmap[j + 1]], i)
+            out[e] = B[f].T
+    return out.ravel()
 
 
 def _mesh(pa, coords, conn, grid=True):
@@ -88,12 +102,16 @@ def test_grid_vs_oracle(pa, dims, physics):
     assert rel(sf.gradient_u(u, z).numpy(), om.energy_gradient_u(physics, u, z)) < TOL64
     R2, K2 = vf.value_gradient_u(u, z)
     assert rel(R2.numpy(), Rref) < TOL64
-    if physics == "thermal":
-        Kref = om.gradient_u(physics, u, z)
-        assert rel(vf.gradient_u(u, z).entries(), Kref) < TOL64
-        assert rel(K2.entries(), Kref) < TOL64
-        assert rel(vf.gradient_u(u, z, precision="fp32").entries(), Kref) < TOL32
-        assert rel(vf.gradient_u_T(u, z).entries(), om.gradient_u_T(physics, u, z)) < TOL64
+    Kref = om.gradient_u(physics, u, z)
+    K = vf.gradient_u(u, z).entries()
+    assert rel(K, Kref) < TOL64
+    assert rel(K2.entries(), Kref) < TOL64
+    assert rel(vf.gradient_u(u, z, precision="fp32").entries(), Kref) < TOL32
+    assert rel(vf.gradient_u_T(u, z).entries(), om.gradient_u_T(physics, u, z)) < TOL64
+    if physics == "elastic" and N < 20000:
+        # block (m, n) is block (n, m) transposed, bit for bit
+        rowmap, colidx = mesh.graph()
+        assert np.array_equal(block_transpose(rowmap, colidx, K, 3), K)
 
 
 @pytest.mark.parametrize("physics", ["elastic", "thermal"])
@@ -111,15 +129,14 @@ def test_grid_vs_tiles(pa, dims, physics):
     z = rng.uniform(0.3, 1.0, N)
     a, b = _setup(pa, mg, coords, conn, physics, L[0])[0], _setup(pa, mt, coords, conn, physics, L[0])[0]
     assert rel(a.value(u, z).numpy(), b.value(u, z).numpy()) < 1e-13
-    if physics == "thermal":
-        assert rel(a.gradient_u(u, z).entries(), b.gradient_u(u, z).entries()) < 1e-13
-        ka = a.gradient_u(u, z, precision="fp32").entries()
-        kb = b.gradient_u(u, z, precision="fp32").entries()
-        assert rel(ka, kb) < 1e-7
+    assert rel(a.gradient_u(u, z).entries(), b.gradient_u(u, z).entries()) < 1e-13
+    ka = a.gradient_u(u, z, precision="fp32").entries()
+    kb = b.gradient_u(u, z, precision="fp32").entries()
+    assert rel(ka, kb) < 1e-7
 
 
 def test_grid_deterministic(pa):
-    print("case: grid kernels bitwise repeat, 12x9x70 thermal", flush=True)
+    print("case: grid kernels bitwise repeat, 12x9x70 thermal and elastic", flush=True)
     coords, conn = pa.box_mesh(12, 9, 70, 1.0, 1.0, 1.0)
     mesh = pa.Mesh(coords, conn)
     N = mesh.num_nodes
@@ -129,6 +146,11 @@ def test_grid_deterministic(pa):
     vf.add_body_load(0, 1.0)
     assert np.array_equal(vf.value(T, z).numpy(), vf.value(T, z).numpy())
     assert np.array_equal(vf.gradient_u(T, z).entries(), vf.gradient_u(T, z).entries())
+    ef = pa.VectorFunction(mesh, "Mechanics", youngs=1.0, poisson=0.3, penalty_exponent=3.0, penalty_min=1e-9)
+    u = rng.standard_normal(3 * N) * 1e-2
+    assert np.array_equal(ef.gradient_u(u, z).entries(), ef.gradient_u(u, z).entries())
+    assert np.array_equal(ef.gradient_u(u, z, precision="fp32").entries(),
+                          ef.gradient_u(u, z, precision="fp32").entries())
 
 
 def test_shuffled_cells_keep_grid(pa):
@@ -148,6 +170,7 @@ def test_shuffled_cells_keep_grid(pa):
     om = oracle.Model(coords, conn2, youngs=1.0, poisson=0.3, penalty_exponent=3.0, penalty_min=1e-9)
     vf = pa.VectorFunction(mesh, "Mechanics", youngs=1.0, poisson=0.3, penalty_exponent=3.0, penalty_min=1e-9)
     assert rel(vf.value(u, z).numpy(), om.value("elastic", u, z)) < TOL64
+    assert rel(vf.gradient_u(u, z).entries(), om.gradient_u("elastic", u, z)) < TOL64
     T = rng.standard_normal(N)
     ot = oracle.Model(coords, conn2, conductivity=1.0, penalty_exponent=3.0, penalty_min=1e-9)
     tf = pa.VectorFunction(mesh, "Thermal", conductivity=1.0, penalty_exponent=3.0, penalty_min=1e-9)
@@ -203,3 +226,4 @@ def test_grid_penalty_exponents(pa, p):
     oe = oracle.Model(coords, conn, youngs=1.0, poisson=0.3, penalty_exponent=p, penalty_min=1e-3)
     ef = pa.VectorFunction(mesh, "Mechanics", youngs=1.0, poisson=0.3, penalty_exponent=p, penalty_min=1e-3)
     assert rel(ef.value(u, z).numpy(), oe.value("elastic", u, z)) < TOL64
+    assert rel(ef.gradient_u(u, z).entries(), oe.gradient_u("elastic", u, z)) < TOL64
