@@ -1024,13 +1024,20 @@ __global__ __launch_bounds__(BX * BY * BZ * kJ3Slots) void k_grid_jacobian3(Grid
             }
         }
         __syncthreads();
-        // stage 4: wave w stores lines w, w + waves, ...: each one contiguous CRS range
+        // stage 4: wave w stores lines w, w + waves, ...: each one contiguous CRS
+        // range, in 16-byte pairs (a single value first when the range starts
+        // at an odd double, and last when one is left over)
+        static_assert(B::line_blocks * 9 % 2 == 0, "16-byte aligned lines in the stage");
         const int wv = tid >> 6, lane = tid & 63;
         for (int line = wv; line < B::lines; line += B::waves) {
             const int n = llen[line] * 9;
             const VT* src = stage + line * B::line_blocks * 9;
             VT* dstg = vals + (long)lstart[line] * 9;
-            for (int e = lane; e < n; e += 64) dstg[e] = src[e];
+            const int h = (int)(lstart[line] & 1);  // 9 * lstart is odd iff lstart is
+            if (lane == 0 && h && n > 0) dstg[0] = src[0];
+            for (int e = h + 2 * lane; e + 1 < n; e += 128)
+                *reinterpret_cast<double2*>(dstg + e) = make_double2(src[e], src[e + 1]);
+            if (lane == 0 && n > h && ((n - h) & 1)) dstg[n - 1] = src[n - 1];
         }
     }
 }
