@@ -148,6 +148,23 @@ int pa_vector_function_add_uniform_natural_bc(pa_vector_function* vf, const char
  */
 int pa_vector_function_add_body_load(pa_vector_function* vf, int dof, double value);
 
+/* A body load given by a 'Function' string of x, y and z on any dof of any
+ * physics: BodyLoad::get (src/BodyLoads.hpp:54-122) with the function
+ * evaluated at each cell's cubature point, the centroid (src/PlatoMeshExpr.hpp:
+ * 16-50); R[cell node, dof] += -(1/6) |det J| f (1/4) mean cell density.  The
+ * function is evaluated once, here, on the host.  gradient_z, gradient_x and the
+ * problem's criterion gradient include the term; like the reference,
+ * gradient_x takes f as constant per cell (its point is mapped in plain
+ * doubles).  Syntax: numbers, x y z t, pi, + - * / ^, parentheses, sin cos tan
+ * asin acos atan sinh cosh tanh exp log log10 sqrt abs fabs pow atan2 min max,
+ * leading 'name = expr;' assignments. */
+int pa_vector_function_add_body_load_function(pa_vector_function* vf, int dof, const char* function);
+
+/* Evaluates a 'Function' string at n points (host arrays; a null coordinate
+ * array reads as zeros) and time t.  No GPU work. */
+int pa_expression_eval(const char* function, int n, const double* x, const double* y, const double* z, double t,
+                       double* out);
+
 /* VectorFunction::value (src/elliptic/VectorFunction.hpp:250-334): R[N*dpn]. */
 int pa_vector_function_value(const pa_vector_function* vf, const double* u_dev, const double* z_dev, double* r_dev,
                              void* stream);

@@ -411,99 +411,23 @@ inline std::shared_ptr<ParameterList> getParametersFromXmlString(const std::stri
     return tRoot;
 }
 
-// Constant or linear-in-time expressions of the input's Function strings
-// (numbers, t, + - * / ^, parentheses).  The reference evaluates them with
-// Omega_h's expression reader (src/PlatoMeshExpr.hpp:16-50); functions of
-// x, y, z are not on this path and throw.
+// The input's Function strings, evaluated by the library's reader
+// (pa_expression_eval: numbers, x y z t, pi, + - * / ^, parentheses, the
+// common functions, leading 'name = expr;'), here as functions of t (the
+// reference's Teuchos MathExpr, src/PlatoMathExpr.hpp).
 class Expression {
   public:
     explicit Expression(std::string aText) : mText(std::move(aText)) {}
     double operator()(double t) const {
-        mPos = 0;
-        mT = t;
-        const double v = sum();
-        skip();
-        if (mPos != mText.size()) fail();
+        double v = 0.0;
+        if (pa_expression_eval(mText.c_str(), 1, nullptr, nullptr, nullptr, t, &v) != PA_OK)
+            throw_error(pa_last_error());
         return v;
     }
     const std::string& text() const { return mText; }
 
   private:
-    [[noreturn]] void fail() const {
-        throw_error("expression '" + mText + "': only numbers, t, + - * / ^ and parentheses are supported here");
-    }
-    void skip() const {
-        while (mPos < mText.size() && std::isspace((unsigned char)mText[mPos])) ++mPos;
-    }
-    double sum() const {
-        double v = product();
-        for (;;) {
-            skip();
-            if (mPos < mText.size() && (mText[mPos] == '+' || mText[mPos] == '-')) {
-                const char op = mText[mPos++];
-                const double r = product();
-                v = op == '+' ? v + r : v - r;
-            } else {
-                return v;
-            }
-        }
-    }
-    double product() const {
-        double v = power();
-        for (;;) {
-            skip();
-            if (mPos < mText.size() && (mText[mPos] == '*' || mText[mPos] == '/')) {
-                const char op = mText[mPos++];
-                const double r = power();
-                v = op == '*' ? v * r : v / r;
-            } else {
-                return v;
-            }
-        }
-    }
-    double power() const {
-        const double b = unary();
-        skip();
-        if (mPos < mText.size() && mText[mPos] == '^') {
-            ++mPos;
-            return std::pow(b, power());
-        }
-        return b;
-    }
-    double unary() const {
-        skip();
-        if (mPos < mText.size() && (mText[mPos] == '-' || mText[mPos] == '+')) {
-            const char op = mText[mPos++];
-            const double v = unary();
-            return op == '-' ? -v : v;
-        }
-        return atom();
-    }
-    double atom() const {
-        skip();
-        if (mPos >= mText.size()) fail();
-        if (mText[mPos] == '(') {
-            ++mPos;
-            const double v = sum();
-            skip();
-            if (mPos >= mText.size() || mText[mPos] != ')') fail();
-            ++mPos;
-            return v;
-        }
-        if (mText[mPos] == 't' && (mPos + 1 == mText.size() || !std::isalnum((unsigned char)mText[mPos + 1]))) {
-            ++mPos;
-            return mT;
-        }
-        const char* s = mText.c_str() + mPos;
-        char* tEnd = nullptr;
-        const double v = std::strtod(s, &tEnd);
-        if (tEnd == s) fail();
-        mPos += (size_t)(tEnd - s);
-        return v;
-    }
     std::string mText;
-    mutable size_t mPos = 0;
-    mutable double mT = 0.0;
 };
 
 // ------------------------------------------------------------------ mesh
@@ -1033,6 +957,16 @@ inline double constant_function(const std::string& aText, const std::string& aWh
     return v0;
 }
 
+// A Function string that is a plain number (no names or operators beyond a sign).
+inline bool plain_number(const std::string& aText, double& aValue) {
+    const char* b = aText.c_str();
+    char* e = nullptr;
+    aValue = std::strtod(b, &e);
+    if (e == b) return false;
+    while (*e && std::isspace((unsigned char)*e)) ++e;
+    return *e == 0;
+}
+
 // Dirichlet dofs and end values of 'Essential Boundary Conditions'
 // (src/EssentialBCs.hpp:81-99, src/EssentialBC.hpp:31-65), sorted, the later
 // entry winning on a repeated dof.  allowTime: 'Time Dependent' functions
@@ -1051,9 +985,10 @@ inline void essential_bcs(const ParameterList& aParams, const Mesh& aMesh, int a
             if (tType == "Zero Value") {
                 tValue = 0.0;
             } else if (tType == "Fixed Value") {
-                tValue = bc.isType<std::string>("Function")
-                             ? constant_function(bc.get<std::string>("Function"), "essential BC '" + tName + "'")
-                             : bc.get<double>("Value");
+                // a Function is taken at time 0, as the reference's elliptic
+                // solve does (src/EssentialBC.hpp:95-105)
+                tValue = bc.isType<std::string>("Function") ? Expression(bc.get<std::string>("Function"))(0.0)
+                                                            : bc.get<double>("Value");
                 if (aAllowTime && tValue != 0.0)
                     throw_error("essential BC '" + tName + "': a constant non-zero value is not supported by the "
                                 "pseudo-time driver (use 'Time Dependent' with a function of t)");
@@ -1225,11 +1160,19 @@ class VectorFunction {
         }
     }
     // BodyLoads (src/BodyLoads.hpp:40-122): constant 'Function' at 'Index'.
+    // 'Body Loads' (src/BodyLoads.hpp:38-122): a constant thermal source takes
+    // the fused path; any other Function of x, y, z (or any elastic load) is
+    // evaluated at the cell centroids by the library.
     void readBodyLoads(const ParameterList& aList) {
         for (const auto& tName : aList.sublistNames()) {
             const auto& bl = aList.sublist(tName);
-            const double v = detail::constant_function(bl.get<std::string>("Function"), "body load '" + tName + "'");
-            check(pa_vector_function_add_body_load(mVf.get(), bl.get<int>("Index", 0), v));
+            const std::string tFunction = bl.get<std::string>("Function");
+            const int tIndex = bl.get<int>("Index", 0);
+            double v = 0.0;
+            if (mNumDofsPerNode == 1 && tIndex == 0 && detail::plain_number(tFunction, v))
+                check(pa_vector_function_add_body_load(mVf.get(), tIndex, v));
+            else
+                check(pa_vector_function_add_body_load_function(mVf.get(), tIndex, tFunction.c_str()));
         }
     }
 

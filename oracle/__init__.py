@@ -61,6 +61,8 @@ class Problem(ctypes.Structure):
         ("has_body", ctypes.c_int),
         ("body_dof", ctypes.c_int),
         ("body_value", ctypes.c_double),
+        ("body_cells", ctypes.c_void_p),
+        ("body_cells_dof", ctypes.c_int),
     ]
 
 
@@ -226,7 +228,8 @@ class Model:
     """Holds the arrays a Problem struct points to (keeps them alive)."""
 
     def __init__(self, coords, conn, *, youngs=1.0, poisson=0.3, conductivity=1.0, penalty_exponent=3.0,
-                 penalty_min=1e-9, face_cells=None, face_lnodes=None, flux=(0.0, 0.0, 0.0), body=None):
+                 penalty_min=1e-9, face_cells=None, face_lnodes=None, flux=(0.0, 0.0, 0.0), body=None,
+                 body_cells=None):
         self.coords = np.ascontiguousarray(coords, np.float64)
         self.conn = np.ascontiguousarray(conn, np.int32)
         self.N = self.coords.size // 3
@@ -245,6 +248,10 @@ class Model:
             P.flux[k] = flux[k]
         if body is not None:
             P.has_body, P.body_dof, P.body_value = 1, int(body[0]), float(body[1])
+        if body_cells is not None:  # (dof, the load function's value at each cell's centroid)
+            self.body_cells = np.ascontiguousarray(body_cells[1], np.float64)
+            assert self.body_cells.size == self.C
+            P.body_cells, P.body_cells_dof = _p(self.body_cells).value, int(body_cells[0])
         self.P = P
         self.rowmap, self.colidx = graph(self.N, self.conn)
         self.nnzb = self.colidx.size

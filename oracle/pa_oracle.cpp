@@ -128,6 +128,8 @@ typedef struct {
     int has_body;             // constant body load/source present
     int body_dof;
     double body_value;
+    const double* body_cells;  // body load given per cell (the function at the centroid), or null
+    int body_cells_dof;
 } orc_problem;
 
 }  // extern "C"
@@ -279,6 +281,21 @@ inline long entry_ordinal(const int* rowmap, const int* colidx, int row, int col
 // (src/StressDivergence.hpp:70-90).  Cubature weight 1/6
 // (src/LinearTetCubRuleDegreeOne.hpp:72-78).
 // ---------------------------------------------------------------------------
+// BodyLoad::get with scale -1 for a load given per cell
+// (src/BodyLoads.hpp:54-122; the function evaluated at the one-point
+// cubature's point, the centroid, src/PlatoMeshExpr.hpp:16-50): weight
+// -1 * (1/6) |det J|, times f_c, basis 1/4, unpenalised mean cell density,
+// on dof body_cells_dof of each node.  f_c enters as a constant, as in the
+// reference, whose cubature points are mapped in plain doubles.
+template <class K, class Z, class R>
+void body_cell_load(const orc_problem* P, int c, const K& detj_abs, const std::vector<Z>& ctrlWS, int dpn, R* out) {
+    Z dens = 0.0;
+    for (int n = 0; n < 4; ++n) dens += ctrlWS[(size_t)c * 4 + n];
+    dens = dens / 4.0;
+    K w = -1.0 * (1.0 / 6.0) * detj_abs;
+    for (int n = 0; n < 4; ++n) out[n * dpn + P->body_cells_dof] += w * P->body_cells[c] * 0.25 * dens;
+}
+
 template <class S, class Z, class R, class K>
 void elastic_evaluate(const orc_problem* P, const std::vector<K>& cfgWS, const std::vector<S>& stateWS,
                       const std::vector<Z>& ctrlWS, std::vector<R>& result) {
@@ -293,6 +310,7 @@ void elastic_evaluate(const orc_problem* P, const std::vector<K>& cfgWS, const s
         K cfg[4][3], g[4][3], vol;
         for (int n = 0; n < 4; ++n) for (int d = 0; d < 3; ++d) cfg[n][d] = cfgWS[(size_t)c * 12 + n * 3 + d];
         compute_gradient(cfg, g, vol);
+        K detj_abs = vol;
         vol *= qw;
         const S* u = &stateWS[(size_t)c * 12];
         // strain (StrainScalarType: the state type, or the config type for gradient_x)
@@ -327,6 +345,7 @@ void elastic_evaluate(const orc_problem* P, const std::vector<K>& cfgWS, const s
             for (int n = 0; n < 4; ++n)
                 for (int dj = 0; dj < 3; ++dj)
                     out[n * 3 + di] += 1.0 * vol * sig[kVoigt[di][dj]] * g[n][dj];
+        if (P->body_cells) body_cell_load(P, c, detj_abs, ctrlWS, 3, out);
     }
 }
 
@@ -378,10 +397,12 @@ void thermal_evaluate(const orc_problem* P, const std::vector<K>& cfgWS, const s
             K w = -1.0 * qw * detj_abs;
             for (int n = 0; n < 4; ++n) out[n] += w * P->body_value * 0.25 * dens;
         }
+        if (P->body_cells) body_cell_load(P, c, detj_abs, ctrlWS, 1, out);
     }
 }
 
-// Elastic body load is not used by the elastic workloads; thermal only.
+// The constant elastic body load is not used by the elastic workloads; both
+// physics take a load given per cell (body_cell_load).
 
 // Uniform natural BC on a side set, src/SurfaceLoadIntegral.hpp:97-149 with
 // CalculateSurfaceJacobians/Area<3> (src/SurfaceIntegralUtilities.hpp): the

@@ -127,6 +127,8 @@ def _load():
         "pa_vector_function_dofs_per_node": ([vp, ip], i),
         "pa_vector_function_add_uniform_natural_bc": ([vp, ctypes.c_char_p, vp], i),
         "pa_vector_function_add_body_load": ([vp, i, d], i),
+        "pa_vector_function_add_body_load_function": ([vp, i, ctypes.c_char_p], i),
+        "pa_expression_eval": ([ctypes.c_char_p, i, vp, vp, vp, d, vp], i),
         "pa_vector_function_value": ([vp, vp, vp, vp, vp], i),
         "pa_vector_function_gradient_u": ([vp, vp, vp, vp, i, vp], i),
         "pa_vector_function_gradient_u_T": ([vp, vp, vp, vp, i, vp], i),
@@ -358,6 +360,17 @@ def box_side_set(coords, conn, axis, value, tol=1e-12):
     return cells[order].copy(), lnodes[order].reshape(-1).copy()
 
 
+
+def evaluate_function(function, x=None, y=None, z=None, t=0.0):
+    """A 'Function' string at points (x, y, z) and time t, by the library's
+    evaluator (host only, no GPU)."""
+    arrs = [None if a is None else np.ascontiguousarray(a, np.float64).ravel() for a in (x, y, z)]
+    n = max([a.size for a in arrs if a is not None], default=1)
+    out = np.empty(n, np.float64)
+    ptr = [None if a is None else _np_ptr(a) for a in arrs]
+    _check(lib.pa_expression_eval(function.encode(), n, ptr[0], ptr[1], ptr[2], float(t), _np_ptr(out)))
+    return out
+
 class Mesh:
     """Mesh + block-CRS graph on the device (Omega_h mesh + NodeNodeGraph)."""
 
@@ -498,6 +511,11 @@ class VectorFunction:
 
     def add_body_load(self, dof, value):
         _check(lib.pa_vector_function_add_body_load(self.handle, dof, value))
+
+    def add_body_load_function(self, dof, function):
+        """Body load 'Function' of x, y, z on any dof, evaluated at the cell
+        centroids (src/BodyLoads.hpp:54-122)."""
+        _check(lib.pa_vector_function_add_body_load_function(self.handle, dof, function.encode()))
 
     def value(self, u, z, out=None, stream=None):
         u, z = _dev(u), _dev(z)

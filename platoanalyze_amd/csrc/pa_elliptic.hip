@@ -1759,6 +1759,116 @@ void residual_rows(const pa_mesh* m, int dpn, const Material& mt, const double* 
     check_launch();
 }
 
+// ---------------------------------------------------------------------------
+// Body loads given per cell (BodyLoad::get, src/BodyLoads.hpp:54-122, with
+// the function's value f_c at the centroid): a cell adds
+// b_c = -(1/6) |det J| f_c (1/4) rho_c to the dof of each of its nodes.  One
+// thread per row, over the row's incident cells in order: no atomics.
+// ---------------------------------------------------------------------------
+// R[row, dof] += sum over the row's cells of b_c
+template <int DPN>
+__global__ __launch_bounds__(256) void k_body_cells_residual(RowMesh m, const double* __restrict__ z,
+                                                             const double* __restrict__ f, int dof,
+                                                             double* __restrict__ R) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= m.N) return;
+    double r = 0.0;
+    for (int p = m.n2c_off[row]; p < m.n2c_off[row + 1]; ++p) {
+        CellEval c;
+        load_cell(m, z, p, c);
+        r += -1.0 * (c.absdet / 6.0) * f[m.pairs[p].x >> 2] * 0.25 * c.rho;
+    }
+    R[(long)row * DPN + dof] += r;
+}
+
+// gradient_z blocks (row = control node b, column = state node a, 1 x DPN):
+// d b_c / d z_b = -(1/6) |det J| f_c (1/4) (1/4) for each node a of the cell
+template <int DPN>
+__global__ __launch_bounds__(256) void k_body_cells_gradz(RowMesh m, const double* __restrict__ z,
+                                                          const double* __restrict__ f, int dof,
+                                                          double* __restrict__ vals) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= m.N) return;
+    const long rs = m.rowmap[row];
+    for (int p = m.n2c_off[row]; p < m.n2c_off[row + 1]; ++p) {
+        CellEval c;
+        load_cell(m, z, p, c);
+        const double d = -1.0 * (c.absdet / 6.0) * f[m.pairs[p].x >> 2] * 0.25 * 0.25;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) vals[(rs + ((c.slots >> (8 * b)) & 0xff)) * DPN + dof] += d;
+    }
+}
+
+// gradient_x blocks (row = coordinate node a, column = state node, 3 x DPN):
+// d b_c / d x_{a,i} = b_c (grad N_a)_i, f_c held constant per cell as in the
+// reference (its cubature point is mapped in plain doubles)
+template <int DPN>
+__global__ __launch_bounds__(256) void k_body_cells_gradx(RowMesh m, const double* __restrict__ z,
+                                                          const double* __restrict__ f, int dof,
+                                                          double* __restrict__ vals) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= m.N) return;
+    const long rs = m.rowmap[row];
+    for (int p = m.n2c_off[row]; p < m.n2c_off[row + 1]; ++p) {
+        CellEval c;
+        load_cell(m, z, p, c);
+        const double bl = -1.0 * (c.absdet / 6.0) * f[m.pairs[p].x >> 2] * 0.25 * c.rho;
+        double ga[3];
+        pick_row(c.g, c.a, ga);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const long e = (rs + ((c.slots >> (8 * b)) & 0xff)) * 3 * DPN;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) vals[e + i * DPN + dof] += bl * ga[i];
+        }
+    }
+}
+
+// out[b] += sum over the row's cells and their nodes a of d b_c / d z_b * lam_a,
+// lam = lscale * u (the fused self-adjoint criterion gradient's (dR/dz)^T lam)
+template <int DPN>
+__global__ __launch_bounds__(256) void k_body_cells_adjoint(RowMesh m, const double* __restrict__ z,
+                                                            const double* __restrict__ f, int dof,
+                                                            const double* __restrict__ u, double lscale,
+                                                            double* __restrict__ out) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= m.N) return;
+    double r = 0.0;
+    for (int p = m.n2c_off[row]; p < m.n2c_off[row + 1]; ++p) {
+        CellEval c;
+        load_cell(m, z, p, c);
+        const double d = -1.0 * (c.absdet / 6.0) * f[m.pairs[p].x >> 2] * 0.25 * 0.25;
+        double s = 0.0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) s += lscale * u[(long)c.nodes[b] * DPN + dof];
+        r += d * s;
+    }
+    out[row] += r;
+}
+
+enum CellLoadTerm { kLoadResidual, kLoadGradZ, kLoadGradX, kLoadAdjoint };
+
+// Adds the per-cell body loads of vf to the residual, gradient_z, gradient_x
+// or the self-adjoint criterion gradient.
+void cell_loads(const pa_vector_function* vf, CellLoadTerm term, const double* u, const double* z, double* out,
+                hipStream_t st) {
+    const pa_mesh* m = vf->mesh;
+    const int nb = (m->N + 255) / 256;
+    for (const auto& L : vf->cell_loads) {
+        const double* f = L->f.p;
+        auto go = [&](auto dpn) {
+            constexpr int D = decltype(dpn)::value;
+            if (term == kLoadResidual) k_body_cells_residual<D><<<nb, 256, 0, st>>>(row_mesh(m), z, f, L->dof, out);
+            else if (term == kLoadGradZ) k_body_cells_gradz<D><<<nb, 256, 0, st>>>(row_mesh(m), z, f, L->dof, out);
+            else if (term == kLoadGradX) k_body_cells_gradx<D><<<nb, 256, 0, st>>>(row_mesh(m), z, f, L->dof, out);
+            else k_body_cells_adjoint<D><<<nb, 256, 0, st>>>(row_mesh(m), z, f, L->dof, u, -1.0, out);
+        };
+        if (vf->dpn == 3) go(std::integral_constant<int, 3>{});
+        else go(std::integral_constant<int, 1>{});
+        check_launch();
+    }
+}
+
 // Tile-kernel Jacobian; false when the mesh has no tiles or the device
 // cannot give the kernel its LDS.  Self-adjoint operators: the transposed
 // assembly is the direct one.
@@ -1820,7 +1930,10 @@ void jacobian(const pa_vector_function* vf, const double* u, const double* z, VT
         if (vf->dpn == 3) dispatch_jac<3, VT, RES, TR>(vf, u, z, vals, R, st);
         else dispatch_jac<1, VT, RES, TR>(vf, u, z, vals, R, st);
     }
-    if (RES) natural_bcs(vf, R, st);
+    if (RES) {
+        natural_bcs(vf, R, st);
+        cell_loads(vf, kLoadResidual, u, z, R, st);
+    }
 }
 
 __global__ void k_zero(long n, double* __restrict__ v) {
@@ -1855,10 +1968,13 @@ void residual(const pa_vector_function* vf, const double* u, const double* z, do
     const Material mt = material_of(vf->physics, vf->material, vf->penalty);
     const double body = vf->dpn == 3 ? 0.0 : (vf->has_body ? vf->body_value : 0.0);
     // on grid and tile meshes the natural BCs' load is added as the rows are written
-    if (m->has_grid && grid_residual(m, vf->dpn, mt, u, z, body, 1.0, natural_load(vf), R, st)) return;
-    if (m->has_tiles && tile_residual(m, vf->dpn, mt, u, z, body, 1.0, natural_load(vf), R, st)) return;
-    residual_rows(m, vf->dpn, mt, u, z, body, 1.0, R, st);
-    natural_bcs(vf, R, st);
+    if (m->has_grid && grid_residual(m, vf->dpn, mt, u, z, body, 1.0, natural_load(vf), R, st)) {
+    } else if (m->has_tiles && tile_residual(m, vf->dpn, mt, u, z, body, 1.0, natural_load(vf), R, st)) {
+    } else {
+        residual_rows(m, vf->dpn, mt, u, z, body, 1.0, R, st);
+        natural_bcs(vf, R, st);
+    }
+    cell_loads(vf, kLoadResidual, u, z, R, st);
 }
 
 void check_vf(const pa_vector_function* vf, const void* a, const void* b, const void* c) {
@@ -1901,6 +2017,7 @@ void self_adjoint_gradient(const pa_vector_function* vf, const pa_scalar_functio
                                                                                   body, out);
     }
     check_launch();
+    cell_loads(vf, kLoadAdjoint, u, z, out, st);
 }
 
 void apply_block_constraints(const pa_mesh* m, int dpn, void* vals, int precision, double* rhs, int nbc,
@@ -2002,6 +2119,34 @@ int pa_vector_function_add_body_load(pa_vector_function* vf, int dof, double val
     });
 }
 
+int pa_vector_function_add_body_load_function(pa_vector_function* vf, int dof, const char* function) {
+    return guarded([&] {
+        PA_REQUIRE(vf && function, PA_ERR_ARG, "pa_vector_function_add_body_load_function: null argument");
+        PA_REQUIRE(dof >= 0 && dof < vf->dpn, PA_ERR_ARG, "pa_vector_function_add_body_load_function: dof out of range");
+        const pa::Expr f(function);
+        const pa_mesh* m = vf->mesh;
+        // the function at each cell's centroid, the one-point cubature of Tet4
+        std::vector<double> xyz((size_t)m->N * 3), fc((size_t)m->C);
+        std::vector<int> conn((size_t)m->C * 4);
+        PA_HIP(hipDeviceSynchronize());
+        PA_HIP(hipMemcpy(xyz.data(), m->d_coords.p, xyz.size() * sizeof(double), hipMemcpyDeviceToHost));
+        PA_HIP(hipMemcpy(conn.data(), m->d_conn.p, conn.size() * sizeof(int), hipMemcpyDeviceToHost));
+        for (int c = 0; c < m->C; ++c) {
+            double x[3] = {0.0, 0.0, 0.0};
+            for (int d = 0; d < 3; ++d) {
+                for (int n = 0; n < 4; ++n) x[d] += xyz[(size_t)conn[(size_t)c * 4 + n] * 3 + d];
+                x[d] *= 0.25;
+            }
+            fc[c] = f.eval(x[0], x[1], x[2], 0.0);
+        }
+        auto L = std::make_unique<pa_vector_function::CellLoad>();
+        L->dof = dof;
+        L->function = function;
+        L->f.upload(fc.data(), fc.size());
+        vf->cell_loads.push_back(std::move(L));
+    });
+}
+
 int pa_vector_function_value(const pa_vector_function* vf, const double* u, const double* z, double* R,
                              void* stream) {
     return guarded([&] {
@@ -2085,6 +2230,7 @@ int pa_vector_function_gradient_z(const pa_vector_function* vf, const double* u,
                 pa::k_gradz_rows<1, 32, 64><<<nb32, pa::kBlock, 0, st>>>(pa::row_mesh(m), mt, u, z, bd, vals);
         }
         pa::check_launch();
+        pa::cell_loads(vf, pa::kLoadGradZ, u, z, vals, st);
     });
 }
 
@@ -2130,6 +2276,7 @@ int pa_vector_function_gradient_x(const pa_vector_function* vf, const double* u,
                     m->d_rowmap.p, m->d_colidx.p, bc.values[0], 0.0, 0.0, scale, vals);
             pa::check_launch();
         }
+        pa::cell_loads(vf, pa::kLoadGradX, u, z, vals, st);
     });
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -19,6 +20,24 @@ struct Error : std::runtime_error {
 };
 
 void set_last_error(const std::string& msg);
+
+// A 'Function' string compiled once (pa_expr.cpp): numbers, x y z t, pi,
+// + - * / ^, parentheses, the common functions, leading 'name = expr;'.
+class Expr {
+  public:
+    struct Inst {
+        int op, i;
+        double v;
+    };
+    explicit Expr(const std::string& text);
+    double eval(double x, double y, double z, double t) const;
+    const std::string& text() const { return text_; }
+
+  private:
+    std::string text_;
+    std::vector<Inst> prog_;
+    int nslots_ = 0, depth_ = 0;
+};
 
 #define PA_HIP(call)                                                                              \
     do {                                                                                          \
@@ -175,6 +194,14 @@ struct pa_vector_function {
     bool has_body = false;
     int body_dof = 0;
     double body_value = 0.0;
+    // body loads given per cell: the function's value at each cell's
+    // centroid (the one-point cubature of Tet4), for any dof
+    struct CellLoad {
+        int dof = 0;
+        std::string function;
+        pa::DevBuf<double> f;  // C values
+    };
+    std::vector<std::unique_ptr<CellLoad>> cell_loads;
     // the natural BCs' nodal load (it depends on the coordinates and the
     // load values only), built on first use: valid for fext_nbc loads and
     // the mesh's side sets at fext_version

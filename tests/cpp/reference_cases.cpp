@@ -571,9 +571,10 @@ int parser_checks() {
     expect(pa::Expression("-0.001*t")(1.0) == -0.001, "-a*t");
     expect(std::fabs(pa::Expression("2*(t+1)^2 - 3/4")(1.0) - 7.25) < 1e-15, "precedence");
     expect(pa::Expression("1.5e-3")(0.3) == 1.5e-3, "constant");
+    expect(std::fabs(pa::Expression("a = 2; a*sin(pi*t/2)")(1.0) - 2.0) < 1e-15, "assignment and functions");
     threw = false;
     try {
-        pa::Expression("x*t")(1.0);
+        pa::Expression("q*t")(1.0);
     } catch (const std::runtime_error&) {
         threw = true;
     }
