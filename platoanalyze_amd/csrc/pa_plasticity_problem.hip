@@ -46,6 +46,7 @@ struct pa_plasticity_problem {
     // adjoint workspace
     pa::DevBuf<double> wu, wu_next, gc, gcp, gcp_next, zc, zc_out, blocal, mu, mu_next, lam, gam, gam_next;
     std::vector<int> newton_its, linear_its;
+    long reorth = 0;  // Gram-Schmidt passes repeated (all GMRES solves)
     // outcome of the last Krylov solve
     bool lin_converged = true;
     double lin_rel = 0.0;
@@ -427,20 +428,44 @@ int bicgstab(pa_plasticity_problem* P, const double* b, double* x, hipStream_t s
 
 // ---------------------------------------------------------------------------
 // flexible restarted GMRES (right preconditioning; the FP32 multigrid cycle
-// need not be an exactly linear operator), classical Gram-Schmidt with one
-// re-orthogonalization; the (m+1) x m Hessenberg least-squares problem and
-// its Givens rotations live on the host
+// need not be an exactly linear operator), classical Gram-Schmidt repeated
+// once when the first pass loses too much of the vector's norm; the
+// (m+1) x m Hessenberg least-squares problem and its Givens rotations live on
+// the host
 // ---------------------------------------------------------------------------
-// part[i*nparts + block] = partial sums of w . V_i, i < k (V_i at V + i*n)
-__global__ __launch_bounds__(kB) void k_mdot(long n, int k, const double* __restrict__ V,
+// part[i*nparts + block] = partial sums of w . V_i, i < k (V_i at V + i*n);
+// with self, row k holds the partial sums of w . w.  Four rows share each
+// block reduction.
+__global__ __launch_bounds__(kB) void k_mdot(long n, int k, int self, const double* __restrict__ V,
                                              const double* __restrict__ w, double* __restrict__ part, int nparts) {
-    __shared__ double sh[kB / 64];
-    for (int i = 0; i < k; ++i) {
-        const double* vi = V + (long)i * n;
-        double s = 0.0;
-        for (long t = (long)blockIdx.x * kB + threadIdx.x; t < n; t += (long)gridDim.x * kB) s += w[t] * vi[t];
-        const double tt = block_sum(s, sh);
-        if (threadIdx.x == 0) part[(long)i * nparts + blockIdx.x] = tt;
+    constexpr int Q = 4;
+    __shared__ double sh[Q][kB / 64];
+    const int rows = k + (self ? 1 : 0);
+    for (int i0 = 0; i0 < rows; i0 += Q) {
+        double s[Q] = {};
+        for (long t = (long)blockIdx.x * kB + threadIdx.x; t < n; t += (long)gridDim.x * kB) {
+            const double wt = w[t];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int i = i0 + q;
+                if (i < k) s[q] += wt * V[(long)i * n + t];
+                else if (i == k && self) s[q] += wt * wt;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            double v = s[q];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+            if (threadIdx.x % 64 == 0) sh[q][threadIdx.x / 64] = v;
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < Q && i0 + (int)threadIdx.x < rows) {
+            double tt = 0.0;
+            for (int wv = 0; wv < kB / 64; ++wv) tt += sh[threadIdx.x][wv];
+            part[(long)(i0 + threadIdx.x) * nparts + blockIdx.x] = tt;
+        }
+        __syncthreads();
     }
 }
 
@@ -455,14 +480,20 @@ __global__ void k_reduce_rows(const double* __restrict__ part, int nb, int npart
     if (threadIdx.x == 0) h[i] = accumulate ? h[i] + tt : tt;
 }
 
-// w -= sum_{i<k} h[i] V_i
+// w -= sum_{i<k} h[i] V_i; part[block] = partial sums of the new w . w
 __global__ __launch_bounds__(kB) void k_mupdate(long n, int k, const double* __restrict__ V,
-                                                const double* __restrict__ h, double* __restrict__ w) {
+                                                const double* __restrict__ h, double* __restrict__ w,
+                                                double* __restrict__ part) {
+    __shared__ double sh[kB / 64];
+    double q = 0.0;
     for (long t = (long)blockIdx.x * kB + threadIdx.x; t < n; t += (long)gridDim.x * kB) {
         double s = w[t];
         for (int i = 0; i < k; ++i) s -= h[i] * V[(long)i * n + t];
         w[t] = s;
+        q += s * s;
     }
+    const double tt = block_sum(q, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = tt;
 }
 
 // x += sum_{i<k} y[i] Z_i
@@ -534,7 +565,7 @@ int gmres(pa_plasticity_problem* P, const double* b, double* x, hipStream_t st) 
     const double target = P->opt.linear_tol * bnorm;
     P->lin_converged = false;
     P->lin_rel = 1.0;
-    std::vector<double> H((M + 1) * M), cs(M), sn(M), g(M + 1), y(M), hcol(M + 1);
+    std::vector<double> H((M + 1) * M), cs(M), sn(M), g(M + 1), y(M), hcol(M + 1), h1(M + 3), h2(M + 3);
     int it = 0;
     while (it < P->opt.linear_max_iter) {
         // r = b - A x into V_0
@@ -564,20 +595,42 @@ int gmres(pa_plasticity_problem* P, const double* b, double* x, hipStream_t st) 
             spmv<<<nsb, kB, 0, st>>>(N, m->d_rowmap.p, m->d_colidx.p, P->vals.p, zj, w, nullptr, nullptr,
                                      P->part.p, np);
             check();
-            // two passes of classical Gram-Schmidt against V_0..V_j
-            for (int pass = 0; pass < 2; ++pass) {
-                k_mdot<<<nvb, kB, 0, st>>>(n, j + 1, V, w, P->gpart.p, nvb);
+            // classical Gram-Schmidt against V_0..V_j: h = V^T w and |w|^2 in
+            // one pass over the basis, w -= V h and |w|^2 in another.  The
+            // pair is repeated when the new norm is below 1/sqrt(2) of the
+            // old (the criterion of Daniel, Gragg, Kaufman and Stewart,
+            // Math. Comp. 30 (1976)); one repetition restores orthogonality
+            // to working precision.
+            double* g1 = P->gh.p;
+            double* g2 = P->gh.p + (M + 3);
+            k_mdot<<<nvb, kB, 0, st>>>(n, j + 1, 1, V, w, P->gpart.p, nvb);
+            check();
+            k_reduce_rows<<<j + 2, 1024, 0, st>>>(P->gpart.p, nvb, nvb, 0, g1);  // h, |w|^2
+            check();
+            k_mupdate<<<nvb, kB, 0, st>>>(n, j + 1, V, g1, w, P->part.p);
+            check();
+            k_reduce_rows<<<1, 1024, 0, st>>>(P->part.p, nvb, nvb, 0, g1 + j + 2);  // |w - V h|^2
+            check();
+            PA_HIP(hipMemcpyAsync(h1.data(), g1, (j + 3) * sizeof(double), hipMemcpyDeviceToHost, st));
+            PA_HIP(hipStreamSynchronize(st));
+            double hn2 = h1[j + 2];
+            for (int i = 0; i <= j; ++i) hcol[i] = h1[i];
+            if (!(hn2 > 0.5 * h1[j + 1])) {
+                k_mdot<<<nvb, kB, 0, st>>>(n, j + 1, 0, V, w, P->gpart.p, nvb);
                 check();
-                k_reduce_rows<<<j + 1, 1024, 0, st>>>(P->gpart.p, nvb, nvb, 0, P->gh.p + (pass ? M + 1 : 0));
+                k_reduce_rows<<<j + 1, 1024, 0, st>>>(P->gpart.p, nvb, nvb, 0, g2);
                 check();
-                k_mupdate<<<nvb, kB, 0, st>>>(n, j + 1, V, P->gh.p + (pass ? M + 1 : 0), w);
+                k_mupdate<<<nvb, kB, 0, st>>>(n, j + 1, V, g2, w, P->part.p);
                 check();
+                k_reduce_rows<<<1, 1024, 0, st>>>(P->part.p, nvb, nvb, 0, g2 + j + 1);
+                check();
+                PA_HIP(hipMemcpyAsync(h2.data(), g2, (j + 2) * sizeof(double), hipMemcpyDeviceToHost, st));
+                PA_HIP(hipStreamSynchronize(st));
+                for (int i = 0; i <= j; ++i) hcol[i] += h2[i];
+                hn2 = h2[j + 1];
+                ++P->reorth;
             }
-            const double hn = norm(P, w, st);  // synchronizes
-            std::vector<double> h1(j + 1), h2(j + 1);
-            PA_HIP(hipMemcpy(h1.data(), P->gh.p, (j + 1) * sizeof(double), hipMemcpyDeviceToHost));
-            PA_HIP(hipMemcpy(h2.data(), P->gh.p + M + 1, (j + 1) * sizeof(double), hipMemcpyDeviceToHost));
-            for (int i = 0; i <= j; ++i) hcol[i] = h1[i] + h2[i];
+            const double hn = std::sqrt(hn2);
             hcol[j + 1] = hn;
             if (!std::isfinite(hn)) return it;
             if (hn > 0.0) {
@@ -717,7 +770,9 @@ bool forward_step(pa_plasticity_problem* P, int k, const double* z, bool record,
         PA_HIP(hipMemsetAsync(P->du.p, 0, nu * sizeof(double), st));
         const int li = linear_solve(P, P->rhs.p, P->du.p, st);
         lin += li;
-        if (P->verbose) std::fprintf(stderr, "plasticity: step %d newton %d |r| %.6e linear %d\n", k, it, nrm, li);
+        if (P->verbose)
+            std::fprintf(stderr, "plasticity: step %d newton %d |r| %.6e linear %d (Gram-Schmidt repeated %ld times so far)\n",
+                         k, it, nrm, li, P->reorth);
         k_axpy<<<vec_blocks(P), kB, 0, st>>>(nu, 1.0, P->du.p, uk);
         check();
         update_local_state(P->p, uk, cp, z, ck, st);
@@ -917,7 +972,7 @@ int pa_plasticity_problem_create(pa_plasticity* p, int nbc, const int* dofs, con
                 P->gV.alloc((M + 1) * nu);
                 P->gZ.alloc(M * nu);
                 P->gpart.alloc((M + 1) * pa::j2::vec_blocks(P));
-                P->gh.alloc(2 * (M + 1));
+                P->gh.alloc(2 * (M + 3));
                 P->gy.alloc(M);
             }
             const int np = std::max(pa::j2::vec_blocks(P), pa::j2::row_blocks(P));
