@@ -749,7 +749,15 @@ __global__ __launch_bounds__(W * 64, 2) void k_grid_residual(GridArgs g, Materia
 // atomics: deterministic.  Measured on the 128 x 64 x 64 grid: bricks of
 // 2 x 3 x 4 rows (4 workgroups per CU) 0.24 ms, 2 x 4 x 4 0.21 ms, 2 x 6 x 4
 // 0.19 ms, 4 x 4 x 4 (2 workgroups, 8 waves per SIMD) 0.19 ms; 8 lanes per
-// row (two blocks each) 0.21 ms.
+// row (two blocks each) 0.21 ms.  With the per-lane tables below (no index
+// arithmetic) 0.179 ms FP64, 0.139 ms FP32.  The FP64 kernel is bound by its
+// load-stage-store structure, not by arithmetic: with the tet and entry work
+// skipped it still takes 0.175 ms (0.144 ms without the node-halo loads),
+// while a plain fill of the 577 MB writes in 0.092 ms.  Measured slower:
+// persistent workgroups that prefetch the next brick's halo into registers
+// (0.221 ms, 64 VGPRs), an L2 prefetch of a later brick's halo (0.193 ms),
+// FP64 blocks stored directly by each lane (0.255 ms), the diagonal sums
+// exchanged through ds_bpermute instead of DPP (0.226 ms).
 constexpr int kJ3Slots = 16;  // lanes per row: the 15 CRS slots and one idle
 constexpr int kJ3MaxEnt = 6;  // tets shared by a node and a neighbour
 
@@ -797,33 +805,6 @@ static_assert(j3_count(0) == 6 && j3_count(1) == 4 && j3_count(2) == 4 && j3_cou
 static_assert(j3_count(4) == 4 && j3_count(5) == 6 && j3_count(6) == 6 && j3_count(7) == 0, "Kuhn edge tets");
 static_assert(j3_count(8) == 6 && j3_count(14) == 6 && j3_count(15) == 0, "Kuhn edge tets");
 
-// LDS offsets (doubles) of u_a and u_b of every entry, relative to the tet
-// data of the row's own hex (the hex with the row node as corner 0), packed as
-// two int16; kJ3Pad marks the unused entries, which read a zero vector.
-constexpr int kJ3Pad = 0x7fff;
-struct alignas(16) J3RelTable {
-    int v[kJ3Slots][8];
-};
-template <int HY, int HZ, int HS>
-__host__ __device__ constexpr J3RelTable make_j3_rel() {
-    constexpr J3Table T = make_j3_table();
-    J3RelTable R{};
-    for (int s = 0; s < kJ3Slots; ++s)
-        for (int e = 0; e < 8; ++e) {
-            if (e >= T.n[s]) {
-                R.v[s][e] = (int)((unsigned)kJ3Pad | (unsigned)kJ3Pad << 16);
-                continue;
-            }
-            const J3Entry E = T.e[s][e];
-            const int h = (E.hx * HY + E.hy) * HZ + E.hz;
-            const int ra = h * HS + E.t * 12 + 3 * E.a, rb = h * HS + E.t * 12 + 3 * E.b;
-            R.v[s][e] = (int)(((unsigned)ra & 0xffffu) | ((unsigned)rb << 16));
-        }
-    return R;
-}
-template <int HY, int HZ, int HS>
-__constant__ J3RelTable kJ3Rel = make_j3_rel<HY, HZ, HS>();
-
 // v from the lane given by a DPP row control (both halves of the double).
 template <int CTRL>
 __device__ inline double dpp_f64(double v) {
@@ -853,9 +834,72 @@ struct J3Brick {
     static constexpr int line_blocks = BZ * 15;  // staged blocks per line
 };
 
-// Corner v of tet t = (0, p, q, 7): p and q packed one nibble per tet.
-constexpr unsigned kTetP = 0x451326u, kTetQ = 0x645132u;
-static_assert(((kTetP >> 0) & 15) == 6 && ((kTetP >> 20) & 15) == 4 && ((kTetQ >> 12) & 15) == 5, "tet corners");
+// Per-lane tables of a brick, resolved at compile time so that the kernel
+// spends no instructions on index arithmetic:
+// - entries: for thread tid (row tid / 16, slot tid % 16) the LDS offsets
+//   (doubles) of u_a and u_b of each of its six entries, packed as two
+//   uint16; unused entries read the zero vector after the tet data;
+// - tets: for thread tid < tets, the tet it evaluates (tet tid % 6 of hex
+//   tid / 6): the LDS node index of the hex's corner 0 (bits 0-7), the node
+//   offsets of the tet's corners p (8-13) and q (14-19), and the hex's
+//   position in the brick (3 bits each from bit 20).
+template <int BX, int BY, int BZ>
+struct J3LaneTables {
+    uint2 ent[J3Brick<BX, BY, BZ>::threads][3];
+    unsigned tet[J3Brick<BX, BY, BZ>::threads];
+};
+template <int BX, int BY, int BZ>
+__host__ __device__ constexpr J3LaneTables<BX, BY, BZ> make_j3_lanes() {
+    using B = J3Brick<BX, BY, BZ>;
+    constexpr J3Table T = make_j3_table();
+    constexpr int zero = B::tet_doubles;
+    static_assert(zero + 3 < 65536, "16-bit LDS offsets");
+    J3LaneTables<BX, BY, BZ> L{};
+    for (int tid = 0; tid < B::threads; ++tid) {
+        const int slot = tid % kJ3Slots, r = tid / kJ3Slots;
+        const int rc = r % BZ, rb = (r / BZ) % BY, ra = r / (BZ * BY);
+        const int rowbase = (((ra + 1) * B::HY + (rb + 1)) * B::HZ + (rc + 1)) * B::hex_stride;
+        unsigned w[6] = {};
+        for (int e = 0; e < kJ3MaxEnt; ++e) {
+            int oa = zero, ob = zero;
+            if (e < T.n[slot]) {
+                const J3Entry E = T.e[slot][e];
+                const int h = (E.hx * B::HY + E.hy) * B::HZ + E.hz;
+                oa = rowbase + h * B::hex_stride + E.t * 12 + 3 * E.a;
+                ob = rowbase + h * B::hex_stride + E.t * 12 + 3 * E.b;
+            }
+            w[e] = (unsigned)oa | (unsigned)ob << 16;
+        }
+        for (int k = 0; k < 3; ++k) L.ent[tid][k] = uint2{w[2 * k], w[2 * k + 1]};
+        L.tet[tid] = 0;
+        if (tid < B::tets) {
+            const int h = tid / 6, t = tid % 6;
+            const int hc = h % B::HZ, hb = (h / B::HZ) % B::HY, ha = h / (B::HZ * B::HY);
+            const int n0 = (ha * B::NY + hb) * B::NZ + hc;
+            auto at = [](int v) { return ((v >> 2) * B::NY + ((v >> 1) & 1)) * B::NZ + (v & 1); };
+            L.tet[tid] = (unsigned)n0 | (unsigned)at(tet_p(t)) << 8 | (unsigned)at(tet_q(t)) << 14 |
+                         (unsigned)ha << 20 | (unsigned)hb << 23 | (unsigned)hc << 26;
+        }
+    }
+    return L;
+}
+template <int BX, int BY, int BZ>
+__constant__ J3LaneTables<BX, BY, BZ> kJ3Lanes = make_j3_lanes<BX, BY, BZ>();
+
+// sqrt(s) for a positive normal s: the hardware reciprocal square root
+// estimate refined by a Goldschmidt step and two Newton corrections of the
+// root (within an ulp, without the range scaling a general sqrt needs).
+__device__ inline double sqrt_pos(double s) {
+    const double y = __builtin_amdgcn_rsq(s);
+    double g = s * y, h = 0.5 * y;
+    const double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double d = fma(-g, g, s);
+    g = fma(d, h, g);
+    d = fma(-g, g, s);
+    return fma(d, h, g);
+}
 
 // (launch bound: 8 waves per SIMD, two 16-wave workgroups per CU)
 template <class VT, int BX, int BY, int BZ, int PK>
@@ -873,23 +917,14 @@ __global__ __launch_bounds__(BX * BY * BZ * kJ3Slots) void k_grid_jacobian3(Grid
     const int nbz = (g.d2 + BZ - 1) / BZ, nby = (g.d1 + BY - 1) / BY;
     const int bk = xcd_block();
     const int X0 = (bk / (nbz * nby)) * BX, Y0 = ((bk / nbz) % nby) * BY, Z0 = (bk % nbz) * BZ;
-    // this lane's row and entry offsets (the table is fixed per slot): LDS
-    // offsets of u_a (low 16 bits) and u_b (high 16 bits)
-    static_assert(kZero + 4 < 65536, "16-bit LDS offsets");
+    // this lane's row and its entries' LDS offsets of u_a (low 16 bits) and
+    // u_b (high 16 bits)
     const int slot = tid & (kJ3Slots - 1), r = tid / kJ3Slots;
     const int rc = r % BZ, rb = (r / BZ) % BY, ra = r / (BZ * BY);
-    const int rowbase = (((ra + 1) * B::HY + (rb + 1)) * B::HZ + (rc + 1)) * B::hex_stride;
-    const int4* trow = reinterpret_cast<const int4*>(kJ3Rel<B::HY, B::HZ, B::hex_stride>.v[slot]);
-    const int4 w0 = trow[0], w1 = trow[1];
-    const int wv6[kJ3MaxEnt] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y};
-    unsigned off[kJ3MaxEnt];
-#pragma unroll
-    for (int e = 0; e < kJ3MaxEnt; ++e) {
-        const int ea = (int)(short)(wv6[e] & 0xffff), eb = wv6[e] >> 16;
-        const unsigned oa = ea == kJ3Pad ? kZero : rowbase + ea;
-        const unsigned ob = eb == kJ3Pad ? kZero : rowbase + eb;
-        off[e] = oa | ob << 16;
-    }
+    const uint2 w0 = kJ3Lanes<BX, BY, BZ>.ent[tid][0], w1 = kJ3Lanes<BX, BY, BZ>.ent[tid][1],
+                w2 = kJ3Lanes<BX, BY, BZ>.ent[tid][2];
+    const unsigned off[kJ3MaxEnt] = {w0.x, w0.y, w1.x, w1.y, w2.x, w2.y};
+    const unsigned tw = kJ3Lanes<BX, BY, BZ>.tet[tid];
     // stage 0: node halo, row offsets, line ranges, the zero vector
     if (tid < B::nodes) {
         const int nc = tid % B::NZ, nb = (tid / B::NZ) % B::NY, na = tid / (B::NZ * B::NY);
@@ -922,18 +957,16 @@ __global__ __launch_bounds__(BX * BY * BZ * kJ3Slots) void k_grid_jacobian3(Grid
         tet[kZero + tid - (B::nodes + B::rows + B::lines)] = 0.0;
     }
     __syncthreads();
-    // stage 1: the tets, one per thread at a time
-    for (int T = tid; T < B::tets; T += B::threads) {
-        const int h = T / 6, t = T - 6 * (T / 6);
-        const int hc = h % B::HZ, hb = (h / B::HZ) % B::HY, ha = h / (B::HZ * B::HY);
+    // stage 1: the tets, one per thread (tet tid % 6 of hex tid / 6)
+    static_assert(B::tets <= B::threads, "one tet per thread");
+    if (tid < B::tets) {
+        const int ha = (tw >> 20) & 7, hb = (tw >> 23) & 7, hc = (tw >> 26) & 7;
         const int i = X0 - 1 + ha, j = Y0 - 1 + hb, k = Z0 - 1 + hc;
         // branch-free: a hex outside the grid (its corners read as zeros or
         // as nodes of the grid) gets sqrt(s) = 0
         const bool in = i >= 0 && i < g.d0 - 1 && j >= 0 && j < g.d1 - 1 && k >= 0 && k < g.d2 - 1;
-        const int n0 = (ha * B::NY + hb) * B::NZ + hc;
-        const int p = (kTetP >> (4 * t)) & 7u, q = (kTetQ >> (4 * t)) & 7u;
-        auto at = [](int v) { return ((v >> 2) * B::NY + ((v >> 1) & 1)) * B::NZ + (v & 1); };
-        const double4 x0 = node[n0], xp = node[n0 + at(p)], xq = node[n0 + at(q)];
+        const int n0 = tw & 255;
+        const double4 x0 = node[n0], xp = node[n0 + ((tw >> 8) & 63)], xq = node[n0 + ((tw >> 14) & 63)];
         const double4 x7 = node[n0 + (B::NY + 1) * B::NZ + 1];
         const double e0[3] = {x0.x - x7.x, x0.y - x7.y, x0.z - x7.z};
         const double e1[3] = {xp.x - x7.x, xp.y - x7.y, xp.z - x7.z};
@@ -945,7 +978,7 @@ __global__ __launch_bounds__(BX * BY * BZ * kJ3Slots) void k_grid_jacobian3(Grid
         const double det = dot3(e0, u[0]);
         const double rho = (((x0.w + xp.w) + xq.w) + x7.w) * 0.25;
         const double s = penal_k<PK>(rho, mt.p, mt.emin) * recip(6.0 * fabs(det));
-        const double sw = in ? sqrt(s) : 0.0;
+        const double sw = in ? sqrt_pos(s) : 0.0;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             u[3][d] = -((u[0][d] + u[1][d]) + u[2][d]) * sw;
@@ -953,7 +986,7 @@ __global__ __launch_bounds__(BX * BY * BZ * kJ3Slots) void k_grid_jacobian3(Grid
             u[1][d] *= sw;
             u[2][d] *= sw;
         }
-        double2* dst = reinterpret_cast<double2*>(tet + h * B::hex_stride + t * 12);
+        double2* dst = reinterpret_cast<double2*>(tet + tid * 12);
 #pragma unroll
         for (int a = 0; a < 4; a += 2) {
             dst[3 * (a / 2) + 0] = make_double2(u[a][0], u[a][1]);
