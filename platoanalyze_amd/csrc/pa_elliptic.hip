@@ -2187,7 +2187,21 @@ int pa_vector_function_value_gradient_u(const pa_vector_function* vf, const doub
         pa::check_vf(vf, u, z, vals);
         PA_REQUIRE(R, PA_ERR_ARG, "value_gradient_u: null residual");
         PA_REQUIRE(precision == PA_FP64 || precision == PA_FP32, PA_ERR_ARG, "value_gradient_u: unknown precision");
-        if (vf->mesh->has_tiles || vf->mesh->has_grid) {
+        const pa_mesh* m = vf->mesh;
+        if (vf->dpn == 3 && m->has_grid) {
+            // elastic Kuhn grids: the residual from the blocks in the Jacobian
+            // kernel (R = K u + f), then the body-load functions
+            const pa::Material mt = pa::material_of(vf->physics, vf->material, vf->penalty);
+            const double* f = pa::natural_load(vf);
+            const bool done = precision == PA_FP64
+                                  ? pa::grid_value_jacobian3(m, mt, u, z, f, R, (double*)vals, (hipStream_t)stream)
+                                  : pa::grid_value_jacobian3(m, mt, u, z, f, R, (float*)vals, (hipStream_t)stream);
+            if (done) {
+                pa::cell_loads(vf, pa::kLoadResidual, u, z, R, (hipStream_t)stream);
+                return;
+            }
+        }
+        if (m->has_tiles || m->has_grid) {
             // on grid and tile meshes the residual kernel and the Jacobian
             // kernel one after the other beat one fused pass (whose cell data
             // halves the workgroups per CU)

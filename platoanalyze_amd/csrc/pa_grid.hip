@@ -901,11 +901,20 @@ __device__ inline double sqrt_pos(double s) {
     return fma(d, h, g);
 }
 
+// Residual with the Jacobian (WR): linear elastostatics has R(u) = K u + f,
+// so the lanes of a row multiply their blocks by the neighbours'
+// displacements (the diagonal lane adds the natural-BC load f) and, after
+// the stores are issued, sum the products over the row's 16 lanes by DPP
+// (fixed order); lane 7 writes the row of R.  This costs a few instructions
+// per lane instead of a second pass over the tets.
 // (launch bound: 8 waves per SIMD, two 16-wave workgroups per CU)
-template <class VT, int BX, int BY, int BZ, int PK>
+template <class VT, int BX, int BY, int BZ, int PK, bool WR>
 __global__ __launch_bounds__(BX * BY * BZ * kJ3Slots) void k_grid_jacobian3(GridArgs g, Material mt,
                                                                              const int* __restrict__ rowmap,
-                                                                             VT* __restrict__ vals) {
+                                                                             VT* __restrict__ vals,
+                                                                             const double* __restrict__ u,
+                                                                             const double* __restrict__ fext,
+                                                                             double* __restrict__ R) {
     using B = J3Brick<BX, BY, BZ>;
     static_assert(B::nodes + B::rows + B::lines + 4 <= B::threads, "one load per thread");
     static_assert(B::lines * B::line_blocks * 9 * sizeof(VT) <= B::tet_doubles * sizeof(double), "stage fits");
@@ -925,6 +934,7 @@ __global__ __launch_bounds__(BX * BY * BZ * kJ3Slots) void k_grid_jacobian3(Grid
                 w2 = kJ3Lanes<BX, BY, BZ>.ent[tid][2];
     const unsigned off[kJ3MaxEnt] = {w0.x, w0.y, w1.x, w1.y, w2.x, w2.y};
     const unsigned tw = kJ3Lanes<BX, BY, BZ>.tet[tid];
+    const int ri = X0 + ra, rj = Y0 + rb, rk = Z0 + rc;
     // stage 0: node halo, row offsets, line ranges, the zero vector
     if (tid < B::nodes) {
         const int nc = tid % B::NZ, nb = (tid / B::NZ) % B::NY, na = tid / (B::NZ * B::NY);
@@ -995,6 +1005,23 @@ __global__ __launch_bounds__(BX * BY * BZ * kJ3Slots) void k_grid_jacobian3(Grid
         }
     }
     __syncthreads();
+    // WR: the displacement of the lane's neighbour and, on the diagonal lane,
+    // the row's load (in flight during stage 2)
+    double un[3] = {0.0, 0.0, 0.0}, fr[3] = {0.0, 0.0, 0.0};
+    if constexpr (WR) {
+        const bool in = ri < g.d0 && rj < g.d1 && rk < g.d2;
+        if (in && slot < 15 && ((row_exists(g, ri, rj, rk) >> slot) & 1u)) {
+            const long n = ((long)(ri + nb_di(slot)) * g.d1 + (rj + nb_dj(slot))) * g.d2 + (rk + nb_dk(slot));
+            un[0] = u[3 * n];
+            un[1] = u[3 * n + 1];
+            un[2] = u[3 * n + 2];
+            if (slot == kDiagSlot && fext) {
+                fr[0] = fext[3 * n];
+                fr[1] = fext[3 * n + 1];
+                fr[2] = fext[3 * n + 2];
+            }
+        }
+    }
     // stage 2: the lane's block from its (padded) entries
     double A[9];
 #pragma unroll
@@ -1029,7 +1056,12 @@ __global__ __launch_bounds__(BX * BY * BZ * kJ3Slots) void k_grid_jacobian3(Grid
         K[2] = K[6] = -0.5 * S02;
         K[5] = K[7] = -0.5 * S12;
     }
-    const int ri = X0 + ra, rj = Y0 + rb, rk = Z0 + rc;
+    // WR: the lane's part of row m of K u + f (the load on the diagonal lane)
+    double q[3] = {0.0, 0.0, 0.0};
+    if constexpr (WR) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) q[i] = fma(K[3 * i + 2], un[2], fma(K[3 * i + 1], un[1], fma(K[3 * i], un[0], fr[i])));
+    }
     if constexpr (sizeof(VT) == 4) {
         // FP32 values: each lane stores its 36-byte block where it belongs (a
         // row's blocks are contiguous in CRS, a wave's rows consecutive);
@@ -1071,6 +1103,17 @@ __global__ __launch_bounds__(BX * BY * BZ * kJ3Slots) void k_grid_jacobian3(Grid
             for (int e = h + 2 * lane; e + 1 < n; e += 128)
                 *reinterpret_cast<double2*>(dstg + e) = make_double2(src[e], src[e + 1]);
             if (lane == 0 && n > h && ((n - h) & 1)) dstg[n - 1] = src[n - 1];
+        }
+    }
+    if constexpr (WR) {
+        // the row of R = K u + f, summed over the row's lanes while the
+        // stores drain
+#pragma unroll
+        for (int i = 0; i < 3; ++i) q[i] = row16_sum(q[i]);
+        if (slot == kDiagSlot && ri < g.d0 && rj < g.d1 && rk < g.d2) {
+            const long o = 3 * (((long)ri * g.d1 + rj) * g.d2 + rk);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) R[o + i] = q[i];
         }
     }
 }
@@ -1146,24 +1189,30 @@ void grid_jacobian1_w(const pa_mesh* m, const Material& mt, const double* z, VT*
 }
 
 template <class VT, int BX, int BY, int BZ>
-void grid_jacobian3_brick(const pa_mesh* m, const GridArgs& g, const Material& mt, VT* vals, hipStream_t st) {
+void grid_jacobian3_brick(const pa_mesh* m, const GridArgs& g, const Material& mt, VT* vals, const double* u,
+                          const double* fext, double* R, hipStream_t st) {
     const long nb = (long)((g.d0 + BX - 1) / BX) * ((g.d1 + BY - 1) / BY) * ((g.d2 + BZ - 1) / BZ);
     PA_REQUIRE(nb < (1L << 31), PA_ERR_LIMIT, "gradient_u: grid too large for the brick launch");
     constexpr int nt = J3Brick<BX, BY, BZ>::threads;
-    auto run = [&](auto kernel) { kernel<<<(int)nb, nt, 0, st>>>(g, mt, m->d_rowmap.p, vals); };
-    by_penalty(mt.p, run, k_grid_jacobian3<VT, BX, BY, BZ, 3>, k_grid_jacobian3<VT, BX, BY, BZ, 1>,
-               k_grid_jacobian3<VT, BX, BY, BZ, 0>);
+    auto run = [&](auto kernel) { kernel<<<(int)nb, nt, 0, st>>>(g, mt, m->d_rowmap.p, vals, u, fext, R); };
+    if (R)
+        by_penalty(mt.p, run, k_grid_jacobian3<VT, BX, BY, BZ, 3, true>, k_grid_jacobian3<VT, BX, BY, BZ, 1, true>,
+                   k_grid_jacobian3<VT, BX, BY, BZ, 0, true>);
+    else
+        by_penalty(mt.p, run, k_grid_jacobian3<VT, BX, BY, BZ, 3, false>, k_grid_jacobian3<VT, BX, BY, BZ, 1, false>,
+                   k_grid_jacobian3<VT, BX, BY, BZ, 0, false>);
 }
 
 template <class VT>
-void grid_jacobian3_run(const pa_mesh* m, const Material& mt, const double* z, VT* vals, hipStream_t st) {
+void grid_jacobian3_run(const pa_mesh* m, const Material& mt, const double* z, VT* vals, const double* u,
+                        const double* fext, double* R, hipStream_t st) {
     GridArgs g{};
     g.d0 = m->kgrid[0];
     g.d1 = m->kgrid[1];
     g.d2 = m->kgrid[2];
     g.x4 = reinterpret_cast<const double4*>(m->d_coords4.p);
     g.z = z;
-    grid_jacobian3_brick<VT, 4, 4, 4>(m, g, mt, vals, st);
+    grid_jacobian3_brick<VT, 4, 4, 4>(m, g, mt, vals, u, fext, R, st);
 }
 
 }  // namespace
@@ -1195,7 +1244,22 @@ bool grid_jacobian3(const pa_mesh* m, const Material& mt, const double* z, VT* v
         return e != nullptr && std::atoi(e) == 0;
     }();
     if (!m->has_grid || off) return false;
-    grid_jacobian3_run<VT>(m, mt, z, vals, st);
+    grid_jacobian3_run<VT>(m, mt, z, vals, nullptr, nullptr, nullptr, st);
+    PA_HIP(hipGetLastError());
+    return true;
+}
+
+// Elastic value and gradient_u on Kuhn grids in one pass: the residual from
+// the blocks (R = K u + f, f the natural-BC load or null).
+template <class VT>
+bool grid_value_jacobian3(const pa_mesh* m, const Material& mt, const double* u, const double* z,
+                          const double* fext, double* R, VT* vals, hipStream_t st) {
+    static const bool off = [] {
+        const char* e = std::getenv("PA_GRID_J3");
+        return e != nullptr && std::atoi(e) == 0;
+    }();
+    if (!m->has_grid || off) return false;
+    grid_jacobian3_run<VT>(m, mt, z, vals, u, fext, R, st);
     PA_HIP(hipGetLastError());
     return true;
 }
@@ -1203,6 +1267,10 @@ bool grid_jacobian3(const pa_mesh* m, const Material& mt, const double* z, VT* v
 template bool grid_jacobian1<double>(const pa_mesh*, const Material&, const double*, double*, hipStream_t);
 template bool grid_jacobian1<float>(const pa_mesh*, const Material&, const double*, float*, hipStream_t);
 template bool grid_jacobian3<double>(const pa_mesh*, const Material&, const double*, double*, hipStream_t);
+template bool grid_value_jacobian3<double>(const pa_mesh*, const Material&, const double*, const double*,
+                                           const double*, double*, double*, hipStream_t);
+template bool grid_value_jacobian3<float>(const pa_mesh*, const Material&, const double*, const double*,
+                                          const double*, double*, float*, hipStream_t);
 template bool grid_jacobian3<float>(const pa_mesh*, const Material&, const double*, float*, hipStream_t);
 
 }  // namespace pa

@@ -222,6 +222,11 @@ bool grid_jacobian1(const pa_mesh* m, const Material& mt, const double* z, VT* v
 // Elastic (3 x 3 block) Jacobian values in the mesh's CRS order.
 template <class VT>
 bool grid_jacobian3(const pa_mesh* m, const Material& mt, const double* z, VT* vals, hipStream_t st);
+// The elastic residual R = K u + fext (fext may be null) and the Jacobian
+// values in one pass.
+template <class VT>
+bool grid_value_jacobian3(const pa_mesh* m, const Material& mt, const double* u, const double* z,
+                          const double* fext, double* R, VT* vals, hipStream_t st);
 // Criterion gradient_x (N*3 doubles), scaled (the self-adjoint total
 // gradient wrt the coordinates is -1 times it).
 void criterion_gradient_x(const pa_scalar_function* sf, const double* u, const double* z, double scale, double* g,

@@ -106,6 +106,13 @@ def test_grid_vs_oracle(pa, dims, physics):
     K = vf.gradient_u(u, z).entries()
     assert rel(K, Kref) < TOL64
     assert rel(K2.entries(), Kref) < TOL64
+    # FP32 values: the residual still comes from the FP64 blocks
+    R3, K3 = vf.value_gradient_u(u, z, precision="fp32")
+    assert rel(R3.numpy(), Rref) < TOL64
+    assert rel(K3.entries(), Kref) < TOL32
+    if physics == "elastic":
+        # the one-pass value_gradient_u writes the same matrix as gradient_u
+        assert np.array_equal(K2.entries(), K)
     assert rel(vf.gradient_u(u, z, precision="fp32").entries(), Kref) < TOL32
     assert rel(vf.gradient_u_T(u, z).entries(), om.gradient_u_T(physics, u, z)) < TOL64
     if physics == "elastic" and N < 20000:
