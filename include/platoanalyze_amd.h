@@ -179,9 +179,12 @@ int pa_vector_function_gradient_u(const pa_vector_function* vf, const double* u_
 int pa_vector_function_gradient_u_T(const pa_vector_function* vf, const double* u_dev, const double* z_dev,
                                     void* values_dev, int precision, void* stream);
 /*
- * value and gradient_u in one call (the two separate calls' results, up to
- * rounding; one fused pass over the mesh where no cell tiles exist);
- * r_dev: N*dpn doubles.
+ * value and gradient_u in one call, as a Newton step needs them (no
+ * reference counterpart; src/elliptic/Problem.hpp:250-262 calls the two):
+ * the two separate calls' results, up to rounding.  On elastic Kuhn grids
+ * one kernel pass forms the blocks and R = K u + f from them; elsewhere the
+ * residual and Jacobian kernels run back to back, or one fused pass where no
+ * grid or cell tiles exist.  r_dev: N*dpn doubles, always FP64.
  */
 int pa_vector_function_value_gradient_u(const pa_vector_function* vf, const double* u_dev, const double* z_dev,
                                         double* r_dev, void* values_dev, int precision, void* stream);
