@@ -229,6 +229,16 @@ class ParameterList {
     void set(const std::string& aName, bool aValue) { set(aName, "bool", aValue ? "true" : "false"); }
     void set(const std::string& aName, const std::string& aValue) { set(aName, "string", aValue); }
     void set(const std::string& aName, const char* aValue) { set(aName, "string", std::string(aValue)); }
+    // a deep copy (sublists copied, not shared)
+    std::shared_ptr<ParameterList> clone() const {
+        auto tOut = std::make_shared<ParameterList>(mName);
+        for (const auto& kv : mEntries) {
+            Entry e = kv.second;
+            if (e.list) e.list = e.list->clone();
+            tOut->mEntries.push_back({kv.first, e});
+        }
+        return tOut;
+    }
     // names of the sublists, in order
     std::vector<std::string> sublistNames() const {
         std::vector<std::string> tOut;
@@ -886,7 +896,8 @@ struct Mechanics {
     static constexpr int mPhysics = PA_MECHANICS;
     static constexpr int mEnergy = PA_INTERNAL_ELASTIC_ENERGY;
     static constexpr const char* mEnergyName = "Internal Elastic Energy";
-    static std::vector<std::string> dofNames() { return {"Displacement X", "Displacement Y", "Displacement Z"}; }
+    // the residual's dof names (src/elliptic/ElastostaticResidual.hpp:104-106)
+    static std::vector<std::string> dofNames() { return {"displacement X", "displacement Y", "displacement Z"}; }
 };
 
 template <int SpaceDim>
@@ -897,7 +908,8 @@ struct Thermal {
     static constexpr int mPhysics = PA_THERMAL;
     static constexpr int mEnergy = PA_INTERNAL_THERMAL_ENERGY;
     static constexpr const char* mEnergyName = "Internal Thermal Energy";
-    static std::vector<std::string> dofNames() { return {"Temperature"}; }
+    // src/elliptic/ThermostaticResidual.hpp:91
+    static std::vector<std::string> dofNames() { return {"temperature"}; }
 };
 
 // Physics "Plasticity": the stabilized mixed u-p J2 form (4 dofs per node).
@@ -908,6 +920,10 @@ struct InfinitesimalStrainPlasticity {
     static constexpr int mNumDofsPerNode = SpaceDim + 1;
     static constexpr int mNumLocalDofsPerCell = 14;
     static constexpr int mFormulation = PA_J2_MIXED;
+    // the stabilized residual's dof names (src/StabilizedElastostaticResidual.hpp:86-89)
+    static std::vector<std::string> dofNames() {
+        return {"displacement X", "displacement Y", "displacement Z", "pressure"};
+    }
 };
 
 // Pure-displacement J2 (3 dofs per node); no counterpart in the reference.
@@ -918,6 +934,7 @@ struct DisplacementPlasticity {
     static constexpr int mNumDofsPerNode = SpaceDim;
     static constexpr int mNumLocalDofsPerCell = 14;
     static constexpr int mFormulation = PA_J2_DISPLACEMENT;
+    static std::vector<std::string> dofNames() { return {"displacement X", "displacement Y", "displacement Z"}; }
 };
 
 namespace detail {
@@ -1720,6 +1737,465 @@ class PlasticityProblem {
     std::vector<int> mBcDofs;
     std::vector<double> mBcValues;
 };
+
+// ------------------------------------------- Plato Engine operations
+namespace detail {
+
+// Element tree of a plain XML document, the subset of Plato::InputData that
+// the operation definitions use: <Operation><Function>...</Function>
+// <Name>...</Name> <Output><Argument>Value</Argument>
+// <ArgumentName>...</ArgumentName></Output> ...</Operation>.
+struct XmlElement {
+    std::string name, text;
+    std::vector<XmlElement> children;
+
+    std::vector<const XmlElement*> all(const std::string& aName) const {
+        std::vector<const XmlElement*> tOut;
+        for (const auto& c : children)
+            if (c.name == aName) tOut.push_back(&c);
+        return tOut;
+    }
+    const XmlElement* first(const std::string& aName) const {
+        for (const auto& c : children)
+            if (c.name == aName) return &c;
+        return nullptr;
+    }
+    // the trimmed text of the first child named aName, or aDefault
+    std::string get(const std::string& aName, const std::string& aDefault = "") const {
+        const XmlElement* c = first(aName);
+        if (!c) return aDefault;
+        const size_t i = c->text.find_first_not_of(" \t\n\r"), j = c->text.find_last_not_of(" \t\n\r");
+        return i == std::string::npos ? std::string() : c->text.substr(i, j - i + 1);
+    }
+};
+
+// The document as a root element holding its top-level elements (comments
+// and processing instructions are skipped; attributes are ignored).
+inline XmlElement parseXmlElements(const std::string& aXml) {
+    XmlElement tRoot;
+    std::vector<XmlElement*> tStack{&tRoot};
+    size_t i = 0;
+    while (i < aXml.size()) {
+        const size_t lt = aXml.find('<', i);
+        const std::string tText = aXml.substr(i, (lt == std::string::npos ? aXml.size() : lt) - i);
+        if (tStack.size() > 1) tStack.back()->text += xml_unescape(tText);
+        if (lt == std::string::npos) break;
+        if (aXml.compare(lt, 4, "<!--") == 0) {
+            const size_t e = aXml.find("-->", lt + 4);
+            if (e == std::string::npos) throw_error("XML: unterminated comment");
+            i = e + 3;
+            continue;
+        }
+        const size_t gt = aXml.find('>', lt);
+        if (gt == std::string::npos) throw_error("XML: unterminated tag");
+        std::string tBody = aXml.substr(lt + 1, gt - lt - 1);
+        i = gt + 1;
+        if (!tBody.empty() && (tBody[0] == '?' || tBody[0] == '!')) continue;
+        if (!tBody.empty() && tBody[0] == '/') {
+            std::string tName = tBody.substr(1);
+            tName.erase(tName.find_last_not_of(" \t\n\r") + 1);
+            if (tStack.size() < 2 || tStack.back()->name != tName)
+                throw_error("XML: unexpected closing tag </" + tName + ">");
+            tStack.pop_back();
+            continue;
+        }
+        const bool tSelfClosing = !tBody.empty() && tBody.back() == '/';
+        if (tSelfClosing) tBody.pop_back();
+        const size_t n = tBody.find_first_of(" \t\n\r");
+        XmlElement tElement;
+        tElement.name = tBody.substr(0, n);
+        if (tElement.name.empty()) throw_error("XML: empty tag name");
+        tStack.back()->children.push_back(tElement);
+        if (!tSelfClosing) tStack.push_back(&tStack.back()->children.back());
+    }
+    if (tStack.size() != 1) throw_error("XML: unterminated element <" + tStack.back()->name + ">");
+    return tRoot;
+}
+
+// AnalyzeAppUtils' parse_inline (src/AnalyzeAppUtils.cpp:46-118): the target
+// "[list]:[list]:name" or "...:name(i)" set to aValue.
+inline void parse_inline(ParameterList& aParams, const std::string& aTarget, double aValue) {
+    std::vector<std::string> tTokens;
+    {
+        std::istringstream s(aTarget);
+        std::string t;
+        while (std::getline(s, t, ':')) tTokens.push_back(t);
+    }
+    if (tTokens.empty()) throw_error("parameter target '" + aTarget + "' is empty");
+    ParameterList* tList = &aParams;
+    size_t k = 0;
+    for (; k + 1 < tTokens.size() && tTokens[k].size() > 1 && tTokens[k].front() == '[' && tTokens[k].back() == ']';
+         ++k) {
+        const std::string tName = tTokens[k].substr(1, tTokens[k].size() - 2);
+        if (!tList->isSublist(tName)) throw_error("parameter target '" + aTarget + "': no sublist '" + tName + "'");
+        tList = &tList->sublist(tName);
+    }
+    const std::string& tLast = tTokens[k];
+    const size_t p1 = tLast.find('('), p2 = tLast.find(')');
+    if (p1 != std::string::npos && p2 != std::string::npos && p2 > p1) {
+        const std::string tName = tLast.substr(0, p1);
+        auto tVec = tList->get<std::vector<double>>(tName);
+        const int tEntry = std::stoi(tLast.substr(p1 + 1, p2 - p1 - 1));
+        if (tEntry < 0 || tEntry >= (int)tVec.size())
+            throw_error("parameter target '" + aTarget + "': entry out of range");
+        tVec[tEntry] = aValue;
+        std::ostringstream o;
+        o.precision(17);
+        o << "{";
+        for (size_t q = 0; q < tVec.size(); ++q) o << (q ? ", " : "") << tVec[q];
+        o << "}";
+        tList->set(tName, "Array(double)", o.str());
+    } else {
+        tList->set(tLast, aValue);
+    }
+}
+
+// The problem an application drives, whatever its physics.
+class AppProblem {
+  public:
+    virtual ~AppProblem() = default;
+    virtual void solve(const ScalarVector& aControl) = 0;
+    virtual bool solved() const = 0;
+    virtual Scalar value(const ScalarVector& aControl, const std::string& aCriterion) = 0;
+    virtual ScalarVector gradient(const ScalarVector& aControl, const std::string& aCriterion) = 0;
+    virtual ScalarVector gradientX(const ScalarVector& aControl, const std::string& aCriterion) = 0;
+    virtual std::vector<std::string> dofNames() const = 0;
+    virtual ScalarVector lastState() const = 0;
+};
+
+template <class PhysicsT>
+class EllipticAppProblem : public AppProblem {
+  public:
+    EllipticAppProblem(std::shared_ptr<Mesh> aMesh, const ParameterList& aParams) : mProblem(aMesh, aParams) {}
+    void solve(const ScalarVector& aControl) override { mSolution = mProblem.solution(aControl); }
+    bool solved() const override { return !mSolution.empty(); }
+    Scalar value(const ScalarVector& aControl, const std::string& aCriterion) override {
+        return mProblem.criterionValue(aControl, mSolution, aCriterion);
+    }
+    ScalarVector gradient(const ScalarVector& aControl, const std::string& aCriterion) override {
+        return mProblem.criterionGradient(aControl, mSolution, aCriterion);
+    }
+    ScalarVector gradientX(const ScalarVector& aControl, const std::string& aCriterion) override {
+        return mProblem.criterionGradientX(aControl, mSolution, aCriterion);
+    }
+    std::vector<std::string> dofNames() const override { return PhysicsT::dofNames(); }
+    ScalarVector lastState() const override { return mSolution.get("State", mSolution.numSteps() - 1); }
+
+  private:
+    Elliptic::Problem<PhysicsT> mProblem;
+    Solutions mSolution;
+};
+
+template <class PhysicsT>
+class PlasticityAppProblem : public AppProblem {
+  public:
+    PlasticityAppProblem(std::shared_ptr<Mesh> aMesh, const ParameterList& aParams) : mProblem(aMesh, aParams) {}
+    void solve(const ScalarVector& aControl) override { mSolution = mProblem.solution(aControl); }
+    bool solved() const override { return !mSolution.empty(); }
+    Scalar value(const ScalarVector& aControl, const std::string& aCriterion) override {
+        return mProblem.criterionValue(aControl, aCriterion);
+    }
+    ScalarVector gradient(const ScalarVector& aControl, const std::string& aCriterion) override {
+        return mProblem.criterionGradient(aControl, aCriterion);
+    }
+    ScalarVector gradientX(const ScalarVector&, const std::string&) override {
+        throw_error("AnalyzeApp: the coordinate gradient of a plasticity criterion is not on this path");
+    }
+    std::vector<std::string> dofNames() const override { return PhysicsT::dofNames(); }
+    ScalarVector lastState() const override { return mSolution.get("State", mSolution.numSteps() - 1); }
+
+  private:
+    PlasticityProblem<PhysicsT> mProblem;
+    Solutions mSolution;
+};
+
+}  // namespace detail
+
+// The Plato Engine application of this path (MPMD_App, src/Analyze_App.hpp:
+// 64-1000, src/Analyze_App.cpp): operations named in an operations document
+// run on the shared data the engine imports and exports.  Functions:
+// ComputeSolution, ComputeCriterion, ComputeCriterionValue,
+// ComputeCriterionGradient, ComputeCriterionX, ComputeCriterionGradientX,
+// Reinitialize (with OnChange) and UpdateProblem (src/Analyze_App.cpp:
+// 368-500, 700-1320); an operation's Parameters retarget the input deck as
+// parse_inline does.  Physics 'Mechanical' and 'Thermal' (elliptic,
+// self-adjoint criteria) and 'Plasticity' (the mixed J2 form, the plastic work
+// criterion).  Other functions (output, ESP, MLS, Helmholtz, finite
+// differences, mesh reload) and ProblemDefinitions other than the input deck
+// throw at construction.  One rank: a scalar field's local ids are the node
+// ids.
+class AnalyzeApp {
+  public:
+    // aInput: the input deck, whose 'Plato Problem' sublist (or, without one,
+    // the list itself) defines the problem; parameter targets are paths from
+    // the deck's root, as in the reference ("[Plato Problem]:[...]:name")
+    AnalyzeApp(std::shared_ptr<Mesh> aMesh, const ParameterList& aInput, const std::string& aOperationsXml)
+        : mMesh(std::move(aMesh)), mDef(aInput.clone()) {
+        const detail::XmlElement tDoc = detail::parseXmlElements(aOperationsXml);
+        std::vector<const detail::XmlElement*> tOps;
+        for (const auto& c : tDoc.children) {
+            if (c.name == "Operation") tOps.push_back(&c);
+            for (const auto* o : c.all("Operation")) tOps.push_back(o);  // operations under a wrapper element
+        }
+        for (const auto* o : tOps) addOperation(*o);
+    }
+
+    // MPMD_App::initialize (src/Analyze_App.cpp:338-349): the control starts at 1
+    void initialize() {
+        mControl = toDevice(std::vector<Scalar>((size_t)mMesh->NumNodes(), 1.0), "control");
+        createProblem();
+    }
+    // MPMD_App::reinitialize (src/Analyze_App.cpp:511-518)
+    void reinitialize() { createProblem(); }
+
+    // MPMD_App::compute (src/Analyze_App.cpp:563-590)
+    void compute(const std::string& aOperationName) {
+        auto it = mOperations.find(aOperationName);
+        if (it == mOperations.end()) throw_error("AnalyzeApp: no operation named '" + aOperationName + "'");
+        if (!mProblem || mModified) createProblem();
+        run(it->second);
+    }
+
+    // importData: a SCALAR_FIELD named "Topology" (the control, one value per node)
+    void importData(const std::string& aName, const std::vector<Scalar>& aField) {
+        if (aName != "Topology") throw_error("AnalyzeApp: cannot import field '" + aName + "'");
+        if (aField.size() != (size_t)mMesh->NumNodes()) throw_error("AnalyzeApp: 'Topology' needs one value per node");
+        mControl = toDevice(aField, "control");
+    }
+    // importData of a SCALAR_PARAMETER: the value of parameter aName of
+    // operation aOperation (MPMD_App::importScalarParameter, src/Analyze_App.hpp:
+    // 228-241); a parameter with a Target updates the input deck, and the
+    // problem is recreated when an operation next runs
+    void importParameter(const std::string& aName, const std::string& aOperation, Scalar aValue) {
+        auto it = mOperations.find(aOperation);
+        if (it == mOperations.end()) throw_error("AnalyzeApp: no operation named '" + aOperation + "'");
+        auto p = it->second.parameters.find(aName);
+        if (p == it->second.parameters.end())
+            throw_error("Attempted to update a parameter ('" + aName + "') that wasn't defined for this operation");
+        p->second.value = aValue;
+        if (!p->second.target.empty()) {
+            detail::parse_inline(*mDef, p->second.target, aValue);
+            mModified = true;
+        }
+    }
+    // importData of a SCALAR value (MPMD_App::importScalarValue,
+    // src/Analyze_App.hpp:248-260): "Parameters" feeds Reinitialize's OnChange
+    void importValue(const std::string& aName, const std::vector<Scalar>& aValues) {
+        auto it = mValues.find(aName);
+        if (it == mValues.end())
+            throw_error("Attempted to import SharedValue ('" + aName + "') that doesn't exist.");
+        it->second = aValues;
+    }
+
+    // exportData (src/Analyze_App.hpp:287-450): "Topology"; a criterion value
+    // (minus its target), a gradient with respect to the control, the
+    // component i of a coordinate gradient ("name@i", i = 0 by default), or a
+    // component of the last solution by dof name ("displacement X", ...)
+    void exportData(const std::string& aName, std::vector<Scalar>& aOut) const {
+        const size_t at = aName.find('@');
+        const std::string tName = aName.substr(0, at);
+        const int tIndex = at == std::string::npos ? 0 : std::atoi(aName.c_str() + at + 1);
+        if (tName == "Topology") {
+            aOut = toHost(mControl);
+            return;
+        }
+        if (auto v = mValueNames.find(tName); v != mValueNames.end()) {
+            auto c = mCriterionValues.find(v->second);
+            if (c == mCriterionValues.end())
+                throw_error("Attempted to export SharedValue ('" + tName + "') that doesn't exist.");
+            aOut.assign(1, c->second);
+            return;
+        }
+        if (auto g = mGradZNames.find(tName); g != mGradZNames.end()) {
+            auto c = mGradZ.find(g->second);
+            if (c == mGradZ.end()) throw_error("AnalyzeApp: gradient '" + tName + "' has not been computed");
+            aOut = toHost(c->second);
+            return;
+        }
+        if (auto g = mGradXNames.find(tName); g != mGradXNames.end()) {
+            auto c = mGradX.find(g->second);
+            if (c == mGradX.end()) throw_error("AnalyzeApp: gradient '" + tName + "' has not been computed");
+            if (tIndex < 0 || tIndex > 2) throw_error("AnalyzeApp: '" + aName + "': component out of range");
+            const auto h = toHost(c->second);
+            aOut.resize(h.size() / 3);
+            for (size_t n = 0; n < aOut.size(); ++n) aOut[n] = h[3 * n + tIndex];
+            return;
+        }
+        if (auto v = mValues.find(tName); v != mValues.end()) {
+            aOut = v->second;
+            return;
+        }
+        if (mProblem && mProblem->solved()) {
+            const auto tDofs = mProblem->dofNames();
+            const auto d = std::find(tDofs.begin(), tDofs.end(), tName);
+            if (d != tDofs.end()) {
+                const auto h = toHost(mProblem->lastState());
+                const size_t tStride = tDofs.size(), tDof = (size_t)(d - tDofs.begin());
+                aOut.resize(h.size() / tStride);
+                for (size_t n = 0; n < aOut.size(); ++n) aOut[n] = h[tStride * n + tDof];
+                return;
+            }
+        }
+        throw_error("AnalyzeApp: nothing to export under the name '" + aName + "'");
+    }
+    // exportDataMap(SCALAR_FIELD): the node ids this rank owns (all of them)
+    void exportDataMap(std::vector<int>& aLocalIDs) const {
+        aLocalIDs.resize((size_t)mMesh->NumNodes());
+        for (size_t n = 0; n < aLocalIDs.size(); ++n) aLocalIDs[n] = (int)n;
+    }
+    std::vector<std::string> operationNames() const {
+        std::vector<std::string> tOut;
+        for (const auto& kv : mOperations) tOut.push_back(kv.first);
+        return tOut;
+    }
+
+  private:
+    struct Parameter {
+        std::string target;
+        Scalar value = 0.0;
+    };
+    struct Operation {
+        std::string function, criterion;
+        Scalar target = 0.0;
+        bool onChange = false;
+        std::vector<Scalar> lastParameters;
+        bool hasLast = false;
+        std::map<std::string, Parameter> parameters;
+    };
+
+    // getArgumentName (src/Analyze_App.cpp:80-110): the ArgumentName of the
+    // Output whose Argument is aArgument
+    static std::string argumentName(const detail::XmlElement& aOp, const std::string& aArgument) {
+        for (const auto* o : aOp.all("Output")) {
+            if (o->get("Argument") != aArgument) continue;
+            const std::string n = o->get("ArgumentName");
+            if (n.empty()) throw_error("Parsing operation: 'ArgumentName' missing");
+            return n;
+        }
+        return "";
+    }
+    // addUnique (src/Analyze_App.cpp:33-60)
+    static void addUnique(std::map<std::string, std::string>& aMap, const std::string& aKey,
+                          const std::string& aCriterion) {
+        if (aKey.empty()) return;
+        auto it = aMap.find(aKey);
+        if (it != aMap.end() && it->second != aCriterion)
+            throw_error("ArgumentName '" + aKey + "' is already associated with Criterion '" + it->second +
+                        "' and cannot be reassigned to Criterion '" + aCriterion + "'");
+        aMap[aKey] = aCriterion;
+    }
+
+    void addOperation(const detail::XmlElement& aNode) {
+        Operation op;
+        op.function = aNode.get("Function");
+        const std::string tName = aNode.get("Name");
+        if (tName.empty()) throw_error("AnalyzeApp: an Operation has no Name");
+        if (mOperations.count(tName)) throw_error("AnalyzeApp: operation '" + tName + "' is defined twice");
+        const std::string tProblem = aNode.get("ProblemDefinition");
+        if (!tProblem.empty())
+            throw_error("AnalyzeApp: operation '" + tName + "': ProblemDefinition '" + tProblem +
+                        "' (only the input deck's problem is on this path)");
+        for (const auto* pn : aNode.all("Parameter")) {
+            const std::string n = pn->get("ArgumentName");
+            if (op.parameters.count(n)) throw_error("ArgumentNames must be unique.");
+            Parameter p;
+            p.target = pn->get("Target");
+            p.value = std::atof(pn->get("InitialValue", "0").c_str());
+            op.parameters[n] = p;
+        }
+        const std::string& f = op.function;
+        const bool tCriterionOp = f == "ComputeCriterion" || f == "ComputeCriterionValue" ||
+                                  f == "ComputeCriterionGradient" || f == "ComputeCriterionX" ||
+                                  f == "ComputeCriterionGradientX";
+        if (tCriterionOp) {
+            // CriterionOp (src/Analyze_App.cpp:621-633)
+            op.criterion = aNode.get("Criterion");
+            if (op.criterion.empty()) throw_error("Required keyword ('Criterion') missing.");
+            op.target = std::atof(aNode.get("Target", "0").c_str());
+            const std::string v = argumentName(aNode, "Value"), g = argumentName(aNode, "Gradient");
+            if (f == "ComputeCriterion" || f == "ComputeCriterionValue" || f == "ComputeCriterionX")
+                addUnique(mValueNames, v, op.criterion);
+            if (f == "ComputeCriterion" || f == "ComputeCriterionGradient") addUnique(mGradZNames, g, op.criterion);
+            if (f == "ComputeCriterionX" || f == "ComputeCriterionGradientX") addUnique(mGradXNames, g, op.criterion);
+            if (f == "ComputeCriterionX" && !aNode.get("OutputFile").empty())
+                throw_error("AnalyzeApp: ComputeCriterionX 'OutputFile' (HDF5) is not on this path");
+        } else if (f == "ComputeSolution") {
+            if (aNode.first("WriteOutput"))
+                throw_error("AnalyzeApp: ComputeSolution 'WriteOutput' (visualization files) is not on this path");
+        } else if (f == "Reinitialize") {
+            // OnChangeOp (src/Analyze_App.cpp:636-643)
+            op.onChange = aNode.get("OnChange", "false") == "true";
+            mValues["Parameters"];
+        } else if (f != "UpdateProblem") {
+            throw_error("AnalyzeApp: operation function '" + f + "' is not on this path");
+        }
+        mOperations[tName] = op;
+    }
+
+    void createProblem() {
+        const ParameterList& tDef = mDef->isSublist("Plato Problem") ? mDef->sublist("Plato Problem") : *mDef;
+        const std::string tPhysics = tDef.get<std::string>("Physics", std::string(""));
+        if (tPhysics == "Mechanical")
+            mProblem = std::make_unique<detail::EllipticAppProblem<Mechanics<3>>>(mMesh, tDef);
+        else if (tPhysics == "Thermal")
+            mProblem = std::make_unique<detail::EllipticAppProblem<Thermal<3>>>(mMesh, tDef);
+        else if (tPhysics == "Plasticity")
+            mProblem = std::make_unique<detail::PlasticityAppProblem<InfinitesimalStrainPlasticity<3>>>(mMesh, tDef);
+        else
+            throw_error("AnalyzeApp: Physics '" + tPhysics + "' is not on this path");
+        if (mControl.size() == 0)
+            mControl = toDevice(std::vector<Scalar>((size_t)mMesh->NumNodes(), 1.0), "control");
+        mModified = false;
+    }
+
+    void run(Operation& op) {
+        const std::string& f = op.function;
+        if (f == "ComputeSolution") {
+            mProblem->solve(mControl);
+        } else if (f == "ComputeCriterion") {
+            // src/Analyze_App.cpp:727-745: solve, value minus target, gradient
+            mProblem->solve(mControl);
+            mCriterionValues[op.criterion] = mProblem->value(mControl, op.criterion) - op.target;
+            mGradZ[op.criterion] = mProblem->gradient(mControl, op.criterion);
+        } else if (f == "ComputeCriterionValue") {
+            mProblem->solve(mControl);
+            mCriterionValues[op.criterion] = mProblem->value(mControl, op.criterion) - op.target;
+        } else if (f == "ComputeCriterionGradient") {
+            // at the last solution (src/Analyze_App.cpp:996-1003)
+            if (!mProblem->solved()) mProblem->solve(mControl);
+            mGradZ[op.criterion] = mProblem->gradient(mControl, op.criterion);
+        } else if (f == "ComputeCriterionX") {
+            mProblem->solve(mControl);
+            mCriterionValues[op.criterion] = mProblem->value(mControl, op.criterion) - op.target;
+            mGradX[op.criterion] = mProblem->gradientX(mControl, op.criterion);
+        } else if (f == "ComputeCriterionGradientX") {
+            if (!mProblem->solved()) mProblem->solve(mControl);
+            mGradX[op.criterion] = mProblem->gradientX(mControl, op.criterion);
+        } else if (f == "Reinitialize") {
+            // src/Analyze_App.cpp:1214-1235: with OnChange, only when the
+            // imported "Parameters" differ from the last call's
+            const auto& tParams = mValues["Parameters"];
+            const bool tChanged = !op.hasLast || tParams != op.lastParameters;
+            op.lastParameters = tParams;
+            op.hasLast = true;
+            if (!op.onChange || tChanged) createProblem();
+        }
+        // UpdateProblem (src/Analyze_App.cpp:1310-1318) updates the criteria's
+        // continuation parameters; the criteria on this path have none
+    }
+
+    std::shared_ptr<Mesh> mMesh;
+    std::shared_ptr<ParameterList> mDef;
+    bool mModified = false;
+    std::unique_ptr<detail::AppProblem> mProblem;
+    ScalarVector mControl;
+    std::map<std::string, Operation> mOperations;
+    std::map<std::string, std::string> mValueNames, mGradZNames, mGradXNames;
+    std::map<std::string, Scalar> mCriterionValues;
+    std::map<std::string, ScalarVector> mGradZ, mGradX;
+    std::map<std::string, std::vector<Scalar>> mValues;
+};
+
 
 }  // namespace amd
 }  // namespace Plato

@@ -524,6 +524,135 @@ std::string exodus_problem() {
     return j.str();
 }
 
+// The compliance deck driven through the Plato Engine operations
+// (AnalyzeApp): the engine's control import, a solution, criterion value and
+// gradient, a value with a target, the coordinate gradient by component, and
+// an operation Parameter that retargets the traction.
+const char* kComplianceOperations = R"(<?xml version="1.0"?>
+<!-- operations of the compliance deck -->
+<Operation>
+  <Function>ComputeSolution</Function>
+  <Name>Compute Displacement Solution</Name>
+</Operation>
+<Operation>
+  <Function>ComputeCriterion</Function>
+  <Name>Compute Objective</Name>
+  <Criterion>Compliance</Criterion>
+  <Output><Argument>Value</Argument><ArgumentName>Objective Value</ArgumentName></Output>
+  <Output><Argument>Gradient</Argument><ArgumentName>Objective Gradient</ArgumentName></Output>
+</Operation>
+<Operation>
+  <Function>ComputeCriterionValue</Function>
+  <Name>Compute Constraint Value</Name>
+  <Criterion>Compliance</Criterion>
+  <Target>2.5</Target>
+  <Output><Argument>Value</Argument><ArgumentName>Constraint Value</ArgumentName></Output>
+</Operation>
+<Operation>
+  <Function>ComputeCriterionGradientX</Function>
+  <Name>Compute Objective Gradient X</Name>
+  <Criterion>Compliance</Criterion>
+  <Output><Argument>Gradient</Argument><ArgumentName>Objective Gradient X</ArgumentName></Output>
+</Operation>
+<Operation>
+  <Function>ComputeCriterionValue</Function>
+  <Name>Compute Value Scaled Load</Name>
+  <Criterion>Compliance</Criterion>
+  <Parameter>
+    <ArgumentName>Traction Z</ArgumentName>
+    <Target>[Plato Problem]:[Natural Boundary Conditions]:[Traction Vector Boundary Condition]:Values(2)</Target>
+    <InitialValue>-1.0</InitialValue>
+  </Parameter>
+  <Output><Argument>Value</Argument><ArgumentName>Scaled Value</ArgumentName></Output>
+</Operation>
+<Operation>
+  <Function>Reinitialize</Function>
+  <Name>Reinitialize on Change</Name>
+  <OnChange>true</OnChange>
+</Operation>
+<Operation>
+  <Function>UpdateProblem</Function>
+  <Name>Update Problem</Name>
+</Operation>
+)";
+
+std::string analyze_app() {
+    auto tParams = pa::getParametersFromXmlString(std::string("<ParameterList name='Problem'>\n") + kComplianceDeck +
+                                                  "</ParameterList>\n");
+    auto tMesh = pa::getBoxMesh(6, 3, 3, 2.0, 1.0, 1.0);
+    const int N = tMesh->NumNodes();
+    std::vector<double> zh(N);
+    for (int i = 0; i < N; ++i) zh[i] = 0.3 + 0.7 * (double)((i * 7919) % 101) / 100.0;
+    pa::AnalyzeApp app(tMesh, *tParams, kComplianceOperations);
+    app.initialize();
+    std::vector<int> ids;
+    app.exportDataMap(ids);
+    app.importData("Topology", zh);
+    Json j;
+    std::vector<double> out;
+    app.exportData("Topology", out);
+    j.put("topology", out);
+    j.put("num_ids", (double)ids.size());
+    app.compute("Compute Displacement Solution");
+    for (const char* d : {"displacement X", "displacement Y", "displacement Z"}) {
+        app.exportData(d, out);
+        j.put(d, out);
+    }
+    app.compute("Compute Objective");
+    app.exportData("Objective Value", out);
+    j.put("objective_value", out[0]);
+    app.exportData("Objective Gradient", out);
+    j.put("objective_gradient", out);
+    app.compute("Compute Objective Gradient X");
+    for (int d = 0; d < 3; ++d) {
+        app.exportData("Objective Gradient X@" + std::to_string(d), out);
+        j.put("gradient_x" + std::to_string(d), out);
+    }
+    app.compute("Compute Constraint Value");
+    app.exportData("Constraint Value", out);
+    j.put("constraint_value", out[0]);
+    app.compute("Update Problem");
+    // OnChange: the first call reinitializes, a second with the same
+    // "Parameters" does not, a changed value does
+    app.importValue("Parameters", {1.0});
+    app.compute("Reinitialize on Change");
+    app.compute("Reinitialize on Change");
+    app.importValue("Parameters", {2.0});
+    app.compute("Reinitialize on Change");
+    app.compute("Compute Objective");
+    app.exportData("Objective Value", out);
+    j.put("objective_value_after_reinit", out[0]);
+    app.importParameter("Traction Z", "Compute Value Scaled Load", -2.0);
+    app.compute("Compute Value Scaled Load");
+    app.exportData("Scaled Value", out);
+    j.put("scaled_value", out[0]);
+    // errors the reference raises too
+    auto throws = [&](auto&& f) {
+        try {
+            f();
+        } catch (const std::runtime_error&) {
+            return 1.0;
+        }
+        return 0.0;
+    };
+    j.put("unknown_operation_throws", throws([&] { app.compute("No Such Operation"); }));
+    j.put("undefined_parameter_throws", throws([&] { app.importParameter("Nope", "Compute Objective", 1.0); }));
+    j.put("write_output_throws", throws([&] {
+              pa::AnalyzeApp a2(tMesh, *tParams,
+                                "<Operation><Function>WriteOutput</Function><Name>Write</Name></Operation>");
+          }));
+    j.put("reassigned_name_throws", throws([&] {
+              pa::AnalyzeApp a3(tMesh, *tParams,
+                                "<Operation><Function>ComputeCriterionValue</Function><Name>A</Name>"
+                                "<Criterion>Compliance</Criterion><Output><Argument>Value</Argument>"
+                                "<ArgumentName>V</ArgumentName></Output></Operation>"
+                                "<Operation><Function>ComputeCriterionValue</Function><Name>B</Name>"
+                                "<Criterion>Other</Criterion><Output><Argument>Value</Argument>"
+                                "<ArgumentName>V</ArgumentName></Output></Operation>");
+          }));
+    return j.str();
+}
+
 // ---------------------------------------------------------------- parser
 int parser_checks() {
     int failures = 0;
@@ -579,6 +708,49 @@ int parser_checks() {
         threw = true;
     }
     expect(threw, "unknown symbol throws");
+    // operation documents and parameter targets
+    const auto ops = pa::detail::parseXmlElements(kComplianceOperations);
+    expect(ops.all("Operation").size() == 7, "operation count");
+    expect(ops.all("Operation")[0]->get("Function") == "ComputeSolution", "element text");
+    expect(ops.all("Operation")[1]->all("Output").size() == 2, "repeated elements");
+    expect(ops.all("Operation")[1]->all("Output")[1]->get("ArgumentName") == "Objective Gradient", "nested text");
+    expect(ops.all("Operation")[4]->first("Parameter")->get("Target") ==
+               "[Plato Problem]:[Natural Boundary Conditions]:[Traction Vector Boundary Condition]:Values(2)",
+           "parameter target");
+    expect(pa::detail::parseXmlElements("<a><b>x &amp; y</b><c/></a>").first("a")->get("b") == "x & y",
+           "entities and self-closing elements");
+    auto D = pa::getParametersFromXmlString(std::string("<ParameterList name='Problem'>\n") + kComplianceDeck +
+                                            "</ParameterList>\n");
+    auto D2 = D->clone();
+    pa::detail::parse_inline(*D2, "[Plato Problem]:[Natural Boundary Conditions]:[Traction Vector Boundary Condition]:Values(2)",
+                             -2.5);
+    pa::detail::parse_inline(*D2, "[Plato Problem]:[Linear Solver]:Tolerance", 1e-10);
+    const auto& tr = D2->sublist("Plato Problem").sublist("Natural Boundary Conditions");
+    expect(tr.sublist("Traction Vector Boundary Condition").get<std::vector<double>>("Values")[2] == -2.5,
+           "inline target, array entry");
+    expect(D2->sublist("Plato Problem").sublist("Linear Solver").get<double>("Tolerance") == 1e-10,
+           "inline target, scalar");
+    expect(D->sublist("Plato Problem")
+                   .sublist("Natural Boundary Conditions")
+                   .sublist("Traction Vector Boundary Condition")
+                   .get<std::vector<double>>("Values")[2] == -1.0,
+           "clone is deep");
+    for (const char* bad : {"<a><b></a>", "<a>", "</a>", "<!-- x"}) {
+        threw = false;
+        try {
+            pa::detail::parseXmlElements(bad);
+        } catch (const std::runtime_error&) {
+            threw = true;
+        }
+        expect(threw, std::string("malformed XML throws: ") + bad);
+    }
+    threw = false;
+    try {
+        pa::detail::parse_inline(*D2, "[Plato Problem]:[No Such List]:x", 1.0);
+    } catch (const std::runtime_error&) {
+        threw = true;
+    }
+    expect(threw, "inline target into a missing list throws");
     std::printf("%s %d\n", failures ? "parser checks failed:" : "parser checks passed", failures);
     return failures ? 1 : 0;
 }
@@ -596,6 +768,7 @@ int main(int argc, char** argv) {
         {"ElastoPlasticity_CriterionTest_3D", criterion_test_j2},
         {"J2Worksets", j2_worksets},
         {"ExodusMesh", exodus_problem},
+        {"AnalyzeApp", analyze_app},
     };
     if (argc < 2) {
         std::fprintf(stderr, "usage: %s <case>|all|--list|--parser|--exodus <file>\n", argv[0]);

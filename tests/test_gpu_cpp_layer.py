@@ -138,6 +138,33 @@ def test_compliance_problem(results):
     assert rel(o["gradient_x"], -om.energy_gradient_x("elastic", u, z)) < 1e-12
 
 
+def test_analyze_app_operations(results):
+    """The compliance deck through the Plato Engine operations (AnalyzeApp,
+    the reference's MPMD_App, src/Analyze_App.cpp): the same state, value and
+    gradients as the Problem calls of the Compliance case, the value minus the
+    operation's Target, the coordinate gradient by component ("name@i"), and
+    an operation Parameter that doubles the traction (compliance x4, linear
+    elasticity)."""
+    a, o = results["AnalyzeApp"], results["Compliance"]
+    z = np.asarray(o["z"])
+    assert a["num_ids"] == z.size
+    assert np.array_equal(a["topology"], z)
+    u = np.asarray(o["u"]).reshape(-1, 3)
+    for d, name in enumerate(("displacement X", "displacement Y", "displacement Z")):
+        assert np.array_equal(a[name], u[:, d])
+    assert a["objective_value"] == o["value"]
+    assert np.array_equal(a["objective_gradient"], o["gradient"])
+    gx = np.asarray(o["gradient_x"]).reshape(-1, 3)
+    for d in range(3):
+        assert np.array_equal(a[f"gradient_x{d}"], gx[:, d])
+    assert a["constraint_value"] == o["value"] - 2.5
+    assert a["objective_value_after_reinit"] == o["value"]
+    assert abs(a["scaled_value"] - 4.0 * o["value"]) <= 1e-12 * abs(4.0 * o["value"])
+    for k in ("unknown_operation_throws", "undefined_parameter_throws", "write_output_throws",
+              "reassigned_name_throws"):
+        assert a[k] == 1.0, k
+
+
 def test_j2_elastic_solution_golden(results):
     g = JGOLD["ElastoPlasticity_ElasticSolution3D"]
     o = results["ElastoPlasticity_ElasticSolution3D"]
