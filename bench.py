@@ -214,6 +214,12 @@ def compliance_gradient(pa, mesh, vf, z, dims):
 J2_DIMS, J2_LENGTHS = (64, 32, 32), (2.0, 1.0, 1.0)
 J2_PARAMS = dict(youngs=1e3, poisson=0.3, hardening_isotropic=10.0, hardening_kinematic=0.0, initial_yield_stress=1.0)
 J2_END_DISPLACEMENT, J2_STEPS = 0.008, 10
+# damped block-Jacobi smoothing of the J2 multigrid per formulation: the mixed
+# form's 4x4 saddle-point blocks need a smaller damping and more sweeps
+# (forward solve, 64x32x32: 4+4 sweeps at 0.6 took 2.58 s and 4,108 FGMRES
+# iterations; 5+5 at 0.45 1.77 s and 2,559; 0.7 diverges towards 12,900);
+# the displacement form 0.81 s at 4+4 / 0.6, 0.75 s at 3+3 / 0.7
+J2_SMOOTHING = {"mixed": (5, 0.45), "displacement": (3, 0.7)}
 
 
 WARMSTART_WEIGHTS = ROOT / "platoanalyze_amd" / "data" / "warmstart_96x48x48.npz"
@@ -328,8 +334,9 @@ def j2_plasticity(pa, stream, with_cpu=True, formulation="mixed"):
     # flexible GMRES(200) to 1e-8 relative, preconditioned by the DxD-block
     # geometric multigrid with 4 pre- and post-smoothing sweeps
     prob = pa.PlasticityProblem(pl, dofs, vals, num_steps=J2_STEPS, max_newton=10, linear_tol=1e-8, restart=200)
+    sweeps, omega = J2_SMOOTHING[formulation]
     if os.environ.get("PA_J2_PRECOND", "multigrid") == "multigrid":
-        prob.set_multigrid(*J2_DIMS, pre_sweeps=4, post_sweeps=4)
+        prob.set_multigrid(*J2_DIMS, pre_sweeps=sweeps, post_sweeps=sweeps, omega=omega)
     pa.synchronize()
     t0 = time.perf_counter()
     converged = prob.solution(z)
@@ -345,7 +352,7 @@ def j2_plasticity(pa, stream, with_cpu=True, formulation="mixed"):
     pck = pa.PlasticityProblem(pl, dofs, vals, num_steps=J2_STEPS, max_newton=10, linear_tol=1e-8, restart=200,
                                checkpoint_stride=m_ck)
     if os.environ.get("PA_J2_PRECOND", "multigrid") == "multigrid":
-        pck.set_multigrid(*J2_DIMS, pre_sweeps=4, post_sweeps=4)
+        pck.set_multigrid(*J2_DIMS, pre_sweeps=sweeps, post_sweeps=sweeps, omega=omega)
     pck.solution(z)
     pa.synchronize()
     t0 = time.perf_counter()
@@ -385,7 +392,8 @@ def j2_plasticity(pa, stream, with_cpu=True, formulation="mixed"):
     name = "mixed u-p" if D == 4 else "pure-displacement"
     out = {"workload": f"J2 plasticity {name} Tet4, {J2_DIMS[0]}x{J2_DIMS[1]}x{J2_DIMS[2]} hex x6 "
                        f"({C} cells, {N} nodes, {D * N} dof), {J2_STEPS} steps to ux={J2_END_DISPLACEMENT}",
-           "solver": f"condensed Newton; FGMRES(200) to 1e-8, {D}x{D}-block geometric multigrid (4+4 sweeps)",
+           "solver": f"condensed Newton; FGMRES(200) to 1e-8, {D}x{D}-block geometric multigrid "
+                     f"({sweeps}+{sweeps} damped block-Jacobi sweeps, omega {omega})",
            "forward_solve_s": t_fwd, "converged": converged,
            "criterion_value_and_adjoint_gradient_s": t_adj, "plastic_work": F,
            "gradient_norm": float(np.linalg.norm(g.numpy())),

@@ -1623,7 +1623,9 @@ class GlobalVectorFunctionInc {
 // Conditions' (Zero Value, Time Dependent a*t).  'Linear Solver' keys, with
 // no reference counterpart: Solver ('GMRES' | 'BiCGStab'), Restart (50),
 // Tolerance (1e-12), Iterations (20000), Preconditioner ('Block Jacobi' |
-// 'Multigrid'; the multigrid needs a box mesh).
+// 'Multigrid'; the multigrid needs a box mesh), Smoother Sweeps and Smoother
+// Damping of the multigrid (5 and 0.45 for the mixed form, 3 and 0.7 for the
+// displacement form).
 template <class PhysicsT>
 class PlasticityProblem {
   public:
@@ -1652,6 +1654,8 @@ class PlasticityProblem {
             mOptions.linear_tol = ls.get<double>("Tolerance", 1e-12);
             mOptions.linear_max_iter = ls.get<int>("Iterations", 20000);
             mMultigrid = ls.get<std::string>("Preconditioner", std::string("Block Jacobi")) == "Multigrid";
+            mSmootherSweeps = ls.get<int>("Smoother Sweeps", 0);
+            mSmootherDamping = ls.get<double>("Smoother Damping", 0.0);
         }
         readEssentialBoundaryConditions(aParams);
     }
@@ -1668,8 +1672,9 @@ class PlasticityProblem {
         if (mMultigrid) {
             if (!mMesh->boxDims[0]) throw_error("PlasticityProblem: the multigrid preconditioner needs a box mesh");
             pa_multigrid_options o{};
-            o.pre_sweeps = o.post_sweeps = 4;
-            o.omega = 0.6;
+            // the mixed form's saddle-point blocks take a smaller damping
+            o.pre_sweeps = o.post_sweeps = mSmootherSweeps > 0 ? mSmootherSweeps : (PhysicsT::mNumDofsPerNode == 4 ? 5 : 3);
+            o.omega = mSmootherDamping > 0.0 ? mSmootherDamping : (PhysicsT::mNumDofsPerNode == 4 ? 0.45 : 0.7);
             check(pa_plasticity_problem_set_multigrid(mProblem.get(), mMesh->boxDims[0], mMesh->boxDims[1],
                                                       mMesh->boxDims[2], &o));
         }
@@ -1733,6 +1738,8 @@ class PlasticityProblem {
     std::string mCriterion;
     double mMultiplier = 1.0;
     bool mMultigrid = false;
+    int mSmootherSweeps = 0;
+    double mSmootherDamping = 0.0;
     bool mConverged = false;
     std::vector<int> mBcDofs;
     std::vector<double> mBcValues;
