@@ -1110,10 +1110,11 @@ __global__ __launch_bounds__(BX * BY * BZ * kJ3Slots) void k_grid_jacobian3(Grid
         // stores drain
 #pragma unroll
         for (int i = 0; i < 3; ++i) q[i] = row16_sum(q[i]);
-        if (slot == kDiagSlot && ri < g.d0 && rj < g.d1 && rk < g.d2) {
+        // lanes 0, 1, 2 of the row write its three components: a wave's
+        // four rows are consecutive nodes, so one store of 96 bytes
+        if (slot < 3 && ri < g.d0 && rj < g.d1 && rk < g.d2) {
             const long o = 3 * (((long)ri * g.d1 + rj) * g.d2 + rk);
-#pragma unroll
-            for (int i = 0; i < 3; ++i) R[o + i] = q[i];
+            R[o + slot] = slot == 0 ? q[0] : (slot == 1 ? q[1] : q[2]);
         }
     }
 }
