@@ -11,7 +11,7 @@ BIN = HERE / "cpp" / "reference_cases"
 
 CASES = ["ElastostaticResidual3D", "InternalElasticEnergy3D", "ThermostaticResidual3D", "InternalThermalEnergy3D",
          "Compliance", "ElastoPlasticity_ElasticSolution3D", "ElastoPlasticity_CriterionTest_3D", "J2Worksets",
-         "ExodusMesh", "AnalyzeApp"]
+         "ExodusMesh", "AnalyzeApp", "AnalyzeAppPlasticity"]
 
 
 @pytest.fixture(scope="module")

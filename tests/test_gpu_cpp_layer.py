@@ -10,6 +10,8 @@ its own tolerances and against the oracle:
   against a direct solve and the reference's composite gradient formula;
 * ElastoPlasticity ElasticSolution3D and CriterionTest_3D goldens through
   PlasticityProblem with 'Time Dependent' Dirichlet functions;
+* the Plato Engine operations (AnalyzeApp) on the compliance and plastic-work
+  decks, against the Problem calls bit for bit;
 * the J2 local and global functions at a converged plastic state against the
   oracle (1e-12).
 """
@@ -163,6 +165,21 @@ def test_analyze_app_operations(results):
     for k in ("unknown_operation_throws", "undefined_parameter_throws", "write_output_throws",
               "reassigned_name_throws"):
         assert a[k] == 1.0, k
+
+
+def test_analyze_app_plasticity(results):
+    """The reference's plastic-work criterion deck through the Plato Engine
+    operations (AnalyzeApp, Physics 'Plasticity'): ComputeCriterion gives the
+    value and adjoint gradient of the PlasticityProblem calls bit for bit (and
+    so the reference's golden gradient), the solution exports by the mixed
+    form's dof names, and the coordinate gradient raises an error."""
+    a, o = results["AnalyzeAppPlasticity"], results["ElastoPlasticity_CriterionTest_3D"]
+    assert a["objective_value"] == o["value"]
+    assert np.array_equal(a["objective_gradient"], o["gradient"])
+    state = np.asarray(o["final_state"]).reshape(-1, 4)
+    for d, name in enumerate(("displacement X", "displacement Y", "displacement Z", "pressure")):
+        assert np.array_equal(a[name], state[:, d])
+    assert a["gradient_x_throws"] == 1.0
 
 
 def test_j2_elastic_solution_golden(results):
