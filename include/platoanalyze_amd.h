@@ -281,7 +281,10 @@ typedef struct {
     int pre_sweeps;       /* block-Jacobi sweeps before the coarse correction */
     int post_sweeps;      /* ... and after */
     double omega;         /* Jacobi damping (0: 0.6) */
-    int max_coarse_nodes; /* limit of the dense coarsest level (0: 1000) */
+    int max_coarse_nodes; /* limit of the dense coarsest level (0: 2000) */
+    int symmetric;        /* 1: the matrices given to setup are symmetric, block (m, n) the transpose of
+                             (n, m); the fine level then keeps half of its block stencil (the elliptic
+                             problems set this themselves) */
 } pa_multigrid_options;
 
 typedef struct pa_multigrid pa_multigrid;

@@ -51,7 +51,7 @@ PRECOND_MULTIGRID = 3
 
 class MultigridOptions(ctypes.Structure):
     _fields_ = [("max_levels", ctypes.c_int), ("pre_sweeps", ctypes.c_int), ("post_sweeps", ctypes.c_int),
-                ("omega", ctypes.c_double), ("max_coarse_nodes", ctypes.c_int)]
+                ("omega", ctypes.c_double), ("max_coarse_nodes", ctypes.c_int), ("symmetric", ctypes.c_int)]
 
 
 class ProblemOptions(ctypes.Structure):
@@ -717,8 +717,8 @@ class Multigrid:
     """Geometric multigrid preconditioner on a box mesh (FP32 V-cycles)."""
 
     def __init__(self, mesh, nx, ny, nz, dpn, *, max_levels=0, pre_sweeps=2, post_sweeps=2, omega=0.6,
-                 max_coarse_nodes=1000):
-        opts = MultigridOptions(max_levels, pre_sweeps, post_sweeps, omega, max_coarse_nodes)
+                 max_coarse_nodes=1000, symmetric=False):
+        opts = MultigridOptions(max_levels, pre_sweeps, post_sweeps, omega, max_coarse_nodes, int(symmetric))
         h = ctypes.c_void_p()
         _check(lib.pa_multigrid_create(mesh.handle, nx, ny, nz, dpn, ctypes.byref(opts), ctypes.byref(h)))
         self.handle, self.mesh, self.dpn = h, mesh, dpn

@@ -5,8 +5,11 @@
 // (fine vertex i takes weight 1 from the coarse vertex at i/2 when i is even,
 // 1/2 from each neighbour when i is odd, per direction).  Coarse operators
 // are Galerkin products A_{l+1} = P^T A_l P, computed on the device and
-// stored as 27-point block stencils; the finest level uses the assembled
-// block-CRS matrix itself (its Dirichlet dofs masked out of P).  Smoother:
+// stored as 27-point block stencils; the finest level is an FP32 copy of the
+// assembled block-CRS matrix laid out as a stencil over the offsets the mesh
+// graph holds (15 on Kuhn grids; symmetric operators store the diagonal and
+// the 7 blocks towards larger offsets only), its Dirichlet dofs masked out of
+// P.  Smoother:
 // damped block Jacobi, symmetric pre/post sweeps, so the V-cycle is an SPD
 // preconditioner for CG.  Coarsest level: dense inverse (FP64 Gauss-Jordan in
 // one workgroup).  Level matrices and vectors are FP32.
@@ -15,6 +18,7 @@
 #include <cmath>
 #include <vector>
 
+#include "pa_cell.h"
 #include "pa_internal.h"
 
 namespace pa {
@@ -100,16 +104,67 @@ __device__ inline void block_inverse(const double* a, float* out) {
     }
 }
 
-// ----------------------------------------------------------- fine level (CRS)
+// ------------------------------------------------------- fine level (stencil)
+// The fine matrix as a block stencil over the S offsets (in stencil-index
+// order) that the mesh graph holds.  Stored slot t of node n holds the block
+// A(n, n + offset(t)); with a symmetric operator only the diagonal and the
+// offsets of larger stencil index are stored, and the offset d < 0 is read
+// from the neighbour's block A(n + d, n), transposed.
+//
+// The smoother gives each node a quad of lanes; lane l takes the offsets
+// l, l + 4, ...  The description of offset l + 4m is 16 bits of word
+// lane[m][l / 2]: dx + 1, dy + 1, dz + 1 (2 bits each), read transposed
+// (bit 6), present (bit 7), stored slot read (bits 8-12).
+constexpr int kFsMaxSlots = 7;  // offsets per lane: 27 over 4 lanes
+struct FineStencil {
+    int S, SS;                       // offsets in the graph; stored blocks per node
+    unsigned lane[kFsMaxSlots][2];   // per-lane offset descriptions
+    unsigned ssig[6];                // stencil index of stored slot t, 5 bits each, six per word
+    int diag;                        // stored slot of the diagonal block
+    double rny, rnz;                 // 1 / (ny + 1), 1 / (nz + 1): node coordinates by products
+};
+
+// The fine CRS matrix (FP64 or FP32) into the stencil: thread per (node,
+// stored slot); a slot the node's row has no entry for gets a zero block.
 template <int D, class VT>
-__global__ void k_diag_inv_crs(int N, const int* __restrict__ diag, const VT* __restrict__ vals,
-                               float* __restrict__ dinv) {
+__global__ void k_crs_to_fst(Grid g, FineStencil fs, const int* __restrict__ rowmap, const int* __restrict__ colidx,
+                             const VT* __restrict__ vals, float* __restrict__ A) {
+    constexpr int BS = D * D;
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long)g.nodes() * fs.SS) return;
+    const int n = (int)(q / fs.SS), t = (int)(q % fs.SS);
+    const int sig = (int)((fs.ssig[t / 6] >> ((t % 6) * 5)) & 31u);
+    int i, j, k;
+    g.ijk(n, i, j, k);
+    const int ii = i + sig / 9 - 1, jj = j + (sig / 3) % 3 - 1, kk = k + sig % 3 - 1;
+    int e = -1;
+    if (g.in(ii, jj, kk)) {
+        const int c = g.id(ii, jj, kk);
+        for (int f = rowmap[n]; f < rowmap[n + 1]; ++f)
+            if (colidx[f] == c) {
+                e = f;
+                break;
+            }
+    }
+    float* dst = A + q * BS;
+#pragma unroll
+    for (int r = 0; r < BS; ++r) dst[r] = e >= 0 ? (float)vals[(long)e * BS + r] : 0.0f;
+}
+
+template <int D>
+__global__ void k_diag_inv_fst(int N, FineStencil fs, const float* __restrict__ A, float* __restrict__ dinv) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     double a[D * D];
 #pragma unroll
-    for (int q = 0; q < D * D; ++q) a[q] = (double)vals[(long)diag[n] * D * D + q];
+    for (int q = 0; q < D * D; ++q) a[q] = (double)A[((long)n * fs.SS + fs.diag) * D * D + q];
     block_inverse<D>(a, dinv + (long)n * D * D);
+}
+
+// v from lane l ^ 1 / l ^ 2 of the quad (DPP quad permutations)
+template <int CTRL>
+__device__ inline float quad_dpp(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, false));
 }
 
 template <int D>
@@ -122,55 +177,95 @@ __global__ void k_diag_inv_st(int N, const float* __restrict__ A, float* __restr
     block_inverse<D>(a, dinv + (long)n * D * D);
 }
 
-// y = b - A x on the fine CRS matrix (16 lanes per row); masked dofs get 0.
-// zero_x: x is zero (y = b).  With omega != 0 the result is the Jacobi update
-// x_out = x + omega Dinv (b - A x) instead.
-template <int D, class VT>
-__global__ __launch_bounds__(kT) void k_crs_resid_jac(int N, const int* __restrict__ rowmap,
-                                                      const int* __restrict__ colidx, const VT* __restrict__ vals,
-                                                      const float* __restrict__ dinv, const unsigned char* __restrict__ mask,
+// y = b - A x on the fine stencil, a quad of lanes per node, NS offsets per
+// lane; masked dofs get 0.  zero_x: x is zero (y = b).  With omega != 0 the
+// result is the Jacobi update x_out = x + omega Dinv (b - A x) instead.
+// Branch-free: an offset that is absent or points out of the grid reads the
+// node's own block and x and adds zero, so every load of the lane issues at
+// once.  SYM: some offsets are read transposed.  Lane l < D finishes dof l.
+template <int D, int NS, bool SYM>
+__global__ __launch_bounds__(kT) void k_fst_resid_jac(Grid g, FineStencil fs, const float* __restrict__ A,
+                                                      const float* __restrict__ dinv,
+                                                      const unsigned char* __restrict__ mask,
                                                       const float* __restrict__ b, const float* __restrict__ x,
                                                       float* __restrict__ out, float omega, int zero_x) {
-    constexpr int G = 16, ROWS = kT / G;
-    const int grp = threadIdx.x / G, lane = threadIdx.x % G;
-    const int row = blockIdx.x * ROWS + grp;
-    if (row >= N) return;
+    constexpr int ROWS = kT / 4, BS = D * D;
+    const int l = threadIdx.x & 3;
+    const int n = xcd_block() * ROWS + (threadIdx.x >> 2);
+    if (n >= g.nodes()) return;
+    const int nz1 = g.nz + 1, nyz = (g.ny + 1) * nz1;
+    // node coordinates: (n + 1/2) / d rounded down is exact in FP64
+    const int jk = (int)(((double)n + 0.5) * fs.rnz);
+    const int k = n - jk * nz1;
+    const int i = (int)(((double)jk + 0.5) * fs.rny);
+    const int j = jk - i * (g.ny + 1);
+    // the node's dof min(l, D - 1) for the update, in flight with the stencil
+    const int ld = l < D ? l : D - 1;
+    const float bl = b[(long)n * D + ld], xl = x[(long)n * D + ld];
+    const unsigned ml = mask[(long)n * D + ld];
+    float dl[D];
+#pragma unroll
+    for (int q = 0; q < D; ++q) dl[q] = dinv[((long)n * D + ld) * D + q];
     float acc[D];
 #pragma unroll
-    for (int i = 0; i < D; ++i) acc[i] = 0.0f;
+    for (int p = 0; p < D; ++p) acc[p] = 0.0f;
     if (!zero_x) {
-        for (int e = rowmap[row] + lane; e < rowmap[row + 1]; e += G) {
-            const long c = colidx[e];
-            const VT* v = vals + (long)e * D * D;
+        float blk[NS][BS], xc[NS][D];
+        bool ok[NS], tr[NS];
 #pragma unroll
-            for (int i = 0; i < D; ++i)
+        for (int m = 0; m < NS; ++m) {
+            const unsigned w = __builtin_amdgcn_readfirstlane(fs.lane[m][0]);
+            const unsigned w1 = __builtin_amdgcn_readfirstlane(fs.lane[m][1]);
+            const unsigned info = ((l & 2) ? w1 : w) >> ((l & 1) * 16);
+            const int ii = i + (int)(info & 3u) - 1, jj = j + (int)((info >> 2) & 3u) - 1,
+                      kk = k + (int)((info >> 4) & 3u) - 1;
+            ok[m] = ((info >> 7) & 1u) && ii >= 0 && ii <= g.nx && jj >= 0 && jj <= g.ny && kk >= 0 && kk <= g.nz;
+            tr[m] = SYM && ((info >> 6) & 1u);
+            const long c = ok[m] ? ((long)ii * nyz + jj * nz1 + kk) : (long)n;
+            const long src = ok[m] ? (long)((info >> 8) & 31u) : (long)fs.diag;
+            const float* v = A + ((tr[m] ? c : (long)n) * fs.SS + src) * BS;
 #pragma unroll
-                for (int j = 0; j < D; ++j) acc[i] += (float)v[i * D + j] * x[c * D + j];
+            for (int q = 0; q < BS; ++q) blk[m][q] = v[q];
+#pragma unroll
+            for (int q = 0; q < D; ++q) xc[m][q] = x[c * D + q];
         }
 #pragma unroll
-        for (int i = 0; i < D; ++i)
+        for (int m = 0; m < NS; ++m) {
 #pragma unroll
-            for (int o = G / 2; o > 0; o >>= 1) acc[i] += __shfl_xor(acc[i], o, G);
-    }
-    if (lane == 0) {
-        float r[D];
+            for (int p = 0; p < D; ++p) {
+                float yn = 0.0f, yt = 0.0f;
 #pragma unroll
-        for (int i = 0; i < D; ++i) r[i] = b[(long)row * D + i] - acc[i];
-        if (omega != 0.0f) {
-            const float* m = dinv + (long)row * D * D;
-#pragma unroll
-            for (int i = 0; i < D; ++i) {
-                float s = 0.0f;
-#pragma unroll
-                for (int j = 0; j < D; ++j) s += m[i * D + j] * r[j];
-                const float xo = zero_x ? 0.0f : x[(long)row * D + i];
-                out[(long)row * D + i] = mask[(long)row * D + i] ? xo + omega * s : 0.0f;
+                for (int q = 0; q < D; ++q) {
+                    yn = fmaf(blk[m][p * D + q], xc[m][q], yn);
+                    if constexpr (SYM) yt = fmaf(blk[m][q * D + p], xc[m][q], yt);
+                }
+                const float y = (SYM && tr[m]) ? yt : yn;
+                acc[p] += ok[m] ? y : 0.0f;
             }
-        } else {
+        }
+        // sum over the quad: lanes l ^ 1, then l ^ 2
 #pragma unroll
-            for (int i = 0; i < D; ++i) out[(long)row * D + i] = mask[(long)row * D + i] ? r[i] : 0.0f;
+        for (int p = 0; p < D; ++p) {
+            acc[p] += quad_dpp<0xb1>(acc[p]);  // quad_perm [1, 0, 3, 2]
+            acc[p] += quad_dpp<0x4e>(acc[p]);  // quad_perm [2, 3, 0, 1]
         }
     }
+    // r_ld on lane l, then r_q from lane q of the quad for the block row
+    float al = acc[0];
+#pragma unroll
+    for (int p = 1; p < D; ++p) al = ld == p ? acc[p] : al;
+    const float rl = bl - al;
+    float t = 0.0f;
+    if (omega != 0.0f) {
+        float rq[D];
+        rq[0] = quad_dpp<0x00>(rl);  // quad_perm [0, 0, 0, 0]
+        if constexpr (D > 1) rq[1] = quad_dpp<0x55>(rl);
+        if constexpr (D > 2) rq[2] = quad_dpp<0xaa>(rl);
+        if constexpr (D > 3) rq[3] = quad_dpp<0xff>(rl);
+#pragma unroll
+        for (int q = 0; q < D; ++q) t += dl[q] * rq[q];
+    }
+    if (l < D) out[(long)n * D + l] = ml ? (omega != 0.0f ? (zero_x ? 0.0f : xl) + omega * t : rl) : 0.0f;
 }
 
 // ------------------------------------------------------ coarse levels (stencil)
@@ -475,10 +570,6 @@ __global__ void k_to_f32(long n, const double* __restrict__ a, float* __restrict
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = (float)a[i];
 }
-__global__ void k_vals_to_f32(long n, const double* __restrict__ a, float* __restrict__ b) {
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-        b[i] = (float)a[i];
-}
 __global__ void k_to_f64(long n, const float* __restrict__ a, double* __restrict__ b) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = (double)a[i];
@@ -495,13 +586,13 @@ struct pa_multigrid {
     int dpn = 3;
     pa_multigrid_options opt{};
     std::vector<pa::Grid> grids;  // grids[0] is the finest
-    pa::DevBuf<int> d_diag;
     pa::DevBuf<unsigned char> mask;
     std::vector<pa::DevBuf<float>*> A, dinv, b, x, t;  // per level (A[0] unused: CRS)
     pa::DevBuf<double> dense, dense_inv;
-    pa::DevBuf<float> fine32;  // FP32 copy of an FP64 fine matrix
-    const void* fine_vals = nullptr;
-    int fine_prec = PA_FP32;
+    unsigned graph_offsets = 0;  // bit sidx: the mesh graph holds that offset
+    bool symmetric = false;      // the fine operator is symmetric: half stencil
+    pa::FineStencil fs{};
+    pa::DevBuf<float> fine;  // the fine matrix as an FP32 stencil
     bool ready = false;
     ~pa_multigrid() {
         for (auto* v : {&A, &dinv, &b, &x, &t})
@@ -521,7 +612,6 @@ void vcycle(pa_multigrid* mg, int l, hipStream_t st) {
     float* x = mg->x[l]->p;
     float* t = mg->t[l]->p;
     const float w = (float)mg->opt.omega;
-    const pa_mesh* m = mg->mesh;
     if (l == L - 1) {
         k_dense_apply<<<nblk(n), kT, 0, st>>>((int)n, mg->dense_inv.p, b, x);
         chk();
@@ -529,14 +619,18 @@ void vcycle(pa_multigrid* mg, int l, hipStream_t st) {
     }
     auto sweep = [&](const float* xin, float* xout, float om, int zero) {
         if (l == 0) {
-            if (mg->fine_prec == PA_FP64)
-                k_crs_resid_jac<D, double><<<nblk(g.nodes(), kT / 16), kT, 0, st>>>(
-                    g.nodes(), m->d_rowmap.p, m->d_colidx.p, (const double*)mg->fine_vals, mg->dinv[0]->p,
-                    mg->mask.p, b, xin, xout, om, zero);
-            else
-                k_crs_resid_jac<D, float><<<nblk(g.nodes(), kT / 16), kT, 0, st>>>(
-                    g.nodes(), m->d_rowmap.p, m->d_colidx.p, (const float*)mg->fine_vals, mg->dinv[0]->p,
-                    mg->mask.p, b, xin, xout, om, zero);
+            auto run = [&](auto kernel) {
+                kernel<<<nblk(g.nodes(), kT / 4), kT, 0, st>>>(g, mg->fs, mg->fine.p, mg->dinv[0]->p, mg->mask.p, b,
+                                                               xin, xout, om, zero);
+            };
+            const bool sym = mg->fs.SS < mg->fs.S;
+            if (mg->fs.S <= 16) {
+                if (sym) run(k_fst_resid_jac<D, 4, true>);
+                else run(k_fst_resid_jac<D, 4, false>);
+            } else {
+                if (sym) run(k_fst_resid_jac<D, kFsMaxSlots, true>);
+                else run(k_fst_resid_jac<D, kFsMaxSlots, false>);
+            }
         } else {
             k_st_resid_jac<D><<<nblk(g.nodes(), kT / 32), kT, 0, st>>>(g, mg->A[l]->p, mg->dinv[l]->p, b, xin, xout, om, zero);
         }
@@ -568,12 +662,51 @@ void vcycle(pa_multigrid* mg, int l, hipStream_t st) {
     }
 }
 
+// The lanes and stored slots of the fine stencil from the offsets of the
+// mesh graph (bit sidx set when some row has that neighbour offset).
+FineStencil make_fine_stencil(const Grid& g, unsigned offsets, bool symmetric) {
+    FineStencil f{};
+    int lanes[27], stored[27], slot_of[27];
+    f.S = 0;
+    for (int q = 0; q < 27; ++q)
+        if ((offsets >> q) & 1u) lanes[f.S++] = q;
+    // a half stencil needs the offsets in pairs (the graph of a symmetric
+    // operator is symmetric)
+    bool pairs = true;
+    for (int q = 0; q < 27; ++q) pairs = pairs && (((offsets >> q) & 1u) == ((offsets >> (26 - q)) & 1u));
+    const bool half = symmetric && pairs;
+    f.SS = 0;
+    for (int q = 0; q < 27; ++q) slot_of[q] = -1;
+    for (int s = 0; s < f.S; ++s)
+        if (!half || lanes[s] >= 13) {
+            slot_of[lanes[s]] = f.SS;
+            stored[f.SS++] = lanes[s];
+        }
+    PA_REQUIRE(slot_of[13] >= 0, PA_ERR_ARG, "multigrid: the mesh graph has no diagonal entries");
+    f.diag = slot_of[13];
+    for (int s = 0; s < f.S; ++s) {
+        const int q = lanes[s];
+        const bool tr = half && q < 13;
+        const unsigned info = (unsigned)(q / 9) | (unsigned)((q / 3) % 3) << 2 | (unsigned)(q % 3) << 4 |
+                              (tr ? 1u << 6 : 0u) | 1u << 7 | (unsigned)slot_of[tr ? 26 - q : q] << 8;
+        const int m = s / 4, l = s % 4;
+        f.lane[m][l / 2] |= info << ((l % 2) * 16);
+    }
+    for (int t = 0; t < f.SS; ++t) f.ssig[t / 6] |= (unsigned)stored[t] << ((t % 6) * 5);
+    f.rny = 1.0 / (g.ny + 1);
+    f.rnz = 1.0 / (g.nz + 1);
+    return f;
+}
+
 template <int D, class VT>
 void setup_impl(pa_multigrid* mg, const VT* vals, hipStream_t st) {
     const pa_mesh* m = mg->mesh;
     const int L = (int)mg->grids.size();
     const Grid g0 = mg->grids[0];
-    k_diag_inv_crs<D, VT><<<nblk(g0.nodes()), kT, 0, st>>>(g0.nodes(), mg->d_diag.p, vals, mg->dinv[0]->p);
+    const long nq = (long)g0.nodes() * mg->fs.SS;
+    k_crs_to_fst<D, VT><<<nblk(nq), kT, 0, st>>>(g0, mg->fs, m->d_rowmap.p, m->d_colidx.p, vals, mg->fine.p);
+    chk();
+    k_diag_inv_fst<D><<<nblk(g0.nodes()), kT, 0, st>>>(g0.nodes(), mg->fs, mg->fine.p, mg->dinv[0]->p);
     chk();
     for (int l = 1; l < L; ++l) {
         const Grid gf = mg->grids[l - 1], gc = mg->grids[l];
@@ -600,6 +733,10 @@ void setup_impl(pa_multigrid* mg, const VT* vals, hipStream_t st) {
 }
 
 }  // namespace
+
+// The operator handed to setup is symmetric (A(m, n) = A(n, m)^T): the fine
+// level stores half of its stencil.
+void multigrid_set_symmetric(pa_multigrid* mg, bool symmetric) { mg->symmetric = symmetric; }
 
 // used by the PCG of pa_solver.hip
 void multigrid_apply(pa_multigrid* mg, const double* r, double* z, hipStream_t st) {
@@ -629,6 +766,7 @@ int pa_multigrid_create(const pa_mesh* mesh, int nx, int ny, int nz, int dpn, co
         PA_REQUIRE(nx > 0 && ny > 0 && nz > 0 && g0.nodes() == mesh->N, PA_ERR_ARG,
                    "pa_multigrid_create: grid does not match the mesh node count");
         // the graph must be a vertex-neighbourhood graph of this grid
+        unsigned offsets = 0;
         for (int r = 0; r < mesh->N; ++r) {
             int i, j, k;
             g0.ijk(r, i, j, k);
@@ -637,11 +775,14 @@ int pa_multigrid_create(const pa_mesh* mesh, int nx, int ny, int nz, int dpn, co
                 g0.ijk(mesh->h_colidx[e], a, b, c);
                 PA_REQUIRE(std::abs(a - i) <= 1 && std::abs(b - j) <= 1 && std::abs(c - k) <= 1, PA_ERR_ARG,
                            "pa_multigrid_create: mesh graph is not a structured-grid neighbourhood graph");
+                offsets |= 1u << ((a - i + 1) * 9 + (b - j + 1) * 3 + (c - k + 1));
             }
         }
         auto* mg = new pa_multigrid();
         try {
             mg->mesh = mesh;
+            mg->graph_offsets = offsets;
+            mg->symmetric = opt->symmetric != 0;
             mg->dpn = dpn;
             mg->opt = *opt;
             if (mg->opt.omega <= 0.0) mg->opt.omega = 0.6;
@@ -656,11 +797,6 @@ int pa_multigrid_create(const pa_mesh* mesh, int nx, int ny, int nz, int dpn, co
             PA_REQUIRE(mg->grids.size() >= 2, PA_ERR_ARG, "pa_multigrid_create: the grid cannot be coarsened (odd cell counts)");
             PA_REQUIRE(mg->grids.back().nodes() <= maxc, PA_ERR_LIMIT,
                        "pa_multigrid_create: coarsest level too large for the dense solve");
-            std::vector<int> diag(mesh->N);
-            for (int i = 0; i < mesh->N; ++i)
-                for (int e = mesh->h_rowmap[i]; e < mesh->h_rowmap[i + 1]; ++e)
-                    if (mesh->h_colidx[e] == i) diag[i] = e;
-            mg->d_diag.upload(diag.data(), diag.size());
             const int L = (int)mg->grids.size();
             const int bs = dpn * dpn;
             for (int l = 0; l < L; ++l) {
@@ -713,21 +849,19 @@ int pa_multigrid_setup(pa_multigrid* mg, const void* vals, int precision, int nb
             pa::k_mask_bc<<<pa::nblk(nbc), pa::kT, 0, st>>>(nbc, bc_dofs, mg->mask.p);
             pa::chk();
         }
-        mg->fine_vals = vals;
-        mg->fine_prec = precision;
-        if (precision == PA_FP64) {
-            // the smoother reads an FP32 copy of the fine matrix (half the bytes per sweep)
-            const long nv = (long)mg->mesh->nnzb * mg->dpn * mg->dpn;
-            if (mg->fine32.n != (size_t)nv) mg->fine32.alloc(nv);
-            pa::k_vals_to_f32<<<pa::nblk(nv), pa::kT, 0, st>>>(nv, (const double*)vals, mg->fine32.p);
-            pa::chk();
-            mg->fine_vals = mg->fine32.p;
-            mg->fine_prec = PA_FP32;
-        }
-        const float* v32 = (const float*)mg->fine_vals;
-        if (mg->dpn == 3) pa::setup_impl<3, float>(mg, v32, st);
-        else if (mg->dpn == 4) pa::setup_impl<4, float>(mg, v32, st);
-        else pa::setup_impl<1, float>(mg, v32, st);
+        // the smoother reads an FP32 stencil copy of the fine matrix
+        mg->fs = pa::make_fine_stencil(mg->grids[0], mg->graph_offsets, mg->symmetric);
+        const long nf = (long)mg->mesh->N * mg->fs.SS * mg->dpn * mg->dpn;
+        if (mg->fine.n != (size_t)nf) mg->fine.alloc(nf);
+        auto run = [&](auto tag) {
+            using VT = decltype(tag);
+            const VT* v = (const VT*)vals;
+            if (mg->dpn == 3) pa::setup_impl<3, VT>(mg, v, st);
+            else if (mg->dpn == 4) pa::setup_impl<4, VT>(mg, v, st);
+            else pa::setup_impl<1, VT>(mg, v, st);
+        };
+        if (precision == PA_FP64) run(0.0);
+        else run(0.0f);
         mg->ready = true;
     });
 }

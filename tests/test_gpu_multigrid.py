@@ -5,7 +5,10 @@
   Dirichlet dofs masked out (FP32 storage: 1e-5 relative);
 * MG-preconditioned CG solves the SIMP problem to the FP64 tolerance and
   agrees with the block-Jacobi solution; it needs far fewer iterations;
-* the V-cycle is symmetric (x.M y == y.M x), as CG requires.
+* the V-cycle is symmetric (x.M y == y.M x), as CG requires;
+* with a symmetric operator the fine level's half stencil (the diagonal and
+  the blocks towards larger offsets, the others read transposed from the
+  neighbour) gives the full stencil's V-cycle bit for bit.
 """
 import numpy as np
 import pytest
@@ -126,6 +129,38 @@ def test_vcycle_is_symmetric(pa):
     My = mg.apply(pa.DeviceArray.from_numpy(y), pa.DeviceArray(n)).numpy()
     assert abs(y @ Mx - x @ My) <= 1e-4 * abs(y @ Mx)
     assert x @ Mx > 0 and y @ My > 0
+
+
+@pytest.mark.parametrize("physics", ["Mechanics", "Thermal"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_half_stencil_vcycle_bitwise(pa, physics, precision):
+    """The assembled Kuhn-grid Jacobians are symmetric bit for bit, and so
+    are they after the Dirichlet elimination: the half fine stencil then
+    reads the same blocks as the full one, transposed, and the V-cycles are
+    equal bit for bit (on a second setup as well)."""
+    dims = (6, 4, 4)
+    print(f"case: half-stencil V-cycle, {physics} {precision}, {dims}", flush=True)
+    mesh, vf, sf, z, dofs = elastic_setup(pa, dims, 9)
+    dpn = 3
+    if physics == "Thermal":
+        vf = pa.VectorFunction(mesh, "Thermal", conductivity=1.0, penalty_exponent=3.0, penalty_min=1e-9)
+        dofs = mesh.node_set("x-").astype(np.int32)
+        dpn = 1
+    n = dpn * mesh.num_nodes
+    rng = np.random.default_rng(4)
+    for rep in range(2):
+        K = vf.gradient_u(np.zeros(n), z * (1.0 + 0.1 * rep), precision=precision)
+        rhs = pa.DeviceArray(n)
+        rhs.zero()
+        pa.apply_block_constraints(K, rhs, dofs, np.zeros(dofs.size))
+        out = []
+        for sym in (False, True):
+            mg = pa.Multigrid(mesh, *dims, dpn, symmetric=sym)
+            mg.setup(K, dofs)
+            r = rng.standard_normal(n) if not sym else r
+            out.append(mg.apply(pa.DeviceArray.from_numpy(r), pa.DeviceArray(n)).numpy())
+        assert np.all(np.isfinite(out[0])) and np.any(out[0] != 0)
+        assert np.array_equal(out[0], out[1])
 
 
 @pytest.mark.parametrize("mode", ["fp64", "fp32"])
