@@ -111,14 +111,16 @@ __device__ inline void block_inverse(const double* a, float* out) {
 // offsets of larger stencil index are stored, and the offset d < 0 is read
 // from the neighbour's block A(n + d, n), transposed.
 //
-// The smoother gives each node a quad of lanes; lane l takes the offsets
-// l, l + 4, ...  The description of offset l + 4m is 16 bits of word
-// lane[m][l / 2]: dx + 1, dy + 1, dz + 1 (2 bits each), read transposed
-// (bit 6), present (bit 7), stored slot read (bits 8-12).
-constexpr int kFsMaxSlots = 7;  // offsets per lane: 27 over 4 lanes
+// The smoother gives each node G = 4 lanes (at most 16 offsets) or 8; lane l
+// takes the offsets l, l + G, ...  The description of offset l + Gm is 16
+// bits of word lane[m][l / 2]: dx + 1, dy + 1, dz + 1 (2 bits each), read
+// transposed (bit 6), present (bit 7), stored slot read (bits 8-12).
+constexpr int kFsSlots = 4;  // offsets per lane
 struct FineStencil {
-    int S, SS;                       // offsets in the graph; stored blocks per node
-    unsigned lane[kFsMaxSlots][2];   // per-lane offset descriptions
+    int S, SS;                    // offsets in the graph; stored blocks per node
+    int G;                        // lanes per node
+    int half;                     // some offsets are read transposed
+    unsigned lane[kFsSlots][4];   // per-lane offset descriptions
     unsigned ssig[6];                // stencil index of stored slot t, 5 bits each, six per word
     int diag;                        // stored slot of the diagonal block
     double rny, rnz;                 // 1 / (ny + 1), 1 / (nz + 1): node coordinates by products
@@ -177,21 +179,22 @@ __global__ void k_diag_inv_st(int N, const float* __restrict__ A, float* __restr
     block_inverse<D>(a, dinv + (long)n * D * D);
 }
 
-// y = b - A x on the fine stencil, a quad of lanes per node, NS offsets per
-// lane; masked dofs get 0.  zero_x: x is zero (y = b).  With omega != 0 the
-// result is the Jacobi update x_out = x + omega Dinv (b - A x) instead.
-// Branch-free: an offset that is absent or points out of the grid reads the
-// node's own block and x and adds zero, so every load of the lane issues at
-// once.  SYM: some offsets are read transposed.  Lane l < D finishes dof l.
-template <int D, int NS, bool SYM>
+// y = b - A x on a level's stencil, G lanes per node, kFsSlots offsets per
+// lane; masked dofs get 0 (mask null: none).  zero_x: x is zero (y = b).
+// With omega != 0 the result is the Jacobi update x_out = x + omega Dinv
+// (b - A x) instead.  Branch-free: an offset that is absent or points out of
+// the grid reads the node's own block and x and adds zero, so every load of
+// the lane issues at once.  SYM: some offsets are read transposed.  Lane
+// l < D finishes dof l.
+template <int D, int G, bool SYM>
 __global__ __launch_bounds__(kT) void k_fst_resid_jac(Grid g, FineStencil fs, const float* __restrict__ A,
                                                       const float* __restrict__ dinv,
                                                       const unsigned char* __restrict__ mask,
                                                       const float* __restrict__ b, const float* __restrict__ x,
                                                       float* __restrict__ out, float omega, int zero_x) {
-    constexpr int ROWS = kT / 4, BS = D * D;
-    const int l = threadIdx.x & 3;
-    const int n = xcd_block() * ROWS + (threadIdx.x >> 2);
+    constexpr int ROWS = kT / G, BS = D * D, NS = kFsSlots;
+    const int l = threadIdx.x & (G - 1);
+    const int n = xcd_block() * ROWS + threadIdx.x / G;
     if (n >= g.nodes()) return;
     const int nz1 = g.nz + 1, nyz = (g.ny + 1) * nz1;
     // node coordinates: (n + 1/2) / d rounded down is exact in FP64
@@ -202,7 +205,7 @@ __global__ __launch_bounds__(kT) void k_fst_resid_jac(Grid g, FineStencil fs, co
     // the node's dof min(l, D - 1) for the update, in flight with the stencil
     const int ld = l < D ? l : D - 1;
     const float bl = b[(long)n * D + ld], xl = x[(long)n * D + ld];
-    const unsigned ml = mask[(long)n * D + ld];
+    const unsigned ml = mask ? mask[(long)n * D + ld] : 1u;
     float dl[D];
 #pragma unroll
     for (int q = 0; q < D; ++q) dl[q] = dinv[((long)n * D + ld) * D + q];
@@ -214,9 +217,15 @@ __global__ __launch_bounds__(kT) void k_fst_resid_jac(Grid g, FineStencil fs, co
         bool ok[NS], tr[NS];
 #pragma unroll
         for (int m = 0; m < NS; ++m) {
-            const unsigned w = __builtin_amdgcn_readfirstlane(fs.lane[m][0]);
+            const unsigned w0 = __builtin_amdgcn_readfirstlane(fs.lane[m][0]);
             const unsigned w1 = __builtin_amdgcn_readfirstlane(fs.lane[m][1]);
-            const unsigned info = ((l & 2) ? w1 : w) >> ((l & 1) * 16);
+            unsigned w = (l & 2) ? w1 : w0;
+            if constexpr (G == 8) {
+                const unsigned w2 = __builtin_amdgcn_readfirstlane(fs.lane[m][2]);
+                const unsigned w3 = __builtin_amdgcn_readfirstlane(fs.lane[m][3]);
+                w = (l & 4) ? ((l & 2) ? w3 : w2) : w;
+            }
+            const unsigned info = w >> ((l & 1) * 16);
             const int ii = i + (int)(info & 3u) - 1, jj = j + (int)((info >> 2) & 3u) - 1,
                       kk = k + (int)((info >> 4) & 3u) - 1;
             ok[m] = ((info >> 7) & 1u) && ii >= 0 && ii <= g.nx && jj >= 0 && jj <= g.ny && kk >= 0 && kk <= g.nz;
@@ -243,11 +252,12 @@ __global__ __launch_bounds__(kT) void k_fst_resid_jac(Grid g, FineStencil fs, co
                 acc[p] += ok[m] ? y : 0.0f;
             }
         }
-        // sum over the quad: lanes l ^ 1, then l ^ 2
+        // sum over the node's lanes: l ^ 1, l ^ 2, then the other quad
 #pragma unroll
         for (int p = 0; p < D; ++p) {
             acc[p] += quad_dpp<0xb1>(acc[p]);  // quad_perm [1, 0, 3, 2]
             acc[p] += quad_dpp<0x4e>(acc[p]);  // quad_perm [2, 3, 0, 1]
+            if constexpr (G == 8) acc[p] += quad_dpp<0x141>(acc[p]);  // row_half_mirror
         }
     }
     // r_ld on lane l, then r_q from lane q of the quad for the block row
@@ -266,60 +276,6 @@ __global__ __launch_bounds__(kT) void k_fst_resid_jac(Grid g, FineStencil fs, co
         for (int q = 0; q < D; ++q) t += dl[q] * rq[q];
     }
     if (l < D) out[(long)n * D + l] = ml ? (omega != 0.0f ? (zero_x ? 0.0f : xl) + omega * t : rl) : 0.0f;
-}
-
-// ------------------------------------------------------ coarse levels (stencil)
-template <int D>
-__global__ __launch_bounds__(kT) void k_st_resid_jac(Grid g, const float* __restrict__ A, const float* __restrict__ dinv,
-                                                     const float* __restrict__ b, const float* __restrict__ x,
-                                                     float* __restrict__ out, float omega, int zero_x) {
-    // One 32-lane group per node: lane s < 27 owns stencil block s, i.e. the
-    // neighbour at offset (s/9 - 1, s/3 % 3 - 1, s % 3 - 1); the D partial
-    // products are then summed across the group.  A node's 27 blocks are
-    // contiguous, so the group reads its row of A in one coalesced sweep.
-    constexpr int G = 32, NODES = kT / G;
-    const int grp = threadIdx.x / G, s = threadIdx.x % G;
-    const int n = blockIdx.x * NODES + grp;
-    if (n >= g.nodes()) return;
-    float acc[D];
-#pragma unroll
-    for (int q = 0; q < D; ++q) acc[q] = 0.0f;
-    if (!zero_x) {
-        if (s < 27) {
-            int i, j, k;
-            g.ijk(n, i, j, k);
-            const int ii = i + s / 9 - 1, jj = j + (s / 3) % 3 - 1, kk = k + s % 3 - 1;
-            if (g.in(ii, jj, kk)) {
-                const long c = g.id(ii, jj, kk);
-                const float* v = A + ((long)n * 27 + s) * D * D;
-#pragma unroll
-                for (int p = 0; p < D; ++p)
-#pragma unroll
-                    for (int q = 0; q < D; ++q) acc[p] += v[p * D + q] * x[c * D + q];
-            }
-        }
-#pragma unroll
-        for (int p = 0; p < D; ++p)
-#pragma unroll
-            for (int o = G / 2; o > 0; o >>= 1) acc[p] += __shfl_xor(acc[p], o, G);
-    }
-    if (s != 0) return;
-    float r[D];
-#pragma unroll
-    for (int p = 0; p < D; ++p) r[p] = b[(long)n * D + p] - acc[p];
-    if (omega != 0.0f) {
-        const float* m = dinv + (long)n * D * D;
-#pragma unroll
-        for (int p = 0; p < D; ++p) {
-            float t = 0.0f;
-#pragma unroll
-            for (int q = 0; q < D; ++q) t += m[p * D + q] * r[q];
-            out[(long)n * D + p] = (zero_x ? 0.0f : x[(long)n * D + p]) + omega * t;
-        }
-    } else {
-#pragma unroll
-        for (int p = 0; p < D; ++p) out[(long)n * D + p] = r[p];
-    }
 }
 
 // ------------------------------------------------------------------ transfer
@@ -591,7 +547,7 @@ struct pa_multigrid {
     pa::DevBuf<double> dense, dense_inv;
     unsigned graph_offsets = 0;  // bit sidx: the mesh graph holds that offset
     bool symmetric = false;      // the fine operator is symmetric: half stencil
-    pa::FineStencil fs{};
+    std::vector<pa::FineStencil> fsl;  // stencil of each level's smoother
     pa::DevBuf<float> fine;  // the fine matrix as an FP32 stencil
     bool ready = false;
     ~pa_multigrid() {
@@ -618,21 +574,17 @@ void vcycle(pa_multigrid* mg, int l, hipStream_t st) {
         return;
     }
     auto sweep = [&](const float* xin, float* xout, float om, int zero) {
-        if (l == 0) {
-            auto run = [&](auto kernel) {
-                kernel<<<nblk(g.nodes(), kT / 4), kT, 0, st>>>(g, mg->fs, mg->fine.p, mg->dinv[0]->p, mg->mask.p, b,
-                                                               xin, xout, om, zero);
-            };
-            const bool sym = mg->fs.SS < mg->fs.S;
-            if (mg->fs.S <= 16) {
-                if (sym) run(k_fst_resid_jac<D, 4, true>);
-                else run(k_fst_resid_jac<D, 4, false>);
-            } else {
-                if (sym) run(k_fst_resid_jac<D, kFsMaxSlots, true>);
-                else run(k_fst_resid_jac<D, kFsMaxSlots, false>);
-            }
+        const FineStencil& f = mg->fsl[l];
+        auto run = [&](auto kernel) {
+            kernel<<<nblk(g.nodes(), kT / f.G), kT, 0, st>>>(g, f, l == 0 ? mg->fine.p : mg->A[l]->p, mg->dinv[l]->p,
+                                                             l == 0 ? mg->mask.p : nullptr, b, xin, xout, om, zero);
+        };
+        if (f.G == 4) {
+            if (f.half) run(k_fst_resid_jac<D, 4, true>);
+            else run(k_fst_resid_jac<D, 4, false>);
         } else {
-            k_st_resid_jac<D><<<nblk(g.nodes(), kT / 32), kT, 0, st>>>(g, mg->A[l]->p, mg->dinv[l]->p, b, xin, xout, om, zero);
+            if (f.half) run(k_fst_resid_jac<D, 8, true>);
+            else run(k_fst_resid_jac<D, 8, false>);
         }
         chk();
     };
@@ -684,15 +636,40 @@ FineStencil make_fine_stencil(const Grid& g, unsigned offsets, bool symmetric) {
         }
     PA_REQUIRE(slot_of[13] >= 0, PA_ERR_ARG, "multigrid: the mesh graph has no diagonal entries");
     f.diag = slot_of[13];
+    f.G = f.S <= 4 * kFsSlots ? 4 : 8;
     for (int s = 0; s < f.S; ++s) {
         const int q = lanes[s];
         const bool tr = half && q < 13;
         const unsigned info = (unsigned)(q / 9) | (unsigned)((q / 3) % 3) << 2 | (unsigned)(q % 3) << 4 |
                               (tr ? 1u << 6 : 0u) | 1u << 7 | (unsigned)slot_of[tr ? 26 - q : q] << 8;
-        const int m = s / 4, l = s % 4;
+        const int m = s / f.G, l = s % f.G;
         f.lane[m][l / 2] |= info << ((l % 2) * 16);
     }
     for (int t = 0; t < f.SS; ++t) f.ssig[t / 6] |= (unsigned)stored[t] << ((t % 6) * 5);
+    f.half = half;
+    f.rny = 1.0 / (g.ny + 1);
+    f.rnz = 1.0 / (g.nz + 1);
+    return f;
+}
+
+// A coarse level's smoother stencil: all 27 offsets stored (the Galerkin
+// stencil); with a symmetric operator the offsets of smaller stencil index
+// are read transposed from the neighbour, so the smoother's operator is
+// symmetric to the last bit.
+FineStencil make_coarse_stencil(const Grid& g, bool symmetric) {
+    FineStencil f{};
+    f.S = f.SS = 27;
+    f.G = 8;
+    f.half = symmetric;
+    f.diag = 13;
+    for (int q = 0; q < 27; ++q) {
+        const bool tr = symmetric && q < 13;
+        const unsigned info = (unsigned)(q / 9) | (unsigned)((q / 3) % 3) << 2 | (unsigned)(q % 3) << 4 |
+                              (tr ? 1u << 6 : 0u) | 1u << 7 | (unsigned)(tr ? 26 - q : q) << 8;
+        const int m = q / 8, l = q % 8;
+        f.lane[m][l / 2] |= info << ((l % 2) * 16);
+        f.ssig[q / 6] |= (unsigned)q << ((q % 6) * 5);
+    }
     f.rny = 1.0 / (g.ny + 1);
     f.rnz = 1.0 / (g.nz + 1);
     return f;
@@ -703,10 +680,10 @@ void setup_impl(pa_multigrid* mg, const VT* vals, hipStream_t st) {
     const pa_mesh* m = mg->mesh;
     const int L = (int)mg->grids.size();
     const Grid g0 = mg->grids[0];
-    const long nq = (long)g0.nodes() * mg->fs.SS;
-    k_crs_to_fst<D, VT><<<nblk(nq), kT, 0, st>>>(g0, mg->fs, m->d_rowmap.p, m->d_colidx.p, vals, mg->fine.p);
+    const long nq = (long)g0.nodes() * mg->fsl[0].SS;
+    k_crs_to_fst<D, VT><<<nblk(nq), kT, 0, st>>>(g0, mg->fsl[0], m->d_rowmap.p, m->d_colidx.p, vals, mg->fine.p);
     chk();
-    k_diag_inv_fst<D><<<nblk(g0.nodes()), kT, 0, st>>>(g0.nodes(), mg->fs, mg->fine.p, mg->dinv[0]->p);
+    k_diag_inv_fst<D><<<nblk(g0.nodes()), kT, 0, st>>>(g0.nodes(), mg->fsl[0], mg->fine.p, mg->dinv[0]->p);
     chk();
     for (int l = 1; l < L; ++l) {
         const Grid gf = mg->grids[l - 1], gc = mg->grids[l];
@@ -850,8 +827,10 @@ int pa_multigrid_setup(pa_multigrid* mg, const void* vals, int precision, int nb
             pa::chk();
         }
         // the smoother reads an FP32 stencil copy of the fine matrix
-        mg->fs = pa::make_fine_stencil(mg->grids[0], mg->graph_offsets, mg->symmetric);
-        const long nf = (long)mg->mesh->N * mg->fs.SS * mg->dpn * mg->dpn;
+        mg->fsl.assign(1, pa::make_fine_stencil(mg->grids[0], mg->graph_offsets, mg->symmetric));
+        for (size_t l = 1; l < mg->grids.size(); ++l)
+            mg->fsl.push_back(pa::make_coarse_stencil(mg->grids[l], mg->symmetric));
+        const long nf = (long)mg->mesh->N * mg->fsl[0].SS * mg->dpn * mg->dpn;
         if (mg->fine.n != (size_t)nf) mg->fine.alloc(nf);
         auto run = [&](auto tag) {
             using VT = decltype(tag);
