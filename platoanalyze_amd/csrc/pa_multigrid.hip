@@ -20,25 +20,12 @@
 
 #include "pa_cell.h"
 #include "pa_internal.h"
+#include "pa_stencil.h"
 
 namespace pa {
 namespace {
 
 constexpr int kT = 256;
-
-struct Grid {
-    int nx, ny, nz;  // cells
-    __host__ __device__ int nodes() const { return (nx + 1) * (ny + 1) * (nz + 1); }
-    __host__ __device__ int id(int i, int j, int k) const { return (i * (ny + 1) + j) * (nz + 1) + k; }
-    __host__ __device__ void ijk(int n, int& i, int& j, int& k) const {
-        k = n % (nz + 1);
-        j = (n / (nz + 1)) % (ny + 1);
-        i = n / ((nz + 1) * (ny + 1));
-    }
-    __host__ __device__ bool in(int i, int j, int k) const {
-        return i >= 0 && i <= nx && j >= 0 && j <= ny && k >= 0 && k <= nz;
-    }
-};
 
 __device__ inline int sidx(int dx, int dy, int dz) { return (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1); }
 
@@ -105,56 +92,8 @@ __device__ inline void block_inverse(const double* a, float* out) {
 }
 
 // ------------------------------------------------------- fine level (stencil)
-// The fine matrix as a block stencil over the S offsets (in stencil-index
-// order) that the mesh graph holds.  Stored slot t of node n holds the block
-// A(n, n + offset(t)); with a symmetric operator only the diagonal and the
-// offsets of larger stencil index are stored, and the offset d < 0 is read
-// from the neighbour's block A(n + d, n), transposed.
-//
-// The smoother gives each node G = 4 lanes (at most 16 offsets) or 8; lane l
-// takes the offsets l, l + G, ...  The description of offset l + Gm is 16
-// bits of word lane[m][l / 2]: dx + 1, dy + 1, dz + 1 (2 bits each), read
-// transposed (bit 6), present (bit 7), stored slot read (bits 8-12).
-constexpr int kFsSlots = 4;  // offsets per lane
-struct FineStencil {
-    int S, SS;                    // offsets in the graph; stored blocks per node
-    int G;                        // lanes per node
-    int half;                     // some offsets are read transposed
-    unsigned lane[kFsSlots][4];   // per-lane offset descriptions
-    unsigned ssig[6];                // stencil index of stored slot t, 5 bits each, six per word
-    int diag;                        // stored slot of the diagonal block
-    double rny, rnz;                 // 1 / (ny + 1), 1 / (nz + 1): node coordinates by products
-};
-
-// The fine CRS matrix (FP64 or FP32) into the stencil: thread per (node,
-// stored slot); a slot the node's row has no entry for gets a zero block.
-template <int D, class VT>
-__global__ void k_crs_to_fst(Grid g, FineStencil fs, const int* __restrict__ rowmap, const int* __restrict__ colidx,
-                             const VT* __restrict__ vals, float* __restrict__ A) {
-    constexpr int BS = D * D;
-    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= (long)g.nodes() * fs.SS) return;
-    const int n = (int)(q / fs.SS), t = (int)(q % fs.SS);
-    const int sig = (int)((fs.ssig[t / 6] >> ((t % 6) * 5)) & 31u);
-    int i, j, k;
-    g.ijk(n, i, j, k);
-    const int ii = i + sig / 9 - 1, jj = j + (sig / 3) % 3 - 1, kk = k + sig % 3 - 1;
-    int e = -1;
-    if (g.in(ii, jj, kk)) {
-        const int c = g.id(ii, jj, kk);
-        for (int f = rowmap[n]; f < rowmap[n + 1]; ++f)
-            if (colidx[f] == c) {
-                e = f;
-                break;
-            }
-    }
-    float* dst = A + q * BS;
-#pragma unroll
-    for (int r = 0; r < BS; ++r) dst[r] = e >= 0 ? (float)vals[(long)e * BS + r] : 0.0f;
-}
-
 template <int D>
-__global__ void k_diag_inv_fst(int N, FineStencil fs, const float* __restrict__ A, float* __restrict__ dinv) {
+__global__ void k_diag_inv_fst(int N, BlockStencil fs, const float* __restrict__ A, float* __restrict__ dinv) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     double a[D * D];
@@ -163,11 +102,6 @@ __global__ void k_diag_inv_fst(int N, FineStencil fs, const float* __restrict__ 
     block_inverse<D>(a, dinv + (long)n * D * D);
 }
 
-// v from lane l ^ 1 / l ^ 2 of the quad (DPP quad permutations)
-template <int CTRL>
-__device__ inline float quad_dpp(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, false));
-}
 
 template <int D>
 __global__ void k_diag_inv_st(int N, const float* __restrict__ A, float* __restrict__ dinv) {
@@ -187,7 +121,7 @@ __global__ void k_diag_inv_st(int N, const float* __restrict__ A, float* __restr
 // the lane issues at once.  SYM: some offsets are read transposed.  Lane
 // l < D finishes dof l.
 template <int D, int G, bool SYM>
-__global__ __launch_bounds__(kT) void k_fst_resid_jac(Grid g, FineStencil fs, const float* __restrict__ A,
+__global__ __launch_bounds__(kT) void k_fst_resid_jac(Grid g, BlockStencil fs, const float* __restrict__ A,
                                                       const float* __restrict__ dinv,
                                                       const unsigned char* __restrict__ mask,
                                                       const float* __restrict__ b, const float* __restrict__ x,
@@ -196,12 +130,8 @@ __global__ __launch_bounds__(kT) void k_fst_resid_jac(Grid g, FineStencil fs, co
     const int l = threadIdx.x & (G - 1);
     const int n = xcd_block() * ROWS + threadIdx.x / G;
     if (n >= g.nodes()) return;
-    const int nz1 = g.nz + 1, nyz = (g.ny + 1) * nz1;
-    // node coordinates: (n + 1/2) / d rounded down is exact in FP64
-    const int jk = (int)(((double)n + 0.5) * fs.rnz);
-    const int k = n - jk * nz1;
-    const int i = (int)(((double)jk + 0.5) * fs.rny);
-    const int j = jk - i * (g.ny + 1);
+    int i, j, k;
+    stencil_ijk(g, fs, n, i, j, k);
     // the node's dof min(l, D - 1) for the update, in flight with the stencil
     const int ld = l < D ? l : D - 1;
     const float bl = b[(long)n * D + ld], xl = x[(long)n * D + ld];
@@ -217,22 +147,9 @@ __global__ __launch_bounds__(kT) void k_fst_resid_jac(Grid g, FineStencil fs, co
         bool ok[NS], tr[NS];
 #pragma unroll
         for (int m = 0; m < NS; ++m) {
-            const unsigned w0 = __builtin_amdgcn_readfirstlane(fs.lane[m][0]);
-            const unsigned w1 = __builtin_amdgcn_readfirstlane(fs.lane[m][1]);
-            unsigned w = (l & 2) ? w1 : w0;
-            if constexpr (G == 8) {
-                const unsigned w2 = __builtin_amdgcn_readfirstlane(fs.lane[m][2]);
-                const unsigned w3 = __builtin_amdgcn_readfirstlane(fs.lane[m][3]);
-                w = (l & 4) ? ((l & 2) ? w3 : w2) : w;
-            }
-            const unsigned info = w >> ((l & 1) * 16);
-            const int ii = i + (int)(info & 3u) - 1, jj = j + (int)((info >> 2) & 3u) - 1,
-                      kk = k + (int)((info >> 4) & 3u) - 1;
-            ok[m] = ((info >> 7) & 1u) && ii >= 0 && ii <= g.nx && jj >= 0 && jj <= g.ny && kk >= 0 && kk <= g.nz;
-            tr[m] = SYM && ((info >> 6) & 1u);
-            const long c = ok[m] ? ((long)ii * nyz + jj * nz1 + kk) : (long)n;
-            const long src = ok[m] ? (long)((info >> 8) & 31u) : (long)fs.diag;
-            const float* v = A + ((tr[m] ? c : (long)n) * fs.SS + src) * BS;
+            long c, base;
+            stencil_slot<G, SYM>(g, fs, m, l, n, i, j, k, BS, ok[m], tr[m], c, base);
+            const float* v = A + base;
 #pragma unroll
             for (int q = 0; q < BS; ++q) blk[m][q] = v[q];
 #pragma unroll
@@ -252,13 +169,8 @@ __global__ __launch_bounds__(kT) void k_fst_resid_jac(Grid g, FineStencil fs, co
                 acc[p] += ok[m] ? y : 0.0f;
             }
         }
-        // sum over the node's lanes: l ^ 1, l ^ 2, then the other quad
 #pragma unroll
-        for (int p = 0; p < D; ++p) {
-            acc[p] += quad_dpp<0xb1>(acc[p]);  // quad_perm [1, 0, 3, 2]
-            acc[p] += quad_dpp<0x4e>(acc[p]);  // quad_perm [2, 3, 0, 1]
-            if constexpr (G == 8) acc[p] += quad_dpp<0x141>(acc[p]);  // row_half_mirror
-        }
+        for (int p = 0; p < D; ++p) acc[p] = node_lanes_sum<G>(acc[p]);
     }
     // r_ld on lane l, then r_q from lane q of the quad for the block row
     float al = acc[0];
@@ -268,10 +180,10 @@ __global__ __launch_bounds__(kT) void k_fst_resid_jac(Grid g, FineStencil fs, co
     float t = 0.0f;
     if (omega != 0.0f) {
         float rq[D];
-        rq[0] = quad_dpp<0x00>(rl);  // quad_perm [0, 0, 0, 0]
-        if constexpr (D > 1) rq[1] = quad_dpp<0x55>(rl);
-        if constexpr (D > 2) rq[2] = quad_dpp<0xaa>(rl);
-        if constexpr (D > 3) rq[3] = quad_dpp<0xff>(rl);
+        rq[0] = dpp_f32<0x00>(rl);  // quad_perm [0, 0, 0, 0]
+        if constexpr (D > 1) rq[1] = dpp_f32<0x55>(rl);
+        if constexpr (D > 2) rq[2] = dpp_f32<0xaa>(rl);
+        if constexpr (D > 3) rq[3] = dpp_f32<0xff>(rl);
 #pragma unroll
         for (int q = 0; q < D; ++q) t += dl[q] * rq[q];
     }
@@ -547,7 +459,7 @@ struct pa_multigrid {
     pa::DevBuf<double> dense, dense_inv;
     unsigned graph_offsets = 0;  // bit sidx: the mesh graph holds that offset
     bool symmetric = false;      // the fine operator is symmetric: half stencil
-    std::vector<pa::FineStencil> fsl;  // stencil of each level's smoother
+    std::vector<pa::BlockStencil> fsl;  // stencil of each level's smoother
     pa::DevBuf<float> fine;  // the fine matrix as an FP32 stencil
     bool ready = false;
     ~pa_multigrid() {
@@ -574,7 +486,7 @@ void vcycle(pa_multigrid* mg, int l, hipStream_t st) {
         return;
     }
     auto sweep = [&](const float* xin, float* xout, float om, int zero) {
-        const FineStencil& f = mg->fsl[l];
+        const BlockStencil& f = mg->fsl[l];
         auto run = [&](auto kernel) {
             kernel<<<nblk(g.nodes(), kT / f.G), kT, 0, st>>>(g, f, l == 0 ? mg->fine.p : mg->A[l]->p, mg->dinv[l]->p,
                                                              l == 0 ? mg->mask.p : nullptr, b, xin, xout, om, zero);
@@ -614,50 +526,12 @@ void vcycle(pa_multigrid* mg, int l, hipStream_t st) {
     }
 }
 
-// The lanes and stored slots of the fine stencil from the offsets of the
-// mesh graph (bit sidx set when some row has that neighbour offset).
-FineStencil make_fine_stencil(const Grid& g, unsigned offsets, bool symmetric) {
-    FineStencil f{};
-    int lanes[27], stored[27], slot_of[27];
-    f.S = 0;
-    for (int q = 0; q < 27; ++q)
-        if ((offsets >> q) & 1u) lanes[f.S++] = q;
-    // a half stencil needs the offsets in pairs (the graph of a symmetric
-    // operator is symmetric)
-    bool pairs = true;
-    for (int q = 0; q < 27; ++q) pairs = pairs && (((offsets >> q) & 1u) == ((offsets >> (26 - q)) & 1u));
-    const bool half = symmetric && pairs;
-    f.SS = 0;
-    for (int q = 0; q < 27; ++q) slot_of[q] = -1;
-    for (int s = 0; s < f.S; ++s)
-        if (!half || lanes[s] >= 13) {
-            slot_of[lanes[s]] = f.SS;
-            stored[f.SS++] = lanes[s];
-        }
-    PA_REQUIRE(slot_of[13] >= 0, PA_ERR_ARG, "multigrid: the mesh graph has no diagonal entries");
-    f.diag = slot_of[13];
-    f.G = f.S <= 4 * kFsSlots ? 4 : 8;
-    for (int s = 0; s < f.S; ++s) {
-        const int q = lanes[s];
-        const bool tr = half && q < 13;
-        const unsigned info = (unsigned)(q / 9) | (unsigned)((q / 3) % 3) << 2 | (unsigned)(q % 3) << 4 |
-                              (tr ? 1u << 6 : 0u) | 1u << 7 | (unsigned)slot_of[tr ? 26 - q : q] << 8;
-        const int m = s / f.G, l = s % f.G;
-        f.lane[m][l / 2] |= info << ((l % 2) * 16);
-    }
-    for (int t = 0; t < f.SS; ++t) f.ssig[t / 6] |= (unsigned)stored[t] << ((t % 6) * 5);
-    f.half = half;
-    f.rny = 1.0 / (g.ny + 1);
-    f.rnz = 1.0 / (g.nz + 1);
-    return f;
-}
-
 // A coarse level's smoother stencil: all 27 offsets stored (the Galerkin
 // stencil); with a symmetric operator the offsets of smaller stencil index
 // are read transposed from the neighbour, so the smoother's operator is
 // symmetric to the last bit.
-FineStencil make_coarse_stencil(const Grid& g, bool symmetric) {
-    FineStencil f{};
+BlockStencil make_coarse_stencil(const Grid& g, bool symmetric) {
+    BlockStencil f{};
     f.S = f.SS = 27;
     f.G = 8;
     f.half = symmetric;
@@ -681,7 +555,7 @@ void setup_impl(pa_multigrid* mg, const VT* vals, hipStream_t st) {
     const int L = (int)mg->grids.size();
     const Grid g0 = mg->grids[0];
     const long nq = (long)g0.nodes() * mg->fsl[0].SS;
-    k_crs_to_fst<D, VT><<<nblk(nq), kT, 0, st>>>(g0, mg->fsl[0], m->d_rowmap.p, m->d_colidx.p, vals, mg->fine.p);
+    k_crs_to_stencil<D, VT, float><<<nblk(nq), kT, 0, st>>>(g0, mg->fsl[0], m->d_rowmap.p, m->d_colidx.p, vals, mg->fine.p);
     chk();
     k_diag_inv_fst<D><<<nblk(g0.nodes()), kT, 0, st>>>(g0.nodes(), mg->fsl[0], mg->fine.p, mg->dinv[0]->p);
     chk();
@@ -744,17 +618,8 @@ int pa_multigrid_create(const pa_mesh* mesh, int nx, int ny, int nz, int dpn, co
                    "pa_multigrid_create: grid does not match the mesh node count");
         // the graph must be a vertex-neighbourhood graph of this grid
         unsigned offsets = 0;
-        for (int r = 0; r < mesh->N; ++r) {
-            int i, j, k;
-            g0.ijk(r, i, j, k);
-            for (int e = mesh->h_rowmap[r]; e < mesh->h_rowmap[r + 1]; ++e) {
-                int a, b, c;
-                g0.ijk(mesh->h_colidx[e], a, b, c);
-                PA_REQUIRE(std::abs(a - i) <= 1 && std::abs(b - j) <= 1 && std::abs(c - k) <= 1, PA_ERR_ARG,
-                           "pa_multigrid_create: mesh graph is not a structured-grid neighbourhood graph");
-                offsets |= 1u << ((a - i + 1) * 9 + (b - j + 1) * 3 + (c - k + 1));
-            }
-        }
+        PA_REQUIRE(pa::grid_graph_offsets(mesh, g0, &offsets), PA_ERR_ARG,
+                   "pa_multigrid_create: mesh graph is not a structured-grid neighbourhood graph");
         auto* mg = new pa_multigrid();
         try {
             mg->mesh = mesh;
@@ -827,7 +692,7 @@ int pa_multigrid_setup(pa_multigrid* mg, const void* vals, int precision, int nb
             pa::chk();
         }
         // the smoother reads an FP32 stencil copy of the fine matrix
-        mg->fsl.assign(1, pa::make_fine_stencil(mg->grids[0], mg->graph_offsets, mg->symmetric));
+        mg->fsl.assign(1, pa::make_block_stencil(mg->grids[0], mg->graph_offsets, mg->symmetric));
         for (size_t l = 1; l < mg->grids.size(); ++l)
             mg->fsl.push_back(pa::make_coarse_stencil(mg->grids[l], mg->symmetric));
         const long nf = (long)mg->mesh->N * mg->fsl[0].SS * mg->dpn * mg->dpn;

@@ -12,7 +12,9 @@
 #include <functional>
 #include <vector>
 
+#include "pa_cell.h"
 #include "pa_internal.h"
+#include "pa_stencil.h"
 
 struct pa_solver {
     const pa_mesh* mesh = nullptr;
@@ -27,6 +29,17 @@ struct pa_solver {
     pa::DevBuf<double> scal;      // device scalars
     std::vector<double> h_scal;
     pa_multigrid* mg = nullptr;   // PA_PRECOND_MULTIGRID (not owned)
+    // structured grid meshes: the matrix of a solve is copied once into a
+    // block stencil that the SpMV reads (half of it for symmetric matrices)
+    bool stencil = false;
+    bool symmetric = false;
+    pa::Grid sgrid{};
+    unsigned soffsets = 0;
+    pa::BlockStencil bs{};
+    pa::DevBuf<double> st64;
+    pa::DevBuf<float> st32;
+    bool st_ready = false;  // the stencil copy of the current solve is built
+    int nsp = 0;            // partial sums written by the last SpMV
 };
 
 struct pa_problem {
@@ -278,10 +291,102 @@ __global__ __launch_bounds__(kB) void k_dot_part(long n, const double* __restric
 
 void check() { PA_HIP(hipGetLastError()); }
 
+// y = A x and partial sums of x.y per thread block on the block stencil: G
+// lanes per node, NS offsets per lane, FP64 accumulation; lane l < D writes
+// dof l.
+template <int D, class VT, int G, int NS, bool SYM>
+__global__ __launch_bounds__(kB) void k_st_spmv_dot(Grid g, BlockStencil fs, const VT* __restrict__ A,
+                                                    const double* __restrict__ x, double* __restrict__ y,
+                                                    double* __restrict__ part) {
+    constexpr int ROWS = kB / G, BS = D * D;
+    __shared__ double sh[kB / 64];
+    const int l = threadIdx.x & (G - 1);
+    const int n = xcd_block() * ROWS + threadIdx.x / G;
+    double d = 0.0;
+    if (n < g.nodes()) {
+        int i, j, k;
+        stencil_ijk(g, fs, n, i, j, k);
+        const int ld = l < D ? l : D - 1;
+        const double xl = x[(long)n * D + ld];
+        VT blk[NS][BS];
+        double xc[NS][D];
+        bool ok[NS], tr[NS];
+#pragma unroll
+        for (int m = 0; m < NS; ++m) {
+            long c, base;
+            stencil_slot<G, SYM>(g, fs, m, l, n, i, j, k, BS, ok[m], tr[m], c, base);
+#pragma unroll
+            for (int q = 0; q < BS; ++q) blk[m][q] = A[base + q];
+#pragma unroll
+            for (int q = 0; q < D; ++q) xc[m][q] = x[c * D + q];
+        }
+        double acc[D];
+#pragma unroll
+        for (int p = 0; p < D; ++p) {
+            acc[p] = 0.0;
+#pragma unroll
+            for (int m = 0; m < NS; ++m) {
+                double yn = 0.0, yt = 0.0;
+#pragma unroll
+                for (int q = 0; q < D; ++q) {
+                    yn = fma((double)blk[m][p * D + q], xc[m][q], yn);
+                    if constexpr (SYM) yt = fma((double)blk[m][q * D + p], xc[m][q], yt);
+                }
+                acc[p] += ok[m] ? ((SYM && tr[m]) ? yt : yn) : 0.0;
+            }
+            acc[p] = node_lanes_sum<G>(acc[p]);
+        }
+        double al = acc[0];
+#pragma unroll
+        for (int p = 1; p < D; ++p) al = ld == p ? acc[p] : al;
+        if (l < D) {
+            y[(long)n * D + l] = al;
+            d = al * xl;
+        }
+    }
+    const double t = block_sum(d, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// The stencil copy of the solve's matrix (once per solve).
+template <int D, class VT>
+void stencil_copy(pa_solver* s, const VT* vals, hipStream_t st) {
+    const long nq = (long)s->mesh->N * s->bs.SS;
+    VT* A;
+    if constexpr (sizeof(VT) == 8) {
+        if (s->st64.n != (size_t)(nq * D * D)) s->st64.alloc(nq * D * D);
+        A = s->st64.p;
+    } else {
+        if (s->st32.n != (size_t)(nq * D * D)) s->st32.alloc(nq * D * D);
+        A = s->st32.p;
+    }
+    k_crs_to_stencil<D, VT, VT><<<(int)((nq + kB - 1) / kB), kB, 0, st>>>(s->sgrid, s->bs, s->mesh->d_rowmap.p,
+                                                                         s->mesh->d_colidx.p, vals, A);
+    check();
+    s->st_ready = true;
+}
+
 template <int D, class VT>
 void spmv_dot(pa_solver* s, const VT* vals, const double* x, double* y, hipStream_t st) {
     const pa_mesh* m = s->mesh;
-    k_spmv_dot<D, VT><<<s->nsb, kB, 0, st>>>(m->N, m->d_rowmap.p, m->d_colidx.p, vals, x, y, s->part.p);
+    if (s->st_ready) {
+        const VT* A;
+        if constexpr (sizeof(VT) == 8) A = s->st64.p;
+        else A = s->st32.p;
+        const int nb = (m->N + kB / 8 - 1) / (kB / 8);
+        auto run = [&](auto kernel) { kernel<<<nb, kB, 0, st>>>(s->sgrid, s->bs, A, x, y, s->part.p); };
+        if (s->bs.S <= 16) {
+            if (s->bs.half) run(k_st_spmv_dot<D, VT, 8, 2, true>);
+            else run(k_st_spmv_dot<D, VT, 8, 2, false>);
+        } else {
+            if (s->bs.half) run(k_st_spmv_dot<D, VT, 8, 4, true>);
+            else run(k_st_spmv_dot<D, VT, 8, 4, false>);
+        }
+        s->nsp = nb;
+    } else {
+        k_spmv_dot<D, VT><<<s->nsb, kB, 0, st>>>(m->N, m->d_rowmap.p, m->d_colidx.p, vals, x, y, s->part.p);
+        s->nsp = s->nsb;
+    }
     check();
 }
 
@@ -354,6 +459,8 @@ void pcg_impl(pa_solver* s, const VT* vals, const double* b, double* x, double t
         k_prec_setup<D, VT><<<(N + 255) / 256, 256, 0, st>>>(N, kprec, s->d_diag.p, vals, s->d_minv.p);
         check();
     }
+    s->st_ready = false;
+    if (s->stencil) stencil_copy<D, VT>(s, vals, st);
     spmv_dot<D, VT>(s, vals, x, s->q.p, st);
     k_pcg_init<D><<<nvb, kB, 0, st>>>(N, kprec, s->d_minv.p, b, s->q.p, s->r.p, s->zv.p, s->p.p, s->part.p, nvb);
     check();
@@ -374,7 +481,7 @@ void pcg_impl(pa_solver* s, const VT* vals, const double* b, double* x, double t
         const int burst = std::min(check_every, max_iter - it);
         for (int k = 0; k < burst; ++k) {
             spmv_dot<D, VT>(s, vals, s->p.p, s->q.p, st);
-            k_reduce<<<1, 1024, 0, st>>>(s->part.p, s->nsb, s->nsb, 1, S_PQ, 0, 0, 0, s->scal.p);
+            k_reduce<<<1, 1024, 0, st>>>(s->part.p, s->nsp, s->nsp, 1, S_PQ, 0, 0, 0, s->scal.p);
             check();
             k_pcg_update<D><<<nvb, kB, 0, st>>>(N, kprec, s->d_minv.p, s->scal.p, s->p.p, s->q.p, x, s->r.p,
                                                 s->zv.p, s->part.p, nvb);
@@ -419,6 +526,14 @@ void pcg_impl(pa_solver* s, const VT* vals, const double* b, double* x, double t
     stats->iterations = it;
     stats->rel_residual = bnorm > 0.0 ? rn / bnorm : 0.0;
     stats->converged = stats->rel_residual <= tol ? 1 : 0;
+}
+
+// The matrices this solver is given are symmetric (the elliptic problems:
+// assembled blocks (m, n) and (n, m) are transposes bit for bit, and stay so
+// under the symmetric Dirichlet elimination): the SpMV reads half a stencil.
+void solver_set_symmetric(pa_solver* s) {
+    s->symmetric = true;
+    if (s->stencil) s->bs = make_block_stencil(s->sgrid, s->soffsets, true, 8);
 }
 
 void pcg(pa_solver* s, const void* vals, int precision, const double* b, double* x, double tol, int max_iter,
@@ -473,6 +588,13 @@ int pa_solver_create(const pa_mesh* mesh, int dpn, int precond, pa_solver** out)
             s->p.alloc(n);
             s->q.alloc(n);
             s->part.alloc(4L * std::max(s->nvb, s->nsb));
+            // a lexicographic node grid whose graph is a neighbourhood graph
+            const int* gd = mesh->grid_dims[0] > 0 ? mesh->grid_dims : (mesh->has_grid ? mesh->kgrid : nullptr);
+            if (gd) {
+                s->sgrid = pa::Grid{gd[0] - 1, gd[1] - 1, gd[2] - 1};
+                s->stencil = gd[0] > 1 && gd[1] > 1 && gd[2] > 1 && pa::grid_graph_offsets(mesh, s->sgrid, &s->soffsets);
+                if (s->stencil) s->bs = pa::make_block_stencil(s->sgrid, s->soffsets, false, 8);
+            }
             s->scal.alloc(pa::S_N);
             PA_HIP(hipMemset(s->scal.p, 0, pa::S_N * sizeof(double)));
         } catch (...) {
@@ -526,6 +648,7 @@ int pa_problem_create(pa_vector_function* vf, pa_scalar_function* sf, int nbc, c
                 P->bc_touch.alloc(vf->mesh->N);
             }
             pa::check_status(pa_solver_create(vf->mesh, vf->dpn, opt->precond, &P->solver));
+            pa::solver_set_symmetric(P->solver);
             P->rhs.alloc(n);
             P->res.alloc(n);
             P->dx.alloc(n);
@@ -555,6 +678,7 @@ int pa_problem_set_options(pa_problem* P, const pa_problem_options* opt) {
         if (opt->precond != P->opt.precond) {
             pa_solver* s = nullptr;
             pa::check_status(pa_solver_create(P->vf->mesh, P->vf->dpn, opt->precond, &s));
+            pa::solver_set_symmetric(s);
             if (opt->precond == PA_PRECOND_MULTIGRID) s->mg = P->mg;
             pa_solver_destroy(P->solver);
             P->solver = s;
@@ -575,6 +699,7 @@ int pa_problem_set_multigrid(pa_problem* P, int nx, int ny, int nz, const pa_mul
         P->mg = mg;
         pa_solver* s = nullptr;
         pa::check_status(pa_solver_create(P->vf->mesh, P->vf->dpn, PA_PRECOND_MULTIGRID, &s));
+        pa::solver_set_symmetric(s);
         s->mg = mg;
         pa_solver_destroy(P->solver);
         P->solver = s;
