@@ -15,6 +15,7 @@
 // one workgroup).  Level matrices and vectors are FP32.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -258,77 +259,98 @@ __global__ void k_prolong_add(Grid gf, Grid gc, const float* __restrict__ xc, co
 }
 
 // --------------------------------------------------------- Galerkin products
-// A_c[I][s] = sum_{i in supp(I)} sum_{j ~ i} sum_{J in supp(j), J - I = s}
-//             w_iI w_jJ mask_i A_ij mask_j.
-// 32 lanes per coarse node: lane t < 27 owns the fine vertex 2I + offset(t);
-// contributions are summed in an LDS copy of the coarse row (FP32 atomics).
-template <int D, class VT, bool CRS>
-__global__ __launch_bounds__(kT) void k_rap(Grid gf, Grid gc, const int* __restrict__ rowmap,
-                                            const int* __restrict__ colidx, const VT* __restrict__ vals_crs,
-                                            const float* __restrict__ A_st, const unsigned char* __restrict__ mask,
-                                            float* __restrict__ Ac) {
-    constexpr int G = 32, ROWS = kT / G, BS = D * D;
-    __shared__ float acc[ROWS][27 * BS];
-    const int grp = threadIdx.x / G, lane = threadIdx.x % G;
-    const int cn = blockIdx.x * ROWS + grp;
-    const bool active = cn < gc.nodes();
-    for (int t = lane; t < 27 * BS; t += G) acc[grp][t] = 0.0f;
-    __syncthreads();
-    if (active && lane < 27) {
-        int I, J, K;
-        gc.ijk(cn, I, J, K);
-        const int ox = lane / 9 - 1, oy = (lane / 3) % 3 - 1, oz = lane % 3 - 1;
-        const int i = 2 * I + ox, j = 2 * J + oy, k = 2 * K + oz;
-        if (gf.in(i, j, k)) {
-            const float wi = w1(i, I) * w1(j, J) * w1(k, K);
-            const long fi = gf.id(i, j, k);
-            float mi[D];
+// Galerkin product by gather.  Block A_c(I, I + offset(S)) is a fixed-order
+// sum over the pairs (a, d) of the list of S: fine node i = 2I + a (a in
+// {-1, 0, 1}^3), its neighbour j = i + d (d an offset of the source stencil,
+// with e = j - 2(I + offset(S)) in {-1, 0, 1}^3), weight w(i, I) w(j, J) =
+// 2^-(nonzeros of a and e), and the source block A(i, j) (read transposed
+// from node j when the source stencil keeps half); pairs whose nodes fall
+// outside the fine grid add nothing.  One wave per coarse node: the lists
+// are cut into at most 64 chunks of about equal length, one per lane, and
+// the chunks of one list are added in lane order through LDS.  No atomics:
+// deterministic.  MASK: the fine level's Dirichlet dofs (bit p of mbits[n])
+// are masked out of P.  Entry bits: a (0-4), source slot (5-9), transposed
+// (10), d (11-15), weight exponent (16-18).
+struct RapLanes {
+    int lane[64];  // S (bits 0-4), chunk start (5-18), chunk length (19-26), first lane of S (bit 27 set)
+    int nlanes;
+};
+
+template <int D, bool MASK>
+__global__ __launch_bounds__(256) void k_rap_gather(Grid gf, Grid gc, RapLanes rl, const unsigned* __restrict__ tab,
+                                                    const float* __restrict__ Af, int SSf,
+                                                    const unsigned char* __restrict__ mbits, float* __restrict__ Ac) {
+    constexpr int BS = D * D;
+    __shared__ float part[4][64][BS];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int cn = blockIdx.x * 4 + w;
+    const bool active = cn < gc.nodes() && l < rl.nlanes;
+    const int desc = active ? rl.lane[l] : 0;
+    const int S = desc & 31;
+    int I = 0, J = 0, K = 0;
+    if (cn < gc.nodes()) gc.ijk(cn, I, J, K);
+    const int CI = I + S / 9 - 1, CJ = J + (S / 3) % 3 - 1, CK = K + S % 3 - 1;
+    float acc[BS];
 #pragma unroll
-            for (int p = 0; p < D; ++p) mi[p] = (mask == nullptr || mask[fi * D + p]) ? wi : 0.0f;
-            // fine neighbours of fi with their blocks
-            auto visit = [&](int jj, int jx, int jy, int jz, const float* blk) {
-                float mj[D];
+    for (int q = 0; q < BS; ++q) acc[q] = 0.0f;
+    if (active && gc.in(CI, CJ, CK)) {
+        const int t0 = (desc >> 5) & 0x3fff, t1 = t0 + ((desc >> 19) & 0xff);
+        for (int t = t0; t < t1; ++t) {
+            const unsigned e = tab[t];
+            const int as = (int)(e & 31u), slot = (int)((e >> 5) & 31u), ds = (int)((e >> 11) & 31u);
+            const bool tr = (e >> 10) & 1u;
+            const int i = 2 * I + as / 9 - 1, j = 2 * J + (as / 3) % 3 - 1, k = 2 * K + as % 3 - 1;
+            const int ii = i + ds / 9 - 1, jj = j + (ds / 3) % 3 - 1, kk = k + ds % 3 - 1;
+            if (!gf.in(i, j, k) || !gf.in(ii, jj, kk)) continue;
+            const float wgt = __builtin_amdgcn_ldexpf(1.0f, -(int)((e >> 16) & 7u));
+            const long fi = gf.id(i, j, k), fj = gf.id(ii, jj, kk);
+            const float* blk = Af + ((tr ? fj : fi) * SSf + slot) * BS;
+            float v[BS];
 #pragma unroll
-                for (int q = 0; q < D; ++q) mj[q] = (mask == nullptr || mask[(long)jj * D + q]) ? 1.0f : 0.0f;
-                for (int X = jx >> 1; X <= (jx + 1) >> 1; ++X)
-                    for (int Y = jy >> 1; Y <= (jy + 1) >> 1; ++Y)
-                        for (int Z = jz >> 1; Z <= (jz + 1) >> 1; ++Z) {
-                            const float wj = w1(jx, X) * w1(jy, Y) * w1(jz, Z);
-                            const int s = sidx(X - I, Y - J, Z - K);
-                            float* dst = &acc[grp][s * BS];
-#pragma unroll
-                            for (int p = 0; p < D; ++p)
-#pragma unroll
-                                for (int q = 0; q < D; ++q) {
-                                    const float v = mi[p] * blk[p * D + q] * mj[q] * wj;
-                                    if (v != 0.0f) atomicAdd(&dst[p * D + q], v);
-                                }
-                        }
-            };
-            if constexpr (CRS) {
-                for (int e = rowmap[fi]; e < rowmap[fi + 1]; ++e) {
-                    const int jj = colidx[e];
-                    int jx, jy, jz;
-                    gf.ijk(jj, jx, jy, jz);
-                    float blk[BS];
-#pragma unroll
-                    for (int q = 0; q < BS; ++q) blk[q] = (float)vals_crs[(long)e * BS + q];
-                    visit(jj, jx, jy, jz, blk);
-                }
-            } else {
-                for (int dx = -1; dx <= 1; ++dx)
-                    for (int dy = -1; dy <= 1; ++dy)
-                        for (int dz = -1; dz <= 1; ++dz) {
-                            if (!gf.in(i + dx, j + dy, k + dz)) continue;
-                            const float* blk = A_st + (fi * 27 + sidx(dx, dy, dz)) * BS;
-                            visit(gf.id(i + dx, j + dy, k + dz), i + dx, j + dy, k + dz, blk);
-                        }
+            for (int q = 0; q < BS; ++q) {
+                v[q] = blk[q];
+                asm volatile("" : "+v"(v[q]));  // keep the block in registers (no per-element address select)
             }
+            float mi[D], mj[D];
+            const unsigned bi = MASK ? mbits[fi] : 0xffu, bj = MASK ? mbits[fj] : 0xffu;
+#pragma unroll
+            for (int p = 0; p < D; ++p) {
+                mi[p] = ((bi >> p) & 1u) ? wgt : 0.0f;
+                mj[p] = ((bj >> p) & 1u) ? 1.0f : 0.0f;
+            }
+#pragma unroll
+            for (int p = 0; p < D; ++p)
+#pragma unroll
+                for (int q = 0; q < D; ++q) {
+                    const float a = tr ? v[q * D + p] : v[p * D + q];
+                    acc[p * D + q] = fmaf(mi[p] * mj[q], a, acc[p * D + q]);
+                }
         }
     }
+#pragma unroll
+    for (int q = 0; q < BS; ++q) part[w][l][q] = acc[q];
     __syncthreads();
-    if (active)
-        for (int t = lane; t < 27 * BS; t += G) Ac[(long)cn * 27 * BS + t] = acc[grp][t];
+    if (active && ((desc >> 27) & 1)) {
+        // this lane is the first of its list: the list's chunks in lane order
+        float sum[BS];
+#pragma unroll
+        for (int q = 0; q < BS; ++q) sum[q] = part[w][l][q];
+        for (int r = l + 1; r < rl.nlanes && (rl.lane[r] & 31) == S; ++r)
+#pragma unroll
+            for (int q = 0; q < BS; ++q) sum[q] += part[w][r][q];
+        float* dst = Ac + ((long)cn * 27 + S) * BS;
+#pragma unroll
+        for (int q = 0; q < BS; ++q) dst[q] = sum[q];
+    }
+}
+
+// Bit p of bits[n]: dof p of node n is free.
+__global__ void k_mask_bits(int N, int D, const unsigned char* __restrict__ mask, unsigned char* __restrict__ bits) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    unsigned b = 0;
+    for (int p = 0; p < D; ++p) b |= mask[(long)n * D + p] ? 1u << p : 0u;
+    bits[n] = (unsigned char)b;
 }
 
 // ------------------------------------------------------------ coarsest level
@@ -352,17 +374,20 @@ __global__ void k_dense_from_stencil(Grid g, const float* __restrict__ A, double
 
 // In-place Gauss-Jordan inverse with partial pivoting (rows swapped in both
 // matrices; ties go to the lower row, so the result is deterministic), one
-// workgroup of 1024 threads.
+// workgroup of 1024 threads as 32 rows of 32 lanes: lane tx of row group ty
+// updates the columns tx, tx + 32, ... of the rows ty, ty + 32, ...
 __global__ __launch_bounds__(1024) void k_gauss_jordan(int n, double* __restrict__ a, double* __restrict__ inv) {
-    for (long t = threadIdx.x; t < (long)n * n; t += blockDim.x) inv[t] = (t / n == t % n) ? 1.0 : 0.0;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int r = ty; r < n; r += 32)
+        for (int c = tx; c < n; c += 32) inv[(long)r * n + c] = r == c ? 1.0 : 0.0;
     __syncthreads();
-    __shared__ double piv;
     __shared__ double sval[1024];
     __shared__ int sidx_[1024];
+    __shared__ double piv;
     for (int k = 0; k < n; ++k) {
         double best = -1.0;
         int bi = k;
-        for (int r = k + threadIdx.x; r < n; r += blockDim.x) {
+        for (int r = k + (int)threadIdx.x; r < n; r += 1024) {
             const double v = fabs(a[(long)r * n + k]);
             if (v > best) {
                 best = v;
@@ -372,7 +397,7 @@ __global__ __launch_bounds__(1024) void k_gauss_jordan(int n, double* __restrict
         sval[threadIdx.x] = best;
         sidx_[threadIdx.x] = bi;
         __syncthreads();
-        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        for (int s = 512; s > 0; s >>= 1) {
             if ((int)threadIdx.x < s) {
                 const double o = sval[threadIdx.x + s];
                 const int oi = sidx_[threadIdx.x + s];
@@ -385,7 +410,7 @@ __global__ __launch_bounds__(1024) void k_gauss_jordan(int n, double* __restrict
         }
         const int pr = sidx_[0];
         if (pr != k)
-            for (int c = threadIdx.x; c < n; c += blockDim.x) {
+            for (int c = threadIdx.x; c < n; c += 1024) {
                 double t0 = a[(long)k * n + c];
                 a[(long)k * n + c] = a[(long)pr * n + c];
                 a[(long)pr * n + c] = t0;
@@ -397,32 +422,37 @@ __global__ __launch_bounds__(1024) void k_gauss_jordan(int n, double* __restrict
         if (threadIdx.x == 0) piv = a[(long)k * n + k];
         __syncthreads();
         const double ip = piv != 0.0 ? 1.0 / piv : 0.0;
-        for (int c = threadIdx.x; c < n; c += blockDim.x) {
+        for (int c = threadIdx.x; c < n; c += 1024) {
             a[(long)k * n + c] *= ip;
             inv[(long)k * n + c] *= ip;
         }
         __syncthreads();
-        for (long t = threadIdx.x; t < (long)n * n; t += blockDim.x) {
-            const int r = (int)(t / n), c = (int)(t % n);
+        for (int r = ty; r < n; r += 32) {
             if (r == k) continue;
             const double f = a[(long)r * n + k];
-            if (c != k) a[t] -= f * a[(long)k * n + c];
-            inv[t] -= f * inv[(long)k * n + c];
+            for (int c = tx; c < n; c += 32) {
+                if (c != k) a[(long)r * n + c] -= f * a[(long)k * n + c];
+                inv[(long)r * n + c] -= f * inv[(long)k * n + c];
+            }
         }
         __syncthreads();
-        for (int r = threadIdx.x; r < n; r += blockDim.x)
+        for (int r = threadIdx.x; r < n; r += 1024)
             if (r != k) a[(long)r * n + k] = 0.0;
         __syncthreads();
     }
 }
 
-__global__ void k_dense_apply(int n, const double* __restrict__ inv, const float* __restrict__ b,
-                              float* __restrict__ x) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+// x = inv b: one wave per row, lanes over the columns, FP64 sums in a fixed
+// order.
+__global__ __launch_bounds__(256) void k_dense_apply(int n, const double* __restrict__ inv, const float* __restrict__ b,
+                                                     float* __restrict__ x) {
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (r >= n) return;
     double s = 0.0;
-    for (int c = 0; c < n; ++c) s += inv[(long)r * n + c] * (double)b[c];
-    x[r] = (float)s;
+    for (int c = lane; c < n; c += 64) s = fma(inv[(long)r * n + c], (double)b[c], s);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) x[r] = (float)s;
 }
 
 // ----------------------------------------------------------------- helpers
@@ -460,6 +490,10 @@ struct pa_multigrid {
     unsigned graph_offsets = 0;  // bit sidx: the mesh graph holds that offset
     bool symmetric = false;      // the fine operator is symmetric: half stencil
     std::vector<pa::BlockStencil> fsl;  // stencil of each level's smoother
+    pa::RapLanes rap_fine{}, rap_coarse{};  // Galerkin pair tables: from the fine stencil, from a 27-point one
+    pa::DevBuf<unsigned> d_rap_fine, d_rap_coarse;
+    pa::DevBuf<unsigned char> mbits;  // free dofs of each fine node as bits
+    int rap_half = -1;  // fine layout the tables were built for (-1: none)
     pa::DevBuf<float> fine;  // the fine matrix as an FP32 stencil
     bool ready = false;
     ~pa_multigrid() {
@@ -481,7 +515,7 @@ void vcycle(pa_multigrid* mg, int l, hipStream_t st) {
     float* t = mg->t[l]->p;
     const float w = (float)mg->opt.omega;
     if (l == L - 1) {
-        k_dense_apply<<<nblk(n), kT, 0, st>>>((int)n, mg->dense_inv.p, b, x);
+        k_dense_apply<<<(int)((n + 3) / 4), 256, 0, st>>>((int)n, mg->dense_inv.p, b, x);
         chk();
         return;
     }
@@ -549,6 +583,62 @@ BlockStencil make_coarse_stencil(const Grid& g, bool symmetric) {
     return f;
 }
 
+// The pair lists of the gather Galerkin product for a source stencil, and
+// their cut into at most 64 lane chunks.
+RapLanes make_rap_table(const BlockStencil& src, std::vector<unsigned>& tab) {
+    struct Src {
+        bool ok;
+        int slot, tr;
+    } of[27] = {};
+    for (int s = 0; s < src.S; ++s) {
+        const int m = s / src.G, l = s % src.G;
+        const unsigned info = (src.lane[m][l / 2] >> ((l % 2) * 16)) & 0xffffu;
+        const int sig = (int)(info & 3u) * 9 + (int)((info >> 2) & 3u) * 3 + (int)((info >> 4) & 3u);
+        of[sig] = Src{true, (int)((info >> 8) & 31u), (int)((info >> 6) & 1u)};
+    }
+    tab.clear();
+    int off[28];
+    for (int S = 0; S < 27; ++S) {
+        off[S] = (int)tab.size();
+        for (int a = 0; a < 27; ++a)
+            for (int d = 0; d < 27; ++d) {
+                if (!of[d].ok) continue;
+                bool in = true;
+                int nz = 0;
+                for (int c = 0; c < 3; ++c) {
+                    const int av = (c == 0 ? a / 9 : (c == 1 ? (a / 3) % 3 : a % 3)) - 1;
+                    const int dv = (c == 0 ? d / 9 : (c == 1 ? (d / 3) % 3 : d % 3)) - 1;
+                    const int sv = (c == 0 ? S / 9 : (c == 1 ? (S / 3) % 3 : S % 3)) - 1;
+                    const int e = av + dv - 2 * sv;
+                    in = in && e >= -1 && e <= 1;
+                    nz += (av != 0) + (e != 0);
+                }
+                if (in)
+                    tab.push_back((unsigned)(a | of[d].slot << 5 | of[d].tr << 10 | d << 11 | nz << 16));
+            }
+    }
+    off[27] = (int)tab.size();
+    PA_REQUIRE(off[27] < (1 << 14), PA_ERR_LIMIT, "multigrid: Galerkin pair table too long");
+    // chunks of at most T entries, T the smallest length that needs <= 64 lanes
+    RapLanes rl{};
+    for (int T = 1;; ++T) {
+        PA_REQUIRE(T <= 255, PA_ERR_LIMIT, "multigrid: Galerkin pair lists too long for 64 lanes");
+        int lanes = 0;
+        for (int S = 0; S < 27; ++S) lanes += std::max(1, (off[S + 1] - off[S] + T - 1) / T);
+        if (lanes > 64) continue;
+        rl.nlanes = 0;
+        for (int S = 0; S < 27; ++S) {
+            const int c = off[S + 1] - off[S], n = std::max(1, (c + T - 1) / T);
+            for (int r = 0; r < n; ++r) {
+                const int t0 = off[S] + (c * r) / n, t1 = off[S] + (c * (r + 1)) / n;
+                rl.lane[rl.nlanes++] = S | t0 << 5 | (t1 - t0) << 19 | (r == 0 ? 1 << 27 : 0);
+            }
+        }
+        break;
+    }
+    return rl;
+}
+
 template <int D, class VT>
 void setup_impl(pa_multigrid* mg, const VT* vals, hipStream_t st) {
     const pa_mesh* m = mg->mesh;
@@ -559,15 +649,17 @@ void setup_impl(pa_multigrid* mg, const VT* vals, hipStream_t st) {
     chk();
     k_diag_inv_fst<D><<<nblk(g0.nodes()), kT, 0, st>>>(g0.nodes(), mg->fsl[0], mg->fine.p, mg->dinv[0]->p);
     chk();
+    k_mask_bits<<<nblk(g0.nodes()), kT, 0, st>>>(g0.nodes(), D, mg->mask.p, mg->mbits.p);
+    chk();
     for (int l = 1; l < L; ++l) {
         const Grid gf = mg->grids[l - 1], gc = mg->grids[l];
-        const int nb = nblk(gc.nodes(), kT / 32);
+        const int nw = (gc.nodes() + 3) / 4;  // a wave per coarse node
         if (l == 1)
-            k_rap<D, VT, true><<<nb, kT, 0, st>>>(gf, gc, m->d_rowmap.p, m->d_colidx.p, vals, nullptr, mg->mask.p,
-                                                   mg->A[1]->p);
+            k_rap_gather<D, true><<<nw, 256, 0, st>>>(gf, gc, mg->rap_fine, mg->d_rap_fine.p, mg->fine.p,
+                                                      mg->fsl[0].SS, mg->mbits.p, mg->A[1]->p);
         else
-            k_rap<D, float, false><<<nb, kT, 0, st>>>(gf, gc, nullptr, nullptr, nullptr, mg->A[l - 1]->p, nullptr,
-                                                       mg->A[l]->p);
+            k_rap_gather<D, false><<<nw, 256, 0, st>>>(gf, gc, mg->rap_coarse, mg->d_rap_coarse.p,
+                                                       mg->A[l - 1]->p, 27, nullptr, mg->A[l]->p);
         chk();
         if (l < L - 1) {
             k_diag_inv_st<D><<<nblk(gc.nodes()), kT, 0, st>>>(gc.nodes(), mg->A[l]->p, mg->dinv[l]->p);
@@ -658,6 +750,7 @@ int pa_multigrid_create(const pa_mesh* mesh, int nx, int ny, int nz, int dpn, co
             mg->dense.alloc(nd * nd);
             mg->dense_inv.alloc(nd * nd);
             mg->mask.alloc((long)mesh->N * dpn);
+            mg->mbits.alloc(mesh->N);
         } catch (...) {
             delete mg;
             throw;
@@ -695,6 +788,15 @@ int pa_multigrid_setup(pa_multigrid* mg, const void* vals, int precision, int nb
         mg->fsl.assign(1, pa::make_block_stencil(mg->grids[0], mg->graph_offsets, mg->symmetric));
         for (size_t l = 1; l < mg->grids.size(); ++l)
             mg->fsl.push_back(pa::make_coarse_stencil(mg->grids[l], mg->symmetric));
+        if (mg->rap_half != mg->fsl[0].half) {
+            // the pair tables follow the fine stencil's layout (built once)
+            std::vector<unsigned> tab;
+            mg->rap_fine = pa::make_rap_table(mg->fsl[0], tab);
+            mg->d_rap_fine.upload(tab.data(), tab.size());
+            mg->rap_coarse = pa::make_rap_table(pa::make_coarse_stencil(mg->grids[0], false), tab);
+            mg->d_rap_coarse.upload(tab.data(), tab.size());
+            mg->rap_half = mg->fsl[0].half;
+        }
         const long nf = (long)mg->mesh->N * mg->fsl[0].SS * mg->dpn * mg->dpn;
         if (mg->fine.n != (size_t)nf) mg->fine.alloc(nf);
         auto run = [&](auto tag) {

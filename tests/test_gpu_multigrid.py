@@ -111,6 +111,35 @@ def test_galerkin_level_operator(pa):
     assert rel(stencil_to_dense(cdims, st), ref) < 1e-5
 
 
+@pytest.mark.parametrize("symmetric", [False, True])
+def test_galerkin_second_level(pa, symmetric):
+    """Level 1 from the fine stencil (full or half layout) and level 2 from
+    level 1's 27-point stencil against P^T A P with numpy, on 8x4x4."""
+    print(f"case: Galerkin levels 1 and 2 on 8x4x4 elastic, symmetric={symmetric}", flush=True)
+    dims = (8, 4, 4)
+    mesh, vf, sf, z, dofs = elastic_setup(pa, dims, 6)
+    n = 3 * mesh.num_nodes
+    K = vf.gradient_u(np.zeros(n), z)
+    rhs = pa.DeviceArray(n)
+    rhs.zero()
+    pa.apply_block_constraints(K, rhs, dofs, np.zeros(dofs.size))
+    mg = pa.Multigrid(mesh, *dims, 3, symmetric=symmetric)
+    assert mg.levels()[0] == 3
+    mg.setup(K, dofs)
+    mask = np.ones(n, bool)
+    mask[dofs] = False
+    P1, c1 = prolongation(dims, 3, mask)
+    A = oracle.to_scipy(3, 3, *mesh.graph(), K.entries()).toarray()
+    cdims, st1 = mg.level_operator(1)
+    assert cdims == c1 == (4, 2, 2)
+    A1 = stencil_to_dense(cdims, st1)
+    assert rel(A1, P1.T @ A @ P1) < 1e-5
+    P2, c2 = prolongation(cdims, 3, np.ones(A1.shape[0], bool))
+    cdims2, st2 = mg.level_operator(2)
+    assert cdims2 == c2 == (2, 1, 1)
+    assert rel(stencil_to_dense(cdims2, st2), P2.T @ A1 @ P2) < 1e-5
+
+
 def test_vcycle_is_symmetric(pa):
     dims = (8, 4, 4)
     mesh, vf, sf, z, dofs = elastic_setup(pa, dims, 5)
