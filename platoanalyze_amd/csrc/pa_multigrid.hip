@@ -295,36 +295,54 @@ __global__ __launch_bounds__(256) void k_rap_gather(Grid gf, Grid gc, RapLanes r
     for (int q = 0; q < BS; ++q) acc[q] = 0.0f;
     if (active && gc.in(CI, CJ, CK)) {
         const int t0 = (desc >> 5) & 0x3fff, t1 = t0 + ((desc >> 19) & 0xff);
-        for (int t = t0; t < t1; ++t) {
-            const unsigned e = tab[t];
-            const int as = (int)(e & 31u), slot = (int)((e >> 5) & 31u), ds = (int)((e >> 11) & 31u);
-            const bool tr = (e >> 10) & 1u;
-            const int i = 2 * I + as / 9 - 1, j = 2 * J + (as / 3) % 3 - 1, k = 2 * K + as % 3 - 1;
-            const int ii = i + ds / 9 - 1, jj = j + (ds / 3) % 3 - 1, kk = k + ds % 3 - 1;
-            if (!gf.in(i, j, k) || !gf.in(ii, jj, kk)) continue;
-            const float wgt = __builtin_amdgcn_ldexpf(1.0f, -(int)((e >> 16) & 7u));
-            const long fi = gf.id(i, j, k), fj = gf.id(ii, jj, kk);
-            const float* blk = Af + ((tr ? fj : fi) * SSf + slot) * BS;
-            float v[BS];
+        // branch-free body, U pairs per trip so that their loads are in
+        // flight together; a pair outside the grid or past the chunk reads the
+        // coarse node's own fine node and adds zero
+        constexpr int U = D == 4 ? 2 : 4;
+        const long f0 = gf.id(2 * I, 2 * J, 2 * K);
+        for (int t = t0; t < t1; t += U) {
+            float v[U][BS], wgt[U];
+            unsigned bi[U], bj[U], eu[U];
+            bool tr[U];
 #pragma unroll
-            for (int q = 0; q < BS; ++q) {
-                v[q] = blk[q];
-                asm volatile("" : "+v"(v[q]));  // keep the block in registers (no per-element address select)
+            for (int u = 0; u < U; ++u) eu[u] = tab[min(t + u, t1 - 1)];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const unsigned e = eu[u];
+                const int as = (int)(e & 31u), slot = (int)((e >> 5) & 31u), ds = (int)((e >> 11) & 31u);
+                tr[u] = (e >> 10) & 1u;
+                const int i = 2 * I + as / 9 - 1, j = 2 * J + (as / 3) % 3 - 1, k = 2 * K + as % 3 - 1;
+                const int ii = i + ds / 9 - 1, jj = j + (ds / 3) % 3 - 1, kk = k + ds % 3 - 1;
+                const bool ok = ((int)(t + u < t1) & (int)gf.in(i, j, k) & (int)gf.in(ii, jj, kk)) != 0;
+                wgt[u] = ok ? __builtin_amdgcn_ldexpf(1.0f, -(int)((e >> 16) & 7u)) : 0.0f;
+                const long fi = ok ? gf.id(i, j, k) : f0, fj = ok ? gf.id(ii, jj, kk) : f0;
+                const float* blk = Af + ((tr[u] ? fj : fi) * SSf + slot) * BS;
+#pragma unroll
+                for (int q = 0; q < BS; ++q) v[u][q] = blk[q];
+                bi[u] = MASK ? mbits[fi] : 0xffu;
+                bj[u] = MASK ? mbits[fj] : 0xffu;
             }
-            float mi[D], mj[D];
-            const unsigned bi = MASK ? mbits[fi] : 0xffu, bj = MASK ? mbits[fj] : 0xffu;
+            // the blocks are used from registers (not re-read through selected addresses)
 #pragma unroll
-            for (int p = 0; p < D; ++p) {
-                mi[p] = ((bi >> p) & 1u) ? wgt : 0.0f;
-                mj[p] = ((bj >> p) & 1u) ? 1.0f : 0.0f;
-            }
+            for (int u = 0; u < U; ++u)
 #pragma unroll
-            for (int p = 0; p < D; ++p)
+                for (int q = 0; q < BS; ++q) asm volatile("" : "+v"(v[u][q]));
 #pragma unroll
-                for (int q = 0; q < D; ++q) {
-                    const float a = tr ? v[q * D + p] : v[p * D + q];
-                    acc[p * D + q] = fmaf(mi[p] * mj[q], a, acc[p * D + q]);
+            for (int u = 0; u < U; ++u) {
+                float mi[D], mj[D];
+#pragma unroll
+                for (int p = 0; p < D; ++p) {
+                    mi[p] = ((bi[u] >> p) & 1u) ? wgt[u] : 0.0f;
+                    mj[p] = ((bj[u] >> p) & 1u) ? 1.0f : 0.0f;
                 }
+#pragma unroll
+                for (int p = 0; p < D; ++p)
+#pragma unroll
+                    for (int q = 0; q < D; ++q) {
+                        const float a = tr[u] ? v[u][q * D + p] : v[u][p * D + q];
+                        acc[p * D + q] = fmaf(mi[p] * mj[q], a, acc[p * D + q]);
+                    }
+            }
         }
     }
 #pragma unroll
