@@ -192,70 +192,79 @@ __global__ __launch_bounds__(kT) void k_fst_resid_jac(Grid g, BlockStencil fs, c
 }
 
 // ------------------------------------------------------------------ transfer
-// weight of coarse coordinate C for fine coordinate f (|f - 2C| <= 1)
-__device__ inline float w1(int f, int C) {
-    const int d = f - 2 * C;
-    return d == 0 ? 1.0f : ((d == 1 || d == -1) ? 0.5f : 0.0f);
-}
-
 // b_c[J] = sum_i w_iJ mask_i r_f[i]
-// One 32-lane group per coarse node: lane t < 27 owns the fine vertex
-// 2J + (t/9 - 1, t/3 % 3 - 1, t % 3 - 1); the group sums the D components.
-template <int D>
+// A quad of lanes per coarse node: lane l takes the fine vertices 2J +
+// offset(t), t = l, l + 4, ... < 27, branch-free (a vertex outside the fine
+// grid reads the centre vertex with weight 0), then the quad sums by DPP and
+// lane p < D writes component p.  MASK: bit p of mbits[i] is dof p of fine
+// vertex i being free.
+template <int D, bool MASK>
 __global__ __launch_bounds__(kT) void k_restrict(Grid gf, Grid gc, const float* __restrict__ rf,
-                                                 const unsigned char* __restrict__ mask, float* __restrict__ bc) {
-    constexpr int G = 32, NODES = kT / G;
-    const int grp = threadIdx.x / G, t = threadIdx.x % G;
-    const int n = blockIdx.x * NODES + grp;
+                                                 const unsigned char* __restrict__ mbits, float* __restrict__ bc) {
+    constexpr int NODES = kT / 4, NV = 7;
+    const int l = threadIdx.x & 3;
+    const int n = blockIdx.x * NODES + (threadIdx.x >> 2);
     if (n >= gc.nodes()) return;
+    int I, J, K;
+    gc.ijk(n, I, J, K);
+    const long f0 = gf.id(2 * I, 2 * J, 2 * K);
+    float v[NV][D], w[NV];
+    unsigned b[NV];
+#pragma unroll
+    for (int m = 0; m < NV; ++m) {
+        const int t = l + 4 * m;
+        const int a = t / 9 - 1, c = (t / 3) % 3 - 1, e = t % 3 - 1;
+        const int i = 2 * I + a, j = 2 * J + c, k = 2 * K + e;
+        const bool ok = t < 27 && gf.in(i, j, k);
+        w[m] = ok ? (a ? 0.5f : 1.0f) * (c ? 0.5f : 1.0f) * (e ? 0.5f : 1.0f) : 0.0f;
+        const long f = ok ? gf.id(i, j, k) : f0;
+#pragma unroll
+        for (int p = 0; p < D; ++p) v[m][p] = rf[f * D + p];
+        b[m] = MASK ? mbits[f] : 0xffu;
+    }
     float acc[D];
 #pragma unroll
-    for (int p = 0; p < D; ++p) acc[p] = 0.0f;
-    if (t < 27) {
-        int I, J, K;
-        gc.ijk(n, I, J, K);
-        const int i = 2 * I + t / 9 - 1, j = 2 * J + (t / 3) % 3 - 1, k = 2 * K + t % 3 - 1;
-        if (gf.in(i, j, k)) {
-            const float w = w1(i, I) * w1(j, J) * w1(k, K);
-            const long f = gf.id(i, j, k);
+    for (int p = 0; p < D; ++p) {
+        acc[p] = 0.0f;
 #pragma unroll
-            for (int p = 0; p < D; ++p)
-                acc[p] = (mask == nullptr || mask[f * D + p]) ? w * rf[f * D + p] : 0.0f;
-        }
+        for (int m = 0; m < NV; ++m) acc[p] += ((b[m] >> p) & 1u) ? w[m] * v[m][p] : 0.0f;
+        acc[p] = node_lanes_sum<4>(acc[p]);
     }
+    float al = acc[0];
 #pragma unroll
-    for (int p = 0; p < D; ++p)
-#pragma unroll
-        for (int o = G / 2; o > 0; o >>= 1) acc[p] += __shfl_xor(acc[p], o, G);
-    if (t == 0) {
-#pragma unroll
-        for (int p = 0; p < D; ++p) bc[(long)n * D + p] = acc[p];
-    }
+    for (int p = 1; p < D; ++p) al = l == p ? acc[p] : al;
+    if (l < D) bc[(long)n * D + l] = al;
 }
 
-// x_f[i] += mask_i sum_J w_iJ x_c[J]
-template <int D>
-__global__ void k_prolong_add(Grid gf, Grid gc, const float* __restrict__ xc, const unsigned char* __restrict__ mask,
+// x_f[i] += mask_i sum_J w_iJ x_c[J]: thread per fine node, its eight
+// candidate parents per trilinear cell read branch-free (a parent repeated
+// for an even coordinate has weight 0).
+template <int D, bool MASK>
+__global__ void k_prolong_add(Grid gf, Grid gc, const float* __restrict__ xc, const unsigned char* __restrict__ mbits,
                               float* __restrict__ xf) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= gf.nodes()) return;
     int i, j, k;
     gf.ijk(n, i, j, k);
-    const int I0 = i >> 1, I1 = (i + 1) >> 1, J0 = j >> 1, J1 = (j + 1) >> 1, K0 = k >> 1, K1 = (k + 1) >> 1;
-    float acc[D];
+    const int I[2] = {i >> 1, (i + 1) >> 1}, J[2] = {j >> 1, (j + 1) >> 1}, K[2] = {k >> 1, (k + 1) >> 1};
+    const float wi[2] = {(i & 1) ? 0.5f : 1.0f, (i & 1) ? 0.5f : 0.0f};
+    const float wj[2] = {(j & 1) ? 0.5f : 1.0f, (j & 1) ? 0.5f : 0.0f};
+    const float wk[2] = {(k & 1) ? 0.5f : 1.0f, (k & 1) ? 0.5f : 0.0f};
+    float v[8][D];
 #pragma unroll
-    for (int p = 0; p < D; ++p) acc[p] = 0.0f;
-    for (int I = I0; I <= I1; ++I)
-        for (int J = J0; J <= J1; ++J)
-            for (int K = K0; K <= K1; ++K) {
-                const float w = w1(i, I) * w1(j, J) * w1(k, K);
-                const long c = gc.id(I, J, K);
+    for (int q = 0; q < 8; ++q) {
+        const long c = gc.id(I[q >> 2], J[(q >> 1) & 1], K[q & 1]);
 #pragma unroll
-                for (int p = 0; p < D; ++p) acc[p] += w * xc[c * D + p];
-            }
+        for (int p = 0; p < D; ++p) v[q][p] = xc[c * D + p];
+    }
+    const unsigned b = MASK ? mbits[n] : 0xffu;
 #pragma unroll
-    for (int p = 0; p < D; ++p)
-        if (mask == nullptr || mask[(long)n * D + p]) xf[(long)n * D + p] += acc[p];
+    for (int p = 0; p < D; ++p) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc += (wi[q >> 2] * wj[(q >> 1) & 1] * wk[q & 1]) * v[q][p];
+        if ((b >> p) & 1u) xf[(long)n * D + p] += acc;
+    }
 }
 
 // --------------------------------------------------------- Galerkin products
@@ -564,10 +573,16 @@ void vcycle(pa_multigrid* mg, int l, hipStream_t st) {
     // residual, restriction, coarse correction
     sweep(x, t, 0.0f, 0);
     const Grid gc = mg->grids[l + 1];
-    k_restrict<D><<<nblk(gc.nodes(), kT / 32), kT, 0, st>>>(g, gc, t, l == 0 ? mg->mask.p : nullptr, mg->b[l + 1]->p);
+    if (l == 0)
+        k_restrict<D, true><<<nblk(gc.nodes(), kT / 4), kT, 0, st>>>(g, gc, t, mg->mbits.p, mg->b[l + 1]->p);
+    else
+        k_restrict<D, false><<<nblk(gc.nodes(), kT / 4), kT, 0, st>>>(g, gc, t, nullptr, mg->b[l + 1]->p);
     chk();
     vcycle<D>(mg, l + 1, st);
-    k_prolong_add<D><<<nblk(g.nodes()), kT, 0, st>>>(g, gc, mg->x[l + 1]->p, l == 0 ? mg->mask.p : nullptr, x);
+    if (l == 0)
+        k_prolong_add<D, true><<<nblk(g.nodes()), kT, 0, st>>>(g, gc, mg->x[l + 1]->p, mg->mbits.p, x);
+    else
+        k_prolong_add<D, false><<<nblk(g.nodes()), kT, 0, st>>>(g, gc, mg->x[l + 1]->p, nullptr, x);
     chk();
     // post-smoothing
     for (int s = 0; s < std::max(1, mg->opt.post_sweeps); ++s) {
