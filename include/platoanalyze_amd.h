@@ -276,6 +276,12 @@ enum { PA_PRECOND_NONE = 0, PA_PRECOND_JACOBI = 1, PA_PRECOND_BLOCK_JACOBI = 2, 
  * coarsest level.  FP32 throughout (a preconditioner).  No counterpart in the
  * reference (its AmgX path, src/alg/AmgXConfigs.cpp, is algebraic multigrid).
  */
+/* Smoothers: damped block Jacobi with `omega`, or the Chebyshev polynomial of
+   degree `pre_sweeps` / `post_sweeps` in D^-1 A (for symmetric positive definite
+   operators) on [b/12, b], b = 1.2 x the largest eigenvalue of D^-1 A that power
+   iteration finds on each level at setup.  DEFAULT: Chebyshev for the elliptic
+   problems' multigrid, Jacobi otherwise. */
+enum { PA_SMOOTHER_DEFAULT = 0, PA_SMOOTHER_JACOBI = 1, PA_SMOOTHER_CHEBYSHEV = 2 };
 typedef struct {
     int max_levels;       /* 0: coarsen while possible */
     int pre_sweeps;       /* block-Jacobi sweeps before the coarse correction */
@@ -285,6 +291,7 @@ typedef struct {
     int symmetric;        /* 1: the matrices given to setup are symmetric, block (m, n) the transpose of
                              (n, m); the fine level then keeps half of its block stencil (the elliptic
                              problems set this themselves) */
+    int smoother;         /* PA_SMOOTHER_* */
 } pa_multigrid_options;
 
 typedef struct pa_multigrid pa_multigrid;

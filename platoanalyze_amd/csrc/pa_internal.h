@@ -242,9 +242,10 @@ void apply_block_constraints(const pa_mesh* m, int dpn, void* vals, int precisio
                              hipStream_t st);
 // z = M^-1 r by one multigrid V-cycle (pa_multigrid.hip)
 void multigrid_apply(pa_multigrid* mg, const double* r, double* z, hipStream_t st);
-// the operators given to pa_multigrid_setup are symmetric (the elliptic
-// problems): the fine level keeps half of its block stencil
-void multigrid_set_symmetric(pa_multigrid* mg, bool symmetric);
+// the operators given to pa_multigrid_setup are symmetric positive definite
+// (the elliptic problems): the fine level keeps half of its block stencil and
+// the default smoother is Chebyshev
+void multigrid_set_elliptic(pa_multigrid* mg);
 }  // namespace pa
 
 struct pa_scalar_function {

@@ -694,7 +694,7 @@ int pa_problem_set_multigrid(pa_problem* P, int nx, int ny, int nz, const pa_mul
         pa::check_status(pa_multigrid_create(P->vf->mesh, nx, ny, nz, P->vf->dpn, mopt, &mg));
         // elastostatic and thermal Jacobians are symmetric, and so are they
         // after the symmetric Dirichlet elimination
-        pa::multigrid_set_symmetric(mg, true);
+        pa::multigrid_set_elliptic(mg);
         if (P->mg) pa_multigrid_destroy(P->mg);
         P->mg = mg;
         pa_solver* s = nullptr;

@@ -5,7 +5,8 @@
   Dirichlet dofs masked out (FP32 storage: 1e-5 relative);
 * MG-preconditioned CG solves the SIMP problem to the FP64 tolerance and
   agrees with the block-Jacobi solution; it needs far fewer iterations;
-* the V-cycle is symmetric (x.M y == y.M x), as CG requires;
+* the V-cycle is symmetric (x.M y == y.M x), as CG requires, with damped
+  Jacobi and with Chebyshev smoothing;
 * with a symmetric operator the fine level's half stencil (the diagonal and
   the blocks towards larger offsets, the others read transposed from the
   neighbour) gives the full stencil's V-cycle bit for bit.
@@ -140,7 +141,12 @@ def test_galerkin_second_level(pa, symmetric):
     assert rel(stencil_to_dense(cdims2, st2), P2.T @ A1 @ P2) < 1e-5
 
 
-def test_vcycle_is_symmetric(pa):
+@pytest.mark.parametrize("smoother", ["jacobi", "chebyshev"])
+def test_vcycle_is_symmetric(pa, smoother):
+    """x.M y == y.M x and M positive, with damped Jacobi and with the
+    Chebyshev smoother (dampings from the power-iteration eigenvalue bound,
+    post-smoothing in reverse order)."""
+    print(f"case: V-cycle symmetry, {smoother}", flush=True)
     dims = (8, 4, 4)
     mesh, vf, sf, z, dofs = elastic_setup(pa, dims, 5)
     n = 3 * mesh.num_nodes
@@ -148,7 +154,7 @@ def test_vcycle_is_symmetric(pa):
     rhs = pa.DeviceArray(n)
     rhs.zero()
     pa.apply_block_constraints(K, rhs, dofs, np.zeros(dofs.size))
-    mg = pa.Multigrid(mesh, *dims, 3)
+    mg = pa.Multigrid(mesh, *dims, 3, symmetric=True, smoother=smoother)
     mg.setup(K, dofs)
     rng = np.random.default_rng(1)
     x, y = rng.standard_normal(n), rng.standard_normal(n)
