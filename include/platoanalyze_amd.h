@@ -279,8 +279,10 @@ enum { PA_PRECOND_NONE = 0, PA_PRECOND_JACOBI = 1, PA_PRECOND_BLOCK_JACOBI = 2, 
 /* Smoothers: damped block Jacobi with `omega`, or the Chebyshev polynomial of
    degree `pre_sweeps` / `post_sweeps` in D^-1 A (for symmetric positive definite
    operators) on [b/12, b], b = 1.2 x the largest eigenvalue of D^-1 A that power
-   iteration finds on each level at setup.  DEFAULT: Chebyshev for the elliptic
-   problems' multigrid, Jacobi otherwise. */
+   iteration finds on each level at setup; the post-smoothing steps run in reverse
+   order, so Chebyshev needs pre_sweeps == post_sweeps.  DEFAULT: Chebyshev for
+   the elliptic problems' multigrid when the sweep counts agree, Jacobi
+   otherwise. */
 enum { PA_SMOOTHER_DEFAULT = 0, PA_SMOOTHER_JACOBI = 1, PA_SMOOTHER_CHEBYSHEV = 2 };
 typedef struct {
     int max_levels;       /* 0: coarsen while possible */

@@ -820,7 +820,9 @@ void setup_impl(pa_multigrid* mg, const VT* vals, hipStream_t st) {
 // default smoother is Chebyshev.
 void multigrid_set_elliptic(pa_multigrid* mg) {
     mg->symmetric = true;
-    if (mg->opt.smoother == PA_SMOOTHER_DEFAULT) mg->chebyshev = true;
+    // (a Chebyshev V-cycle is symmetric only when the post-smoother is the
+    // pre-smoother's adjoint: the same number of sweeps)
+    if (mg->opt.smoother == PA_SMOOTHER_DEFAULT && mg->opt.pre_sweeps == mg->opt.post_sweeps) mg->chebyshev = true;
 }
 
 // used by the PCG of pa_solver.hip
@@ -862,6 +864,8 @@ int pa_multigrid_create(const pa_mesh* mesh, int nx, int ny, int nz, int dpn, co
             PA_REQUIRE(opt->smoother >= PA_SMOOTHER_DEFAULT && opt->smoother <= PA_SMOOTHER_CHEBYSHEV, PA_ERR_ARG,
                        "pa_multigrid_create: unknown smoother");
             mg->chebyshev = opt->smoother == PA_SMOOTHER_CHEBYSHEV;
+            PA_REQUIRE(!mg->chebyshev || opt->pre_sweeps == opt->post_sweeps, PA_ERR_ARG,
+                       "pa_multigrid_create: Chebyshev smoothing needs as many post- as pre-smoothing sweeps");
             mg->dpn = dpn;
             mg->opt = *opt;
             if (mg->opt.omega <= 0.0) mg->opt.omega = 0.6;
