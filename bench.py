@@ -181,7 +181,7 @@ def compliance_gradient(pa, mesh, vf, z, dims):
         P = pa.Problem(vf, sf, dofs, mode=mode, precond=pa.PRECOND_BLOCK_JACOBI, tol=1e-8, max_iter=100000,
                        check_every=50)
         if name != "plain_fp64_block_jacobi":
-            P.set_multigrid(*dims, pre_sweeps=2, post_sweeps=2, omega=0.6)
+            P.set_multigrid(*dims, pre_sweeps=2, post_sweeps=2)  # Chebyshev smoothing (the elliptic default)
             P.set_options(check_every=5)
         u = P.solution(z)  # warm-up (allocations, code objects)
         times = []
@@ -214,12 +214,14 @@ def compliance_gradient(pa, mesh, vf, z, dims):
 J2_DIMS, J2_LENGTHS = (64, 32, 32), (2.0, 1.0, 1.0)
 J2_PARAMS = dict(youngs=1e3, poisson=0.3, hardening_isotropic=10.0, hardening_kinematic=0.0, initial_yield_stress=1.0)
 J2_END_DISPLACEMENT, J2_STEPS = 0.008, 10
-# damped block-Jacobi smoothing of the J2 multigrid per formulation: the mixed
-# form's 4x4 saddle-point blocks need a smaller damping and more sweeps
-# (forward solve, 64x32x32: 4+4 sweeps at 0.6 took 2.58 s and 4,108 FGMRES
-# iterations; 5+5 at 0.45 1.77 s and 2,559; 0.7 diverges towards 12,900);
-# the displacement form 0.81 s at 4+4 / 0.6, 0.75 s at 3+3 / 0.7
-J2_SMOOTHING = {"mixed": (5, 0.45), "displacement": (3, 0.7)}
+# smoothing of the J2 multigrid per formulation (sweeps, damping, smoother):
+# the mixed form's 4x4 saddle-point blocks are indefinite, so damped block
+# Jacobi with a smaller damping and more sweeps (forward solve, 64x32x32:
+# 4+4 sweeps at 0.6 took 4,108 FGMRES iterations, 5+5 at 0.45 2,559, 0.7
+# diverges towards 12,900); the displacement form's tangent is positive
+# definite, so Chebyshev (3+3: 1,337 iterations against 1,525 with Jacobi
+# at 0.7)
+J2_SMOOTHING = {"mixed": (5, 0.45, "jacobi"), "displacement": (3, 0.7, "chebyshev")}
 
 
 WARMSTART_WEIGHTS = ROOT / "platoanalyze_amd" / "data" / "warmstart_96x48x48.npz"
@@ -334,9 +336,9 @@ def j2_plasticity(pa, stream, with_cpu=True, formulation="mixed"):
     # flexible GMRES(200) to 1e-8 relative, preconditioned by the DxD-block
     # geometric multigrid with 4 pre- and post-smoothing sweeps
     prob = pa.PlasticityProblem(pl, dofs, vals, num_steps=J2_STEPS, max_newton=10, linear_tol=1e-8, restart=200)
-    sweeps, omega = J2_SMOOTHING[formulation]
+    sweeps, omega, smoother = J2_SMOOTHING[formulation]
     if os.environ.get("PA_J2_PRECOND", "multigrid") == "multigrid":
-        prob.set_multigrid(*J2_DIMS, pre_sweeps=sweeps, post_sweeps=sweeps, omega=omega)
+        prob.set_multigrid(*J2_DIMS, pre_sweeps=sweeps, post_sweeps=sweeps, omega=omega, smoother=smoother)
     pa.synchronize()
     t0 = time.perf_counter()
     converged = prob.solution(z)
@@ -352,7 +354,7 @@ def j2_plasticity(pa, stream, with_cpu=True, formulation="mixed"):
     pck = pa.PlasticityProblem(pl, dofs, vals, num_steps=J2_STEPS, max_newton=10, linear_tol=1e-8, restart=200,
                                checkpoint_stride=m_ck)
     if os.environ.get("PA_J2_PRECOND", "multigrid") == "multigrid":
-        pck.set_multigrid(*J2_DIMS, pre_sweeps=sweeps, post_sweeps=sweeps, omega=omega)
+        pck.set_multigrid(*J2_DIMS, pre_sweeps=sweeps, post_sweeps=sweeps, omega=omega, smoother=smoother)
     pck.solution(z)
     pa.synchronize()
     t0 = time.perf_counter()
@@ -393,7 +395,8 @@ def j2_plasticity(pa, stream, with_cpu=True, formulation="mixed"):
     out = {"workload": f"J2 plasticity {name} Tet4, {J2_DIMS[0]}x{J2_DIMS[1]}x{J2_DIMS[2]} hex x6 "
                        f"({C} cells, {N} nodes, {D * N} dof), {J2_STEPS} steps to ux={J2_END_DISPLACEMENT}",
            "solver": f"condensed Newton; FGMRES(200) to 1e-8, {D}x{D}-block geometric multigrid "
-                     f"({sweeps}+{sweeps} damped block-Jacobi sweeps, omega {omega})",
+                     + (f"({sweeps}+{sweeps} damped block-Jacobi sweeps, omega {omega})" if smoother == "jacobi"
+                        else f"({sweeps}+{sweeps} Chebyshev sweeps)"),
            "forward_solve_s": t_fwd, "converged": converged,
            "criterion_value_and_adjoint_gradient_s": t_adj, "plastic_work": F,
            "gradient_norm": float(np.linalg.norm(g.numpy())),

@@ -166,6 +166,15 @@ def test_vcycle_is_symmetric(pa, smoother):
     assert x @ Mx > 0 and y @ My > 0
 
 
+def test_chebyshev_needs_equal_sweeps(pa):
+    """A Chebyshev V-cycle is symmetric only with the post-smoother the
+    adjoint of the pre-smoother: unequal sweep counts are refused."""
+    mesh = pa.Mesh.box(4, 2, 2, 2.0, 1.0, 1.0)
+    with pytest.raises(pa.PlatoError):
+        pa.Multigrid(mesh, 4, 2, 2, 3, pre_sweeps=1, post_sweeps=2, smoother="chebyshev")
+    pa.Multigrid(mesh, 4, 2, 2, 3, pre_sweeps=1, post_sweeps=2, smoother="jacobi")
+
+
 @pytest.mark.parametrize("physics", ["Mechanics", "Thermal"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_half_stencil_vcycle_bitwise(pa, physics, precision):
