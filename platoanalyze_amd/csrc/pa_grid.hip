@@ -1090,7 +1090,18 @@ __global__ __launch_bounds__(BX * BY * BZ * kJ3Slots) void k_grid_jacobian3(Grid
                 for (int q = 0; q < 9; ++q) dst[q] = (VT)K[q];
             }
         }
-        __syncthreads();
+        if constexpr (WR && BZ * kJ3Slots == 64) {
+            // a line's rows are one wave's rows: the wave stores the line it
+            // staged, so only the wave waits for its LDS writes (measured:
+            // value_gradient_u 0.2060 -> 0.2005 ms; gradient_u alone, with no
+            // row sums after the stores, 0.1795 -> 0.1900 ms, so it keeps the
+            // workgroup barrier)
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        } else {
+            __syncthreads();
+        }
         // stage 4: wave w stores lines w, w + waves, ...: each one contiguous CRS
         // range, in 16-byte pairs (a single value first when the range starts
         // at an odd double, and last when one is left over)
