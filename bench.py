@@ -159,6 +159,14 @@ def cpu_baseline(nx_slab=32):
     return out
 
 
+# multigrid of the elastic legs: Chebyshev smoothing (the elliptic default),
+# 2+2 sweeps on the fine grid and the first coarse level, 6+6 on the coarse
+# levels of at most 16,384 nodes, whose sweeps cost little more than a
+# launch (128x64x64: 60 PCG iterations and 0.046 s against 85 and 0.052 s
+# with 2+2 everywhere; 8+8: 55 iterations, no faster)
+MG_ELASTIC = dict(pre_sweeps=2, post_sweeps=2, coarse_sweeps=6)
+
+
 def compliance_gradient(pa, mesh, vf, z, dims):
     """Second metric: time from the density on the device to the compliance
     and its gradient on the device at ||r|| <= 1e-8 ||b|| (FP64 residual),
@@ -181,7 +189,7 @@ def compliance_gradient(pa, mesh, vf, z, dims):
         P = pa.Problem(vf, sf, dofs, mode=mode, precond=pa.PRECOND_BLOCK_JACOBI, tol=1e-8, max_iter=100000,
                        check_every=50)
         if name != "plain_fp64_block_jacobi":
-            P.set_multigrid(*dims, pre_sweeps=2, post_sweeps=2)  # Chebyshev smoothing (the elliptic default)
+            P.set_multigrid(*dims, **MG_ELASTIC)
             P.set_options(check_every=5)
         u = P.solution(z)  # warm-up (allocations, code objects)
         times = []
@@ -494,7 +502,7 @@ def main():
                                       penalty_exponent=3.0, penalty_min=1e-9)
         dofs = (3 * mesh.node_set("x-")[:, None] + np.arange(3)).ravel()
         P = pa.Problem(vf, sf, dofs, mode="fp64", tol=1e-8, max_iter=2000, check_every=5)
-        P.set_multigrid(*dims, pre_sweeps=2, post_sweeps=2, omega=0.6)
+        P.set_multigrid(*dims, **MG_ELASTIC)
         for _ in range(max(1, args.steps)):
             u = P.solution(z)
             P.criterion_gradient(z, u)

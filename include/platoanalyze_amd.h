@@ -294,6 +294,9 @@ typedef struct {
                              (n, m); the fine level then keeps half of its block stencil (the elliptic
                              problems set this themselves) */
     int smoother;         /* PA_SMOOTHER_* */
+    int coarse_sweeps;    /* pre- and post-smoothing sweeps on the coarse levels of at most coarse_max_nodes
+                             nodes, whose sweeps cost little more than a launch (0: as pre/post_sweeps) */
+    int coarse_max_nodes; /* 0: 16384 */
 } pa_multigrid_options;
 
 typedef struct pa_multigrid pa_multigrid;

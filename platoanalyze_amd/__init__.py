@@ -55,7 +55,7 @@ SMOOTHERS = {"default": 0, "jacobi": 1, "chebyshev": 2}
 class MultigridOptions(ctypes.Structure):
     _fields_ = [("max_levels", ctypes.c_int), ("pre_sweeps", ctypes.c_int), ("post_sweeps", ctypes.c_int),
                 ("omega", ctypes.c_double), ("max_coarse_nodes", ctypes.c_int), ("symmetric", ctypes.c_int),
-                ("smoother", ctypes.c_int)]
+                ("smoother", ctypes.c_int), ("coarse_sweeps", ctypes.c_int), ("coarse_max_nodes", ctypes.c_int)]
 
 
 class ProblemOptions(ctypes.Structure):
@@ -676,9 +676,10 @@ class Problem:
         _check(lib.pa_problem_set_options(self.handle, ctypes.byref(self.options)))
 
     def set_multigrid(self, nx, ny, nz, *, max_levels=0, pre_sweeps=2, post_sweeps=2, omega=0.6,
-                      max_coarse_nodes=1000, smoother="default"):
+                      max_coarse_nodes=1000, smoother="default", coarse_sweeps=0, coarse_max_nodes=0):
         """Geometric multigrid preconditioner on the nx x ny x nz box mesh."""
-        opts = MultigridOptions(max_levels, pre_sweeps, post_sweeps, omega, max_coarse_nodes, 0, SMOOTHERS[smoother])
+        opts = MultigridOptions(max_levels, pre_sweeps, post_sweeps, omega, max_coarse_nodes, 0, SMOOTHERS[smoother],
+                                coarse_sweeps, coarse_max_nodes)
         _check(lib.pa_problem_set_multigrid(self.handle, nx, ny, nz, ctypes.byref(opts)))
         self.options.precond = PRECOND_MULTIGRID
 
@@ -721,9 +722,9 @@ class Multigrid:
     """Geometric multigrid preconditioner on a box mesh (FP32 V-cycles)."""
 
     def __init__(self, mesh, nx, ny, nz, dpn, *, max_levels=0, pre_sweeps=2, post_sweeps=2, omega=0.6,
-                 max_coarse_nodes=1000, symmetric=False, smoother="default"):
+                 max_coarse_nodes=1000, symmetric=False, smoother="default", coarse_sweeps=0, coarse_max_nodes=0):
         opts = MultigridOptions(max_levels, pre_sweeps, post_sweeps, omega, max_coarse_nodes, int(symmetric),
-                                SMOOTHERS[smoother])
+                                SMOOTHERS[smoother], coarse_sweeps, coarse_max_nodes)
         h = ctypes.c_void_p()
         _check(lib.pa_multigrid_create(mesh.handle, nx, ny, nz, dpn, ctypes.byref(opts), ctypes.byref(h)))
         self.handle, self.mesh, self.dpn = h, mesh, dpn
@@ -882,9 +883,10 @@ class PlasticityProblem:
         self.num_steps = num_steps
 
     def set_multigrid(self, nx, ny, nz, *, max_levels=0, pre_sweeps=2, post_sweeps=2, omega=0.6,
-                      max_coarse_nodes=1000, smoother="default"):
+                      max_coarse_nodes=1000, smoother="default", coarse_sweeps=0, coarse_max_nodes=0):
         """Geometric multigrid (dpn x dpn nodal blocks) as the Krylov preconditioner."""
-        opts = MultigridOptions(max_levels, pre_sweeps, post_sweeps, omega, max_coarse_nodes, 0, SMOOTHERS[smoother])
+        opts = MultigridOptions(max_levels, pre_sweeps, post_sweeps, omega, max_coarse_nodes, 0, SMOOTHERS[smoother],
+                                coarse_sweeps, coarse_max_nodes)
         _check(lib.pa_plasticity_problem_set_multigrid(self.handle, nx, ny, nz, ctypes.byref(opts)))
 
     def state_bytes(self):

@@ -626,6 +626,14 @@ void sweep_level(pa_multigrid* mg, int l, const float* b, const float* xin, floa
     chk();
 }
 
+// Smoothing sweeps of level l (pre or post): coarse_sweeps on the coarse
+// levels of at most coarse_max_nodes nodes when set.
+int level_sweeps(const pa_multigrid* mg, int l, bool pre) {
+    const int dflt = std::max(1, pre ? mg->opt.pre_sweeps : mg->opt.post_sweeps);
+    const int lim = mg->opt.coarse_max_nodes > 0 ? mg->opt.coarse_max_nodes : 16384;
+    return (l > 0 && mg->opt.coarse_sweeps > 0 && mg->grids[l].nodes() <= lim) ? mg->opt.coarse_sweeps : dflt;
+}
+
 template <int D>
 void vcycle(pa_multigrid* mg, int l, hipStream_t st) {
     const int L = (int)mg->grids.size();
@@ -646,7 +654,7 @@ void vcycle(pa_multigrid* mg, int l, hipStream_t st) {
         sweep_level<D>(mg, l, b, xin, xout, om, om_dev, zero, st);
     };
     // pre-smoothing (ping-pong x <-> t; the first sweep starts from zero)
-    const int pre = std::max(1, mg->opt.pre_sweeps);
+    const int pre = level_sweeps(mg, l, true);
     sweep(x, x, w, dev(0), 1);
     for (int s = 1; s < pre; ++s) {
         sweep(x, t, w, dev(s), 0);
@@ -669,7 +677,7 @@ void vcycle(pa_multigrid* mg, int l, hipStream_t st) {
         k_prolong_add<D, false><<<nblk(g.nodes()), kT, 0, st>>>(g, gc, mg->x[l + 1]->p, nullptr, x);
     chk();
     // post-smoothing
-    for (int s = 0; s < std::max(1, mg->opt.post_sweeps); ++s) {
+    for (int s = 0, post = level_sweeps(mg, l, false); s < post; ++s) {
         sweep(x, t, w, dev(kChebMax + s), 0);
         std::swap(mg->x[l], mg->t[l]);
         x = mg->x[l]->p;
@@ -807,7 +815,7 @@ void setup_impl(pa_multigrid* mg, const VT* vals, hipStream_t st) {
             chk();
         }
         k_cheb_dampings<<<1, 256, 0, st>>>(mg->powpart.p + ((K - 2) & 1) * 1024, mg->powpart.p + ((K - 1) & 1) * 1024,
-                                           nb, std::max(1, mg->opt.pre_sweeps), std::max(1, mg->opt.post_sweeps),
+                                           nb, level_sweeps(mg, l, true), level_sweeps(mg, l, false),
                                            mg->omegas.p + (long)l * 2 * kChebMax);
         chk();
     }
@@ -863,6 +871,8 @@ int pa_multigrid_create(const pa_mesh* mesh, int nx, int ny, int nz, int dpn, co
             mg->symmetric = opt->symmetric != 0;
             PA_REQUIRE(opt->smoother >= PA_SMOOTHER_DEFAULT && opt->smoother <= PA_SMOOTHER_CHEBYSHEV, PA_ERR_ARG,
                        "pa_multigrid_create: unknown smoother");
+            PA_REQUIRE(opt->coarse_sweeps >= 0 && opt->coarse_max_nodes >= 0, PA_ERR_ARG,
+                       "pa_multigrid_create: negative coarse_sweeps or coarse_max_nodes");
             mg->chebyshev = opt->smoother == PA_SMOOTHER_CHEBYSHEV;
             PA_REQUIRE(!mg->chebyshev || opt->pre_sweeps == opt->post_sweeps, PA_ERR_ARG,
                        "pa_multigrid_create: Chebyshev smoothing needs as many post- as pre-smoothing sweeps");
@@ -935,7 +945,8 @@ int pa_multigrid_setup(pa_multigrid* mg, const void* vals, int precision, int nb
             pa::k_mask_bc<<<pa::nblk(nbc), pa::kT, 0, st>>>(nbc, bc_dofs, mg->mask.p);
             pa::chk();
         }
-        PA_REQUIRE(!mg->chebyshev || (mg->opt.pre_sweeps <= pa::kChebMax && mg->opt.post_sweeps <= pa::kChebMax),
+        PA_REQUIRE(!mg->chebyshev || (mg->opt.pre_sweeps <= pa::kChebMax && mg->opt.post_sweeps <= pa::kChebMax &&
+                                      mg->opt.coarse_sweeps <= pa::kChebMax),
                    PA_ERR_ARG, "pa_multigrid_setup: at most 8 Chebyshev sweeps");
         // the smoother reads an FP32 stencil copy of the fine matrix
         mg->fsl.assign(1, pa::make_block_stencil(mg->grids[0], mg->graph_offsets, mg->symmetric));
