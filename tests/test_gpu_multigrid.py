@@ -141,12 +141,12 @@ def test_galerkin_second_level(pa, symmetric):
     assert rel(stencil_to_dense(cdims2, st2), P2.T @ A1 @ P2) < 1e-5
 
 
-@pytest.mark.parametrize("smoother", ["jacobi", "chebyshev"])
-def test_vcycle_is_symmetric(pa, smoother):
+@pytest.mark.parametrize("smoother,coarse_sweeps", [("jacobi", 0), ("chebyshev", 0), ("chebyshev", 6)])
+def test_vcycle_is_symmetric(pa, smoother, coarse_sweeps):
     """x.M y == y.M x and M positive, with damped Jacobi and with the
     Chebyshev smoother (dampings from the power-iteration eigenvalue bound,
     post-smoothing in reverse order)."""
-    print(f"case: V-cycle symmetry, {smoother}", flush=True)
+    print(f"case: V-cycle symmetry, {smoother}, coarse sweeps {coarse_sweeps}", flush=True)
     dims = (8, 4, 4)
     mesh, vf, sf, z, dofs = elastic_setup(pa, dims, 5)
     n = 3 * mesh.num_nodes
@@ -154,7 +154,7 @@ def test_vcycle_is_symmetric(pa, smoother):
     rhs = pa.DeviceArray(n)
     rhs.zero()
     pa.apply_block_constraints(K, rhs, dofs, np.zeros(dofs.size))
-    mg = pa.Multigrid(mesh, *dims, 3, symmetric=True, smoother=smoother)
+    mg = pa.Multigrid(mesh, *dims, 3, symmetric=True, smoother=smoother, coarse_sweeps=coarse_sweeps)
     mg.setup(K, dofs)
     rng = np.random.default_rng(1)
     x, y = rng.standard_normal(n), rng.standard_normal(n)
@@ -208,15 +208,16 @@ def test_half_stencil_vcycle_bitwise(pa, physics, precision):
 
 
 @pytest.mark.parametrize("mode", ["fp64", "fp32"])
-def test_multigrid_pcg_matches_block_jacobi(pa, mode):
-    print(f"case: MG-PCG 16x8x8 elastic, {mode}", flush=True)
+@pytest.mark.parametrize("coarse_sweeps", [0, 6])
+def test_multigrid_pcg_matches_block_jacobi(pa, mode, coarse_sweeps):
+    print(f"case: MG-PCG 16x8x8 elastic, {mode}, coarse sweeps {coarse_sweeps}", flush=True)
     dims = (16, 8, 8)
     mesh, vf, sf, z, dofs = elastic_setup(pa, dims, 7)
     P = pa.Problem(vf, sf, dofs, mode="fp64", tol=1e-10, max_iter=20000)
     u_bj = P.solution(z).numpy()
     it_bj = P.last_stats["iterations"]
     P2 = pa.Problem(vf, sf, dofs, mode=mode, tol=1e-10, max_iter=2000, replace_every=50)
-    P2.set_multigrid(*dims)
+    P2.set_multigrid(*dims, coarse_sweeps=coarse_sweeps)
     u_mg = P2.solution(z).numpy()
     st = P2.last_stats
     print("block-Jacobi", it_bj, "multigrid", st, flush=True)
