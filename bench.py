@@ -160,11 +160,13 @@ def cpu_baseline(nx_slab=32):
 
 
 # multigrid of the elastic legs: Chebyshev smoothing (the elliptic default),
-# 2+2 sweeps on the fine grid and the first coarse level, 6+6 on the coarse
-# levels of at most 16,384 nodes, whose sweeps cost little more than a
-# launch (128x64x64: 60 PCG iterations and 0.046 s against 85 and 0.052 s
-# with 2+2 everywhere; 8+8: 55 iterations, no faster)
-MG_ELASTIC = dict(pre_sweeps=2, post_sweeps=2, coarse_sweeps=6)
+# one sweep before and after the coarse correction on the fine grid and the
+# first coarse level, 5+5 on the coarse levels of at most 16,384 nodes,
+# whose sweeps cost little more than a launch.  128x64x64 (PCG iterations,
+# s): 2+2 everywhere 85, 0.052; 2+2 with 6+6 on those levels 60, 0.046;
+# 1+1 with 4+4, 5+5, 6+6, 7+7 on them 70, 0.040 / 65, 0.040 / 65, 0.042 /
+# 60, 0.041
+MG_ELASTIC = dict(pre_sweeps=1, post_sweeps=1, coarse_sweeps=5)
 
 
 def compliance_gradient(pa, mesh, vf, z, dims):
