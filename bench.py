@@ -224,14 +224,16 @@ def compliance_gradient(pa, mesh, vf, z, dims):
 J2_DIMS, J2_LENGTHS = (64, 32, 32), (2.0, 1.0, 1.0)
 J2_PARAMS = dict(youngs=1e3, poisson=0.3, hardening_isotropic=10.0, hardening_kinematic=0.0, initial_yield_stress=1.0)
 J2_END_DISPLACEMENT, J2_STEPS = 0.008, 10
-# smoothing of the J2 multigrid per formulation (sweeps, damping, smoother):
-# the mixed form's 4x4 saddle-point blocks are indefinite, so damped block
-# Jacobi with a smaller damping and more sweeps (forward solve, 64x32x32:
-# 4+4 sweeps at 0.6 took 4,108 FGMRES iterations, 5+5 at 0.45 2,559, 0.7
-# diverges towards 12,900); the displacement form's tangent is positive
-# definite, so Chebyshev (3+3: 1,337 iterations against 1,525 with Jacobi
-# at 0.7)
-J2_SMOOTHING = {"mixed": (5, 0.45, "jacobi"), "displacement": (3, 0.7, "chebyshev")}
+# smoothing of the J2 multigrid per formulation (sweeps, damping, smoother,
+# sweeps on the coarse levels of at most 16,384 nodes): the mixed form's 4x4
+# saddle-point blocks are indefinite, so damped block Jacobi with a smaller
+# damping and more sweeps on the fine grid, fewer on the launch-bound coarse
+# levels (forward solve, 64x32x32, FGMRES iterations and s: 4+4 at 0.6
+# everywhere 4,108; 5+5 at 0.45 2,559, 1.46; 5+5 / 3+3 at 0.45 2,453, 1.26;
+# 7+7 / 3+3 at 0.4 1,866, 1.04; 0.7 diverges); the displacement form's
+# tangent is positive definite, so Chebyshev (3+3: 1,337 iterations and
+# 0.59 s against 1,525 with Jacobi at 0.7; 3+3 / 2+2 1,536, 0.63 s)
+J2_SMOOTHING = {"mixed": (7, 0.4, "jacobi", 3), "displacement": (3, 0.7, "chebyshev", 0)}
 
 
 WARMSTART_WEIGHTS = ROOT / "platoanalyze_amd" / "data" / "warmstart_96x48x48.npz"
@@ -346,9 +348,10 @@ def j2_plasticity(pa, stream, with_cpu=True, formulation="mixed"):
     # flexible GMRES(200) to 1e-8 relative, preconditioned by the DxD-block
     # geometric multigrid with 4 pre- and post-smoothing sweeps
     prob = pa.PlasticityProblem(pl, dofs, vals, num_steps=J2_STEPS, max_newton=10, linear_tol=1e-8, restart=200)
-    sweeps, omega, smoother = J2_SMOOTHING[formulation]
+    sweeps, omega, smoother, coarse = J2_SMOOTHING[formulation]
     if os.environ.get("PA_J2_PRECOND", "multigrid") == "multigrid":
-        prob.set_multigrid(*J2_DIMS, pre_sweeps=sweeps, post_sweeps=sweeps, omega=omega, smoother=smoother)
+        prob.set_multigrid(*J2_DIMS, pre_sweeps=sweeps, post_sweeps=sweeps, omega=omega, smoother=smoother,
+                           coarse_sweeps=coarse)
     pa.synchronize()
     t0 = time.perf_counter()
     converged = prob.solution(z)
@@ -364,7 +367,8 @@ def j2_plasticity(pa, stream, with_cpu=True, formulation="mixed"):
     pck = pa.PlasticityProblem(pl, dofs, vals, num_steps=J2_STEPS, max_newton=10, linear_tol=1e-8, restart=200,
                                checkpoint_stride=m_ck)
     if os.environ.get("PA_J2_PRECOND", "multigrid") == "multigrid":
-        pck.set_multigrid(*J2_DIMS, pre_sweeps=sweeps, post_sweeps=sweeps, omega=omega, smoother=smoother)
+        pck.set_multigrid(*J2_DIMS, pre_sweeps=sweeps, post_sweeps=sweeps, omega=omega, smoother=smoother,
+                          coarse_sweeps=coarse)
     pck.solution(z)
     pa.synchronize()
     t0 = time.perf_counter()
@@ -405,8 +409,9 @@ def j2_plasticity(pa, stream, with_cpu=True, formulation="mixed"):
     out = {"workload": f"J2 plasticity {name} Tet4, {J2_DIMS[0]}x{J2_DIMS[1]}x{J2_DIMS[2]} hex x6 "
                        f"({C} cells, {N} nodes, {D * N} dof), {J2_STEPS} steps to ux={J2_END_DISPLACEMENT}",
            "solver": f"condensed Newton; FGMRES(200) to 1e-8, {D}x{D}-block geometric multigrid "
-                     + (f"({sweeps}+{sweeps} damped block-Jacobi sweeps, omega {omega})" if smoother == "jacobi"
-                        else f"({sweeps}+{sweeps} Chebyshev sweeps)"),
+                     + (f"({sweeps}+{sweeps} damped block-Jacobi sweeps, omega {omega}" if smoother == "jacobi"
+                        else f"({sweeps}+{sweeps} Chebyshev sweeps")
+                     + (f"; {coarse}+{coarse} on the coarse levels)" if coarse else ")"),
            "forward_solve_s": t_fwd, "converged": converged,
            "criterion_value_and_adjoint_gradient_s": t_adj, "plastic_work": F,
            "gradient_norm": float(np.linalg.norm(g.numpy())),
