@@ -472,6 +472,76 @@ __global__ __launch_bounds__(1024) void k_gauss_jordan(int n, double* __restrict
     }
 }
 
+// Gauss-Jordan inverse of a larger coarsest level, one launch per pivot step:
+// M = [A | I] (n x 2n) ping-pongs between two buffers and rows are not
+// swapped.  Every block finds the same pivot (largest magnitude in column k
+// among the rows not yet pivots, ties to the lower row; used[r] = k + 1
+// marks the pivot row of step k, so a block of step k never counts the row
+// marked in this step), scales the pivot row and eliminates column k from
+// the other rows.  Row k of the inverse is then the right half of the pivot
+// row of step k.
+__global__ void k_gj_init(int n, const double* __restrict__ a, double* __restrict__ m, int* __restrict__ used) {
+    const long w = 2L * n;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < (long)n * w; e += (long)gridDim.x * blockDim.x) {
+        const int r = (int)(e / w), c = (int)(e % w);
+        m[e] = c < n ? a[(long)r * n + c] : (c - n == r ? 1.0 : 0.0);
+        if (c == 0) used[r] = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gj_step(int n, int k, const double* __restrict__ mi, double* __restrict__ mo,
+                                                 int* __restrict__ used, int* __restrict__ pivrow) {
+    __shared__ double sv[256];
+    __shared__ int si[256];
+    const long w = 2L * n;
+    double best = -1.0;
+    int bi = n;
+    for (int r = threadIdx.x; r < n; r += 256) {
+        const int u = used[r];
+        if (u > 0 && u <= k) continue;
+        const double v = fabs(mi[(long)r * w + k]);
+        if (v > best) {
+            best = v;
+            bi = r;
+        }
+    }
+    sv[threadIdx.x] = best;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            const double o = sv[threadIdx.x + h];
+            const int oi = si[threadIdx.x + h];
+            if (o > sv[threadIdx.x] || (o == sv[threadIdx.x] && oi < si[threadIdx.x])) {
+                sv[threadIdx.x] = o;
+                si[threadIdx.x] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    const int p = si[0];
+    if (p >= n) return;  // no row left (cannot happen for k < n)
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        used[p] = k + 1;
+        pivrow[k] = p;
+    }
+    const double pv = mi[(long)p * w + k];
+    const double ip = pv != 0.0 ? 1.0 / pv : 0.0;
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < (long)n * w; e += (long)gridDim.x * 256) {
+        const int r = (int)(e / w), c = (int)(e % w);
+        const double prow = mi[(long)p * w + c] * ip;
+        mo[e] = r == p ? prow : (c == k ? 0.0 : mi[e] - mi[(long)r * w + k] * prow);
+    }
+}
+
+__global__ void k_gj_extract(int n, const double* __restrict__ m, const int* __restrict__ pivrow, double* __restrict__ inv) {
+    const long w = 2L * n;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < (long)n * n; e += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(e / n), j = (int)(e % n);
+        inv[e] = m[(long)pivrow[k] * w + n + j];
+    }
+}
+
 // x = inv b: one wave per row, lanes over the columns, FP64 sums in a fixed
 // order.
 __global__ __launch_bounds__(256) void k_dense_apply(int n, const double* __restrict__ inv, const float* __restrict__ b,
@@ -584,6 +654,8 @@ struct pa_multigrid {
     pa::DevBuf<unsigned char> mask;
     std::vector<pa::DevBuf<float>*> A, dinv, b, x, t;  // per level (A[0] unused: the fine stencil)
     pa::DevBuf<double> dense, dense_inv;
+    pa::DevBuf<double> gj[2];           // [A | I] ping-pong of the stepwise Gauss-Jordan inverse
+    pa::DevBuf<int> gj_used, gj_piv;
     unsigned graph_offsets = 0;  // bit sidx: the mesh graph holds that offset
     bool symmetric = false;      // the fine operator is symmetric: half stencil
     std::vector<pa::BlockStencil> fsl;  // stencil of each level's smoother
@@ -796,8 +868,22 @@ void setup_impl(pa_multigrid* mg, const VT* vals, hipStream_t st) {
     PA_HIP(hipMemsetAsync(mg->dense.p, 0, nd * nd * sizeof(double), st));
     k_dense_from_stencil<D><<<nblk(gl.nodes()), kT, 0, st>>>(gl, mg->A[L - 1]->p, mg->dense.p);
     chk();
-    k_gauss_jordan<<<1, 1024, 0, st>>>((int)nd, mg->dense.p, mg->dense_inv.p);
-    chk();
+    if (nd <= 256) {
+        k_gauss_jordan<<<1, 1024, 0, st>>>((int)nd, mg->dense.p, mg->dense_inv.p);
+        chk();
+    } else {
+        // one launch per pivot step over the whole GPU
+        const int gb = (int)std::min<long>(1024, (2 * nd * nd + 255) / 256);
+        k_gj_init<<<gb, 256, 0, st>>>((int)nd, mg->dense.p, mg->gj[0].p, mg->gj_used.p);
+        chk();
+        for (int k = 0; k < (int)nd; ++k) {
+            k_gj_step<<<gb, 256, 0, st>>>((int)nd, k, mg->gj[k & 1].p, mg->gj[(k + 1) & 1].p, mg->gj_used.p,
+                                          mg->gj_piv.p);
+            chk();
+        }
+        k_gj_extract<<<gb, 256, 0, st>>>((int)nd, mg->gj[nd & 1].p, mg->gj_piv.p, mg->dense_inv.p);
+        chk();
+    }
     if (!mg->chebyshev) return;
     // Chebyshev dampings of each smoothed level from its largest eigenvalue of D^-1 A
     constexpr int K = 12;
@@ -908,6 +994,12 @@ int pa_multigrid_create(const pa_mesh* mesh, int nx, int ny, int nz, int dpn, co
             const long nd = (long)mg->grids.back().nodes() * dpn;
             mg->dense.alloc(nd * nd);
             mg->dense_inv.alloc(nd * nd);
+            if (nd > 256) {
+                mg->gj[0].alloc(2 * nd * nd);
+                mg->gj[1].alloc(2 * nd * nd);
+                mg->gj_used.alloc(nd);
+                mg->gj_piv.alloc(nd);
+            }
             mg->mask.alloc((long)mesh->N * dpn);
             mg->omegas.alloc((long)L * 2 * pa::kChebMax);
             mg->powpart.alloc(2 * 1024);

@@ -229,6 +229,31 @@ def test_multigrid_pcg_matches_block_jacobi(pa, mode, coarse_sweeps):
     assert rel(g2, g1) < 1e-7
 
 
+def test_multigrid_large_dense_coarsest(pa):
+    """A coarsest level of 675 dofs, inverted by the stepwise Gauss-Jordan
+    (one launch per pivot step): MG-PCG matches block-Jacobi PCG, and the
+    exact solve on the 8x4x4 level needs no more iterations than the deeper
+    hierarchy that smooths it."""
+    print("case: MG-PCG 16x8x8 elastic, dense coarsest 8x4x4 (675 dofs)", flush=True)
+    dims = (16, 8, 8)
+    mesh, vf, sf, z, dofs = elastic_setup(pa, dims, 8)
+    P = pa.Problem(vf, sf, dofs, mode="fp64", tol=1e-10, max_iter=20000)
+    u_bj = P.solution(z).numpy()
+    its = []
+    for ml in (0, 2):
+        P2 = pa.Problem(vf, sf, dofs, mode="fp64", tol=1e-10, max_iter=2000, check_every=1)
+        P2.set_multigrid(*dims, max_levels=ml, max_coarse_nodes=2000)
+        u_mg = P2.solution(z).numpy()
+        st = P2.last_stats
+        assert st["converged"] and st["rel_residual"] <= 1e-10, st
+        assert rel(u_mg, u_bj) < 1e-7
+        its.append(st["iterations"])
+    print("iterations (deep, dense 8x4x4):", its, flush=True)
+    assert its[1] <= its[0]
+    mg = pa.Multigrid(mesh, *dims, 3, max_levels=2, max_coarse_nodes=2000)
+    assert mg.levels() == (2, 675)
+
+
 def test_multigrid_thermal(pa):
     print("case: MG-PCG 16x16x16 thermal", flush=True)
     dims = (16, 16, 16)
