@@ -128,7 +128,9 @@ __global__ __launch_bounds__(kT) void k_fst_resid_jac(Grid g, BlockStencil fs, c
                                                       const unsigned char* __restrict__ mask,
                                                       const float* __restrict__ b, const float* __restrict__ x,
                                                       float* __restrict__ out, float omega,
-                                                      const float* __restrict__ omega_dev, int zero_x) {
+                                                      const float* __restrict__ omega_dev, int zero_x,
+                                                      const double* __restrict__ b64, float* __restrict__ b32,
+                                                      double* __restrict__ out64) {
     constexpr int ROWS = kT / G, BS = D * D, NS = kFsSlots;
     if (omega_dev) omega = omega_dev[0];  // a Chebyshev step's damping, computed on the device
     const int l = threadIdx.x & (G - 1);
@@ -138,7 +140,9 @@ __global__ __launch_bounds__(kT) void k_fst_resid_jac(Grid g, BlockStencil fs, c
     stencil_ijk(g, fs, n, i, j, k);
     // the node's dof min(l, D - 1) for the update, in flight with the stencil
     const int ld = l < D ? l : D - 1;
-    const float bl = b[(long)n * D + ld], xl = x[(long)n * D + ld];
+    // b64: the right-hand side in FP64 (the V-cycle's first sweep), kept as
+    // b32 for the level's later sweeps
+    const float bl = b64 ? (float)b64[(long)n * D + ld] : b[(long)n * D + ld], xl = x[(long)n * D + ld];
     const unsigned ml = mask ? mask[(long)n * D + ld] : 1u;
     float dl[D];
 #pragma unroll
@@ -191,7 +195,12 @@ __global__ __launch_bounds__(kT) void k_fst_resid_jac(Grid g, BlockStencil fs, c
 #pragma unroll
         for (int q = 0; q < D; ++q) t += dl[q] * rq[q];
     }
-    if (l < D) out[(long)n * D + l] = ml ? (omega != 0.0f ? (zero_x ? 0.0f : xl) + omega * t : rl) : 0.0f;
+    if (l < D) {
+        const float v = ml ? (omega != 0.0f ? (zero_x ? 0.0f : xl) + omega * t : rl) : 0.0f;
+        if (out64) out64[(long)n * D + l] = (double)v;  // the V-cycle's last sweep writes z in FP64
+        else out[(long)n * D + l] = v;
+        if (b64) b32[(long)n * D + l] = bl;
+    }
 }
 
 // ------------------------------------------------------------------ transfer
@@ -680,13 +689,14 @@ namespace {
 // One smoothing sweep (or, with om = 0, the residual) on level l.
 template <int D>
 void sweep_level(pa_multigrid* mg, int l, const float* b, const float* xin, float* xout, float om,
-                 const float* om_dev, int zero, hipStream_t st) {
+                 const float* om_dev, int zero, hipStream_t st, const double* b64 = nullptr, float* b32 = nullptr,
+                 double* out64 = nullptr) {
     const Grid g = mg->grids[l];
     const BlockStencil& f = mg->fsl[l];
     auto run = [&](auto kernel) {
         kernel<<<nblk(g.nodes(), kT / f.G), kT, 0, st>>>(g, f, l == 0 ? mg->fine.p : mg->A[l]->p, mg->dinv[l]->p,
                                                          l == 0 ? mg->mask.p : nullptr, b, xin, xout, om, om_dev,
-                                                         zero);
+                                                         zero, b64, b32, out64);
     };
     if (f.G == 4) {
         if (f.half) run(k_fst_resid_jac<D, 4, true>);
@@ -706,8 +716,10 @@ int level_sweeps(const pa_multigrid* mg, int l, bool pre) {
     return (l > 0 && mg->opt.coarse_sweeps > 0 && mg->grids[l].nodes() <= lim) ? mg->opt.coarse_sweeps : dflt;
 }
 
+// r64/z64 (level 0 only): the FP64 right-hand side, read by the first sweep,
+// and the FP64 correction, written by the last sweep.
 template <int D>
-void vcycle(pa_multigrid* mg, int l, hipStream_t st) {
+void vcycle(pa_multigrid* mg, int l, hipStream_t st, const double* r64 = nullptr, double* z64 = nullptr) {
     const int L = (int)mg->grids.size();
     const Grid g = mg->grids[l];
     const long n = (long)g.nodes() * D;
@@ -722,12 +734,13 @@ void vcycle(pa_multigrid* mg, int l, hipStream_t st) {
     }
     // damping of pre-smoothing sweep s (post: s + kChebMax)
     auto dev = [&](int s) { return mg->chebyshev ? mg->omegas.p + (long)l * 2 * kChebMax + s : nullptr; };
-    auto sweep = [&](const float* xin, float* xout, float om, const float* om_dev, int zero) {
-        sweep_level<D>(mg, l, b, xin, xout, om, om_dev, zero, st);
+    auto sweep = [&](const float* xin, float* xout, float om, const float* om_dev, int zero,
+                     const double* in64 = nullptr, double* out64 = nullptr) {
+        sweep_level<D>(mg, l, b, xin, xout, om, om_dev, zero, st, in64, in64 ? b : nullptr, out64);
     };
     // pre-smoothing (ping-pong x <-> t; the first sweep starts from zero)
     const int pre = level_sweeps(mg, l, true);
-    sweep(x, x, w, dev(0), 1);
+    sweep(x, x, w, dev(0), 1, r64);
     for (int s = 1; s < pre; ++s) {
         sweep(x, t, w, dev(s), 0);
         std::swap(mg->x[l], mg->t[l]);
@@ -750,7 +763,7 @@ void vcycle(pa_multigrid* mg, int l, hipStream_t st) {
     chk();
     // post-smoothing
     for (int s = 0, post = level_sweeps(mg, l, false); s < post; ++s) {
-        sweep(x, t, w, dev(kChebMax + s), 0);
+        sweep(x, t, w, dev(kChebMax + s), 0, nullptr, s == post - 1 ? z64 : nullptr);
         std::swap(mg->x[l], mg->t[l]);
         x = mg->x[l]->p;
         t = mg->t[l]->p;
@@ -923,6 +936,12 @@ void multigrid_set_elliptic(pa_multigrid* mg) {
 void multigrid_apply(pa_multigrid* mg, const double* r, double* z, hipStream_t st) {
     PA_REQUIRE(mg->ready, PA_ERR_STATE, "multigrid: setup has not been called");
     const long n = (long)mg->grids[0].nodes() * mg->dpn;
+    if (mg->grids.size() > 1) {  // the fine level's first and last sweeps convert r and z
+        if (mg->dpn == 3) vcycle<3>(mg, 0, st, r, z);
+        else if (mg->dpn == 4) vcycle<4>(mg, 0, st, r, z);
+        else vcycle<1>(mg, 0, st, r, z);
+        return;
+    }
     k_to_f32<<<nblk(n), kT, 0, st>>>(n, r, mg->b[0]->p);
     chk();
     if (mg->dpn == 3) vcycle<3>(mg, 0, st);

@@ -195,11 +195,12 @@ __global__ void k_reduce(const double* __restrict__ part, int n, int nparts, int
 }
 
 // x += a p; r -= a q; z = M r; partials of r.z and r.r (a = rz / pq).
+// store_z = 0: z is not written (a V-cycle computes it next).
 template <int D>
 __global__ __launch_bounds__(kB) void k_pcg_update(int N, int prec, const double* __restrict__ minv,
                                                    const double* __restrict__ scal, const double* __restrict__ p,
                                                    const double* __restrict__ q, double* __restrict__ x,
-                                                   double* __restrict__ r, double* __restrict__ zv,
+                                                   double* __restrict__ r, double* __restrict__ zv, int store_z,
                                                    double* __restrict__ part, int nparts) {
     __shared__ double sh[kB / 64];
     const double pq = scal[S_PQ];
@@ -217,7 +218,7 @@ __global__ __launch_bounds__(kB) void k_pcg_update(int N, int prec, const double
 #pragma unroll
         for (int i = 0; i < D; ++i) {
             r[node * D + i] = rv[i];
-            zv[node * D + i] = zz[i];
+            if (store_z) zv[node * D + i] = zz[i];
             rz += rv[i] * zz[i];
             rr += rv[i] * rv[i];
         }
@@ -484,7 +485,7 @@ void pcg_impl(pa_solver* s, const VT* vals, const double* b, double* x, double t
             k_reduce<<<1, 1024, 0, st>>>(s->part.p, s->nsp, s->nsp, 1, S_PQ, 0, 0, 0, s->scal.p);
             check();
             k_pcg_update<D><<<nvb, kB, 0, st>>>(N, kprec, s->d_minv.p, s->scal.p, s->p.p, s->q.p, x, s->r.p,
-                                                s->zv.p, s->part.p, nvb);
+                                                s->zv.p, mgp ? 0 : 1, s->part.p, nvb);
             check();
             k_reduce<<<1, 1024, 0, st>>>(s->part.p, nvb, nvb, 2, S_RZ, S_RR, 0, 1, s->scal.p);
             check();
