@@ -640,14 +640,6 @@ __global__ void k_mask_bc(int nbc, const int* __restrict__ dofs, unsigned char* 
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nbc) m[dofs[i]] = 0;
 }
-__global__ void k_to_f32(long n, const double* __restrict__ a, float* __restrict__ b) {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) b[i] = (float)a[i];
-}
-__global__ void k_to_f64(long n, const float* __restrict__ a, double* __restrict__ b) {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) b[i] = (double)a[i];
-}
 
 inline int nblk(long n, int t = kT) { return (int)((n + t - 1) / t); }
 void chk() { PA_HIP(hipGetLastError()); }
@@ -935,20 +927,11 @@ void multigrid_set_elliptic(pa_multigrid* mg) {
 // used by the PCG of pa_solver.hip
 void multigrid_apply(pa_multigrid* mg, const double* r, double* z, hipStream_t st) {
     PA_REQUIRE(mg->ready, PA_ERR_STATE, "multigrid: setup has not been called");
-    const long n = (long)mg->grids[0].nodes() * mg->dpn;
-    if (mg->grids.size() > 1) {  // the fine level's first and last sweeps convert r and z
-        if (mg->dpn == 3) vcycle<3>(mg, 0, st, r, z);
-        else if (mg->dpn == 4) vcycle<4>(mg, 0, st, r, z);
-        else vcycle<1>(mg, 0, st, r, z);
-        return;
-    }
-    k_to_f32<<<nblk(n), kT, 0, st>>>(n, r, mg->b[0]->p);
-    chk();
-    if (mg->dpn == 3) vcycle<3>(mg, 0, st);
-    else if (mg->dpn == 4) vcycle<4>(mg, 0, st);
-    else vcycle<1>(mg, 0, st);
-    k_to_f64<<<nblk(n), kT, 0, st>>>(n, mg->x[0]->p, z);
-    chk();
+    // at least two levels (pa_multigrid_create): the fine level's first and
+    // last sweeps convert r and z
+    if (mg->dpn == 3) vcycle<3>(mg, 0, st, r, z);
+    else if (mg->dpn == 4) vcycle<4>(mg, 0, st, r, z);
+    else vcycle<1>(mg, 0, st, r, z);
 }
 
 }  // namespace pa
