@@ -434,38 +434,37 @@ int bicgstab(pa_plasticity_problem* P, const double* b, double* x, hipStream_t s
 // the host
 // ---------------------------------------------------------------------------
 // part[i*nparts + block] = partial sums of w . V_i, i < k (V_i at V + i*n);
-// with self, row k holds the partial sums of w . w.  Four rows share each
-// block reduction.
+// with self, row k holds the partial sums of w . w.  blockIdx.y picks a
+// group of kMdotRows rows, so the groups' loads are in flight at once.
+constexpr int kMdotRows = 4;
 __global__ __launch_bounds__(kB) void k_mdot(long n, int k, int self, const double* __restrict__ V,
                                              const double* __restrict__ w, double* __restrict__ part, int nparts) {
-    constexpr int Q = 4;
+    constexpr int Q = kMdotRows;
     __shared__ double sh[Q][kB / 64];
     const int rows = k + (self ? 1 : 0);
-    for (int i0 = 0; i0 < rows; i0 += Q) {
-        double s[Q] = {};
-        for (long t = (long)blockIdx.x * kB + threadIdx.x; t < n; t += (long)gridDim.x * kB) {
-            const double wt = w[t];
-#pragma unroll
-            for (int q = 0; q < Q; ++q) {
-                const int i = i0 + q;
-                if (i < k) s[q] += wt * V[(long)i * n + t];
-                else if (i == k && self) s[q] += wt * wt;
-            }
-        }
+    const int i0 = blockIdx.y * Q;
+    double s[Q] = {};
+    for (long t = (long)blockIdx.x * kB + threadIdx.x; t < n; t += (long)gridDim.x * kB) {
+        const double wt = w[t];
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            double v = s[q];
+            const int i = i0 + q;
+            if (i < k) s[q] += wt * V[(long)i * n + t];
+            else if (i == k && self) s[q] += wt * wt;
+        }
+    }
 #pragma unroll
-            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-            if (threadIdx.x % 64 == 0) sh[q][threadIdx.x / 64] = v;
-        }
-        __syncthreads();
-        if ((int)threadIdx.x < Q && i0 + (int)threadIdx.x < rows) {
-            double tt = 0.0;
-            for (int wv = 0; wv < kB / 64; ++wv) tt += sh[threadIdx.x][wv];
-            part[(long)(i0 + threadIdx.x) * nparts + blockIdx.x] = tt;
-        }
-        __syncthreads();
+    for (int q = 0; q < Q; ++q) {
+        double v = s[q];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (threadIdx.x % 64 == 0) sh[q][threadIdx.x / 64] = v;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < Q && i0 + (int)threadIdx.x < rows) {
+        double tt = 0.0;
+        for (int wv = 0; wv < kB / 64; ++wv) tt += sh[threadIdx.x][wv];
+        part[(long)(i0 + threadIdx.x) * nparts + blockIdx.x] = tt;
     }
 }
 
@@ -603,7 +602,7 @@ int gmres(pa_plasticity_problem* P, const double* b, double* x, hipStream_t st) 
             // to working precision.
             double* g1 = P->gh.p;
             double* g2 = P->gh.p + (M + 3);
-            k_mdot<<<nvb, kB, 0, st>>>(n, j + 1, 1, V, w, P->gpart.p, nvb);
+            k_mdot<<<dim3(nvb, (j + 2 + kMdotRows - 1) / kMdotRows), kB, 0, st>>>(n, j + 1, 1, V, w, P->gpart.p, nvb);
             check();
             k_reduce_rows<<<j + 2, 1024, 0, st>>>(P->gpart.p, nvb, nvb, 0, g1);  // h, |w|^2
             check();
@@ -616,7 +615,8 @@ int gmres(pa_plasticity_problem* P, const double* b, double* x, hipStream_t st) 
             double hn2 = h1[j + 2];
             for (int i = 0; i <= j; ++i) hcol[i] = h1[i];
             if (!(hn2 > 0.5 * h1[j + 1])) {
-                k_mdot<<<nvb, kB, 0, st>>>(n, j + 1, 0, V, w, P->gpart.p, nvb);
+                k_mdot<<<dim3(nvb, (j + 1 + kMdotRows - 1) / kMdotRows), kB, 0, st>>>(n, j + 1, 0, V, w, P->gpart.p,
+                                                                                      nvb);
                 check();
                 k_reduce_rows<<<j + 1, 1024, 0, st>>>(P->gpart.p, nvb, nvb, 0, g2);
                 check();
