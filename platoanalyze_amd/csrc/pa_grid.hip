@@ -52,6 +52,19 @@ namespace {
 
 constexpr int kGridWaves = 4;  // waves per workgroup (one halo line)
 
+// Matrix values are written once and never read back by the kernel that
+// writes them: non-temporal stores (measured at 128x64x64: FP64
+// value_gradient_u 0.2015 -> 0.1817 ms, gradient_u 0.1795 -> 0.165 ms).
+template <class T>
+__device__ inline void store_nt(T* p, T v) {
+    __builtin_nontemporal_store(v, p);
+}
+typedef double nt_double2 __attribute__((ext_vector_type(2)));
+__device__ inline void store_nt2(double* p, double a, double b) {
+    const nt_double2 v = {a, b};
+    __builtin_nontemporal_store(v, reinterpret_cast<nt_double2*>(p));
+}
+
 // Compile-time loop: f(std::integral_constant<int, i>) for i = 0..N-1, so
 // table lookups on i are constant expressions and the hex value arrays stay
 // in registers.
@@ -486,7 +499,7 @@ __global__ __launch_bounds__(W * 64, 2) void k_grid_jacobian1(GridArgs g, Materi
                 for (int q = 0; q < 4; ++q) a[q] = t + 64 * q < n ? stage[t + 64 * q] : VT(0);
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    if (t + 64 * q < n) dst[t + 64 * q] = a[q];
+                    if (t + 64 * q < n) store_nt(dst + t + 64 * q, a[q]);
             }
         }
     }
@@ -1073,7 +1086,7 @@ __global__ __launch_bounds__(BX * BY * BZ * kJ3Slots) void k_grid_jacobian3(Grid
             if ((m >> slot) & 1u) {
                 VT* dst = vals + ((long)rstart[r] + __popc(m & ((1u << slot) - 1u))) * 9;
 #pragma unroll
-                for (int q = 0; q < 9; ++q) dst[q] = (VT)K[q];
+                for (int q = 0; q < 9; ++q) store_nt(dst + q, (VT)K[q]);
             }
         }
     } else {
@@ -1112,10 +1125,9 @@ __global__ __launch_bounds__(BX * BY * BZ * kJ3Slots) void k_grid_jacobian3(Grid
             const VT* src = stage + line * B::line_blocks * 9;
             VT* dstg = vals + (long)lstart[line] * 9;
             const int h = (int)(lstart[line] & 1);  // 9 * lstart is odd iff lstart is
-            if (lane == 0 && h && n > 0) dstg[0] = src[0];
-            for (int e = h + 2 * lane; e + 1 < n; e += 128)
-                *reinterpret_cast<double2*>(dstg + e) = make_double2(src[e], src[e + 1]);
-            if (lane == 0 && n > h && ((n - h) & 1)) dstg[n - 1] = src[n - 1];
+            if (lane == 0 && h && n > 0) store_nt(dstg, src[0]);
+            for (int e = h + 2 * lane; e + 1 < n; e += 128) store_nt2(dstg + e, src[e], src[e + 1]);
+            if (lane == 0 && n > h && ((n - h) & 1)) store_nt(dstg + n - 1, src[n - 1]);
         }
     }
     if constexpr (WR) {
